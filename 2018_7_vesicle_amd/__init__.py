@@ -1,0 +1,14 @@
+"""MI355X-native level-set hot path of tiankuizhang/2018_7_Vesicle.
+
+HIP kernels for gfx950 behind the C ABI of include/vesicle_ls.h, with a host
+mirror of the reference's @GD3/@SDF3 MATLAB classes (sdf3.py).  The directory
+name starts with a digit, so import it with
+importlib.import_module("2018_7_vesicle_amd").
+"""
+from ._lib import LIB_PATH, VLSError, exported_names, lib  # noqa: F401
+from .sdf3 import (GD3, SDF3, TOOLBOX_FIELDS, feval_advection_step, feval_boundary_correction,  # noqa: F401
+                   feval_boundary_interpolant, feval_boundary_location, feval_eno2_re_step, feval_extend_step,
+                   feval_re_step, feval_set_calculus_toolbox, feval_upwind_normal, feval_weno_derivative)
+from . import pipeline  # noqa: F401
+
+__version__ = "0.1.0"

@@ -1,0 +1,55 @@
+"""Build the HIP library of the package in-tree with hipcc (gfx950 only).
+
+    python -m <package>.build      or      __graft_entry__.build()
+
+Produces libvesicle_ls.so next to this file.  -ffp-contract=off: every fused
+multiply-add in the kernels is explicit, so results are one fixed IEEE
+sequence (the CPU oracle states the same one).  The flag
+`-Xclang -target-feature -Xclang -packed-fp32-ops` keeps the compiler from
+emitting packed FP32 instructions; the host pass warns that it does not know
+the feature, which is harmless.
+"""
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB_NAME = "libvesicle_ls.so"
+LIB_PATH = os.path.join(HERE, LIB_NAME)
+SOURCES = ["vls_kernels.hip", "vls_api.hip"]
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall",
+    "-Wno-unused-command-line-argument",
+]
+
+
+def build(verbose=False):
+    objs = []
+    for src in SOURCES:
+        obj = os.path.join(CSRC, src.replace(".hip", ".o"))
+        cmd = [HIPCC, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj] + FLAGS
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        objs.append(obj)
+    tmp = LIB_PATH + ".tmp"
+    cmd = [HIPCC, "-shared", "-o", tmp] + objs + ["--offload-arch=gfx950", "-fPIC", "-Wl,-rpath,/opt/rocm/lib"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB_PATH)
+    for o in objs:
+        os.remove(o)
+    return LIB_PATH
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -1,0 +1,379 @@
+// Device arithmetic of the level-set hot path, written for gfx950 (wave64,
+// full-rate FP64 VALU).  Every function states the reference kernel it
+// reproduces.  The formulas are the reference's; the uniform WENO5 weights use
+// one division instead of six.  Everything is compiled without contraction
+// (-ffp-contract=off) and every fused multiply-add is written out, so each
+// result is one fixed IEEE-754 sequence: the CPU oracle evaluates the same
+// sequences and the two agree bit for bit.  Against the reference's own build
+// (which contracts freely and calls pow) they agree to rounding.
+// Paths are relative to src/7_0_paper/2_0_Vesicle/CUDA_Code/.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vls {
+
+typedef int64_t idx_t;
+
+__device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }
+__device__ __forceinline__ double dmin(double a, double b) { return (a < b) ? a : b; }
+__device__ __forceinline__ double sq(double a) { return a * a; }
+// kernel sign(): +1 for x>0, -1 otherwise (1_3_WENO5RK3_Reinitialization/shared_utilities.hpp:17-18)
+__device__ __forceinline__ double ksign(double x) { return (x > 0) ? 1.0 : -1.0; }
+// MATLAB sign(): 0 at 0 (used by the extension velocity, +SD/@SDF3/WENO5RK3Extend.m:29-31)
+__device__ __forceinline__ double msign(double x) { return (x > 0) ? 1.0 : ((x < 0) ? -1.0 : 0.0); }
+__device__ __forceinline__ double min_mod(double x, double y)
+{
+    return (x * y < 0) ? 0.0 : (fabs(x) < fabs(y) ? x : y);
+}
+
+// Box geometry: rows = y (contiguous), cols = x, pges = z; clamped
+// neighbours (shared_utilities.hpp:21-31).
+struct Box {
+    int rows, cols, pges;
+    double dx, dy, dz;
+};
+
+__device__ __forceinline__ int clampi(int v, int hi) { return v < 0 ? 0 : (v > hi ? hi : v); }
+
+// the 7 values f(offset -3..+3) along `axis` (0 = x/cols, 1 = y/rows, 2 = z/pges)
+template <typename T>
+__device__ __forceinline__ void line7(const T* __restrict__ f, int r, int c, int p, int axis,
+                                      const Box& b, double out[7])
+{
+    const idx_t plane = (idx_t)b.rows * b.cols;
+    if (axis == 1) {
+        const idx_t base = (idx_t)p * plane + (idx_t)c * b.rows;
+#pragma unroll
+        for (int s = -3; s <= 3; ++s) out[s + 3] = (double)f[base + clampi(r + s, b.rows - 1)];
+    } else if (axis == 0) {
+        const idx_t base = (idx_t)p * plane + r;
+#pragma unroll
+        for (int s = -3; s <= 3; ++s) out[s + 3] = (double)f[base + (idx_t)clampi(c + s, b.cols - 1) * b.rows];
+    } else {
+        const idx_t base = (idx_t)c * b.rows + r;
+#pragma unroll
+        for (int s = -3; s <= 3; ++s) out[s + 3] = (double)f[base + (idx_t)clampi(p + s, b.pges - 1) * plane];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Uniform WENO5 (Jiang-Peng), 1_3_WENO5RK3_Reinitialization/re_step.cu:11-37.
+// alpha_k = c_k/(S_k+eps)^2 and omega_k = alpha_k/sum are folded into one
+// division: sum_k c_k phi_k prod_{j!=k} a_j / sum_k c_k prod_{j!=k} a_j with
+// a_k = (S_k+eps)^2.  For |v| < 1e30 nothing overflows (a_j a_k ~ v^8).
+// The library is compiled without floating-point contraction; the fused
+// multiply-adds below are explicit, so the result is one fixed IEEE sequence
+// (the CPU oracle states the same sequence).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double weno5_onesided(double v1, double v2, double v3, double v4, double v5)
+{
+    const double c13 = 1. / 3., c76 = 7. / 6., c116 = 11. / 6., c16 = 1. / 6., c56 = 5. / 6., c1312 = 13. / 12.;
+    const double phi1 = fma(c116, v3, fma(-c76, v2, c13 * v1));
+    const double phi2 = fma(c13, v4, fma(c56, v3, -c16 * v2));
+    const double phi3 = fma(-c16, v5, fma(c56, v4, c13 * v3));
+    const double d1 = fma(-2.0, v2, v1) + v3, e1 = fma(3.0, v3, fma(-4.0, v2, v1));
+    const double d2 = fma(-2.0, v3, v2) + v4, e2 = v2 - v4;
+    const double d3 = fma(-2.0, v4, v3) + v5, e3 = fma(3.0, v3, fma(-4.0, v4, v5));
+    const double S1 = fma(c1312 * d1, d1, 0.25 * e1 * e1);
+    const double S2 = fma(c1312 * d2, d2, 0.25 * e2 * e2);
+    const double S3 = fma(c1312 * d3, d3, 0.25 * e3 * e3);
+    const double eps = 1e-6;
+    const double s1 = S1 + eps, s2 = S2 + eps, s3 = S3 + eps;
+    const double a1 = s1 * s1, a2 = s2 * s2, a3 = s3 * s3;
+    const double w1 = 0.1 * (a2 * a3), w2 = 0.6 * (a1 * a3), w3 = 0.3 * (a1 * a2);
+    return fma(w1, phi1, fma(w2, phi2, w3 * phi3)) / (w1 + w2 + w3);
+}
+
+// back/fore pair from 7 nodal values (re_step.cu:260-267), 1/ds precomputed
+__device__ __forceinline__ void weno5_pair(double& d_fore, double& d_back, const double p[7], double inv_ds)
+{
+    double v[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v[k] = (p[k + 1] - p[k]) * inv_ds;
+    d_back = weno5_onesided(v[0], v[1], v[2], v[3], v[4]);
+    d_fore = weno5_onesided(v[5], v[4], v[3], v[2], v[1]);
+}
+
+// ---------------------------------------------------------------------------
+// Nonuniform three-stencil WENO at a node next to the interface:
+// re_step.cu:79-243 (weno_nonuniform), reconstructing at face a[2] (fore) or
+// a[3] (back) from first divided differences u[k] over [a[k], a[k+1]].
+// ---------------------------------------------------------------------------
+__device__ __noinline__ double weno_nonuniform_face(const double* a, const double* u, bool back)
+{
+#pragma clang fp contract(off)
+    const double xm25 = a[0], xm15 = a[1], xm05 = a[2], x05 = a[3], x15 = a[4], x25 = a[5];
+    const double um2 = u[0], um1 = u[1], u0 = u[2], u1 = u[3], u2 = u[4];
+    double v0, v1, v2, c0, c1, c2;
+    if (!back) {
+        v0 = u1 + (u0 - u1) * (1.0 + (x05 - xm05) / (x15 - xm05) + (x05 - xm05) / (x25 - xm05))
+                + (u2 - u1) * ((x05 - xm05) / (x25 - xm05)) * ((x15 - xm05) / (x25 - x05));
+        v1 = u0 + (um1 - u0) * ((x05 - xm05) / (x15 - xm15)) * ((x15 - xm05) / (x05 - xm15))
+                - (u1 - u0) * ((x05 - xm05) / (x15 - xm15)) * ((xm05 - xm15) / (x15 - xm05));
+        v2 = um1 + (u0 - um1) * ((xm05 - xm15) / (x05 - xm25)) * ((xm05 - xm25) / (x05 - xm15))
+                 - (um2 - um1) * ((xm05 - xm15) / (x05 - xm25)) * ((x05 - xm05) / (xm05 - xm25));
+        c0 = ((xm05 - xm15) / (x25 - xm25)) * ((xm05 - xm25) / (x25 - xm15));
+        c1 = ((xm05 - xm25) / (x25 - xm25)) * ((x25 - xm05) / (x25 - xm15))
+             * (1.0 + (x25 - xm15) / (x15 - xm25));
+        c2 = ((x15 - xm05) / (x25 - xm25)) * ((x25 - xm05) / (x15 - xm25));
+    } else {
+        v0 = u1 + (u0 - u1) * ((x15 - x05) / (x25 - xm05)) * ((x25 - x05) / (x15 - xm05))
+                - (u2 - u1) * ((x15 - x05) / (x25 - xm05)) * ((x05 - xm05) / (x25 - x05));
+        v1 = u0 + (u1 - u0) * ((x05 - xm05) / (x15 - xm15)) * ((x05 - xm15) / (x15 - xm05))
+                - (um1 - u0) * ((x05 - xm05) / (x15 - xm15)) * ((x15 - x05) / (x05 - xm15));
+        v2 = um1 + (um2 - um1) * ((x05 - xm05) / (xm05 - xm25)) * ((x05 - xm15) / (x05 - xm25))
+                 + (u0 - um1) * (1.0 + (x05 - xm05) / (x05 - xm15) + (x05 - xm05) / (x05 - xm25));
+        c0 = ((x05 - xm25) / (x25 - xm25)) * ((x05 - xm15) / (x25 - xm15));
+        c1 = ((x05 - xm25) / (x25 - xm25)) * ((x25 - x05) / (x25 - xm15))
+             * (1.0 + (x25 - xm15) / (x15 - xm25));
+        c2 = ((x15 - x05) / (x25 - xm25)) * ((x25 - x05) / (x15 - xm25));
+    }
+    const double h2 = sq(x05 - xm05);
+    const double s0 = 4.0 * sq((x05 - xm05) / (x25 - xm05)) *
+        (sq((u2 - u1) / (x25 - x05)) * (10.0 * h2 + (x15 - xm05) * (x15 - x05))
+         + (u2 - u1) * (u0 - u1) / ((x25 - x05) * (x15 - xm05)) *
+               (20.0 * h2 + 2.0 * (x15 - xm05) * (x15 - x05) + (x25 - xm05) * (2.0 * x15 - x05 - xm05))
+         + sq((u0 - u1) / (x15 - xm05)) * (10.0 * h2 + (x25 + x15 - 2.0 * xm05) * (x25 + x15 - x05 - xm05)));
+    const double s1 = 4.0 * sq((x05 - xm05) / (x15 - xm15)) *
+        (sq((um1 - u0) / (x05 - xm15)) * (10.0 * h2 + (x15 - xm05) * (x15 - x05))
+         + (u1 - u0) * (um1 - u0) / ((x15 - xm05) * (x05 - xm15)) *
+               (20.0 * h2 - (x15 - x05) * (xm05 - xm15) - (x15 - xm05) * (x05 - xm15))
+         + sq((u1 - u0) / (x15 - xm05)) * (10.0 * h2 + (xm05 - xm15) * (x05 - xm15)));
+    const double s2 = 4.0 * sq((x05 - xm05) / (x05 - xm25)) *
+        (sq((um2 - um1) / (xm05 - xm25)) * (10.0 * h2 + (x05 - xm15) * (xm05 - xm15))
+         + (u0 - um1) * (um2 - um1) / ((x05 - xm15) * (xm05 - xm25)) *
+               (20.0 * h2 + 2.0 * (x05 - xm15) * (xm05 - xm15) + (x05 - xm25) * (x05 + xm05 - 2.0 * xm15))
+         + sq((u0 - um1) / (x05 - xm15)) *
+               (10.0 * h2 + (2.0 * x05 - xm25 - xm15) * (x05 + xm05 - xm15 - xm25)));
+    const double eps = 1e-6;
+    const double al0 = c0 / sq(s0 + eps), al1 = c1 / sq(s1 + eps), al2 = c2 / sq(s2 + eps);
+    const double sum = al0 + al1 + al2;
+    return v0 * (al0 / sum) + v1 * (al1 / sum) + v2 * (al2 / sum);
+}
+
+// select_stencil + weno_nonuniform (re_step.cu:41-77; interface values
+// h_fore/h_back as in 2_2_WENO5RK3_Extend/shared_utilities.cup:49-85)
+__device__ __noinline__ void weno_nonuniform_pair(double& d_fore, double& d_back, const double* p,
+                                                  double r1, double l1, double ds, double h_fore,
+                                                  double h_back)
+{
+#pragma clang fp contract(off)
+    double x[7], h[7];
+    h[3] = p[3];
+    x[3] = 0.0;
+    if (r1 < ds) {
+        x[4] = r1; x[5] = ds; x[6] = 2 * ds;
+        h[4] = h_fore; h[5] = p[4]; h[6] = p[5];
+    } else {
+        x[4] = ds; x[5] = 2 * ds; x[6] = 3 * ds;
+        h[4] = p[4]; h[5] = p[5]; h[6] = p[6];
+    }
+    if (l1 < ds) {
+        x[2] = -l1; x[1] = -ds; x[0] = -2 * ds;
+        h[2] = h_back; h[1] = p[2]; h[0] = p[1];
+    } else {
+        x[2] = -ds; x[1] = -2 * ds; x[0] = -3 * ds;
+        h[2] = p[2]; h[1] = p[1]; h[0] = p[0];
+    }
+    double d[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) d[k] = (h[k + 1] - h[k]) / (x[k + 1] - x[k]);
+    d_fore = weno_nonuniform_face(&x[1], &d[1], false);
+    d_back = weno_nonuniform_face(&x[0], &d[0], true);
+}
+
+// ---------------------------------------------------------------------------
+// Godunov Hamiltonian residual, re_step.cu:382-392
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double godunov_step(bool inside, double xR, double xL, double yF, double yB,
+                                               double zU, double zD, double dt)
+{
+    double g;
+    if (inside)
+        g = dmax(sq(dmin(0.0, xL)), sq(dmax(0.0, xR))) + dmax(sq(dmin(0.0, yB)), sq(dmax(0.0, yF)))
+            + dmax(sq(dmin(0.0, zD)), sq(dmax(0.0, zU)));
+    else
+        g = dmax(sq(dmax(0.0, xL)), sq(dmin(0.0, xR))) + dmax(sq(dmax(0.0, yB)), sq(dmin(0.0, yF)))
+            + dmax(sq(dmax(0.0, zD)), sq(dmin(0.0, zU)));
+    const double s = (sqrt(g) - 1) * dt;
+    return inside ? -s : s;
+}
+
+// ---------------------------------------------------------------------------
+// ENO2 one-sided derivatives with subcell node value 0,
+// 1_0_ENORK2_Reinitialization/boundary_correction.cu:80-100
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void eno2_pair(double& sR, double& sL, const double p[7], double pr, double pl,
+                                          double ds)
+{
+    const double v4 = p[1], v1 = p[2], v0 = p[3], v2 = p[4], v3 = p[5];
+    const double p2 = v1 - 2.0 * v0 + v2;
+    const double p2r = v0 - 2.0 * v2 + v3;
+    double p2m = 0.5 * min_mod(p2, p2r) / (ds * ds);
+    const double vr = (pr == ds) ? v2 : 0.0;
+    sR = (vr - v0) / pr - pr * p2m;
+    const double p2l = v0 - 2.0 * v1 + v4;
+    p2m = 0.5 * min_mod(p2, p2l) / (ds * ds);
+    const double vl = (pl == ds) ? v1 : 0.0;
+    sL = (v0 - vl) / pl + pl * p2m;
+}
+
+// ---------------------------------------------------------------------------
+// subcell crossing distances
+// ---------------------------------------------------------------------------
+// quadratic ENO: 1_3_WENO5RK3_Reinitialization/boundary_location.cu:6-20
+__device__ __forceinline__ double eno_quadratic_distance(double v1, double v0, double v2, double v3, double ds)
+{
+#pragma clang fp contract(off)
+    const double eps = 10e-10;
+    const double p2l = v2 - 2.0 * v0 + v1;
+    const double p2r = v3 - 2.0 * v2 + v0;
+    const double p2 = min_mod(p2l, p2r);
+    if (p2 > eps || p2 < -eps) {
+        const double disc = sq(0.5 * p2 - v0 - v2) - 4. * v0 * v2;
+        return ds * (0.5 + (v0 - v2 - ksign(v0 - v2) * sqrt(disc)) / p2);
+    }
+    return ds * v0 / (v0 - v2);
+}
+
+// Lagrange -> monomial coefficients of the quintic through (x_i, y_i)
+// (Poly<5>::setInterpolation, shared_utilities.hpp:57-103).  The monomial form
+// is ill-conditioned (~1e-11 relative near the data), so this and the
+// evaluation below keep one operation order without contraction; interface
+// cells are few, the cost does not matter.
+__device__ __forceinline__ void poly5_interp(const double* x, const double* y, double coef[6])
+{
+#pragma clang fp contract(off)
+    double c[6][6];
+    double kk[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        double k = y[i];
+        c[i][0] = 1.0;
+#pragma unroll
+        for (int m = 1; m < 6; ++m) c[i][m] = 0.0;
+        int deg = 0;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            if (j == i) continue;
+            k /= x[i] - x[j];
+            // multiply c[i] (degree deg) by (x - x[j])
+#pragma unroll
+            for (int m = 5; m >= 1; --m)
+                if (m <= deg + 1) c[i][m] = c[i][m] - x[j] * c[i][m - 1];
+            ++deg;
+        }
+        kk[i] = k;
+    }
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) acc += kk[j] * c[j][5 - m];
+        coef[m] = acc;
+    }
+}
+
+// sum_i c_i x^i in index order (Poly<n>::operator(), shared_utilities.hpp:43-49)
+__device__ __forceinline__ double poly5_eval(const double coef[6], double x)
+{
+#pragma clang fp contract(off)
+    double v = 0.0, p = 1.0;
+#pragma unroll
+    for (int i = 0; i <= 5; ++i) {
+        v += coef[i] * p;
+        p *= x;
+    }
+    return v;
+}
+
+// Brent root of the quintic in [x1,x2] with the reference's control flow
+// (zeroin, shared_utilities.hpp:109-173; abs() of doubles is fabs())
+__device__ __noinline__ double zeroin_poly5(const double* coef, double x1, double x2, double toler)
+{
+#pragma clang fp contract(off)
+    const int ITMAX = 100;
+    double d = 0, e = 0, p, q, r, s, xm;
+    double a = x1, b = x2, c = x2;
+    double fa = poly5_eval(coef, a), fb = poly5_eval(coef, b), fc = fb;
+    for (int it = 0; it < ITMAX; ++it) {
+        if ((fb > 0.0 && fc > 0.0) || (fb < 0.0 && fc < 0.0)) {
+            c = a; fc = fa;
+            d = b - a; e = d;
+        }
+        if (fabs(fc) < fabs(fb)) {
+            a = b; b = c; c = a;
+            fa = fb; fb = fc; fc = fa;
+        }
+        xm = 0.5 * (c - b);
+        if (fabs(xm) <= toler || fb == 0.0) return b;
+        if (fabs(e) < toler || fabs(fa) <= fabs(fb)) {
+            d = xm; e = xm;
+        } else {
+            s = fb / fa;
+            if (a == c) {
+                p = 2.0 * xm * s;
+                q = 1.0 - s;
+            } else {
+                q = fa / fc;
+                r = fb / fc;
+                p = s * (2.0 * xm * q * (q - r) - (b - a) * (r - 1.0));
+                q = (q - 1.0) * (r - 1.0) * (s - 1.0);
+            }
+            if (p > 0) q = -q;
+            p = fabs(p);
+            const double m1 = 3.0 * xm * q - fabs(toler * q);
+            const double m2 = fabs(e * q);
+            if (2.0 * p < dmin(m1, m2)) {
+                e = d; d = p / q;
+            } else {
+                d = xm; e = d;
+            }
+        }
+        a = b; fa = fb;
+        if (fabs(d) > toler) b = b + d;
+        else b += ksign(xm) * toler;
+        fb = poly5_eval(coef, b);
+    }
+    return b;
+}
+
+// six_distance: boundary_location.cu:53-80
+__device__ __noinline__ double six_distance(double v0, double v1, double v2, double v3, double v4, double v5,
+                                            double s)
+{
+#pragma clang fp contract(off)
+    if (v1 * v2 < 0 || v3 * v4 < 0) return eno_quadratic_distance(v1, v2, v3, v4, s);
+    const double xx[6] = {0.0, s, 2 * s, 3 * s, 4 * s, 5 * s};
+    const double yy[6] = {v0, v1, v2, v3, v4, v5};
+    double coef[6];
+    poly5_interp(xx, yy, coef);
+    double xc = zeroin_poly5(coef, 2 * s, 3 * s, 3e-16);
+    xc = dmax(xc, 2 * s);
+    xc = dmin(xc, 3 * s);
+    return xc - 2 * s;
+}
+
+// sixth_interp: 2_2_WENO5RK3_Extend/boundary_interpolant.cu:77-99
+__device__ __noinline__ void sixth_interp(double& c_fore, double& c_back, double dis_f, double dis_b, double s,
+                                          const double* v)
+{
+#pragma clang fp contract(off)
+    c_fore = 0.0;
+    c_back = 0.0;
+    if (dis_f < s) {
+        const double xf[6] = {s, 2 * s, 3 * s, 4 * s, 5 * s, 6 * s};
+        double coef[6];
+        poly5_interp(xf, v + 1, coef);
+        c_fore = poly5_eval(coef, 3 * s + dis_f);
+    }
+    if (dis_b < s) {
+        const double xb[6] = {0.0, s, 2 * s, 3 * s, 4 * s, 5 * s};
+        double coef[6];
+        poly5_interp(xb, v, coef);
+        c_back = poly5_eval(coef, 3 * s - dis_b);
+    }
+}
+
+}  // namespace vls

@@ -1,0 +1,949 @@
+// HIP kernels of the level-set hot path for gfx950.
+//
+// Layout: double[rows*cols*pges], ind = p*rows*cols + c*rows + r (rows = y is
+// the contiguous axis).  A block is 64 rows x 4 cols of one z-page: every
+// wave64 reads 64 consecutive doubles (one 512-byte segment) per stencil tap
+// along x and z, and the +-3 y-taps of a wave fall in the same or the two
+// neighbouring segments.  All neighbour indices are clamped to the box.
+//
+// Two families:
+//   * drop-in kernels with the exact argument meaning of the reference's
+//     __global__ entry points (re_step, extend_step, ...);
+//   * fused stage kernels that evaluate one operator and its Shu-Osher RK
+//     combination in one pass (the reference runs the combination as separate
+//     MATLAB gpuArray kernels), read the six subcell distances only where a
+//     per-point flag says they differ from the grid spacing, and derive the
+//     pseudo-time step 0.3*min_dist from them.
+#include "vls_device.hpp"
+#include "vls_kernels.hpp"
+
+namespace vls {
+
+// ---------------------------------------------------------------------------
+// launch geometry helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool point_of_thread(const Box& b, int& r, int& c, int& p)
+{
+    r = blockIdx.x * blockDim.x + threadIdx.x;
+    c = blockIdx.y * blockDim.y + threadIdx.y;
+    p = blockIdx.z;
+    return r < b.rows && c < b.cols;
+}
+
+__device__ __forceinline__ idx_t lin(const Box& b, int r, int c, int p)
+{
+    return (idx_t)p * b.rows * b.cols + (idx_t)c * b.rows + r;
+}
+
+// RK combinations in the order the MATLAB wrappers evaluate them
+// (+SD/@SDF3/WENO5RK3Reinitialization.m:20-37), without fused multiply-adds.
+__device__ __forceinline__ double rk_combine(int stage, double fs, double step, double fn)
+{
+#pragma clang fp contract(off)
+    const double t = fs - step;
+    if (stage == 0) return t;
+    if (stage == 1) return 3. / 4. * fn + 1. / 4. * t;
+    return 1. / 3. * fn + 2. / 3. * t;
+}
+
+// ---------------------------------------------------------------------------
+// setDistance: fill, crossing distances, min_dist + flags
+// ---------------------------------------------------------------------------
+__global__ void k_fill_spacing(Dist6 d, Box b, idx_t n)
+{
+    const idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    d.r[0][i] = b.dx; d.l[0][i] = b.dx;
+    d.r[1][i] = b.dy; d.l[1][i] = b.dy;
+    d.r[2][i] = b.dz; d.l[2][i] = b.dz;
+}
+
+// boundary_location (order 6, 1_3_WENO5RK3_Reinitialization/boundary_location.cu:84-132)
+// and boundary_correction (order 2, 1_0_ENORK2_Reinitialization/boundary_correction.cu:103-145).
+// One writer per element: r-side at ind, l-side at the +1 neighbour.
+template <int ORDER>
+__global__ void __launch_bounds__(256) k_boundary_location(Dist6 d, const double* __restrict__ lsf, Box b)
+{
+    int r, c, p;
+    if (!point_of_thread(b, r, c, p)) return;
+    const idx_t ind = lin(b, r, c, p);
+    const double f0 = lsf[ind];
+    const double sp[3] = {b.dx, b.dy, b.dz};
+#pragma unroll 1
+    for (int axis = 0; axis < 3; ++axis) {
+        int rr = r, cc = c, pp = p;
+        if (axis == 0) cc = min(c + 1, b.cols - 1);
+        else if (axis == 1) rr = min(r + 1, b.rows - 1);
+        else pp = min(p + 1, b.pges - 1);
+        const idx_t nb = lin(b, rr, cc, pp);
+        const double f2 = lsf[nb];
+        if (!(f0 * f2 < 0)) continue;
+        double v[7];
+        line7(lsf, r, c, p, axis, b, v);
+        const double ds = sp[axis];
+        const double dist = (ORDER == 2) ? eno_quadratic_distance(v[2], f0, f2, v[5], ds)
+                                         : six_distance(v[1], v[2], f0, f2, v[5], v[6], ds);
+        d.r[axis][ind] = dist;
+        d.l[axis][nb] = ds - dist;
+    }
+}
+
+// min_dist (setDistance.m:17-21) and the per-point flags of the fused kernels.
+// mask_src may be null (no mask bit).
+__global__ void k_min_dist_flags(double* __restrict__ min_dist, uint8_t* __restrict__ flags, Dist6c d,
+                                 const double* __restrict__ mask_src, Box b, idx_t n)
+{
+    const idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double xr = d.r[0][i], xl = d.l[0][i], yf = d.r[1][i], yb = d.l[1][i], zu = d.r[2][i], zd = d.l[2][i];
+    if (min_dist) min_dist[i] = dmin(dmin(xr, xl), dmin(dmin(yf, yb), dmin(zu, zd)));
+    if (flags) {
+        uint8_t f = 0;
+        if (xr < b.dx || xl < b.dx || yf < b.dy || yb < b.dy || zu < b.dz || zd < b.dz) f |= FLAG_SUBCELL;
+        if (mask_src && mask_src[i] < 0) f |= FLAG_MASK;
+        flags[i] = f;
+    }
+}
+
+// mask bit only (reinit with distances from an earlier setDistance)
+__global__ void k_set_mask_bit(uint8_t* __restrict__ flags, const double* __restrict__ F, idx_t n)
+{
+    const idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint8_t f = flags[i] & (uint8_t)~FLAG_MASK;
+    if (F[i] < 0) f |= FLAG_MASK;
+    flags[i] = f;
+}
+
+// ---------------------------------------------------------------------------
+// WENO5 reinitialization residual (re_step.cu:283-393)
+// ---------------------------------------------------------------------------
+struct Dists {
+    double r[3], l[3];
+};
+
+__device__ __forceinline__ bool frozen(const Dists& q, double dx)
+{
+    const double eps = 1e-6 * dx;
+    return q.r[0] < eps || q.l[0] < eps || q.r[1] < eps || q.l[1] < eps || q.r[2] < eps || q.l[2] < eps;
+}
+
+__device__ __forceinline__ double min6(const Dists& q)
+{
+    return dmin(dmin(q.r[0], q.l[0]), dmin(dmin(q.r[1], q.l[1]), dmin(q.r[2], q.l[2])));
+}
+
+// per-axis derivative pair with the sign-change interface test (re_step.cu:248-269)
+__device__ __forceinline__ void reinit_axis(double& fore, double& back, const double v[7], double r1, double l1,
+                                            double ds, double inv_ds)
+{
+    if (v[2] * v[3] < 0 || v[3] * v[4] < 0)
+        weno_nonuniform_pair(fore, back, v, r1, l1, ds, 0.0, 0.0);
+    else
+        weno5_pair(fore, back, v, inv_ds);
+}
+
+__device__ __forceinline__ double reinit_residual(const double* __restrict__ f, int r, int c, int p, const Box& b,
+                                                  const Dists& q, bool inside, double dt)
+{
+    const double sp[3] = {b.dx, b.dy, b.dz};
+    double fo[3], bk[3];
+#pragma unroll 1
+    for (int a = 0; a < 3; ++a) {
+        double v[7];
+        line7(f, r, c, p, a, b, v);
+        reinit_axis(fo[a], bk[a], v, q.r[a], q.l[a], sp[a], 1.0 / sp[a]);
+    }
+    return godunov_step(inside, fo[0], bk[0], fo[1], bk[1], fo[2], bk[2], dt);
+}
+
+// drop-in re_step
+__global__ void __launch_bounds__(256) k_re_step(double* __restrict__ step, const double* __restrict__ lsf,
+                                                 const uint8_t* __restrict__ mask,
+                                                 const double* __restrict__ deltat, Dist6c d, Box b)
+{
+    int r, c, p;
+    if (!point_of_thread(b, r, c, p)) return;
+    const idx_t ind = lin(b, r, c, p);
+    Dists q;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { q.r[a] = d.r[a][ind]; q.l[a] = d.l[a][ind]; }
+    if (frozen(q, b.dx)) { step[ind] = 0; return; }
+    step[ind] = reinit_residual(lsf, r, c, p, b, q, mask[ind] != 0, deltat[ind]);
+}
+
+// fused WENO5-RK3 reinitialization stage: out = combine(fs - L(fs), fn)
+__global__ void __launch_bounds__(256) k_reinit_stage(double* __restrict__ out, const double* __restrict__ fs,
+                                                      const double* __restrict__ fn,
+                                                      const uint8_t* __restrict__ flags, Dist6c d, Box b,
+                                                      int stage)
+{
+    int r, c, p;
+    if (!point_of_thread(b, r, c, p)) return;
+    const idx_t ind = lin(b, r, c, p);
+    const uint8_t fl = flags[ind];
+    Dists q;
+    if (fl & FLAG_SUBCELL) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) { q.r[a] = d.r[a][ind]; q.l[a] = d.l[a][ind]; }
+    } else {
+        q.r[0] = q.l[0] = b.dx; q.r[1] = q.l[1] = b.dy; q.r[2] = q.l[2] = b.dz;
+    }
+    const double fsv = fs[ind];
+    const double fnv = (stage == 0) ? 0.0 : fn[ind];
+    double step = 0.0;
+    if (!((fl & FLAG_SUBCELL) && frozen(q, b.dx)))
+        step = reinit_residual(fs, r, c, p, b, q, (fl & FLAG_MASK) != 0, 0.3 * min6(q));
+    out[ind] = rk_combine(stage, fsv, step, fnv);
+}
+
+// ---------------------------------------------------------------------------
+// ENO2 reinitialization (1_0_ENORK2_Reinitialization/boundary_correction.cu:147-200)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double eno2_residual(const double* __restrict__ f, int r, int c, int p, const Box& b,
+                                                const Dists& q, bool inside, double dt)
+{
+    const double sp[3] = {b.dx, b.dy, b.dz};
+    double fo[3], bk[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        double v[7];
+        line7(f, r, c, p, a, b, v);
+        eno2_pair(fo[a], bk[a], v, q.r[a], q.l[a], sp[a]);
+    }
+    return godunov_step(inside, fo[0], bk[0], fo[1], bk[1], fo[2], bk[2], dt);
+}
+
+__global__ void __launch_bounds__(256) k_eno2_re_step(double* __restrict__ step, const double* __restrict__ lsf,
+                                                      const uint8_t* __restrict__ mask,
+                                                      const double* __restrict__ deltat, Dist6c d, Box b)
+{
+    int r, c, p;
+    if (!point_of_thread(b, r, c, p)) return;
+    const idx_t ind = lin(b, r, c, p);
+    Dists q;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { q.r[a] = d.r[a][ind]; q.l[a] = d.l[a][ind]; }
+    step[ind] = eno2_residual(lsf, r, c, p, b, q, mask[ind] != 0, deltat[ind]);
+}
+
+// fused ENO2 stage.  rk: 0 = forward Euler (out = fs - L(fs)); 1 = Heun stage 1
+// (out = fs - L(fs)); 2 = Heun stage 2 (out = (fn + fs - L(fs))/2,
+// ENORK2Reinitialization.m:39).  dt = cfl*min_dist.
+__global__ void __launch_bounds__(256) k_eno2_stage(double* __restrict__ out, const double* __restrict__ fs,
+                                                    const double* __restrict__ fn, const uint8_t* __restrict__ flags,
+                                                    Dist6c d, Box b, double cfl, int heun2)
+{
+    int r, c, p;
+    if (!point_of_thread(b, r, c, p)) return;
+    const idx_t ind = lin(b, r, c, p);
+    const uint8_t fl = flags[ind];
+    Dists q;
+    if (fl & FLAG_SUBCELL) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) { q.r[a] = d.r[a][ind]; q.l[a] = d.l[a][ind]; }
+    } else {
+        q.r[0] = q.l[0] = b.dx; q.r[1] = q.l[1] = b.dy; q.r[2] = q.l[2] = b.dz;
+    }
+    const double step = eno2_residual(fs, r, c, p, b, q, (fl & FLAG_MASK) != 0, cfl * min6(q));
+    out[ind] = heun2 ? (fn[ind] + fs[ind] - step) / 2 : fs[ind] - step;
+}
+
+// ---------------------------------------------------------------------------
+// extension (2_2_WENO5RK3_Extend)
+// ---------------------------------------------------------------------------
+// per-axis pair with the distance interface test (shared_utilities.cup:256-278)
+__device__ __forceinline__ void extend_axis(double& fore, double& back, const double v[7], double r1, double l1,
+                                            double ds, double inv_ds, double h_fore, double h_back)
+{
+    if (r1 < ds || l1 < ds)
+        weno_nonuniform_pair(fore, back, v, r1, l1, ds, h_fore, h_back);
+    else
+        weno5_pair(fore, back, v, inv_ds);
+}
+
+// upwind_normal: upwind_normal.cu:4-104
+__global__ void __launch_bounds__(256) k_upwind_normal(double* __restrict__ nx, double* __restrict__ ny,
+                                                       double* __restrict__ nz, const double* __restrict__ lsf,
+                                                       Dist6c d, Box b)
+{
+    int r, c, p;
+    if (!point_of_thread(b, r, c, p)) return;
+    const idx_t ind = lin(b, r, c, p);
+    const double sp[3] = {b.dx, b.dy, b.dz};
+    double* out[3] = {nx, ny, nz};
+#pragma unroll 1
+    for (int a = 0; a < 3; ++a) {
+        double v[7], fo, bk;
+        line7(lsf, r, c, p, a, b, v);
+        extend_axis(fo, bk, v, d.r[a][ind], d.l[a][ind], sp[a], 1.0 / sp[a], 0.0, 0.0);
+        out[a][ind] = (fabs(v[4]) < fabs(v[2])) ? fo : bk;
+    }
+}
+
+// v = sign(F)*n/max(|n|,floor) in place on (vx,vy,vz) (WENO5RK3Extend.m:23-31)
+__global__ void k_extension_velocity(double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz,
+                                     const double* __restrict__ F, double floor_, idx_t n)
+{
+    const idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double fx = vx[i], fy = vy[i], fz = vz[i];
+    const double g = dmax(sqrt(fx * fx + fy * fy + fz * fz), floor_);
+    const double s = msign(F[i]);
+    vx[i] = s * (fx / g);
+    vy[i] = s * (fy / g);
+    vz[i] = s * (fz / g);
+}
+
+// boundary_interpolant: boundary_interpolant.cu:102-141
+__global__ void __launch_bounds__(256) k_boundary_interpolant(Dist6 cp, Dist6c d, const double* __restrict__ C,
+                                                              Box b)
+{
+    int r, c, p;
+    if (!point_of_thread(b, r, c, p)) return;
+    const idx_t ind = lin(b, r, c, p);
+    const double sp[3] = {b.dx, b.dy, b.dz};
+#pragma unroll 1
+    for (int a = 0; a < 3; ++a) {
+        const double rf = d.r[a][ind], lb = d.l[a][ind];
+        double cf = 0.0, cb = 0.0;
+        if (rf < sp[a] || lb < sp[a]) {
+            double v[7];
+            line7(C, r, c, p, a, b, v);
+            sixth_interp(cf, cb, rf, lb, sp[a], v);
+        }
+        cp.r[a][ind] = cf;
+        cp.l[a][ind] = cb;
+    }
+}
+
+__device__ __forceinline__ double extend_rate(const double* __restrict__ C, int r, int c, int p, idx_t ind,
+                                              const Box& b, const Dists& q, const double vel[3], const Dist6c& cp,
+                                              bool subcell)
+{
+    const double sp[3] = {b.dx, b.dy, b.dz};
+    double acc = 0.0;
+#pragma unroll 1
+    for (int a = 0; a < 3; ++a) {
+        double v[7], fo, bk;
+        line7(C, r, c, p, a, b, v);
+        double hf = 0.0, hb = 0.0;
+        if (subcell) { hf = cp.r[a][ind]; hb = cp.l[a][ind]; }
+        extend_axis(fo, bk, v, q.r[a], q.l[a], sp[a], 1.0 / sp[a], hf, hb);
+        // extend_step.cu:121-123 sums x, y, z terms left to right
+        acc = acc + dmin(0.0, vel[a]) * fo;
+        acc = acc + dmax(0.0, vel[a]) * bk;
+    }
+    return acc;
+}
+
+// drop-in extend_step: extend_step.cu:10-125
+__global__ void __launch_bounds__(256) k_extend_step(double* __restrict__ step, const double* __restrict__ deltat,
+                                                     const double* __restrict__ C, const double* __restrict__ vx,
+                                                     const double* __restrict__ vy, const double* __restrict__ vz,
+                                                     Dist6c d, Dist6c cp, Box b)
+{
+    int r, c, p;
+    if (!point_of_thread(b, r, c, p)) return;
+    const idx_t ind = lin(b, r, c, p);
+    Dists q;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { q.r[a] = d.r[a][ind]; q.l[a] = d.l[a][ind]; }
+    if (frozen(q, b.dx)) { step[ind] = 0; return; }
+    const double vel[3] = {vx[ind], vy[ind], vz[ind]};
+    step[ind] = extend_rate(C, r, c, p, ind, b, q, vel, cp, true) * deltat[ind];
+}
+
+// fused WENO5-RK3 extension stage
+__global__ void __launch_bounds__(256) k_extend_stage(double* __restrict__ out, const double* __restrict__ cs,
+                                                      const double* __restrict__ cn, const double* __restrict__ vx,
+                                                      const double* __restrict__ vy, const double* __restrict__ vz,
+                                                      const uint8_t* __restrict__ flags, Dist6c d, Dist6c cp,
+                                                      Box b, int stage)
+{
+    int r, c, p;
+    if (!point_of_thread(b, r, c, p)) return;
+    const idx_t ind = lin(b, r, c, p);
+    const bool sub = (flags[ind] & FLAG_SUBCELL) != 0;
+    Dists q;
+    if (sub) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) { q.r[a] = d.r[a][ind]; q.l[a] = d.l[a][ind]; }
+    } else {
+        q.r[0] = q.l[0] = b.dx; q.r[1] = q.l[1] = b.dy; q.r[2] = q.l[2] = b.dz;
+    }
+    const double csv = cs[ind];
+    const double cnv = (stage == 0) ? 0.0 : cn[ind];
+    double step = 0.0;
+    if (!(sub && frozen(q, b.dx))) {
+        const double vel[3] = {vx[ind], vy[ind], vz[ind]};
+        step = extend_rate(cs, r, c, p, ind, b, q, vel, cp, sub) * (0.3 * min6(q));
+    }
+    out[ind] = rk_combine(stage, csv, step, cnv);
+}
+
+// ---------------------------------------------------------------------------
+// WENO5 advection operator (4_0_Calculus_ToolBox/advection_step.cu:30-56) and
+// its six-output twin (5_0_Nonoscillating_Reconstruction/nonoscillating_interpolant.cu:59-122)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double advect_rate(const double* __restrict__ f, int r, int c, int p, const Box& b,
+                                              const double vel[3])
+{
+    const double sp[3] = {b.dx, b.dy, b.dz};
+    double acc = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        double v[7], fo, bk;
+        line7(f, r, c, p, a, b, v);
+        weno5_pair(fo, bk, v, 1.0 / sp[a]);
+        acc = acc + dmin(0.0, vel[a]) * fo;
+        acc = acc + dmax(0.0, vel[a]) * bk;
+    }
+    return acc;
+}
+
+__global__ void __launch_bounds__(256) k_advection_step(double* __restrict__ astep, const double* __restrict__ vx,
+                                                        const double* __restrict__ vy,
+                                                        const double* __restrict__ vz,
+                                                        const double* __restrict__ lsf, Box b)
+{
+    int r, c, p;
+    if (!point_of_thread(b, r, c, p)) return;
+    const idx_t ind = lin(b, r, c, p);
+    const double vel[3] = {vx[ind], vy[ind], vz[ind]};
+    astep[ind] = advect_rate(lsf, r, c, p, b, vel);
+}
+
+// fused TVD-RK3 advection stage: out = combine(fs - dt*L(fs), fn)
+__global__ void __launch_bounds__(256) k_advect_stage(double* __restrict__ out, const double* __restrict__ fs,
+                                                      const double* __restrict__ fn, const double* __restrict__ vx,
+                                                      const double* __restrict__ vy, const double* __restrict__ vz,
+                                                      double dt, Box b, int stage)
+{
+    int r, c, p;
+    if (!point_of_thread(b, r, c, p)) return;
+    const idx_t ind = lin(b, r, c, p);
+    const double vel[3] = {vx[ind], vy[ind], vz[ind]};
+    const double a = advect_rate(fs, r, c, p, b, vel);
+    const double fnv = (stage == 0) ? 0.0 : fn[ind];
+    double step;
+    {
+#pragma clang fp contract(off)
+        step = dt * a;
+    }
+    out[ind] = rk_combine(stage, fs[ind], step, fnv);
+}
+
+__global__ void __launch_bounds__(256) k_weno_derivative(Dist6 out, const double* __restrict__ lsf, Box b)
+{
+    int r, c, p;
+    if (!point_of_thread(b, r, c, p)) return;
+    const idx_t ind = lin(b, r, c, p);
+    const double sp[3] = {b.dx, b.dy, b.dz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        double v[7], fo, bk;
+        line7(lsf, r, c, p, a, b, v);
+        weno5_pair(fo, bk, v, 1.0 / sp[a]);
+        out.l[a][ind] = bk;  // back_x, back_y, back_z
+        out.r[a][ind] = fo;  // fore_x, fore_y, fore_z
+    }
+}
+
+// ---------------------------------------------------------------------------
+// calculus toolbox (4_0_Calculus_ToolBox/set_calculus_toolbox.cu:9-107)
+// ---------------------------------------------------------------------------
+struct Box27 {
+    double v[3][3][3];  // [dz+1][dc+1][dr+1]
+};
+
+__device__ __forceinline__ void load_box27(Box27& o, const double* __restrict__ f, int r, int c, int p, const Box& b)
+{
+#pragma unroll
+    for (int k = -1; k <= 1; ++k) {
+        const int pp = clampi(p + k, b.pges - 1);
+#pragma unroll
+        for (int j = -1; j <= 1; ++j) {
+            const int cc = clampi(c + j, b.cols - 1);
+#pragma unroll
+            for (int i = -1; i <= 1; ++i) {
+                const int rr = clampi(r + i, b.rows - 1);
+                o.v[k + 1][j + 1][i + 1] = f[(idx_t)pp * b.rows * b.cols + (idx_t)cc * b.rows + rr];
+            }
+        }
+    }
+}
+
+struct Toolbox {
+    double fx, fy, fz, g, fxx, fyy, fzz, fxy, fyz, fzx, lap;
+};
+
+// F_(dr,dc,dp) of the reference's neighbour names
+#define T27(o, dr, dc, dp) (o).v[(dp) + 1][(dc) + 1][(dr) + 1]
+
+__device__ __forceinline__ Toolbox toolbox_of(const Box27& F, const Box& b, double ds)
+{
+    Toolbox t;
+    t.fx = (T27(F, 0, 1, 0) - T27(F, 0, -1, 0)) / (2 * b.dx);
+    t.fy = (T27(F, 1, 0, 0) - T27(F, -1, 0, 0)) / (2 * b.dy);
+    t.fz = (T27(F, 0, 0, 1) - T27(F, 0, 0, -1)) / (2 * b.dz);
+    t.g = dmax(sqrt(t.fx * t.fx + t.fy * t.fy + t.fz * t.fz), 1e-14);
+    const double f0 = T27(F, 0, 0, 0);
+    t.fxx = (T27(F, 0, 1, 0) - 2 * f0 + T27(F, 0, -1, 0)) / (b.dx * b.dx);
+    t.fyy = (T27(F, 1, 0, 0) - 2 * f0 + T27(F, -1, 0, 0)) / (b.dy * b.dy);
+    t.fzz = (T27(F, 0, 0, 1) - 2 * f0 + T27(F, 0, 0, -1)) / (b.dz * b.dz);
+    const double den = 4 * ds * ds;
+    t.fxy = (T27(F, 1, 1, 0) + T27(F, -1, -1, 0) - T27(F, 1, -1, 0) - T27(F, -1, 1, 0)) / den;
+    t.fyz = (T27(F, 1, 0, 1) + T27(F, -1, 0, -1) - T27(F, 1, 0, -1) - T27(F, -1, 0, 1)) / den;
+    t.fzx = (T27(F, 0, 1, 1) + T27(F, 0, -1, -1) - T27(F, 0, 1, -1) - T27(F, 0, -1, 1)) / den;
+    t.lap = t.fxx + t.fyy + t.fzz;
+    return t;
+}
+
+// Towers primal-function Heaviside and delta (set_calculus_toolbox.cu:183-198)
+__device__ __forceinline__ void towers_h_delta(double& H, double& delta, const Box27& P, const Toolbox& t,
+                                               const Box& b)
+{
+    const double px = (T27(P, 0, 1, 0) - T27(P, 0, -1, 0)) / (2 * b.dx);
+    const double py = (T27(P, 1, 0, 0) - T27(P, -1, 0, 0)) / (2 * b.dy);
+    const double pz = (T27(P, 0, 0, 1) - T27(P, 0, 0, -1)) / (2 * b.dz);
+    const double dot = px * t.fx + py * t.fy + pz * t.fz;
+    const double g2 = t.g * t.g;
+    H = dot / g2;
+    const double p0 = T27(P, 0, 0, 0);
+    const double pxx = (T27(P, 0, 1, 0) - 2 * p0 + T27(P, 0, -1, 0)) / (b.dx * b.dx);
+    const double pyy = (T27(P, 1, 0, 0) - 2 * p0 + T27(P, -1, 0, 0)) / (b.dy * b.dy);
+    const double pzz = (T27(P, 0, 0, 1) - 2 * p0 + T27(P, 0, 0, -1)) / (b.dz * b.dz);
+    delta = (pxx + pyy + pzz) / g2 - dot * t.lap / (g2 * t.g * t.g);
+}
+
+__global__ void __launch_bounds__(256) k_set_calculus_toolbox(Fields18 o, const double* __restrict__ lsf,
+                                                              const double* __restrict__ hp, Box b, double ds)
+{
+    int r, c, p;
+    if (!point_of_thread(b, r, c, p)) return;
+    const idx_t ind = lin(b, r, c, p);
+    Box27 F, P;
+    load_box27(F, lsf, r, c, p, b);
+    load_box27(P, hp, r, c, p, b);
+    const Toolbox t = toolbox_of(F, b, ds);
+    o.f[0][ind] = t.fx; o.f[1][ind] = t.fy; o.f[2][ind] = t.fz; o.f[3][ind] = t.g;
+    o.f[4][ind] = t.fx / t.g; o.f[5][ind] = t.fy / t.g; o.f[6][ind] = t.fz / t.g;
+    o.f[7][ind] = t.fxx; o.f[8][ind] = t.fyy; o.f[9][ind] = t.fzz;
+    o.f[10][ind] = t.fxy; o.f[11][ind] = t.fyz; o.f[12][ind] = t.fzx; o.f[13][ind] = t.lap;
+    double c1 = t.fxx * t.fx + t.fxy * t.fy + t.fzx * t.fz;
+    double c2 = t.fxy * t.fx + t.fyy * t.fy + t.fyz * t.fz;
+    double c3 = t.fzx * t.fx + t.fyz * t.fy + t.fzz * t.fz;
+    const double g3 = t.g * t.g * t.g;
+    o.f[14][ind] = -t.lap / t.g + (t.fx * c1 + t.fy * c2 + t.fz * c3) / g3;
+    c1 = (t.fyy * t.fzz - t.fyz * t.fyz) * t.fx + (t.fzx * t.fyz - t.fxy * t.fzz) * t.fy
+         + (t.fxy * t.fyz - t.fzx * t.fyy) * t.fz;
+    c2 = (t.fyz * t.fzx - t.fxy * t.fzz) * t.fx + (t.fxx * t.fzz - t.fzx * t.fzx) * t.fy
+         + (t.fzx * t.fxy - t.fxx * t.fyz) * t.fz;
+    c3 = (t.fxy * t.fyz - t.fyy * t.fzx) * t.fx + (t.fzx * t.fxy - t.fxx * t.fyz) * t.fy
+         + (t.fxx * t.fyy - t.fxy * t.fxy) * t.fz;
+    o.f[15][ind] = (t.fx * c1 + t.fy * c2 + t.fz * c3) / (g3 * t.g);
+    double H, delta;
+    towers_h_delta(H, delta, P, t, b);
+    o.f[16][ind] = H;
+    o.f[17][ind] = delta;
+}
+#undef T27
+
+// normal component of a velocity, u.N with N = grad F/max(|grad F|,1e-14) from
+// clamped central differences (set_calculus_toolbox.cu:123-136)
+__global__ void __launch_bounds__(256) k_normal_velocity(double* __restrict__ out, const double* __restrict__ F,
+                                                         const double* __restrict__ vx, const double* __restrict__ vy,
+                                                         const double* __restrict__ vz, Box b)
+{
+    int r, c, p;
+    if (!point_of_thread(b, r, c, p)) return;
+    const idx_t ind = lin(b, r, c, p);
+    const double fx = (F[lin(b, r, min(c + 1, b.cols - 1), p)] - F[lin(b, r, max(c - 1, 0), p)]) / (2 * b.dx);
+    const double fy = (F[lin(b, min(r + 1, b.rows - 1), c, p)] - F[lin(b, max(r - 1, 0), c, p)]) / (2 * b.dy);
+    const double fz = (F[lin(b, r, c, min(p + 1, b.pges - 1))] - F[lin(b, r, c, max(p - 1, 0))]) / (2 * b.dz);
+    const double g = dmax(sqrt(fx * fx + fy * fy + fz * fz), 1e-14);
+    out[ind] = vx[ind] * (fx / g) + vy[ind] * (fy / g) + vz[ind] * (fz / g);
+}
+
+// ---------------------------------------------------------------------------
+// compensated reductions (area, volume, max|u|)
+// ---------------------------------------------------------------------------
+// Neumaier accumulator
+struct KSum {
+    double s, c;
+    __device__ __forceinline__ void add(double x)
+    {
+        const double t = s + x;
+        c += (fabs(s) >= fabs(x)) ? ((s - t) + x) : ((x - t) + s);
+        s = t;
+    }
+    __device__ __forceinline__ void merge(const KSum& o)
+    {
+        add(o.s);
+        c += o.c;
+    }
+};
+
+__device__ __forceinline__ double shfl_xor_d(double v, int m)
+{
+    return __shfl_xor(v, m, 64);
+}
+
+__device__ __forceinline__ void wave_reduce(KSum& k)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        KSum o;
+        o.s = shfl_xor_d(k.s, m);
+        o.c = shfl_xor_d(k.c, m);
+        k.merge(o);
+    }
+}
+
+// block of 256 threads -> thread 0 holds the merged (s,c) of each quantity
+template <int NQ>
+__device__ __forceinline__ void block_reduce(KSum (&k)[NQ])
+{
+    __shared__ double lds[NQ][2][4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        wave_reduce(k[q]);
+        if (lane == 0) { lds[q][0][wave] = k[q].s; lds[q][1][wave] = k[q].c; }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            KSum t{0.0, 0.0};
+            for (int w = 0; w < 4; ++w) { KSum o{lds[q][0][w], lds[q][1][w]}; t.merge(o); }
+            k[q] = t;
+        }
+    }
+}
+
+template <int NQ>
+__device__ __forceinline__ void block_reduce_store(KSum (&k)[NQ], double* __restrict__ partial)
+{
+    block_reduce<NQ>(k);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            partial[(size_t)blockIdx.x * 2 * NQ + 2 * q] = k[q].s;
+            partial[(size_t)blockIdx.x * 2 * NQ + 2 * q + 1] = k[q].c;
+        }
+}
+
+// cosine-smoothed delta/H of width eps (src/0_0_setup/+SD/@SDF3/SDF3.m:126-136);
+// |grad F| by periodic central differences (src/0_0_setup/+SD/@GD3/GD3.m:212-223)
+__global__ void __launch_bounds__(256) k_area_volume_cosine(double* __restrict__ partial, const double* __restrict__ F,
+                                                            Box b, double eps, idx_t n)
+{
+    KSum k[2] = {{0.0, 0.0}, {0.0, 0.0}};
+    const double pi = 3.14159265358979323846;
+    const idx_t stride = (idx_t)gridDim.x * blockDim.x;
+    for (idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const double v = F[i];
+        double delta = 0.0, heav;
+        if (v > eps) {
+            heav = 1.0;
+        } else if (v < -eps) {
+            heav = 0.0;
+        } else {
+            const idx_t plane = (idx_t)b.rows * b.cols;
+            const int p = (int)(i / plane);
+            const idx_t rem = i - (idx_t)p * plane;
+            const int c = (int)(rem / b.rows);
+            const int r = (int)(rem - (idx_t)c * b.rows);
+            const int cp = (c + 1 == b.cols) ? 0 : c + 1, cm = (c == 0) ? b.cols - 1 : c - 1;
+            const int rp = (r + 1 == b.rows) ? 0 : r + 1, rm = (r == 0) ? b.rows - 1 : r - 1;
+            const int pp = (p + 1 == b.pges) ? 0 : p + 1, pm = (p == 0) ? b.pges - 1 : p - 1;
+            const double fx = (F[(idx_t)p * plane + (idx_t)cp * b.rows + r] - F[(idx_t)p * plane + (idx_t)cm * b.rows + r]) / (2 * b.dx);
+            const double fy = (F[(idx_t)p * plane + (idx_t)c * b.rows + rp] - F[(idx_t)p * plane + (idx_t)c * b.rows + rm]) / (2 * b.dy);
+            const double fz = (F[(idx_t)pp * plane + (idx_t)c * b.rows + r] - F[(idx_t)pm * plane + (idx_t)c * b.rows + r]) / (2 * b.dz);
+            const double g = sqrt(fx * fx + fy * fy + fz * fz);
+            delta = g * (1 + cos(pi * v / eps)) / eps / 2;
+            heav = (1 + v / eps + sin(pi * v / eps) / pi) / 2;
+        }
+        k[0].add(delta);
+        k[1].add(1 - heav);
+    }
+    block_reduce_store<2>(k, partial);
+}
+
+// Towers delta/H area and volume (+SD/@SDF3/SDF3.m:73-80) fused with the
+// toolbox stencil: only points in the masks evaluate it.
+__global__ void __launch_bounds__(256) k_area_volume_towers(double* __restrict__ partial, const double* __restrict__ F,
+                                                            Box b, double ds, idx_t n)
+{
+    KSum k[2] = {{0.0, 0.0}, {0.0, 0.0}};
+    const idx_t stride = (idx_t)gridDim.x * blockDim.x;
+    for (idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const double v = F[i];
+        const bool in_area = fabs(v) < 2 * ds, in_vol = v < 2 * ds;
+        if (!(in_area || in_vol)) continue;
+        const idx_t plane = (idx_t)b.rows * b.cols;
+        const int p = (int)(i / plane);
+        const idx_t rem = i - (idx_t)p * plane;
+        const int c = (int)(rem / b.rows);
+        const int r = (int)(rem - (idx_t)c * b.rows);
+        Box27 Fb, Pb;
+        load_box27(Fb, F, r, c, p, b);
+#pragma unroll
+        for (int x = 0; x < 3; ++x)
+#pragma unroll
+            for (int y = 0; y < 3; ++y)
+#pragma unroll
+                for (int z = 0; z < 3; ++z) Pb.v[x][y][z] = dmax(Fb.v[x][y][z], 0.0);
+        const Toolbox t = toolbox_of(Fb, b, ds);
+        double H, delta;
+        towers_h_delta(H, delta, Pb, t, b);
+        if (in_area) k[0].add(delta);
+        if (in_vol) k[1].add(1 - H);
+    }
+    block_reduce_store<2>(k, partial);
+}
+
+// final: one block of 256 merges `nblk` partial (s,c) pairs per quantity in a
+// fixed order; result[q] = s + c
+template <int NQ>
+__global__ void __launch_bounds__(256) k_reduce_final(double* __restrict__ result, const double* __restrict__ partial,
+                                                      int nblk, double scale)
+{
+    KSum k[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) k[q] = {0.0, 0.0};
+    for (int i = threadIdx.x; i < nblk; i += blockDim.x)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            KSum o{partial[(size_t)i * 2 * NQ + 2 * q], partial[(size_t)i * 2 * NQ + 2 * q + 1]};
+            k[q].merge(o);
+        }
+    block_reduce<NQ>(k);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) result[q] = (k[q].s + k[q].c) * scale;
+}
+
+__global__ void __launch_bounds__(256) k_max_abs3(double* __restrict__ partial, const double* __restrict__ a,
+                                                  const double* __restrict__ b2, const double* __restrict__ c3, idx_t n)
+{
+    double m = 0.0;
+    const idx_t stride = (idx_t)gridDim.x * blockDim.x;
+    for (idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        m = dmax(m, dmax(fabs(a[i]), dmax(fabs(b2[i]), fabs(c3[i]))));
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) m = dmax(m, shfl_xor_d(m, s));
+    __shared__ double lds[4];
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = dmax(dmax(lds[0], lds[1]), dmax(lds[2], lds[3]));
+}
+
+__global__ void k_max_final(double* __restrict__ result, const double* __restrict__ partial, int nblk)
+{
+    double m = 0.0;
+    for (int i = threadIdx.x; i < nblk; i += blockDim.x) m = dmax(m, partial[i]);
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) m = dmax(m, shfl_xor_d(m, s));
+    __shared__ double lds[4];
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) result[0] = dmax(dmax(lds[0], lds[1]), dmax(lds[2], lds[3]));
+}
+
+__global__ void k_copy(double* __restrict__ dst, const double* __restrict__ src, idx_t n)
+{
+    const idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+// explicit instantiations
+template __global__ void k_boundary_location<6>(Dist6, const double*, Box);
+template __global__ void k_boundary_location<2>(Dist6, const double*, Box);
+template __global__ void k_reduce_final<2>(double*, const double*, int, double);
+
+}  // namespace vls
+
+// ===========================================================================
+// host launchers
+// ===========================================================================
+namespace vls {
+
+static inline Box box_of(const Geom& g) { return Box{g.rows, g.cols, g.pges, g.dx, g.dy, g.dz}; }
+
+// one thread per point: 64 rows x 4 cols per block, one z-page per grid.z
+static inline void point_grid(const Geom& g, dim3& grid, dim3& block)
+{
+    block = dim3(64, 4, 1);
+    grid = dim3((g.rows + 63) / 64, (g.cols + 3) / 4, g.pges);
+}
+
+static inline dim3 flat_grid(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+#define VLS_POINT_LAUNCH(kernel, ...)                                   \
+    do {                                                                \
+        dim3 grid_, block_;                                             \
+        point_grid(g, grid_, block_);                                   \
+        hipLaunchKernelGGL(kernel, grid_, block_, 0, s, __VA_ARGS__);   \
+        return hipGetLastError();                                       \
+    } while (0)
+
+hipError_t launch_fill_spacing(const Dist6& d, const Geom& g, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill_spacing, flat_grid(g.n()), dim3(256), 0, s, d, box_of(g), (idx_t)g.n());
+    return hipGetLastError();
+}
+
+hipError_t launch_boundary_location(int order, const Dist6& d, const double* lsf, const Geom& g, hipStream_t s)
+{
+    if (order == 2) VLS_POINT_LAUNCH(k_boundary_location<2>, d, lsf, box_of(g));
+    VLS_POINT_LAUNCH(k_boundary_location<6>, d, lsf, box_of(g));
+}
+
+hipError_t launch_min_dist_flags(double* min_dist, uint8_t* flags, const Dist6c& d, const double* mask_src,
+                                 const Geom& g, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_min_dist_flags, flat_grid(g.n()), dim3(256), 0, s, min_dist, flags, d, mask_src,
+                       box_of(g), (idx_t)g.n());
+    return hipGetLastError();
+}
+
+hipError_t launch_set_mask_bit(uint8_t* flags, const double* F, const Geom& g, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_set_mask_bit, flat_grid(g.n()), dim3(256), 0, s, flags, F, (idx_t)g.n());
+    return hipGetLastError();
+}
+
+hipError_t launch_re_step(double* step, const double* lsf, const uint8_t* mask, const double* deltat,
+                          const Dist6c& d, const Geom& g, hipStream_t s)
+{
+    VLS_POINT_LAUNCH(k_re_step, step, lsf, mask, deltat, d, box_of(g));
+}
+
+hipError_t launch_eno2_re_step(double* step, const double* lsf, const uint8_t* mask, const double* deltat,
+                               const Dist6c& d, const Geom& g, hipStream_t s)
+{
+    VLS_POINT_LAUNCH(k_eno2_re_step, step, lsf, mask, deltat, d, box_of(g));
+}
+
+hipError_t launch_reinit_stage(double* out, const double* fs, const double* fn, const uint8_t* flags,
+                               const Dist6c& d, const Geom& g, int stage, hipStream_t s)
+{
+    VLS_POINT_LAUNCH(k_reinit_stage, out, fs, fn, flags, d, box_of(g), stage);
+}
+
+hipError_t launch_eno2_stage(double* out, const double* fs, const double* fn, const uint8_t* flags,
+                             const Dist6c& d, const Geom& g, double cfl, int heun2, hipStream_t s)
+{
+    VLS_POINT_LAUNCH(k_eno2_stage, out, fs, fn, flags, d, box_of(g), cfl, heun2);
+}
+
+hipError_t launch_upwind_normal(double* nx, double* ny, double* nz, const double* lsf, const Dist6c& d,
+                                const Geom& g, hipStream_t s)
+{
+    VLS_POINT_LAUNCH(k_upwind_normal, nx, ny, nz, lsf, d, box_of(g));
+}
+
+hipError_t launch_extension_velocity(double* vx, double* vy, double* vz, const double* F, double floor_,
+                                     const Geom& g, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_extension_velocity, flat_grid(g.n()), dim3(256), 0, s, vx, vy, vz, F, floor_,
+                       (idx_t)g.n());
+    return hipGetLastError();
+}
+
+hipError_t launch_boundary_interpolant(const Dist6& cp, const Dist6c& d, const double* C, const Geom& g,
+                                       hipStream_t s)
+{
+    VLS_POINT_LAUNCH(k_boundary_interpolant, cp, d, C, box_of(g));
+}
+
+hipError_t launch_extend_step(double* step, const double* deltat, const double* C, const double* vx,
+                              const double* vy, const double* vz, const Dist6c& d, const Dist6c& cp,
+                              const Geom& g, hipStream_t s)
+{
+    VLS_POINT_LAUNCH(k_extend_step, step, deltat, C, vx, vy, vz, d, cp, box_of(g));
+}
+
+hipError_t launch_extend_stage(double* out, const double* cs, const double* cn, const double* vx,
+                               const double* vy, const double* vz, const uint8_t* flags, const Dist6c& d,
+                               const Dist6c& cp, const Geom& g, int stage, hipStream_t s)
+{
+    VLS_POINT_LAUNCH(k_extend_stage, out, cs, cn, vx, vy, vz, flags, d, cp, box_of(g), stage);
+}
+
+hipError_t launch_advection_step(double* astep, const double* vx, const double* vy, const double* vz,
+                                 const double* lsf, const Geom& g, hipStream_t s)
+{
+    VLS_POINT_LAUNCH(k_advection_step, astep, vx, vy, vz, lsf, box_of(g));
+}
+
+hipError_t launch_advect_stage(double* out, const double* fs, const double* fn, const double* vx,
+                               const double* vy, const double* vz, double dt, const Geom& g, int stage,
+                               hipStream_t s)
+{
+    VLS_POINT_LAUNCH(k_advect_stage, out, fs, fn, vx, vy, vz, dt, box_of(g), stage);
+}
+
+hipError_t launch_normal_velocity(double* out, const double* F, const double* vx, const double* vy, const double* vz,
+                                  const Geom& g, hipStream_t s)
+{
+    VLS_POINT_LAUNCH(k_normal_velocity, out, F, vx, vy, vz, box_of(g));
+}
+
+hipError_t launch_weno_derivative(const Dist6& out, const double* lsf, const Geom& g, hipStream_t s)
+{
+    VLS_POINT_LAUNCH(k_weno_derivative, out, lsf, box_of(g));
+}
+
+hipError_t launch_set_calculus_toolbox(const Fields18& o, const double* lsf, const double* hp, const Geom& g,
+                                       double ds, hipStream_t s)
+{
+    VLS_POINT_LAUNCH(k_set_calculus_toolbox, o, lsf, hp, box_of(g), ds);
+}
+
+// 1024 blocks x 256 threads: 4 blocks per CU on 256 CUs
+int reduce_blocks() { return 1024; }
+
+static inline int blocks_for(int64_t n)
+{
+    const int64_t want = (n + 255) / 256;
+    return (int)(want < reduce_blocks() ? (want < 1 ? 1 : want) : reduce_blocks());
+}
+
+hipError_t launch_area_volume_cosine(double* result, double* partial, const double* F, const Geom& g, double eps,
+                                     double scale, hipStream_t s)
+{
+    const int nb = blocks_for(g.n());
+    hipLaunchKernelGGL(k_area_volume_cosine, dim3(nb), dim3(256), 0, s, partial, F, box_of(g), eps, (idx_t)g.n());
+    hipLaunchKernelGGL(k_reduce_final<2>, dim3(1), dim3(256), 0, s, result, partial, nb, scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_area_volume_towers(double* result, double* partial, const double* F, const Geom& g, double ds,
+                                     double scale, hipStream_t s)
+{
+    const int nb = blocks_for(g.n());
+    hipLaunchKernelGGL(k_area_volume_towers, dim3(nb), dim3(256), 0, s, partial, F, box_of(g), ds, (idx_t)g.n());
+    hipLaunchKernelGGL(k_reduce_final<2>, dim3(1), dim3(256), 0, s, result, partial, nb, scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_max_abs3(double* result, double* partial, const double* a, const double* b, const double* c,
+                           const Geom& g, hipStream_t s)
+{
+    const int nb = blocks_for(g.n());
+    hipLaunchKernelGGL(k_max_abs3, dim3(nb), dim3(256), 0, s, partial, a, b, c, (idx_t)g.n());
+    hipLaunchKernelGGL(k_max_final, dim3(1), dim3(256), 0, s, result, partial, nb);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(double* dst, const double* src, int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_copy, flat_grid(n), dim3(256), 0, s, dst, src, (idx_t)n);
+    return hipGetLastError();
+}
+
+}  // namespace vls

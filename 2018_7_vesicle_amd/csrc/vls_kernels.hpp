@@ -1,0 +1,90 @@
+// Internal interface between the kernels (vls_kernels.hip) and the C-ABI
+// layer (vls_api.hip).  Not installed; the public interface is
+// include/vesicle_ls.h.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vls {
+
+struct Box;
+
+// six per-axis arrays: r[a] = right/front/up distance (xpr, ypf, zpu),
+// l[a] = left/back/down distance (xpl, ypb, zpd); a = 0 x(cols), 1 y(rows), 2 z(pges)
+struct Dist6 {
+    double* r[3];
+    double* l[3];
+};
+struct Dist6c {
+    const double* r[3];
+    const double* l[3];
+};
+struct Fields18 {
+    double* f[18];
+};
+
+// per-point flag bits of the fused stage kernels
+enum : uint8_t {
+    FLAG_MASK = 1,     // phi0 < 0 at entry of the reinitialization
+    FLAG_SUBCELL = 2,  // some subcell distance differs from the grid spacing
+};
+
+inline Dist6c constify(const Dist6& d)
+{
+    return Dist6c{{d.r[0], d.r[1], d.r[2]}, {d.l[0], d.l[1], d.l[2]}};
+}
+
+// ---- host launchers (all asynchronous on `s`) -------------------------------
+struct Geom {
+    int rows, cols, pges;
+    double dx, dy, dz;
+    int64_t n() const { return (int64_t)rows * cols * pges; }
+};
+
+hipError_t launch_fill_spacing(const Dist6& d, const Geom& g, hipStream_t s);
+hipError_t launch_boundary_location(int order, const Dist6& d, const double* lsf, const Geom& g, hipStream_t s);
+hipError_t launch_min_dist_flags(double* min_dist, uint8_t* flags, const Dist6c& d, const double* mask_src,
+                                 const Geom& g, hipStream_t s);
+hipError_t launch_set_mask_bit(uint8_t* flags, const double* F, const Geom& g, hipStream_t s);
+hipError_t launch_re_step(double* step, const double* lsf, const uint8_t* mask, const double* deltat,
+                          const Dist6c& d, const Geom& g, hipStream_t s);
+hipError_t launch_eno2_re_step(double* step, const double* lsf, const uint8_t* mask, const double* deltat,
+                               const Dist6c& d, const Geom& g, hipStream_t s);
+hipError_t launch_reinit_stage(double* out, const double* fs, const double* fn, const uint8_t* flags,
+                               const Dist6c& d, const Geom& g, int stage, hipStream_t s);
+hipError_t launch_eno2_stage(double* out, const double* fs, const double* fn, const uint8_t* flags,
+                             const Dist6c& d, const Geom& g, double cfl, int heun2, hipStream_t s);
+hipError_t launch_upwind_normal(double* nx, double* ny, double* nz, const double* lsf, const Dist6c& d,
+                                const Geom& g, hipStream_t s);
+hipError_t launch_extension_velocity(double* vx, double* vy, double* vz, const double* F, double floor_,
+                                     const Geom& g, hipStream_t s);
+hipError_t launch_boundary_interpolant(const Dist6& cp, const Dist6c& d, const double* C, const Geom& g,
+                                       hipStream_t s);
+hipError_t launch_extend_step(double* step, const double* deltat, const double* C, const double* vx,
+                              const double* vy, const double* vz, const Dist6c& d, const Dist6c& cp,
+                              const Geom& g, hipStream_t s);
+hipError_t launch_extend_stage(double* out, const double* cs, const double* cn, const double* vx,
+                               const double* vy, const double* vz, const uint8_t* flags, const Dist6c& d,
+                               const Dist6c& cp, const Geom& g, int stage, hipStream_t s);
+hipError_t launch_advection_step(double* astep, const double* vx, const double* vy, const double* vz,
+                                 const double* lsf, const Geom& g, hipStream_t s);
+hipError_t launch_advect_stage(double* out, const double* fs, const double* fn, const double* vx,
+                               const double* vy, const double* vz, double dt, const Geom& g, int stage,
+                               hipStream_t s);
+hipError_t launch_weno_derivative(const Dist6& out, const double* lsf, const Geom& g, hipStream_t s);
+hipError_t launch_set_calculus_toolbox(const Fields18& o, const double* lsf, const double* hp, const Geom& g,
+                                       double ds, hipStream_t s);
+// reductions: `partial` holds at least reduce_blocks()*4 doubles, `result` 2
+int reduce_blocks();
+// result[0..1] = (area, volume) sums times `scale`
+hipError_t launch_area_volume_cosine(double* result, double* partial, const double* F, const Geom& g, double eps,
+                                     double scale, hipStream_t s);
+hipError_t launch_area_volume_towers(double* result, double* partial, const double* F, const Geom& g, double ds,
+                                     double scale, hipStream_t s);
+hipError_t launch_normal_velocity(double* out, const double* F, const double* vx, const double* vy, const double* vz,
+                                  const Geom& g, hipStream_t s);
+hipError_t launch_max_abs3(double* result, double* partial, const double* a, const double* b, const double* c,
+                           const Geom& g, hipStream_t s);
+hipError_t launch_copy(double* dst, const double* src, int64_t n, hipStream_t s);
+
+}  // namespace vls

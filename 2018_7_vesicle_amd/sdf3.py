@@ -1,0 +1,301 @@
+"""Host-side mirror of the reference's level-set objects over the C ABI.
+
+`GD3` and `SDF3` keep the names, argument meaning and results of the
+reference's MATLAB classes (src/7_0_paper/2_0_Vesicle/+SD/@GD3/GD3.m,
++SD/@SDF3/*.m) for the hot-path methods.  Fields are torch float64 tensors on
+the GPU, contiguous, with shape (pges, cols, rows) = (Nz, Nx, Ny): in memory
+that is the reference's column-major [rows, cols, pges] array
+(ind = p*rows*cols + c*rows + r).  torch supplies device memory and the
+stream; all arithmetic runs in the HIP library.
+
+The module-level `feval_*` functions are the kernel-level drop-ins: the
+reference's feval(kernel, args...) with the same positional arguments, writing
+the non-const arguments in place.
+"""
+import ctypes
+
+import torch
+
+from ._lib import D, P, PD, VLSError, call
+
+__all__ = ["GD3", "SDF3", "VLSError"] + [
+    "feval_boundary_location", "feval_boundary_correction", "feval_re_step", "feval_eno2_re_step",
+    "feval_upwind_normal", "feval_boundary_interpolant", "feval_extend_step", "feval_advection_step",
+    "feval_weno_derivative", "feval_set_calculus_toolbox", "TOOLBOX_FIELDS",
+]
+
+TOOLBOX_FIELDS = ["Fx", "Fy", "Fz", "FGradMag", "Nx", "Ny", "Nz", "Fxx", "Fyy", "Fzz", "Fxy", "Fyz", "Fzx",
+                  "FLaplacian", "MeanCurvature", "GaussianCurvature", "Heaviside", "DiracDelta"]
+
+
+def _stream():
+    return P(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t, dtype=torch.float64):
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise VLSError("expected a CUDA tensor")
+    if t.dtype != dtype:
+        raise VLSError(f"expected dtype {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise VLSError("expected a contiguous tensor")
+    return P(t.data_ptr())
+
+
+def _dims(t):
+    if t.dim() != 3:
+        raise VLSError("fields are 3-D tensors of shape (pges, cols, rows)")
+    pges, cols, rows = t.shape
+    return rows, cols, pges
+
+
+class GD3:
+    """Uniform grid (+SD/@GD3/GD3.m:86-105).  Built from the three coordinate
+    vectors of MATLAB's meshgrid(xv, yv, zv)."""
+
+    def __init__(self, xv, yv, zv, device="cuda"):
+        self.xv = torch.as_tensor(xv, dtype=torch.float64)
+        self.yv = torch.as_tensor(yv, dtype=torch.float64)
+        self.zv = torch.as_tensor(zv, dtype=torch.float64)
+        self.device = torch.device(device)
+        self.Dx = float(self.xv[1] - self.xv[0])
+        self.Dy = float(self.yv[1] - self.yv[0])
+        self.Dz = float(self.zv[1] - self.zv[0])
+        self.Ds = (self.Dx + self.Dy + self.Dz) / 3
+        self.EltVol = self.Dx * self.Dy * self.Dz
+        self.mrows, self.ncols, self.lshts = len(self.yv), len(self.xv), len(self.zv)
+        self.NumElt = self.mrows * self.ncols * self.lshts
+        self.Size = (self.lshts, self.ncols, self.mrows)
+
+    def meshgrid(self):
+        """X, Y, Z of MATLAB meshgrid as (pges, cols, rows) tensors on the device."""
+        Z, X, Y = torch.meshgrid(self.zv, self.xv, self.yv, indexing="ij")
+        return (X.contiguous().to(self.device), Y.contiguous().to(self.device), Z.contiguous().to(self.device))
+
+    def zeros(self):
+        return torch.zeros(self.Size, dtype=torch.float64, device=self.device)
+
+
+class SDF3:
+    """Signed-distance level set (+SD/@SDF3/SDF3.m) with its hot-path methods.
+
+    Owns the device state of the MATLAB object (xpr..zpd, min_dist and the
+    work fields of the iteration loops) through one C-ABI handle."""
+
+    def __init__(self, grid, F):
+        self.GD3 = grid
+        self._h = ctypes.c_void_p()
+        call("vls_sdf3_create", ctypes.byref(self._h), grid.mrows, grid.ncols, grid.lshts, grid.Dx, grid.Dy,
+             grid.Dz)
+        self.F = F
+        for name in TOOLBOX_FIELDS:
+            setattr(self, name, None)
+        self.HPrimal = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                torch.cuda.synchronize()
+                call("vls_sdf3_destroy", h)
+            except Exception:
+                pass
+            self._h = None
+
+    def _check(self, t):
+        if tuple(t.shape) != self.GD3.Size:
+            raise VLSError(f"field shape {tuple(t.shape)} != grid {self.GD3.Size}")
+        return _ptr(t)
+
+    # -- setDistance (+SD/@SDF3/setDistance.m:3-21) --------------------------
+    def setDistance(self):
+        call("vls_sdf3_set_distance", self._h, self._check(self.F), _stream())
+
+    def distances(self):
+        """copies of (xpr, xpl, ypf, ypb, zpu, zpd, min_dist) of the last setDistance"""
+        ptrs = (ctypes.c_void_p * 7)()
+        call("vls_sdf3_distances", self._h, ptrs)
+        out = []
+        n = self.GD3.NumElt
+        for k in range(7):
+            t = self.GD3.zeros()
+            _hip_copy_d2d(t.data_ptr(), ptrs[k], n * 8)
+            out.append(t)
+        return out
+
+    # -- reinitialization ------------------------------------------------------
+    def WENO5RK3Reinitialization(self, F, iteration):
+        """+SD/@SDF3/WENO5RK3Reinitialization.m:3-43: mask = F<0, distances of the last setDistance."""
+        NewF = F.clone()
+        call("vls_sdf3_weno5_rk3_reinit", self._h, self._check(NewF), int(iteration), _stream())
+        return NewF
+
+    def ENORK2Reinitialization(self, F, iteration, rk2=True, cfl=0.3):
+        """+SD/@SDF3/ENORK2Reinitialization.m:3-45 (rk2=True, cfl=0.3 is the reference;
+        rk2=False is forward Euler)."""
+        NewF = F.clone()
+        call("vls_sdf3_eno2_reinit", self._h, self._check(NewF), int(iteration), 1 if rk2 else 0, float(cfl),
+             _stream())
+        return NewF
+
+    # -- extension and surface redistance -------------------------------------
+    def WENO5RK3Extend(self, C, iteration):
+        """+SD/@SDF3/WENO5RK3Extend.m:9-76 (self.F and its last setDistance define the interface)."""
+        NewC = C.clone()
+        call("vls_sdf3_weno5_rk3_extend", self._h, self._check(self.F), self._check(NewC), int(iteration),
+             _stream())
+        return NewC
+
+    def WENO5RK3ClosetPointSurfaceRedistance(self, A, iter1, iter2):
+        """+SD/@SDF3/WENO5RK3ClosetPointSurfaceRedistance.m:4-157"""
+        NewA = A.clone()
+        call("vls_sdf3_surface_redistance", self._h, self._check(self.F), self._check(NewA), int(iter1),
+             int(iter2), _stream())
+        return NewA
+
+    # -- advection -------------------------------------------------------------
+    def WENO5RK3Advection(self, F, vx, vy, vz, dt, steps=1):
+        """TVD-RK3 (Shu-Osher form of +SD/@SDF3/WENORK3SurfaceConservationLaw.m:10-29) of
+        F_t + v.grad F = 0 with the WENO5 operator of advection_step.cu:30-56."""
+        NewF = F.clone()
+        call("vls_sdf3_advect_rk3", self._h, self._check(NewF), self._check(vx), self._check(vy),
+             self._check(vz), float(dt), int(steps), _stream())
+        return NewF
+
+    def maxSpeed(self, vx, vy, vz):
+        out = ctypes.c_double()
+        call("vls_sdf3_max_speed", self._h, self._check(vx), self._check(vy), self._check(vz),
+             ctypes.byref(out), _stream())
+        return out.value
+
+    # -- calculus toolbox, area and volume --------------------------------------
+    def GPUsetCalculusToolBox(self):
+        """+SD/@SDF3/GPUsetCalculusToolBox.m:2-40 (HPrimal = max(F,0))."""
+        g = self.GD3
+        self.HPrimal = torch.clamp(self.F, min=0.0).contiguous()
+        outs = [g.zeros() for _ in TOOLBOX_FIELDS]
+        feval_set_calculus_toolbox(outs, self.F, self.HPrimal, g.mrows, g.ncols, g.lshts, g.Dx, g.Dy, g.Dz, g.Ds)
+        for name, t in zip(TOOLBOX_FIELDS, outs):
+            setattr(self, name, t)
+
+    def calAreaVolume(self):
+        """(calArea, calVolume) of +SD/@SDF3/SDF3.m:73-80 with the Towers delta/H,
+        fused with the toolbox stencil (no toolbox fields are written)."""
+        a, v = ctypes.c_double(), ctypes.c_double()
+        call("vls_sdf3_area_volume_towers", self._h, self._check(self.F), ctypes.byref(a), ctypes.byref(v),
+             _stream())
+        return a.value, v.value
+
+    def calArea(self):
+        return self.calAreaVolume()[0]
+
+    def calVolume(self):
+        return self.calAreaVolume()[1]
+
+    def surfaceIntegral(self, field):
+        """+SD/@SDF3/SDF3.m:82-85 (needs GPUsetCalculusToolBox first)."""
+        mask = self.F.abs() < 2 * self.GD3.Ds
+        return float((field[mask] * self.DiracDelta[mask] * self.FGradMag[mask]).sum()) * self.GD3.Ds ** 3
+
+    def cosineAreaVolume(self, eps=None):
+        """SurfaceIntegral(1), VolumeIntegral(1) with the cosine-smoothed delta/H of
+        src/0_0_setup/+SD/@SDF3/SDF3.m:126-136,178-187; eps defaults to 1.5*Dx (:59)."""
+        if eps is None:
+            eps = 1.5 * self.GD3.Dx
+        a, v = ctypes.c_double(), ctypes.c_double()
+        call("vls_sdf3_area_volume_cosine", self._h, self._check(self.F), float(eps), ctypes.byref(a),
+             ctypes.byref(v), _stream())
+        return a.value, v.value
+
+
+_hip = None
+
+
+def _hip_copy_d2d(dst, src, nbytes):
+    """device-to-device copy on torch's current stream through the process's HIP runtime"""
+    global _hip
+    if _hip is None:
+        import os
+
+        for line in open("/proc/self/maps"):
+            path = line.split()[-1]
+            if "libamdhip64" in os.path.basename(path):
+                _hip = ctypes.CDLL(path)
+                break
+        else:
+            _hip = ctypes.CDLL("libamdhip64.so")
+        _hip.hipMemcpyAsync.argtypes = [P, P, ctypes.c_size_t, ctypes.c_int, P]
+    err = _hip.hipMemcpyAsync(P(dst), P(src), nbytes, 3, _stream())  # 3 = hipMemcpyDeviceToDevice
+    if err != 0:
+        raise VLSError(f"hipMemcpyAsync failed with {err}")
+
+
+# ---------------------------------------------------------------------------
+# kernel-level drop-ins (feval mirrors)
+# ---------------------------------------------------------------------------
+def feval_boundary_location(xpr, xpl, ypf, ypb, zpu, zpd, lsf, num_ele, rows, cols, pges, dx, dy, dz):
+    call("vls_boundary_location", *[_ptr(t) for t in (xpr, xpl, ypf, ypb, zpu, zpd, lsf)], num_ele, rows, cols,
+         pges, dx, dy, dz, _stream())
+    return xpr, xpl, ypf, ypb, zpu, zpd
+
+
+def feval_boundary_correction(xpr, xpl, ypf, ypb, zpu, zpd, lsf, num_ele, rows, cols, pges, dx, dy, dz):
+    call("vls_boundary_correction", *[_ptr(t) for t in (xpr, xpl, ypf, ypb, zpu, zpd, lsf)], num_ele, rows, cols,
+         pges, dx, dy, dz, _stream())
+    return xpr, xpl, ypf, ypb, zpu, zpd
+
+
+def feval_re_step(step, lsf, mask, deltat, xpr, xpl, ypf, ypb, zpu, zpd, rows, cols, pges, dx, dy, dz, num_ele):
+    call("vls_re_step", _ptr(step), _ptr(lsf), _ptr(mask, torch.uint8), _ptr(deltat),
+         *[_ptr(t) for t in (xpr, xpl, ypf, ypb, zpu, zpd)], rows, cols, pges, dx, dy, dz, num_ele, _stream())
+    return step
+
+
+def feval_eno2_re_step(step, lsf, mask, deltat, xpr, xpl, ypf, ypb, zpu, zpd, rows, cols, pges, dx, dy, dz,
+                       num_ele):
+    call("vls_eno2_re_step", _ptr(step), _ptr(lsf), _ptr(mask, torch.uint8), _ptr(deltat),
+         *[_ptr(t) for t in (xpr, xpl, ypf, ypb, zpu, zpd)], rows, cols, pges, dx, dy, dz, num_ele, _stream())
+    return step
+
+
+def feval_upwind_normal(nx, ny, nz, lsf, xpr, xpl, ypf, ypb, zpu, zpd, rows, cols, pges, dx, dy, dz, num_ele):
+    call("vls_upwind_normal", *[_ptr(t) for t in (nx, ny, nz, lsf, xpr, xpl, ypf, ypb, zpu, zpd)], rows, cols,
+         pges, dx, dy, dz, num_ele, _stream())
+    return nx, ny, nz
+
+
+def feval_boundary_interpolant(cpr, cpl, cpf, cpb, cpu, cpd, xpr, xpl, ypf, ypb, zpu, zpd, lsf, rows, cols, pges,
+                               dx, dy, dz, num_ele):
+    call("vls_boundary_interpolant",
+         *[_ptr(t) for t in (cpr, cpl, cpf, cpb, cpu, cpd, xpr, xpl, ypf, ypb, zpu, zpd, lsf)], rows, cols, pges,
+         dx, dy, dz, num_ele, _stream())
+    return cpr, cpl, cpf, cpb, cpu, cpd
+
+
+def feval_extend_step(step, deltat, lsf, vx, vy, vz, xpr, xpl, ypf, ypb, zpu, zpd, cpr, cpl, cpf, cpb, cpu, cpd,
+                      rows, cols, pges, dx, dy, dz, num_ele):
+    call("vls_extend_step", *[_ptr(t) for t in (step, deltat, lsf, vx, vy, vz, xpr, xpl, ypf, ypb, zpu, zpd, cpr,
+                                                 cpl, cpf, cpb, cpu, cpd)], rows, cols, pges, dx, dy, dz, num_ele,
+         _stream())
+    return step
+
+
+def feval_advection_step(astep, vx, vy, vz, lsf, rows, cols, pges, dx, dy, dz):
+    call("vls_advection_step", *[_ptr(t) for t in (astep, vx, vy, vz, lsf)], rows, cols, pges, dx, dy, dz,
+         _stream())
+    return astep
+
+
+def feval_weno_derivative(back_x, fore_x, back_y, fore_y, back_z, fore_z, lsf, rows, cols, pges, dx, dy, dz):
+    call("vls_weno_derivative", *[_ptr(t) for t in (back_x, fore_x, back_y, fore_y, back_z, fore_z, lsf)], rows,
+         cols, pges, dx, dy, dz, _stream())
+    return back_x, fore_x, back_y, fore_y, back_z, fore_z
+
+
+def feval_set_calculus_toolbox(out18, lsf, hprimal, rows, cols, pges, dx, dy, dz, ds):
+    if len(out18) != 18:
+        raise VLSError("set_calculus_toolbox writes 18 fields")
+    arr = (ctypes.c_void_p * 18)(*[_ptr(t).value for t in out18])
+    call("vls_set_calculus_toolbox", arr, _ptr(lsf), _ptr(hprimal), rows, cols, pges, dx, dy, dz, ds,
+         rows * cols * pges, _stream())
+    return out18

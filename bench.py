@@ -1,0 +1,184 @@
+"""Benchmark: the full FP64 vesicle-step pipeline at 128^3 on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Workload (one "step"): WENO5 + TVD-RK3 advection of F and psi, setDistance,
+normal velocity and its 40-iteration WENO5-RK3 extension, every 5th step a
+100-iteration WENO5-RK3 reinitialization plus a (10,5) closest-point surface
+redistance, and the cosine delta/H area and volume; 128^3 on [-1.2,1.2]^3,
+ellipsoid (0.8,0.5,0.5), seeded 16-mode curl velocity, CFL 0.4 (see
+2018_7_vesicle_amd/pipeline.py).  Inputs are resident in HBM before timing.
+
+value = WENO grid-point updates per second: (operator-stage passes counted by
+pipeline.updates_per_step) x points / wall time of the K timed steps.
+
+roofline: the dominant kernel is the fused WENO5-RK3 extension stage
+(k_extend_stage).  Its average duration is measured with HIP events around the
+extension calls inside the timed steps, minus the extension's per-call set-up
+(upwind normal, velocity, interface values: timed separately with 0
+iterations), divided by the stage launches.  achieved = 37.3 B/point (the
+algorithmic bytes of one FP64 extension stage) x points / that duration.
+
+cpu_baseline: the CPU oracle (OpenMP, FP64) on a bounded sample of the same
+workload, timed on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+B_ALG_EXTEND_FP64 = 37.3  # bytes per point per FP64 WENO5 extension stage
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-ext-iters", type=int, default=10)
+    return ap.parse_args()
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch of the dominant kernel measured with rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), if recorded."""
+    path = os.path.join(ROOT, "profiles", "extend_stage_128_traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        return float(json.load(open(path))["bytes_per_launch"])
+    except Exception:
+        return None
+
+
+def cpu_baseline(cfg, ext_iters):
+    """one non-reinitializing pipeline step with `ext_iters` extension iterations,
+    run by the CPU oracle on all host threads"""
+    import numpy as np
+
+    from oracle import pyoracle as po
+    import importlib
+
+    pl = importlib.import_module("2018_7_vesicle_amd.pipeline")
+    xv, F0, psi0, u = pl.make_inputs(cfg)
+    dx = float(xv[1] - xv[0])
+    sp = (dx, dx, dx)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    po.set_num_threads(threads)
+    umax = max(float(np.abs(c).max()) for c in u)
+    dt = cfg.cfl * dx / umax
+    F = np.ascontiguousarray(F0)
+    t0 = time.perf_counter()
+    F = po.advect_rk3(F, 1, dt, u, *sp)
+    psi = po.advect_rk3(psi0, 1, dt, u, *sp)
+    d6, md = po.set_distance(F, *sp)
+    Vn = po.normal_velocity(F, u, *sp)
+    Vn = po.weno5_rk3_extend(Vn, ext_iters, F, d6, md, *sp)
+    t1 = time.perf_counter()
+    del psi, Vn
+    n = F.size
+    updates = (6 + 1 + 3 * ext_iters) * n
+    return {
+        "value": updates / (t1 - t0),
+        "unit": "grid-pt updates/s",
+        "cores": po.num_threads(),
+        "kind": "port",
+        "sample": f"one pipeline step without reinitialization at {cfg.n}^3 FP64 with {ext_iters} extension "
+                  f"iterations ({6 + 1 + 3 * ext_iters} WENO stage passes, {t1 - t0:.1f} s)",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported by this build "
+              "(the multi-GPU z-slab decomposition is a design note, not code)", file=sys.stderr)
+        return 2
+    import importlib
+
+    import torch
+
+    vls = importlib.import_module("2018_7_vesicle_amd")
+    pl = vls.pipeline
+    torch.cuda.set_device(0)
+    cfg = pl.PipelineConfig(n=args.n)
+    p = pl.Pipeline(cfg)
+    n = p.grid.NumElt
+    if args.warmup:
+        p.run(args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    updates = p.run(args.steps, record_extension=True)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    wall = t1 - t0
+
+    # dominant kernel: extension stage duration
+    ext_ms = [a.elapsed_time(b) for a, b in p.ext_events]
+    setup_ms = []
+    for _ in range(5):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        p.map.WENO5RK3Extend(p.Vn, 0)
+        e1.record()
+        torch.cuda.synchronize()
+        setup_ms.append(e0.elapsed_time(e1))
+    stage_s = (sum(ext_ms) / len(ext_ms) - sum(setup_ms) / len(setup_ms)) / (3 * cfg.ext_iters) * 1e-3
+    achieved = B_ALG_EXTEND_FP64 * n / stage_s / 1e9
+
+    av = p.area_volume.cpu().numpy()
+    out = {
+        "metric": "WENO grid-pt updates/s (FP64 vesicle-step pipeline)",
+        "value": updates / wall,
+        "unit": "grid-pt updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"config-2 vesicle step: {cfg.n}^3 on [{cfg.lo},{cfg.hi}]^3, ellipsoid {cfg.axes}, "
+                        f"{cfg.modes}-mode curl velocity (seed {cfg.seed}), WENO5-RK3 advection CFL {cfg.cfl}, "
+                        f"{cfg.ext_iters}-iteration extension, reinit {cfg.reinit_iters} iterations every "
+                        f"{cfg.reinit_every} steps, surface redistance {cfg.redist_iters}, cosine delta/H 1.5dx",
+            "grid": [cfg.n, cfg.n, cfg.n],
+            "points": n,
+            "updates_per_step_avg": updates / args.steps / n,
+        },
+        "roofline": {
+            "kernel": "k_extend_stage (fused WENO5-RK3 extension stage, FP64)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic_from_profiles(),
+            "avg_launch_us": stage_s * 1e6,
+            "bytes_per_point": B_ALG_EXTEND_FP64,
+        },
+        "diagnostics": {
+            "area_first_last": [float(av[0, 0]), float(av[-1, 0])],
+            "volume_first_last": [float(av[0, 1]), float(av[-1, 1])],
+            "dt": p.dt,
+        },
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_ext_iters)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,183 @@
+/*
+ * vesicle_ls.h -- C ABI of the MI355X level-set hot path.
+ *
+ * Drop-in replacement for the CUDA kernels and MATLAB wrappers of the
+ * level-set path of tiankuizhang/2018_7_Vesicle (paths below are relative to
+ * src/7_0_paper/2_0_Vesicle/).  The reference binds its kernels through
+ * parallel.gpu.CUDAKernel objects (+SD/@SDF3/GPUInitialize.m:16-180) and calls
+ * them as feval(kernel, args...); the functions in section 1 keep each
+ * kernel's positional arguments and meaning, so a MATLAB mex shim or any other
+ * host can swap feval for a call here.  Section 2 replaces the MATLAB wrapper
+ * methods of @SDF3 with single calls on a handle that owns the object's
+ * device state.
+ *
+ * Conventions
+ *   - fields are double[rows*cols*pges] in device memory, column-major
+ *     [rows, cols, pges] (rows = y, cols = x, pges = z):
+ *     ind = pge*rows*cols + col*rows + row; stencil neighbours are clamped;
+ *   - pointers are caller-owned device pointers; non-const pointers are
+ *     outputs written in place (feval's in/out semantics); `mask` is one byte
+ *     per point (MATLAB logical);
+ *   - `stream` is a hipStream_t, NULL = the default stream; every call is
+ *     asynchronous on it except where a host result is returned;
+ *   - return value: VLS_OK (0) or a negative VLS_ERR_* code; vls_last_error()
+ *     describes the last failure of the calling thread.  No C++ exception
+ *     crosses the ABI.
+ */
+#ifndef VESICLE_LS_H
+#define VESICLE_LS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VLS_OK 0
+#define VLS_ERR_INVALID (-1) /* bad size, null pointer or option */
+#define VLS_ERR_HIP (-2)     /* a HIP runtime call failed */
+#define VLS_ERR_ALLOC (-3)   /* device allocation failed */
+
+typedef void *vls_stream_t; /* hipStream_t */
+
+const char *vls_last_error(void);
+const char *vls_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* 1. kernel-level drop-ins (one per reference __global__ entry point)      */
+/* ------------------------------------------------------------------------ */
+
+/* boundary_location: CUDA_Code/1_3_WENO5RK3_Reinitialization/boundary_location.cu:84-85.
+ * Sixth-order subcell distances; the six arrays are in/out and must be
+ * pre-filled with dx,dx,dy,dy,dz,dz (+SD/@SDF3/setDistance.m:5-10). */
+int vls_boundary_location(double *xpr, double *xpl, double *ypf, double *ypb, double *zpu, double *zpd,
+                          const double *lsf, int num_ele, int rows, int cols, int pges, double dx, double dy,
+                          double dz, vls_stream_t stream);
+
+/* boundary_correction: CUDA_Code/1_0_ENORK2_Reinitialization/boundary_correction.cu:103-104.
+ * Quadratic-ENO subcell distances, same in/out convention. */
+int vls_boundary_correction(double *xpr, double *xpl, double *ypf, double *ypb, double *zpu, double *zpd,
+                            const double *lsf, int num_ele, int rows, int cols, int pges, double dx, double dy,
+                            double dz, vls_stream_t stream);
+
+/* re_step (WENO5): CUDA_Code/1_3_WENO5RK3_Reinitialization/re_step.cu:283-284. */
+int vls_re_step(double *step, const double *lsf, const uint8_t *mask, const double *deltat, const double *xpr,
+                const double *xpl, const double *ypf, const double *ypb, const double *zpu, const double *zpd,
+                int rows, int cols, int pges, double dx, double dy, double dz, int num_ele, vls_stream_t stream);
+
+/* re_step (ENO2): CUDA_Code/1_0_ENORK2_Reinitialization/boundary_correction.cu:147-148. */
+int vls_eno2_re_step(double *step, const double *lsf, const uint8_t *mask, const double *deltat,
+                     const double *xpr, const double *xpl, const double *ypf, const double *ypb,
+                     const double *zpu, const double *zpd, int rows, int cols, int pges, double dx, double dy,
+                     double dz, int num_ele, vls_stream_t stream);
+
+/* upwind_normal: CUDA_Code/2_2_WENO5RK3_Extend/upwind_normal.cu:14-15. */
+int vls_upwind_normal(double *nx, double *ny, double *nz, const double *lsf, const double *xpr, const double *xpl,
+                      const double *ypf, const double *ypb, const double *zpu, const double *zpd, int rows,
+                      int cols, int pges, double dx, double dy, double dz, int num_ele, vls_stream_t stream);
+
+/* boundary_interpolant: CUDA_Code/2_2_WENO5RK3_Extend/boundary_interpolant.cu:206-207.
+ * `lsf` is the field being extended (the reference passes C there). */
+int vls_boundary_interpolant(double *cpr, double *cpl, double *cpf, double *cpb, double *cpu, double *cpd,
+                             const double *xpr, const double *xpl, const double *ypf, const double *ypb,
+                             const double *zpu, const double *zpd, const double *lsf, int rows, int cols,
+                             int pges, double dx, double dy, double dz, int num_ele, vls_stream_t stream);
+
+/* extend_step: CUDA_Code/2_2_WENO5RK3_Extend/extend_step.cu:262-263. */
+int vls_extend_step(double *step, const double *deltat, const double *lsf, const double *vx, const double *vy,
+                    const double *vz, const double *xpr, const double *xpl, const double *ypf, const double *ypb,
+                    const double *zpu, const double *zpd, const double *cpr, const double *cpl, const double *cpf,
+                    const double *cpb, const double *cpu, const double *cpd, int rows, int cols, int pges,
+                    double dx, double dy, double dz, int num_ele, vls_stream_t stream);
+
+/* advection_step: CUDA_Code/4_0_Calculus_ToolBox/advection_step.cu:30-31 (v.grad(lsf), no dt). */
+int vls_advection_step(double *astep, const double *vx, const double *vy, const double *vz, const double *lsf,
+                       int rows, int cols, int pges, double dx, double dy, double dz, vls_stream_t stream);
+
+/* weno_derivative: CUDA_Code/5_0_Nonoscillating_Reconstruction/nonoscillating_interpolant.cu:59-60. */
+int vls_weno_derivative(double *back_x, double *fore_x, double *back_y, double *fore_y, double *back_z,
+                        double *fore_z, const double *lsf, int rows, int cols, int pges, double dx, double dy,
+                        double dz, vls_stream_t stream);
+
+/* set_calculus_toolbox: CUDA_Code/4_0_Calculus_ToolBox/set_calculus_toolbox.cu:101-102.
+ * out18 = {Fx,Fy,Fz,FGradMag,Nx,Ny,Nz,Fxx,Fyy,Fzz,Fxy,Fyz,Fzx,FLaplacian,
+ *          MeanCurvature,GaussianCurvature,Heaviside,DiracDelta} (18 device pointers). */
+int vls_set_calculus_toolbox(double *const *out18, const double *lsf, const double *hprimal, int rows, int cols,
+                             int pges, double dx, double dy, double dz, double ds, int num_ele,
+                             vls_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* 2. the @SDF3 object: one handle, one call per MATLAB method              */
+/* ------------------------------------------------------------------------ */
+typedef struct vls_sdf3 *vls_sdf3_t;
+
+/* SD.SDF3(grid, x, y, z, F) with grid = SD.GD3 (+SD/@GD3/GD3.m:86-105,
+ * +SD/@SDF3/SDF3.m:11-15): allocates the object's device state for a
+ * rows x cols x pges box with spacings dx, dy, dz.  No limit on rows (the
+ * reference's launch shape needs rows <= 256, GPUInitialize.m:7). */
+int vls_sdf3_create(vls_sdf3_t *out, int rows, int cols, int pges, double dx, double dy, double dz);
+int vls_sdf3_destroy(vls_sdf3_t h);
+
+/* setDistance (+SD/@SDF3/setDistance.m:3-21): subcell distances of F with the
+ * sixth-order locator, and min_dist.  Stored in the handle. */
+int vls_sdf3_set_distance(vls_sdf3_t h, const double *F, vls_stream_t stream);
+
+/* device pointers of the handle's xpr,xpl,ypf,ypb,zpu,zpd,min_dist (7 entries, read-only) */
+int vls_sdf3_distances(vls_sdf3_t h, const double **out7);
+
+/* WENO5RK3Reinitialization (+SD/@SDF3/WENO5RK3Reinitialization.m:3-43): F in
+ * place; mask = F<0 at entry, distances from the last set_distance. */
+int vls_sdf3_weno5_rk3_reinit(vls_sdf3_t h, double *F, int iterations, vls_stream_t stream);
+
+/* ENORK2Reinitialization (+SD/@SDF3/ENORK2Reinitialization.m:3-45): ENO2
+ * distances of the argument F, dt = cfl*min_dist.  rk2 = 1 is the reference's
+ * Heun RK2 with cfl 0.3; rk2 = 0 is forward Euler. */
+int vls_sdf3_eno2_reinit(vls_sdf3_t h, double *F, int iterations, int rk2, double cfl, vls_stream_t stream);
+
+/* WENO5RK3Extend (+SD/@SDF3/WENO5RK3Extend.m:9-76): extends C (in place) off
+ * the zero set of F, which must be the field of the last set_distance. */
+int vls_sdf3_weno5_rk3_extend(vls_sdf3_t h, const double *F, double *C, int iterations, vls_stream_t stream);
+
+/* WENO5RK3ClosetPointSurfaceRedistance
+ * (+SD/@SDF3/WENO5RK3ClosetPointSurfaceRedistance.m:4-157): A in place. */
+int vls_sdf3_surface_redistance(vls_sdf3_t h, const double *F, double *A, int iter1, int iter2,
+                                vls_stream_t stream);
+
+/* TVD-RK3 WENO5 advection of F by (vx,vy,vz) for `steps` steps of dt: the
+ * Shu-Osher form of +SD/@SDF3/WENORK3SurfaceConservationLaw.m:10-29 with the
+ * operator of advection_step.cu:30-56.  No reference entry point composes the
+ * two; this is that composition. */
+int vls_sdf3_advect_rk3(vls_sdf3_t h, double *F, const double *vx, const double *vy, const double *vz, double dt,
+                        int steps, vls_stream_t stream);
+
+/* max over the box of max(|vx|,|vy|,|vz|), returned to the host (synchronizes) */
+int vls_sdf3_max_speed(vls_sdf3_t h, const double *vx, const double *vy, const double *vz, double *out,
+                       vls_stream_t stream);
+
+/* calArea/calVolume (+SD/@SDF3/SDF3.m:73-80) with the Towers delta/H of
+ * GPUsetCalculusToolBox (+SD/@SDF3/GPUsetCalculusToolBox.m:4,30-40), fused.
+ * Compensated FP64 sums; returns to the host (synchronizes). */
+int vls_sdf3_area_volume_towers(vls_sdf3_t h, const double *F, double *area, double *volume,
+                                vls_stream_t stream);
+
+/* Cosine-smoothed delta/H of half-width eps (src/0_0_setup/+SD/@SDF3/SDF3.m:59,126-136,
+ * 178-187; the reference uses eps = 1.5*dx).  Returns to the host (synchronizes). */
+int vls_sdf3_area_volume_cosine(vls_sdf3_t h, const double *F, double eps, double *area, double *volume,
+                                vls_stream_t stream);
+
+/* Either area/volume pair written to device memory out2[0..1] without
+ * synchronizing: kind 0 = Towers (eps unused), kind 1 = cosine of half-width eps. */
+int vls_sdf3_area_volume_device(vls_sdf3_t h, const double *F, int kind, double eps, double *out2,
+                                vls_stream_t stream);
+
+/* Normal component u.N of a velocity with N = grad F / max(|grad F|, 1e-14)
+ * from central differences (set_calculus_toolbox.cu:123-136): the field the
+ * vesicle drivers extend off the interface (V/p00_test_singlephase_relax.m:76-84). */
+int vls_sdf3_normal_velocity(vls_sdf3_t h, const double *F, const double *vx, const double *vy, const double *vz,
+                             double *out, vls_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VESICLE_LS_H */

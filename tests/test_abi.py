@@ -1,0 +1,66 @@
+"""C-ABI checks that need no GPU: the library loads, exports every function
+include/vesicle_ls.h declares (and the binding declares the same set), and
+argument errors come back as VLS_ERR_INVALID before any device work."""
+import ctypes
+import importlib
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "vesicle_ls.h")
+
+vls = importlib.import_module("2018_7_vesicle_amd")
+
+
+def declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(vls_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_api():
+    names = declared()
+    assert "vls_re_step" in names and "vls_sdf3_create" in names
+    assert len(names) >= 25
+
+
+def test_library_exports_every_declared_symbol():
+    lib = vls.lib()
+    for name in declared():
+        assert hasattr(lib, name), f"{name} not exported"
+
+
+def test_binding_matches_header():
+    assert sorted(vls.exported_names()) == declared()
+
+
+def test_version_string():
+    assert b"gfx950" in vls.lib().vls_version()
+
+
+def test_invalid_geometry_rejected_without_device_work():
+    lib = vls.lib()
+    h = ctypes.c_void_p()
+    rc = lib.vls_sdf3_create(ctypes.byref(h), 0, 8, 8, 0.1, 0.1, 0.1)
+    assert rc == -1 and h.value is None
+    assert b"empty" in lib.vls_last_error()
+    rc = lib.vls_sdf3_create(ctypes.byref(h), 8, 8, 8, 0.1, -0.1, 0.1)
+    assert rc == -1
+    # num_ele mismatch on a drop-in entry point
+    p = ctypes.c_void_p(16)
+    rc = lib.vls_boundary_location(p, p, p, p, p, p, p, 10, 4, 4, 4, 0.1, 0.1, 0.1, None)
+    assert rc == -1 and b"num_ele" in lib.vls_last_error()
+    # null pointers
+    z = ctypes.c_void_p()
+    rc = lib.vls_advection_step(z, p, p, p, p, 4, 4, 4, 0.1, 0.1, 0.1, None)
+    assert rc == -1 and b"null" in lib.vls_last_error()
+
+
+def test_missing_library_fails_loudly(monkeypatch):
+    lib_mod = importlib.import_module("2018_7_vesicle_amd._lib")
+    monkeypatch.setattr(lib_mod, "_lib", None)
+    monkeypatch.setattr(lib_mod, "LIB_PATH", "/nonexistent/libvesicle_ls.so")
+    with pytest.raises(lib_mod.VLSError):
+        lib_mod.lib()

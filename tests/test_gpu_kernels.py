@@ -1,0 +1,189 @@
+"""Parity of the HIP kernel-level drop-ins with the CPU oracle, through the C ABI.
+
+Inputs: the golden-fixture inputs (written by the reference's own kernels)
+and larger seeded cases with distinct spacings per axis.  Bar: bit-for-bit
+equality.  The kernels are compiled without contraction and the oracle
+evaluates the same IEEE-754 sequence of +, -, *, /, sqrt and explicit fma, so
+any difference at all is a bug (1e-12 relative, is
+what the oracle itself meets against the reference's own kernels).
+"""
+import glob
+import importlib
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import cases
+from oracle import pyoracle as po
+
+vls = importlib.import_module("2018_7_vesicle_amd")
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def T(a, dtype=torch.float64):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda", dtype)
+
+
+def N(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def close(a, b, rtol=1e-12, what=""):
+    scale = max(np.abs(b).max(), 1e-300)
+    err = np.abs(a - b).max()
+    assert err <= rtol * scale, f"{what}: max|diff| {err:.3e} > {rtol:.0e} * {scale:.3e}"
+
+
+def exact(a, b, what=""):
+    """bit-for-bit equality (the kernels and the oracle evaluate the same IEEE sequence)"""
+    if not np.array_equal(a, b):
+        d = np.abs(a - b)
+        n = int((a != b).sum())
+        scale = max(np.abs(b).max(), 1e-300)
+        raise AssertionError(f"{what}: {n} of {a.size} values differ, max|diff| {d.max():.3e} "
+                             f"({d.max() / scale:.1e} relative)")
+
+
+def _golden_inputs():
+    out = []
+    for path in sorted(glob.glob(os.path.join(HERE, "golden", "golden_*.npz"))):
+        z = np.load(path)
+        g = cases.Grid(z["xv"], z["yv"], z["zv"])
+        out.append((os.path.basename(path)[7:-4], g, z["F"]))
+    return out
+
+
+def _large_inputs():
+    g1 = cases.Grid(np.linspace(-1.0, 1.0, 44), np.linspace(-0.95, 0.95, 40), np.linspace(-1.05, 1.05, 36))
+    g2 = cases.Grid(np.linspace(-1.0, 1.0, 40), np.linspace(-1.0, 1.0, 48), np.linspace(-1.25, 1.25, 52))
+    return [("star44", g1, cases.star_phi0(g1)), ("kink48", g2, cases.merging_spheres(g2))]
+
+
+CASES = _golden_inputs() + _large_inputs()
+
+
+@pytest.fixture(scope="module", params=CASES, ids=[c[0] for c in CASES])
+def case(request):
+    name, g, F = request.param
+    dx, dy, dz = g.spacing
+    d6 = po.prefilled_distances(F.shape, dx, dy, dz)
+    po.boundary_location(d6, F, dx, dy, dz)
+    md = np.min(np.stack(d6), axis=0)
+    return dict(name=name, g=g, F=np.ascontiguousarray(F), d6=d6, md=md, mask=(F < 0).astype(np.uint8),
+                deltat=0.3 * md)
+
+
+def _dims(g):
+    return g.rows, g.cols, g.pges
+
+
+def test_boundary_location(case):
+    g, F = case["g"], case["F"]
+    dx, dy, dz = g.spacing
+    rows, cols, pges = _dims(g)
+    for order, fn in ((6, vls.feval_boundary_location), (2, vls.feval_boundary_correction)):
+        ref = po.prefilled_distances(F.shape, dx, dy, dz)
+        po.boundary_location(ref, F, dx, dy, dz, order=order)
+        d6 = [T(a) for a in po.prefilled_distances(F.shape, dx, dy, dz)]
+        fn(*d6, T(F), F.size, rows, cols, pges, dx, dy, dz)
+        got = [N(t) for t in d6]
+        for k, (a, b) in enumerate(zip(got, ref)):
+            exact(a, b, f"order {order} distance array {k}")
+
+
+def test_re_step_weno5(case):
+    g, F = case["g"], case["F"]
+    dx, dy, dz = g.spacing
+    rows, cols, pges = _dims(g)
+    Fs = F + 0.3 * g.dx * np.sin(3.0 * g.X + 2.0 * g.Y - g.Z)
+    for lsf in (F, Fs):
+        ref = po.re_step(lsf, case["mask"], case["deltat"], case["d6"], dx, dy, dz)
+        step = torch.zeros_like(T(lsf))
+        vls.feval_re_step(step, T(lsf), T(case["mask"], torch.uint8), T(case["deltat"]), *[T(a) for a in case["d6"]],
+                          rows, cols, pges, dx, dy, dz, F.size)
+        exact(N(step), ref, "re_step")
+
+
+def test_re_step_eno2(case):
+    g, F = case["g"], case["F"]
+    dx, dy, dz = g.spacing
+    rows, cols, pges = _dims(g)
+    d6 = po.prefilled_distances(F.shape, dx, dy, dz)
+    po.boundary_location(d6, F, dx, dy, dz, order=2)
+    dt = 0.3 * np.min(np.stack(d6), axis=0)
+    ref = po.re_step(F, case["mask"], dt, d6, dx, dy, dz, scheme="eno2")
+    step = torch.zeros_like(T(F))
+    vls.feval_eno2_re_step(step, T(F), T(case["mask"], torch.uint8), T(dt), *[T(a) for a in d6], rows, cols, pges,
+                           dx, dy, dz, F.size)
+    exact(N(step), ref, "eno2 re_step")
+
+
+def test_upwind_normal(case):
+    g, F = case["g"], case["F"]
+    dx, dy, dz = g.spacing
+    rows, cols, pges = _dims(g)
+    ref = po.upwind_normal(F, case["d6"], dx, dy, dz)
+    out = [torch.zeros_like(T(F)) for _ in range(3)]
+    vls.feval_upwind_normal(*out, T(F), *[T(a) for a in case["d6"]], rows, cols, pges, dx, dy, dz, F.size)
+    for a, b in zip(out, ref):
+        exact(N(a), b, "upwind_normal")
+
+
+def test_boundary_interpolant_and_extend_step(case):
+    g, F = case["g"], case["F"]
+    dx, dy, dz = g.spacing
+    rows, cols, pges = _dims(g)
+    C = cases.smooth_scalar(g)
+    cp_ref = po.boundary_interpolant(case["d6"], C, dx, dy, dz)
+    cp = [torch.zeros_like(T(F)) for _ in range(6)]
+    vls.feval_boundary_interpolant(*cp, *[T(a) for a in case["d6"]], T(C), rows, cols, pges, dx, dy, dz, F.size)
+    for a, b in zip(cp, cp_ref):
+        exact(N(a), b, "boundary_interpolant")
+    n3 = po.upwind_normal(F, case["d6"], dx, dy, dz)
+    v3 = cases.extension_velocity_from_normal(F, n3, 1e-6)
+    ref = po.extend_step(case["deltat"], C, v3, case["d6"], cp_ref, dx, dy, dz)
+    step = torch.zeros_like(T(F))
+    vls.feval_extend_step(step, T(case["deltat"]), T(C), *[T(a) for a in v3], *[T(a) for a in case["d6"]],
+                          *[T(a) for a in cp_ref], rows, cols, pges, dx, dy, dz, F.size)
+    exact(N(step), ref, "extend_step")
+
+
+def test_advection_and_weno_derivative(case):
+    g, F = case["g"], case["F"]
+    dx, dy, dz = g.spacing
+    rows, cols, pges = _dims(g)
+    u3 = cases.smooth_velocity(g)
+    ref = po.advection_step(u3, F, dx, dy, dz)
+    out = torch.zeros_like(T(F))
+    vls.feval_advection_step(out, *[T(a) for a in u3], T(F), rows, cols, pges, dx, dy, dz)
+    exact(N(out), ref, "advection_step")
+    ref6 = po.weno_derivative(F, dx, dy, dz)
+    out6 = [torch.zeros_like(T(F)) for _ in range(6)]
+    vls.feval_weno_derivative(*out6, T(F), rows, cols, pges, dx, dy, dz)
+    for a, b in zip(out6, ref6):
+        exact(N(a), b, "weno_derivative")
+
+
+def test_set_calculus_toolbox(case):
+    g, F = case["g"], case["F"]
+    dx, dy, dz = g.spacing
+    rows, cols, pges = _dims(g)
+    hp = np.maximum(F, 0)
+    ref = po.set_calculus_toolbox(F, hp, dx, dy, dz, g.ds)
+    out = [torch.zeros_like(T(F)) for _ in range(18)]
+    vls.feval_set_calculus_toolbox(out, T(F), T(hp), rows, cols, pges, dx, dy, dz, g.ds)
+    for name, t in zip(po.TOOLBOX_FIELDS, out):
+        exact(N(t), ref[name], name)
+
+
+def test_invalid_arguments_raise():
+    t = torch.zeros((4, 5, 6), dtype=torch.float64, device="cuda")
+    with pytest.raises(vls.VLSError):
+        # num_ele does not match rows*cols*pges
+        vls.feval_boundary_location(t, t, t, t, t, t, t, 7, 6, 5, 4, 0.1, 0.1, 0.1)
+    with pytest.raises(vls.VLSError):
+        vls.feval_advection_step(t, t, t, t, t, 6, 5, 4, 0.1, -0.1, 0.1)

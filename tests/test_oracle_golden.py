@@ -1,0 +1,123 @@
+"""The CPU oracle against the golden vectors written by the reference's own
+kernels (tests/golden/make_golden.py ran them, compiled unmodified from the
+reference's CUDA sources, on an MI355X).
+
+Tolerances: the reference evaluates the same formulas in a different order
+(FMA contraction, pow()), so agreement is to rounding.  Distances are pinned to
+1e-12*ds absolute (the monomial-form quintic root moves by ~1e-13*ds under
+rounding); every other field to 1e-12 of the field's largest magnitude.  Masks
+(which distances are below ds, the frozen set) must agree exactly.
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FILES = sorted(glob.glob(os.path.join(HERE, "golden", "golden_*.npz")))
+
+
+def _load(path):
+    z = np.load(path)  # allow_pickle=False by default
+    return {k: z[k] for k in z.files}
+
+
+def _spacing(rec):
+    return (rec["xv"][1] - rec["xv"][0], rec["yv"][1] - rec["yv"][0], rec["zv"][1] - rec["zv"][0])
+
+
+def _close(a, b, rtol=1e-12):
+    scale = max(np.abs(b).max(), 1e-300)
+    err = np.abs(a - b).max()
+    assert err <= rtol * scale, f"max|diff| {err:.3e} > {rtol:.0e} * {scale:.3e}"
+
+
+@pytest.fixture(scope="module", params=FILES, ids=[os.path.basename(f)[7:-4] for f in FILES])
+def rec(request):
+    return _load(request.param)
+
+
+def test_fixture_count():
+    assert len(FILES) == 3
+
+
+def test_boundary_location(rec):
+    F = rec["F"]
+    dx, dy, dz = _spacing(rec)
+    d6 = po.prefilled_distances(F.shape, dx, dy, dz)
+    po.boundary_location(d6, F, dx, dy, dz)
+    ds = np.array([dx, dx, dy, dy, dz, dz])[:, None, None, None]
+    ref = rec["d6_six"]
+    got = np.stack(d6)
+    assert np.array_equal(got < ds, ref < ds)
+    assert np.abs(got - ref).max() <= 1e-12 * min(dx, dy, dz)
+
+
+def test_boundary_correction_eno2(rec):
+    F = rec["F"]
+    dx, dy, dz = _spacing(rec)
+    d6 = po.prefilled_distances(F.shape, dx, dy, dz)
+    po.boundary_location(d6, F, dx, dy, dz, order=2)
+    ref = rec["d6_eno"]
+    assert np.abs(np.stack(d6) - ref).max() <= 1e-12 * min(dx, dy, dz)
+
+
+def test_re_step_weno5(rec):
+    dx, dy, dz = _spacing(rec)
+    for lsf, key in ((rec["F"], "re_step"), (rec["F_stage"], "re_step_stage")):
+        got = po.re_step(lsf, rec["mask"], rec["deltat"], list(rec["d6_six"]), dx, dy, dz)
+        assert np.array_equal(got == 0, rec[key] == 0)
+        _close(got, rec[key])
+
+
+def test_re_step_eno2(rec):
+    dx, dy, dz = _spacing(rec)
+    got = po.re_step(rec["F"], rec["mask"], rec["deltat_eno"], list(rec["d6_eno"]), dx, dy, dz, scheme="eno2")
+    _close(got, rec["eno2_re_step"])
+
+
+def test_upwind_normal(rec):
+    dx, dy, dz = _spacing(rec)
+    got = po.upwind_normal(rec["F"], list(rec["d6_six"]), dx, dy, dz)
+    for a, b in zip(got, rec["upwind_normal"]):
+        _close(a, b)
+
+
+def test_boundary_interpolant(rec):
+    dx, dy, dz = _spacing(rec)
+    got = po.boundary_interpolant(list(rec["d6_six"]), rec["C"], dx, dy, dz)
+    for a, b in zip(got, rec["cp6"]):
+        assert np.array_equal(a == 0, b == 0)
+        _close(a, b, rtol=1e-11)
+
+
+def test_extend_step(rec):
+    dx, dy, dz = _spacing(rec)
+    got = po.extend_step(rec["deltat"], rec["C"], list(rec["v3"]), list(rec["d6_six"]), list(rec["cp6"]),
+                         dx, dy, dz)
+    _close(got, rec["extend_step"])
+
+
+def test_advection_step(rec):
+    dx, dy, dz = _spacing(rec)
+    got = po.advection_step(list(rec["u3"]), rec["F"], dx, dy, dz)
+    _close(got, rec["advection_step"])
+
+
+def test_weno_derivative(rec):
+    dx, dy, dz = _spacing(rec)
+    got = po.weno_derivative(rec["F"], dx, dy, dz)
+    for a, b in zip(got, rec["weno_derivative"]):
+        _close(a, b)
+
+
+def test_set_calculus_toolbox(rec):
+    dx, dy, dz = _spacing(rec)
+    ds = (dx + dy + dz) / 3
+    F = rec["F"]
+    got = po.set_calculus_toolbox(F, np.maximum(F, 0), dx, dy, dz, ds)
+    for k, name in enumerate(po.TOOLBOX_FIELDS):
+        _close(got[name], rec["toolbox"][k], rtol=1e-11)
