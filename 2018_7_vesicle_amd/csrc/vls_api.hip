@@ -83,14 +83,23 @@ struct vls_sdf3 {
     Geom g;
     double* pool = nullptr;  // every field below lives in one allocation
     uint8_t* flag_pool = nullptr;
+    int* int_pool = nullptr;
     Dist6 d{};        // distances of the last set_distance (xpr..zpd)
     double* min_dist = nullptr;
     uint8_t* flags = nullptr;  // FLAG_SUBCELL of d, FLAG_MASK of the last reinit
+    int* list = nullptr;       // points of d with a subcell distance, count in counts[0]
     Dist6 ad{};       // distances of an auxiliary field (ENO2 reinit, redistance of A)
     uint8_t* aflags = nullptr;
+    int* alist = nullptr;      // counts[1]
     Dist6 fd{};       // distances of F inside the surface redistance
     uint8_t* fflags = nullptr;
-    Dist6 cp{};       // interface values of the extended field
+    int* flist = nullptr;      // counts[2]
+    int* over = nullptr;       // reinitialization points deferred by the bulk pass
+    int* counts = nullptr;     // [0..2] list counts, [4..5] deferred counts
+    double* ldist[3] = {nullptr, nullptr, nullptr};  // list-order distances of d, ad, fd (6 rows of n)
+    double* ldt[3] = {nullptr, nullptr, nullptr};    // list-order pseudo-time steps
+    double* lcp = nullptr;   // interface values of the extended field, list order (6 rows of n)
+    double* lvel = nullptr;  // extension velocity at the listed points (3 rows of n)
     double* v[3] = {nullptr, nullptr, nullptr};
     double* t[4] = {nullptr, nullptr, nullptr, nullptr};
     double* partial = nullptr;
@@ -99,7 +108,8 @@ struct vls_sdf3 {
 
 namespace {
 
-constexpr int kFields = 6 /*d*/ + 1 /*min*/ + 6 /*ad*/ + 6 /*fd*/ + 6 /*cp*/ + 3 /*v*/ + 4 /*t*/;
+constexpr int kFields = 6 /*d*/ + 1 /*min*/ + 6 /*ad*/ + 6 /*fd*/ + 3 /*v*/ + 4 /*t*/ + 3 * 7 /*ldist, ldt*/ +
+                        6 /*lcp*/ + 3 /*lvel*/;
 
 Dist6 carve6(double*& p, int64_t n)
 {
@@ -111,26 +121,82 @@ Dist6 carve6(double*& p, int64_t n)
 // extension loop (WENO5RK3Extend.m:50-74, and the local extension of the
 // surface redistance, WENO5RK3ClosetPointSurfaceRedistance.m:119-157):
 // cp from C once, then `iterations` RK3 steps; C is updated in place.
-int extend_loop(vls_sdf3* h, double* C, int iterations, const Dist6& d, const uint8_t* flags, hipStream_t s)
+// a distance set: the six arrays, their flags and their interface list
+struct DSet {
+    Dist6 d;
+    uint8_t* flags;
+    ListView lv;
+};
+DSet make_set(vls_sdf3* h, const Dist6& d, uint8_t* flags, int* list, int k)
+{
+    return DSet{d, flags, ListView{list, h->counts + k, h->ldist[k], h->ldt[k], (int)h->g.n()}};
+}
+DSet main_set(vls_sdf3* h) { return make_set(h, h->d, h->flags, h->list, 0); }
+DSet aux_set(vls_sdf3* h) { return make_set(h, h->ad, h->aflags, h->alist, 1); }
+DSet f_set(vls_sdf3* h) { return make_set(h, h->fd, h->fflags, h->flist, 2); }
+
+// extension loop (WENO5RK3Extend.m:50-74, and the local extension of the
+// surface redistance, WENO5RK3ClosetPointSurfaceRedistance.m:119-157):
+// interface values of C once, then `iterations` RK3 steps; C in place.  The
+// velocity (h->v) must be set; its listed values are gathered here.
+int extend_loop(vls_sdf3* h, double* C, int iterations, const DSet& ds, hipStream_t s)
 {
     const Geom& g = h->g;
-    const Dist6c dc = constify(d), cpc = constify(h->cp);
-    VLS_HIP(launch_boundary_interpolant(h->cp, dc, C, g, s));
+    VLS_HIP(launch_list_interp(h->lcp, ds.lv, C, g, s));
     for (int it = 0; it < iterations; ++it) {
-        VLS_HIP(launch_extend_stage(h->t[0], C, nullptr, h->v[0], h->v[1], h->v[2], flags, dc, cpc, g, 0, s));
-        VLS_HIP(launch_extend_stage(h->t[1], h->t[0], C, h->v[0], h->v[1], h->v[2], flags, dc, cpc, g, 1, s));
-        VLS_HIP(launch_extend_stage(C, h->t[1], C, h->v[0], h->v[1], h->v[2], flags, dc, cpc, g, 2, s));
+        VLS_HIP(launch_extend_stage(h->t[0], C, nullptr, h->v[0], h->v[1], h->v[2], ds.flags, ds.lv, h->lcp, h->lvel,
+                                    g, 0, s));
+        VLS_HIP(launch_extend_stage(h->t[1], h->t[0], C, h->v[0], h->v[1], h->v[2], ds.flags, ds.lv, h->lcp, h->lvel,
+                                    g, 1, s));
+        VLS_HIP(launch_extend_stage(C, h->t[1], C, h->v[0], h->v[1], h->v[2], ds.flags, ds.lv, h->lcp, h->lvel, g, 2,
+                                    s));
     }
     return VLS_OK;
 }
 
-int set_distance_into(vls_sdf3* h, int order, const Dist6& d, double* min_dist, uint8_t* flags,
-                      const double* F, const double* mask_src, hipStream_t s)
+// WENO5-RK3 reinitialization stages of one call share a sequence number for
+// the deferred-point counters (see launch_reinit_stage)
+struct ReinitSeq {
+    int* counts;
+    int seq;
+};
+
+int reinit_stage(vls_sdf3* h, ReinitSeq& rs, double* out, const double* fs, const double* fn, const DSet& ds,
+                 int stage, hipStream_t s)
+{
+    VLS_HIP(launch_reinit_stage(out, fs, fn, ds.flags, ds.lv, h->g, stage, h->over, rs.counts, rs.seq++, s));
+    return VLS_OK;
+}
+
+int reinit_begin(vls_sdf3* h, ReinitSeq& rs, hipStream_t s)
+{
+    rs.counts = h->counts + 4;
+    rs.seq = 0;
+    VLS_HIP(launch_zero_ints(rs.counts, 2, s));
+    return VLS_OK;
+}
+
+// WENO5-RK3 reinitialization loop of F in place with mask and distances of `ds`
+int reinit_loop(vls_sdf3* h, double* F, int iterations, const DSet& ds, hipStream_t s)
+{
+    ReinitSeq rs;
+    int rc = reinit_begin(h, rs, s);
+    for (int it = 0; it < iterations && rc == VLS_OK; ++it) {
+        if ((rc = reinit_stage(h, rs, h->t[0], F, nullptr, ds, 0, s))) break;
+        if ((rc = reinit_stage(h, rs, h->t[1], h->t[0], F, ds, 1, s))) break;
+        rc = reinit_stage(h, rs, F, h->t[1], F, ds, 2, s);
+    }
+    return rc;
+}
+
+int set_distance_into(vls_sdf3* h, int order, const DSet& ds, double* min_dist, const double* F,
+                      const double* mask_src, hipStream_t s)
 {
     const Geom& g = h->g;
-    VLS_HIP(launch_fill_spacing(d, g, s));
-    VLS_HIP(launch_boundary_location(order, d, F, g, s));
-    VLS_HIP(launch_min_dist_flags(min_dist, flags, constify(d), mask_src, g, s));
+    VLS_HIP(launch_fill_spacing(ds.d, g, ds.lv.count, s));
+    VLS_HIP(launch_boundary_location(order, ds.d, F, g, s));
+    VLS_HIP(launch_min_dist_flags(min_dist, ds.flags, constify(ds.d), mask_src, g, ds.lv.idx, ds.lv.count, s));
+    VLS_HIP(launch_list_gather(ds.lv, constify(ds.d), g, s));
     return VLS_OK;
 }
 
@@ -324,12 +390,30 @@ int vls_sdf3_create(vls_sdf3_t* out, int rows, int cols, int pges, double dx, do
         delete h;
         return fail(VLS_ERR_ALLOC, "hipMalloc of flags failed");
     }
+    if (hipMalloc((void**)&h->int_pool, sizeof(int) * ((size_t)n * 4 + 16)) != hipSuccess) {
+        (void)hipFree(h->pool);
+        (void)hipFree(h->flag_pool);
+        delete h;
+        return fail(VLS_ERR_ALLOC, "hipMalloc of point lists failed");
+    }
+    if (hipMemset(h->int_pool, 0, sizeof(int) * ((size_t)n * 4 + 16)) != hipSuccess) {
+        (void)hipFree(h->pool);
+        (void)hipFree(h->flag_pool);
+        (void)hipFree(h->int_pool);
+        delete h;
+        return fail(VLS_ERR_HIP, "hipMemset of point lists failed");
+    }
     double* p = h->pool;
     h->d = carve6(p, n);
     h->min_dist = p; p += n;
     h->ad = carve6(p, n);
     h->fd = carve6(p, n);
-    h->cp = carve6(p, n);
+    for (int k = 0; k < 3; ++k) {
+        h->ldist[k] = p; p += 6 * n;
+        h->ldt[k] = p; p += n;
+    }
+    h->lcp = p; p += 6 * n;
+    h->lvel = p; p += 3 * n;
     for (int a = 0; a < 3; ++a) { h->v[a] = p; p += n; }
     for (int k = 0; k < 4; ++k) { h->t[k] = p; p += n; }
     h->partial = p; p += (size_t)reduce_blocks() * 4;
@@ -337,6 +421,11 @@ int vls_sdf3_create(vls_sdf3_t* out, int rows, int cols, int pges, double dx, do
     h->flags = h->flag_pool;
     h->aflags = h->flag_pool + n;
     h->fflags = h->flag_pool + 2 * n;
+    h->list = h->int_pool;
+    h->alist = h->int_pool + n;
+    h->flist = h->int_pool + 2 * n;
+    h->over = h->int_pool + 3 * n;
+    h->counts = h->int_pool + 4 * n;
     *out = h;
     return VLS_OK;
 }
@@ -344,16 +433,16 @@ int vls_sdf3_create(vls_sdf3_t* out, int rows, int cols, int pges, double dx, do
 int vls_sdf3_destroy(vls_sdf3_t h)
 {
     if (!h) return VLS_OK;
-    hipError_t e1 = hipFree(h->pool), e2 = hipFree(h->flag_pool);
+    hipError_t e1 = hipFree(h->pool), e2 = hipFree(h->flag_pool), e3 = hipFree(h->int_pool);
     delete h;
-    if (e1 != hipSuccess || e2 != hipSuccess) return fail(VLS_ERR_HIP, "hipFree failed");
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) return fail(VLS_ERR_HIP, "hipFree failed");
     return VLS_OK;
 }
 
 int vls_sdf3_set_distance(vls_sdf3_t h, const double* F, vls_stream_t stream)
 {
     VLS_REQUIRE(h && F, "null argument");
-    return set_distance_into(h, 6, h->d, h->min_dist, h->flags, F, nullptr, S(stream));
+    return set_distance_into(h, 6, main_set(h), h->min_dist, F, nullptr, S(stream));
 }
 
 int vls_sdf3_distances(vls_sdf3_t h, const double** out7)
@@ -371,15 +460,8 @@ int vls_sdf3_weno5_rk3_reinit(vls_sdf3_t h, double* F, int iterations, vls_strea
     VLS_REQUIRE(h && F, "null argument");
     VLS_REQUIRE(iterations >= 0, "negative iteration count");
     const hipStream_t s = S(stream);
-    const Geom& g = h->g;
-    const Dist6c dc = constify(h->d);
-    VLS_HIP(launch_set_mask_bit(h->flags, F, g, s));
-    for (int it = 0; it < iterations; ++it) {
-        VLS_HIP(launch_reinit_stage(h->t[0], F, nullptr, h->flags, dc, g, 0, s));
-        VLS_HIP(launch_reinit_stage(h->t[1], h->t[0], F, h->flags, dc, g, 1, s));
-        VLS_HIP(launch_reinit_stage(F, h->t[1], F, h->flags, dc, g, 2, s));
-    }
-    return VLS_OK;
+    VLS_HIP(launch_set_mask_bit(h->flags, F, h->g, s));
+    return reinit_loop(h, F, iterations, main_set(h), s);
 }
 
 int vls_sdf3_eno2_reinit(vls_sdf3_t h, double* F, int iterations, int rk2, double cfl, vls_stream_t stream)
@@ -389,7 +471,7 @@ int vls_sdf3_eno2_reinit(vls_sdf3_t h, double* F, int iterations, int rk2, doubl
     const hipStream_t s = S(stream);
     const Geom& g = h->g;
     // ENO2 distances of the argument and mask = F<0 (ENORK2Reinitialization.m:7-25)
-    int rc = set_distance_into(h, 2, h->ad, h->t[2], h->aflags, F, F, s);
+    int rc = set_distance_into(h, 2, aux_set(h), nullptr, F, F, s);
     if (rc) return rc;
     const Dist6c dc = constify(h->ad);
     if (rk2) {
@@ -417,9 +499,11 @@ int vls_sdf3_weno5_rk3_extend(vls_sdf3_t h, const double* F, double* C, int iter
     VLS_REQUIRE(F != C, "C must not alias F");
     const hipStream_t s = S(stream);
     const Geom& g = h->g;
+    const DSet ms = main_set(h);
     VLS_HIP(launch_upwind_normal(h->v[0], h->v[1], h->v[2], F, constify(h->d), g, s));
     VLS_HIP(launch_extension_velocity(h->v[0], h->v[1], h->v[2], F, 1e-6, g, s));
-    return extend_loop(h, C, iterations, h->d, h->flags, s);
+    VLS_HIP(launch_list_gather_vel(h->lvel, ms.lv, h->v[0], h->v[1], h->v[2], g, s));
+    return extend_loop(h, C, iterations, ms, s);
 }
 
 int vls_sdf3_surface_redistance(vls_sdf3_t h, const double* F, double* A, int iter1, int iter2, vls_stream_t stream)
@@ -431,23 +515,26 @@ int vls_sdf3_surface_redistance(vls_sdf3_t h, const double* F, double* A, int it
     const Geom& g = h->g;
     int rc;
     // A's distances, deltat and mask (WENO5RK3ClosetPointSurfaceRedistance.m:8-25)
-    if ((rc = set_distance_into(h, 6, h->ad, h->t[2], h->aflags, A, A, s))) return rc;
+    const DSet as = aux_set(h), fs = f_set(h);
+    if ((rc = set_distance_into(h, 6, as, nullptr, A, A, s))) return rc;
     // F's distances and the extension velocity with floor 1e-14 (:30-74)
-    if ((rc = set_distance_into(h, 6, h->fd, h->t[3], h->fflags, F, nullptr, s))) return rc;
+    if ((rc = set_distance_into(h, 6, fs, nullptr, F, nullptr, s))) return rc;
     VLS_HIP(launch_upwind_normal(h->v[0], h->v[1], h->v[2], F, constify(h->fd), g, s));
     VLS_HIP(launch_extension_velocity(h->v[0], h->v[1], h->v[2], F, 1e-14, g, s));
-    const Dist6c adc = constify(h->ad);
+    VLS_HIP(launch_list_gather_vel(h->lvel, fs.lv, h->v[0], h->v[1], h->v[2], g, s));
     double* a1 = h->t[2];
     double* a05 = h->t[3];
+    ReinitSeq rs;
+    if ((rc = reinit_begin(h, rs, s))) return rc;
     for (int it = 0; it < iter1; ++it) {
-        if ((rc = extend_loop(h, A, iter2, h->fd, h->fflags, s))) return rc;
-        VLS_HIP(launch_reinit_stage(a1, A, nullptr, h->aflags, adc, g, 0, s));
-        if ((rc = extend_loop(h, a1, iter2, h->fd, h->fflags, s))) return rc;
-        VLS_HIP(launch_reinit_stage(a05, a1, A, h->aflags, adc, g, 1, s));
-        if ((rc = extend_loop(h, a05, iter2, h->fd, h->fflags, s))) return rc;
-        VLS_HIP(launch_reinit_stage(A, a05, A, h->aflags, adc, g, 2, s));
+        if ((rc = extend_loop(h, A, iter2, fs, s))) return rc;
+        if ((rc = reinit_stage(h, rs, a1, A, nullptr, as, 0, s))) return rc;
+        if ((rc = extend_loop(h, a1, iter2, fs, s))) return rc;
+        if ((rc = reinit_stage(h, rs, a05, a1, A, as, 1, s))) return rc;
+        if ((rc = extend_loop(h, a05, iter2, fs, s))) return rc;
+        if ((rc = reinit_stage(h, rs, A, a05, A, as, 2, s))) return rc;
     }
-    return extend_loop(h, A, iter2, h->fd, h->fflags, s);
+    return extend_loop(h, A, iter2, fs, s);
 }
 
 int vls_sdf3_advect_rk3(vls_sdf3_t h, double* F, const double* vx, const double* vy, const double* vz, double dt,

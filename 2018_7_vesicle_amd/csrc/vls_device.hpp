@@ -85,6 +85,24 @@ __device__ __forceinline__ double weno5_onesided(double v1, double v2, double v3
     return fma(w1, phi1, fma(w2, phi2, w3 * phi3)) / (w1 + w2 + w3);
 }
 
+// Upwind term of one axis, min(0,vel)*D+ + max(0,vel)*D- (extend_step.cu:121-123,
+// advection_step.cu:51-53), added to acc in that order.  Only one of the two
+// one-sided derivatives has a nonzero weight, so only that one is evaluated:
+// the stencil is picked per lane (no divergence) and the other term, an exact
+// zero times a finite value, is the +-0 the sum would have added unchanged.
+// acc is never -0 (it starts at +0 and x + -x rounds to +0), so the result
+// is bit-identical to evaluating both derivatives.
+__device__ __forceinline__ double upwind_axis(double acc, double vel, const double p[7], double inv_ds)
+{
+    double v[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v[k] = (p[k + 1] - p[k]) * inv_ds;
+    const bool fore = vel < 0;
+    const double d = weno5_onesided(fore ? v[5] : v[0], fore ? v[4] : v[1], fore ? v[3] : v[2],
+                                     fore ? v[2] : v[3], fore ? v[1] : v[4]);
+    return acc + (fore ? vel : dmax(0.0, vel)) * d;
+}
+
 // back/fore pair from 7 nodal values (re_step.cu:260-267), 1/ds precomputed
 __device__ __forceinline__ void weno5_pair(double& d_fore, double& d_back, const double p[7], double inv_ds)
 {
@@ -100,7 +118,7 @@ __device__ __forceinline__ void weno5_pair(double& d_fore, double& d_back, const
 // re_step.cu:79-243 (weno_nonuniform), reconstructing at face a[2] (fore) or
 // a[3] (back) from first divided differences u[k] over [a[k], a[k+1]].
 // ---------------------------------------------------------------------------
-__device__ __noinline__ double weno_nonuniform_face(const double* a, const double* u, bool back)
+__device__ __forceinline__ double weno_nonuniform_face(const double* a, const double* u, bool back)
 {
 #pragma clang fp contract(off)
     const double xm25 = a[0], xm15 = a[1], xm05 = a[2], x05 = a[3], x15 = a[4], x25 = a[5];
@@ -152,35 +170,72 @@ __device__ __noinline__ double weno_nonuniform_face(const double* a, const doubl
     return v0 * (al0 / sum) + v1 * (al1 / sum) + v2 * (al2 / sum);
 }
 
-// select_stencil + weno_nonuniform (re_step.cu:41-77; interface values
-// h_fore/h_back as in 2_2_WENO5RK3_Extend/shared_utilities.cup:49-85)
-__device__ __noinline__ void weno_nonuniform_pair(double& d_fore, double& d_back, const double* p,
-                                                  double r1, double l1, double ds, double h_fore,
-                                                  double h_back)
+struct Pair {
+    double fore, back;
+};
+
+// select_stencil (re_step.cu:41-77; interface values h_fore/h_back as in
+// 2_2_WENO5RK3_Extend/shared_utilities.cup:49-85) and the first divided
+// differences (re_step.cu:83-88)
+struct NonuniformStencil {
+    double x[7], d[6];
+};
+
+__device__ __forceinline__ NonuniformStencil nonuniform_stencil(double p0, double p1, double p2, double p3, double p4,
+                                                                double p5, double p6, double r1, double l1, double ds,
+                                                                double h_fore, double h_back)
 {
 #pragma clang fp contract(off)
-    double x[7], h[7];
-    h[3] = p[3];
-    x[3] = 0.0;
+    NonuniformStencil st;
+    double h[7];
+    h[3] = p3;
+    st.x[3] = 0.0;
     if (r1 < ds) {
-        x[4] = r1; x[5] = ds; x[6] = 2 * ds;
-        h[4] = h_fore; h[5] = p[4]; h[6] = p[5];
+        st.x[4] = r1; st.x[5] = ds; st.x[6] = 2 * ds;
+        h[4] = h_fore; h[5] = p4; h[6] = p5;
     } else {
-        x[4] = ds; x[5] = 2 * ds; x[6] = 3 * ds;
-        h[4] = p[4]; h[5] = p[5]; h[6] = p[6];
+        st.x[4] = ds; st.x[5] = 2 * ds; st.x[6] = 3 * ds;
+        h[4] = p4; h[5] = p5; h[6] = p6;
     }
     if (l1 < ds) {
-        x[2] = -l1; x[1] = -ds; x[0] = -2 * ds;
-        h[2] = h_back; h[1] = p[2]; h[0] = p[1];
+        st.x[2] = -l1; st.x[1] = -ds; st.x[0] = -2 * ds;
+        h[2] = h_back; h[1] = p2; h[0] = p1;
     } else {
-        x[2] = -ds; x[1] = -2 * ds; x[0] = -3 * ds;
-        h[2] = p[2]; h[1] = p[1]; h[0] = p[0];
+        st.x[2] = -ds; st.x[1] = -2 * ds; st.x[0] = -3 * ds;
+        h[2] = p2; h[1] = p1; h[0] = p0;
     }
-    double d[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) d[k] = (h[k + 1] - h[k]) / (x[k + 1] - x[k]);
-    d_fore = weno_nonuniform_face(&x[1], &d[1], false);
-    d_back = weno_nonuniform_face(&x[0], &d[0], true);
+    for (int k = 0; k < 6; ++k) st.d[k] = (h[k + 1] - h[k]) / (st.x[k + 1] - st.x[k]);
+    return st;
+}
+
+// weno_nonuniform (re_step.cu:79-243), both faces.  Not inlined: it runs only
+// at interface nodes; scalar arguments and a returned struct keep the
+// caller's stencil in registers.
+__device__ __noinline__ Pair weno_nonuniform_pair(double p0, double p1, double p2, double p3, double p4, double p5,
+                                                  double p6, double r1, double l1, double ds, double h_fore,
+                                                  double h_back)
+{
+    const NonuniformStencil st = nonuniform_stencil(p0, p1, p2, p3, p4, p5, p6, r1, l1, ds, h_fore, h_back);
+    Pair out;
+    out.fore = weno_nonuniform_face(&st.x[1], &st.d[1], false);
+    out.back = weno_nonuniform_face(&st.x[0], &st.d[0], true);
+    return out;
+}
+
+// one face only: d_fore (back == false) or d_back
+__device__ __noinline__ double weno_nonuniform_one(double p0, double p1, double p2, double p3, double p4, double p5,
+                                                   double p6, double r1, double l1, double ds, double h_fore,
+                                                   double h_back, bool back)
+{
+    const NonuniformStencil st = nonuniform_stencil(p0, p1, p2, p3, p4, p5, p6, r1, l1, ds, h_fore, h_back);
+    return back ? weno_nonuniform_face(&st.x[0], &st.d[0], true) : weno_nonuniform_face(&st.x[1], &st.d[1], false);
+}
+
+__device__ __forceinline__ Pair weno_nonuniform_pair(const double v[7], double r1, double l1, double ds,
+                                                     double h_fore, double h_back)
+{
+    return weno_nonuniform_pair(v[0], v[1], v[2], v[3], v[4], v[5], v[6], r1, l1, ds, h_fore, h_back);
 }
 
 // ---------------------------------------------------------------------------
@@ -290,7 +345,7 @@ __device__ __forceinline__ double poly5_eval(const double coef[6], double x)
 
 // Brent root of the quintic in [x1,x2] with the reference's control flow
 // (zeroin, shared_utilities.hpp:109-173; abs() of doubles is fabs())
-__device__ __noinline__ double zeroin_poly5(const double* coef, double x1, double x2, double toler)
+__device__ __forceinline__ double zeroin_poly5(const double* coef, double x1, double x2, double toler)
 {
 #pragma clang fp contract(off)
     const int ITMAX = 100;
@@ -356,24 +411,25 @@ __device__ __noinline__ double six_distance(double v0, double v1, double v2, dou
 }
 
 // sixth_interp: 2_2_WENO5RK3_Extend/boundary_interpolant.cu:77-99
-__device__ __noinline__ void sixth_interp(double& c_fore, double& c_back, double dis_f, double dis_b, double s,
-                                          const double* v)
+__device__ __noinline__ Pair sixth_interp(double dis_f, double dis_b, double s, double v0, double v1, double v2,
+                                          double v3, double v4, double v5, double v6)
 {
-#pragma clang fp contract(off)
-    c_fore = 0.0;
-    c_back = 0.0;
+    Pair out{0.0, 0.0};
     if (dis_f < s) {
         const double xf[6] = {s, 2 * s, 3 * s, 4 * s, 5 * s, 6 * s};
+        const double yf[6] = {v1, v2, v3, v4, v5, v6};
         double coef[6];
-        poly5_interp(xf, v + 1, coef);
-        c_fore = poly5_eval(coef, 3 * s + dis_f);
+        poly5_interp(xf, yf, coef);
+        out.fore = poly5_eval(coef, 3 * s + dis_f);
     }
     if (dis_b < s) {
         const double xb[6] = {0.0, s, 2 * s, 3 * s, 4 * s, 5 * s};
+        const double yb[6] = {v0, v1, v2, v3, v4, v5};
         double coef[6];
-        poly5_interp(xb, v, coef);
-        c_back = poly5_eval(coef, 3 * s - dis_b);
+        poly5_interp(xb, yb, coef);
+        out.back = poly5_eval(coef, 3 * s - dis_b);
     }
+    return out;
 }
 
 }  // namespace vls

@@ -14,6 +14,8 @@
 //     MATLAB gpuArray kernels), read the six subcell distances only where a
 //     per-point flag says they differ from the grid spacing, and derive the
 //     pseudo-time step 0.3*min_dist from them.
+#include <algorithm>
+
 #include "vls_device.hpp"
 #include "vls_kernels.hpp"
 
@@ -35,6 +37,14 @@ __device__ __forceinline__ idx_t lin(const Box& b, int r, int c, int p)
     return (idx_t)p * b.rows * b.cols + (idx_t)c * b.rows + r;
 }
 
+// 1/dx, 1/dy, 1/dz and the pseudo-time step of a point without subcell
+// distances, 0.3*min(dx,dx,dy,dy,dz,dz) (setDistance.m:17-21 with every
+// distance equal to the spacing), computed once on the host
+struct Inv {
+    double i[3];
+    double dt_bulk;
+};
+
 // RK combinations in the order the MATLAB wrappers evaluate them
 // (+SD/@SDF3/WENO5RK3Reinitialization.m:20-37), without fused multiply-adds.
 __device__ __forceinline__ double rk_combine(int stage, double fs, double step, double fn)
@@ -49,9 +59,10 @@ __device__ __forceinline__ double rk_combine(int stage, double fs, double step, 
 // ---------------------------------------------------------------------------
 // setDistance: fill, crossing distances, min_dist + flags
 // ---------------------------------------------------------------------------
-__global__ void k_fill_spacing(Dist6 d, Box b, idx_t n)
+__global__ void k_fill_spacing(Dist6 d, Box b, idx_t n, int* __restrict__ list_count)
 {
     const idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0 && list_count) *list_count = 0;
     if (i >= n) return;
     d.r[0][i] = b.dx; d.l[0][i] = b.dx;
     d.r[1][i] = b.dy; d.l[1][i] = b.dy;
@@ -88,21 +99,44 @@ __global__ void __launch_bounds__(256) k_boundary_location(Dist6 d, const double
     }
 }
 
-// min_dist (setDistance.m:17-21) and the per-point flags of the fused kernels.
-// mask_src may be null (no mask bit).
+// append `take` lanes' values to list[*count ...] with one atomic per wave
+__device__ __forceinline__ void wave_append(bool take, int value, int* __restrict__ list, int* __restrict__ count)
+{
+    const unsigned long long m = __ballot(take);
+    if (m == 0) return;
+    const int lane = (int)__lane_id();
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(count, __popcll(m));
+    base = __shfl(base, leader, 64);
+    if (take) list[base + __popcll(m & ((1ull << lane) - 1ull))] = value;
+}
+
+// min_dist (setDistance.m:17-21), the per-point flags of the fused kernels and
+// the list of interface points (some distance below the spacing).
+// mask_src may be null (no mask bit); list may be null.
 __global__ void k_min_dist_flags(double* __restrict__ min_dist, uint8_t* __restrict__ flags, Dist6c d,
-                                 const double* __restrict__ mask_src, Box b, idx_t n)
+                                 const double* __restrict__ mask_src, Box b, idx_t n, int* __restrict__ list,
+                                 int* __restrict__ list_count)
 {
     const idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double xr = d.r[0][i], xl = d.l[0][i], yf = d.r[1][i], yb = d.l[1][i], zu = d.r[2][i], zd = d.l[2][i];
-    if (min_dist) min_dist[i] = dmin(dmin(xr, xl), dmin(dmin(yf, yb), dmin(zu, zd)));
-    if (flags) {
-        uint8_t f = 0;
-        if (xr < b.dx || xl < b.dx || yf < b.dy || yb < b.dy || zu < b.dz || zd < b.dz) f |= FLAG_SUBCELL;
-        if (mask_src && mask_src[i] < 0) f |= FLAG_MASK;
-        flags[i] = f;
+    bool sub = false;
+    if (i < n) {
+        const double xr = d.r[0][i], xl = d.l[0][i], yf = d.r[1][i], yb = d.l[1][i], zu = d.r[2][i], zd = d.l[2][i];
+        if (min_dist) min_dist[i] = dmin(dmin(xr, xl), dmin(dmin(yf, yb), dmin(zu, zd)));
+        sub = xr < b.dx || xl < b.dx || yf < b.dy || yb < b.dy || zu < b.dz || zd < b.dz;
+        if (flags) {
+            uint8_t f = sub ? FLAG_SUBCELL : 0;
+            if (mask_src && mask_src[i] < 0) f |= FLAG_MASK;
+            flags[i] = f;
+        }
     }
+    if (list) wave_append(sub, (int)i, list, list_count);
+}
+
+__global__ void k_zero_ints(int* __restrict__ a, int n)
+{
+    if ((int)threadIdx.x < n) a[threadIdx.x] = 0;
 }
 
 // mask bit only (reinit with distances from an earlier setDistance)
@@ -122,6 +156,25 @@ struct Dists {
     double r[3], l[3];
 };
 
+// an interface list in the form the stage kernels read: indices and, in list
+// order, the six distances (row k of `dist` is xpr,xpl,ypf,ypb,zpu,zpd for
+// k = 0..5, rows `cap` apart) and the pseudo-time step 0.3*min_dist, or -1 for
+// a frozen point
+struct IList {
+    const int* idx;
+    const int* count;
+    const double* dist;
+    const double* dt;
+    int cap;
+};
+struct IListW {
+    const int* idx;
+    const int* count;
+    double* dist;
+    double* dt;
+    int cap;
+};
+
 __device__ __forceinline__ bool frozen(const Dists& q, double dx)
 {
     const double eps = 1e-6 * dx;
@@ -137,10 +190,13 @@ __device__ __forceinline__ double min6(const Dists& q)
 __device__ __forceinline__ void reinit_axis(double& fore, double& back, const double v[7], double r1, double l1,
                                             double ds, double inv_ds)
 {
-    if (v[2] * v[3] < 0 || v[3] * v[4] < 0)
-        weno_nonuniform_pair(fore, back, v, r1, l1, ds, 0.0, 0.0);
-    else
+    if (v[2] * v[3] < 0 || v[3] * v[4] < 0) {
+        const Pair q = weno_nonuniform_pair(v, r1, l1, ds, 0.0, 0.0);
+        fore = q.fore;
+        back = q.back;
+    } else {
         weno5_pair(fore, back, v, inv_ds);
+    }
 }
 
 __device__ __forceinline__ double reinit_residual(const double* __restrict__ f, int r, int c, int p, const Box& b,
@@ -148,7 +204,7 @@ __device__ __forceinline__ double reinit_residual(const double* __restrict__ f, 
 {
     const double sp[3] = {b.dx, b.dy, b.dz};
     double fo[3], bk[3];
-#pragma unroll 1
+#pragma unroll
     for (int a = 0; a < 3; ++a) {
         double v[7];
         line7(f, r, c, p, a, b, v);
@@ -172,29 +228,206 @@ __global__ void __launch_bounds__(256) k_re_step(double* __restrict__ step, cons
     step[ind] = reinit_residual(lsf, r, c, p, b, q, mask[ind] != 0, deltat[ind]);
 }
 
-// fused WENO5-RK3 reinitialization stage: out = combine(fs - L(fs), fn)
-__global__ void __launch_bounds__(256) k_reinit_stage(double* __restrict__ out, const double* __restrict__ fs,
-                                                      const double* __restrict__ fn,
-                                                      const uint8_t* __restrict__ flags, Dist6c d, Box b,
-                                                      int stage)
+// ---------------------------------------------------------------------------
+// split stage kernels.  Bulk kernels cover the box in 64x4 (rows x cols)
+// tiles of one page and take every point without subcell distances through
+// uniform WENO5 only (few registers, high occupancy); interface kernels
+// stride over a list of points and evaluate the full residual.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool bulk_point(const Box& b, int blk, int& r, int& c, int& p)
 {
-    int r, c, p;
-    if (!point_of_thread(b, r, c, p)) return;
-    const idx_t ind = lin(b, r, c, p);
-    const uint8_t fl = flags[ind];
-    Dists q;
-    if (fl & FLAG_SUBCELL) {
+    const int ntr = (b.rows + 63) >> 6, ntc = (b.cols + 3) >> 2;
+    const int rb = blk % ntr;
+    const int t = blk / ntr;
+    const int cb = t % ntc;
+    p = t / ntc;
+    r = rb * 64 + (int)(threadIdx.x & 63);
+    c = cb * 4 + (int)(threadIdx.x >> 6);
+    return r < b.rows && c < b.cols;
+}
+
+__device__ __forceinline__ void coords_of(const Box& b, idx_t ind, int& r, int& c, int& p)
+{
+    const idx_t plane = (idx_t)b.rows * b.cols;
+    p = (int)(ind / plane);
+    const int rem = (int)(ind - (idx_t)p * plane);
+    c = rem / b.rows;
+    r = rem - c * b.rows;
+}
+
+__device__ __forceinline__ void load_dists(Dists& q, const Dist6c& d, idx_t ind)
+{
 #pragma unroll
-        for (int a = 0; a < 3; ++a) { q.r[a] = d.r[a][ind]; q.l[a] = d.l[a][ind]; }
-    } else {
-        q.r[0] = q.l[0] = b.dx; q.r[1] = q.l[1] = b.dy; q.r[2] = q.l[2] = b.dz;
+    for (int a = 0; a < 3; ++a) { q.r[a] = d.r[a][ind]; q.l[a] = d.l[a][ind]; }
+}
+
+// WENO5-RK3 reinitialization stage, launch 1 of 2: the bulk.  A point with
+// subcell distances is left to launch 2; so is a point whose stage field
+// changes sign next to it although it has none (it needs the nonuniform
+// stencil): those are appended to `over`, counted in over_count[seq & 1].
+// The other slot is cleared here for the next stage.
+__global__ void __launch_bounds__(256) k_reinit_bulk(double* __restrict__ out, const double* __restrict__ fs,
+                                                     const double* __restrict__ fn, const uint8_t* __restrict__ flags,
+                                                     Box b, Inv invs, int stage, int* __restrict__ over,
+                                                     int* __restrict__ over_count, int seq)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) over_count[(seq + 1) & 1] = 0;
+    int r, c, p;
+    const bool valid = bulk_point(b, (int)blockIdx.x, r, c, p);
+    const idx_t ind = valid ? lin(b, r, c, p) : 0;
+    const uint8_t fl = valid ? flags[ind] : (uint8_t)FLAG_SUBCELL;
+    bool defer = false;
+    if (!(fl & FLAG_SUBCELL)) {
+        double fo[3], bk[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            double v[7];
+            line7(fs, r, c, p, a, b, v);
+            defer = defer || (v[2] * v[3] < 0 || v[3] * v[4] < 0);
+            weno5_pair(fo[a], bk[a], v, invs.i[a]);
+        }
+        if (!defer) {
+            const double step = godunov_step((fl & FLAG_MASK) != 0, fo[0], bk[0], fo[1], bk[1], fo[2], bk[2],
+                                             invs.dt_bulk);
+            out[ind] = rk_combine(stage, fs[ind], step, (stage == 0) ? 0.0 : fn[ind]);
+        }
     }
-    const double fsv = fs[ind];
-    const double fnv = (stage == 0) ? 0.0 : fn[ind];
-    double step = 0.0;
-    if (!((fl & FLAG_SUBCELL) && frozen(q, b.dx)))
-        step = reinit_residual(fs, r, c, p, b, q, (fl & FLAG_MASK) != 0, 0.3 * min6(q));
-    out[ind] = rk_combine(stage, fsv, step, fnv);
+    wave_append(defer, (int)ind, over, &over_count[seq & 1]);
+}
+
+// Interface work is spread over three lanes per point, one per axis (21
+// points per wave64, lane 63 idle): the per-axis chains are long (the
+// nonuniform WENO divides ~30 times) and the lists short, so this triples the
+// parallelism.  The axis results meet in the group's first lane, which adds
+// them in the reference's x, y, z order.
+struct AxisLane {
+    int point;  // index into the point list, or -1
+    int axis;   // 0 x, 1 y, 2 z
+    int base;   // lane of the group's x member
+};
+
+__device__ __forceinline__ AxisLane axis_lane(int j0)
+{
+    const int lane = (int)(threadIdx.x & 63);
+    const int wave = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    AxisLane a;
+    a.axis = lane % 3;
+    a.base = lane - a.axis;
+    a.point = (lane < 63) ? j0 + wave * 21 + lane / 3 : -1;
+    return a;
+}
+
+// stride, in points, between the groups a grid visits per sweep
+__device__ __forceinline__ int axis_lane_stride() { return (int)(gridDim.x * (blockDim.x >> 6)) * 21; }
+
+// launch 2 of 2: the interface list, then the deferred bulk points (no
+// subcell distances: spacing distances and the bulk pseudo-time step)
+__global__ void __launch_bounds__(256) k_reinit_iface(double* __restrict__ out, const double* __restrict__ fs,
+                                                      const double* __restrict__ fn,
+                                                      const uint8_t* __restrict__ flags, IList il, Box b, Inv inv,
+                                                      int stage, const int* __restrict__ over,
+                                                      const int* __restrict__ over_count, int seq)
+{
+    const int n1 = *il.count, n2 = over_count[seq & 1];
+    const int total = n1 + n2;
+    const double sp[3] = {b.dx, b.dy, b.dz};
+    for (int j0 = 0; j0 < total; j0 += axis_lane_stride()) {
+        const AxisLane al = axis_lane(j0);
+        const bool active = al.point >= 0 && al.point < total;
+        const bool listed = al.point < n1;
+        double m = 0.0;  // this axis's Godunov term
+        double dt = inv.dt_bulk;
+        idx_t ind = 0;
+        bool inside = false;
+        if (active) {
+            ind = (idx_t)(listed ? il.idx[al.point] : over[al.point - n1]);
+            int r, c, p;
+            coords_of(b, ind, r, c, p);
+            inside = (flags[ind] & FLAG_MASK) != 0;
+            const int a = al.axis;
+            double r1 = sp[a], l1 = sp[a];
+            if (listed) {
+                dt = il.dt[al.point];
+                r1 = il.dist[(size_t)(2 * a) * il.cap + al.point];
+                l1 = il.dist[(size_t)(2 * a + 1) * il.cap + al.point];
+            }
+            if (dt >= 0) {
+                double v[7], fo, bk;
+                line7(fs, r, c, p, a, b, v);
+                reinit_axis(fo, bk, v, r1, l1, sp[a], inv.i[a]);
+                m = inside ? dmax(sq(dmin(0.0, bk)), sq(dmax(0.0, fo))) : dmax(sq(dmax(0.0, bk)), sq(dmin(0.0, fo)));
+            }
+        }
+        const double my = __shfl(m, al.base + 1, 64), mz = __shfl(m, al.base + 2, 64);
+        if (active && al.axis == 0) {
+            double step = 0.0;  // frozen point (dt < 0)
+            if (dt >= 0) {
+                // godunov_step: (sqrt((mx + my) + mz) - 1) * dt, negated inside
+                const double s1 = (sqrt((m + my) + mz) - 1) * dt;
+                step = inside ? -s1 : s1;
+            }
+            out[ind] = rk_combine(stage, fs[ind], step, (stage == 0) ? 0.0 : fn[ind]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// interface lists: per-point data in list order, so the interface kernels
+// read it coalesced
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_list_gather(IListW il, Dist6c d, Box b)
+{
+    const int cnt = *il.count;
+    for (int j = (int)(blockIdx.x * 256 + threadIdx.x); j < cnt; j += (int)(gridDim.x * 256)) {
+        const idx_t ind = (idx_t)il.idx[j];
+        Dists q;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            q.r[a] = d.r[a][ind];
+            q.l[a] = d.l[a][ind];
+            il.dist[(size_t)(2 * a) * il.cap + j] = q.r[a];
+            il.dist[(size_t)(2 * a + 1) * il.cap + j] = q.l[a];
+        }
+        il.dt[j] = frozen(q, b.dx) ? -1.0 : 0.3 * min6(q);
+    }
+}
+
+// boundary_interpolant (boundary_interpolant.cu:102-141) at the listed points
+// only (it is zero elsewhere), one lane per (point, axis), in list order:
+// cp[2a][j] = fore value, cp[2a+1][j] = back value
+__global__ void __launch_bounds__(256) k_list_interp(double* __restrict__ cp, IList il, const double* __restrict__ C,
+                                                     Box b)
+{
+    const int total = *il.count;
+    const double sp[3] = {b.dx, b.dy, b.dz};
+    for (int j0 = 0; j0 < total; j0 += axis_lane_stride()) {
+        const AxisLane al = axis_lane(j0);
+        if (!(al.point >= 0 && al.point < total)) continue;
+        const int a = al.axis, j = al.point;
+        const double rf = il.dist[(size_t)(2 * a) * il.cap + j], lb = il.dist[(size_t)(2 * a + 1) * il.cap + j];
+        Pair q{0.0, 0.0};
+        if (rf < sp[a] || lb < sp[a]) {
+            int r, c, p;
+            coords_of(b, (idx_t)il.idx[j], r, c, p);
+            double v[7];
+            line7(C, r, c, p, a, b, v);
+            q = sixth_interp(rf, lb, sp[a], v[0], v[1], v[2], v[3], v[4], v[5], v[6]);
+        }
+        cp[(size_t)(2 * a) * il.cap + j] = q.fore;
+        cp[(size_t)(2 * a + 1) * il.cap + j] = q.back;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_list_gather_vel(double* __restrict__ lv, IList il, const double* __restrict__ vx,
+                                                         const double* __restrict__ vy, const double* __restrict__ vz)
+{
+    const int cnt = *il.count;
+    for (int j = (int)(blockIdx.x * 256 + threadIdx.x); j < cnt; j += (int)(gridDim.x * 256)) {
+        const idx_t ind = (idx_t)il.idx[j];
+        lv[j] = vx[ind];
+        lv[(size_t)il.cap + j] = vy[ind];
+        lv[(size_t)2 * il.cap + j] = vz[ind];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -256,10 +489,13 @@ __global__ void __launch_bounds__(256) k_eno2_stage(double* __restrict__ out, co
 __device__ __forceinline__ void extend_axis(double& fore, double& back, const double v[7], double r1, double l1,
                                             double ds, double inv_ds, double h_fore, double h_back)
 {
-    if (r1 < ds || l1 < ds)
-        weno_nonuniform_pair(fore, back, v, r1, l1, ds, h_fore, h_back);
-    else
+    if (r1 < ds || l1 < ds) {
+        const Pair q = weno_nonuniform_pair(v, r1, l1, ds, h_fore, h_back);
+        fore = q.fore;
+        back = q.back;
+    } else {
         weno5_pair(fore, back, v, inv_ds);
+    }
 }
 
 // upwind_normal: upwind_normal.cu:4-104
@@ -272,7 +508,7 @@ __global__ void __launch_bounds__(256) k_upwind_normal(double* __restrict__ nx, 
     const idx_t ind = lin(b, r, c, p);
     const double sp[3] = {b.dx, b.dy, b.dz};
     double* out[3] = {nx, ny, nz};
-#pragma unroll 1
+#pragma unroll
     for (int a = 0; a < 3; ++a) {
         double v[7], fo, bk;
         line7(lsf, r, c, p, a, b, v);
@@ -303,17 +539,17 @@ __global__ void __launch_bounds__(256) k_boundary_interpolant(Dist6 cp, Dist6c d
     if (!point_of_thread(b, r, c, p)) return;
     const idx_t ind = lin(b, r, c, p);
     const double sp[3] = {b.dx, b.dy, b.dz};
-#pragma unroll 1
+#pragma unroll
     for (int a = 0; a < 3; ++a) {
         const double rf = d.r[a][ind], lb = d.l[a][ind];
-        double cf = 0.0, cb = 0.0;
+        Pair q{0.0, 0.0};
         if (rf < sp[a] || lb < sp[a]) {
             double v[7];
             line7(C, r, c, p, a, b, v);
-            sixth_interp(cf, cb, rf, lb, sp[a], v);
+            q = sixth_interp(rf, lb, sp[a], v[0], v[1], v[2], v[3], v[4], v[5], v[6]);
         }
-        cp.r[a][ind] = cf;
-        cp.l[a][ind] = cb;
+        cp.r[a][ind] = q.fore;
+        cp.l[a][ind] = q.back;
     }
 }
 
@@ -323,7 +559,7 @@ __device__ __forceinline__ double extend_rate(const double* __restrict__ C, int 
 {
     const double sp[3] = {b.dx, b.dy, b.dz};
     double acc = 0.0;
-#pragma unroll 1
+#pragma unroll
     for (int a = 0; a < 3; ++a) {
         double v[7], fo, bk;
         line7(C, r, c, p, a, b, v);
@@ -354,32 +590,73 @@ __global__ void __launch_bounds__(256) k_extend_step(double* __restrict__ step, 
     step[ind] = extend_rate(C, r, c, p, ind, b, q, vel, cp, true) * deltat[ind];
 }
 
-// fused WENO5-RK3 extension stage
-__global__ void __launch_bounds__(256) k_extend_stage(double* __restrict__ out, const double* __restrict__ cs,
-                                                      const double* __restrict__ cn, const double* __restrict__ vx,
-                                                      const double* __restrict__ vy, const double* __restrict__ vz,
-                                                      const uint8_t* __restrict__ flags, Dist6c d, Dist6c cp,
-                                                      Box b, int stage)
+// fused WENO5-RK3 extension stage, bulk: every point without subcell distances
+__global__ void __launch_bounds__(256) k_extend_bulk(double* __restrict__ out, const double* __restrict__ cs,
+                                                     const double* __restrict__ cn, const double* __restrict__ vx,
+                                                     const double* __restrict__ vy, const double* __restrict__ vz,
+                                                     const uint8_t* __restrict__ flags, Box b, Inv inv, int stage)
 {
     int r, c, p;
-    if (!point_of_thread(b, r, c, p)) return;
+    if (!bulk_point(b, (int)blockIdx.x, r, c, p)) return;
     const idx_t ind = lin(b, r, c, p);
-    const bool sub = (flags[ind] & FLAG_SUBCELL) != 0;
-    Dists q;
-    if (sub) {
+    if (flags[ind] & FLAG_SUBCELL) return;
+    const double vel[3] = {vx[ind], vy[ind], vz[ind]};
+    double acc = 0.0;
 #pragma unroll
-        for (int a = 0; a < 3; ++a) { q.r[a] = d.r[a][ind]; q.l[a] = d.l[a][ind]; }
-    } else {
-        q.r[0] = q.l[0] = b.dx; q.r[1] = q.l[1] = b.dy; q.r[2] = q.l[2] = b.dz;
+    for (int a = 0; a < 3; ++a) {
+        double v[7];
+        line7(cs, r, c, p, a, b, v);
+        acc = upwind_axis(acc, vel[a], v, inv.i[a]);
     }
-    const double csv = cs[ind];
-    const double cnv = (stage == 0) ? 0.0 : cn[ind];
-    double step = 0.0;
-    if (!(sub && frozen(q, b.dx))) {
-        const double vel[3] = {vx[ind], vy[ind], vz[ind]};
-        step = extend_rate(cs, r, c, p, ind, b, q, vel, cp, sub) * (0.3 * min6(q));
+    out[ind] = rk_combine(stage, cs[ind], acc * inv.dt_bulk, (stage == 0) ? 0.0 : cn[ind]);
+}
+
+// fused WENO5-RK3 extension stage, interface list: three lanes per point
+// (see AxisLane); each lane evaluates the one upwind derivative its axis
+// needs (see upwind_axis), on the nonuniform stencil where the axis crosses.
+// lcp, lv: interface values and velocity in list order.
+__global__ void __launch_bounds__(256) k_extend_iface(double* __restrict__ out, const double* __restrict__ cs,
+                                                      const double* __restrict__ cn, IList il,
+                                                      const double* __restrict__ lcp, const double* __restrict__ lv,
+                                                      Box b, Inv inv, int stage)
+{
+    const int total = *il.count;
+    const double sp[3] = {b.dx, b.dy, b.dz};
+    for (int j0 = 0; j0 < total; j0 += axis_lane_stride()) {
+        const AxisLane al = axis_lane(j0);
+        const bool active = al.point >= 0 && al.point < total;
+        double t = 0.0;  // this axis's term coef * derivative
+        double dt = -1.0;
+        idx_t ind = 0;
+        if (active) {
+            const int j = al.point, a = al.axis;
+            ind = (idx_t)il.idx[j];
+            dt = il.dt[j];
+            if (dt >= 0) {
+                int r, c, p;
+                coords_of(b, ind, r, c, p);
+                double v[7];
+                line7(cs, r, c, p, a, b, v);
+                const double vel = lv[(size_t)a * il.cap + j];
+                const double r1 = il.dist[(size_t)(2 * a) * il.cap + j];
+                const double l1 = il.dist[(size_t)(2 * a + 1) * il.cap + j];
+                if (r1 < sp[a] || l1 < sp[a]) {
+                    const bool fore = vel < 0;
+                    const double dv = weno_nonuniform_one(v[0], v[1], v[2], v[3], v[4], v[5], v[6], r1, l1, sp[a],
+                                                          lcp[(size_t)(2 * a) * il.cap + j],
+                                                          lcp[(size_t)(2 * a + 1) * il.cap + j], !fore);
+                    t = 0.0 + (fore ? vel : dmax(0.0, vel)) * dv;
+                } else {
+                    t = upwind_axis(0.0, vel, v, inv.i[a]);
+                }
+            }
+        }
+        const double ty = __shfl(t, al.base + 1, 64), tz = __shfl(t, al.base + 2, 64);
+        if (active && al.axis == 0) {
+            const double step = (dt >= 0) ? ((t + ty) + tz) * dt : 0.0;
+            out[ind] = rk_combine(stage, cs[ind], step, (stage == 0) ? 0.0 : cn[ind]);
+        }
     }
-    out[ind] = rk_combine(stage, csv, step, cnv);
 }
 
 // ---------------------------------------------------------------------------
@@ -387,17 +664,14 @@ __global__ void __launch_bounds__(256) k_extend_stage(double* __restrict__ out, 
 // its six-output twin (5_0_Nonoscillating_Reconstruction/nonoscillating_interpolant.cu:59-122)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double advect_rate(const double* __restrict__ f, int r, int c, int p, const Box& b,
-                                              const double vel[3])
+                                              const Inv& inv, const double vel[3])
 {
-    const double sp[3] = {b.dx, b.dy, b.dz};
     double acc = 0.0;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        double v[7], fo, bk;
+        double v[7];
         line7(f, r, c, p, a, b, v);
-        weno5_pair(fo, bk, v, 1.0 / sp[a]);
-        acc = acc + dmin(0.0, vel[a]) * fo;
-        acc = acc + dmax(0.0, vel[a]) * bk;
+        acc = upwind_axis(acc, vel[a], v, inv.i[a]);
     }
     return acc;
 }
@@ -405,26 +679,26 @@ __device__ __forceinline__ double advect_rate(const double* __restrict__ f, int 
 __global__ void __launch_bounds__(256) k_advection_step(double* __restrict__ astep, const double* __restrict__ vx,
                                                         const double* __restrict__ vy,
                                                         const double* __restrict__ vz,
-                                                        const double* __restrict__ lsf, Box b)
+                                                        const double* __restrict__ lsf, Box b, Inv inv)
 {
     int r, c, p;
     if (!point_of_thread(b, r, c, p)) return;
     const idx_t ind = lin(b, r, c, p);
     const double vel[3] = {vx[ind], vy[ind], vz[ind]};
-    astep[ind] = advect_rate(lsf, r, c, p, b, vel);
+    astep[ind] = advect_rate(lsf, r, c, p, b, inv, vel);
 }
 
 // fused TVD-RK3 advection stage: out = combine(fs - dt*L(fs), fn)
 __global__ void __launch_bounds__(256) k_advect_stage(double* __restrict__ out, const double* __restrict__ fs,
                                                       const double* __restrict__ fn, const double* __restrict__ vx,
                                                       const double* __restrict__ vy, const double* __restrict__ vz,
-                                                      double dt, Box b, int stage)
+                                                      double dt, Box b, Inv inv, int stage)
 {
     int r, c, p;
     if (!point_of_thread(b, r, c, p)) return;
     const idx_t ind = lin(b, r, c, p);
     const double vel[3] = {vx[ind], vy[ind], vz[ind]};
-    const double a = advect_rate(fs, r, c, p, b, vel);
+    const double a = advect_rate(fs, r, c, p, b, inv, vel);
     const double fnv = (stage == 0) ? 0.0 : fn[ind];
     double step;
     {
@@ -773,6 +1047,12 @@ namespace vls {
 
 static inline Box box_of(const Geom& g) { return Box{g.rows, g.cols, g.pges, g.dx, g.dy, g.dz}; }
 
+static inline Inv inv_of(const Geom& g)
+{
+    const double m = std::min(std::min(g.dx, g.dx), std::min(std::min(g.dy, g.dy), std::min(g.dz, g.dz)));
+    return Inv{{1.0 / g.dx, 1.0 / g.dy, 1.0 / g.dz}, 0.3 * m};
+}
+
 // one thread per point: 64 rows x 4 cols per block, one z-page per grid.z
 static inline void point_grid(const Geom& g, dim3& grid, dim3& block)
 {
@@ -790,9 +1070,9 @@ static inline dim3 flat_grid(int64_t n) { return dim3((unsigned)((n + 255) / 256
         return hipGetLastError();                                       \
     } while (0)
 
-hipError_t launch_fill_spacing(const Dist6& d, const Geom& g, hipStream_t s)
+hipError_t launch_fill_spacing(const Dist6& d, const Geom& g, int* list_count, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_fill_spacing, flat_grid(g.n()), dim3(256), 0, s, d, box_of(g), (idx_t)g.n());
+    hipLaunchKernelGGL(k_fill_spacing, flat_grid(g.n()), dim3(256), 0, s, d, box_of(g), (idx_t)g.n(), list_count);
     return hipGetLastError();
 }
 
@@ -803,11 +1083,32 @@ hipError_t launch_boundary_location(int order, const Dist6& d, const double* lsf
 }
 
 hipError_t launch_min_dist_flags(double* min_dist, uint8_t* flags, const Dist6c& d, const double* mask_src,
-                                 const Geom& g, hipStream_t s)
+                                 const Geom& g, int* list, int* list_count, hipStream_t s)
 {
     hipLaunchKernelGGL(k_min_dist_flags, flat_grid(g.n()), dim3(256), 0, s, min_dist, flags, d, mask_src,
-                       box_of(g), (idx_t)g.n());
+                       box_of(g), (idx_t)g.n(), list, list_count);
     return hipGetLastError();
+}
+
+hipError_t launch_zero_ints(int* a, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, a, n);
+    return hipGetLastError();
+}
+
+// blocks of an interface-list launch (84 points per block of 256, three lanes
+// per point): one sweep covers lists up to 1/48 of the box (a closed surface
+// in a box of n^3 points lists O(n^2) of them); longer lists are covered by
+// striding
+int iface_blocks(const Geom& g)
+{
+    const int64_t b = g.n() / (84 * 48);
+    return (int)(b < 16 ? 16 : (b > 4096 ? 4096 : b));
+}
+
+static inline int bulk_blocks(const Geom& g)
+{
+    return ((g.rows + 63) / 64) * ((g.cols + 3) / 4) * g.pges;
 }
 
 hipError_t launch_set_mask_bit(uint8_t* flags, const double* F, const Geom& g, hipStream_t s)
@@ -828,10 +1129,39 @@ hipError_t launch_eno2_re_step(double* step, const double* lsf, const uint8_t* m
     VLS_POINT_LAUNCH(k_eno2_re_step, step, lsf, mask, deltat, d, box_of(g));
 }
 
+// bulk, then the interface list and the deferred bulk points, in stream order
 hipError_t launch_reinit_stage(double* out, const double* fs, const double* fn, const uint8_t* flags,
-                               const Dist6c& d, const Geom& g, int stage, hipStream_t s)
+                               const ListView& lv, const Geom& g, int stage, int* over, int* over_count, int seq,
+                               hipStream_t s)
 {
-    VLS_POINT_LAUNCH(k_reinit_stage, out, fs, fn, flags, d, box_of(g), stage);
+    hipLaunchKernelGGL(k_reinit_bulk, dim3(bulk_blocks(g)), dim3(256), 0, s, out, fs, fn, flags, box_of(g), inv_of(g),
+                       stage, over, over_count, seq);
+    const IList il{lv.idx, lv.count, lv.dist, lv.dt, lv.cap};
+    hipLaunchKernelGGL(k_reinit_iface, dim3(iface_blocks(g)), dim3(256), 0, s, out, fs, fn, flags, il, box_of(g),
+                       inv_of(g), stage, (const int*)over, (const int*)over_count, seq);
+    return hipGetLastError();
+}
+
+hipError_t launch_list_gather(const ListView& lv, const Dist6c& d, const Geom& g, hipStream_t s)
+{
+    const IListW il{lv.idx, lv.count, lv.dist, lv.dt, lv.cap};
+    hipLaunchKernelGGL(k_list_gather, dim3(iface_blocks(g)), dim3(256), 0, s, il, d, box_of(g));
+    return hipGetLastError();
+}
+
+hipError_t launch_list_interp(double* lcp, const ListView& lv, const double* C, const Geom& g, hipStream_t s)
+{
+    const IList il{lv.idx, lv.count, lv.dist, lv.dt, lv.cap};
+    hipLaunchKernelGGL(k_list_interp, dim3(iface_blocks(g)), dim3(256), 0, s, lcp, il, C, box_of(g));
+    return hipGetLastError();
+}
+
+hipError_t launch_list_gather_vel(double* lvel, const ListView& lv, const double* vx, const double* vy,
+                                  const double* vz, const Geom& g, hipStream_t s)
+{
+    const IList il{lv.idx, lv.count, lv.dist, lv.dt, lv.cap};
+    hipLaunchKernelGGL(k_list_gather_vel, dim3(iface_blocks(g)), dim3(256), 0, s, lvel, il, vx, vy, vz);
+    return hipGetLastError();
 }
 
 hipError_t launch_eno2_stage(double* out, const double* fs, const double* fn, const uint8_t* flags,
@@ -868,23 +1198,28 @@ hipError_t launch_extend_step(double* step, const double* deltat, const double* 
 }
 
 hipError_t launch_extend_stage(double* out, const double* cs, const double* cn, const double* vx,
-                               const double* vy, const double* vz, const uint8_t* flags, const Dist6c& d,
-                               const Dist6c& cp, const Geom& g, int stage, hipStream_t s)
+                               const double* vy, const double* vz, const uint8_t* flags, const ListView& lv,
+                               const double* lcp, const double* lvel, const Geom& g, int stage, hipStream_t s)
 {
-    VLS_POINT_LAUNCH(k_extend_stage, out, cs, cn, vx, vy, vz, flags, d, cp, box_of(g), stage);
+    hipLaunchKernelGGL(k_extend_bulk, dim3(bulk_blocks(g)), dim3(256), 0, s, out, cs, cn, vx, vy, vz, flags, box_of(g),
+                       inv_of(g), stage);
+    const IList il{lv.idx, lv.count, lv.dist, lv.dt, lv.cap};
+    hipLaunchKernelGGL(k_extend_iface, dim3(iface_blocks(g)), dim3(256), 0, s, out, cs, cn, il, lcp, lvel, box_of(g),
+                       inv_of(g), stage);
+    return hipGetLastError();
 }
 
 hipError_t launch_advection_step(double* astep, const double* vx, const double* vy, const double* vz,
                                  const double* lsf, const Geom& g, hipStream_t s)
 {
-    VLS_POINT_LAUNCH(k_advection_step, astep, vx, vy, vz, lsf, box_of(g));
+    VLS_POINT_LAUNCH(k_advection_step, astep, vx, vy, vz, lsf, box_of(g), inv_of(g));
 }
 
 hipError_t launch_advect_stage(double* out, const double* fs, const double* fn, const double* vx,
                                const double* vy, const double* vz, double dt, const Geom& g, int stage,
                                hipStream_t s)
 {
-    VLS_POINT_LAUNCH(k_advect_stage, out, fs, fn, vx, vy, vz, dt, box_of(g), stage);
+    VLS_POINT_LAUNCH(k_advect_stage, out, fs, fn, vx, vy, vz, dt, box_of(g), inv_of(g), stage);
 }
 
 hipError_t launch_normal_velocity(double* out, const double* F, const double* vx, const double* vy, const double* vz,

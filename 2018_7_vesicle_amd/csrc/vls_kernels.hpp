@@ -35,23 +35,51 @@ inline Dist6c constify(const Dist6& d)
 }
 
 // ---- host launchers (all asynchronous on `s`) -------------------------------
+// an interface list: point indices (count on the device) and, in list order,
+// the six distances (rows of `cap` doubles: xpr,xpl,ypf,ypb,zpu,zpd) and the
+// pseudo-time step (0.3*min_dist, -1 when frozen)
+struct ListView {
+    int* idx;
+    int* count;
+    double* dist;
+    double* dt;
+    int cap;
+};
+
+
 struct Geom {
     int rows, cols, pges;
     double dx, dy, dz;
     int64_t n() const { return (int64_t)rows * cols * pges; }
 };
 
-hipError_t launch_fill_spacing(const Dist6& d, const Geom& g, hipStream_t s);
+// pre-fill with the spacings; also zeroes *list_count (may be null)
+hipError_t launch_fill_spacing(const Dist6& d, const Geom& g, int* list_count, hipStream_t s);
 hipError_t launch_boundary_location(int order, const Dist6& d, const double* lsf, const Geom& g, hipStream_t s);
+// min_dist (may be null), flags (may be null) and the list of points with a
+// subcell distance (list may be null; *list_count must be 0 on entry)
 hipError_t launch_min_dist_flags(double* min_dist, uint8_t* flags, const Dist6c& d, const double* mask_src,
-                                 const Geom& g, hipStream_t s);
+                                 const Geom& g, int* list, int* list_count, hipStream_t s);
+hipError_t launch_zero_ints(int* a, int n, hipStream_t s);
+int iface_blocks(const Geom& g);
 hipError_t launch_set_mask_bit(uint8_t* flags, const double* F, const Geom& g, hipStream_t s);
 hipError_t launch_re_step(double* step, const double* lsf, const uint8_t* mask, const double* deltat,
                           const Dist6c& d, const Geom& g, hipStream_t s);
 hipError_t launch_eno2_re_step(double* step, const double* lsf, const uint8_t* mask, const double* deltat,
                                const Dist6c& d, const Geom& g, hipStream_t s);
+// one WENO5-RK3 reinitialization stage (two launches); `seq` numbers the
+// stages of one reinitialization call from 0, over_count holds 2 ints that
+// are zero before seq 0
 hipError_t launch_reinit_stage(double* out, const double* fs, const double* fn, const uint8_t* flags,
-                               const Dist6c& d, const Geom& g, int stage, hipStream_t s);
+                               const ListView& lv, const Geom& g, int stage, int* over, int* over_count, int seq,
+                               hipStream_t s);
+// fill the list-ordered distances and steps of `lv` from the full arrays
+hipError_t launch_list_gather(const ListView& lv, const Dist6c& d, const Geom& g, hipStream_t s);
+// interface values of C at the listed points, list order (6 rows of cap)
+hipError_t launch_list_interp(double* lcp, const ListView& lv, const double* C, const Geom& g, hipStream_t s);
+// velocity at the listed points, list order (3 rows of cap)
+hipError_t launch_list_gather_vel(double* lvel, const ListView& lv, const double* vx, const double* vy,
+                                  const double* vz, const Geom& g, hipStream_t s);
 hipError_t launch_eno2_stage(double* out, const double* fs, const double* fn, const uint8_t* flags,
                              const Dist6c& d, const Geom& g, double cfl, int heun2, hipStream_t s);
 hipError_t launch_upwind_normal(double* nx, double* ny, double* nz, const double* lsf, const Dist6c& d,
@@ -64,8 +92,8 @@ hipError_t launch_extend_step(double* step, const double* deltat, const double* 
                               const double* vy, const double* vz, const Dist6c& d, const Dist6c& cp,
                               const Geom& g, hipStream_t s);
 hipError_t launch_extend_stage(double* out, const double* cs, const double* cn, const double* vx,
-                               const double* vy, const double* vz, const uint8_t* flags, const Dist6c& d,
-                               const Dist6c& cp, const Geom& g, int stage, hipStream_t s);
+                               const double* vy, const double* vz, const uint8_t* flags, const ListView& lv,
+                               const double* lcp, const double* lvel, const Geom& g, int stage, hipStream_t s);
 hipError_t launch_advection_step(double* astep, const double* vx, const double* vy, const double* vz,
                                  const double* lsf, const Geom& g, hipStream_t s);
 hipError_t launch_advect_stage(double* out, const double* fs, const double* fn, const double* vx,
