@@ -12,15 +12,19 @@ ellipsoid (0.8,0.5,0.5), seeded 16-mode curl velocity, CFL 0.4 (see
 value = WENO grid-point updates per second: (operator-stage passes counted by
 pipeline.updates_per_step) x points / wall time of the K timed steps.
 
-roofline: the dominant kernel is the fused WENO5-RK3 extension stage
-(k_extend_stage).  Its average duration is measured with HIP events around the
+roofline: the dominant kernel is the fused WENO5-RK3 extension stage (a bulk
+launch and an interface-list launch).  Its average duration is measured with HIP events around the
 extension calls inside the timed steps, minus the extension's per-call set-up
 (upwind normal, velocity, interface values: timed separately with 0
 iterations), divided by the stage launches.  achieved = 37.3 B/point (the
 algorithmic bytes of one FP64 extension stage) x points / that duration.
 
-cpu_baseline: the CPU oracle (OpenMP, FP64) on a bounded sample of the same
-workload, timed on this host.
+traffic: HBM bytes per stage from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs
+of `bench.py --probe extend`, calibrated on a 256 MiB copy
+(profiles/extend_stage_128_traffic.txt).
+
+cpu_baseline: the CPU oracle (OpenMP, FP64) on one full pipeline step of the
+same workload, timed on this host.
 """
 import argparse
 import json
@@ -42,8 +46,34 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-ext-iters", type=int, default=10)
+    ap.add_argument("--probe", choices=["extend"], default=None,
+                    help="run only a few extension stages and a 256 MiB device copy (for rocprofv3 --pmc)")
     return ap.parse_args()
+
+
+def probe_extend(args):
+    """A short run for hardware counters: 3 iterations (9 stage launches) of the
+    128^3 extension, then a plain 256 MiB float64 device-to-device copy whose
+    counter readings calibrate FETCH_SIZE/WRITE_SIZE (2^25 doubles read and
+    written)."""
+    import importlib
+
+    import torch
+
+    vls = importlib.import_module("2018_7_vesicle_amd")
+    pl = vls.pipeline
+    cfg = pl.PipelineConfig(n=args.n, initial_reinit=10)
+    p = pl.Pipeline(cfg)
+    p.run(1)
+    torch.cuda.synchronize()
+    p.map.setDistance()
+    p.map.WENO5RK3Extend(p.Vn, 3)
+    src = torch.randn(1 << 25, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"probe": "extend", "n": args.n, "stage_launches": 9, "copy_bytes": 8 << 25}), flush=True)
+    return 0
 
 
 def traffic_from_profiles():
@@ -58,45 +88,44 @@ def traffic_from_profiles():
         return None
 
 
-def cpu_baseline(cfg, ext_iters):
-    """one non-reinitializing pipeline step with `ext_iters` extension iterations,
-    run by the CPU oracle on all host threads"""
+def cpu_baseline(cfg):
+    """The CPU oracle (OpenMP, FP64, all host threads) on one full pipeline step
+    of the same workload: a step that also reinitializes and redistances
+    (advection of F and psi, setDistance, normal velocity, extension,
+    reinitialization, surface redistance, area/volume), timed as the GPU's."""
+    import importlib
+
     import numpy as np
 
     from oracle import pyoracle as po
-    import importlib
 
     pl = importlib.import_module("2018_7_vesicle_amd.pipeline")
     xv, F0, psi0, u = pl.make_inputs(cfg)
     dx = float(xv[1] - xv[0])
-    sp = (dx, dx, dx)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     po.set_num_threads(threads)
     umax = max(float(np.abs(c).max()) for c in u)
     dt = cfg.cfl * dx / umax
-    F = np.ascontiguousarray(F0)
+    step_index = cfg.reinit_every - 1
     t0 = time.perf_counter()
-    F = po.advect_rk3(F, 1, dt, u, *sp)
-    psi = po.advect_rk3(psi0, 1, dt, u, *sp)
-    d6, md = po.set_distance(F, *sp)
-    Vn = po.normal_velocity(F, u, *sp)
-    Vn = po.weno5_rk3_extend(Vn, ext_iters, F, d6, md, *sp)
+    po.pipeline(F0, psi0, u, dt, cfg.eps_factor * dx, (dx, dx, dx), 1, cfg.ext_iters, cfg.reinit_every,
+                cfg.reinit_iters, cfg.redist_iters, 0, first_step=step_index)
     t1 = time.perf_counter()
-    del psi, Vn
-    n = F.size
-    updates = (6 + 1 + 3 * ext_iters) * n
+    passes = pl.updates_per_step(cfg, step_index)
     return {
-        "value": updates / (t1 - t0),
+        "value": passes * F0.size / (t1 - t0),
         "unit": "grid-pt updates/s",
         "cores": po.num_threads(),
         "kind": "port",
-        "sample": f"one pipeline step without reinitialization at {cfg.n}^3 FP64 with {ext_iters} extension "
-                  f"iterations ({6 + 1 + 3 * ext_iters} WENO stage passes, {t1 - t0:.1f} s)",
+        "sample": f"one full pipeline step (with reinitialization and surface redistance) at {cfg.n}^3 FP64: "
+                  f"{passes} WENO stage passes in {t1 - t0:.1f} s",
     }
 
 
 def main():
     args = parse()
+    if args.probe == "extend":
+        return probe_extend(args)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported by this build "
               "(the multi-GPU z-slab decomposition is a design note, not code)", file=sys.stderr)
@@ -158,7 +187,7 @@ def main():
             "updates_per_step_avg": updates / args.steps / n,
         },
         "roofline": {
-            "kernel": "k_extend_stage (fused WENO5-RK3 extension stage, FP64)",
+            "kernel": "FP64 WENO5-RK3 extension stage (k_extend_bulk + k_extend_iface launches)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
@@ -175,7 +204,7 @@ def main():
         },
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_ext_iters)
+        out["cpu_baseline"] = cpu_baseline(cfg)
     print(json.dumps(out), flush=True)
     return 0
 
