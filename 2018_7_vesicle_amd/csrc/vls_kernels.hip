@@ -261,6 +261,90 @@ __device__ __forceinline__ void load_dists(Dists& q, const Dist6c& d, idx_t ind)
     for (int a = 0; a < 3; ++a) { q.r[a] = d.r[a][ind]; q.l[a] = d.l[a][ind]; }
 }
 
+// ---------------------------------------------------------------------------
+// plane tiles: a block covers 64 x 4 (rows x cols) points of one page.  The
+// page's values with a 3-point halo in rows and cols are staged in LDS once
+// (clamped there), so the x- and y-taps of the WENO stencil are LDS reads at
+// constant offsets; the z-taps are global loads at a wave-uniform page offset.
+// ---------------------------------------------------------------------------
+constexpr int TR = 64, TC = 4, HALO = 3, LR = TR + 2 * HALO, LC = TC + 2 * HALO;
+
+struct Tile {
+    int r0, c0, p;  // tile origin (row, col) and page
+    int r, c;       // this thread's point
+    bool valid;
+    int base;       // LDS index of this thread's point
+};
+
+// Blocks are dispatched round-robin over the 8 XCDs (block i to XCD i % 8),
+// each with its own L2.  Give every XCD a contiguous range of tiles instead,
+// so neighbouring tiles share their halos in one L2.
+__device__ __forceinline__ int xcd_remap(int blk, int nblk)
+{
+    if ((nblk & 7) != 0) return blk;
+    return (blk & 7) * (nblk >> 3) + (blk >> 3);
+}
+
+__device__ __forceinline__ Tile tile_of(const Box& b)
+{
+    const int ntr = (b.rows + TR - 1) / TR, ntc = (b.cols + TC - 1) / TC;
+    const int blk = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    Tile t;
+    const int rb = blk % ntr;
+    const int q = blk / ntr;
+    const int cb = q % ntc;
+    t.p = q / ntc;
+    t.r0 = rb * TR;
+    t.c0 = cb * TC;
+    const int ty = (int)(threadIdx.x & 63), tx = (int)(threadIdx.x >> 6);
+    t.r = t.r0 + ty;
+    t.c = t.c0 + tx;
+    t.valid = t.r < b.rows && t.c < b.cols;
+    t.base = (tx + HALO) * LR + ty + HALO;
+    return t;
+}
+
+// stage the tile of page t.p of f (with clamped halo) in lds[LC*LR]; ends with a barrier
+__device__ __forceinline__ void fill_tile(double* lds, const double* __restrict__ f, const Box& b, const Tile& t)
+{
+    const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
+    const double* fp = f + (idx_t)t.p * b.rows * b.cols;
+    const int rlo = clampi(t.r0 + lane - HALO, b.rows - 1);
+    const int rhi = clampi(t.r0 + TR + lane - HALO, b.rows - 1);
+    for (int lc = wave; lc < LC; lc += 4) {
+        const double* col = fp + (idx_t)clampi(t.c0 + lc - HALO, b.cols - 1) * b.rows;
+        lds[lc * LR + lane] = col[rlo];
+        if (lane < 2 * HALO) lds[lc * LR + TR + lane] = col[rhi];
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void tile_line_y(const double* lds, int base, double v[7])
+{
+#pragma unroll
+    for (int k = 0; k < 7; ++k) v[k] = lds[base + k - 3];
+}
+
+__device__ __forceinline__ void tile_line_x(const double* lds, int base, double v[7])
+{
+#pragma unroll
+    for (int k = 0; k < 7; ++k) v[k] = lds[base + (k - 3) * LR];
+}
+
+// z-taps: page offsets are the same for the whole block
+__device__ __forceinline__ void tile_line_z(const double* __restrict__ f, const Box& b, const Tile& t, double v[7])
+{
+    const idx_t plane = (idx_t)b.rows * b.cols;
+    const idx_t inplane = (idx_t)t.c * b.rows + t.r;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) v[k] = f[(idx_t)clampi(t.p + k - 3, b.pges - 1) * plane + inplane];
+}
+
+__device__ __forceinline__ int tile_blocks(const Box& b)
+{
+    return ((b.rows + TR - 1) / TR) * ((b.cols + TC - 1) / TC) * b.pges;
+}
+
 // WENO5-RK3 reinitialization stage, launch 1 of 2: the bulk.  A point with
 // subcell distances is left to launch 2; so is a point whose stage field
 // changes sign next to it although it has none (it needs the nonuniform
@@ -271,25 +355,28 @@ __global__ void __launch_bounds__(256) k_reinit_bulk(double* __restrict__ out, c
                                                      Box b, Inv invs, int stage, int* __restrict__ over,
                                                      int* __restrict__ over_count, int seq)
 {
+    __shared__ double lds[LC * LR];
     if (blockIdx.x == 0 && threadIdx.x == 0) over_count[(seq + 1) & 1] = 0;
-    int r, c, p;
-    const bool valid = bulk_point(b, (int)blockIdx.x, r, c, p);
-    const idx_t ind = valid ? lin(b, r, c, p) : 0;
-    const uint8_t fl = valid ? flags[ind] : (uint8_t)FLAG_SUBCELL;
+    const Tile t = tile_of(b);
+    fill_tile(lds, fs, b, t);
+    const idx_t ind = t.valid ? lin(b, t.r, t.c, t.p) : 0;
+    const uint8_t fl = t.valid ? flags[ind] : (uint8_t)FLAG_SUBCELL;
     bool defer = false;
     if (!(fl & FLAG_SUBCELL)) {
         double fo[3], bk[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             double v[7];
-            line7(fs, r, c, p, a, b, v);
+            if (a == 0) tile_line_x(lds, t.base, v);
+            else if (a == 1) tile_line_y(lds, t.base, v);
+            else tile_line_z(fs, b, t, v);
             defer = defer || (v[2] * v[3] < 0 || v[3] * v[4] < 0);
             weno5_pair(fo[a], bk[a], v, invs.i[a]);
         }
         if (!defer) {
             const double step = godunov_step((fl & FLAG_MASK) != 0, fo[0], bk[0], fo[1], bk[1], fo[2], bk[2],
                                              invs.dt_bulk);
-            out[ind] = rk_combine(stage, fs[ind], step, (stage == 0) ? 0.0 : fn[ind]);
+            out[ind] = rk_combine(stage, lds[t.base], step, (stage == 0) ? 0.0 : fn[ind]);
         }
     }
     wave_append(defer, (int)ind, over, &over_count[seq & 1]);
@@ -596,19 +683,22 @@ __global__ void __launch_bounds__(256) k_extend_bulk(double* __restrict__ out, c
                                                      const double* __restrict__ vy, const double* __restrict__ vz,
                                                      const uint8_t* __restrict__ flags, Box b, Inv inv, int stage)
 {
-    int r, c, p;
-    if (!bulk_point(b, (int)blockIdx.x, r, c, p)) return;
-    const idx_t ind = lin(b, r, c, p);
+    __shared__ double lds[LC * LR];
+    const Tile t = tile_of(b);
+    fill_tile(lds, cs, b, t);
+    if (!t.valid) return;
+    const idx_t ind = lin(b, t.r, t.c, t.p);
     if (flags[ind] & FLAG_SUBCELL) return;
     const double vel[3] = {vx[ind], vy[ind], vz[ind]};
     double acc = 0.0;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        double v[7];
-        line7(cs, r, c, p, a, b, v);
-        acc = upwind_axis(acc, vel[a], v, inv.i[a]);
-    }
-    out[ind] = rk_combine(stage, cs[ind], acc * inv.dt_bulk, (stage == 0) ? 0.0 : cn[ind]);
+    double v[7];
+    tile_line_x(lds, t.base, v);
+    acc = upwind_axis(acc, vel[0], v, inv.i[0]);
+    tile_line_y(lds, t.base, v);
+    acc = upwind_axis(acc, vel[1], v, inv.i[1]);
+    tile_line_z(cs, b, t, v);
+    acc = upwind_axis(acc, vel[2], v, inv.i[2]);
+    out[ind] = rk_combine(stage, lds[t.base], acc * inv.dt_bulk, (stage == 0) ? 0.0 : cn[ind]);
 }
 
 // fused WENO5-RK3 extension stage, interface list: three lanes per point
@@ -688,24 +778,33 @@ __global__ void __launch_bounds__(256) k_advection_step(double* __restrict__ ast
     astep[ind] = advect_rate(lsf, r, c, p, b, inv, vel);
 }
 
-// fused TVD-RK3 advection stage: out = combine(fs - dt*L(fs), fn)
+// fused TVD-RK3 advection stage: out = combine(fs - dt*L(fs), fn), plane tiles
 __global__ void __launch_bounds__(256) k_advect_stage(double* __restrict__ out, const double* __restrict__ fs,
                                                       const double* __restrict__ fn, const double* __restrict__ vx,
                                                       const double* __restrict__ vy, const double* __restrict__ vz,
                                                       double dt, Box b, Inv inv, int stage)
 {
-    int r, c, p;
-    if (!point_of_thread(b, r, c, p)) return;
-    const idx_t ind = lin(b, r, c, p);
+    __shared__ double lds[LC * LR];
+    const Tile t = tile_of(b);
+    fill_tile(lds, fs, b, t);
+    if (!t.valid) return;
+    const idx_t ind = lin(b, t.r, t.c, t.p);
     const double vel[3] = {vx[ind], vy[ind], vz[ind]};
-    const double a = advect_rate(fs, r, c, p, b, inv, vel);
+    double acc = 0.0;
+    double v[7];
+    tile_line_x(lds, t.base, v);
+    acc = upwind_axis(acc, vel[0], v, inv.i[0]);
+    tile_line_y(lds, t.base, v);
+    acc = upwind_axis(acc, vel[1], v, inv.i[1]);
+    tile_line_z(fs, b, t, v);
+    acc = upwind_axis(acc, vel[2], v, inv.i[2]);
     const double fnv = (stage == 0) ? 0.0 : fn[ind];
     double step;
     {
 #pragma clang fp contract(off)
-        step = dt * a;
+        step = dt * acc;
     }
-    out[ind] = rk_combine(stage, fs[ind], step, fnv);
+    out[ind] = rk_combine(stage, lds[t.base], step, fnv);
 }
 
 __global__ void __launch_bounds__(256) k_weno_derivative(Dist6 out, const double* __restrict__ lsf, Box b)
@@ -1219,7 +1318,9 @@ hipError_t launch_advect_stage(double* out, const double* fs, const double* fn, 
                                const double* vy, const double* vz, double dt, const Geom& g, int stage,
                                hipStream_t s)
 {
-    VLS_POINT_LAUNCH(k_advect_stage, out, fs, fn, vx, vy, vz, dt, box_of(g), inv_of(g), stage);
+    hipLaunchKernelGGL(k_advect_stage, dim3(bulk_blocks(g)), dim3(256), 0, s, out, fs, fn, vx, vy, vz, dt, box_of(g),
+                       inv_of(g), stage);
+    return hipGetLastError();
 }
 
 hipError_t launch_normal_velocity(double* out, const double* F, const double* vx, const double* vy, const double* vz,
