@@ -122,6 +122,47 @@ def cpu_baseline(cfg):
     }
 
 
+def config1(vls, with_cpu):
+    """Config 1 beside the main line: 64^3 ENO2 forward-Euler reinitialization,
+    CFL 0.5, 100 iterations, on the GPU (and the CPU oracle), with its error
+    measures."""
+    import numpy as np
+    import torch
+
+    diag = __import__("importlib").import_module("2018_7_vesicle_amd.diagnostics")
+    xv = np.linspace(-1.0, 1.0, 64)
+    Z, X, Y = np.meshgrid(xv, xv, xv, indexing="ij")
+    r = np.sqrt(X ** 2 + Y ** 2 + Z ** 2)
+    theta = np.arctan2(np.sqrt(X ** 2 + Y ** 2), Z)
+    varphi = np.arctan2(Y, X)
+    F0 = (r - 0.6) * (1 + 0.3 * np.sin(3 * theta) * np.cos(2 * varphi)) * np.exp(0.5 * X)
+    G = vls.GD3(xv, xv, xv)
+    F0t = torch.from_numpy(np.ascontiguousarray(F0)).cuda()
+    m = vls.SDF3(G, F0t)
+    m.ENORK2Reinitialization(F0t, 100, rk2=False, cfl=0.5)  # warm-up
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    F = m.ENORK2Reinitialization(F0t, 100, rk2=False, cfl=0.5)
+    e1.record()
+    torch.cuda.synchronize()
+    gpu_ms = e0.elapsed_time(e1)
+    e_max, e_mean, _ = diag.grad_error(G, F)
+    disp, ncross = diag.sphere_displacement(G, F, 0.6)
+    out = {"grid": 64, "iterations": 100, "gpu_ms": gpu_ms,
+           "gpu_updates_per_s": 100 * 64 ** 3 / (gpu_ms * 1e-3),
+           "grad_err_max": e_max, "grad_err_mean": e_mean, "displacement_dx": disp / G.Dx, "crossings": ncross}
+    if with_cpu:
+        from oracle import pyoracle as po
+
+        t0 = time.perf_counter()
+        ref = po.eno2_reinit(F0, 100, G.Dx, G.Dy, G.Dz, rk2=False, cfl=0.5)
+        out["cpu_ms"] = (time.perf_counter() - t0) * 1e3
+        out["gpu_equals_cpu_oracle"] = bool(np.array_equal(ref, F.cpu().numpy()))
+    return out
+
+
 def main():
     args = parse()
     if args.probe == "extend":
@@ -203,6 +244,7 @@ def main():
             "dt": p.dt,
         },
     }
+    out["config1"] = config1(vls, not args.no_cpu)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg)
     print(json.dumps(out), flush=True)
