@@ -46,6 +46,14 @@ SIGNATURES = {
     "vls_sdf3_area_volume_cosine": [P, P, D, PD, PD, P],
     "vls_sdf3_area_volume_device": [P, P, I, D, P, P],
     "vls_sdf3_normal_velocity": [P] * 7,
+    "vls_sdf3_set_distance_f32": [P, P, P],
+    "vls_sdf3_weno5_rk3_reinit_f32": [P, P, I, P],
+    "vls_sdf3_weno5_rk3_extend_f32": [P, P, P, I, P],
+    "vls_sdf3_surface_redistance_f32": [P, P, P, I, I, P],
+    "vls_sdf3_advect_rk3_f32": [P, P, P, P, P, D, I, P],
+    "vls_sdf3_max_speed_f32": [P, P, P, P, PD, P],
+    "vls_sdf3_area_volume_device_f32": [P, P, I, D, P, P],
+    "vls_sdf3_normal_velocity_f32": [P] * 7,
 }
 
 _lib = None

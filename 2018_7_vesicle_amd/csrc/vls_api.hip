@@ -118,9 +118,6 @@ Dist6 carve6(double*& p, int64_t n)
     return d;
 }
 
-// extension loop (WENO5RK3Extend.m:50-74, and the local extension of the
-// surface redistance, WENO5RK3ClosetPointSurfaceRedistance.m:119-157):
-// cp from C once, then `iterations` RK3 steps; C is updated in place.
 // a distance set: the six arrays, their flags and their interface list
 struct DSet {
     Dist6 d;
@@ -135,21 +132,35 @@ DSet main_set(vls_sdf3* h) { return make_set(h, h->d, h->flags, h->list, 0); }
 DSet aux_set(vls_sdf3* h) { return make_set(h, h->ad, h->aflags, h->alist, 1); }
 DSet f_set(vls_sdf3* h) { return make_set(h, h->fd, h->fflags, h->flist, 2); }
 
+// work fields of the handle as fields of T (FP32 calls use the first half of
+// each FP64 field)
+template <typename T>
+T* tmp(vls_sdf3* h, int k)
+{
+    return reinterpret_cast<T*>(h->t[k]);
+}
+template <typename T>
+T* vel(vls_sdf3* h, int a)
+{
+    return reinterpret_cast<T*>(h->v[a]);
+}
+
 // extension loop (WENO5RK3Extend.m:50-74, and the local extension of the
 // surface redistance, WENO5RK3ClosetPointSurfaceRedistance.m:119-157):
 // interface values of C once, then `iterations` RK3 steps; C in place.  The
 // velocity (h->v) must be set; its listed values are gathered here.
-int extend_loop(vls_sdf3* h, double* C, int iterations, const DSet& ds, hipStream_t s)
+template <typename T>
+int extend_loop(vls_sdf3* h, T* C, int iterations, const DSet& ds, hipStream_t s)
 {
     const Geom& g = h->g;
-    VLS_HIP(launch_list_interp(h->lcp, ds.lv, C, g, s));
+    T* t0 = tmp<T>(h, 0);
+    T* t1 = tmp<T>(h, 1);
+    const T *vx = vel<T>(h, 0), *vy = vel<T>(h, 1), *vz = vel<T>(h, 2);
+    VLS_HIP(launch_list_interp(h->lcp, ds.lv, (const T*)C, g, s));
     for (int it = 0; it < iterations; ++it) {
-        VLS_HIP(launch_extend_stage(h->t[0], C, nullptr, h->v[0], h->v[1], h->v[2], ds.flags, ds.lv, h->lcp, h->lvel,
-                                    g, 0, s));
-        VLS_HIP(launch_extend_stage(h->t[1], h->t[0], C, h->v[0], h->v[1], h->v[2], ds.flags, ds.lv, h->lcp, h->lvel,
-                                    g, 1, s));
-        VLS_HIP(launch_extend_stage(C, h->t[1], C, h->v[0], h->v[1], h->v[2], ds.flags, ds.lv, h->lcp, h->lvel, g, 2,
-                                    s));
+        VLS_HIP(launch_extend_stage<T>(t0, C, nullptr, vx, vy, vz, ds.flags, ds.lv, h->lcp, h->lvel, g, 0, s));
+        VLS_HIP(launch_extend_stage<T>(t1, t0, C, vx, vy, vz, ds.flags, ds.lv, h->lcp, h->lvel, g, 1, s));
+        VLS_HIP(launch_extend_stage<T>(C, t1, C, vx, vy, vz, ds.flags, ds.lv, h->lcp, h->lvel, g, 2, s));
     }
     return VLS_OK;
 }
@@ -161,10 +172,11 @@ struct ReinitSeq {
     int seq;
 };
 
-int reinit_stage(vls_sdf3* h, ReinitSeq& rs, double* out, const double* fs, const double* fn, const DSet& ds,
-                 int stage, hipStream_t s)
+template <typename T>
+int reinit_stage(vls_sdf3* h, ReinitSeq& rs, T* out, const T* fs, const T* fn, const DSet& ds, int stage,
+                 hipStream_t s)
 {
-    VLS_HIP(launch_reinit_stage(out, fs, fn, ds.flags, ds.lv, h->g, stage, h->over, rs.counts, rs.seq++, s));
+    VLS_HIP(launch_reinit_stage<T>(out, fs, fn, ds.flags, ds.lv, h->g, stage, h->over, rs.counts, rs.seq++, s));
     return VLS_OK;
 }
 
@@ -177,26 +189,119 @@ int reinit_begin(vls_sdf3* h, ReinitSeq& rs, hipStream_t s)
 }
 
 // WENO5-RK3 reinitialization loop of F in place with mask and distances of `ds`
-int reinit_loop(vls_sdf3* h, double* F, int iterations, const DSet& ds, hipStream_t s)
+template <typename T>
+int reinit_loop(vls_sdf3* h, T* F, int iterations, const DSet& ds, hipStream_t s)
 {
     ReinitSeq rs;
+    T* t0 = tmp<T>(h, 0);
+    T* t1 = tmp<T>(h, 1);
     int rc = reinit_begin(h, rs, s);
     for (int it = 0; it < iterations && rc == VLS_OK; ++it) {
-        if ((rc = reinit_stage(h, rs, h->t[0], F, nullptr, ds, 0, s))) break;
-        if ((rc = reinit_stage(h, rs, h->t[1], h->t[0], F, ds, 1, s))) break;
-        rc = reinit_stage(h, rs, F, h->t[1], F, ds, 2, s);
+        if ((rc = reinit_stage<T>(h, rs, t0, F, nullptr, ds, 0, s))) break;
+        if ((rc = reinit_stage<T>(h, rs, t1, t0, F, ds, 1, s))) break;
+        rc = reinit_stage<T>(h, rs, F, t1, F, ds, 2, s);
     }
     return rc;
 }
 
-int set_distance_into(vls_sdf3* h, int order, const DSet& ds, double* min_dist, const double* F,
-                      const double* mask_src, hipStream_t s)
+template <typename T>
+int set_distance_into(vls_sdf3* h, int order, const DSet& ds, double* min_dist, const T* F, const T* mask_src,
+                      hipStream_t s)
 {
     const Geom& g = h->g;
     VLS_HIP(launch_fill_spacing(ds.d, g, ds.lv.count, s));
-    VLS_HIP(launch_boundary_location(order, ds.d, F, g, s));
-    VLS_HIP(launch_min_dist_flags(min_dist, ds.flags, constify(ds.d), mask_src, g, ds.lv.idx, ds.lv.count, s));
+    VLS_HIP(launch_boundary_location<T>(order, ds.d, F, g, s));
+    VLS_HIP(launch_min_dist_flags<T>(min_dist, ds.flags, constify(ds.d), mask_src, g, ds.lv.idx, ds.lv.count, s));
     VLS_HIP(launch_list_gather(ds.lv, constify(ds.d), g, s));
+    return VLS_OK;
+}
+
+// ---- wrapper-level operations, for fields of T ------------------------------
+template <typename T>
+int sdf3_weno5_rk3_reinit(vls_sdf3* h, T* F, int iterations, hipStream_t s)
+{
+    VLS_REQUIRE(h && F, "null argument");
+    VLS_REQUIRE(iterations >= 0, "negative iteration count");
+    VLS_HIP(launch_set_mask_bit<T>(h->flags, F, h->g, s));
+    return reinit_loop<T>(h, F, iterations, main_set(h), s);
+}
+
+template <typename T>
+int sdf3_weno5_rk3_extend(vls_sdf3* h, const T* F, T* C, int iterations, hipStream_t s)
+{
+    VLS_REQUIRE(h && F && C, "null argument");
+    VLS_REQUIRE(iterations >= 0, "negative iteration count");
+    VLS_REQUIRE((const void*)F != (const void*)C, "C must not alias F");
+    const Geom& g = h->g;
+    const DSet ms = main_set(h);
+    T *vx = vel<T>(h, 0), *vy = vel<T>(h, 1), *vz = vel<T>(h, 2);
+    VLS_HIP(launch_upwind_normal<T>(vx, vy, vz, F, constify(h->d), g, s));
+    VLS_HIP(launch_extension_velocity<T>(vx, vy, vz, F, 1e-6, g, s));
+    VLS_HIP(launch_list_gather_vel<T>(h->lvel, ms.lv, vx, vy, vz, g, s));
+    return extend_loop<T>(h, C, iterations, ms, s);
+}
+
+template <typename T>
+int sdf3_surface_redistance(vls_sdf3* h, const T* F, T* A, int iter1, int iter2, hipStream_t s)
+{
+    VLS_REQUIRE(h && F && A, "null argument");
+    VLS_REQUIRE(iter1 >= 0 && iter2 >= 0, "negative iteration count");
+    VLS_REQUIRE((const void*)F != (const void*)A, "A must not alias F");
+    const Geom& g = h->g;
+    int rc;
+    // A's distances, deltat and mask (WENO5RK3ClosetPointSurfaceRedistance.m:8-25)
+    const DSet as = aux_set(h), fs = f_set(h);
+    if ((rc = set_distance_into<T>(h, 6, as, nullptr, A, A, s))) return rc;
+    // F's distances and the extension velocity with floor 1e-14 (:30-74)
+    if ((rc = set_distance_into<T>(h, 6, fs, nullptr, F, nullptr, s))) return rc;
+    T *vx = vel<T>(h, 0), *vy = vel<T>(h, 1), *vz = vel<T>(h, 2);
+    VLS_HIP(launch_upwind_normal<T>(vx, vy, vz, F, constify(h->fd), g, s));
+    VLS_HIP(launch_extension_velocity<T>(vx, vy, vz, F, 1e-14, g, s));
+    VLS_HIP(launch_list_gather_vel<T>(h->lvel, fs.lv, vx, vy, vz, g, s));
+    T* a1 = tmp<T>(h, 2);
+    T* a05 = tmp<T>(h, 3);
+    ReinitSeq rs;
+    if ((rc = reinit_begin(h, rs, s))) return rc;
+    for (int it = 0; it < iter1; ++it) {
+        if ((rc = extend_loop<T>(h, A, iter2, fs, s))) return rc;
+        if ((rc = reinit_stage<T>(h, rs, a1, A, nullptr, as, 0, s))) return rc;
+        if ((rc = extend_loop<T>(h, a1, iter2, fs, s))) return rc;
+        if ((rc = reinit_stage<T>(h, rs, a05, a1, A, as, 1, s))) return rc;
+        if ((rc = extend_loop<T>(h, a05, iter2, fs, s))) return rc;
+        if ((rc = reinit_stage<T>(h, rs, A, a05, A, as, 2, s))) return rc;
+    }
+    return extend_loop<T>(h, A, iter2, fs, s);
+}
+
+template <typename T>
+int sdf3_advect_rk3(vls_sdf3* h, T* F, const T* vx, const T* vy, const T* vz, double dt, int steps, hipStream_t s)
+{
+    VLS_REQUIRE(h && F && vx && vy && vz, "null argument");
+    VLS_REQUIRE(steps >= 0, "negative step count");
+    const Geom& g = h->g;
+    T* t0 = tmp<T>(h, 0);
+    T* t1 = tmp<T>(h, 1);
+    for (int it = 0; it < steps; ++it) {
+        VLS_HIP(launch_advect_stage<T>(t0, F, nullptr, vx, vy, vz, dt, g, 0, s));
+        VLS_HIP(launch_advect_stage<T>(t1, t0, F, vx, vy, vz, dt, g, 1, s));
+        VLS_HIP(launch_advect_stage<T>(F, t1, F, vx, vy, vz, dt, g, 2, s));
+    }
+    return VLS_OK;
+}
+
+template <typename T>
+int sdf3_max_speed(vls_sdf3* h, const T* vx, const T* vy, const T* vz, double* out, hipStream_t s);
+
+template <typename T>
+int area_volume(vls_sdf3* h, int kind, const T* F, double eps, double* dev_out, double* area, double* volume,
+                hipStream_t s);
+
+template <typename T>
+int sdf3_normal_velocity(vls_sdf3* h, const T* F, const T* vx, const T* vy, const T* vz, T* out, hipStream_t s)
+{
+    VLS_REQUIRE(h && F && vx && vy && vz && out, "null argument");
+    VLS_REQUIRE((const void*)out != (const void*)F, "out must not alias F");
+    VLS_HIP(launch_normal_velocity<T>(out, F, vx, vy, vz, h->g, s));
     return VLS_OK;
 }
 
@@ -204,6 +309,38 @@ int read_result(vls_sdf3* h, double* out, int n, hipStream_t s)
 {
     VLS_HIP(hipMemcpyAsync(out, h->result, sizeof(double) * n, hipMemcpyDeviceToHost, s));
     VLS_HIP(hipStreamSynchronize(s));
+    return VLS_OK;
+}
+
+template <typename T>
+int sdf3_max_speed(vls_sdf3* h, const T* vx, const T* vy, const T* vz, double* out, hipStream_t s)
+{
+    VLS_REQUIRE(h && vx && vy && vz && out, "null argument");
+    VLS_HIP(launch_max_abs3<T>(h->result, h->partial, vx, vy, vz, h->g, s));
+    return read_result(h, out, 1, s);
+}
+
+template <typename T>
+int area_volume(vls_sdf3* h, int kind, const T* F, double eps, double* dev_out, double* area, double* volume,
+                hipStream_t s)
+{
+    VLS_REQUIRE(h && F, "null argument");
+    VLS_REQUIRE(kind == 0 || kind == 1, "kind must be 0 (Towers) or 1 (cosine)");
+    const Geom& g = h->g;
+    double* dst = dev_out ? dev_out : h->result;
+    if (kind == 0) {
+        const double ds = (g.dx + g.dy + g.dz) / 3;
+        VLS_HIP(launch_area_volume_towers<T>(dst, h->partial, F, g, ds, ds * ds * ds, s));
+    } else {
+        VLS_REQUIRE(eps > 0, "eps must be positive");
+        VLS_HIP(launch_area_volume_cosine<T>(dst, h->partial, F, g, eps, g.dx * g.dy * g.dz, s));
+    }
+    if (dev_out) return VLS_OK;
+    double r[2];
+    int rc = read_result(h, r, 2, s);
+    if (rc) return rc;
+    *area = r[0];
+    *volume = r[1];
     return VLS_OK;
 }
 
@@ -442,7 +579,13 @@ int vls_sdf3_destroy(vls_sdf3_t h)
 int vls_sdf3_set_distance(vls_sdf3_t h, const double* F, vls_stream_t stream)
 {
     VLS_REQUIRE(h && F, "null argument");
-    return set_distance_into(h, 6, main_set(h), h->min_dist, F, nullptr, S(stream));
+    return set_distance_into<double>(h, 6, main_set(h), h->min_dist, F, nullptr, S(stream));
+}
+
+int vls_sdf3_set_distance_f32(vls_sdf3_t h, const float* F, vls_stream_t stream)
+{
+    VLS_REQUIRE(h && F, "null argument");
+    return set_distance_into<float>(h, 6, main_set(h), h->min_dist, F, nullptr, S(stream));
 }
 
 int vls_sdf3_distances(vls_sdf3_t h, const double** out7)
@@ -457,11 +600,12 @@ int vls_sdf3_distances(vls_sdf3_t h, const double** out7)
 
 int vls_sdf3_weno5_rk3_reinit(vls_sdf3_t h, double* F, int iterations, vls_stream_t stream)
 {
-    VLS_REQUIRE(h && F, "null argument");
-    VLS_REQUIRE(iterations >= 0, "negative iteration count");
-    const hipStream_t s = S(stream);
-    VLS_HIP(launch_set_mask_bit(h->flags, F, h->g, s));
-    return reinit_loop(h, F, iterations, main_set(h), s);
+    return sdf3_weno5_rk3_reinit<double>(h, F, iterations, S(stream));
+}
+
+int vls_sdf3_weno5_rk3_reinit_f32(vls_sdf3_t h, float* F, int iterations, vls_stream_t stream)
+{
+    return sdf3_weno5_rk3_reinit<float>(h, F, iterations, S(stream));
 }
 
 int vls_sdf3_eno2_reinit(vls_sdf3_t h, double* F, int iterations, int rk2, double cfl, vls_stream_t stream)
@@ -471,7 +615,7 @@ int vls_sdf3_eno2_reinit(vls_sdf3_t h, double* F, int iterations, int rk2, doubl
     const hipStream_t s = S(stream);
     const Geom& g = h->g;
     // ENO2 distances of the argument and mask = F<0 (ENORK2Reinitialization.m:7-25)
-    int rc = set_distance_into(h, 2, aux_set(h), nullptr, F, F, s);
+    int rc = set_distance_into<double>(h, 2, aux_set(h), nullptr, F, F, s);
     if (rc) return rc;
     const Dist6c dc = constify(h->ad);
     if (rk2) {
@@ -488,128 +632,92 @@ int vls_sdf3_eno2_reinit(vls_sdf3_t h, double* F, int iterations, int rk2, doubl
         VLS_HIP(launch_eno2_stage(nxt, cur, nullptr, h->aflags, dc, g, cfl, 0, s));
         double* tmp = cur; cur = nxt; nxt = tmp;
     }
-    if (cur != F) VLS_HIP(launch_copy(F, cur, g.n(), s));
+    if (cur != F) VLS_HIP(launch_copy<double>(F, cur, g.n(), s));
     return VLS_OK;
 }
 
 int vls_sdf3_weno5_rk3_extend(vls_sdf3_t h, const double* F, double* C, int iterations, vls_stream_t stream)
 {
-    VLS_REQUIRE(h && F && C, "null argument");
-    VLS_REQUIRE(iterations >= 0, "negative iteration count");
-    VLS_REQUIRE(F != C, "C must not alias F");
-    const hipStream_t s = S(stream);
-    const Geom& g = h->g;
-    const DSet ms = main_set(h);
-    VLS_HIP(launch_upwind_normal(h->v[0], h->v[1], h->v[2], F, constify(h->d), g, s));
-    VLS_HIP(launch_extension_velocity(h->v[0], h->v[1], h->v[2], F, 1e-6, g, s));
-    VLS_HIP(launch_list_gather_vel(h->lvel, ms.lv, h->v[0], h->v[1], h->v[2], g, s));
-    return extend_loop(h, C, iterations, ms, s);
+    return sdf3_weno5_rk3_extend<double>(h, F, C, iterations, S(stream));
+}
+
+int vls_sdf3_weno5_rk3_extend_f32(vls_sdf3_t h, const float* F, float* C, int iterations, vls_stream_t stream)
+{
+    return sdf3_weno5_rk3_extend<float>(h, F, C, iterations, S(stream));
 }
 
 int vls_sdf3_surface_redistance(vls_sdf3_t h, const double* F, double* A, int iter1, int iter2, vls_stream_t stream)
 {
-    VLS_REQUIRE(h && F && A, "null argument");
-    VLS_REQUIRE(iter1 >= 0 && iter2 >= 0, "negative iteration count");
-    VLS_REQUIRE(F != A, "A must not alias F");
-    const hipStream_t s = S(stream);
-    const Geom& g = h->g;
-    int rc;
-    // A's distances, deltat and mask (WENO5RK3ClosetPointSurfaceRedistance.m:8-25)
-    const DSet as = aux_set(h), fs = f_set(h);
-    if ((rc = set_distance_into(h, 6, as, nullptr, A, A, s))) return rc;
-    // F's distances and the extension velocity with floor 1e-14 (:30-74)
-    if ((rc = set_distance_into(h, 6, fs, nullptr, F, nullptr, s))) return rc;
-    VLS_HIP(launch_upwind_normal(h->v[0], h->v[1], h->v[2], F, constify(h->fd), g, s));
-    VLS_HIP(launch_extension_velocity(h->v[0], h->v[1], h->v[2], F, 1e-14, g, s));
-    VLS_HIP(launch_list_gather_vel(h->lvel, fs.lv, h->v[0], h->v[1], h->v[2], g, s));
-    double* a1 = h->t[2];
-    double* a05 = h->t[3];
-    ReinitSeq rs;
-    if ((rc = reinit_begin(h, rs, s))) return rc;
-    for (int it = 0; it < iter1; ++it) {
-        if ((rc = extend_loop(h, A, iter2, fs, s))) return rc;
-        if ((rc = reinit_stage(h, rs, a1, A, nullptr, as, 0, s))) return rc;
-        if ((rc = extend_loop(h, a1, iter2, fs, s))) return rc;
-        if ((rc = reinit_stage(h, rs, a05, a1, A, as, 1, s))) return rc;
-        if ((rc = extend_loop(h, a05, iter2, fs, s))) return rc;
-        if ((rc = reinit_stage(h, rs, A, a05, A, as, 2, s))) return rc;
-    }
-    return extend_loop(h, A, iter2, fs, s);
+    return sdf3_surface_redistance<double>(h, F, A, iter1, iter2, S(stream));
+}
+
+int vls_sdf3_surface_redistance_f32(vls_sdf3_t h, const float* F, float* A, int iter1, int iter2,
+                                    vls_stream_t stream)
+{
+    return sdf3_surface_redistance<float>(h, F, A, iter1, iter2, S(stream));
 }
 
 int vls_sdf3_advect_rk3(vls_sdf3_t h, double* F, const double* vx, const double* vy, const double* vz, double dt,
                         int steps, vls_stream_t stream)
 {
-    VLS_REQUIRE(h && F && vx && vy && vz, "null argument");
-    VLS_REQUIRE(steps >= 0, "negative step count");
-    const hipStream_t s = S(stream);
-    const Geom& g = h->g;
-    for (int it = 0; it < steps; ++it) {
-        VLS_HIP(launch_advect_stage(h->t[0], F, nullptr, vx, vy, vz, dt, g, 0, s));
-        VLS_HIP(launch_advect_stage(h->t[1], h->t[0], F, vx, vy, vz, dt, g, 1, s));
-        VLS_HIP(launch_advect_stage(F, h->t[1], F, vx, vy, vz, dt, g, 2, s));
-    }
-    return VLS_OK;
+    return sdf3_advect_rk3<double>(h, F, vx, vy, vz, dt, steps, S(stream));
+}
+
+int vls_sdf3_advect_rk3_f32(vls_sdf3_t h, float* F, const float* vx, const float* vy, const float* vz, double dt,
+                            int steps, vls_stream_t stream)
+{
+    return sdf3_advect_rk3<float>(h, F, vx, vy, vz, dt, steps, S(stream));
 }
 
 int vls_sdf3_max_speed(vls_sdf3_t h, const double* vx, const double* vy, const double* vz, double* out,
                        vls_stream_t stream)
 {
-    VLS_REQUIRE(h && vx && vy && vz && out, "null argument");
-    VLS_HIP(launch_max_abs3(h->result, h->partial, vx, vy, vz, h->g, S(stream)));
-    return read_result(h, out, 1, S(stream));
+    return sdf3_max_speed<double>(h, vx, vy, vz, out, S(stream));
 }
 
-static int area_volume(vls_sdf3_t h, int kind, const double* F, double eps, double* dev_out, double* area,
-                       double* volume, vls_stream_t stream)
+int vls_sdf3_max_speed_f32(vls_sdf3_t h, const float* vx, const float* vy, const float* vz, double* out,
+                           vls_stream_t stream)
 {
-    VLS_REQUIRE(h && F, "null argument");
-    const Geom& g = h->g;
-    double* dst = dev_out ? dev_out : h->result;
-    if (kind == 0) {
-        const double ds = (g.dx + g.dy + g.dz) / 3;
-        VLS_HIP(launch_area_volume_towers(dst, h->partial, F, g, ds, ds * ds * ds, S(stream)));
-    } else {
-        VLS_REQUIRE(eps > 0, "eps must be positive");
-        VLS_HIP(launch_area_volume_cosine(dst, h->partial, F, g, eps, g.dx * g.dy * g.dz, S(stream)));
-    }
-    if (dev_out) return VLS_OK;
-    double r[2];
-    int rc = read_result(h, r, 2, S(stream));
-    if (rc) return rc;
-    *area = r[0];
-    *volume = r[1];
-    return VLS_OK;
+    return sdf3_max_speed<float>(h, vx, vy, vz, out, S(stream));
 }
 
 int vls_sdf3_area_volume_towers(vls_sdf3_t h, const double* F, double* area, double* volume, vls_stream_t stream)
 {
     VLS_REQUIRE(area && volume, "null argument");
-    return area_volume(h, 0, F, 0.0, nullptr, area, volume, stream);
+    return area_volume<double>(h, 0, F, 0.0, nullptr, area, volume, S(stream));
 }
 
 int vls_sdf3_area_volume_cosine(vls_sdf3_t h, const double* F, double eps, double* area, double* volume,
                                 vls_stream_t stream)
 {
     VLS_REQUIRE(area && volume, "null argument");
-    return area_volume(h, 1, F, eps, nullptr, area, volume, stream);
+    return area_volume<double>(h, 1, F, eps, nullptr, area, volume, S(stream));
 }
 
 int vls_sdf3_area_volume_device(vls_sdf3_t h, const double* F, int kind, double eps, double* out2,
                                 vls_stream_t stream)
 {
     VLS_REQUIRE(out2, "null argument");
-    VLS_REQUIRE(kind == 0 || kind == 1, "kind must be 0 (Towers) or 1 (cosine)");
-    return area_volume(h, kind, F, eps, out2, nullptr, nullptr, stream);
+    return area_volume<double>(h, kind, F, eps, out2, nullptr, nullptr, S(stream));
+}
+
+int vls_sdf3_area_volume_device_f32(vls_sdf3_t h, const float* F, int kind, double eps, double* out2,
+                                    vls_stream_t stream)
+{
+    VLS_REQUIRE(out2, "null argument");
+    return area_volume<float>(h, kind, F, eps, out2, nullptr, nullptr, S(stream));
 }
 
 int vls_sdf3_normal_velocity(vls_sdf3_t h, const double* F, const double* vx, const double* vy, const double* vz,
                              double* out, vls_stream_t stream)
 {
-    VLS_REQUIRE(h && F && vx && vy && vz && out, "null argument");
-    VLS_REQUIRE(out != F, "out must not alias F");
-    VLS_HIP(launch_normal_velocity(out, F, vx, vy, vz, h->g, S(stream)));
-    return VLS_OK;
+    return sdf3_normal_velocity<double>(h, F, vx, vy, vz, out, S(stream));
+}
+
+int vls_sdf3_normal_velocity_f32(vls_sdf3_t h, const float* F, const float* vx, const float* vy, const float* vz,
+                                 float* out, vls_stream_t stream)
+{
+    return sdf3_normal_velocity<float>(h, F, vx, vy, vz, out, S(stream));
 }
 
 }  // extern "C"

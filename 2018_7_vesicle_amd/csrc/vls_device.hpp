@@ -114,6 +114,62 @@ __device__ __forceinline__ void weno5_pair(double& d_fore, double& d_back, const
 }
 
 // ---------------------------------------------------------------------------
+// FP32 bulk arithmetic: the same WENO5 and upwinding in single precision for
+// the points away from the interface.  The final division of the weights is a
+// hardware reciprocal (v_rcp_f32, 1 ulp) times the numerator.  The products
+// a_j a_k ~ S^4 would overflow FP32 for S > ~4e9, so each S_k is clamped at
+// 2^32 first: a stencil that rough has a relative weight below 1e-19 next to
+// any smooth one, and the clamp only matters when all three stencils exceed
+// it (divided differences above ~1e4 throughout), where the weights fall
+// back to the linear ones.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float fmaxf_(float a, float b) { return (a < b) ? b : a; }
+__device__ __forceinline__ float fminf_(float a, float b) { return (a < b) ? a : b; }
+
+__device__ __forceinline__ float weno5_onesided(float v1, float v2, float v3, float v4, float v5)
+{
+    const float c13 = 1.f / 3.f, c76 = 7.f / 6.f, c116 = 11.f / 6.f, c16 = 1.f / 6.f, c56 = 5.f / 6.f,
+                c1312 = 13.f / 12.f;
+    const float phi1 = fmaf(c116, v3, fmaf(-c76, v2, c13 * v1));
+    const float phi2 = fmaf(c13, v4, fmaf(c56, v3, -c16 * v2));
+    const float phi3 = fmaf(-c16, v5, fmaf(c56, v4, c13 * v3));
+    const float d1 = fmaf(-2.f, v2, v1) + v3, e1 = fmaf(3.f, v3, fmaf(-4.f, v2, v1));
+    const float d2 = fmaf(-2.f, v3, v2) + v4, e2 = v2 - v4;
+    const float d3 = fmaf(-2.f, v4, v3) + v5, e3 = fmaf(3.f, v3, fmaf(-4.f, v4, v5));
+    const float smax = 4294967296.f;
+    const float s1 = fminf_(fmaf(c1312 * d1, d1, 0.25f * e1 * e1), smax) + 1e-6f;
+    const float s2 = fminf_(fmaf(c1312 * d2, d2, 0.25f * e2 * e2), smax) + 1e-6f;
+    const float s3 = fminf_(fmaf(c1312 * d3, d3, 0.25f * e3 * e3), smax) + 1e-6f;
+    const float a1 = s1 * s1, a2 = s2 * s2, a3 = s3 * s3;
+    const float w1 = 0.1f * (a2 * a3), w2 = 0.6f * (a1 * a3), w3 = 0.3f * (a1 * a2);
+    return fmaf(w1, phi1, fmaf(w2, phi2, w3 * phi3)) * __builtin_amdgcn_rcpf(w1 + w2 + w3);
+}
+
+__device__ __forceinline__ float upwind_axis(float acc, float vel, const float p[7], float inv_ds)
+{
+    float v[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v[k] = (p[k + 1] - p[k]) * inv_ds;
+    const bool fore = vel < 0;
+    const float d = weno5_onesided(fore ? v[5] : v[0], fore ? v[4] : v[1], fore ? v[3] : v[2],
+                                    fore ? v[2] : v[3], fore ? v[1] : v[4]);
+    return acc + (fore ? vel : fmaxf_(0.f, vel)) * d;
+}
+
+__device__ __forceinline__ void weno5_pair(float& d_fore, float& d_back, const float p[7], float inv_ds)
+{
+    float v[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v[k] = (p[k + 1] - p[k]) * inv_ds;
+    d_back = weno5_onesided(v[0], v[1], v[2], v[3], v[4]);
+    d_fore = weno5_onesided(v[5], v[4], v[3], v[2], v[1]);
+}
+
+// a*b < 0 without forming the product (an FP32 product of two tiny values
+// can underflow to -0 and hide the sign change the FP64 test sees)
+__device__ __forceinline__ bool opposite(float a, float b) { return (a < 0.f) ? (b > 0.f) : (a > 0.f && b < 0.f); }
+
+// ---------------------------------------------------------------------------
 // Nonuniform three-stencil WENO at a node next to the interface:
 // re_step.cu:79-243 (weno_nonuniform), reconstructing at face a[2] (fore) or
 // a[3] (back) from first divided differences u[k] over [a[k], a[k+1]].
@@ -252,6 +308,23 @@ __device__ __forceinline__ double godunov_step(bool inside, double xR, double xL
         g = dmax(sq(dmax(0.0, xL)), sq(dmin(0.0, xR))) + dmax(sq(dmax(0.0, yB)), sq(dmin(0.0, yF)))
             + dmax(sq(dmax(0.0, zD)), sq(dmin(0.0, zU)));
     const double s = (sqrt(g) - 1) * dt;
+    return inside ? -s : s;
+}
+
+__device__ __forceinline__ float fsq(float a) { return a * a; }
+
+// FP32 form of godunov_step (bulk points)
+__device__ __forceinline__ float godunov_step(bool inside, float xR, float xL, float yF, float yB, float zU,
+                                              float zD, float dt)
+{
+    float g;
+    if (inside)
+        g = fmaxf_(fsq(fminf_(0.f, xL)), fsq(fmaxf_(0.f, xR))) + fmaxf_(fsq(fminf_(0.f, yB)), fsq(fmaxf_(0.f, yF)))
+            + fmaxf_(fsq(fminf_(0.f, zD)), fsq(fmaxf_(0.f, zU)));
+    else
+        g = fmaxf_(fsq(fmaxf_(0.f, xL)), fsq(fminf_(0.f, xR))) + fmaxf_(fsq(fmaxf_(0.f, yB)), fsq(fminf_(0.f, yF)))
+            + fmaxf_(fsq(fmaxf_(0.f, zD)), fsq(fminf_(0.f, zU)));
+    const float s = (__builtin_sqrtf(g) - 1.f) * dt;
     return inside ? -s : s;
 }
 

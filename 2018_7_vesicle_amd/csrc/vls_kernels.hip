@@ -56,6 +56,18 @@ __device__ __forceinline__ double rk_combine(int stage, double fs, double step, 
     return 1. / 3. * fn + 2. / 3. * t;
 }
 
+__device__ __forceinline__ float rk_combine(int stage, float fs, float step, float fn)
+{
+    const float t = fs - step;
+    if (stage == 0) return t;
+    if (stage == 1) return fmaf(0.75f, fn, 0.25f * t);
+    return fmaf(1.f / 3.f, fn, (2.f / 3.f) * t);
+}
+
+// the interface test of re_step.cu:260-267 on a pair of nodal values
+__device__ __forceinline__ bool sign_change(double a, double b) { return a * b < 0; }
+__device__ __forceinline__ bool sign_change(float a, float b) { return opposite(a, b); }
+
 // ---------------------------------------------------------------------------
 // setDistance: fill, crossing distances, min_dist + flags
 // ---------------------------------------------------------------------------
@@ -72,13 +84,13 @@ __global__ void k_fill_spacing(Dist6 d, Box b, idx_t n, int* __restrict__ list_c
 // boundary_location (order 6, 1_3_WENO5RK3_Reinitialization/boundary_location.cu:84-132)
 // and boundary_correction (order 2, 1_0_ENORK2_Reinitialization/boundary_correction.cu:103-145).
 // One writer per element: r-side at ind, l-side at the +1 neighbour.
-template <int ORDER>
-__global__ void __launch_bounds__(256) k_boundary_location(Dist6 d, const double* __restrict__ lsf, Box b)
+template <int ORDER, typename T>
+__global__ void __launch_bounds__(256) k_boundary_location(Dist6 d, const T* __restrict__ lsf, Box b)
 {
     int r, c, p;
     if (!point_of_thread(b, r, c, p)) return;
     const idx_t ind = lin(b, r, c, p);
-    const double f0 = lsf[ind];
+    const double f0 = (double)lsf[ind];
     const double sp[3] = {b.dx, b.dy, b.dz};
 #pragma unroll 1
     for (int axis = 0; axis < 3; ++axis) {
@@ -87,7 +99,7 @@ __global__ void __launch_bounds__(256) k_boundary_location(Dist6 d, const double
         else if (axis == 1) rr = min(r + 1, b.rows - 1);
         else pp = min(p + 1, b.pges - 1);
         const idx_t nb = lin(b, rr, cc, pp);
-        const double f2 = lsf[nb];
+        const double f2 = (double)lsf[nb];
         if (!(f0 * f2 < 0)) continue;
         double v[7];
         line7(lsf, r, c, p, axis, b, v);
@@ -115,8 +127,9 @@ __device__ __forceinline__ void wave_append(bool take, int value, int* __restric
 // min_dist (setDistance.m:17-21), the per-point flags of the fused kernels and
 // the list of interface points (some distance below the spacing).
 // mask_src may be null (no mask bit); list may be null.
+template <typename T>
 __global__ void k_min_dist_flags(double* __restrict__ min_dist, uint8_t* __restrict__ flags, Dist6c d,
-                                 const double* __restrict__ mask_src, Box b, idx_t n, int* __restrict__ list,
+                                 const T* __restrict__ mask_src, Box b, idx_t n, int* __restrict__ list,
                                  int* __restrict__ list_count)
 {
     const idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -140,7 +153,8 @@ __global__ void k_zero_ints(int* __restrict__ a, int n)
 }
 
 // mask bit only (reinit with distances from an earlier setDistance)
-__global__ void k_set_mask_bit(uint8_t* __restrict__ flags, const double* __restrict__ F, idx_t n)
+template <typename T>
+__global__ void k_set_mask_bit(uint8_t* __restrict__ flags, const T* __restrict__ F, idx_t n)
 {
     const idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -305,34 +319,38 @@ __device__ __forceinline__ Tile tile_of(const Box& b)
 }
 
 // stage the tile of page t.p of f (with clamped halo) in lds[LC*LR]; ends with a barrier
-__device__ __forceinline__ void fill_tile(double* lds, const double* __restrict__ f, const Box& b, const Tile& t)
+template <typename T>
+__device__ __forceinline__ void fill_tile(T* lds, const T* __restrict__ f, const Box& b, const Tile& t)
 {
     const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
-    const double* fp = f + (idx_t)t.p * b.rows * b.cols;
+    const T* fp = f + (idx_t)t.p * b.rows * b.cols;
     const int rlo = clampi(t.r0 + lane - HALO, b.rows - 1);
     const int rhi = clampi(t.r0 + TR + lane - HALO, b.rows - 1);
     for (int lc = wave; lc < LC; lc += 4) {
-        const double* col = fp + (idx_t)clampi(t.c0 + lc - HALO, b.cols - 1) * b.rows;
+        const T* col = fp + (idx_t)clampi(t.c0 + lc - HALO, b.cols - 1) * b.rows;
         lds[lc * LR + lane] = col[rlo];
         if (lane < 2 * HALO) lds[lc * LR + TR + lane] = col[rhi];
     }
     __syncthreads();
 }
 
-__device__ __forceinline__ void tile_line_y(const double* lds, int base, double v[7])
+template <typename T>
+__device__ __forceinline__ void tile_line_y(const T* lds, int base, T v[7])
 {
 #pragma unroll
     for (int k = 0; k < 7; ++k) v[k] = lds[base + k - 3];
 }
 
-__device__ __forceinline__ void tile_line_x(const double* lds, int base, double v[7])
+template <typename T>
+__device__ __forceinline__ void tile_line_x(const T* lds, int base, T v[7])
 {
 #pragma unroll
     for (int k = 0; k < 7; ++k) v[k] = lds[base + (k - 3) * LR];
 }
 
 // z-taps: page offsets are the same for the whole block
-__device__ __forceinline__ void tile_line_z(const double* __restrict__ f, const Box& b, const Tile& t, double v[7])
+template <typename T>
+__device__ __forceinline__ void tile_line_z(const T* __restrict__ f, const Box& b, const Tile& t, T v[7])
 {
     const idx_t plane = (idx_t)b.rows * b.cols;
     const idx_t inplane = (idx_t)t.c * b.rows + t.r;
@@ -350,12 +368,13 @@ __device__ __forceinline__ int tile_blocks(const Box& b)
 // changes sign next to it although it has none (it needs the nonuniform
 // stencil): those are appended to `over`, counted in over_count[seq & 1].
 // The other slot is cleared here for the next stage.
-__global__ void __launch_bounds__(256) k_reinit_bulk(double* __restrict__ out, const double* __restrict__ fs,
-                                                     const double* __restrict__ fn, const uint8_t* __restrict__ flags,
+template <typename T>
+__global__ void __launch_bounds__(256) k_reinit_bulk(T* __restrict__ out, const T* __restrict__ fs,
+                                                     const T* __restrict__ fn, const uint8_t* __restrict__ flags,
                                                      Box b, Inv invs, int stage, int* __restrict__ over,
                                                      int* __restrict__ over_count, int seq)
 {
-    __shared__ double lds[LC * LR];
+    __shared__ T lds[LC * LR];
     if (blockIdx.x == 0 && threadIdx.x == 0) over_count[(seq + 1) & 1] = 0;
     const Tile t = tile_of(b);
     fill_tile(lds, fs, b, t);
@@ -363,20 +382,20 @@ __global__ void __launch_bounds__(256) k_reinit_bulk(double* __restrict__ out, c
     const uint8_t fl = t.valid ? flags[ind] : (uint8_t)FLAG_SUBCELL;
     bool defer = false;
     if (!(fl & FLAG_SUBCELL)) {
-        double fo[3], bk[3];
+        T fo[3], bk[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            double v[7];
+            T v[7];
             if (a == 0) tile_line_x(lds, t.base, v);
             else if (a == 1) tile_line_y(lds, t.base, v);
             else tile_line_z(fs, b, t, v);
-            defer = defer || (v[2] * v[3] < 0 || v[3] * v[4] < 0);
-            weno5_pair(fo[a], bk[a], v, invs.i[a]);
+            defer = defer || (sign_change(v[2], v[3]) || sign_change(v[3], v[4]));
+            weno5_pair(fo[a], bk[a], v, (T)invs.i[a]);
         }
         if (!defer) {
-            const double step = godunov_step((fl & FLAG_MASK) != 0, fo[0], bk[0], fo[1], bk[1], fo[2], bk[2],
-                                             invs.dt_bulk);
-            out[ind] = rk_combine(stage, lds[t.base], step, (stage == 0) ? 0.0 : fn[ind]);
+            const T step = godunov_step((fl & FLAG_MASK) != 0, fo[0], bk[0], fo[1], bk[1], fo[2], bk[2],
+                                        (T)invs.dt_bulk);
+            out[ind] = rk_combine(stage, lds[t.base], step, (stage == 0) ? (T)0 : fn[ind]);
         }
     }
     wave_append(defer, (int)ind, over, &over_count[seq & 1]);
@@ -409,8 +428,9 @@ __device__ __forceinline__ int axis_lane_stride() { return (int)(gridDim.x * (bl
 
 // launch 2 of 2: the interface list, then the deferred bulk points (no
 // subcell distances: spacing distances and the bulk pseudo-time step)
-__global__ void __launch_bounds__(256) k_reinit_iface(double* __restrict__ out, const double* __restrict__ fs,
-                                                      const double* __restrict__ fn,
+template <typename T>
+__global__ void __launch_bounds__(256) k_reinit_iface(T* __restrict__ out, const T* __restrict__ fs,
+                                                      const T* __restrict__ fn,
                                                       const uint8_t* __restrict__ flags, IList il, Box b, Inv inv,
                                                       int stage, const int* __restrict__ over,
                                                       const int* __restrict__ over_count, int seq)
@@ -453,7 +473,7 @@ __global__ void __launch_bounds__(256) k_reinit_iface(double* __restrict__ out, 
                 const double s1 = (sqrt((m + my) + mz) - 1) * dt;
                 step = inside ? -s1 : s1;
             }
-            out[ind] = rk_combine(stage, fs[ind], step, (stage == 0) ? 0.0 : fn[ind]);
+            out[ind] = (T)rk_combine(stage, (double)fs[ind], step, (stage == 0) ? 0.0 : (double)fn[ind]);
         }
     }
 }
@@ -482,7 +502,8 @@ __global__ void __launch_bounds__(256) k_list_gather(IListW il, Dist6c d, Box b)
 // boundary_interpolant (boundary_interpolant.cu:102-141) at the listed points
 // only (it is zero elsewhere), one lane per (point, axis), in list order:
 // cp[2a][j] = fore value, cp[2a+1][j] = back value
-__global__ void __launch_bounds__(256) k_list_interp(double* __restrict__ cp, IList il, const double* __restrict__ C,
+template <typename T>
+__global__ void __launch_bounds__(256) k_list_interp(double* __restrict__ cp, IList il, const T* __restrict__ C,
                                                      Box b)
 {
     const int total = *il.count;
@@ -505,15 +526,16 @@ __global__ void __launch_bounds__(256) k_list_interp(double* __restrict__ cp, IL
     }
 }
 
-__global__ void __launch_bounds__(256) k_list_gather_vel(double* __restrict__ lv, IList il, const double* __restrict__ vx,
-                                                         const double* __restrict__ vy, const double* __restrict__ vz)
+template <typename T>
+__global__ void __launch_bounds__(256) k_list_gather_vel(double* __restrict__ lv, IList il, const T* __restrict__ vx,
+                                                         const T* __restrict__ vy, const T* __restrict__ vz)
 {
     const int cnt = *il.count;
     for (int j = (int)(blockIdx.x * 256 + threadIdx.x); j < cnt; j += (int)(gridDim.x * 256)) {
         const idx_t ind = (idx_t)il.idx[j];
-        lv[j] = vx[ind];
-        lv[(size_t)il.cap + j] = vy[ind];
-        lv[(size_t)2 * il.cap + j] = vz[ind];
+        lv[j] = (double)vx[ind];
+        lv[(size_t)il.cap + j] = (double)vy[ind];
+        lv[(size_t)2 * il.cap + j] = (double)vz[ind];
     }
 }
 
@@ -586,36 +608,38 @@ __device__ __forceinline__ void extend_axis(double& fore, double& back, const do
 }
 
 // upwind_normal: upwind_normal.cu:4-104
-__global__ void __launch_bounds__(256) k_upwind_normal(double* __restrict__ nx, double* __restrict__ ny,
-                                                       double* __restrict__ nz, const double* __restrict__ lsf,
+template <typename T>
+__global__ void __launch_bounds__(256) k_upwind_normal(T* __restrict__ nx, T* __restrict__ ny,
+                                                       T* __restrict__ nz, const T* __restrict__ lsf,
                                                        Dist6c d, Box b)
 {
     int r, c, p;
     if (!point_of_thread(b, r, c, p)) return;
     const idx_t ind = lin(b, r, c, p);
     const double sp[3] = {b.dx, b.dy, b.dz};
-    double* out[3] = {nx, ny, nz};
+    T* out[3] = {nx, ny, nz};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         double v[7], fo, bk;
         line7(lsf, r, c, p, a, b, v);
         extend_axis(fo, bk, v, d.r[a][ind], d.l[a][ind], sp[a], 1.0 / sp[a], 0.0, 0.0);
-        out[a][ind] = (fabs(v[4]) < fabs(v[2])) ? fo : bk;
+        out[a][ind] = (T)((fabs(v[4]) < fabs(v[2])) ? fo : bk);
     }
 }
 
 // v = sign(F)*n/max(|n|,floor) in place on (vx,vy,vz) (WENO5RK3Extend.m:23-31)
-__global__ void k_extension_velocity(double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz,
-                                     const double* __restrict__ F, double floor_, idx_t n)
+template <typename T>
+__global__ void k_extension_velocity(T* __restrict__ vx, T* __restrict__ vy, T* __restrict__ vz,
+                                     const T* __restrict__ F, double floor_, idx_t n)
 {
     const idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double fx = vx[i], fy = vy[i], fz = vz[i];
     const double g = dmax(sqrt(fx * fx + fy * fy + fz * fz), floor_);
-    const double s = msign(F[i]);
-    vx[i] = s * (fx / g);
-    vy[i] = s * (fy / g);
-    vz[i] = s * (fz / g);
+    const double s = msign((double)F[i]);
+    vx[i] = (T)(s * (fx / g));
+    vy[i] = (T)(s * (fy / g));
+    vz[i] = (T)(s * (fz / g));
 }
 
 // boundary_interpolant: boundary_interpolant.cu:102-141
@@ -678,35 +702,37 @@ __global__ void __launch_bounds__(256) k_extend_step(double* __restrict__ step, 
 }
 
 // fused WENO5-RK3 extension stage, bulk: every point without subcell distances
-__global__ void __launch_bounds__(256) k_extend_bulk(double* __restrict__ out, const double* __restrict__ cs,
-                                                     const double* __restrict__ cn, const double* __restrict__ vx,
-                                                     const double* __restrict__ vy, const double* __restrict__ vz,
+template <typename T>
+__global__ void __launch_bounds__(256) k_extend_bulk(T* __restrict__ out, const T* __restrict__ cs,
+                                                     const T* __restrict__ cn, const T* __restrict__ vx,
+                                                     const T* __restrict__ vy, const T* __restrict__ vz,
                                                      const uint8_t* __restrict__ flags, Box b, Inv inv, int stage)
 {
-    __shared__ double lds[LC * LR];
+    __shared__ T lds[LC * LR];
     const Tile t = tile_of(b);
     fill_tile(lds, cs, b, t);
     if (!t.valid) return;
     const idx_t ind = lin(b, t.r, t.c, t.p);
     if (flags[ind] & FLAG_SUBCELL) return;
-    const double vel[3] = {vx[ind], vy[ind], vz[ind]};
-    double acc = 0.0;
-    double v[7];
+    const T vel[3] = {vx[ind], vy[ind], vz[ind]};
+    T acc = 0;
+    T v[7];
     tile_line_x(lds, t.base, v);
-    acc = upwind_axis(acc, vel[0], v, inv.i[0]);
+    acc = upwind_axis(acc, vel[0], v, (T)inv.i[0]);
     tile_line_y(lds, t.base, v);
-    acc = upwind_axis(acc, vel[1], v, inv.i[1]);
+    acc = upwind_axis(acc, vel[1], v, (T)inv.i[1]);
     tile_line_z(cs, b, t, v);
-    acc = upwind_axis(acc, vel[2], v, inv.i[2]);
-    out[ind] = rk_combine(stage, lds[t.base], acc * inv.dt_bulk, (stage == 0) ? 0.0 : cn[ind]);
+    acc = upwind_axis(acc, vel[2], v, (T)inv.i[2]);
+    out[ind] = rk_combine(stage, lds[t.base], acc * (T)inv.dt_bulk, (stage == 0) ? (T)0 : cn[ind]);
 }
 
 // fused WENO5-RK3 extension stage, interface list: three lanes per point
 // (see AxisLane); each lane evaluates the one upwind derivative its axis
 // needs (see upwind_axis), on the nonuniform stencil where the axis crosses.
 // lcp, lv: interface values and velocity in list order.
-__global__ void __launch_bounds__(256) k_extend_iface(double* __restrict__ out, const double* __restrict__ cs,
-                                                      const double* __restrict__ cn, IList il,
+template <typename T>
+__global__ void __launch_bounds__(256) k_extend_iface(T* __restrict__ out, const T* __restrict__ cs,
+                                                      const T* __restrict__ cn, IList il,
                                                       const double* __restrict__ lcp, const double* __restrict__ lv,
                                                       Box b, Inv inv, int stage)
 {
@@ -744,7 +770,7 @@ __global__ void __launch_bounds__(256) k_extend_iface(double* __restrict__ out, 
         const double ty = __shfl(t, al.base + 1, 64), tz = __shfl(t, al.base + 2, 64);
         if (active && al.axis == 0) {
             const double step = (dt >= 0) ? ((t + ty) + tz) * dt : 0.0;
-            out[ind] = rk_combine(stage, cs[ind], step, (stage == 0) ? 0.0 : cn[ind]);
+            out[ind] = (T)rk_combine(stage, (double)cs[ind], step, (stage == 0) ? 0.0 : (double)cn[ind]);
         }
     }
 }
@@ -779,31 +805,28 @@ __global__ void __launch_bounds__(256) k_advection_step(double* __restrict__ ast
 }
 
 // fused TVD-RK3 advection stage: out = combine(fs - dt*L(fs), fn), plane tiles
-__global__ void __launch_bounds__(256) k_advect_stage(double* __restrict__ out, const double* __restrict__ fs,
-                                                      const double* __restrict__ fn, const double* __restrict__ vx,
-                                                      const double* __restrict__ vy, const double* __restrict__ vz,
+template <typename T>
+__global__ void __launch_bounds__(256) k_advect_stage(T* __restrict__ out, const T* __restrict__ fs,
+                                                      const T* __restrict__ fn, const T* __restrict__ vx,
+                                                      const T* __restrict__ vy, const T* __restrict__ vz,
                                                       double dt, Box b, Inv inv, int stage)
 {
-    __shared__ double lds[LC * LR];
+    __shared__ T lds[LC * LR];
     const Tile t = tile_of(b);
     fill_tile(lds, fs, b, t);
     if (!t.valid) return;
     const idx_t ind = lin(b, t.r, t.c, t.p);
-    const double vel[3] = {vx[ind], vy[ind], vz[ind]};
-    double acc = 0.0;
-    double v[7];
+    const T vel[3] = {vx[ind], vy[ind], vz[ind]};
+    T acc = 0;
+    T v[7];
     tile_line_x(lds, t.base, v);
-    acc = upwind_axis(acc, vel[0], v, inv.i[0]);
+    acc = upwind_axis(acc, vel[0], v, (T)inv.i[0]);
     tile_line_y(lds, t.base, v);
-    acc = upwind_axis(acc, vel[1], v, inv.i[1]);
+    acc = upwind_axis(acc, vel[1], v, (T)inv.i[1]);
     tile_line_z(fs, b, t, v);
-    acc = upwind_axis(acc, vel[2], v, inv.i[2]);
-    const double fnv = (stage == 0) ? 0.0 : fn[ind];
-    double step;
-    {
-#pragma clang fp contract(off)
-        step = dt * acc;
-    }
+    acc = upwind_axis(acc, vel[2], v, (T)inv.i[2]);
+    const T fnv = (stage == 0) ? (T)0 : fn[ind];
+    const T step = (T)dt * acc;
     out[ind] = rk_combine(stage, lds[t.base], step, fnv);
 }
 
@@ -830,7 +853,8 @@ struct Box27 {
     double v[3][3][3];  // [dz+1][dc+1][dr+1]
 };
 
-__device__ __forceinline__ void load_box27(Box27& o, const double* __restrict__ f, int r, int c, int p, const Box& b)
+template <typename T>
+__device__ __forceinline__ void load_box27(Box27& o, const T* __restrict__ f, int r, int c, int p, const Box& b)
 {
 #pragma unroll
     for (int k = -1; k <= 1; ++k) {
@@ -841,7 +865,7 @@ __device__ __forceinline__ void load_box27(Box27& o, const double* __restrict__ 
 #pragma unroll
             for (int i = -1; i <= 1; ++i) {
                 const int rr = clampi(r + i, b.rows - 1);
-                o.v[k + 1][j + 1][i + 1] = f[(idx_t)pp * b.rows * b.cols + (idx_t)cc * b.rows + rr];
+                o.v[k + 1][j + 1][i + 1] = (double)f[(idx_t)pp * b.rows * b.cols + (idx_t)cc * b.rows + rr];
             }
         }
     }
@@ -925,18 +949,22 @@ __global__ void __launch_bounds__(256) k_set_calculus_toolbox(Fields18 o, const 
 
 // normal component of a velocity, u.N with N = grad F/max(|grad F|,1e-14) from
 // clamped central differences (set_calculus_toolbox.cu:123-136)
-__global__ void __launch_bounds__(256) k_normal_velocity(double* __restrict__ out, const double* __restrict__ F,
-                                                         const double* __restrict__ vx, const double* __restrict__ vy,
-                                                         const double* __restrict__ vz, Box b)
+template <typename T>
+__global__ void __launch_bounds__(256) k_normal_velocity(T* __restrict__ out, const T* __restrict__ F,
+                                                         const T* __restrict__ vx, const T* __restrict__ vy,
+                                                         const T* __restrict__ vz, Box b)
 {
     int r, c, p;
     if (!point_of_thread(b, r, c, p)) return;
     const idx_t ind = lin(b, r, c, p);
-    const double fx = (F[lin(b, r, min(c + 1, b.cols - 1), p)] - F[lin(b, r, max(c - 1, 0), p)]) / (2 * b.dx);
-    const double fy = (F[lin(b, min(r + 1, b.rows - 1), c, p)] - F[lin(b, max(r - 1, 0), c, p)]) / (2 * b.dy);
-    const double fz = (F[lin(b, r, c, min(p + 1, b.pges - 1))] - F[lin(b, r, c, max(p - 1, 0))]) / (2 * b.dz);
+    const double fx = ((double)F[lin(b, r, min(c + 1, b.cols - 1), p)] - (double)F[lin(b, r, max(c - 1, 0), p)]) /
+                      (2 * b.dx);
+    const double fy = ((double)F[lin(b, min(r + 1, b.rows - 1), c, p)] - (double)F[lin(b, max(r - 1, 0), c, p)]) /
+                      (2 * b.dy);
+    const double fz = ((double)F[lin(b, r, c, min(p + 1, b.pges - 1))] - (double)F[lin(b, r, c, max(p - 1, 0))]) /
+                      (2 * b.dz);
     const double g = dmax(sqrt(fx * fx + fy * fy + fz * fz), 1e-14);
-    out[ind] = vx[ind] * (fx / g) + vy[ind] * (fy / g) + vz[ind] * (fz / g);
+    out[ind] = (T)((double)vx[ind] * (fx / g) + (double)vy[ind] * (fy / g) + (double)vz[ind] * (fz / g));
 }
 
 // ---------------------------------------------------------------------------
@@ -1010,14 +1038,15 @@ __device__ __forceinline__ void block_reduce_store(KSum (&k)[NQ], double* __rest
 
 // cosine-smoothed delta/H of width eps (src/0_0_setup/+SD/@SDF3/SDF3.m:126-136);
 // |grad F| by periodic central differences (src/0_0_setup/+SD/@GD3/GD3.m:212-223)
-__global__ void __launch_bounds__(256) k_area_volume_cosine(double* __restrict__ partial, const double* __restrict__ F,
+template <typename T>
+__global__ void __launch_bounds__(256) k_area_volume_cosine(double* __restrict__ partial, const T* __restrict__ F,
                                                             Box b, double eps, idx_t n)
 {
     KSum k[2] = {{0.0, 0.0}, {0.0, 0.0}};
     const double pi = 3.14159265358979323846;
     const idx_t stride = (idx_t)gridDim.x * blockDim.x;
     for (idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const double v = F[i];
+        const double v = (double)F[i];
         double delta = 0.0, heav;
         if (v > eps) {
             heav = 1.0;
@@ -1032,9 +1061,12 @@ __global__ void __launch_bounds__(256) k_area_volume_cosine(double* __restrict__
             const int cp = (c + 1 == b.cols) ? 0 : c + 1, cm = (c == 0) ? b.cols - 1 : c - 1;
             const int rp = (r + 1 == b.rows) ? 0 : r + 1, rm = (r == 0) ? b.rows - 1 : r - 1;
             const int pp = (p + 1 == b.pges) ? 0 : p + 1, pm = (p == 0) ? b.pges - 1 : p - 1;
-            const double fx = (F[(idx_t)p * plane + (idx_t)cp * b.rows + r] - F[(idx_t)p * plane + (idx_t)cm * b.rows + r]) / (2 * b.dx);
-            const double fy = (F[(idx_t)p * plane + (idx_t)c * b.rows + rp] - F[(idx_t)p * plane + (idx_t)c * b.rows + rm]) / (2 * b.dy);
-            const double fz = (F[(idx_t)pp * plane + (idx_t)c * b.rows + r] - F[(idx_t)pm * plane + (idx_t)c * b.rows + r]) / (2 * b.dz);
+            const double fx = ((double)F[(idx_t)p * plane + (idx_t)cp * b.rows + r] -
+                               (double)F[(idx_t)p * plane + (idx_t)cm * b.rows + r]) / (2 * b.dx);
+            const double fy = ((double)F[(idx_t)p * plane + (idx_t)c * b.rows + rp] -
+                               (double)F[(idx_t)p * plane + (idx_t)c * b.rows + rm]) / (2 * b.dy);
+            const double fz = ((double)F[(idx_t)pp * plane + (idx_t)c * b.rows + r] -
+                               (double)F[(idx_t)pm * plane + (idx_t)c * b.rows + r]) / (2 * b.dz);
             const double g = sqrt(fx * fx + fy * fy + fz * fz);
             delta = g * (1 + cos(pi * v / eps)) / eps / 2;
             heav = (1 + v / eps + sin(pi * v / eps) / pi) / 2;
@@ -1047,13 +1079,14 @@ __global__ void __launch_bounds__(256) k_area_volume_cosine(double* __restrict__
 
 // Towers delta/H area and volume (+SD/@SDF3/SDF3.m:73-80) fused with the
 // toolbox stencil: only points in the masks evaluate it.
-__global__ void __launch_bounds__(256) k_area_volume_towers(double* __restrict__ partial, const double* __restrict__ F,
+template <typename T>
+__global__ void __launch_bounds__(256) k_area_volume_towers(double* __restrict__ partial, const T* __restrict__ F,
                                                             Box b, double ds, idx_t n)
 {
     KSum k[2] = {{0.0, 0.0}, {0.0, 0.0}};
     const idx_t stride = (idx_t)gridDim.x * blockDim.x;
     for (idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const double v = F[i];
+        const double v = (double)F[i];
         const bool in_area = fabs(v) < 2 * ds, in_vol = v < 2 * ds;
         if (!(in_area || in_vol)) continue;
         const idx_t plane = (idx_t)b.rows * b.cols;
@@ -1099,13 +1132,14 @@ __global__ void __launch_bounds__(256) k_reduce_final(double* __restrict__ resul
         for (int q = 0; q < NQ; ++q) result[q] = (k[q].s + k[q].c) * scale;
 }
 
-__global__ void __launch_bounds__(256) k_max_abs3(double* __restrict__ partial, const double* __restrict__ a,
-                                                  const double* __restrict__ b2, const double* __restrict__ c3, idx_t n)
+template <typename T>
+__global__ void __launch_bounds__(256) k_max_abs3(double* __restrict__ partial, const T* __restrict__ a,
+                                                  const T* __restrict__ b2, const T* __restrict__ c3, idx_t n)
 {
     double m = 0.0;
     const idx_t stride = (idx_t)gridDim.x * blockDim.x;
     for (idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        m = dmax(m, dmax(fabs(a[i]), dmax(fabs(b2[i]), fabs(c3[i]))));
+        m = dmax(m, dmax(fabs((double)a[i]), dmax(fabs((double)b2[i]), fabs((double)c3[i]))));
 #pragma unroll
     for (int s = 32; s >= 1; s >>= 1) m = dmax(m, shfl_xor_d(m, s));
     __shared__ double lds[4];
@@ -1126,15 +1160,13 @@ __global__ void k_max_final(double* __restrict__ result, const double* __restric
     if (threadIdx.x == 0) result[0] = dmax(dmax(lds[0], lds[1]), dmax(lds[2], lds[3]));
 }
 
-__global__ void k_copy(double* __restrict__ dst, const double* __restrict__ src, idx_t n)
+template <typename T>
+__global__ void k_copy(T* __restrict__ dst, const T* __restrict__ src, idx_t n)
 {
     const idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) dst[i] = src[i];
 }
 
-// explicit instantiations
-template __global__ void k_boundary_location<6>(Dist6, const double*, Box);
-template __global__ void k_boundary_location<2>(Dist6, const double*, Box);
 template __global__ void k_reduce_final<2>(double*, const double*, int, double);
 
 }  // namespace vls
@@ -1175,17 +1207,22 @@ hipError_t launch_fill_spacing(const Dist6& d, const Geom& g, int* list_count, h
     return hipGetLastError();
 }
 
-hipError_t launch_boundary_location(int order, const Dist6& d, const double* lsf, const Geom& g, hipStream_t s)
+template <typename T>
+hipError_t launch_boundary_location(int order, const Dist6& d, const T* lsf, const Geom& g, hipStream_t s)
 {
-    if (order == 2) VLS_POINT_LAUNCH(k_boundary_location<2>, d, lsf, box_of(g));
-    VLS_POINT_LAUNCH(k_boundary_location<6>, d, lsf, box_of(g));
+    dim3 grid, block;
+    point_grid(g, grid, block);
+    if (order == 2) k_boundary_location<2, T><<<grid, block, 0, s>>>(d, lsf, box_of(g));
+    else k_boundary_location<6, T><<<grid, block, 0, s>>>(d, lsf, box_of(g));
+    return hipGetLastError();
 }
 
-hipError_t launch_min_dist_flags(double* min_dist, uint8_t* flags, const Dist6c& d, const double* mask_src,
+template <typename T>
+hipError_t launch_min_dist_flags(double* min_dist, uint8_t* flags, const Dist6c& d, const T* mask_src,
                                  const Geom& g, int* list, int* list_count, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_min_dist_flags, flat_grid(g.n()), dim3(256), 0, s, min_dist, flags, d, mask_src,
-                       box_of(g), (idx_t)g.n(), list, list_count);
+    k_min_dist_flags<T><<<flat_grid(g.n()), dim3(256), 0, s>>>(min_dist, flags, d, mask_src, box_of(g),
+                                                                 (idx_t)g.n(), list, list_count);
     return hipGetLastError();
 }
 
@@ -1210,9 +1247,10 @@ static inline int bulk_blocks(const Geom& g)
     return ((g.rows + 63) / 64) * ((g.cols + 3) / 4) * g.pges;
 }
 
-hipError_t launch_set_mask_bit(uint8_t* flags, const double* F, const Geom& g, hipStream_t s)
+template <typename T>
+hipError_t launch_set_mask_bit(uint8_t* flags, const T* F, const Geom& g, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_set_mask_bit, flat_grid(g.n()), dim3(256), 0, s, flags, F, (idx_t)g.n());
+    k_set_mask_bit<T><<<flat_grid(g.n()), dim3(256), 0, s>>>(flags, F, (idx_t)g.n());
     return hipGetLastError();
 }
 
@@ -1229,15 +1267,16 @@ hipError_t launch_eno2_re_step(double* step, const double* lsf, const uint8_t* m
 }
 
 // bulk, then the interface list and the deferred bulk points, in stream order
-hipError_t launch_reinit_stage(double* out, const double* fs, const double* fn, const uint8_t* flags,
-                               const ListView& lv, const Geom& g, int stage, int* over, int* over_count, int seq,
-                               hipStream_t s)
+template <typename T>
+hipError_t launch_reinit_stage(T* out, const T* fs, const T* fn, const uint8_t* flags, const ListView& lv,
+                               const Geom& g, int stage, int* over, int* over_count, int seq, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_reinit_bulk, dim3(bulk_blocks(g)), dim3(256), 0, s, out, fs, fn, flags, box_of(g), inv_of(g),
-                       stage, over, over_count, seq);
+    k_reinit_bulk<T><<<dim3(bulk_blocks(g)), dim3(256), 0, s>>>(out, fs, fn, flags, box_of(g), inv_of(g), stage,
+                                                                 over, over_count, seq);
     const IList il{lv.idx, lv.count, lv.dist, lv.dt, lv.cap};
-    hipLaunchKernelGGL(k_reinit_iface, dim3(iface_blocks(g)), dim3(256), 0, s, out, fs, fn, flags, il, box_of(g),
-                       inv_of(g), stage, (const int*)over, (const int*)over_count, seq);
+    k_reinit_iface<T><<<dim3(iface_blocks(g)), dim3(256), 0, s>>>(out, fs, fn, flags, il, box_of(g), inv_of(g),
+                                                                   stage, (const int*)over, (const int*)over_count,
+                                                                   seq);
     return hipGetLastError();
 }
 
@@ -1248,18 +1287,20 @@ hipError_t launch_list_gather(const ListView& lv, const Dist6c& d, const Geom& g
     return hipGetLastError();
 }
 
-hipError_t launch_list_interp(double* lcp, const ListView& lv, const double* C, const Geom& g, hipStream_t s)
+template <typename T>
+hipError_t launch_list_interp(double* lcp, const ListView& lv, const T* C, const Geom& g, hipStream_t s)
 {
     const IList il{lv.idx, lv.count, lv.dist, lv.dt, lv.cap};
-    hipLaunchKernelGGL(k_list_interp, dim3(iface_blocks(g)), dim3(256), 0, s, lcp, il, C, box_of(g));
+    k_list_interp<T><<<dim3(iface_blocks(g)), dim3(256), 0, s>>>(lcp, il, C, box_of(g));
     return hipGetLastError();
 }
 
-hipError_t launch_list_gather_vel(double* lvel, const ListView& lv, const double* vx, const double* vy,
-                                  const double* vz, const Geom& g, hipStream_t s)
+template <typename T>
+hipError_t launch_list_gather_vel(double* lvel, const ListView& lv, const T* vx, const T* vy, const T* vz,
+                                  const Geom& g, hipStream_t s)
 {
     const IList il{lv.idx, lv.count, lv.dist, lv.dt, lv.cap};
-    hipLaunchKernelGGL(k_list_gather_vel, dim3(iface_blocks(g)), dim3(256), 0, s, lvel, il, vx, vy, vz);
+    k_list_gather_vel<T><<<dim3(iface_blocks(g)), dim3(256), 0, s>>>(lvel, il, vx, vy, vz);
     return hipGetLastError();
 }
 
@@ -1269,17 +1310,19 @@ hipError_t launch_eno2_stage(double* out, const double* fs, const double* fn, co
     VLS_POINT_LAUNCH(k_eno2_stage, out, fs, fn, flags, d, box_of(g), cfl, heun2);
 }
 
-hipError_t launch_upwind_normal(double* nx, double* ny, double* nz, const double* lsf, const Dist6c& d,
-                                const Geom& g, hipStream_t s)
+template <typename T>
+hipError_t launch_upwind_normal(T* nx, T* ny, T* nz, const T* lsf, const Dist6c& d, const Geom& g, hipStream_t s)
 {
-    VLS_POINT_LAUNCH(k_upwind_normal, nx, ny, nz, lsf, d, box_of(g));
+    dim3 grid, block;
+    point_grid(g, grid, block);
+    k_upwind_normal<T><<<grid, block, 0, s>>>(nx, ny, nz, lsf, d, box_of(g));
+    return hipGetLastError();
 }
 
-hipError_t launch_extension_velocity(double* vx, double* vy, double* vz, const double* F, double floor_,
-                                     const Geom& g, hipStream_t s)
+template <typename T>
+hipError_t launch_extension_velocity(T* vx, T* vy, T* vz, const T* F, double floor_, const Geom& g, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_extension_velocity, flat_grid(g.n()), dim3(256), 0, s, vx, vy, vz, F, floor_,
-                       (idx_t)g.n());
+    k_extension_velocity<T><<<flat_grid(g.n()), dim3(256), 0, s>>>(vx, vy, vz, F, floor_, (idx_t)g.n());
     return hipGetLastError();
 }
 
@@ -1296,15 +1339,16 @@ hipError_t launch_extend_step(double* step, const double* deltat, const double* 
     VLS_POINT_LAUNCH(k_extend_step, step, deltat, C, vx, vy, vz, d, cp, box_of(g));
 }
 
-hipError_t launch_extend_stage(double* out, const double* cs, const double* cn, const double* vx,
-                               const double* vy, const double* vz, const uint8_t* flags, const ListView& lv,
-                               const double* lcp, const double* lvel, const Geom& g, int stage, hipStream_t s)
+template <typename T>
+hipError_t launch_extend_stage(T* out, const T* cs, const T* cn, const T* vx, const T* vy, const T* vz,
+                               const uint8_t* flags, const ListView& lv, const double* lcp, const double* lvel,
+                               const Geom& g, int stage, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_extend_bulk, dim3(bulk_blocks(g)), dim3(256), 0, s, out, cs, cn, vx, vy, vz, flags, box_of(g),
-                       inv_of(g), stage);
+    k_extend_bulk<T><<<dim3(bulk_blocks(g)), dim3(256), 0, s>>>(out, cs, cn, vx, vy, vz, flags, box_of(g),
+                                                                 inv_of(g), stage);
     const IList il{lv.idx, lv.count, lv.dist, lv.dt, lv.cap};
-    hipLaunchKernelGGL(k_extend_iface, dim3(iface_blocks(g)), dim3(256), 0, s, out, cs, cn, il, lcp, lvel, box_of(g),
-                       inv_of(g), stage);
+    k_extend_iface<T><<<dim3(iface_blocks(g)), dim3(256), 0, s>>>(out, cs, cn, il, lcp, lvel, box_of(g), inv_of(g),
+                                                                   stage);
     return hipGetLastError();
 }
 
@@ -1314,19 +1358,23 @@ hipError_t launch_advection_step(double* astep, const double* vx, const double* 
     VLS_POINT_LAUNCH(k_advection_step, astep, vx, vy, vz, lsf, box_of(g), inv_of(g));
 }
 
-hipError_t launch_advect_stage(double* out, const double* fs, const double* fn, const double* vx,
-                               const double* vy, const double* vz, double dt, const Geom& g, int stage,
-                               hipStream_t s)
+template <typename T>
+hipError_t launch_advect_stage(T* out, const T* fs, const T* fn, const T* vx, const T* vy, const T* vz, double dt,
+                               const Geom& g, int stage, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_advect_stage, dim3(bulk_blocks(g)), dim3(256), 0, s, out, fs, fn, vx, vy, vz, dt, box_of(g),
-                       inv_of(g), stage);
+    k_advect_stage<T><<<dim3(bulk_blocks(g)), dim3(256), 0, s>>>(out, fs, fn, vx, vy, vz, dt, box_of(g), inv_of(g),
+                                                                  stage);
     return hipGetLastError();
 }
 
-hipError_t launch_normal_velocity(double* out, const double* F, const double* vx, const double* vy, const double* vz,
-                                  const Geom& g, hipStream_t s)
+template <typename T>
+hipError_t launch_normal_velocity(T* out, const T* F, const T* vx, const T* vy, const T* vz, const Geom& g,
+                                  hipStream_t s)
 {
-    VLS_POINT_LAUNCH(k_normal_velocity, out, F, vx, vy, vz, box_of(g));
+    dim3 grid, block;
+    point_grid(g, grid, block);
+    k_normal_velocity<T><<<grid, block, 0, s>>>(out, F, vx, vy, vz, box_of(g));
+    return hipGetLastError();
 }
 
 hipError_t launch_weno_derivative(const Dist6& out, const double* lsf, const Geom& g, hipStream_t s)
@@ -1349,37 +1397,72 @@ static inline int blocks_for(int64_t n)
     return (int)(want < reduce_blocks() ? (want < 1 ? 1 : want) : reduce_blocks());
 }
 
-hipError_t launch_area_volume_cosine(double* result, double* partial, const double* F, const Geom& g, double eps,
+template <typename T>
+hipError_t launch_area_volume_cosine(double* result, double* partial, const T* F, const Geom& g, double eps,
                                      double scale, hipStream_t s)
 {
     const int nb = blocks_for(g.n());
-    hipLaunchKernelGGL(k_area_volume_cosine, dim3(nb), dim3(256), 0, s, partial, F, box_of(g), eps, (idx_t)g.n());
+    k_area_volume_cosine<T><<<dim3(nb), dim3(256), 0, s>>>(partial, F, box_of(g), eps, (idx_t)g.n());
     hipLaunchKernelGGL(k_reduce_final<2>, dim3(1), dim3(256), 0, s, result, partial, nb, scale);
     return hipGetLastError();
 }
 
-hipError_t launch_area_volume_towers(double* result, double* partial, const double* F, const Geom& g, double ds,
+template <typename T>
+hipError_t launch_area_volume_towers(double* result, double* partial, const T* F, const Geom& g, double ds,
                                      double scale, hipStream_t s)
 {
     const int nb = blocks_for(g.n());
-    hipLaunchKernelGGL(k_area_volume_towers, dim3(nb), dim3(256), 0, s, partial, F, box_of(g), ds, (idx_t)g.n());
+    k_area_volume_towers<T><<<dim3(nb), dim3(256), 0, s>>>(partial, F, box_of(g), ds, (idx_t)g.n());
     hipLaunchKernelGGL(k_reduce_final<2>, dim3(1), dim3(256), 0, s, result, partial, nb, scale);
     return hipGetLastError();
 }
 
-hipError_t launch_max_abs3(double* result, double* partial, const double* a, const double* b, const double* c,
-                           const Geom& g, hipStream_t s)
+template <typename T>
+hipError_t launch_max_abs3(double* result, double* partial, const T* a, const T* b, const T* c, const Geom& g,
+                           hipStream_t s)
 {
     const int nb = blocks_for(g.n());
-    hipLaunchKernelGGL(k_max_abs3, dim3(nb), dim3(256), 0, s, partial, a, b, c, (idx_t)g.n());
+    k_max_abs3<T><<<dim3(nb), dim3(256), 0, s>>>(partial, a, b, c, (idx_t)g.n());
     hipLaunchKernelGGL(k_max_final, dim3(1), dim3(256), 0, s, result, partial, nb);
     return hipGetLastError();
 }
 
-hipError_t launch_copy(double* dst, const double* src, int64_t n, hipStream_t s)
+template <typename T>
+hipError_t launch_copy(T* dst, const T* src, int64_t n, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_copy, flat_grid(n), dim3(256), 0, s, dst, src, (idx_t)n);
+    k_copy<T><<<flat_grid(n), dim3(256), 0, s>>>(dst, src, (idx_t)n);
     return hipGetLastError();
 }
+
+// instantiations for FP64 fields and FP32 (bulk) fields
+#define VLS_INSTANTIATE(T)                                                                                          \
+    template hipError_t launch_boundary_location<T>(int, const Dist6&, const T*, const Geom&, hipStream_t);         \
+    template hipError_t launch_min_dist_flags<T>(double*, uint8_t*, const Dist6c&, const T*, const Geom&, int*,     \
+                                                 int*, hipStream_t);                                                \
+    template hipError_t launch_set_mask_bit<T>(uint8_t*, const T*, const Geom&, hipStream_t);                       \
+    template hipError_t launch_reinit_stage<T>(T*, const T*, const T*, const uint8_t*, const ListView&, const Geom&, \
+                                               int, int*, int*, int, hipStream_t);                                  \
+    template hipError_t launch_list_interp<T>(double*, const ListView&, const T*, const Geom&, hipStream_t);        \
+    template hipError_t launch_list_gather_vel<T>(double*, const ListView&, const T*, const T*, const T*,          \
+                                                  const Geom&, hipStream_t);                                        \
+    template hipError_t launch_upwind_normal<T>(T*, T*, T*, const T*, const Dist6c&, const Geom&, hipStream_t);     \
+    template hipError_t launch_extension_velocity<T>(T*, T*, T*, const T*, double, const Geom&, hipStream_t);       \
+    template hipError_t launch_extend_stage<T>(T*, const T*, const T*, const T*, const T*, const T*,               \
+                                               const uint8_t*, const ListView&, const double*, const double*,       \
+                                               const Geom&, int, hipStream_t);                                      \
+    template hipError_t launch_advect_stage<T>(T*, const T*, const T*, const T*, const T*, const T*, double,        \
+                                               const Geom&, int, hipStream_t);                                      \
+    template hipError_t launch_normal_velocity<T>(T*, const T*, const T*, const T*, const T*, const Geom&,          \
+                                                  hipStream_t);                                                     \
+    template hipError_t launch_area_volume_cosine<T>(double*, double*, const T*, const Geom&, double, double,       \
+                                                     hipStream_t);                                                  \
+    template hipError_t launch_area_volume_towers<T>(double*, double*, const T*, const Geom&, double, double,       \
+                                                     hipStream_t);                                                  \
+    template hipError_t launch_max_abs3<T>(double*, double*, const T*, const T*, const T*, const Geom&, hipStream_t); \
+    template hipError_t launch_copy<T>(T*, const T*, int64_t, hipStream_t);
+
+VLS_INSTANTIATE(double)
+VLS_INSTANTIATE(float)
+#undef VLS_INSTANTIATE
 
 }  // namespace vls

@@ -55,14 +55,20 @@ struct Geom {
 
 // pre-fill with the spacings; also zeroes *list_count (may be null)
 hipError_t launch_fill_spacing(const Dist6& d, const Geom& g, int* list_count, hipStream_t s);
-hipError_t launch_boundary_location(int order, const Dist6& d, const double* lsf, const Geom& g, hipStream_t s);
+// Launchers templated on T take fields of T = double or float (FP32 bulk: the
+// stage kernels' bulk arithmetic is FP32; subcell distances, interface-list
+// arithmetic and reductions stay FP64).
+template <typename T>
+hipError_t launch_boundary_location(int order, const Dist6& d, const T* lsf, const Geom& g, hipStream_t s);
 // min_dist (may be null), flags (may be null) and the list of points with a
 // subcell distance (list may be null; *list_count must be 0 on entry)
-hipError_t launch_min_dist_flags(double* min_dist, uint8_t* flags, const Dist6c& d, const double* mask_src,
+template <typename T>
+hipError_t launch_min_dist_flags(double* min_dist, uint8_t* flags, const Dist6c& d, const T* mask_src,
                                  const Geom& g, int* list, int* list_count, hipStream_t s);
 hipError_t launch_zero_ints(int* a, int n, hipStream_t s);
 int iface_blocks(const Geom& g);
-hipError_t launch_set_mask_bit(uint8_t* flags, const double* F, const Geom& g, hipStream_t s);
+template <typename T>
+hipError_t launch_set_mask_bit(uint8_t* flags, const T* F, const Geom& g, hipStream_t s);
 hipError_t launch_re_step(double* step, const double* lsf, const uint8_t* mask, const double* deltat,
                           const Dist6c& d, const Geom& g, hipStream_t s);
 hipError_t launch_eno2_re_step(double* step, const double* lsf, const uint8_t* mask, const double* deltat,
@@ -70,49 +76,57 @@ hipError_t launch_eno2_re_step(double* step, const double* lsf, const uint8_t* m
 // one WENO5-RK3 reinitialization stage (two launches); `seq` numbers the
 // stages of one reinitialization call from 0, over_count holds 2 ints that
 // are zero before seq 0
-hipError_t launch_reinit_stage(double* out, const double* fs, const double* fn, const uint8_t* flags,
-                               const ListView& lv, const Geom& g, int stage, int* over, int* over_count, int seq,
-                               hipStream_t s);
+template <typename T>
+hipError_t launch_reinit_stage(T* out, const T* fs, const T* fn, const uint8_t* flags, const ListView& lv,
+                               const Geom& g, int stage, int* over, int* over_count, int seq, hipStream_t s);
 // fill the list-ordered distances and steps of `lv` from the full arrays
 hipError_t launch_list_gather(const ListView& lv, const Dist6c& d, const Geom& g, hipStream_t s);
 // interface values of C at the listed points, list order (6 rows of cap)
-hipError_t launch_list_interp(double* lcp, const ListView& lv, const double* C, const Geom& g, hipStream_t s);
+template <typename T>
+hipError_t launch_list_interp(double* lcp, const ListView& lv, const T* C, const Geom& g, hipStream_t s);
 // velocity at the listed points, list order (3 rows of cap)
-hipError_t launch_list_gather_vel(double* lvel, const ListView& lv, const double* vx, const double* vy,
-                                  const double* vz, const Geom& g, hipStream_t s);
+template <typename T>
+hipError_t launch_list_gather_vel(double* lvel, const ListView& lv, const T* vx, const T* vy, const T* vz,
+                                  const Geom& g, hipStream_t s);
 hipError_t launch_eno2_stage(double* out, const double* fs, const double* fn, const uint8_t* flags,
                              const Dist6c& d, const Geom& g, double cfl, int heun2, hipStream_t s);
-hipError_t launch_upwind_normal(double* nx, double* ny, double* nz, const double* lsf, const Dist6c& d,
-                                const Geom& g, hipStream_t s);
-hipError_t launch_extension_velocity(double* vx, double* vy, double* vz, const double* F, double floor_,
-                                     const Geom& g, hipStream_t s);
+template <typename T>
+hipError_t launch_upwind_normal(T* nx, T* ny, T* nz, const T* lsf, const Dist6c& d, const Geom& g, hipStream_t s);
+template <typename T>
+hipError_t launch_extension_velocity(T* vx, T* vy, T* vz, const T* F, double floor_, const Geom& g, hipStream_t s);
 hipError_t launch_boundary_interpolant(const Dist6& cp, const Dist6c& d, const double* C, const Geom& g,
                                        hipStream_t s);
 hipError_t launch_extend_step(double* step, const double* deltat, const double* C, const double* vx,
                               const double* vy, const double* vz, const Dist6c& d, const Dist6c& cp,
                               const Geom& g, hipStream_t s);
-hipError_t launch_extend_stage(double* out, const double* cs, const double* cn, const double* vx,
-                               const double* vy, const double* vz, const uint8_t* flags, const ListView& lv,
-                               const double* lcp, const double* lvel, const Geom& g, int stage, hipStream_t s);
+template <typename T>
+hipError_t launch_extend_stage(T* out, const T* cs, const T* cn, const T* vx, const T* vy, const T* vz,
+                               const uint8_t* flags, const ListView& lv, const double* lcp, const double* lvel,
+                               const Geom& g, int stage, hipStream_t s);
 hipError_t launch_advection_step(double* astep, const double* vx, const double* vy, const double* vz,
                                  const double* lsf, const Geom& g, hipStream_t s);
-hipError_t launch_advect_stage(double* out, const double* fs, const double* fn, const double* vx,
-                               const double* vy, const double* vz, double dt, const Geom& g, int stage,
-                               hipStream_t s);
+template <typename T>
+hipError_t launch_advect_stage(T* out, const T* fs, const T* fn, const T* vx, const T* vy, const T* vz, double dt,
+                               const Geom& g, int stage, hipStream_t s);
 hipError_t launch_weno_derivative(const Dist6& out, const double* lsf, const Geom& g, hipStream_t s);
 hipError_t launch_set_calculus_toolbox(const Fields18& o, const double* lsf, const double* hp, const Geom& g,
                                        double ds, hipStream_t s);
 // reductions: `partial` holds at least reduce_blocks()*4 doubles, `result` 2
 int reduce_blocks();
 // result[0..1] = (area, volume) sums times `scale`
-hipError_t launch_area_volume_cosine(double* result, double* partial, const double* F, const Geom& g, double eps,
+template <typename T>
+hipError_t launch_area_volume_cosine(double* result, double* partial, const T* F, const Geom& g, double eps,
                                      double scale, hipStream_t s);
-hipError_t launch_area_volume_towers(double* result, double* partial, const double* F, const Geom& g, double ds,
+template <typename T>
+hipError_t launch_area_volume_towers(double* result, double* partial, const T* F, const Geom& g, double ds,
                                      double scale, hipStream_t s);
-hipError_t launch_normal_velocity(double* out, const double* F, const double* vx, const double* vy, const double* vz,
-                                  const Geom& g, hipStream_t s);
-hipError_t launch_max_abs3(double* result, double* partial, const double* a, const double* b, const double* c,
-                           const Geom& g, hipStream_t s);
-hipError_t launch_copy(double* dst, const double* src, int64_t n, hipStream_t s);
+template <typename T>
+hipError_t launch_normal_velocity(T* out, const T* F, const T* vx, const T* vy, const T* vz, const Geom& g,
+                                  hipStream_t s);
+template <typename T>
+hipError_t launch_max_abs3(double* result, double* partial, const T* a, const T* b, const T* c, const Geom& g,
+                           hipStream_t s);
+template <typename T>
+hipError_t launch_copy(T* dst, const T* src, int64_t n, hipStream_t s);
 
 }  // namespace vls

@@ -22,14 +22,17 @@ Synthetic inputs (all seeded, float64):
     a_m ~ N(0, amp^2)^3, psi_m ~ U[0, 2pi), drawn by numpy.random.default_rng(seed);
   * psi0 = x - 0.2 cos(2 pi y);
   * dt = cfl * dx / max|u| (the CFL convention of p00_test_singlephase_relax.m:89).
+
+precision "fp32" runs the same step on float32 fields through the FP32-bulk
+entry points; the initial reinitialization, max|u| and dt are computed in
+FP64 first, so both precisions start from the same state (rounded once).
 """
 from dataclasses import dataclass, field
 
 import numpy as np
 import torch
 
-from .sdf3 import GD3, SDF3, _ptr, _stream
-from ._lib import call
+from .sdf3 import GD3, SDF3
 
 
 @dataclass
@@ -49,6 +52,7 @@ class PipelineConfig:
     redist_iters: tuple = (10, 5)
     initial_reinit: int = 100
     eps_factor: float = 1.5
+    precision: str = "fp64"  # or "fp32" (FP32 bulk)
 
 
 def curl_velocity(X, Y, Z, modes, amp, kmax, seed):
@@ -117,18 +121,24 @@ class Pipeline:
         self.max_speed = self.map.maxSpeed(*self.u)
         self.dt = cfg.cfl * self.grid.Dx / self.max_speed
         self.eps = cfg.eps_factor * self.grid.Dx
-        self.Vn = self.grid.zeros()
+        if cfg.precision == "fp32":
+            F = self.map.F.float()
+            del self.map
+            self.map = SDF3(self.grid, F)
+            self.u = [c.float() for c in self.u]
+            self.psi = self.psi.float()
+        elif cfg.precision != "fp64":
+            raise ValueError(f"precision must be fp64 or fp32, got {cfg.precision}")
+        self.Vn = self.grid.zeros(self.map.dtype)
         self.area_volume = None
         self.steps_done = 0
         self.ext_events = []  # (start, end) event pairs around the extension, when recording
 
     def _normal_velocity(self, out):
-        m = self.map
-        call("vls_sdf3_normal_velocity", m._h, _ptr(m.F), *[_ptr(c) for c in self.u], _ptr(out), _stream())
+        self.map.normalVelocity(*self.u, out)
 
     def _area_volume(self, out_row):
-        m = self.map
-        call("vls_sdf3_area_volume_device", m._h, _ptr(m.F), 1, float(self.eps), _ptr(out_row), _stream())
+        self.map.areaVolumeDevice(out_row, 1, self.eps)
 
     def run(self, steps, record_extension=False):
         """Advance `steps` steps; returns the WENO updates done (full-box passes x points)."""

@@ -8,6 +8,10 @@ that is the reference's column-major [rows, cols, pges] array
 (ind = p*rows*cols + c*rows + r).  torch supplies device memory and the
 stream; all arithmetic runs in the HIP library.
 
+An SDF3 built on a float32 F runs the FP32-bulk variants of the methods
+(the *_f32 entry points): fields are float32, the subcell distances, the
+interface points and the reductions are FP64.
+
 The module-level `feval_*` functions are the kernel-level drop-ins: the
 reference's feval(kernel, args...) with the same positional arguments, writing
 the non-const arguments in place.
@@ -72,8 +76,8 @@ class GD3:
         Z, X, Y = torch.meshgrid(self.zv, self.xv, self.yv, indexing="ij")
         return (X.contiguous().to(self.device), Y.contiguous().to(self.device), Z.contiguous().to(self.device))
 
-    def zeros(self):
-        return torch.zeros(self.Size, dtype=torch.float64, device=self.device)
+    def zeros(self, dtype=torch.float64):
+        return torch.zeros(self.Size, dtype=dtype, device=self.device)
 
 
 class SDF3:
@@ -83,6 +87,9 @@ class SDF3:
     work fields of the iteration loops) through one C-ABI handle."""
 
     def __init__(self, grid, F):
+        if F.dtype not in (torch.float64, torch.float32):
+            raise VLSError(f"F must be float64 or float32, got {F.dtype}")
+        self.dtype = F.dtype
         self.GD3 = grid
         self._h = ctypes.c_void_p()
         call("vls_sdf3_create", ctypes.byref(self._h), grid.mrows, grid.ncols, grid.lshts, grid.Dx, grid.Dy,
@@ -105,11 +112,15 @@ class SDF3:
     def _check(self, t):
         if tuple(t.shape) != self.GD3.Size:
             raise VLSError(f"field shape {tuple(t.shape)} != grid {self.GD3.Size}")
-        return _ptr(t)
+        return _ptr(t, self.dtype)
+
+    def _fn(self, name):
+        """the entry point of `name` for this object's field type"""
+        return name + "_f32" if self.dtype == torch.float32 else name
 
     # -- setDistance (+SD/@SDF3/setDistance.m:3-21) --------------------------
     def setDistance(self):
-        call("vls_sdf3_set_distance", self._h, self._check(self.F), _stream())
+        call(self._fn("vls_sdf3_set_distance"), self._h, self._check(self.F), _stream())
 
     def distances(self):
         """copies of (xpr, xpl, ypf, ypb, zpu, zpd, min_dist) of the last setDistance"""
@@ -127,12 +138,14 @@ class SDF3:
     def WENO5RK3Reinitialization(self, F, iteration):
         """+SD/@SDF3/WENO5RK3Reinitialization.m:3-43: mask = F<0, distances of the last setDistance."""
         NewF = F.clone()
-        call("vls_sdf3_weno5_rk3_reinit", self._h, self._check(NewF), int(iteration), _stream())
+        call(self._fn("vls_sdf3_weno5_rk3_reinit"), self._h, self._check(NewF), int(iteration), _stream())
         return NewF
 
     def ENORK2Reinitialization(self, F, iteration, rk2=True, cfl=0.3):
         """+SD/@SDF3/ENORK2Reinitialization.m:3-45 (rk2=True, cfl=0.3 is the reference;
-        rk2=False is forward Euler)."""
+        rk2=False is forward Euler).  FP64 only."""
+        if self.dtype != torch.float64:
+            raise VLSError("ENORK2Reinitialization is FP64 only")
         NewF = F.clone()
         call("vls_sdf3_eno2_reinit", self._h, self._check(NewF), int(iteration), 1 if rk2 else 0, float(cfl),
              _stream())
@@ -142,15 +155,15 @@ class SDF3:
     def WENO5RK3Extend(self, C, iteration):
         """+SD/@SDF3/WENO5RK3Extend.m:9-76 (self.F and its last setDistance define the interface)."""
         NewC = C.clone()
-        call("vls_sdf3_weno5_rk3_extend", self._h, self._check(self.F), self._check(NewC), int(iteration),
-             _stream())
+        call(self._fn("vls_sdf3_weno5_rk3_extend"), self._h, self._check(self.F), self._check(NewC),
+             int(iteration), _stream())
         return NewC
 
     def WENO5RK3ClosetPointSurfaceRedistance(self, A, iter1, iter2):
         """+SD/@SDF3/WENO5RK3ClosetPointSurfaceRedistance.m:4-157"""
         NewA = A.clone()
-        call("vls_sdf3_surface_redistance", self._h, self._check(self.F), self._check(NewA), int(iter1),
-             int(iter2), _stream())
+        call(self._fn("vls_sdf3_surface_redistance"), self._h, self._check(self.F), self._check(NewA),
+             int(iter1), int(iter2), _stream())
         return NewA
 
     # -- advection -------------------------------------------------------------
@@ -158,19 +171,21 @@ class SDF3:
         """TVD-RK3 (Shu-Osher form of +SD/@SDF3/WENORK3SurfaceConservationLaw.m:10-29) of
         F_t + v.grad F = 0 with the WENO5 operator of advection_step.cu:30-56."""
         NewF = F.clone()
-        call("vls_sdf3_advect_rk3", self._h, self._check(NewF), self._check(vx), self._check(vy),
+        call(self._fn("vls_sdf3_advect_rk3"), self._h, self._check(NewF), self._check(vx), self._check(vy),
              self._check(vz), float(dt), int(steps), _stream())
         return NewF
 
     def maxSpeed(self, vx, vy, vz):
         out = ctypes.c_double()
-        call("vls_sdf3_max_speed", self._h, self._check(vx), self._check(vy), self._check(vz),
+        call(self._fn("vls_sdf3_max_speed"), self._h, self._check(vx), self._check(vy), self._check(vz),
              ctypes.byref(out), _stream())
         return out.value
 
     # -- calculus toolbox, area and volume --------------------------------------
     def GPUsetCalculusToolBox(self):
-        """+SD/@SDF3/GPUsetCalculusToolBox.m:2-40 (HPrimal = max(F,0))."""
+        """+SD/@SDF3/GPUsetCalculusToolBox.m:2-40 (HPrimal = max(F,0)).  FP64 only."""
+        if self.dtype != torch.float64:
+            raise VLSError("GPUsetCalculusToolBox is FP64 only")
         g = self.GD3
         self.HPrimal = torch.clamp(self.F, min=0.0).contiguous()
         outs = [g.zeros() for _ in TOOLBOX_FIELDS]
@@ -181,10 +196,33 @@ class SDF3:
     def calAreaVolume(self):
         """(calArea, calVolume) of +SD/@SDF3/SDF3.m:73-80 with the Towers delta/H,
         fused with the toolbox stencil (no toolbox fields are written)."""
+        if self.dtype == torch.float32:
+            return self._area_volume_f32(0, 0.0)
         a, v = ctypes.c_double(), ctypes.c_double()
         call("vls_sdf3_area_volume_towers", self._h, self._check(self.F), ctypes.byref(a), ctypes.byref(v),
              _stream())
         return a.value, v.value
+
+    def _area_volume_f32(self, kind, eps):
+        out = torch.empty(2, dtype=torch.float64, device=self.F.device)
+        call("vls_sdf3_area_volume_device_f32", self._h, self._check(self.F), int(kind), float(eps), _ptr(out),
+             _stream())
+        a, v = out.tolist()
+        return a, v
+
+    def areaVolumeDevice(self, out2, kind=1, eps=None):
+        """(area, volume) written to the float64 device tensor out2[0:2] without a
+        host synchronization; kind 0 = Towers delta/H, 1 = cosine (eps = 1.5*Dx)."""
+        if eps is None:
+            eps = 1.5 * self.GD3.Dx
+        call(self._fn("vls_sdf3_area_volume_device"), self._h, self._check(self.F), int(kind), float(eps),
+             _ptr(out2), _stream())
+
+    def normalVelocity(self, vx, vy, vz, out):
+        """u.N with N = grad F/max(|grad F|, 1e-14) (central differences), into `out`"""
+        call(self._fn("vls_sdf3_normal_velocity"), self._h, self._check(self.F), self._check(vx),
+             self._check(vy), self._check(vz), self._check(out), _stream())
+        return out
 
     def calArea(self):
         return self.calAreaVolume()[0]
@@ -202,6 +240,8 @@ class SDF3:
         src/0_0_setup/+SD/@SDF3/SDF3.m:126-136,178-187; eps defaults to 1.5*Dx (:59)."""
         if eps is None:
             eps = 1.5 * self.GD3.Dx
+        if self.dtype == torch.float32:
+            return self._area_volume_f32(1, eps)
         a, v = ctypes.c_double(), ctypes.c_double()
         call("vls_sdf3_area_volume_cosine", self._h, self._check(self.F), float(eps), ctypes.byref(a),
              ctypes.byref(v), _stream())

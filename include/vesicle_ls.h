@@ -176,6 +176,35 @@ int vls_sdf3_area_volume_device(vls_sdf3_t h, const double *F, int kind, double 
 int vls_sdf3_normal_velocity(vls_sdf3_t h, const double *F, const double *vx, const double *vy, const double *vz,
                              double *out, vls_stream_t stream);
 
+/* ------------------------------------------------------------------------ */
+/* 3. FP32-bulk variants of the handle methods                              */
+/* ------------------------------------------------------------------------ */
+/* The same methods on float fields (same handle, same semantics).  The
+ * reference computes everything in FP64; these entry points have no
+ * counterpart there.  What stays FP64:
+ *   - the subcell crossing distances (the sixth-order locator and its root
+ *     solve read F converted to double and store double distances);
+ *   - every interface-list point (a point with a subcell distance, and in the
+ *     reinitialization a point next to a sign change): its nonuniform WENO,
+ *     Godunov term and RK combination run in FP64 on the float values;
+ *   - the interface values and velocities of the extension (list order);
+ *   - the area/volume and max-speed reductions (compensated FP64 sums).
+ * The bulk (every other point) runs its WENO5, upwinding, Godunov term and RK
+ * combination in FP32, storing float results. */
+int vls_sdf3_set_distance_f32(vls_sdf3_t h, const float *F, vls_stream_t stream);
+int vls_sdf3_weno5_rk3_reinit_f32(vls_sdf3_t h, float *F, int iterations, vls_stream_t stream);
+int vls_sdf3_weno5_rk3_extend_f32(vls_sdf3_t h, const float *F, float *C, int iterations, vls_stream_t stream);
+int vls_sdf3_surface_redistance_f32(vls_sdf3_t h, const float *F, float *A, int iter1, int iter2,
+                                    vls_stream_t stream);
+int vls_sdf3_advect_rk3_f32(vls_sdf3_t h, float *F, const float *vx, const float *vy, const float *vz, double dt,
+                            int steps, vls_stream_t stream);
+int vls_sdf3_max_speed_f32(vls_sdf3_t h, const float *vx, const float *vy, const float *vz, double *out,
+                           vls_stream_t stream);
+int vls_sdf3_area_volume_device_f32(vls_sdf3_t h, const float *F, int kind, double eps, double *out2,
+                                    vls_stream_t stream);
+int vls_sdf3_normal_velocity_f32(vls_sdf3_t h, const float *F, const float *vx, const float *vy, const float *vz,
+                                 float *out, vls_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif
