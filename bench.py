@@ -25,6 +25,10 @@ of `bench.py --probe extend`, calibrated on a 256 MiB copy
 
 cpu_baseline: the CPU oracle (OpenMP, FP64) on one full pipeline step of the
 same workload, timed on this host.
+
+Beside the main line: config1 (64^3 ENO2 forward-Euler reinitialization with
+its error measures) and config3 (the pipeline at 256^3 with a 32-mode
+velocity, FP32 bulk vs FP64, 200 steps each, with the FP32 run's errors).
 """
 import argparse
 import json
@@ -46,6 +50,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--precision-steps", type=int, default=200,
+                    help="steps of the 256^3 FP32-bulk vs FP64 comparison (0 skips it)")
     ap.add_argument("--probe", choices=["extend"], default=None,
                     help="run only a few extension stages and a 256 MiB device copy (for rocprofv3 --pmc)")
     return ap.parse_args()
@@ -163,6 +169,20 @@ def config1(vls, with_cpu):
     return out
 
 
+def config3(steps):
+    """Config 3 beside the main line: the same vesicle-step pipeline at 256^3
+    with a 32-mode velocity, FP32 bulk vs FP64 from one seed and initial
+    state, `steps` steps each; the FP32 run's errors against the FP64 run
+    (which reproduces the CPU oracle bit for bit) and the speedup."""
+    import importlib
+
+    diag = importlib.import_module("2018_7_vesicle_amd.diagnostics")
+    r = diag.precision_comparison(256, 32, steps, conditioning=True)
+    r["workload"] = (f"256^3 vesicle step, 32-mode curl velocity (seed 2018), {steps} steps, FP32 bulk vs FP64; "
+                     "fp64_rounded_start: FP64 from the FP32-rounded initial state (conditioning floor)")
+    return r
+
+
 def main():
     args = parse()
     if args.probe == "extend":
@@ -245,6 +265,8 @@ def main():
         },
     }
     out["config1"] = config1(vls, not args.no_cpu)
+    if args.precision_steps > 0:
+        out["config3"] = config3(args.precision_steps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg)
     print(json.dumps(out), flush=True)
