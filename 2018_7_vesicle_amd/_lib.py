@@ -54,7 +54,33 @@ SIGNATURES = {
     "vls_sdf3_max_speed_f32": [P, P, P, P, PD, P],
     "vls_sdf3_area_volume_device_f32": [P, P, I, D, P, P],
     "vls_sdf3_normal_velocity_f32": [P] * 7,
+    "vls_set_graphs": [I],
+    "vls_batch_create": [P, I, P, P, P, P, P, P],
+    "vls_batch_destroy": [P],
+    "vls_batch_layout": [P, P, P],
+    "vls_batch_set_blocks": [P, I],
+    "vls_batch_get_blocks": [P, P],
+    "vls_batch_set_tile_rows": [P, I],
+    "vls_batch_get_tile_rows": [P, P],
+    "vls_sdf3_set_tile_rows": [P, I],
+    "vls_batch_autotune": [P, P, I, P, P, P],
+    "vls_batch_autotune_f32": [P, P, I, P, P, P],
+    "vls_sdf3_autotune": [P, P, I, P, P],
+    "vls_sdf3_autotune_f32": [P, P, I, P, P],
 }
+for _sfx in ("", "_f32"):
+    SIGNATURES.update({
+        "vls_batch_set_distance" + _sfx: [P, P, P],
+        "vls_batch_weno5_rk3_reinit" + _sfx: [P, P, I, P],
+        "vls_batch_weno5_rk3_extend" + _sfx: [P, P, P, I, P],
+        "vls_batch_advect_rk3" + _sfx: [P, P, P, P, P, P, I, P],
+        "vls_batch_normal_velocity" + _sfx: [P] * 7,
+        "vls_batch_area_volume_cosine" + _sfx: [P, P, D, P, P],
+        "vls_batch_cfl_dt" + _sfx: [P, P, P, P, D, P, P],
+        "vls_batch_curl_velocity" + _sfx: [P, P, I, P, P, P, P, P],
+        "vls_batch_ellipsoid" + _sfx: [P, P, P, P, P],
+        "vls_batch_step" + _sfx: [P, P, P, P, P, P, P, I, I, D, P, P],
+    })
 
 _lib = None
 

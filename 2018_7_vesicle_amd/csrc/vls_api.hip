@@ -2,10 +2,13 @@
 // iteration loops.  Every loop issues its stage kernels back to back on the
 // caller's stream; nothing here synchronizes except the calls that return a
 // host scalar.
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
+#include <vector>
 
 #include "../../include/vesicle_ls.h"
 #include "vls_kernels.hpp"
@@ -74,6 +77,102 @@ Dist6c mk6c(const double* a, const double* b, const double* c, const double* d, 
     return Dist6c{{a, c, e}, {b, d, f}};
 }
 
+// ---------------------------------------------------------------------------
+// captured graphs.  A wrapper-level call (a whole reinitialization or
+// extension loop, a surface redistance, a batch step) is captured once into a
+// hipGraph on a private stream and replayed on the caller's stream after
+// that.  The graph holds the call's device pointers and counts, so the cache
+// is keyed by them; it keeps the 8 most recently used graphs per handle.  The
+// captured work has no memset nodes: counters are cleared by kernels.
+// ---------------------------------------------------------------------------
+int g_graphs = 1;  // vls_set_graphs
+
+struct GraphKey {
+    uint64_t v[16] = {};
+    bool operator==(const GraphKey& o) const { return std::memcmp(v, o.v, sizeof(v)) == 0; }
+};
+
+template <typename A>
+uint64_t key_word(A a)
+{
+    if constexpr (std::is_pointer_v<A>) return (uint64_t)(uintptr_t)a;
+    else if constexpr (std::is_floating_point_v<A>) {
+        const double d = (double)a;
+        uint64_t u;
+        std::memcpy(&u, &d, sizeof(u));
+        return u;
+    } else return (uint64_t)(int64_t)a;
+}
+
+template <typename... A>
+GraphKey make_key(A... a)
+{
+    static_assert(sizeof...(A) <= 16, "graph key too long");
+    GraphKey k;
+    int i = 0;
+    ((k.v[i++] = key_word(a)), ...);
+    return k;
+}
+
+struct GraphCache {
+    struct Entry {
+        GraphKey key;
+        hipGraphExec_t exec;
+        uint64_t used;
+    };
+    std::vector<Entry> entries;
+    hipStream_t cap = nullptr;
+    uint64_t tick = 0;
+    void clear()
+    {
+        for (auto& e : entries) (void)hipGraphExecDestroy(e.exec);
+        entries.clear();
+    }
+    ~GraphCache()
+    {
+        clear();
+        if (cap) (void)hipStreamDestroy(cap);
+    }
+};
+
+// run `issue(stream)` through the cache: replay the graph of `key`, or
+// capture issue's launches into one and replay it
+template <class Issue>
+int with_graph(GraphCache& gc, const GraphKey& key, hipStream_t s, Issue&& issue)
+{
+    if (!g_graphs) return issue(s);
+    for (auto& e : gc.entries)
+        if (e.key == key) {
+            e.used = ++gc.tick;
+            VLS_HIP(hipGraphLaunch(e.exec, s));
+            return VLS_OK;
+        }
+    if (!gc.cap) VLS_HIP(hipStreamCreateWithFlags(&gc.cap, hipStreamNonBlocking));
+    VLS_HIP(hipStreamBeginCapture(gc.cap, hipStreamCaptureModeThreadLocal));
+    const int rc = issue(gc.cap);
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(gc.cap, &graph);
+    if (rc != VLS_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (ec != hipSuccess) return fail(VLS_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ec));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) return fail(VLS_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+    if (gc.entries.size() >= 8) {
+        size_t lru = 0;
+        for (size_t i = 1; i < gc.entries.size(); ++i)
+            if (gc.entries[i].used < gc.entries[lru].used) lru = i;
+        (void)hipGraphExecDestroy(gc.entries[lru].exec);
+        gc.entries.erase(gc.entries.begin() + (long)lru);
+    }
+    gc.entries.push_back({key, exec, ++gc.tick});
+    VLS_HIP(hipGraphLaunch(exec, s));
+    return VLS_OK;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -104,6 +203,7 @@ struct vls_sdf3 {
     double* t[4] = {nullptr, nullptr, nullptr, nullptr};
     double* partial = nullptr;
     double* result = nullptr;
+    GraphCache graphs;
 };
 
 namespace {
@@ -222,8 +322,10 @@ int sdf3_weno5_rk3_reinit(vls_sdf3* h, T* F, int iterations, hipStream_t s)
 {
     VLS_REQUIRE(h && F, "null argument");
     VLS_REQUIRE(iterations >= 0, "negative iteration count");
-    VLS_HIP(launch_set_mask_bit<T>(h->flags, F, h->g, s));
-    return reinit_loop<T>(h, F, iterations, main_set(h), s);
+    return with_graph(h->graphs, make_key(1, sizeof(T), F, iterations), s, [&](hipStream_t cs) -> int {
+        VLS_HIP(launch_set_mask_bit<T>(h->flags, F, h->g, cs));
+        return reinit_loop<T>(h, F, iterations, main_set(h), cs);
+    });
 }
 
 template <typename T>
@@ -235,11 +337,16 @@ int sdf3_weno5_rk3_extend(vls_sdf3* h, const T* F, T* C, int iterations, hipStre
     const Geom& g = h->g;
     const DSet ms = main_set(h);
     T *vx = vel<T>(h, 0), *vy = vel<T>(h, 1), *vz = vel<T>(h, 2);
-    VLS_HIP(launch_upwind_normal<T>(vx, vy, vz, F, constify(h->d), g, s));
-    VLS_HIP(launch_extension_velocity<T>(vx, vy, vz, F, 1e-6, g, s));
-    VLS_HIP(launch_list_gather_vel<T>(h->lvel, ms.lv, vx, vy, vz, g, s));
-    return extend_loop<T>(h, C, iterations, ms, s);
+    return with_graph(h->graphs, make_key(2, sizeof(T), F, C, iterations), s, [&](hipStream_t cs) -> int {
+        VLS_HIP(launch_upwind_normal<T>(vx, vy, vz, F, constify(h->d), g, cs));
+        VLS_HIP(launch_extension_velocity<T>(vx, vy, vz, F, 1e-6, g, cs));
+        VLS_HIP(launch_list_gather_vel<T>(h->lvel, ms.lv, vx, vy, vz, g, cs));
+        return extend_loop<T>(h, C, iterations, ms, cs);
+    });
 }
+
+template <typename T>
+int surface_redistance_issue(vls_sdf3* h, const T* F, T* A, int iter1, int iter2, hipStream_t s);
 
 template <typename T>
 int sdf3_surface_redistance(vls_sdf3* h, const T* F, T* A, int iter1, int iter2, hipStream_t s)
@@ -247,6 +354,13 @@ int sdf3_surface_redistance(vls_sdf3* h, const T* F, T* A, int iter1, int iter2,
     VLS_REQUIRE(h && F && A, "null argument");
     VLS_REQUIRE(iter1 >= 0 && iter2 >= 0, "negative iteration count");
     VLS_REQUIRE((const void*)F != (const void*)A, "A must not alias F");
+    return with_graph(h->graphs, make_key(3, sizeof(T), F, A, iter1, iter2), s,
+                      [&](hipStream_t cs) -> int { return surface_redistance_issue<T>(h, F, A, iter1, iter2, cs); });
+}
+
+template <typename T>
+int surface_redistance_issue(vls_sdf3* h, const T* F, T* A, int iter1, int iter2, hipStream_t s)
+{
     const Geom& g = h->g;
     int rc;
     // A's distances, deltat and mask (WENO5RK3ClosetPointSurfaceRedistance.m:8-25)
@@ -281,12 +395,14 @@ int sdf3_advect_rk3(vls_sdf3* h, T* F, const T* vx, const T* vy, const T* vz, do
     const Geom& g = h->g;
     T* t0 = tmp<T>(h, 0);
     T* t1 = tmp<T>(h, 1);
-    for (int it = 0; it < steps; ++it) {
-        VLS_HIP(launch_advect_stage<T>(t0, F, nullptr, vx, vy, vz, dt, g, 0, s));
-        VLS_HIP(launch_advect_stage<T>(t1, t0, F, vx, vy, vz, dt, g, 1, s));
-        VLS_HIP(launch_advect_stage<T>(F, t1, F, vx, vy, vz, dt, g, 2, s));
-    }
-    return VLS_OK;
+    return with_graph(h->graphs, make_key(4, sizeof(T), F, vx, vy, vz, dt, steps), s, [&](hipStream_t cs) -> int {
+        for (int it = 0; it < steps; ++it) {
+            VLS_HIP(launch_advect_stage<T>(t0, F, nullptr, vx, vy, vz, dt, g, 0, cs));
+            VLS_HIP(launch_advect_stage<T>(t1, t0, F, vx, vy, vz, dt, g, 1, cs));
+            VLS_HIP(launch_advect_stage<T>(F, t1, F, vx, vy, vz, dt, g, 2, cs));
+        }
+        return VLS_OK;
+    });
 }
 
 template <typename T>
@@ -349,6 +465,13 @@ int area_volume(vls_sdf3* h, int kind, const T* F, double eps, double* dev_out, 
 extern "C" {
 
 const char* vls_last_error(void) { return g_err.c_str(); }
+
+int vls_set_graphs(int enable)
+{
+    const int prev = g_graphs;
+    g_graphs = enable ? 1 : 0;
+    return prev;
+}
 const char* vls_version(void) { return "vesicle_ls 0.1 (gfx950, FP64)"; }
 
 // ---------------------------------------------------------------------------
@@ -567,6 +690,15 @@ int vls_sdf3_create(vls_sdf3_t* out, int rows, int cols, int pges, double dx, do
     return VLS_OK;
 }
 
+int vls_sdf3_set_tile_rows(vls_sdf3_t h, int tile_rows)
+{
+    VLS_REQUIRE(h, "null handle");
+    VLS_REQUIRE(tile_rows == 32 || tile_rows == 64, "tile rows must be 32 or 64");
+    h->g.tr = tile_rows;
+    h->graphs.clear();
+    return VLS_OK;
+}
+
 int vls_sdf3_destroy(vls_sdf3_t h)
 {
     if (!h) return VLS_OK;
@@ -718,6 +850,556 @@ int vls_sdf3_normal_velocity_f32(vls_sdf3_t h, const float* F, const float* vx, 
                                  float* out, vls_stream_t stream)
 {
     return sdf3_normal_velocity<float>(h, F, vx, vy, vz, out, S(stream));
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// 4. batches of grids
+// ---------------------------------------------------------------------------
+struct vls_batch {
+    int ngrids = 0, ntiles = 0;
+    int64_t total = 0;
+    int blocks = 0;  // persistent blocks per tile-loop launch
+    int tr = 64;     // rows per tile
+    std::vector<BGrid> grids;
+    std::vector<int64_t> offs;
+    BGrid* d_grids = nullptr;
+    int* d_tile_grid = nullptr;
+    int64_t* d_offs = nullptr;
+    double* pool = nullptr;  // list data and work fields
+    uint8_t* flags = nullptr;
+    int* int_pool = nullptr;
+    ListView lv{};
+    int* over = nullptr;
+    int* counts = nullptr;  // [0] list count, [4..5] deferred counts
+    double* lcp = nullptr;
+    double* lvel = nullptr;
+    double* t[2] = {nullptr, nullptr};
+    double* v[3] = {nullptr, nullptr, nullptr};
+    double* partial = nullptr;  // per-tile partial sums (4 per tile of either shape)
+    GraphCache graphs;
+
+    BatchView view() const { return BatchView{d_grids, d_tile_grid, d_offs, ngrids, ntiles, blocks, tr}; }
+    ~vls_batch()
+    {
+        (void)hipFree(d_grids);
+        (void)hipFree(d_tile_grid);
+        (void)hipFree(d_offs);
+        (void)hipFree(pool);
+        (void)hipFree(flags);
+        (void)hipFree(int_pool);
+        (void)hipFree(partial);
+    }
+};
+
+namespace {
+
+// persistent blocks of a tile-loop launch: 8 blocks of 256 threads per CU
+// (the occupancy of the bulk kernels) on every CU, a multiple of 8 so that
+// each XCD gets the same share, never more than there are tiles (rounded up
+// to a multiple of 8)
+int default_blocks(int ntiles)
+{
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
+            cus = prop.multiProcessorCount;
+    }
+    int b = 8 * cus;
+    const int cap = (ntiles + 7) / 8 * 8;
+    if (b > cap) b = cap;
+    return b < 8 ? 8 : b;
+}
+
+// the tile list of shape `tr` (host descriptors, device descriptors and tile->grid map)
+int build_tiles(vls_batch* b, int tr)
+{
+    std::vector<int> tile_grid;
+    int64_t tiles = 0;
+    for (int g = 0; g < b->ngrids; ++g) {
+        BGrid& G = b->grids[(size_t)g];
+        G.ntiles = grid_tiles(G.rows, G.cols, G.pges, tr);
+        G.tile0 = (int)tiles;
+        tiles += G.ntiles;
+        if (tiles >= ((int64_t)1 << 31)) return fail(VLS_ERR_INVALID, "too many tiles");
+        tile_grid.insert(tile_grid.end(), (size_t)G.ntiles, g);
+    }
+    (void)hipFree(b->d_tile_grid);
+    b->d_tile_grid = nullptr;
+    if (hipMalloc((void**)&b->d_tile_grid, sizeof(int) * (size_t)tiles) != hipSuccess)
+        return fail(VLS_ERR_ALLOC, "hipMalloc of the tile list failed");
+    VLS_HIP(hipMemcpy(b->d_tile_grid, tile_grid.data(), sizeof(int) * (size_t)tiles, hipMemcpyHostToDevice));
+    VLS_HIP(hipMemcpy(b->d_grids, b->grids.data(), sizeof(BGrid) * (size_t)b->ngrids, hipMemcpyHostToDevice));
+    b->ntiles = (int)tiles;
+    b->tr = tr;
+    return VLS_OK;
+}
+
+template <typename T>
+T* btmp(vls_batch* b, int k)
+{
+    return reinterpret_cast<T*>(b->t[k]);
+}
+template <typename T>
+T* bvel(vls_batch* b, int a)
+{
+    return reinterpret_cast<T*>(b->v[a]);
+}
+
+template <typename T>
+int batch_set_distance(vls_batch* b, const T* F, const T* mask_src, hipStream_t s)
+{
+    VLS_HIP(launch_zero_ints(b->counts, 1, s));
+    VLS_HIP(launch_batch_build_list<T>(b->view(), F, mask_src, b->flags, b->lv, s));
+    return VLS_OK;
+}
+
+template <typename T>
+int batch_reinit_loop(vls_batch* b, T* F, int iterations, hipStream_t s)
+{
+    const BatchView B = b->view();
+    int* oc = b->counts + 4;
+    T* t0 = btmp<T>(b, 0);
+    T* t1 = btmp<T>(b, 1);
+    VLS_HIP(launch_batch_set_mask_bit<T>(b->flags, F, b->total, s));
+    VLS_HIP(launch_zero_ints(oc, 2, s));
+    int seq = 0;
+    for (int it = 0; it < iterations; ++it) {
+        VLS_HIP(launch_batch_reinit_stage<T>(B, t0, F, nullptr, b->flags, b->lv, 0, b->over, oc, seq++, s));
+        VLS_HIP(launch_batch_reinit_stage<T>(B, t1, t0, F, b->flags, b->lv, 1, b->over, oc, seq++, s));
+        VLS_HIP(launch_batch_reinit_stage<T>(B, F, t1, F, b->flags, b->lv, 2, b->over, oc, seq++, s));
+    }
+    return VLS_OK;
+}
+
+template <typename T>
+int batch_extend(vls_batch* b, const T* F, T* C, int iterations, hipStream_t s)
+{
+    const BatchView B = b->view();
+    T *vx = bvel<T>(b, 0), *vy = bvel<T>(b, 1), *vz = bvel<T>(b, 2);
+    T* t0 = btmp<T>(b, 0);
+    T* t1 = btmp<T>(b, 1);
+    VLS_HIP(launch_batch_extension_velocity<T>(B, vx, vy, vz, F, b->flags, b->lv, 1e-6, s));
+    VLS_HIP(launch_batch_list_gather_vel<T>(b->lvel, b->lv, vx, vy, vz, s));
+    VLS_HIP(launch_batch_list_interp<T>(B, b->lcp, b->lv, C, s));
+    for (int it = 0; it < iterations; ++it) {
+        VLS_HIP(launch_batch_extend_stage<T>(B, t0, C, nullptr, vx, vy, vz, b->flags, b->lv, b->lcp, b->lvel, 0, s));
+        VLS_HIP(launch_batch_extend_stage<T>(B, t1, t0, C, vx, vy, vz, b->flags, b->lv, b->lcp, b->lvel, 1, s));
+        VLS_HIP(launch_batch_extend_stage<T>(B, C, t1, C, vx, vy, vz, b->flags, b->lv, b->lcp, b->lvel, 2, s));
+    }
+    return VLS_OK;
+}
+
+template <typename T>
+int batch_advect(vls_batch* b, T* F, const T* vx, const T* vy, const T* vz, const double* dt, int steps,
+                 hipStream_t s)
+{
+    const BatchView B = b->view();
+    T* t0 = btmp<T>(b, 0);
+    T* t1 = btmp<T>(b, 1);
+    for (int it = 0; it < steps; ++it) {
+        VLS_HIP(launch_batch_advect_stage<T>(B, t0, F, nullptr, vx, vy, vz, dt, 0, s));
+        VLS_HIP(launch_batch_advect_stage<T>(B, t1, t0, F, vx, vy, vz, dt, 1, s));
+        VLS_HIP(launch_batch_advect_stage<T>(B, F, t1, F, vx, vy, vz, dt, 2, s));
+    }
+    return VLS_OK;
+}
+
+// one vesicle step of every grid: TVD-RK3 advection of F, setDistance, the
+// normal velocity u.N of F into Vn and its WENO5-RK3 extension, a WENO5-RK3
+// reinitialization of F when reinit_iters > 0, and the cosine area/volume of
+// every grid into av (2 per grid)
+template <typename T>
+int batch_step(vls_batch* b, T* F, T* Vn, const T* vx, const T* vy, const T* vz, const double* dt, int ext_iters,
+               int reinit_iters, double eps_factor, double* av, hipStream_t s)
+{
+    int rc;
+    if ((rc = batch_advect<T>(b, F, vx, vy, vz, dt, 1, s))) return rc;
+    if ((rc = batch_set_distance<T>(b, F, (const T*)nullptr, s))) return rc;
+    VLS_HIP(launch_batch_normal_velocity<T>(b->view(), Vn, F, vx, vy, vz, s));
+    if ((rc = batch_extend<T>(b, F, Vn, ext_iters, s))) return rc;
+    if (reinit_iters > 0 && (rc = batch_reinit_loop<T>(b, F, reinit_iters, s))) return rc;
+    VLS_HIP(launch_batch_area_volume_cosine<T>(b->view(), av, b->partial, F, eps_factor, s));
+    return VLS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vls_batch_create(vls_batch_t* out, int ngrids, const int* rows, const int* cols, const int* pges,
+                     const double* dx, const double* dy, const double* dz)
+{
+    VLS_REQUIRE(out != nullptr, "null handle pointer");
+    *out = nullptr;
+    VLS_REQUIRE(ngrids > 0, "a batch needs at least one grid");
+    VLS_REQUIRE(!any_null(rows, cols, pges, dx, dy, dz), "null argument");
+    vls_batch* b = new (std::nothrow) vls_batch();
+    if (!b) return fail(VLS_ERR_ALLOC, "host allocation failed");
+    b->ngrids = ngrids;
+    b->offs.resize((size_t)ngrids + 1);
+    int64_t off = 0, max_tiles = 0, tiles64 = 0, tiles32 = 0;
+    for (int g = 0; g < ngrids; ++g) {
+        int rc = check_geom(rows[g], cols[g], pges[g], dx[g], dy[g], dz[g]);
+        if (rc) {
+            delete b;
+            return rc;
+        }
+        BGrid G{};
+        G.rows = rows[g];
+        G.cols = cols[g];
+        G.pges = pges[g];
+        G.off = off;
+        G.dx = dx[g];
+        G.dy = dy[g];
+        G.dz = dz[g];
+        G.inv[0] = 1.0 / dx[g];
+        G.inv[1] = 1.0 / dy[g];
+        G.inv[2] = 1.0 / dz[g];
+        const double m = std::min(std::min(dx[g], dx[g]), std::min(std::min(dy[g], dy[g]), std::min(dz[g], dz[g])));
+        G.dt_bulk = 0.3 * m;
+        b->grids.push_back(G);
+        b->offs[(size_t)g] = off;
+        off += (int64_t)rows[g] * cols[g] * pges[g];
+        tiles64 += grid_tiles(rows[g], cols[g], pges[g], 64);
+        tiles32 += grid_tiles(rows[g], cols[g], pges[g], 32);
+        if (off >= ((int64_t)1 << 31)) {
+            delete b;
+            return fail(VLS_ERR_INVALID, "a batch holds fewer than 2^31 points");
+        }
+    }
+    max_tiles = std::max(tiles64, tiles32);
+    b->offs[(size_t)ngrids] = off;
+    b->total = off;
+    const int64_t n = off;
+    // list rows (6 dist + 1 dt + 6 cp + 3 vel) + work fields (2 t + 3 v), one allocation
+    const size_t dbl = (size_t)n * (6 + 1 + 6 + 3 + 2 + 3);
+    if (hipMalloc((void**)&b->pool, sizeof(double) * dbl) != hipSuccess ||
+        hipMalloc((void**)&b->partial, sizeof(double) * ((size_t)max_tiles * 4 + 8)) != hipSuccess ||
+        hipMalloc((void**)&b->flags, (size_t)n) != hipSuccess ||
+        hipMalloc((void**)&b->int_pool, sizeof(int) * ((size_t)n * 2 + 16)) != hipSuccess ||
+        hipMalloc((void**)&b->d_grids, sizeof(BGrid) * (size_t)ngrids) != hipSuccess ||
+        hipMalloc((void**)&b->d_offs, sizeof(int64_t) * ((size_t)ngrids + 1)) != hipSuccess) {
+        delete b;
+        return fail(VLS_ERR_ALLOC, "hipMalloc of the batch (%lld points) failed", (long long)n);
+    }
+    if (hipMemcpy(b->d_offs, b->offs.data(), sizeof(int64_t) * ((size_t)ngrids + 1), hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemset(b->int_pool, 0, sizeof(int) * ((size_t)n * 2 + 16)) != hipSuccess) {
+        delete b;
+        return fail(VLS_ERR_HIP, "copying the batch descriptors failed");
+    }
+    // rows of 32 fill the 64-row tiles of every grid only half: shape 32 when
+    // that needs fewer tiles than 64 (the autotuner may change it)
+    int rc = build_tiles(b, tiles32 < tiles64 ? 32 : 64);
+    if (rc) {
+        delete b;
+        return rc;
+    }
+    b->blocks = default_blocks(b->ntiles);
+    double* p = b->pool;
+    b->lv.dist = p; p += 6 * n;
+    b->lv.dt = p; p += n;
+    b->lcp = p; p += 6 * n;
+    b->lvel = p; p += 3 * n;
+    for (int k = 0; k < 2; ++k) { b->t[k] = p; p += n; }
+    for (int a = 0; a < 3; ++a) { b->v[a] = p; p += n; }
+    b->lv.idx = b->int_pool;
+    b->over = b->int_pool + n;
+    b->counts = b->int_pool + 2 * n;
+    b->lv.count = b->counts;
+    b->lv.cap = (int)n;
+    *out = b;
+    return VLS_OK;
+}
+
+int vls_batch_destroy(vls_batch_t b)
+{
+    delete b;
+    return VLS_OK;
+}
+
+int vls_batch_layout(vls_batch_t b, int64_t* offsets, int* ntiles)
+{
+    VLS_REQUIRE(b && offsets, "null argument");
+    for (size_t g = 0; g < b->offs.size(); ++g) offsets[g] = b->offs[g];
+    if (ntiles) *ntiles = b->ntiles;
+    return VLS_OK;
+}
+
+int vls_batch_set_blocks(vls_batch_t b, int blocks)
+{
+    VLS_REQUIRE(b, "null handle");
+    VLS_REQUIRE(blocks >= 0 && blocks % 8 == 0, "blocks must be a non-negative multiple of 8 (0: default)");
+    b->blocks = blocks ? blocks : default_blocks(b->ntiles);
+    b->graphs.clear();  // the graphs hold the old launch shape
+    return VLS_OK;
+}
+
+int vls_batch_get_blocks(vls_batch_t b, int* blocks)
+{
+    VLS_REQUIRE(b && blocks, "null argument");
+    *blocks = b->blocks;
+    return VLS_OK;
+}
+
+int vls_batch_set_tile_rows(vls_batch_t b, int tile_rows)
+{
+    VLS_REQUIRE(b, "null handle");
+    VLS_REQUIRE(tile_rows == 32 || tile_rows == 64, "tile rows must be 32 or 64");
+    if (tile_rows == b->tr) return VLS_OK;
+    VLS_HIP(hipDeviceSynchronize());  // no launch of this batch may still read the old tile list
+    const int rc = build_tiles(b, tile_rows);
+    if (rc) return rc;
+    b->blocks = std::min(b->blocks, (b->ntiles + 7) / 8 * 8);
+    if (b->blocks < 8) b->blocks = 8;
+    b->graphs.clear();
+    return VLS_OK;
+}
+
+int vls_batch_get_tile_rows(vls_batch_t b, int* tile_rows)
+{
+    VLS_REQUIRE(b && tile_rows, "null argument");
+    *tile_rows = b->tr;
+    return VLS_OK;
+}
+
+#define VLS_BATCH_ENTRIES(T, SFX)                                                                                   \
+    int vls_batch_set_distance##SFX(vls_batch_t b, const T* F, vls_stream_t stream)                                \
+    {                                                                                                              \
+        VLS_REQUIRE(b && F, "null argument");                                                                      \
+        return batch_set_distance<T>(b, F, (const T*)nullptr, S(stream));                                          \
+    }                                                                                                              \
+    int vls_batch_weno5_rk3_reinit##SFX(vls_batch_t b, T* F, int iterations, vls_stream_t stream)                  \
+    {                                                                                                              \
+        VLS_REQUIRE(b && F, "null argument");                                                                      \
+        VLS_REQUIRE(iterations >= 0, "negative iteration count");                                                  \
+        return with_graph(b->graphs, make_key(11, sizeof(T), F, iterations), S(stream),                            \
+                          [&](hipStream_t cs) -> int { return batch_reinit_loop<T>(b, F, iterations, cs); });      \
+    }                                                                                                              \
+    int vls_batch_weno5_rk3_extend##SFX(vls_batch_t b, const T* F, T* C, int iterations, vls_stream_t stream)      \
+    {                                                                                                              \
+        VLS_REQUIRE(b && F && C, "null argument");                                                                 \
+        VLS_REQUIRE(iterations >= 0, "negative iteration count");                                                  \
+        VLS_REQUIRE((const void*)F != (const void*)C, "C must not alias F");                                       \
+        return with_graph(b->graphs, make_key(12, sizeof(T), F, C, iterations), S(stream),                         \
+                          [&](hipStream_t cs) -> int { return batch_extend<T>(b, F, C, iterations, cs); });        \
+    }                                                                                                              \
+    int vls_batch_advect_rk3##SFX(vls_batch_t b, T* F, const T* vx, const T* vy, const T* vz, const double* dt,    \
+                                  int steps, vls_stream_t stream)                                                  \
+    {                                                                                                              \
+        VLS_REQUIRE(b && F && vx && vy && vz && dt, "null argument");                                              \
+        VLS_REQUIRE(steps >= 0, "negative step count");                                                            \
+        return with_graph(b->graphs, make_key(13, sizeof(T), F, vx, vy, vz, dt, steps), S(stream),                 \
+                          [&](hipStream_t cs) -> int { return batch_advect<T>(b, F, vx, vy, vz, dt, steps, cs); });\
+    }                                                                                                              \
+    int vls_batch_normal_velocity##SFX(vls_batch_t b, const T* F, const T* vx, const T* vy, const T* vz, T* out,   \
+                                       vls_stream_t stream)                                                        \
+    {                                                                                                              \
+        VLS_REQUIRE(b && F && vx && vy && vz && out, "null argument");                                             \
+        VLS_REQUIRE((const void*)out != (const void*)F, "out must not alias F");                                   \
+        VLS_HIP(launch_batch_normal_velocity<T>(b->view(), out, F, vx, vy, vz, S(stream)));                       \
+        return VLS_OK;                                                                                             \
+    }                                                                                                              \
+    int vls_batch_area_volume_cosine##SFX(vls_batch_t b, const T* F, double eps_factor, double* out2n,             \
+                                          vls_stream_t stream)                                                     \
+    {                                                                                                              \
+        VLS_REQUIRE(b && F && out2n, "null argument");                                                             \
+        VLS_REQUIRE(eps_factor > 0, "eps_factor must be positive");                                                \
+        VLS_HIP(launch_batch_area_volume_cosine<T>(b->view(), out2n, b->partial, F, eps_factor, S(stream)));      \
+        return VLS_OK;                                                                                             \
+    }                                                                                                              \
+    int vls_batch_cfl_dt##SFX(vls_batch_t b, const T* vx, const T* vy, const T* vz, double cfl, double* dt,        \
+                              vls_stream_t stream)                                                                 \
+    {                                                                                                              \
+        VLS_REQUIRE(b && vx && vy && vz && dt, "null argument");                                                   \
+        VLS_REQUIRE(cfl > 0, "cfl must be positive");                                                             \
+        VLS_HIP(launch_batch_cfl_dt<T>(b->view(), dt, b->partial, vx, vy, vz, cfl, S(stream)));                   \
+        return VLS_OK;                                                                                             \
+    }                                                                                                              \
+    int vls_batch_curl_velocity##SFX(vls_batch_t b, const double* modes, int nmodes, const double* lo, T* vx,      \
+                                     T* vy, T* vz, vls_stream_t stream)                                            \
+    {                                                                                                              \
+        VLS_REQUIRE(b && modes && lo && vx && vy && vz, "null argument");                                          \
+        VLS_REQUIRE(nmodes >= 0, "negative mode count");                                                           \
+        VLS_HIP(launch_batch_curl_velocity<T>(b->view(), vx, vy, vz, modes, nmodes, lo, S(stream)));              \
+        return VLS_OK;                                                                                             \
+    }                                                                                                              \
+    int vls_batch_ellipsoid##SFX(vls_batch_t b, const double* axes, const double* lo, T* F, vls_stream_t stream)   \
+    {                                                                                                              \
+        VLS_REQUIRE(b && axes && lo && F, "null argument");                                                        \
+        VLS_HIP(launch_batch_ellipsoid<T>(b->view(), F, axes, lo, S(stream)));                                    \
+        return VLS_OK;                                                                                             \
+    }                                                                                                              \
+    int vls_batch_step##SFX(vls_batch_t b, T* F, T* Vn, const T* vx, const T* vy, const T* vz, const double* dt,   \
+                            int ext_iters, int reinit_iters, double eps_factor, double* av, vls_stream_t stream)   \
+    {                                                                                                              \
+        VLS_REQUIRE(b && F && Vn && vx && vy && vz && dt && av, "null argument");                                  \
+        VLS_REQUIRE(ext_iters >= 0 && reinit_iters >= 0, "negative iteration count");                              \
+        VLS_REQUIRE(eps_factor > 0, "eps_factor must be positive");                                                \
+        VLS_REQUIRE((const void*)F != (const void*)Vn, "Vn must not alias F");                                     \
+        return with_graph(b->graphs,                                                                               \
+                          make_key(14, sizeof(T), F, Vn, vx, vy, vz, dt, ext_iters, reinit_iters, eps_factor, av), \
+                          S(stream), [&](hipStream_t cs) -> int {                                                  \
+                              return batch_step<T>(b, F, Vn, vx, vy, vz, dt, ext_iters, reinit_iters, eps_factor,  \
+                                                   av, cs);                                                        \
+                          });                                                                                      \
+    }
+
+VLS_BATCH_ENTRIES(double, )
+VLS_BATCH_ENTRIES(float, _f32)
+#undef VLS_BATCH_ENTRIES
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// 5. autotuning of the launch shape
+// ---------------------------------------------------------------------------
+// Times `iterations` WENO5-RK3 extension iterations (the dominant operator)
+// of a scratch copy of F, off F's own interface, for every candidate launch
+// shape and keeps the fastest.  The results of every method are the same for
+// every shape; only the time differs.  Candidates: tile rows 64 and 32
+// (LDS tile 70 x 10 or 38 x 14 values) and, for a batch, 2, 4, 8 or 16
+// persistent blocks of 256 threads per CU (8 to 64 waves per CU).
+namespace {
+
+struct EventTimer {
+    hipEvent_t a = nullptr, b = nullptr;
+    ~EventTimer()
+    {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+
+template <typename T>
+int batch_autotune(vls_batch* b, const T* F, int iterations, hipStream_t s, int* tr_out, int* blocks_out)
+{
+    int cus = 256, dev = 0;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
+            cus = prop.multiProcessorCount;
+    }
+    T* C = nullptr;
+    if (hipMalloc((void**)&C, sizeof(T) * (size_t)b->total) != hipSuccess)
+        return fail(VLS_ERR_ALLOC, "hipMalloc of the autotuning field failed");
+    EventTimer ev;
+    int rc = VLS_OK;
+    float best = 1e30f;
+    int best_tr = b->tr, best_blocks = b->blocks;
+    const int trs[2] = {64, 32};
+    const int per_cu[4] = {2, 4, 8, 16};
+    if (hipEventCreate(&ev.a) != hipSuccess || hipEventCreate(&ev.b) != hipSuccess) rc = fail(VLS_ERR_HIP, "hipEventCreate");
+    for (int i = 0; i < 2 && rc == VLS_OK; ++i) {
+        if ((rc = build_tiles(b, trs[i]))) break;
+        for (int j = 0; j < 4 && rc == VLS_OK; ++j) {
+            b->blocks = std::max(8, std::min(per_cu[j] * cus, (b->ntiles + 7) / 8 * 8) / 8 * 8);
+            if (hipMemcpyAsync(C, F, sizeof(T) * (size_t)b->total, hipMemcpyDeviceToDevice, s) != hipSuccess) {
+                rc = fail(VLS_ERR_HIP, "hipMemcpyAsync");
+                break;
+            }
+            if ((rc = batch_set_distance<T>(b, F, (const T*)nullptr, s))) break;
+            if ((rc = batch_extend<T>(b, F, C, 1, s))) break;
+            if (hipEventRecord(ev.a, s) != hipSuccess) { rc = fail(VLS_ERR_HIP, "hipEventRecord"); break; }
+            if ((rc = batch_extend<T>(b, F, C, iterations, s))) break;
+            float ms = 0;
+            if (hipEventRecord(ev.b, s) != hipSuccess || hipEventSynchronize(ev.b) != hipSuccess ||
+                hipEventElapsedTime(&ms, ev.a, ev.b) != hipSuccess) {
+                rc = fail(VLS_ERR_HIP, "timing the candidate failed");
+                break;
+            }
+            if (ms < best) {
+                best = ms;
+                best_tr = trs[i];
+                best_blocks = b->blocks;
+            }
+        }
+    }
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(C);
+    if (rc) return rc;
+    if ((rc = build_tiles(b, best_tr))) return rc;
+    b->blocks = best_blocks;
+    b->graphs.clear();
+    if (tr_out) *tr_out = best_tr;
+    if (blocks_out) *blocks_out = best_blocks;
+    return VLS_OK;
+}
+
+template <typename T>
+int sdf3_autotune(vls_sdf3* h, const T* F, int iterations, hipStream_t s, int* tr_out)
+{
+    EventTimer ev;
+    VLS_HIP(hipEventCreate(&ev.a));
+    VLS_HIP(hipEventCreate(&ev.b));
+    T* C = tmp<T>(h, 2);
+    const DSet ms = main_set(h);
+    T *vx = vel<T>(h, 0), *vy = vel<T>(h, 1), *vz = vel<T>(h, 2);
+    float best = 1e30f;
+    int best_tr = h->g.tr;
+    const int trs[2] = {64, 32};
+    int rc;
+    for (int i = 0; i < 2; ++i) {
+        h->g.tr = trs[i];
+        const Geom& g = h->g;
+        VLS_HIP(hipMemcpyAsync(C, F, sizeof(T) * (size_t)g.n(), hipMemcpyDeviceToDevice, s));
+        if ((rc = set_distance_into<T>(h, 6, ms, h->min_dist, F, (const T*)nullptr, s))) return rc;
+        VLS_HIP(launch_upwind_normal<T>(vx, vy, vz, F, constify(h->d), g, s));
+        VLS_HIP(launch_extension_velocity<T>(vx, vy, vz, F, 1e-6, g, s));
+        VLS_HIP(launch_list_gather_vel<T>(h->lvel, ms.lv, vx, vy, vz, g, s));
+        if ((rc = extend_loop<T>(h, C, 1, ms, s))) return rc;
+        VLS_HIP(hipEventRecord(ev.a, s));
+        if ((rc = extend_loop<T>(h, C, iterations, ms, s))) return rc;
+        VLS_HIP(hipEventRecord(ev.b, s));
+        VLS_HIP(hipEventSynchronize(ev.b));
+        float ms_ = 0;
+        VLS_HIP(hipEventElapsedTime(&ms_, ev.a, ev.b));
+        if (ms_ < best) {
+            best = ms_;
+            best_tr = trs[i];
+        }
+    }
+    h->g.tr = best_tr;
+    h->graphs.clear();
+    if (tr_out) *tr_out = best_tr;
+    return VLS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vls_batch_autotune(vls_batch_t b, const double* F, int iterations, int* tile_rows, int* blocks,
+                       vls_stream_t stream)
+{
+    VLS_REQUIRE(b && F, "null argument");
+    VLS_REQUIRE(iterations > 0, "iterations must be positive");
+    return batch_autotune<double>(b, F, iterations, S(stream), tile_rows, blocks);
+}
+
+int vls_batch_autotune_f32(vls_batch_t b, const float* F, int iterations, int* tile_rows, int* blocks,
+                           vls_stream_t stream)
+{
+    VLS_REQUIRE(b && F, "null argument");
+    VLS_REQUIRE(iterations > 0, "iterations must be positive");
+    return batch_autotune<float>(b, F, iterations, S(stream), tile_rows, blocks);
+}
+
+int vls_sdf3_autotune(vls_sdf3_t h, const double* F, int iterations, int* tile_rows, vls_stream_t stream)
+{
+    VLS_REQUIRE(h && F, "null argument");
+    VLS_REQUIRE(iterations > 0, "iterations must be positive");
+    return sdf3_autotune<double>(h, F, iterations, S(stream), tile_rows);
+}
+
+int vls_sdf3_autotune_f32(vls_sdf3_t h, const float* F, int iterations, int* tile_rows, vls_stream_t stream)
+{
+    VLS_REQUIRE(h && F, "null argument");
+    VLS_REQUIRE(iterations > 0, "iterations must be positive");
+    return sdf3_autotune<float>(h, F, iterations, S(stream), tile_rows);
 }
 
 }  // extern "C"

@@ -1,0 +1,717 @@
+// Batched kernels: many grids of different sizes advanced by one launch per
+// operator stage.
+//
+// A batch packs every field of all its grids back to back (each grid in the
+// column-major layout of the single-grid kernels) and lists the plane tiles
+// of all grids (64 x 4 or 32 x 8 points of one page, one shape per batch) in
+// one array; tile_grid[] names each tile's grid.  The
+// bulk kernels are persistent: a fixed number of blocks (a multiple of 8)
+// walks the tile list, each of the 8 XCDs taking one contiguous eighth of it
+// so that neighbouring tiles share their halos in one L2.  Interface lists
+// hold packed element indices; a lane finds its grid by binary search over
+// the grid offsets.  Every per-point body is the single-grid kernels' own
+// (vls_tile.hpp), called with the pointers of the point's grid, so each grid
+// of a batch gets the single-grid result bit for bit.
+#include "vls_tile.hpp"
+
+namespace vls {
+
+__device__ __forceinline__ Box box_of(const BGrid& G) { return Box{G.rows, G.cols, G.pges, G.dx, G.dy, G.dz}; }
+
+__device__ __forceinline__ Inv inv_of(const BGrid& G)
+{
+    Inv v;
+    v.i[0] = G.inv[0];
+    v.i[1] = G.inv[1];
+    v.i[2] = G.inv[2];
+    v.dt_bulk = G.dt_bulk;
+    return v;
+}
+
+// the tiles a block of a persistent launch visits: XCD x (= blockIdx.x % 8,
+// the hardware's round-robin) takes the contiguous range
+// [x*chunk, (x+1)*chunk) and its blocks stride through it
+struct TileRange {
+    int first, end, step;
+};
+
+__device__ __forceinline__ TileRange tile_range(int ntiles)
+{
+    TileRange r;
+    const int nb = (int)gridDim.x;
+    if ((nb & 7) == 0) {
+        const int x = (int)(blockIdx.x & 7), j = (int)(blockIdx.x >> 3), per = nb >> 3;
+        const int chunk = (ntiles + 7) >> 3;
+        r.first = x * chunk + j;
+        r.end = min(ntiles, (x + 1) * chunk);
+        r.step = per;
+    } else {
+        r.first = (int)blockIdx.x;
+        r.end = ntiles;
+        r.step = nb;
+    }
+    return r;
+}
+
+// grid of a packed element index: offs[g] <= gind < offs[g+1]
+__device__ __forceinline__ int grid_of(const BatchView& B, int64_t gind)
+{
+    int lo = 0, hi = B.ngrids;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (B.offs[mid] <= gind) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <typename T>
+__device__ __forceinline__ const T* shift(const T* p, int64_t off)
+{
+    return p ? p + off : p;
+}
+template <typename T>
+__device__ __forceinline__ T* shift(T* p, int64_t off)
+{
+    return p ? p + off : p;
+}
+
+// ---------------------------------------------------------------------------
+// list building: setDistance (setDistance.m:5-31 with boundary_location.cu:84-132)
+// and the interface list in one pass.  Each point computes its own six
+// distances: the r-side from the crossing (i, i+1) on the stencil around i,
+// the l-side as spacing minus the distance of the crossing (i-1, i) on the
+// stencil around i-1, which is what boundary_location writes into the l-side
+// array of i; no full distance arrays are stored.
+// ---------------------------------------------------------------------------
+template <int R, typename T>
+__global__ void __launch_bounds__(256) kb_build_list(const T* __restrict__ F, const T* __restrict__ mask_src,
+                                                     uint8_t* __restrict__ flags, ListView il, BatchView B)
+{
+    const TileRange tr = tile_range(B.ntiles);
+    for (int tile = tr.first; tile < tr.end; tile += tr.step) {
+        const BGrid G = B.grids[B.tile_grid[tile]];
+        const Box b = box_of(G);
+        const Tile t = tile_at<R>(b, tile - G.tile0);
+        const T* f = F + G.off;
+        const double sp[3] = {b.dx, b.dy, b.dz};
+        Dists q;
+        bool sub = false;
+        idx_t ind = 0;
+        if (t.valid) {
+            ind = lin(b, t.r, t.c, t.p);
+            const double f0 = (double)f[ind];
+#pragma unroll 1
+            for (int a = 0; a < 3; ++a) {
+                int rp = t.r, cp = t.c, pp = t.p, rm = t.r, cm = t.c, pm = t.p;
+                if (a == 0) { cp = min(t.c + 1, b.cols - 1); cm = max(t.c - 1, 0); }
+                else if (a == 1) { rp = min(t.r + 1, b.rows - 1); rm = max(t.r - 1, 0); }
+                else { pp = min(t.p + 1, b.pges - 1); pm = max(t.p - 1, 0); }
+                const double fp = (double)f[lin(b, rp, cp, pp)], fm = (double)f[lin(b, rm, cm, pm)];
+                q.r[a] = sp[a];
+                q.l[a] = sp[a];
+                const bool cr = f0 * fp < 0, cl = fm * f0 < 0;
+                if (cr || cl) {
+                    double v[7];
+                    line7(f, t.r, t.c, t.p, a, b, v);
+                    if (cr) q.r[a] = six_distance(v[1], v[2], v[3], v[4], v[5], v[6], sp[a]);
+                    if (cl) q.l[a] = sp[a] - six_distance(v[0], v[1], v[2], v[3], v[4], v[5], sp[a]);
+                }
+            }
+            sub = q.r[0] < b.dx || q.l[0] < b.dx || q.r[1] < b.dy || q.l[1] < b.dy || q.r[2] < b.dz ||
+                  q.l[2] < b.dz;
+            uint8_t fl = sub ? FLAG_SUBCELL : 0;
+            if (mask_src && mask_src[G.off + ind] < 0) fl |= FLAG_MASK;
+            flags[G.off + ind] = fl;
+        }
+        const int slot = wave_append_slot(sub, il.count);
+        if (sub) {
+            il.idx[slot] = (int)(G.off + ind);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                il.dist[(size_t)(2 * a) * il.cap + slot] = q.r[a];
+                il.dist[(size_t)(2 * a + 1) * il.cap + slot] = q.l[a];
+            }
+            il.dt[slot] = frozen(q, b.dx) ? -1.0 : 0.3 * min6(q);
+        }
+    }
+}
+
+template <typename T>
+__global__ void kb_set_mask_bit(uint8_t* __restrict__ flags, const T* __restrict__ F, idx_t n)
+{
+    const idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint8_t f = flags[i] & (uint8_t)~FLAG_MASK;
+    if (F[i] < 0) f |= FLAG_MASK;
+    flags[i] = f;
+}
+
+// ---------------------------------------------------------------------------
+// fused stage kernels
+// ---------------------------------------------------------------------------
+template <int R, typename T>
+__global__ void __launch_bounds__(256) kb_reinit_bulk(T* __restrict__ out, const T* __restrict__ fs,
+                                                      const T* __restrict__ fn, const uint8_t* __restrict__ flags,
+                                                      BatchView B, int stage, int* __restrict__ over,
+                                                      int* __restrict__ over_count, int seq)
+{
+    __shared__ T lds[Shape<R>::N];
+    if (blockIdx.x == 0 && threadIdx.x == 0) over_count[(seq + 1) & 1] = 0;
+    const TileRange tr = tile_range(B.ntiles);
+    for (int tile = tr.first; tile < tr.end; tile += tr.step) {
+        const BGrid G = B.grids[B.tile_grid[tile]];
+        const Box b = box_of(G);
+        const Tile t = tile_at<R>(b, tile - G.tile0);
+        fill_tile<R>(lds, fs + G.off, b, t);
+        const idx_t ind = t.valid ? lin(b, t.r, t.c, t.p) : 0;
+        const bool defer = reinit_bulk_point<R>(out + G.off, lds, fs + G.off, shift(fn, G.off), flags + G.off, b, t,
+                                             ind, inv_of(G), stage);
+        wave_append(defer, (int)(G.off + ind), over, &over_count[seq & 1]);
+        __syncthreads();
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) kb_reinit_iface(T* __restrict__ out, const T* __restrict__ fs,
+                                                       const T* __restrict__ fn, const uint8_t* __restrict__ flags,
+                                                       IList il, BatchView B, int stage,
+                                                       const int* __restrict__ over,
+                                                       const int* __restrict__ over_count, int seq)
+{
+    const int n1 = *il.count, n2 = over_count[seq & 1];
+    const int total = n1 + n2;
+    for (int j0 = 0; j0 < total; j0 += axis_lane_stride()) {
+        const AxisLane al = axis_lane(j0);
+        const bool active = al.point >= 0 && al.point < total;
+        ReinitLane q;
+        if (active) {
+            const bool listed = al.point < n1;
+            const int64_t gind = listed ? il.idx[al.point] : over[al.point - n1];
+            const BGrid G = B.grids[grid_of(B, gind)];
+            q = reinit_iface_lane(fs + G.off, flags + G.off, box_of(G), inv_of(G), il, listed ? al.point : -1,
+                                  gind - G.off, gind, al.axis);
+        }
+        reinit_iface_finish(out, fs, fn, q, al, active, stage);
+    }
+}
+
+template <int R, typename T>
+__global__ void __launch_bounds__(256) kb_extend_bulk(T* __restrict__ out, const T* __restrict__ cs,
+                                                      const T* __restrict__ cn, const T* __restrict__ vx,
+                                                      const T* __restrict__ vy, const T* __restrict__ vz,
+                                                      const uint8_t* __restrict__ flags, BatchView B, int stage)
+{
+    __shared__ T lds[Shape<R>::N];
+    const TileRange tr = tile_range(B.ntiles);
+    for (int tile = tr.first; tile < tr.end; tile += tr.step) {
+        const BGrid G = B.grids[B.tile_grid[tile]];
+        const Box b = box_of(G);
+        const Tile t = tile_at<R>(b, tile - G.tile0);
+        fill_tile<R>(lds, cs + G.off, b, t);
+        extend_bulk_point<R>(out + G.off, lds, cs + G.off, shift(cn, G.off), vx + G.off, vy + G.off, vz + G.off,
+                          flags + G.off, b, t, inv_of(G), stage);
+        __syncthreads();
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) kb_extend_iface(T* __restrict__ out, const T* __restrict__ cs,
+                                                       const T* __restrict__ cn, IList il,
+                                                       const double* __restrict__ lcp, const double* __restrict__ lv,
+                                                       BatchView B, int stage)
+{
+    const int total = *il.count;
+    for (int j0 = 0; j0 < total; j0 += axis_lane_stride()) {
+        const AxisLane al = axis_lane(j0);
+        const bool active = al.point >= 0 && al.point < total;
+        ExtendLane q;
+        if (active) {
+            const int64_t gind = il.idx[al.point];
+            const BGrid G = B.grids[grid_of(B, gind)];
+            q = extend_iface_lane(cs + G.off, box_of(G), inv_of(G), il, lcp, lv, al.point, gind - G.off, gind,
+                                  al.axis);
+        }
+        extend_iface_finish(out, cs, cn, q, al, active, stage);
+    }
+}
+
+template <int R, typename T>
+__global__ void __launch_bounds__(256) kb_advect_stage(T* __restrict__ out, const T* __restrict__ fs,
+                                                       const T* __restrict__ fn, const T* __restrict__ vx,
+                                                       const T* __restrict__ vy, const T* __restrict__ vz,
+                                                       const double* __restrict__ dt, BatchView B, int stage)
+{
+    __shared__ T lds[Shape<R>::N];
+    const TileRange tr = tile_range(B.ntiles);
+    for (int tile = tr.first; tile < tr.end; tile += tr.step) {
+        const int g = B.tile_grid[tile];
+        const BGrid G = B.grids[g];
+        const Box b = box_of(G);
+        const Tile t = tile_at<R>(b, tile - G.tile0);
+        fill_tile<R>(lds, fs + G.off, b, t);
+        advect_point<R>(out + G.off, lds, fs + G.off, shift(fn, G.off), vx + G.off, vy + G.off, vz + G.off, dt[g], b, t,
+                     inv_of(G), stage);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// extension velocity: upwind normal (upwind_normal.cu:4-104) rounded to T, then
+// v = sign(F) n/max(|n|, floor) (WENO5RK3Extend.m:23-31), as the single-grid
+// k_upwind_normal + k_extension_velocity pair computes it.  Points without
+// subcell distances in a tile pass, listed points in a list pass.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void store_extension_velocity(T* vx, T* vy, T* vz, const T* F, idx_t i, const double n[3],
+                                                         double floor_)
+{
+    const double fx = (double)(T)n[0], fy = (double)(T)n[1], fz = (double)(T)n[2];
+    const double g = dmax(sqrt(fx * fx + fy * fy + fz * fz), floor_);
+    const double s = msign((double)F[i]);
+    vx[i] = (T)(s * (fx / g));
+    vy[i] = (T)(s * (fy / g));
+    vz[i] = (T)(s * (fz / g));
+}
+
+template <int R, typename T>
+__global__ void __launch_bounds__(256) kb_ext_vel_bulk(T* __restrict__ vx, T* __restrict__ vy, T* __restrict__ vz,
+                                                       const T* __restrict__ F, const uint8_t* __restrict__ flags,
+                                                       double floor_, BatchView B)
+{
+    const TileRange tr = tile_range(B.ntiles);
+    for (int tile = tr.first; tile < tr.end; tile += tr.step) {
+        const BGrid G = B.grids[B.tile_grid[tile]];
+        const Box b = box_of(G);
+        const Tile t = tile_at<R>(b, tile - G.tile0);
+        if (!t.valid) continue;
+        const idx_t ind = lin(b, t.r, t.c, t.p);
+        if (flags[G.off + ind] & FLAG_SUBCELL) continue;
+        const double sp[3] = {b.dx, b.dy, b.dz};
+        double n[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            double v[7], fo, bk;
+            line7(F + G.off, t.r, t.c, t.p, a, b, v);
+            weno5_pair(fo, bk, v, 1.0 / sp[a]);
+            n[a] = (fabs(v[4]) < fabs(v[2])) ? fo : bk;
+        }
+        store_extension_velocity(vx, vy, vz, F, (idx_t)(G.off + ind), n, floor_);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) kb_ext_vel_list(T* __restrict__ vx, T* __restrict__ vy, T* __restrict__ vz,
+                                                       const T* __restrict__ F, IList il, double floor_, BatchView B)
+{
+    const int cnt = *il.count;
+    for (int j = (int)(blockIdx.x * 256 + threadIdx.x); j < cnt; j += (int)(gridDim.x * 256)) {
+        const int64_t gind = il.idx[j];
+        const BGrid G = B.grids[grid_of(B, gind)];
+        const Box b = box_of(G);
+        int r, c, p;
+        coords_of(b, gind - G.off, r, c, p);
+        const double sp[3] = {b.dx, b.dy, b.dz};
+        double n[3];
+#pragma unroll 1
+        for (int a = 0; a < 3; ++a) {
+            double v[7], fo, bk;
+            line7(F + G.off, r, c, p, a, b, v);
+            extend_axis(fo, bk, v, il.dist[(size_t)(2 * a) * il.cap + j], il.dist[(size_t)(2 * a + 1) * il.cap + j],
+                        sp[a], 1.0 / sp[a], 0.0, 0.0);
+            n[a] = (fabs(v[4]) < fabs(v[2])) ? fo : bk;
+        }
+        store_extension_velocity(vx, vy, vz, F, (idx_t)gind, n, floor_);
+    }
+}
+
+// interface values (boundary_interpolant.cu:102-141) at the listed points
+template <typename T>
+__global__ void __launch_bounds__(256) kb_list_interp(double* __restrict__ cp, IList il, const T* __restrict__ C,
+                                                      BatchView B)
+{
+    const int total = *il.count;
+    for (int j0 = 0; j0 < total; j0 += axis_lane_stride()) {
+        const AxisLane al = axis_lane(j0);
+        if (!(al.point >= 0 && al.point < total)) continue;
+        const int a = al.axis, j = al.point;
+        const int64_t gind = il.idx[j];
+        const BGrid G = B.grids[grid_of(B, gind)];
+        const Box b = box_of(G);
+        const double sp[3] = {b.dx, b.dy, b.dz};
+        const double rf = il.dist[(size_t)(2 * a) * il.cap + j], lb = il.dist[(size_t)(2 * a + 1) * il.cap + j];
+        Pair q{0.0, 0.0};
+        if (rf < sp[a] || lb < sp[a]) {
+            int r, c, p;
+            coords_of(b, gind - G.off, r, c, p);
+            double v[7];
+            line7(C + G.off, r, c, p, a, b, v);
+            q = sixth_interp(rf, lb, sp[a], v[0], v[1], v[2], v[3], v[4], v[5], v[6]);
+        }
+        cp[(size_t)(2 * a) * il.cap + j] = q.fore;
+        cp[(size_t)(2 * a + 1) * il.cap + j] = q.back;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) kb_list_gather_vel(double* __restrict__ lv, IList il, const T* __restrict__ vx,
+                                                          const T* __restrict__ vy, const T* __restrict__ vz)
+{
+    const int cnt = *il.count;
+    for (int j = (int)(blockIdx.x * 256 + threadIdx.x); j < cnt; j += (int)(gridDim.x * 256)) {
+        const idx_t ind = (idx_t)il.idx[j];
+        lv[j] = (double)vx[ind];
+        lv[(size_t)il.cap + j] = (double)vy[ind];
+        lv[(size_t)2 * il.cap + j] = (double)vz[ind];
+    }
+}
+
+// u.N, N = grad F/max(|grad F|,1e-14), clamped central differences
+// (set_calculus_toolbox.cu:123-136), as k_normal_velocity
+template <int R, typename T>
+__global__ void __launch_bounds__(256) kb_normal_velocity(T* __restrict__ out, const T* __restrict__ F,
+                                                          const T* __restrict__ vx, const T* __restrict__ vy,
+                                                          const T* __restrict__ vz, BatchView B)
+{
+    const TileRange tr = tile_range(B.ntiles);
+    for (int tile = tr.first; tile < tr.end; tile += tr.step) {
+        const BGrid G = B.grids[B.tile_grid[tile]];
+        const Box b = box_of(G);
+        const Tile t = tile_at<R>(b, tile - G.tile0);
+        if (!t.valid) continue;
+        const int r = t.r, c = t.c, p = t.p;
+        const T* f = F + G.off;
+        const idx_t ind = G.off + lin(b, r, c, p);
+        const double fx = ((double)f[lin(b, r, min(c + 1, b.cols - 1), p)] - (double)f[lin(b, r, max(c - 1, 0), p)]) /
+                          (2 * b.dx);
+        const double fy = ((double)f[lin(b, min(r + 1, b.rows - 1), c, p)] - (double)f[lin(b, max(r - 1, 0), c, p)]) /
+                          (2 * b.dy);
+        const double fz = ((double)f[lin(b, r, c, min(p + 1, b.pges - 1))] - (double)f[lin(b, r, c, max(p - 1, 0))]) /
+                          (2 * b.dz);
+        const double g = dmax(sqrt(fx * fx + fy * fy + fz * fz), 1e-14);
+        out[ind] = (T)((double)vx[ind] * (fx / g) + (double)vy[ind] * (fy / g) + (double)vz[ind] * (fz / g));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// per-grid reductions: one compensated partial per tile, then one block per
+// grid merges its tiles' partials in tile order (deterministic)
+// ---------------------------------------------------------------------------
+// cosine-smoothed delta/H of half-width eps (src/0_0_setup/+SD/@SDF3/SDF3.m:126-136),
+// |grad F| by periodic central differences (src/0_0_setup/+SD/@GD3/GD3.m:212-223)
+template <int R, typename T>
+__global__ void __launch_bounds__(256) kb_area_volume_cosine(double* __restrict__ partial, const T* __restrict__ F,
+                                                             double eps_factor, BatchView B)
+{
+    const double pi = 3.14159265358979323846;
+    const TileRange tr = tile_range(B.ntiles);
+    for (int tile = tr.first; tile < tr.end; tile += tr.step) {
+        const BGrid G = B.grids[B.tile_grid[tile]];
+        const Box b = box_of(G);
+        const Tile t = tile_at<R>(b, tile - G.tile0);
+        const double eps = eps_factor * b.dx;
+        KSum k[2] = {{0.0, 0.0}, {0.0, 0.0}};
+        if (t.valid) {
+            const T* f = F + G.off;
+            const int r = t.r, c = t.c, p = t.p;
+            const double v = (double)f[lin(b, r, c, p)];
+            double delta = 0.0, heav;
+            if (v > eps) {
+                heav = 1.0;
+            } else if (v < -eps) {
+                heav = 0.0;
+            } else {
+                const int cp = (c + 1 == b.cols) ? 0 : c + 1, cm = (c == 0) ? b.cols - 1 : c - 1;
+                const int rp = (r + 1 == b.rows) ? 0 : r + 1, rm = (r == 0) ? b.rows - 1 : r - 1;
+                const int pp = (p + 1 == b.pges) ? 0 : p + 1, pm = (p == 0) ? b.pges - 1 : p - 1;
+                const double fx = ((double)f[lin(b, r, cp, p)] - (double)f[lin(b, r, cm, p)]) / (2 * b.dx);
+                const double fy = ((double)f[lin(b, rp, c, p)] - (double)f[lin(b, rm, c, p)]) / (2 * b.dy);
+                const double fz = ((double)f[lin(b, r, c, pp)] - (double)f[lin(b, r, c, pm)]) / (2 * b.dz);
+                const double g = sqrt(fx * fx + fy * fy + fz * fz);
+                delta = g * (1 + cos(pi * v / eps)) / eps / 2;
+                heav = (1 + v / eps + sin(pi * v / eps) / pi) / 2;
+            }
+            k[0].add(delta);
+            k[1].add(1 - heav);
+        }
+        block_reduce<2>(k);
+        if (threadIdx.x == 0) {
+            partial[(size_t)tile * 4 + 0] = k[0].s;
+            partial[(size_t)tile * 4 + 1] = k[0].c;
+            partial[(size_t)tile * 4 + 2] = k[1].s;
+            partial[(size_t)tile * 4 + 3] = k[1].c;
+        }
+        __syncthreads();
+    }
+}
+
+// one block per grid: result[2g+q] = (sum of its tiles' partials) * dx*dy*dz
+__global__ void __launch_bounds__(256) kb_grid_sum2(double* __restrict__ result, const double* __restrict__ partial,
+                                                    BatchView B)
+{
+    const BGrid G = B.grids[blockIdx.x];
+    const int t0 = G.tile0, nt = G.ntiles;
+    KSum k[2] = {{0.0, 0.0}, {0.0, 0.0}};
+    for (int i = (int)threadIdx.x; i < nt; i += 256)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            KSum o{partial[(size_t)(t0 + i) * 4 + 2 * q], partial[(size_t)(t0 + i) * 4 + 2 * q + 1]};
+            k[q].merge(o);
+        }
+    block_reduce<2>(k);
+    if (threadIdx.x == 0) {
+        const double vol = G.dx * G.dy * G.dz;
+        result[2 * blockIdx.x] = (k[0].s + k[0].c) * vol;
+        result[2 * blockIdx.x + 1] = (k[1].s + k[1].c) * vol;
+    }
+}
+
+template <int R, typename T>
+__global__ void __launch_bounds__(256) kb_max_abs3(double* __restrict__ partial, const T* __restrict__ vx,
+                                                   const T* __restrict__ vy, const T* __restrict__ vz, BatchView B)
+{
+    __shared__ double red[4];
+    const TileRange tr = tile_range(B.ntiles);
+    for (int tile = tr.first; tile < tr.end; tile += tr.step) {
+        const BGrid G = B.grids[B.tile_grid[tile]];
+        const Box b = box_of(G);
+        const Tile t = tile_at<R>(b, tile - G.tile0);
+        double m = 0.0;
+        if (t.valid) {
+            const idx_t i = G.off + lin(b, t.r, t.c, t.p);
+            m = dmax(fabs((double)vx[i]), dmax(fabs((double)vy[i]), fabs((double)vz[i])));
+        }
+#pragma unroll
+        for (int s = 32; s >= 1; s >>= 1) m = dmax(m, __shfl_xor(m, s, 64));
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+        __syncthreads();
+        if (threadIdx.x == 0) partial[tile] = dmax(dmax(red[0], red[1]), dmax(red[2], red[3]));
+        __syncthreads();
+    }
+}
+
+// one block per grid: dt[g] = cfl*dx/max|u| (the CFL step of
+// p00_test_singlephase_relax.m:89, in that operation order)
+__global__ void __launch_bounds__(256) kb_grid_dt(double* __restrict__ dt, const double* __restrict__ partial,
+                                                  double cfl, BatchView B)
+{
+    __shared__ double red[4];
+    const BGrid G = B.grids[blockIdx.x];
+    const int t0 = G.tile0, nt = G.ntiles;
+    double m = 0.0;
+    for (int i = (int)threadIdx.x; i < nt; i += 256) m = dmax(m, partial[t0 + i]);
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) m = dmax(m, __shfl_xor(m, s, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) dt[blockIdx.x] = cfl * G.dx / dmax(dmax(red[0], red[1]), dmax(red[2], red[3]));
+}
+
+// ---------------------------------------------------------------------------
+// synthetic inputs
+// ---------------------------------------------------------------------------
+template <int R, typename T>
+__global__ void __launch_bounds__(256) kb_curl_velocity(T* __restrict__ vx, T* __restrict__ vy, T* __restrict__ vz,
+                                                        const double* __restrict__ modes, int nmodes,
+                                                        const double* __restrict__ lo, BatchView B)
+{
+    const TileRange tr = tile_range(B.ntiles);
+    for (int tile = tr.first; tile < tr.end; tile += tr.step) {
+        const int g = B.tile_grid[tile];
+        const BGrid G = B.grids[g];
+        const Box b = box_of(G);
+        const Tile t = tile_at<R>(b, tile - G.tile0);
+        if (!t.valid) continue;
+        const double x = lo[3 * g] + t.c * b.dx, y = lo[3 * g + 1] + t.r * b.dy, z = lo[3 * g + 2] + t.p * b.dz;
+        double u = 0.0, v = 0.0, w = 0.0;
+        const double* md = modes + (size_t)g * nmodes * 7;
+        for (int m = 0; m < nmodes; ++m, md += 7) {
+            const double s = -sin(md[0] * x + md[1] * y + md[2] * z + md[6]);
+            u += s * md[3];
+            v += s * md[4];
+            w += s * md[5];
+        }
+        const idx_t i = G.off + lin(b, t.r, t.c, t.p);
+        vx[i] = (T)u;
+        vy[i] = (T)v;
+        vz[i] = (T)w;
+    }
+}
+
+template <int R, typename T>
+__global__ void __launch_bounds__(256) kb_ellipsoid(T* __restrict__ F, const double* __restrict__ axes,
+                                                    const double* __restrict__ lo, BatchView B)
+{
+    const TileRange tr = tile_range(B.ntiles);
+    for (int tile = tr.first; tile < tr.end; tile += tr.step) {
+        const int g = B.tile_grid[tile];
+        const BGrid G = B.grids[g];
+        const Box b = box_of(G);
+        const Tile t = tile_at<R>(b, tile - G.tile0);
+        if (!t.valid) continue;
+        const double x = lo[3 * g] + t.c * b.dx, y = lo[3 * g + 1] + t.r * b.dy, z = lo[3 * g + 2] + t.p * b.dz;
+        const double a = axes[3 * g], bb = axes[3 * g + 1], c = axes[3 * g + 2];
+        F[G.off + lin(b, t.r, t.c, t.p)] = (T)(sqrt(x * x / (a * a) + y * y / (bb * bb) + z * z / (c * c)) - 1.0);
+    }
+}
+
+// ===========================================================================
+// host launchers
+// ===========================================================================
+// a tile-loop kernel with the batch's tile shape (B and s in scope)
+#define TR_LAUNCH(KERNEL, T, GRID, ...)                                 \
+    do {                                                                \
+        if (B.tr == 32) KERNEL<32, T><<<GRID, 256, 0, s>>>(__VA_ARGS__); \
+        else KERNEL<64, T><<<GRID, 256, 0, s>>>(__VA_ARGS__);           \
+    } while (0)
+
+static inline int list_blocks(const BatchView& B)
+{
+    // one sweep of an interface-list launch covers 84 points per block
+    const int b = B.ntiles / 12;
+    return b < 16 ? 16 : (b > 4096 ? 4096 : b);
+}
+
+static inline IList ilist(const ListView& lv) { return IList{lv.idx, lv.count, lv.dist, lv.dt, lv.cap}; }
+
+template <typename T>
+hipError_t launch_batch_build_list(const BatchView& B, const T* F, const T* mask_src, uint8_t* flags,
+                                   const ListView& lv, hipStream_t s)
+{
+    TR_LAUNCH(kb_build_list, T, B.blocks, F, mask_src, flags, lv, B);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_batch_set_mask_bit(uint8_t* flags, const T* F, int64_t n, hipStream_t s)
+{
+    kb_set_mask_bit<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(flags, F, (idx_t)n);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_batch_reinit_stage(const BatchView& B, T* out, const T* fs, const T* fn, const uint8_t* flags,
+                                     const ListView& lv, int stage, int* over, int* over_count, int seq,
+                                     hipStream_t s)
+{
+    TR_LAUNCH(kb_reinit_bulk, T, B.blocks, out, fs, fn, flags, B, stage, over, over_count, seq);
+    kb_reinit_iface<T><<<list_blocks(B), 256, 0, s>>>(out, fs, fn, flags, ilist(lv), B, stage, (const int*)over,
+                                                      (const int*)over_count, seq);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_batch_extend_stage(const BatchView& B, T* out, const T* cs, const T* cn, const T* vx,
+                                     const T* vy, const T* vz, const uint8_t* flags, const ListView& lv,
+                                     const double* lcp, const double* lvel, int stage, hipStream_t s)
+{
+    TR_LAUNCH(kb_extend_bulk, T, B.blocks, out, cs, cn, vx, vy, vz, flags, B, stage);
+    kb_extend_iface<T><<<list_blocks(B), 256, 0, s>>>(out, cs, cn, ilist(lv), lcp, lvel, B, stage);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_batch_advect_stage(const BatchView& B, T* out, const T* fs, const T* fn, const T* vx,
+                                     const T* vy, const T* vz, const double* dt, int stage, hipStream_t s)
+{
+    TR_LAUNCH(kb_advect_stage, T, B.blocks, out, fs, fn, vx, vy, vz, dt, B, stage);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_batch_extension_velocity(const BatchView& B, T* vx, T* vy, T* vz, const T* F,
+                                           const uint8_t* flags, const ListView& lv, double floor_, hipStream_t s)
+{
+    TR_LAUNCH(kb_ext_vel_bulk, T, B.blocks, vx, vy, vz, F, flags, floor_, B);
+    kb_ext_vel_list<T><<<list_blocks(B), 256, 0, s>>>(vx, vy, vz, F, ilist(lv), floor_, B);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_batch_list_interp(const BatchView& B, double* lcp, const ListView& lv, const T* C, hipStream_t s)
+{
+    kb_list_interp<T><<<list_blocks(B), 256, 0, s>>>(lcp, ilist(lv), C, B);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_batch_list_gather_vel(double* lvel, const ListView& lv, const T* vx, const T* vy, const T* vz,
+                                        hipStream_t s)
+{
+    kb_list_gather_vel<T><<<1024, 256, 0, s>>>(lvel, ilist(lv), vx, vy, vz);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_batch_normal_velocity(const BatchView& B, T* out, const T* F, const T* vx, const T* vy,
+                                        const T* vz, hipStream_t s)
+{
+    TR_LAUNCH(kb_normal_velocity, T, B.blocks, out, F, vx, vy, vz, B);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_batch_area_volume_cosine(const BatchView& B, double* result, double* partial, const T* F,
+                                           double eps_factor, hipStream_t s)
+{
+    TR_LAUNCH(kb_area_volume_cosine, T, B.blocks, partial, F, eps_factor, B);
+    kb_grid_sum2<<<B.ngrids, 256, 0, s>>>(result, partial, B);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_batch_cfl_dt(const BatchView& B, double* dt, double* partial, const T* vx, const T* vy,
+                               const T* vz, double cfl, hipStream_t s)
+{
+    TR_LAUNCH(kb_max_abs3, T, B.blocks, partial, vx, vy, vz, B);
+    kb_grid_dt<<<B.ngrids, 256, 0, s>>>(dt, partial, cfl, B);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_batch_curl_velocity(const BatchView& B, T* vx, T* vy, T* vz, const double* modes, int nmodes,
+                                      const double* lo, hipStream_t s)
+{
+    TR_LAUNCH(kb_curl_velocity, T, B.blocks, vx, vy, vz, modes, nmodes, lo, B);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_batch_ellipsoid(const BatchView& B, T* F, const double* axes, const double* lo, hipStream_t s)
+{
+    TR_LAUNCH(kb_ellipsoid, T, B.blocks, F, axes, lo, B);
+    return hipGetLastError();
+}
+
+#define VLS_BATCH_INSTANTIATE(T)                                                                                    \
+    template hipError_t launch_batch_build_list<T>(const BatchView&, const T*, const T*, uint8_t*, const ListView&, \
+                                                   hipStream_t);                                                    \
+    template hipError_t launch_batch_set_mask_bit<T>(uint8_t*, const T*, int64_t, hipStream_t);                     \
+    template hipError_t launch_batch_reinit_stage<T>(const BatchView&, T*, const T*, const T*, const uint8_t*,      \
+                                                     const ListView&, int, int*, int*, int, hipStream_t);           \
+    template hipError_t launch_batch_extend_stage<T>(const BatchView&, T*, const T*, const T*, const T*, const T*,  \
+                                                     const T*, const uint8_t*, const ListView&, const double*,      \
+                                                     const double*, int, hipStream_t);                              \
+    template hipError_t launch_batch_advect_stage<T>(const BatchView&, T*, const T*, const T*, const T*, const T*,  \
+                                                     const T*, const double*, int, hipStream_t);                    \
+    template hipError_t launch_batch_extension_velocity<T>(const BatchView&, T*, T*, T*, const T*, const uint8_t*,  \
+                                                           const ListView&, double, hipStream_t);                   \
+    template hipError_t launch_batch_list_interp<T>(const BatchView&, double*, const ListView&, const T*,           \
+                                                    hipStream_t);                                                   \
+    template hipError_t launch_batch_list_gather_vel<T>(double*, const ListView&, const T*, const T*, const T*,     \
+                                                        hipStream_t);                                               \
+    template hipError_t launch_batch_normal_velocity<T>(const BatchView&, T*, const T*, const T*, const T*,         \
+                                                        const T*, hipStream_t);                                     \
+    template hipError_t launch_batch_area_volume_cosine<T>(const BatchView&, double*, double*, const T*, double,    \
+                                                           hipStream_t);                                            \
+    template hipError_t launch_batch_cfl_dt<T>(const BatchView&, double*, double*, const T*, const T*, const T*,    \
+                                               double, hipStream_t);                                                \
+    template hipError_t launch_batch_curl_velocity<T>(const BatchView&, T*, T*, T*, const double*, int,             \
+                                                      const double*, hipStream_t);                                  \
+    template hipError_t launch_batch_ellipsoid<T>(const BatchView&, T*, const double*, const double*, hipStream_t);
+
+VLS_BATCH_INSTANTIATE(double)
+VLS_BATCH_INSTANTIATE(float)
+#undef VLS_BATCH_INSTANTIATE
+
+}  // namespace vls

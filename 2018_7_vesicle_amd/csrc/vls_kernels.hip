@@ -16,57 +16,10 @@
 //     pseudo-time step 0.3*min_dist from them.
 #include <algorithm>
 
-#include "vls_device.hpp"
-#include "vls_kernels.hpp"
+#include "vls_tile.hpp"
 
 namespace vls {
 
-// ---------------------------------------------------------------------------
-// launch geometry helpers
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ bool point_of_thread(const Box& b, int& r, int& c, int& p)
-{
-    r = blockIdx.x * blockDim.x + threadIdx.x;
-    c = blockIdx.y * blockDim.y + threadIdx.y;
-    p = blockIdx.z;
-    return r < b.rows && c < b.cols;
-}
-
-__device__ __forceinline__ idx_t lin(const Box& b, int r, int c, int p)
-{
-    return (idx_t)p * b.rows * b.cols + (idx_t)c * b.rows + r;
-}
-
-// 1/dx, 1/dy, 1/dz and the pseudo-time step of a point without subcell
-// distances, 0.3*min(dx,dx,dy,dy,dz,dz) (setDistance.m:17-21 with every
-// distance equal to the spacing), computed once on the host
-struct Inv {
-    double i[3];
-    double dt_bulk;
-};
-
-// RK combinations in the order the MATLAB wrappers evaluate them
-// (+SD/@SDF3/WENO5RK3Reinitialization.m:20-37), without fused multiply-adds.
-__device__ __forceinline__ double rk_combine(int stage, double fs, double step, double fn)
-{
-#pragma clang fp contract(off)
-    const double t = fs - step;
-    if (stage == 0) return t;
-    if (stage == 1) return 3. / 4. * fn + 1. / 4. * t;
-    return 1. / 3. * fn + 2. / 3. * t;
-}
-
-__device__ __forceinline__ float rk_combine(int stage, float fs, float step, float fn)
-{
-    const float t = fs - step;
-    if (stage == 0) return t;
-    if (stage == 1) return fmaf(0.75f, fn, 0.25f * t);
-    return fmaf(1.f / 3.f, fn, (2.f / 3.f) * t);
-}
-
-// the interface test of re_step.cu:260-267 on a pair of nodal values
-__device__ __forceinline__ bool sign_change(double a, double b) { return a * b < 0; }
-__device__ __forceinline__ bool sign_change(float a, float b) { return opposite(a, b); }
 
 // ---------------------------------------------------------------------------
 // setDistance: fill, crossing distances, min_dist + flags
@@ -111,18 +64,6 @@ __global__ void __launch_bounds__(256) k_boundary_location(Dist6 d, const T* __r
     }
 }
 
-// append `take` lanes' values to list[*count ...] with one atomic per wave
-__device__ __forceinline__ void wave_append(bool take, int value, int* __restrict__ list, int* __restrict__ count)
-{
-    const unsigned long long m = __ballot(take);
-    if (m == 0) return;
-    const int lane = (int)__lane_id();
-    const int leader = __ffsll((long long)m) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(count, __popcll(m));
-    base = __shfl(base, leader, 64);
-    if (take) list[base + __popcll(m & ((1ull << lane) - 1ull))] = value;
-}
 
 // min_dist (setDistance.m:17-21), the per-point flags of the fused kernels and
 // the list of interface points (some distance below the spacing).
@@ -166,52 +107,6 @@ __global__ void k_set_mask_bit(uint8_t* __restrict__ flags, const T* __restrict_
 // ---------------------------------------------------------------------------
 // WENO5 reinitialization residual (re_step.cu:283-393)
 // ---------------------------------------------------------------------------
-struct Dists {
-    double r[3], l[3];
-};
-
-// an interface list in the form the stage kernels read: indices and, in list
-// order, the six distances (row k of `dist` is xpr,xpl,ypf,ypb,zpu,zpd for
-// k = 0..5, rows `cap` apart) and the pseudo-time step 0.3*min_dist, or -1 for
-// a frozen point
-struct IList {
-    const int* idx;
-    const int* count;
-    const double* dist;
-    const double* dt;
-    int cap;
-};
-struct IListW {
-    const int* idx;
-    const int* count;
-    double* dist;
-    double* dt;
-    int cap;
-};
-
-__device__ __forceinline__ bool frozen(const Dists& q, double dx)
-{
-    const double eps = 1e-6 * dx;
-    return q.r[0] < eps || q.l[0] < eps || q.r[1] < eps || q.l[1] < eps || q.r[2] < eps || q.l[2] < eps;
-}
-
-__device__ __forceinline__ double min6(const Dists& q)
-{
-    return dmin(dmin(q.r[0], q.l[0]), dmin(dmin(q.r[1], q.l[1]), dmin(q.r[2], q.l[2])));
-}
-
-// per-axis derivative pair with the sign-change interface test (re_step.cu:248-269)
-__device__ __forceinline__ void reinit_axis(double& fore, double& back, const double v[7], double r1, double l1,
-                                            double ds, double inv_ds)
-{
-    if (v[2] * v[3] < 0 || v[3] * v[4] < 0) {
-        const Pair q = weno_nonuniform_pair(v, r1, l1, ds, 0.0, 0.0);
-        fore = q.fore;
-        back = q.back;
-    } else {
-        weno5_pair(fore, back, v, inv_ds);
-    }
-}
 
 __device__ __forceinline__ double reinit_residual(const double* __restrict__ f, int r, int c, int p, const Box& b,
                                                   const Dists& q, bool inside, double dt)
@@ -248,183 +143,29 @@ __global__ void __launch_bounds__(256) k_re_step(double* __restrict__ step, cons
 // uniform WENO5 only (few registers, high occupancy); interface kernels
 // stride over a list of points and evaluate the full residual.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ bool bulk_point(const Box& b, int blk, int& r, int& c, int& p)
-{
-    const int ntr = (b.rows + 63) >> 6, ntc = (b.cols + 3) >> 2;
-    const int rb = blk % ntr;
-    const int t = blk / ntr;
-    const int cb = t % ntc;
-    p = t / ntc;
-    r = rb * 64 + (int)(threadIdx.x & 63);
-    c = cb * 4 + (int)(threadIdx.x >> 6);
-    return r < b.rows && c < b.cols;
-}
 
-__device__ __forceinline__ void coords_of(const Box& b, idx_t ind, int& r, int& c, int& p)
-{
-    const idx_t plane = (idx_t)b.rows * b.cols;
-    p = (int)(ind / plane);
-    const int rem = (int)(ind - (idx_t)p * plane);
-    c = rem / b.rows;
-    r = rem - c * b.rows;
-}
 
-__device__ __forceinline__ void load_dists(Dists& q, const Dist6c& d, idx_t ind)
-{
-#pragma unroll
-    for (int a = 0; a < 3; ++a) { q.r[a] = d.r[a][ind]; q.l[a] = d.l[a][ind]; }
-}
-
-// ---------------------------------------------------------------------------
-// plane tiles: a block covers 64 x 4 (rows x cols) points of one page.  The
-// page's values with a 3-point halo in rows and cols are staged in LDS once
-// (clamped there), so the x- and y-taps of the WENO stencil are LDS reads at
-// constant offsets; the z-taps are global loads at a wave-uniform page offset.
-// ---------------------------------------------------------------------------
-constexpr int TR = 64, TC = 4, HALO = 3, LR = TR + 2 * HALO, LC = TC + 2 * HALO;
-
-struct Tile {
-    int r0, c0, p;  // tile origin (row, col) and page
-    int r, c;       // this thread's point
-    bool valid;
-    int base;       // LDS index of this thread's point
-};
-
-// Blocks are dispatched round-robin over the 8 XCDs (block i to XCD i % 8),
-// each with its own L2.  Give every XCD a contiguous range of tiles instead,
-// so neighbouring tiles share their halos in one L2.
-__device__ __forceinline__ int xcd_remap(int blk, int nblk)
-{
-    if ((nblk & 7) != 0) return blk;
-    return (blk & 7) * (nblk >> 3) + (blk >> 3);
-}
-
-__device__ __forceinline__ Tile tile_of(const Box& b)
-{
-    const int ntr = (b.rows + TR - 1) / TR, ntc = (b.cols + TC - 1) / TC;
-    const int blk = xcd_remap((int)blockIdx.x, (int)gridDim.x);
-    Tile t;
-    const int rb = blk % ntr;
-    const int q = blk / ntr;
-    const int cb = q % ntc;
-    t.p = q / ntc;
-    t.r0 = rb * TR;
-    t.c0 = cb * TC;
-    const int ty = (int)(threadIdx.x & 63), tx = (int)(threadIdx.x >> 6);
-    t.r = t.r0 + ty;
-    t.c = t.c0 + tx;
-    t.valid = t.r < b.rows && t.c < b.cols;
-    t.base = (tx + HALO) * LR + ty + HALO;
-    return t;
-}
-
-// stage the tile of page t.p of f (with clamped halo) in lds[LC*LR]; ends with a barrier
-template <typename T>
-__device__ __forceinline__ void fill_tile(T* lds, const T* __restrict__ f, const Box& b, const Tile& t)
-{
-    const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
-    const T* fp = f + (idx_t)t.p * b.rows * b.cols;
-    const int rlo = clampi(t.r0 + lane - HALO, b.rows - 1);
-    const int rhi = clampi(t.r0 + TR + lane - HALO, b.rows - 1);
-    for (int lc = wave; lc < LC; lc += 4) {
-        const T* col = fp + (idx_t)clampi(t.c0 + lc - HALO, b.cols - 1) * b.rows;
-        lds[lc * LR + lane] = col[rlo];
-        if (lane < 2 * HALO) lds[lc * LR + TR + lane] = col[rhi];
-    }
-    __syncthreads();
-}
-
-template <typename T>
-__device__ __forceinline__ void tile_line_y(const T* lds, int base, T v[7])
-{
-#pragma unroll
-    for (int k = 0; k < 7; ++k) v[k] = lds[base + k - 3];
-}
-
-template <typename T>
-__device__ __forceinline__ void tile_line_x(const T* lds, int base, T v[7])
-{
-#pragma unroll
-    for (int k = 0; k < 7; ++k) v[k] = lds[base + (k - 3) * LR];
-}
-
-// z-taps: page offsets are the same for the whole block
-template <typename T>
-__device__ __forceinline__ void tile_line_z(const T* __restrict__ f, const Box& b, const Tile& t, T v[7])
-{
-    const idx_t plane = (idx_t)b.rows * b.cols;
-    const idx_t inplane = (idx_t)t.c * b.rows + t.r;
-#pragma unroll
-    for (int k = 0; k < 7; ++k) v[k] = f[(idx_t)clampi(t.p + k - 3, b.pges - 1) * plane + inplane];
-}
-
-__device__ __forceinline__ int tile_blocks(const Box& b)
-{
-    return ((b.rows + TR - 1) / TR) * ((b.cols + TC - 1) / TC) * b.pges;
-}
 
 // WENO5-RK3 reinitialization stage, launch 1 of 2: the bulk.  A point with
 // subcell distances is left to launch 2; so is a point whose stage field
 // changes sign next to it although it has none (it needs the nonuniform
 // stencil): those are appended to `over`, counted in over_count[seq & 1].
 // The other slot is cleared here for the next stage.
-template <typename T>
+template <int R, typename T>
 __global__ void __launch_bounds__(256) k_reinit_bulk(T* __restrict__ out, const T* __restrict__ fs,
                                                      const T* __restrict__ fn, const uint8_t* __restrict__ flags,
                                                      Box b, Inv invs, int stage, int* __restrict__ over,
                                                      int* __restrict__ over_count, int seq)
 {
-    __shared__ T lds[LC * LR];
+    __shared__ T lds[Shape<R>::N];
     if (blockIdx.x == 0 && threadIdx.x == 0) over_count[(seq + 1) & 1] = 0;
-    const Tile t = tile_of(b);
-    fill_tile(lds, fs, b, t);
+    const Tile t = tile_of<R>(b);
+    fill_tile<R>(lds, fs, b, t);
     const idx_t ind = t.valid ? lin(b, t.r, t.c, t.p) : 0;
-    const uint8_t fl = t.valid ? flags[ind] : (uint8_t)FLAG_SUBCELL;
-    bool defer = false;
-    if (!(fl & FLAG_SUBCELL)) {
-        T fo[3], bk[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            T v[7];
-            if (a == 0) tile_line_x(lds, t.base, v);
-            else if (a == 1) tile_line_y(lds, t.base, v);
-            else tile_line_z(fs, b, t, v);
-            defer = defer || (sign_change(v[2], v[3]) || sign_change(v[3], v[4]));
-            weno5_pair(fo[a], bk[a], v, (T)invs.i[a]);
-        }
-        if (!defer) {
-            const T step = godunov_step((fl & FLAG_MASK) != 0, fo[0], bk[0], fo[1], bk[1], fo[2], bk[2],
-                                        (T)invs.dt_bulk);
-            out[ind] = rk_combine(stage, lds[t.base], step, (stage == 0) ? (T)0 : fn[ind]);
-        }
-    }
+    const bool defer = reinit_bulk_point<R>(out, lds, fs, fn, flags, b, t, ind, invs, stage);
     wave_append(defer, (int)ind, over, &over_count[seq & 1]);
 }
 
-// Interface work is spread over three lanes per point, one per axis (21
-// points per wave64, lane 63 idle): the per-axis chains are long (the
-// nonuniform WENO divides ~30 times) and the lists short, so this triples the
-// parallelism.  The axis results meet in the group's first lane, which adds
-// them in the reference's x, y, z order.
-struct AxisLane {
-    int point;  // index into the point list, or -1
-    int axis;   // 0 x, 1 y, 2 z
-    int base;   // lane of the group's x member
-};
-
-__device__ __forceinline__ AxisLane axis_lane(int j0)
-{
-    const int lane = (int)(threadIdx.x & 63);
-    const int wave = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
-    AxisLane a;
-    a.axis = lane % 3;
-    a.base = lane - a.axis;
-    a.point = (lane < 63) ? j0 + wave * 21 + lane / 3 : -1;
-    return a;
-}
-
-// stride, in points, between the groups a grid visits per sweep
-__device__ __forceinline__ int axis_lane_stride() { return (int)(gridDim.x * (blockDim.x >> 6)) * 21; }
 
 // launch 2 of 2: the interface list, then the deferred bulk points (no
 // subcell distances: spacing distances and the bulk pseudo-time step)
@@ -437,44 +178,16 @@ __global__ void __launch_bounds__(256) k_reinit_iface(T* __restrict__ out, const
 {
     const int n1 = *il.count, n2 = over_count[seq & 1];
     const int total = n1 + n2;
-    const double sp[3] = {b.dx, b.dy, b.dz};
     for (int j0 = 0; j0 < total; j0 += axis_lane_stride()) {
         const AxisLane al = axis_lane(j0);
         const bool active = al.point >= 0 && al.point < total;
-        const bool listed = al.point < n1;
-        double m = 0.0;  // this axis's Godunov term
-        double dt = inv.dt_bulk;
-        idx_t ind = 0;
-        bool inside = false;
+        ReinitLane q;
         if (active) {
-            ind = (idx_t)(listed ? il.idx[al.point] : over[al.point - n1]);
-            int r, c, p;
-            coords_of(b, ind, r, c, p);
-            inside = (flags[ind] & FLAG_MASK) != 0;
-            const int a = al.axis;
-            double r1 = sp[a], l1 = sp[a];
-            if (listed) {
-                dt = il.dt[al.point];
-                r1 = il.dist[(size_t)(2 * a) * il.cap + al.point];
-                l1 = il.dist[(size_t)(2 * a + 1) * il.cap + al.point];
-            }
-            if (dt >= 0) {
-                double v[7], fo, bk;
-                line7(fs, r, c, p, a, b, v);
-                reinit_axis(fo, bk, v, r1, l1, sp[a], inv.i[a]);
-                m = inside ? dmax(sq(dmin(0.0, bk)), sq(dmax(0.0, fo))) : dmax(sq(dmax(0.0, bk)), sq(dmin(0.0, fo)));
-            }
+            const bool listed = al.point < n1;
+            const idx_t ind = (idx_t)(listed ? il.idx[al.point] : over[al.point - n1]);
+            q = reinit_iface_lane(fs, flags, b, inv, il, listed ? al.point : -1, ind, ind, al.axis);
         }
-        const double my = __shfl(m, al.base + 1, 64), mz = __shfl(m, al.base + 2, 64);
-        if (active && al.axis == 0) {
-            double step = 0.0;  // frozen point (dt < 0)
-            if (dt >= 0) {
-                // godunov_step: (sqrt((mx + my) + mz) - 1) * dt, negated inside
-                const double s1 = (sqrt((m + my) + mz) - 1) * dt;
-                step = inside ? -s1 : s1;
-            }
-            out[ind] = (T)rk_combine(stage, (double)fs[ind], step, (stage == 0) ? 0.0 : (double)fn[ind]);
-        }
+        reinit_iface_finish(out, fs, fn, q, al, active, stage);
     }
 }
 
@@ -594,18 +307,6 @@ __global__ void __launch_bounds__(256) k_eno2_stage(double* __restrict__ out, co
 // ---------------------------------------------------------------------------
 // extension (2_2_WENO5RK3_Extend)
 // ---------------------------------------------------------------------------
-// per-axis pair with the distance interface test (shared_utilities.cup:256-278)
-__device__ __forceinline__ void extend_axis(double& fore, double& back, const double v[7], double r1, double l1,
-                                            double ds, double inv_ds, double h_fore, double h_back)
-{
-    if (r1 < ds || l1 < ds) {
-        const Pair q = weno_nonuniform_pair(v, r1, l1, ds, h_fore, h_back);
-        fore = q.fore;
-        back = q.back;
-    } else {
-        weno5_pair(fore, back, v, inv_ds);
-    }
-}
 
 // upwind_normal: upwind_normal.cu:4-104
 template <typename T>
@@ -702,28 +403,16 @@ __global__ void __launch_bounds__(256) k_extend_step(double* __restrict__ step, 
 }
 
 // fused WENO5-RK3 extension stage, bulk: every point without subcell distances
-template <typename T>
+template <int R, typename T>
 __global__ void __launch_bounds__(256) k_extend_bulk(T* __restrict__ out, const T* __restrict__ cs,
                                                      const T* __restrict__ cn, const T* __restrict__ vx,
                                                      const T* __restrict__ vy, const T* __restrict__ vz,
                                                      const uint8_t* __restrict__ flags, Box b, Inv inv, int stage)
 {
-    __shared__ T lds[LC * LR];
-    const Tile t = tile_of(b);
-    fill_tile(lds, cs, b, t);
-    if (!t.valid) return;
-    const idx_t ind = lin(b, t.r, t.c, t.p);
-    if (flags[ind] & FLAG_SUBCELL) return;
-    const T vel[3] = {vx[ind], vy[ind], vz[ind]};
-    T acc = 0;
-    T v[7];
-    tile_line_x(lds, t.base, v);
-    acc = upwind_axis(acc, vel[0], v, (T)inv.i[0]);
-    tile_line_y(lds, t.base, v);
-    acc = upwind_axis(acc, vel[1], v, (T)inv.i[1]);
-    tile_line_z(cs, b, t, v);
-    acc = upwind_axis(acc, vel[2], v, (T)inv.i[2]);
-    out[ind] = rk_combine(stage, lds[t.base], acc * (T)inv.dt_bulk, (stage == 0) ? (T)0 : cn[ind]);
+    __shared__ T lds[Shape<R>::N];
+    const Tile t = tile_of<R>(b);
+    fill_tile<R>(lds, cs, b, t);
+    extend_bulk_point<R>(out, lds, cs, cn, vx, vy, vz, flags, b, t, inv, stage);
 }
 
 // fused WENO5-RK3 extension stage, interface list: three lanes per point
@@ -737,41 +426,15 @@ __global__ void __launch_bounds__(256) k_extend_iface(T* __restrict__ out, const
                                                       Box b, Inv inv, int stage)
 {
     const int total = *il.count;
-    const double sp[3] = {b.dx, b.dy, b.dz};
     for (int j0 = 0; j0 < total; j0 += axis_lane_stride()) {
         const AxisLane al = axis_lane(j0);
         const bool active = al.point >= 0 && al.point < total;
-        double t = 0.0;  // this axis's term coef * derivative
-        double dt = -1.0;
-        idx_t ind = 0;
+        ExtendLane q;
         if (active) {
-            const int j = al.point, a = al.axis;
-            ind = (idx_t)il.idx[j];
-            dt = il.dt[j];
-            if (dt >= 0) {
-                int r, c, p;
-                coords_of(b, ind, r, c, p);
-                double v[7];
-                line7(cs, r, c, p, a, b, v);
-                const double vel = lv[(size_t)a * il.cap + j];
-                const double r1 = il.dist[(size_t)(2 * a) * il.cap + j];
-                const double l1 = il.dist[(size_t)(2 * a + 1) * il.cap + j];
-                if (r1 < sp[a] || l1 < sp[a]) {
-                    const bool fore = vel < 0;
-                    const double dv = weno_nonuniform_one(v[0], v[1], v[2], v[3], v[4], v[5], v[6], r1, l1, sp[a],
-                                                          lcp[(size_t)(2 * a) * il.cap + j],
-                                                          lcp[(size_t)(2 * a + 1) * il.cap + j], !fore);
-                    t = 0.0 + (fore ? vel : dmax(0.0, vel)) * dv;
-                } else {
-                    t = upwind_axis(0.0, vel, v, inv.i[a]);
-                }
-            }
+            const idx_t ind = (idx_t)il.idx[al.point];
+            q = extend_iface_lane(cs, b, inv, il, lcp, lv, al.point, ind, ind, al.axis);
         }
-        const double ty = __shfl(t, al.base + 1, 64), tz = __shfl(t, al.base + 2, 64);
-        if (active && al.axis == 0) {
-            const double step = (dt >= 0) ? ((t + ty) + tz) * dt : 0.0;
-            out[ind] = (T)rk_combine(stage, (double)cs[ind], step, (stage == 0) ? 0.0 : (double)cn[ind]);
-        }
+        extend_iface_finish(out, cs, cn, q, al, active, stage);
     }
 }
 
@@ -805,29 +468,16 @@ __global__ void __launch_bounds__(256) k_advection_step(double* __restrict__ ast
 }
 
 // fused TVD-RK3 advection stage: out = combine(fs - dt*L(fs), fn), plane tiles
-template <typename T>
+template <int R, typename T>
 __global__ void __launch_bounds__(256) k_advect_stage(T* __restrict__ out, const T* __restrict__ fs,
                                                       const T* __restrict__ fn, const T* __restrict__ vx,
                                                       const T* __restrict__ vy, const T* __restrict__ vz,
                                                       double dt, Box b, Inv inv, int stage)
 {
-    __shared__ T lds[LC * LR];
-    const Tile t = tile_of(b);
-    fill_tile(lds, fs, b, t);
-    if (!t.valid) return;
-    const idx_t ind = lin(b, t.r, t.c, t.p);
-    const T vel[3] = {vx[ind], vy[ind], vz[ind]};
-    T acc = 0;
-    T v[7];
-    tile_line_x(lds, t.base, v);
-    acc = upwind_axis(acc, vel[0], v, (T)inv.i[0]);
-    tile_line_y(lds, t.base, v);
-    acc = upwind_axis(acc, vel[1], v, (T)inv.i[1]);
-    tile_line_z(fs, b, t, v);
-    acc = upwind_axis(acc, vel[2], v, (T)inv.i[2]);
-    const T fnv = (stage == 0) ? (T)0 : fn[ind];
-    const T step = (T)dt * acc;
-    out[ind] = rk_combine(stage, lds[t.base], step, fnv);
+    __shared__ T lds[Shape<R>::N];
+    const Tile t = tile_of<R>(b);
+    fill_tile<R>(lds, fs, b, t);
+    advect_point<R>(out, lds, fs, fn, vx, vy, vz, dt, b, t, inv, stage);
 }
 
 __global__ void __launch_bounds__(256) k_weno_derivative(Dist6 out, const double* __restrict__ lsf, Box b)
@@ -970,71 +620,6 @@ __global__ void __launch_bounds__(256) k_normal_velocity(T* __restrict__ out, co
 // ---------------------------------------------------------------------------
 // compensated reductions (area, volume, max|u|)
 // ---------------------------------------------------------------------------
-// Neumaier accumulator
-struct KSum {
-    double s, c;
-    __device__ __forceinline__ void add(double x)
-    {
-        const double t = s + x;
-        c += (fabs(s) >= fabs(x)) ? ((s - t) + x) : ((x - t) + s);
-        s = t;
-    }
-    __device__ __forceinline__ void merge(const KSum& o)
-    {
-        add(o.s);
-        c += o.c;
-    }
-};
-
-__device__ __forceinline__ double shfl_xor_d(double v, int m)
-{
-    return __shfl_xor(v, m, 64);
-}
-
-__device__ __forceinline__ void wave_reduce(KSum& k)
-{
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) {
-        KSum o;
-        o.s = shfl_xor_d(k.s, m);
-        o.c = shfl_xor_d(k.c, m);
-        k.merge(o);
-    }
-}
-
-// block of 256 threads -> thread 0 holds the merged (s,c) of each quantity
-template <int NQ>
-__device__ __forceinline__ void block_reduce(KSum (&k)[NQ])
-{
-    __shared__ double lds[NQ][2][4];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        wave_reduce(k[q]);
-        if (lane == 0) { lds[q][0][wave] = k[q].s; lds[q][1][wave] = k[q].c; }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            KSum t{0.0, 0.0};
-            for (int w = 0; w < 4; ++w) { KSum o{lds[q][0][w], lds[q][1][w]}; t.merge(o); }
-            k[q] = t;
-        }
-    }
-}
-
-template <int NQ>
-__device__ __forceinline__ void block_reduce_store(KSum (&k)[NQ], double* __restrict__ partial)
-{
-    block_reduce<NQ>(k);
-    if (threadIdx.x == 0)
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            partial[(size_t)blockIdx.x * 2 * NQ + 2 * q] = k[q].s;
-            partial[(size_t)blockIdx.x * 2 * NQ + 2 * q + 1] = k[q].c;
-        }
-}
 
 // cosine-smoothed delta/H of width eps (src/0_0_setup/+SD/@SDF3/SDF3.m:126-136);
 // |grad F| by periodic central differences (src/0_0_setup/+SD/@GD3/GD3.m:212-223)
@@ -1244,7 +829,7 @@ int iface_blocks(const Geom& g)
 
 static inline int bulk_blocks(const Geom& g)
 {
-    return ((g.rows + 63) / 64) * ((g.cols + 3) / 4) * g.pges;
+    return g.tr == 32 ? box_tiles<32>(g.rows, g.cols, g.pges) : box_tiles<64>(g.rows, g.cols, g.pges);
 }
 
 template <typename T>
@@ -1271,8 +856,12 @@ template <typename T>
 hipError_t launch_reinit_stage(T* out, const T* fs, const T* fn, const uint8_t* flags, const ListView& lv,
                                const Geom& g, int stage, int* over, int* over_count, int seq, hipStream_t s)
 {
-    k_reinit_bulk<T><<<dim3(bulk_blocks(g)), dim3(256), 0, s>>>(out, fs, fn, flags, box_of(g), inv_of(g), stage,
-                                                                 over, over_count, seq);
+    if (g.tr == 32)
+        k_reinit_bulk<32, T><<<dim3(bulk_blocks(g)), dim3(256), 0, s>>>(out, fs, fn, flags, box_of(g), inv_of(g),
+                                                                         stage, over, over_count, seq);
+    else
+        k_reinit_bulk<64, T><<<dim3(bulk_blocks(g)), dim3(256), 0, s>>>(out, fs, fn, flags, box_of(g), inv_of(g),
+                                                                         stage, over, over_count, seq);
     const IList il{lv.idx, lv.count, lv.dist, lv.dt, lv.cap};
     k_reinit_iface<T><<<dim3(iface_blocks(g)), dim3(256), 0, s>>>(out, fs, fn, flags, il, box_of(g), inv_of(g),
                                                                    stage, (const int*)over, (const int*)over_count,
@@ -1344,8 +933,12 @@ hipError_t launch_extend_stage(T* out, const T* cs, const T* cn, const T* vx, co
                                const uint8_t* flags, const ListView& lv, const double* lcp, const double* lvel,
                                const Geom& g, int stage, hipStream_t s)
 {
-    k_extend_bulk<T><<<dim3(bulk_blocks(g)), dim3(256), 0, s>>>(out, cs, cn, vx, vy, vz, flags, box_of(g),
-                                                                 inv_of(g), stage);
+    if (g.tr == 32)
+        k_extend_bulk<32, T><<<dim3(bulk_blocks(g)), dim3(256), 0, s>>>(out, cs, cn, vx, vy, vz, flags, box_of(g),
+                                                                         inv_of(g), stage);
+    else
+        k_extend_bulk<64, T><<<dim3(bulk_blocks(g)), dim3(256), 0, s>>>(out, cs, cn, vx, vy, vz, flags, box_of(g),
+                                                                         inv_of(g), stage);
     const IList il{lv.idx, lv.count, lv.dist, lv.dt, lv.cap};
     k_extend_iface<T><<<dim3(iface_blocks(g)), dim3(256), 0, s>>>(out, cs, cn, il, lcp, lvel, box_of(g), inv_of(g),
                                                                    stage);
@@ -1362,8 +955,12 @@ template <typename T>
 hipError_t launch_advect_stage(T* out, const T* fs, const T* fn, const T* vx, const T* vy, const T* vz, double dt,
                                const Geom& g, int stage, hipStream_t s)
 {
-    k_advect_stage<T><<<dim3(bulk_blocks(g)), dim3(256), 0, s>>>(out, fs, fn, vx, vy, vz, dt, box_of(g), inv_of(g),
-                                                                  stage);
+    if (g.tr == 32)
+        k_advect_stage<32, T><<<dim3(bulk_blocks(g)), dim3(256), 0, s>>>(out, fs, fn, vx, vy, vz, dt, box_of(g),
+                                                                          inv_of(g), stage);
+    else
+        k_advect_stage<64, T><<<dim3(bulk_blocks(g)), dim3(256), 0, s>>>(out, fs, fn, vx, vy, vz, dt, box_of(g),
+                                                                          inv_of(g), stage);
     return hipGetLastError();
 }
 

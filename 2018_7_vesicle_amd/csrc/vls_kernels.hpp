@@ -50,6 +50,7 @@ struct ListView {
 struct Geom {
     int rows, cols, pges;
     double dx, dy, dz;
+    int tr = 64;  // rows per tile of the bulk stage kernels: 64 or 32
     int64_t n() const { return (int64_t)rows * cols * pges; }
 };
 
@@ -128,5 +129,87 @@ hipError_t launch_max_abs3(double* result, double* partial, const T* a, const T*
                            hipStream_t s);
 template <typename T>
 hipError_t launch_copy(T* dst, const T* src, int64_t n, hipStream_t s);
+
+// ---- batches of grids (vls_batch.hip) -----------------------------------------
+// One grid of a batch: its box, its first element in the packed fields (every
+// field of a batch is the grids' column-major arrays back to back) and its
+// first tile in the batch's tile list (tiles of tr rows x 256/tr cols x 1
+// page, page-major).
+struct BGrid {
+    int rows, cols, pges;
+    int ntiles;       // tiles of this grid
+    int tile0;        // first tile of this grid
+    int64_t off;      // first element of this grid
+    double dx, dy, dz;
+    double inv[3];    // 1/dx, 1/dy, 1/dz
+    double dt_bulk;   // 0.3*min(dx, dy, dz)
+};
+
+// device view of a batch
+struct BatchView {
+    const BGrid* grids;    // ngrids descriptors
+    const int* tile_grid;  // grid of each tile
+    const int64_t* offs;   // ngrids + 1 grid offsets (offs[ngrids] = total points)
+    int ngrids, ntiles;
+    int blocks;            // persistent blocks of a tile-loop launch (multiple of 8)
+    int tr;                // rows per tile: 64 or 32
+};
+
+// tiles of one grid with `tr` rows per tile (tr x 256/tr columns)
+__host__ __device__ inline int grid_tiles(int rows, int cols, int pges, int tr)
+{
+    const int tc = 256 / tr;
+    return ((rows + tr - 1) / tr) * ((cols + tc - 1) / tc) * pges;
+}
+
+// list building: subcell distances (sixth-order locator), flags (mask bit
+// from mask_src < 0 when mask_src is not null) and the interface list of
+// every grid, in one pass; *lv.count must be zero on entry
+template <typename T>
+hipError_t launch_batch_build_list(const BatchView& B, const T* F, const T* mask_src, uint8_t* flags,
+                                   const ListView& lv, hipStream_t s);
+template <typename T>
+hipError_t launch_batch_set_mask_bit(uint8_t* flags, const T* F, int64_t n, hipStream_t s);
+template <typename T>
+hipError_t launch_batch_reinit_stage(const BatchView& B, T* out, const T* fs, const T* fn, const uint8_t* flags,
+                                     const ListView& lv, int stage, int* over, int* over_count, int seq,
+                                     hipStream_t s);
+template <typename T>
+hipError_t launch_batch_extend_stage(const BatchView& B, T* out, const T* cs, const T* cn, const T* vx,
+                                     const T* vy, const T* vz, const uint8_t* flags, const ListView& lv,
+                                     const double* lcp, const double* lvel, int stage, hipStream_t s);
+// dt: one step per grid (device)
+template <typename T>
+hipError_t launch_batch_advect_stage(const BatchView& B, T* out, const T* fs, const T* fn, const T* vx,
+                                     const T* vy, const T* vz, const double* dt, int stage, hipStream_t s);
+// upwind normal of F (subcell-aware) and v = sign(F) n/max(|n|, floor)
+template <typename T>
+hipError_t launch_batch_extension_velocity(const BatchView& B, T* vx, T* vy, T* vz, const T* F,
+                                           const uint8_t* flags, const ListView& lv, double floor_, hipStream_t s);
+template <typename T>
+hipError_t launch_batch_list_interp(const BatchView& B, double* lcp, const ListView& lv, const T* C, hipStream_t s);
+template <typename T>
+hipError_t launch_batch_list_gather_vel(double* lvel, const ListView& lv, const T* vx, const T* vy, const T* vz,
+                                        hipStream_t s);
+template <typename T>
+hipError_t launch_batch_normal_velocity(const BatchView& B, T* out, const T* F, const T* vx, const T* vy,
+                                        const T* vz, hipStream_t s);
+// per-grid cosine area and volume (eps = eps_factor*dx of each grid):
+// result[2g], result[2g+1]; partial holds 4*ntiles doubles
+template <typename T>
+hipError_t launch_batch_area_volume_cosine(const BatchView& B, double* result, double* partial, const T* F,
+                                           double eps_factor, hipStream_t s);
+// per-grid step dt[g] = cfl*dx/max|u| (partial: ntiles doubles)
+template <typename T>
+hipError_t launch_batch_cfl_dt(const BatchView& B, double* dt, double* partial, const T* vx, const T* vy,
+                               const T* vz, double cfl, hipStream_t s);
+// synthetic inputs: u = curl of sum_m a_m cos(k_m.x + psi_m) per grid (modes:
+// ngrids x nmodes x 7 doubles k(3), k x a (3), psi), and the ellipsoid
+// sqrt(x^2/a^2+y^2/b^2+z^2/c^2)-1 (axes: 3 per grid); lo: 3 per grid
+template <typename T>
+hipError_t launch_batch_curl_velocity(const BatchView& B, T* vx, T* vy, T* vz, const double* modes, int nmodes,
+                                      const double* lo, hipStream_t s);
+template <typename T>
+hipError_t launch_batch_ellipsoid(const BatchView& B, T* F, const double* axes, const double* lo, hipStream_t s);
 
 }  // namespace vls

@@ -1,0 +1,553 @@
+// Device pieces shared by the single-grid kernels (vls_kernels.hip) and the
+// batched kernels (vls_batch.hip): launch geometry, RK combinations, interface
+// lists, plane tiles, axis-lane grouping and compensated block reductions.
+#pragma once
+#include "vls_device.hpp"
+#include "vls_kernels.hpp"
+
+namespace vls {
+
+// ---------------------------------------------------------------------------
+// launch geometry helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool point_of_thread(const Box& b, int& r, int& c, int& p)
+{
+    r = blockIdx.x * blockDim.x + threadIdx.x;
+    c = blockIdx.y * blockDim.y + threadIdx.y;
+    p = blockIdx.z;
+    return r < b.rows && c < b.cols;
+}
+
+__device__ __forceinline__ idx_t lin(const Box& b, int r, int c, int p)
+{
+    return (idx_t)p * b.rows * b.cols + (idx_t)c * b.rows + r;
+}
+
+// 1/dx, 1/dy, 1/dz and the pseudo-time step of a point without subcell
+// distances, 0.3*min(dx,dx,dy,dy,dz,dz) (setDistance.m:17-21 with every
+// distance equal to the spacing), computed once on the host
+struct Inv {
+    double i[3];
+    double dt_bulk;
+};
+
+// RK combinations in the order the MATLAB wrappers evaluate them
+// (+SD/@SDF3/WENO5RK3Reinitialization.m:20-37), without fused multiply-adds.
+__device__ __forceinline__ double rk_combine(int stage, double fs, double step, double fn)
+{
+#pragma clang fp contract(off)
+    const double t = fs - step;
+    if (stage == 0) return t;
+    if (stage == 1) return 3. / 4. * fn + 1. / 4. * t;
+    return 1. / 3. * fn + 2. / 3. * t;
+}
+
+__device__ __forceinline__ float rk_combine(int stage, float fs, float step, float fn)
+{
+    const float t = fs - step;
+    if (stage == 0) return t;
+    if (stage == 1) return fmaf(0.75f, fn, 0.25f * t);
+    return fmaf(1.f / 3.f, fn, (2.f / 3.f) * t);
+}
+
+// the interface test of re_step.cu:260-267 on a pair of nodal values
+__device__ __forceinline__ bool sign_change(double a, double b) { return a * b < 0; }
+__device__ __forceinline__ bool sign_change(float a, float b) { return opposite(a, b); }
+
+// append `take` lanes' values to list[*count ...] with one atomic per wave
+__device__ __forceinline__ void wave_append(bool take, int value, int* __restrict__ list, int* __restrict__ count)
+{
+    const unsigned long long m = __ballot(take);
+    if (m == 0) return;
+    const int lane = (int)__lane_id();
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(count, __popcll(m));
+    base = __shfl(base, leader, 64);
+    if (take) list[base + __popcll(m & ((1ull << lane) - 1ull))] = value;
+}
+
+// as wave_append, returning each taking lane's slot (-1 for the others)
+__device__ __forceinline__ int wave_append_slot(bool take, int* __restrict__ count)
+{
+    const unsigned long long m = __ballot(take);
+    if (m == 0) return -1;
+    const int lane = (int)__lane_id();
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(count, __popcll(m));
+    base = __shfl(base, leader, 64);
+    return take ? base + __popcll(m & ((1ull << lane) - 1ull)) : -1;
+}
+
+struct Dists {
+    double r[3], l[3];
+};
+
+// an interface list in the form the stage kernels read: indices and, in list
+// order, the six distances (row k of `dist` is xpr,xpl,ypf,ypb,zpu,zpd for
+// k = 0..5, rows `cap` apart) and the pseudo-time step 0.3*min_dist, or -1 for
+// a frozen point
+struct IList {
+    const int* idx;
+    const int* count;
+    const double* dist;
+    const double* dt;
+    int cap;
+};
+struct IListW {
+    const int* idx;
+    const int* count;
+    double* dist;
+    double* dt;
+    int cap;
+};
+
+__device__ __forceinline__ bool frozen(const Dists& q, double dx)
+{
+    const double eps = 1e-6 * dx;
+    return q.r[0] < eps || q.l[0] < eps || q.r[1] < eps || q.l[1] < eps || q.r[2] < eps || q.l[2] < eps;
+}
+
+__device__ __forceinline__ double min6(const Dists& q)
+{
+    return dmin(dmin(q.r[0], q.l[0]), dmin(dmin(q.r[1], q.l[1]), dmin(q.r[2], q.l[2])));
+}
+
+// per-axis derivative pair with the sign-change interface test (re_step.cu:248-269)
+__device__ __forceinline__ void reinit_axis(double& fore, double& back, const double v[7], double r1, double l1,
+                                            double ds, double inv_ds)
+{
+    if (v[2] * v[3] < 0 || v[3] * v[4] < 0) {
+        const Pair q = weno_nonuniform_pair(v, r1, l1, ds, 0.0, 0.0);
+        fore = q.fore;
+        back = q.back;
+    } else {
+        weno5_pair(fore, back, v, inv_ds);
+    }
+}
+
+__device__ __forceinline__ void coords_of(const Box& b, idx_t ind, int& r, int& c, int& p)
+{
+    const idx_t plane = (idx_t)b.rows * b.cols;
+    p = (int)(ind / plane);
+    const int rem = (int)(ind - (idx_t)p * plane);
+    c = rem / b.rows;
+    r = rem - c * b.rows;
+}
+
+__device__ __forceinline__ void load_dists(Dists& q, const Dist6c& d, idx_t ind)
+{
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { q.r[a] = d.r[a][ind]; q.l[a] = d.l[a][ind]; }
+}
+
+// ---------------------------------------------------------------------------
+// plane tiles: a block covers 64 x 4 (rows x cols) points of one page.  The
+// page's values with a 3-point halo in rows and cols are staged in LDS once
+// (clamped there), so the x- and y-taps of the WENO stencil are LDS reads at
+// constant offsets; the z-taps are global loads at a wave-uniform page offset.
+// ---------------------------------------------------------------------------
+constexpr int HALO = 3;
+
+// tile shapes: R rows x 256/R cols of one page (R = 64: a wave is one column
+// of 64 rows; R = 32: a wave is two columns of 32 rows, for boxes whose
+// rows are a multiple of 32 but not of 64), staged with a 3-point halo
+template <int R>
+struct Shape {
+    static constexpr int TR = R, TC = 256 / R, LR = R + 2 * HALO, LC = TC + 2 * HALO, N = LR * LC;
+};
+
+struct Tile {
+    int r0, c0, p;  // tile origin (row, col) and page
+    int r, c;       // this thread's point
+    bool valid;
+    int base;       // LDS index of this thread's point
+};
+
+// Blocks are dispatched round-robin over the 8 XCDs (block i to XCD i % 8),
+// each with its own L2.  Give every XCD a contiguous range of tiles instead,
+// so neighbouring tiles share their halos in one L2.
+__device__ __forceinline__ int xcd_remap(int blk, int nblk)
+{
+    if ((nblk & 7) != 0) return blk;
+    return (blk & 7) * (nblk >> 3) + (blk >> 3);
+}
+
+// tiles per column and per row of a page, and in a box
+template <int R>
+__host__ __device__ inline int tiles_per_col(int rows) { return (rows + R - 1) / R; }
+template <int R>
+__host__ __device__ inline int tiles_per_row(int cols) { return (cols + Shape<R>::TC - 1) / Shape<R>::TC; }
+template <int R>
+__host__ __device__ inline int box_tiles(int rows, int cols, int pges)
+{
+    return tiles_per_col<R>(rows) * tiles_per_row<R>(cols) * pges;
+}
+
+// tile number `blk` of the box (rows fastest, then cols, then pages)
+template <int R>
+__device__ __forceinline__ Tile tile_at(const Box& b, int blk)
+{
+    using S = Shape<R>;
+    const int ntr = tiles_per_col<R>(b.rows), ntc = tiles_per_row<R>(b.cols);
+    Tile t;
+    const int rb = blk % ntr;
+    const int q = blk / ntr;
+    const int cb = q % ntc;
+    t.p = q / ntc;
+    t.r0 = rb * S::TR;
+    t.c0 = cb * S::TC;
+    const int ty = (int)(threadIdx.x % S::TR), tx = (int)(threadIdx.x / S::TR);
+    t.r = t.r0 + ty;
+    t.c = t.c0 + tx;
+    t.valid = t.r < b.rows && t.c < b.cols;
+    t.base = (tx + HALO) * S::LR + ty + HALO;
+    return t;
+}
+
+// this block's tile of a one-block-per-tile launch
+template <int R>
+__device__ __forceinline__ Tile tile_of(const Box& b)
+{
+    return tile_at<R>(b, xcd_remap((int)blockIdx.x, (int)gridDim.x));
+}
+
+// stage the tile of page t.p of f (with clamped halo) in lds[Shape<R>::N],
+// column-major (LR rows per column); ends with a barrier
+template <int R, typename T>
+__device__ __forceinline__ void fill_tile(T* lds, const T* __restrict__ f, const Box& b, const Tile& t)
+{
+    using S = Shape<R>;
+    const T* fp = f + (idx_t)t.p * b.rows * b.cols;
+    for (int e = (int)threadIdx.x; e < S::N; e += 256) {
+        const int lc = e / S::LR, lr = e - lc * S::LR;
+        lds[e] = fp[(idx_t)clampi(t.c0 + lc - HALO, b.cols - 1) * b.rows + clampi(t.r0 + lr - HALO, b.rows - 1)];
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__device__ __forceinline__ void tile_line_y(const T* lds, int base, T v[7])
+{
+#pragma unroll
+    for (int k = 0; k < 7; ++k) v[k] = lds[base + k - 3];
+}
+
+template <int R, typename T>
+__device__ __forceinline__ void tile_line_x(const T* lds, int base, T v[7])
+{
+#pragma unroll
+    for (int k = 0; k < 7; ++k) v[k] = lds[base + (k - 3) * Shape<R>::LR];
+}
+
+// z-taps: page offsets are the same for the whole block
+template <typename T>
+__device__ __forceinline__ void tile_line_z(const T* __restrict__ f, const Box& b, const Tile& t, T v[7])
+{
+    const idx_t plane = (idx_t)b.rows * b.cols;
+    const idx_t inplane = (idx_t)t.c * b.rows + t.r;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) v[k] = f[(idx_t)clampi(t.p + k - 3, b.pges - 1) * plane + inplane];
+}
+
+// Interface work is spread over three lanes per point, one per axis (21
+// points per wave64, lane 63 idle): the per-axis chains are long (the
+// nonuniform WENO divides ~30 times) and the lists short, so this triples the
+// parallelism.  The axis results meet in the group's first lane, which adds
+// them in the reference's x, y, z order.
+struct AxisLane {
+    int point;  // index into the point list, or -1
+    int axis;   // 0 x, 1 y, 2 z
+    int base;   // lane of the group's x member
+};
+
+__device__ __forceinline__ AxisLane axis_lane(int j0)
+{
+    const int lane = (int)(threadIdx.x & 63);
+    const int wave = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    AxisLane a;
+    a.axis = lane % 3;
+    a.base = lane - a.axis;
+    a.point = (lane < 63) ? j0 + wave * 21 + lane / 3 : -1;
+    return a;
+}
+
+// stride, in points, between the groups a grid visits per sweep
+__device__ __forceinline__ int axis_lane_stride() { return (int)(gridDim.x * (blockDim.x >> 6)) * 21; }
+
+// per-axis pair with the distance interface test (shared_utilities.cup:256-278)
+__device__ __forceinline__ void extend_axis(double& fore, double& back, const double v[7], double r1, double l1,
+                                            double ds, double inv_ds, double h_fore, double h_back)
+{
+    if (r1 < ds || l1 < ds) {
+        const Pair q = weno_nonuniform_pair(v, r1, l1, ds, h_fore, h_back);
+        fore = q.fore;
+        back = q.back;
+    } else {
+        weno5_pair(fore, back, v, inv_ds);
+    }
+}
+
+// Neumaier accumulator
+struct KSum {
+    double s, c;
+    __device__ __forceinline__ void add(double x)
+    {
+        const double t = s + x;
+        c += (fabs(s) >= fabs(x)) ? ((s - t) + x) : ((x - t) + s);
+        s = t;
+    }
+    __device__ __forceinline__ void merge(const KSum& o)
+    {
+        add(o.s);
+        c += o.c;
+    }
+};
+
+__device__ __forceinline__ double shfl_xor_d(double v, int m)
+{
+    return __shfl_xor(v, m, 64);
+}
+
+__device__ __forceinline__ void wave_reduce(KSum& k)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        KSum o;
+        o.s = shfl_xor_d(k.s, m);
+        o.c = shfl_xor_d(k.c, m);
+        k.merge(o);
+    }
+}
+
+// block of 256 threads -> thread 0 holds the merged (s,c) of each quantity
+template <int NQ>
+__device__ __forceinline__ void block_reduce(KSum (&k)[NQ])
+{
+    __shared__ double lds[NQ][2][4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        wave_reduce(k[q]);
+        if (lane == 0) { lds[q][0][wave] = k[q].s; lds[q][1][wave] = k[q].c; }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            KSum t{0.0, 0.0};
+            for (int w = 0; w < 4; ++w) { KSum o{lds[q][0][w], lds[q][1][w]}; t.merge(o); }
+            k[q] = t;
+        }
+    }
+}
+
+template <int NQ>
+__device__ __forceinline__ void block_reduce_store(KSum (&k)[NQ], double* __restrict__ partial)
+{
+    block_reduce<NQ>(k);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            partial[(size_t)blockIdx.x * 2 * NQ + 2 * q] = k[q].s;
+            partial[(size_t)blockIdx.x * 2 * NQ + 2 * q + 1] = k[q].c;
+        }
+}
+
+
+// ---------------------------------------------------------------------------
+// per-point bodies of the fused stage kernels.  The single-grid kernels and
+// the batched kernels call the same functions, with the field pointers of one
+// grid (offset into the packed fields for a batch), so a grid gives the same
+// result bit for bit either way.
+// ---------------------------------------------------------------------------
+
+// WENO5-RK3 reinitialization stage, bulk point of a staged tile: writes out
+// and returns false, or returns true (defer to the interface pass) for a point
+// next to a sign change of the stage field.  Points with subcell distances and
+// points outside the box write nothing and return false.
+template <int R, typename T>
+__device__ __forceinline__ bool reinit_bulk_point(T* __restrict__ out, const T* lds, const T* __restrict__ fs,
+                                                  const T* __restrict__ fn, const uint8_t* __restrict__ flags,
+                                                  const Box& b, const Tile& t, idx_t ind, const Inv& invs, int stage)
+{
+    const uint8_t fl = t.valid ? flags[ind] : (uint8_t)FLAG_SUBCELL;
+    bool defer = false;
+    if (!(fl & FLAG_SUBCELL)) {
+        T fo[3], bk[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            T v[7];
+            if (a == 0) tile_line_x<R>(lds, t.base, v);
+            else if (a == 1) tile_line_y(lds, t.base, v);
+            else tile_line_z(fs, b, t, v);
+            defer = defer || (sign_change(v[2], v[3]) || sign_change(v[3], v[4]));
+            weno5_pair(fo[a], bk[a], v, (T)invs.i[a]);
+        }
+        if (!defer) {
+            const T step = godunov_step((fl & FLAG_MASK) != 0, fo[0], bk[0], fo[1], bk[1], fo[2], bk[2],
+                                        (T)invs.dt_bulk);
+            out[ind] = rk_combine(stage, lds[t.base], step, (stage == 0) ? (T)0 : fn[ind]);
+        }
+    }
+    return defer;
+}
+
+// one lane (point, axis) of the reinitialization interface pass
+struct ReinitLane {
+    double m = 0.0;   // this axis's Godunov term
+    double dt = 0.0;  // pseudo-time step; < 0: frozen point
+    idx_t gind = 0;   // where the point's result goes
+    bool inside = false;
+};
+
+// slot: the point's position in the interface list, or -1 for a deferred
+// bulk point (spacing distances, bulk time step); ind: its index in the grid
+// (fs, flags of that grid); gind: its index in the output fields
+template <typename T>
+__device__ __forceinline__ ReinitLane reinit_iface_lane(const T* __restrict__ fs, const uint8_t* __restrict__ flags,
+                                                        const Box& b, const Inv& inv, const IList& il, int slot,
+                                                        idx_t ind, idx_t gind, int a)
+{
+    const double sp[3] = {b.dx, b.dy, b.dz};
+    ReinitLane q;
+    q.gind = gind;
+    q.dt = inv.dt_bulk;
+    int r, c, p;
+    coords_of(b, ind, r, c, p);
+    q.inside = (flags[ind] & FLAG_MASK) != 0;
+    double r1 = sp[a], l1 = sp[a];
+    if (slot >= 0) {
+        q.dt = il.dt[slot];
+        r1 = il.dist[(size_t)(2 * a) * il.cap + slot];
+        l1 = il.dist[(size_t)(2 * a + 1) * il.cap + slot];
+    }
+    if (q.dt >= 0) {
+        double v[7], fo, bk;
+        line7(fs, r, c, p, a, b, v);
+        reinit_axis(fo, bk, v, r1, l1, sp[a], inv.i[a]);
+        q.m = q.inside ? dmax(sq(dmin(0.0, bk)), sq(dmax(0.0, fo))) : dmax(sq(dmax(0.0, bk)), sq(dmin(0.0, fo)));
+    }
+    return q;
+}
+
+// the group's x lane adds the three axis terms in x, y, z order and writes
+// the RK combination (every lane of the wave must call this)
+template <typename T>
+__device__ __forceinline__ void reinit_iface_finish(T* __restrict__ out, const T* __restrict__ fs,
+                                                    const T* __restrict__ fn, const ReinitLane& q,
+                                                    const AxisLane& al, bool active, int stage)
+{
+    const double my = __shfl(q.m, al.base + 1, 64), mz = __shfl(q.m, al.base + 2, 64);
+    if (active && al.axis == 0) {
+        double step = 0.0;  // frozen point (dt < 0)
+        if (q.dt >= 0) {
+            // godunov_step: (sqrt((mx + my) + mz) - 1) * dt, negated inside
+            const double s1 = (sqrt((q.m + my) + mz) - 1) * q.dt;
+            step = q.inside ? -s1 : s1;
+        }
+        out[q.gind] = (T)rk_combine(stage, (double)fs[q.gind], step, (stage == 0) ? 0.0 : (double)fn[q.gind]);
+    }
+}
+
+// WENO5-RK3 extension stage, bulk point of a staged tile (points with subcell
+// distances are left to the interface pass)
+template <int R, typename T>
+__device__ __forceinline__ void extend_bulk_point(T* __restrict__ out, const T* lds, const T* __restrict__ cs,
+                                                  const T* __restrict__ cn, const T* __restrict__ vx,
+                                                  const T* __restrict__ vy, const T* __restrict__ vz,
+                                                  const uint8_t* __restrict__ flags, const Box& b, const Tile& t,
+                                                  const Inv& inv, int stage)
+{
+    if (!t.valid) return;
+    const idx_t ind = lin(b, t.r, t.c, t.p);
+    if (flags[ind] & FLAG_SUBCELL) return;
+    const T vel[3] = {vx[ind], vy[ind], vz[ind]};
+    T acc = 0;
+    T v[7];
+    tile_line_x<R>(lds, t.base, v);
+    acc = upwind_axis(acc, vel[0], v, (T)inv.i[0]);
+    tile_line_y(lds, t.base, v);
+    acc = upwind_axis(acc, vel[1], v, (T)inv.i[1]);
+    tile_line_z(cs, b, t, v);
+    acc = upwind_axis(acc, vel[2], v, (T)inv.i[2]);
+    out[ind] = rk_combine(stage, lds[t.base], acc * (T)inv.dt_bulk, (stage == 0) ? (T)0 : cn[ind]);
+}
+
+// one lane (point, axis) of the extension interface pass
+struct ExtendLane {
+    double t = 0.0;    // this axis's term coef * derivative
+    double dt = -1.0;  // pseudo-time step; < 0: frozen point
+    idx_t gind = 0;
+};
+
+// slot: position in the list (lcp, lv, il.dist, il.dt rows); ind: index in
+// the grid of cs; gind: index in the output fields
+template <typename T>
+__device__ __forceinline__ ExtendLane extend_iface_lane(const T* __restrict__ cs, const Box& b, const Inv& inv,
+                                                        const IList& il, const double* __restrict__ lcp,
+                                                        const double* __restrict__ lv, int slot, idx_t ind,
+                                                        idx_t gind, int a)
+{
+    const double sp[3] = {b.dx, b.dy, b.dz};
+    ExtendLane q;
+    q.gind = gind;
+    q.dt = il.dt[slot];
+    if (q.dt >= 0) {
+        int r, c, p;
+        coords_of(b, ind, r, c, p);
+        double v[7];
+        line7(cs, r, c, p, a, b, v);
+        const double vel = lv[(size_t)a * il.cap + slot];
+        const double r1 = il.dist[(size_t)(2 * a) * il.cap + slot];
+        const double l1 = il.dist[(size_t)(2 * a + 1) * il.cap + slot];
+        if (r1 < sp[a] || l1 < sp[a]) {
+            const bool fore = vel < 0;
+            const double dv = weno_nonuniform_one(v[0], v[1], v[2], v[3], v[4], v[5], v[6], r1, l1, sp[a],
+                                                  lcp[(size_t)(2 * a) * il.cap + slot],
+                                                  lcp[(size_t)(2 * a + 1) * il.cap + slot], !fore);
+            q.t = 0.0 + (fore ? vel : dmax(0.0, vel)) * dv;
+        } else {
+            q.t = upwind_axis(0.0, vel, v, inv.i[a]);
+        }
+    }
+    return q;
+}
+
+template <typename T>
+__device__ __forceinline__ void extend_iface_finish(T* __restrict__ out, const T* __restrict__ cs,
+                                                    const T* __restrict__ cn, const ExtendLane& q,
+                                                    const AxisLane& al, bool active, int stage)
+{
+    const double ty = __shfl(q.t, al.base + 1, 64), tz = __shfl(q.t, al.base + 2, 64);
+    if (active && al.axis == 0) {
+        const double step = (q.dt >= 0) ? ((q.t + ty) + tz) * q.dt : 0.0;
+        out[q.gind] = (T)rk_combine(stage, (double)cs[q.gind], step, (stage == 0) ? 0.0 : (double)cn[q.gind]);
+    }
+}
+
+// TVD-RK3 advection stage, point of a staged tile
+template <int R, typename T>
+__device__ __forceinline__ void advect_point(T* __restrict__ out, const T* lds, const T* __restrict__ fs,
+                                             const T* __restrict__ fn, const T* __restrict__ vx,
+                                             const T* __restrict__ vy, const T* __restrict__ vz, double dt,
+                                             const Box& b, const Tile& t, const Inv& inv, int stage)
+{
+    if (!t.valid) return;
+    const idx_t ind = lin(b, t.r, t.c, t.p);
+    const T vel[3] = {vx[ind], vy[ind], vz[ind]};
+    T acc = 0;
+    T v[7];
+    tile_line_x<R>(lds, t.base, v);
+    acc = upwind_axis(acc, vel[0], v, (T)inv.i[0]);
+    tile_line_y(lds, t.base, v);
+    acc = upwind_axis(acc, vel[1], v, (T)inv.i[1]);
+    tile_line_z(fs, b, t, v);
+    acc = upwind_axis(acc, vel[2], v, (T)inv.i[2]);
+    const T fnv = (stage == 0) ? (T)0 : fn[ind];
+    const T step = (T)dt * acc;
+    out[ind] = rk_combine(stage, lds[t.base], step, fnv);
+}
+
+}  // namespace vls

@@ -130,6 +130,7 @@ class Pipeline:
         elif cfg.precision != "fp64":
             raise ValueError(f"precision must be fp64 or fp32, got {cfg.precision}")
         self.Vn = self.grid.zeros(self.map.dtype)
+        self._av = torch.zeros(2, dtype=torch.float64, device=dev)  # the step's area/volume output
         self.area_volume = None
         self.steps_done = 0
         self.ext_events = []  # (start, end) event pairs around the extension, when recording
@@ -148,22 +149,23 @@ class Pipeline:
         done = 0
         for s in range(steps):
             i = self.steps_done
-            m.F = m.WENO5RK3Advection(m.F, *self.u, self.dt, 1)
-            self.psi = m.WENO5RK3Advection(self.psi, *self.u, self.dt, 1)
+            m.advect_(m.F, *self.u, self.dt, 1)
+            m.advect_(self.psi, *self.u, self.dt, 1)
             m.setDistance()
             self._normal_velocity(self.Vn)
             if record_extension:
                 e0 = torch.cuda.Event(enable_timing=True)
                 e1 = torch.cuda.Event(enable_timing=True)
                 e0.record()
-            self.Vn = m.WENO5RK3Extend(self.Vn, cfg.ext_iters)
+            m.extend_(self.Vn, cfg.ext_iters)
             if record_extension:
                 e1.record()
                 self.ext_events.append((e0, e1))
             if (i + 1) % cfg.reinit_every == 0:
-                m.F = m.WENO5RK3Reinitialization(m.F, cfg.reinit_iters)
-                self.psi = m.WENO5RK3ClosetPointSurfaceRedistance(self.psi, *cfg.redist_iters)
-            self._area_volume(av[s])
+                m.reinitialize_(m.F, cfg.reinit_iters)
+                m.redistance_(self.psi, *cfg.redist_iters)
+            self._area_volume(self._av)
+            av[s].copy_(self._av)
             done += updates_per_step(cfg, i) * n
             self.steps_done += 1
         self.area_volume = av

@@ -118,6 +118,19 @@ class SDF3:
         """the entry point of `name` for this object's field type"""
         return name + "_f32" if self.dtype == torch.float32 else name
 
+    # -- launch shape ------------------------------------------------------------
+    def setTileRows(self, tile_rows):
+        """tile shape of the bulk stage kernels: 64 (64 x 4, default) or 32 (32 x 8)"""
+        call("vls_sdf3_set_tile_rows", self._h, int(tile_rows))
+
+    def autotune(self, iterations=2):
+        """time extension stages off self.F's interface with each tile shape and keep
+        the fastest; leaves the distances of self.F set.  Returns the tile rows."""
+        tr = ctypes.c_int()
+        call(self._fn("vls_sdf3_autotune"), self._h, self._check(self.F), int(iterations), ctypes.byref(tr),
+             _stream())
+        return tr.value
+
     # -- setDistance (+SD/@SDF3/setDistance.m:3-21) --------------------------
     def setDistance(self):
         call(self._fn("vls_sdf3_set_distance"), self._h, self._check(self.F), _stream())
@@ -137,9 +150,29 @@ class SDF3:
     # -- reinitialization ------------------------------------------------------
     def WENO5RK3Reinitialization(self, F, iteration):
         """+SD/@SDF3/WENO5RK3Reinitialization.m:3-43: mask = F<0, distances of the last setDistance."""
-        NewF = F.clone()
-        call(self._fn("vls_sdf3_weno5_rk3_reinit"), self._h, self._check(NewF), int(iteration), _stream())
-        return NewF
+        return self.reinitialize_(F.clone(), iteration)
+
+    # In-place forms of the methods: the argument is updated and returned.  A
+    # call is captured into a graph keyed by its pointers, so a loop that
+    # passes the same tensors replays one graph.
+    def reinitialize_(self, F, iteration):
+        call(self._fn("vls_sdf3_weno5_rk3_reinit"), self._h, self._check(F), int(iteration), _stream())
+        return F
+
+    def extend_(self, C, iteration):
+        call(self._fn("vls_sdf3_weno5_rk3_extend"), self._h, self._check(self.F), self._check(C), int(iteration),
+             _stream())
+        return C
+
+    def redistance_(self, A, iter1, iter2):
+        call(self._fn("vls_sdf3_surface_redistance"), self._h, self._check(self.F), self._check(A), int(iter1),
+             int(iter2), _stream())
+        return A
+
+    def advect_(self, F, vx, vy, vz, dt, steps=1):
+        call(self._fn("vls_sdf3_advect_rk3"), self._h, self._check(F), self._check(vx), self._check(vy),
+             self._check(vz), float(dt), int(steps), _stream())
+        return F
 
     def ENORK2Reinitialization(self, F, iteration, rk2=True, cfl=0.3):
         """+SD/@SDF3/ENORK2Reinitialization.m:3-45 (rk2=True, cfl=0.3 is the reference;
@@ -154,26 +187,17 @@ class SDF3:
     # -- extension and surface redistance -------------------------------------
     def WENO5RK3Extend(self, C, iteration):
         """+SD/@SDF3/WENO5RK3Extend.m:9-76 (self.F and its last setDistance define the interface)."""
-        NewC = C.clone()
-        call(self._fn("vls_sdf3_weno5_rk3_extend"), self._h, self._check(self.F), self._check(NewC),
-             int(iteration), _stream())
-        return NewC
+        return self.extend_(C.clone(), iteration)
 
     def WENO5RK3ClosetPointSurfaceRedistance(self, A, iter1, iter2):
         """+SD/@SDF3/WENO5RK3ClosetPointSurfaceRedistance.m:4-157"""
-        NewA = A.clone()
-        call(self._fn("vls_sdf3_surface_redistance"), self._h, self._check(self.F), self._check(NewA),
-             int(iter1), int(iter2), _stream())
-        return NewA
+        return self.redistance_(A.clone(), iter1, iter2)
 
     # -- advection -------------------------------------------------------------
     def WENO5RK3Advection(self, F, vx, vy, vz, dt, steps=1):
         """TVD-RK3 (Shu-Osher form of +SD/@SDF3/WENORK3SurfaceConservationLaw.m:10-29) of
         F_t + v.grad F = 0 with the WENO5 operator of advection_step.cu:30-56."""
-        NewF = F.clone()
-        call(self._fn("vls_sdf3_advect_rk3"), self._h, self._check(NewF), self._check(vx), self._check(vy),
-             self._check(vz), float(dt), int(steps), _stream())
-        return NewF
+        return self.advect_(F.clone(), vx, vy, vz, dt, steps)
 
     def maxSpeed(self, vx, vy, vz):
         out = ctypes.c_double()

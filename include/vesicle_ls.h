@@ -117,6 +117,9 @@ typedef struct vls_sdf3 *vls_sdf3_t;
  * reference's launch shape needs rows <= 256, GPUInitialize.m:7). */
 int vls_sdf3_create(vls_sdf3_t *out, int rows, int cols, int pges, double dx, double dy, double dz);
 int vls_sdf3_destroy(vls_sdf3_t h);
+/* tile shape of the fused bulk stage kernels: 64 (64 x 4 points of a page,
+ * the default) or 32 (32 x 8); results do not depend on it */
+int vls_sdf3_set_tile_rows(vls_sdf3_t h, int tile_rows);
 
 /* setDistance (+SD/@SDF3/setDistance.m:3-21): subcell distances of F with the
  * sixth-order locator, and min_dist.  Stored in the handle. */
@@ -204,6 +207,106 @@ int vls_sdf3_area_volume_device_f32(vls_sdf3_t h, const float *F, int kind, doub
                                     vls_stream_t stream);
 int vls_sdf3_normal_velocity_f32(vls_sdf3_t h, const float *F, const float *vx, const float *vy, const float *vz,
                                  float *out, vls_stream_t stream);
+
+/* Graph capture of the wrapper-level loops (no counterpart in the reference,
+ * which launches every kernel from MATLAB): with it on (the default), a
+ * reinitialization, extension, surface redistance, advection or batch step
+ * call is captured once into a hipGraph, keyed by its device pointers and
+ * counts, and replayed after that (8 graphs cached per handle).  Returns the
+ * previous setting. */
+int vls_set_graphs(int enable);
+
+/* ------------------------------------------------------------------------ */
+/* 4. batches of grids                                                      */
+/* ------------------------------------------------------------------------ */
+/* Many independent level sets of different box sizes (one vesicle each),
+ * advanced together: every operator stage of every grid is one launch of a
+ * persistent kernel over the batch's tile list.  No counterpart in the
+ * reference, which holds one SDF3 object per grid; each grid of a batch gets
+ * what the vls_sdf3_* method of the same name gives it on its own (bit for
+ * bit, except the area/volume sums, whose summation order differs).
+ *
+ * Fields of a batch are packed: the grids' column-major arrays back to back,
+ * grid g at elements offsets[g] .. offsets[g+1]-1 (vls_batch_layout), at most
+ * 2^31 - 1 points in all.  Methods come in FP64 and FP32-bulk (_f32) forms,
+ * as the vls_sdf3_* ones.  Per-grid scalars are device arrays of doubles. */
+typedef struct vls_batch *vls_batch_t;
+
+int vls_batch_create(vls_batch_t *out, int ngrids, const int *rows, const int *cols, const int *pges,
+                     const double *dx, const double *dy, const double *dz);
+int vls_batch_destroy(vls_batch_t b);
+/* offsets: ngrids + 1 host int64; ntiles (may be null): 64x4x1 tiles in all */
+int vls_batch_layout(vls_batch_t b, int64_t *offsets, int *ntiles);
+/* persistent blocks of a tile-loop launch (a multiple of 8; 0 restores the
+ * default, 8 per CU); cached graphs are dropped */
+int vls_batch_set_blocks(vls_batch_t b, int blocks);
+int vls_batch_get_blocks(vls_batch_t b, int *blocks);
+/* tile shape of the bulk kernels: 64 (64 x 4 points of a page) or 32 (32 x 8);
+ * the default is the one with fewer tiles; cached graphs are dropped */
+int vls_batch_set_tile_rows(vls_batch_t b, int tile_rows);
+int vls_batch_get_tile_rows(vls_batch_t b, int *tile_rows);
+
+int vls_batch_set_distance(vls_batch_t b, const double *F, vls_stream_t stream);
+int vls_batch_weno5_rk3_reinit(vls_batch_t b, double *F, int iterations, vls_stream_t stream);
+int vls_batch_weno5_rk3_extend(vls_batch_t b, const double *F, double *C, int iterations, vls_stream_t stream);
+/* dt: one time step per grid */
+int vls_batch_advect_rk3(vls_batch_t b, double *F, const double *vx, const double *vy, const double *vz,
+                         const double *dt, int steps, vls_stream_t stream);
+int vls_batch_normal_velocity(vls_batch_t b, const double *F, const double *vx, const double *vy, const double *vz,
+                              double *out, vls_stream_t stream);
+/* cosine delta/H of half-width eps_factor*dx per grid: out2n[2g] = area, out2n[2g+1] = volume */
+int vls_batch_area_volume_cosine(vls_batch_t b, const double *F, double eps_factor, double *out2n,
+                                 vls_stream_t stream);
+/* dt[g] = cfl*dx_g/max|u| over grid g */
+int vls_batch_cfl_dt(vls_batch_t b, const double *vx, const double *vy, const double *vz, double cfl, double *dt,
+                     vls_stream_t stream);
+/* synthetic inputs: u = curl of sum_m a_m cos(k_m.x + psi_m) for each grid
+ * (modes: ngrids*nmodes*7 device doubles k(3), k x a (3), psi), and the
+ * ellipsoid field sqrt(x^2/a^2+y^2/b^2+z^2/c^2)-1 (axes: 3 per grid); lo: the
+ * (x, y, z) of each grid's first node, 3 per grid; all device arrays */
+int vls_batch_curl_velocity(vls_batch_t b, const double *modes, int nmodes, const double *lo, double *vx,
+                            double *vy, double *vz, vls_stream_t stream);
+int vls_batch_ellipsoid(vls_batch_t b, const double *axes, const double *lo, double *F, vls_stream_t stream);
+/* one vesicle step of every grid: RK3 advection of F by u with dt, set
+ * distance, Vn = u.N, extension of Vn (ext_iters), reinitialization of F when
+ * reinit_iters > 0, cosine area/volume into av (2 per grid) */
+int vls_batch_step(vls_batch_t b, double *F, double *Vn, const double *vx, const double *vy, const double *vz,
+                   const double *dt, int ext_iters, int reinit_iters, double eps_factor, double *av,
+                   vls_stream_t stream);
+
+int vls_batch_set_distance_f32(vls_batch_t b, const float *F, vls_stream_t stream);
+int vls_batch_weno5_rk3_reinit_f32(vls_batch_t b, float *F, int iterations, vls_stream_t stream);
+int vls_batch_weno5_rk3_extend_f32(vls_batch_t b, const float *F, float *C, int iterations, vls_stream_t stream);
+int vls_batch_advect_rk3_f32(vls_batch_t b, float *F, const float *vx, const float *vy, const float *vz,
+                             const double *dt, int steps, vls_stream_t stream);
+int vls_batch_normal_velocity_f32(vls_batch_t b, const float *F, const float *vx, const float *vy, const float *vz,
+                                  float *out, vls_stream_t stream);
+int vls_batch_area_volume_cosine_f32(vls_batch_t b, const float *F, double eps_factor, double *out2n,
+                                     vls_stream_t stream);
+int vls_batch_cfl_dt_f32(vls_batch_t b, const float *vx, const float *vy, const float *vz, double cfl, double *dt,
+                         vls_stream_t stream);
+int vls_batch_curl_velocity_f32(vls_batch_t b, const double *modes, int nmodes, const double *lo, float *vx,
+                                float *vy, float *vz, vls_stream_t stream);
+int vls_batch_ellipsoid_f32(vls_batch_t b, const double *axes, const double *lo, float *F, vls_stream_t stream);
+int vls_batch_step_f32(vls_batch_t b, float *F, float *Vn, const float *vx, const float *vy, const float *vz,
+                       const double *dt, int ext_iters, int reinit_iters, double eps_factor, double *av,
+                       vls_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* 5. autotuning of the launch shape (no counterpart in the reference)      */
+/* ------------------------------------------------------------------------ */
+/* Times `iterations` extension iterations of a scratch copy of F, off F's
+ * interface, for each candidate launch shape and keeps the fastest: tile rows
+ * 64 or 32 and, for a batch, 2/4/8/16 persistent blocks per CU.  Results of
+ * every method are the same for every shape.  Synchronizes.  The handle is
+ * left with F's distances (as after set_distance(F)); a batch's cached graphs
+ * are dropped.  tile_rows and blocks (may be null) receive the choice. */
+int vls_batch_autotune(vls_batch_t b, const double *F, int iterations, int *tile_rows, int *blocks,
+                       vls_stream_t stream);
+int vls_batch_autotune_f32(vls_batch_t b, const float *F, int iterations, int *tile_rows, int *blocks,
+                           vls_stream_t stream);
+int vls_sdf3_autotune(vls_sdf3_t h, const double *F, int iterations, int *tile_rows, vls_stream_t stream);
+int vls_sdf3_autotune_f32(vls_sdf3_t h, const float *F, int iterations, int *tile_rows, vls_stream_t stream);
 
 #ifdef __cplusplus
 }
