@@ -67,6 +67,10 @@ SIGNATURES = {
     "vls_batch_autotune_f32": [P, P, I, P, P, P],
     "vls_sdf3_autotune": [P, P, I, P, P],
     "vls_sdf3_autotune_f32": [P, P, I, P, P],
+    "vls_sdf3_adjoint_advect_rk3": [P, P, P, P, P, D, P, P, P, P, P, P],
+    "vls_sdf3_tangent_advect_rk3": [P, P, P, P, P, P, P, P, P, D, P, P, P],
+    "vls_sdf3_adjoint_reinit": [P, P, I, P, P, P],
+    "vls_sdf3_tangent_reinit": [P, P, P, I, P, P, P],
 }
 for _sfx in ("", "_f32"):
     SIGNATURES.update({

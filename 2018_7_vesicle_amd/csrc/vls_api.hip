@@ -204,6 +204,8 @@ struct vls_sdf3 {
     double* partial = nullptr;
     double* result = nullptr;
     GraphCache graphs;
+    double* adj = nullptr;  // adjoint/tangent workspace, grown on demand
+    size_t adj_len = 0;
 };
 
 namespace {
@@ -702,6 +704,7 @@ int vls_sdf3_set_tile_rows(vls_sdf3_t h, int tile_rows)
 int vls_sdf3_destroy(vls_sdf3_t h)
 {
     if (!h) return VLS_OK;
+    (void)hipFree(h->adj);
     hipError_t e1 = hipFree(h->pool), e2 = hipFree(h->flag_pool), e3 = hipFree(h->int_pool);
     delete h;
     if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) return fail(VLS_ERR_HIP, "hipFree failed");
@@ -1400,6 +1403,230 @@ int vls_sdf3_autotune_f32(vls_sdf3_t h, const float* F, int iterations, int* til
     VLS_REQUIRE(h && F, "null argument");
     VLS_REQUIRE(iterations > 0, "iterations must be positive");
     return sdf3_autotune<float>(h, F, iterations, S(stream), tile_rows);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// 6. discrete adjoint and tangent (FP64, one grid)
+// ---------------------------------------------------------------------------
+namespace {
+
+int ensure_adj(vls_sdf3* h, size_t len)
+{
+    if (h->adj_len >= len) return VLS_OK;
+    VLS_HIP(hipDeviceSynchronize());  // the old workspace may still be in use
+    (void)hipFree(h->adj);
+    h->adj = nullptr;
+    h->adj_len = 0;
+    if (hipMalloc((void**)&h->adj, sizeof(double) * len) != hipSuccess)
+        return fail(VLS_ERR_ALLOC, "hipMalloc of the adjoint workspace (%zu doubles) failed", len);
+    h->adj_len = len;
+    return VLS_OK;
+}
+
+int check_adj_box(const Geom& g)
+{
+    if (g.rows < 4 || g.cols < 4 || g.pges < 4) return fail(VLS_ERR_INVALID, "the adjoint needs at least 4 points per axis");
+    return VLS_OK;
+}
+
+// stage coefficients of fn: 0, 3/4, 1/3
+inline double stage_e(int stage) { return stage == 0 ? 0.0 : (stage == 1 ? 0.75 : 1.0 / 3.0); }
+
+}  // namespace
+
+extern "C" {
+
+int vls_sdf3_adjoint_advect_rk3(vls_sdf3_t h, const double* F, const double* vx, const double* vy, const double* vz,
+                                double dt, const double* lam_out, double* lam_in, double* gx, double* gy, double* gz,
+                                vls_stream_t stream)
+{
+    VLS_REQUIRE(h && F && vx && vy && vz && lam_out && lam_in && gx && gy && gz, "null argument");
+    int rc = check_adj_box(h->g);
+    if (rc) return rc;
+    const Geom& g = h->g;
+    const int64_t n = g.n();
+    if ((rc = ensure_adj(h, (size_t)n * (21 + 3 + 4)))) return rc;
+    double* J = h->adj;
+    double* JU = J + 21 * n;
+    double* t1 = JU + 3 * n;
+    double* t2 = t1 + n;
+    double* la = t2 + n;
+    double* lb = la + n;
+    const hipStream_t s = S(stream);
+    VLS_HIP(launch_advect_stage<double>(t1, F, nullptr, vx, vy, vz, dt, g, 0, s));
+    VLS_HIP(launch_advect_stage<double>(t2, t1, F, vx, vy, vz, dt, g, 1, s));
+    // stage 2: out = S2(t2; F)
+    VLS_HIP(launch_adj_advect_jac(J, JU, t2, vx, vy, vz, dt, g, 2, s));
+    VLS_HIP(launch_adj_transpose(la, J, lam_out, g, 0.0, s));
+    VLS_HIP(launch_adj_velocity_grad(gx, gy, gz, JU, lam_out, n, s));
+    VLS_HIP(launch_adj_axpby(lb, lam_out, stage_e(2), 0.0, n, s));
+    // stage 1: t2 = S1(t1; F)
+    VLS_HIP(launch_adj_advect_jac(J, JU, t1, vx, vy, vz, dt, g, 1, s));
+    VLS_HIP(launch_adj_transpose(t2, J, la, g, 0.0, s));  // t2 <- adjoint of t1
+    VLS_HIP(launch_adj_velocity_grad(gx, gy, gz, JU, la, n, s));
+    VLS_HIP(launch_adj_axpby(lb, la, stage_e(1), 1.0, n, s));
+    // stage 0: t1 = S0(F)
+    VLS_HIP(launch_adj_advect_jac(J, JU, F, vx, vy, vz, dt, g, 0, s));
+    VLS_HIP(launch_adj_transpose(lb, J, t2, g, 1.0, s));
+    VLS_HIP(launch_adj_velocity_grad(gx, gy, gz, JU, t2, n, s));
+    VLS_HIP(launch_copy<double>(lam_in, lb, n, s));
+    return VLS_OK;
+}
+
+int vls_sdf3_tangent_advect_rk3(vls_sdf3_t h, const double* F, const double* Fdot, const double* vx, const double* vy,
+                                const double* vz, const double* udx, const double* udy, const double* udz, double dt,
+                                double* Fout, double* Fdot_out, vls_stream_t stream)
+{
+    VLS_REQUIRE(h && F && Fdot && vx && vy && vz && udx && udy && udz && Fout && Fdot_out, "null argument");
+    VLS_REQUIRE(Fout != F && Fdot_out != Fdot, "outputs must not alias the inputs");
+    int rc = check_adj_box(h->g);
+    if (rc) return rc;
+    const Geom& g = h->g;
+    const int64_t n = g.n();
+    if ((rc = ensure_adj(h, (size_t)n * (21 + 3 + 4)))) return rc;
+    double* J = h->adj;
+    double* JU = J + 21 * n;
+    double* t1 = JU + 3 * n;
+    double* t2 = t1 + n;
+    double* d1 = t2 + n;
+    double* d2 = d1 + n;
+    const hipStream_t s = S(stream);
+    // stage 0
+    VLS_HIP(launch_adj_advect_jac(J, JU, F, vx, vy, vz, dt, g, 0, s));
+    VLS_HIP(launch_adj_apply(d1, J, Fdot, g, s));
+    VLS_HIP(launch_adj_velocity_tangent(d1, JU, udx, udy, udz, n, s));
+    VLS_HIP(launch_advect_stage<double>(t1, F, nullptr, vx, vy, vz, dt, g, 0, s));
+    // stage 1
+    VLS_HIP(launch_adj_advect_jac(J, JU, t1, vx, vy, vz, dt, g, 1, s));
+    VLS_HIP(launch_adj_apply(d2, J, d1, g, s));
+    VLS_HIP(launch_adj_axpby(d2, Fdot, stage_e(1), 1.0, n, s));
+    VLS_HIP(launch_adj_velocity_tangent(d2, JU, udx, udy, udz, n, s));
+    VLS_HIP(launch_advect_stage<double>(t2, t1, F, vx, vy, vz, dt, g, 1, s));
+    // stage 2
+    VLS_HIP(launch_adj_advect_jac(J, JU, t2, vx, vy, vz, dt, g, 2, s));
+    VLS_HIP(launch_adj_apply(Fdot_out, J, d2, g, s));
+    VLS_HIP(launch_adj_axpby(Fdot_out, Fdot, stage_e(2), 1.0, n, s));
+    VLS_HIP(launch_adj_velocity_tangent(Fdot_out, JU, udx, udy, udz, n, s));
+    VLS_HIP(launch_advect_stage<double>(Fout, t2, F, vx, vy, vz, dt, g, 2, s));
+    return VLS_OK;
+}
+
+// the reinitialization map R(F0): set_distance(F0), mask F0 < 0, `iterations`
+// WENO5-RK3 iterations.  Recomputes the forward iterations, keeping every
+// stage field, then sweeps back.  Leaves the handle with F0's distances.
+int vls_sdf3_adjoint_reinit(vls_sdf3_t h, const double* F0, int iterations, const double* lam_out, double* lam_in,
+                            vls_stream_t stream)
+{
+    VLS_REQUIRE(h && F0 && lam_out && lam_in, "null argument");
+    VLS_REQUIRE(iterations >= 0, "negative iteration count");
+    VLS_REQUIRE((const void*)F0 != (const void*)lam_in, "lam_in must not alias F0");
+    int rc = check_adj_box(h->g);
+    if (rc) return rc;
+    const Geom& g = h->g;
+    const int64_t n = g.n();
+    const int K = iterations;
+    if ((rc = ensure_adj(h, (size_t)n * (21 + 7 + 7 + 4 + 3 * (size_t)K + 1)))) return rc;
+    double* J = h->adj;
+    double* JP = J + 21 * n;
+    double* lamP = JP + 7 * n;
+    double* lam = lamP + 7 * n;
+    double* lt0 = lam + n;
+    double* lt1 = lt0 + n;
+    double* lx = lt1 + n;
+    double* X = lx + n;  // X[k] = X + k*n, k = 0..K; then T0[k], T1[k]
+    double* T0 = X + (size_t)(K + 1) * n;
+    double* T1 = T0 + (size_t)K * n;
+    const hipStream_t s = S(stream);
+    const DSet ms = main_set(h);
+    if ((rc = set_distance_into<double>(h, 6, ms, h->min_dist, F0, (const double*)nullptr, s))) return rc;
+    VLS_HIP(launch_set_mask_bit<double>(h->flags, F0, g, s));
+    VLS_HIP(launch_copy<double>(X, F0, n, s));
+    ReinitSeq rs;
+    if ((rc = reinit_begin(h, rs, s))) return rc;
+    for (int k = 0; k < K; ++k) {
+        double* xk = X + (size_t)k * n;
+        if ((rc = reinit_stage<double>(h, rs, T0 + (size_t)k * n, xk, nullptr, ms, 0, s))) return rc;
+        if ((rc = reinit_stage<double>(h, rs, T1 + (size_t)k * n, T0 + (size_t)k * n, xk, ms, 1, s))) return rc;
+        if ((rc = reinit_stage<double>(h, rs, xk + n, T1 + (size_t)k * n, xk, ms, 2, s))) return rc;
+    }
+    VLS_HIP(launch_copy<double>(lam, lam_out, n, s));
+    VLS_HIP(launch_adj_fill(lamP, 0.0, 7 * n, s));
+    for (int k = K - 1; k >= 0; --k) {
+        const double* xk = X + (size_t)k * n;
+        // stage 2: X[k+1] = S2(T1[k]; X[k])
+        VLS_HIP(launch_adj_reinit_jac(J, JP, T1 + (size_t)k * n, h->flags, ms.lv, g, 2, s));
+        VLS_HIP(launch_adj_transpose(lt1, J, lam, g, 0.0, s));
+        VLS_HIP(launch_adj_param_grad(lamP, JP, lam, ms.lv, s));
+        VLS_HIP(launch_adj_axpby(lx, lam, stage_e(2), 0.0, n, s));
+        // stage 1: T1[k] = S1(T0[k]; X[k])
+        VLS_HIP(launch_adj_reinit_jac(J, JP, T0 + (size_t)k * n, h->flags, ms.lv, g, 1, s));
+        VLS_HIP(launch_adj_transpose(lt0, J, lt1, g, 0.0, s));
+        VLS_HIP(launch_adj_param_grad(lamP, JP, lt1, ms.lv, s));
+        VLS_HIP(launch_adj_axpby(lx, lt1, stage_e(1), 1.0, n, s));
+        // stage 0: T0[k] = S0(X[k])
+        VLS_HIP(launch_adj_reinit_jac(J, JP, xk, h->flags, ms.lv, g, 0, s));
+        VLS_HIP(launch_adj_transpose(lx, J, lt0, g, 1.0, s));
+        VLS_HIP(launch_adj_param_grad(lamP, JP, lt0, ms.lv, s));
+        VLS_HIP(launch_copy<double>(lam, lx, n, s));
+    }
+    VLS_HIP(launch_copy<double>(lam_in, lam, n, s));
+    VLS_HIP(launch_adj_dist(lam_in, F0, lamP, ms.lv, g, s));
+    return VLS_OK;
+}
+
+int vls_sdf3_tangent_reinit(vls_sdf3_t h, const double* F0, const double* F0dot, int iterations, double* Fout,
+                            double* Fdot_out, vls_stream_t stream)
+{
+    VLS_REQUIRE(h && F0 && F0dot && Fout && Fdot_out, "null argument");
+    VLS_REQUIRE(iterations >= 0, "negative iteration count");
+    VLS_REQUIRE(Fout != F0 && Fdot_out != F0dot && (const void*)Fout != (const void*)Fdot_out,
+                "outputs must not alias the inputs");
+    int rc = check_adj_box(h->g);
+    if (rc) return rc;
+    const Geom& g = h->g;
+    const int64_t n = g.n();
+    if ((rc = ensure_adj(h, (size_t)n * (21 + 7 + 7 + 6)))) return rc;
+    double* J = h->adj;
+    double* JP = J + 21 * n;
+    double* Pd = JP + 7 * n;
+    double* t0 = Pd + 7 * n;
+    double* t1 = t0 + n;
+    double* d0 = t1 + n;
+    double* d1 = d0 + n;
+    double* X = d1 + n;
+    double* Xd = X + n;
+    const hipStream_t s = S(stream);
+    const DSet ms = main_set(h);
+    if ((rc = set_distance_into<double>(h, 6, ms, h->min_dist, F0, (const double*)nullptr, s))) return rc;
+    VLS_HIP(launch_set_mask_bit<double>(h->flags, F0, g, s));
+    VLS_HIP(launch_adj_fill(Pd, 0.0, 7 * n, s));
+    VLS_HIP(launch_tan_dist(Pd, F0, F0dot, ms.lv, g, s));
+    VLS_HIP(launch_copy<double>(X, F0, n, s));
+    VLS_HIP(launch_copy<double>(Xd, F0dot, n, s));
+    ReinitSeq rs;
+    if ((rc = reinit_begin(h, rs, s))) return rc;
+    for (int k = 0; k < iterations; ++k) {
+        VLS_HIP(launch_adj_reinit_jac(J, JP, X, h->flags, ms.lv, g, 0, s));
+        VLS_HIP(launch_adj_apply(d0, J, Xd, g, s));
+        VLS_HIP(launch_adj_param_tangent(d0, JP, Pd, ms.lv, s));
+        if ((rc = reinit_stage<double>(h, rs, t0, X, nullptr, ms, 0, s))) return rc;
+        VLS_HIP(launch_adj_reinit_jac(J, JP, t0, h->flags, ms.lv, g, 1, s));
+        VLS_HIP(launch_adj_apply(d1, J, d0, g, s));
+        VLS_HIP(launch_adj_axpby(d1, Xd, stage_e(1), 1.0, n, s));
+        VLS_HIP(launch_adj_param_tangent(d1, JP, Pd, ms.lv, s));
+        if ((rc = reinit_stage<double>(h, rs, t1, t0, X, ms, 1, s))) return rc;
+        VLS_HIP(launch_adj_reinit_jac(J, JP, t1, h->flags, ms.lv, g, 2, s));
+        VLS_HIP(launch_adj_apply(d0, J, d1, g, s));
+        VLS_HIP(launch_adj_axpby(d0, Xd, stage_e(2), 1.0, n, s));
+        VLS_HIP(launch_adj_param_tangent(d0, JP, Pd, ms.lv, s));
+        if ((rc = reinit_stage<double>(h, rs, X, t1, X, ms, 2, s))) return rc;
+        VLS_HIP(launch_copy<double>(Xd, d0, n, s));
+    }
+    VLS_HIP(launch_copy<double>(Fout, X, n, s));
+    VLS_HIP(launch_copy<double>(Fdot_out, Xd, n, s));
+    return VLS_OK;
 }
 
 }  // extern "C"

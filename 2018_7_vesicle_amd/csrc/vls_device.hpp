@@ -18,6 +18,9 @@ typedef int64_t idx_t;
 __device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }
 __device__ __forceinline__ double dmin(double a, double b) { return (a < b) ? a : b; }
 __device__ __forceinline__ double sq(double a) { return a * a; }
+// value part of a scalar (the formulas below are templates on the scalar
+// type: double, or a dual number of the adjoint; branch tests read values)
+__device__ __forceinline__ double val(double a) { return a; }
 // kernel sign(): +1 for x>0, -1 otherwise (1_3_WENO5RK3_Reinitialization/shared_utilities.hpp:17-18)
 __device__ __forceinline__ double ksign(double x) { return (x > 0) ? 1.0 : -1.0; }
 // MATLAB sign(): 0 at 0 (used by the extension velocity, +SD/@SDF3/WENO5RK3Extend.m:29-31)
@@ -66,22 +69,23 @@ __device__ __forceinline__ void line7(const T* __restrict__ f, int r, int c, int
 // multiply-adds below are explicit, so the result is one fixed IEEE sequence
 // (the CPU oracle states the same sequence).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double weno5_onesided(double v1, double v2, double v3, double v4, double v5)
+template <typename S>
+__device__ __forceinline__ S weno5_onesided(S v1, S v2, S v3, S v4, S v5)
 {
     const double c13 = 1. / 3., c76 = 7. / 6., c116 = 11. / 6., c16 = 1. / 6., c56 = 5. / 6., c1312 = 13. / 12.;
-    const double phi1 = fma(c116, v3, fma(-c76, v2, c13 * v1));
-    const double phi2 = fma(c13, v4, fma(c56, v3, -c16 * v2));
-    const double phi3 = fma(-c16, v5, fma(c56, v4, c13 * v3));
-    const double d1 = fma(-2.0, v2, v1) + v3, e1 = fma(3.0, v3, fma(-4.0, v2, v1));
-    const double d2 = fma(-2.0, v3, v2) + v4, e2 = v2 - v4;
-    const double d3 = fma(-2.0, v4, v3) + v5, e3 = fma(3.0, v3, fma(-4.0, v4, v5));
-    const double S1 = fma(c1312 * d1, d1, 0.25 * e1 * e1);
-    const double S2 = fma(c1312 * d2, d2, 0.25 * e2 * e2);
-    const double S3 = fma(c1312 * d3, d3, 0.25 * e3 * e3);
+    const S phi1 = fma(c116, v3, fma(-c76, v2, c13 * v1));
+    const S phi2 = fma(c13, v4, fma(c56, v3, -c16 * v2));
+    const S phi3 = fma(-c16, v5, fma(c56, v4, c13 * v3));
+    const S d1 = fma(-2.0, v2, v1) + v3, e1 = fma(3.0, v3, fma(-4.0, v2, v1));
+    const S d2 = fma(-2.0, v3, v2) + v4, e2 = v2 - v4;
+    const S d3 = fma(-2.0, v4, v3) + v5, e3 = fma(3.0, v3, fma(-4.0, v4, v5));
+    const S S1 = fma(c1312 * d1, d1, 0.25 * e1 * e1);
+    const S S2 = fma(c1312 * d2, d2, 0.25 * e2 * e2);
+    const S S3 = fma(c1312 * d3, d3, 0.25 * e3 * e3);
     const double eps = 1e-6;
-    const double s1 = S1 + eps, s2 = S2 + eps, s3 = S3 + eps;
-    const double a1 = s1 * s1, a2 = s2 * s2, a3 = s3 * s3;
-    const double w1 = 0.1 * (a2 * a3), w2 = 0.6 * (a1 * a3), w3 = 0.3 * (a1 * a2);
+    const S s1 = S1 + eps, s2 = S2 + eps, s3 = S3 + eps;
+    const S a1 = s1 * s1, a2 = s2 * s2, a3 = s3 * s3;
+    const S w1 = 0.1 * (a2 * a3), w2 = 0.6 * (a1 * a3), w3 = 0.3 * (a1 * a2);
     return fma(w1, phi1, fma(w2, phi2, w3 * phi3)) / (w1 + w2 + w3);
 }
 
@@ -92,21 +96,23 @@ __device__ __forceinline__ double weno5_onesided(double v1, double v2, double v3
 // zero times a finite value, is the +-0 the sum would have added unchanged.
 // acc is never -0 (it starts at +0 and x + -x rounds to +0), so the result
 // is bit-identical to evaluating both derivatives.
-__device__ __forceinline__ double upwind_axis(double acc, double vel, const double p[7], double inv_ds)
+template <typename S>
+__device__ __forceinline__ S upwind_axis(S acc, S vel, const S p[7], double inv_ds)
 {
-    double v[6];
+    S v[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) v[k] = (p[k + 1] - p[k]) * inv_ds;
-    const bool fore = vel < 0;
-    const double d = weno5_onesided(fore ? v[5] : v[0], fore ? v[4] : v[1], fore ? v[3] : v[2],
-                                     fore ? v[2] : v[3], fore ? v[1] : v[4]);
-    return acc + (fore ? vel : dmax(0.0, vel)) * d;
+    const bool fore = val(vel) < 0;
+    const S d = weno5_onesided(fore ? v[5] : v[0], fore ? v[4] : v[1], fore ? v[3] : v[2], fore ? v[2] : v[3],
+                               fore ? v[1] : v[4]);
+    return acc + (fore ? vel : dmax(S(0.0), vel)) * d;
 }
 
 // back/fore pair from 7 nodal values (re_step.cu:260-267), 1/ds precomputed
-__device__ __forceinline__ void weno5_pair(double& d_fore, double& d_back, const double p[7], double inv_ds)
+template <typename S>
+__device__ __forceinline__ void weno5_pair(S& d_fore, S& d_back, const S p[7], double inv_ds)
 {
-    double v[6];
+    S v[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) v[k] = (p[k + 1] - p[k]) * inv_ds;
     d_back = weno5_onesided(v[0], v[1], v[2], v[3], v[4]);
@@ -174,12 +180,13 @@ __device__ __forceinline__ bool opposite(float a, float b) { return (a < 0.f) ? 
 // re_step.cu:79-243 (weno_nonuniform), reconstructing at face a[2] (fore) or
 // a[3] (back) from first divided differences u[k] over [a[k], a[k+1]].
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double weno_nonuniform_face(const double* a, const double* u, bool back)
+template <typename S>
+__device__ __forceinline__ S weno_nonuniform_face(const S* a, const S* u, bool back)
 {
 #pragma clang fp contract(off)
-    const double xm25 = a[0], xm15 = a[1], xm05 = a[2], x05 = a[3], x15 = a[4], x25 = a[5];
-    const double um2 = u[0], um1 = u[1], u0 = u[2], u1 = u[3], u2 = u[4];
-    double v0, v1, v2, c0, c1, c2;
+    const S xm25 = a[0], xm15 = a[1], xm05 = a[2], x05 = a[3], x15 = a[4], x25 = a[5];
+    const S um2 = u[0], um1 = u[1], u0 = u[2], u1 = u[3], u2 = u[4];
+    S v0, v1, v2, c0, c1, c2;
     if (!back) {
         v0 = u1 + (u0 - u1) * (1.0 + (x05 - xm05) / (x15 - xm05) + (x05 - xm05) / (x25 - xm05))
                 + (u2 - u1) * ((x05 - xm05) / (x25 - xm05)) * ((x15 - xm05) / (x25 - x05));
@@ -203,61 +210,64 @@ __device__ __forceinline__ double weno_nonuniform_face(const double* a, const do
              * (1.0 + (x25 - xm15) / (x15 - xm25));
         c2 = ((x15 - x05) / (x25 - xm25)) * ((x25 - x05) / (x15 - xm25));
     }
-    const double h2 = sq(x05 - xm05);
-    const double s0 = 4.0 * sq((x05 - xm05) / (x25 - xm05)) *
+    const S h2 = sq(x05 - xm05);
+    const S s0 = 4.0 * sq((x05 - xm05) / (x25 - xm05)) *
         (sq((u2 - u1) / (x25 - x05)) * (10.0 * h2 + (x15 - xm05) * (x15 - x05))
          + (u2 - u1) * (u0 - u1) / ((x25 - x05) * (x15 - xm05)) *
                (20.0 * h2 + 2.0 * (x15 - xm05) * (x15 - x05) + (x25 - xm05) * (2.0 * x15 - x05 - xm05))
          + sq((u0 - u1) / (x15 - xm05)) * (10.0 * h2 + (x25 + x15 - 2.0 * xm05) * (x25 + x15 - x05 - xm05)));
-    const double s1 = 4.0 * sq((x05 - xm05) / (x15 - xm15)) *
+    const S s1 = 4.0 * sq((x05 - xm05) / (x15 - xm15)) *
         (sq((um1 - u0) / (x05 - xm15)) * (10.0 * h2 + (x15 - xm05) * (x15 - x05))
          + (u1 - u0) * (um1 - u0) / ((x15 - xm05) * (x05 - xm15)) *
                (20.0 * h2 - (x15 - x05) * (xm05 - xm15) - (x15 - xm05) * (x05 - xm15))
          + sq((u1 - u0) / (x15 - xm05)) * (10.0 * h2 + (xm05 - xm15) * (x05 - xm15)));
-    const double s2 = 4.0 * sq((x05 - xm05) / (x05 - xm25)) *
+    const S s2 = 4.0 * sq((x05 - xm05) / (x05 - xm25)) *
         (sq((um2 - um1) / (xm05 - xm25)) * (10.0 * h2 + (x05 - xm15) * (xm05 - xm15))
          + (u0 - um1) * (um2 - um1) / ((x05 - xm15) * (xm05 - xm25)) *
                (20.0 * h2 + 2.0 * (x05 - xm15) * (xm05 - xm15) + (x05 - xm25) * (x05 + xm05 - 2.0 * xm15))
          + sq((u0 - um1) / (x05 - xm15)) *
                (10.0 * h2 + (2.0 * x05 - xm25 - xm15) * (x05 + xm05 - xm15 - xm25)));
     const double eps = 1e-6;
-    const double al0 = c0 / sq(s0 + eps), al1 = c1 / sq(s1 + eps), al2 = c2 / sq(s2 + eps);
-    const double sum = al0 + al1 + al2;
+    const S al0 = c0 / sq(s0 + eps), al1 = c1 / sq(s1 + eps), al2 = c2 / sq(s2 + eps);
+    const S sum = al0 + al1 + al2;
     return v0 * (al0 / sum) + v1 * (al1 / sum) + v2 * (al2 / sum);
 }
 
-struct Pair {
-    double fore, back;
+template <typename S>
+struct PairT {
+    S fore, back;
 };
+typedef PairT<double> Pair;
 
 // select_stencil (re_step.cu:41-77; interface values h_fore/h_back as in
 // 2_2_WENO5RK3_Extend/shared_utilities.cup:49-85) and the first divided
 // differences (re_step.cu:83-88)
+template <typename S>
 struct NonuniformStencil {
-    double x[7], d[6];
+    S x[7], d[6];
 };
 
-__device__ __forceinline__ NonuniformStencil nonuniform_stencil(double p0, double p1, double p2, double p3, double p4,
-                                                                double p5, double p6, double r1, double l1, double ds,
-                                                                double h_fore, double h_back)
+template <typename S>
+__device__ __forceinline__ NonuniformStencil<S> nonuniform_stencil(S p0, S p1, S p2, S p3, S p4, S p5, S p6, S r1,
+                                                                   S l1, double ds, S h_fore, S h_back)
 {
 #pragma clang fp contract(off)
-    NonuniformStencil st;
-    double h[7];
+    NonuniformStencil<S> st;
+    S h[7];
     h[3] = p3;
-    st.x[3] = 0.0;
-    if (r1 < ds) {
-        st.x[4] = r1; st.x[5] = ds; st.x[6] = 2 * ds;
+    st.x[3] = S(0.0);
+    if (val(r1) < ds) {
+        st.x[4] = r1; st.x[5] = S(ds); st.x[6] = S(2 * ds);
         h[4] = h_fore; h[5] = p4; h[6] = p5;
     } else {
-        st.x[4] = ds; st.x[5] = 2 * ds; st.x[6] = 3 * ds;
+        st.x[4] = S(ds); st.x[5] = S(2 * ds); st.x[6] = S(3 * ds);
         h[4] = p4; h[5] = p5; h[6] = p6;
     }
-    if (l1 < ds) {
-        st.x[2] = -l1; st.x[1] = -ds; st.x[0] = -2 * ds;
+    if (val(l1) < ds) {
+        st.x[2] = -l1; st.x[1] = S(-ds); st.x[0] = S(-2 * ds);
         h[2] = h_back; h[1] = p2; h[0] = p1;
     } else {
-        st.x[2] = -ds; st.x[1] = -2 * ds; st.x[0] = -3 * ds;
+        st.x[2] = S(-ds); st.x[1] = S(-2 * ds); st.x[0] = S(-3 * ds);
         h[2] = p2; h[1] = p1; h[0] = p0;
     }
 #pragma unroll
@@ -268,28 +278,28 @@ __device__ __forceinline__ NonuniformStencil nonuniform_stencil(double p0, doubl
 // weno_nonuniform (re_step.cu:79-243), both faces.  Not inlined: it runs only
 // at interface nodes; scalar arguments and a returned struct keep the
 // caller's stencil in registers.
-__device__ __noinline__ Pair weno_nonuniform_pair(double p0, double p1, double p2, double p3, double p4, double p5,
-                                                  double p6, double r1, double l1, double ds, double h_fore,
-                                                  double h_back)
+template <typename S>
+__device__ __noinline__ PairT<S> weno_nonuniform_pair(S p0, S p1, S p2, S p3, S p4, S p5, S p6, S r1, S l1,
+                                                      double ds, S h_fore, S h_back)
 {
-    const NonuniformStencil st = nonuniform_stencil(p0, p1, p2, p3, p4, p5, p6, r1, l1, ds, h_fore, h_back);
-    Pair out;
+    const NonuniformStencil<S> st = nonuniform_stencil(p0, p1, p2, p3, p4, p5, p6, r1, l1, ds, h_fore, h_back);
+    PairT<S> out;
     out.fore = weno_nonuniform_face(&st.x[1], &st.d[1], false);
     out.back = weno_nonuniform_face(&st.x[0], &st.d[0], true);
     return out;
 }
 
 // one face only: d_fore (back == false) or d_back
-__device__ __noinline__ double weno_nonuniform_one(double p0, double p1, double p2, double p3, double p4, double p5,
-                                                   double p6, double r1, double l1, double ds, double h_fore,
-                                                   double h_back, bool back)
+template <typename S>
+__device__ __noinline__ S weno_nonuniform_one(S p0, S p1, S p2, S p3, S p4, S p5, S p6, S r1, S l1, double ds,
+                                              S h_fore, S h_back, bool back)
 {
-    const NonuniformStencil st = nonuniform_stencil(p0, p1, p2, p3, p4, p5, p6, r1, l1, ds, h_fore, h_back);
+    const NonuniformStencil<S> st = nonuniform_stencil(p0, p1, p2, p3, p4, p5, p6, r1, l1, ds, h_fore, h_back);
     return back ? weno_nonuniform_face(&st.x[0], &st.d[0], true) : weno_nonuniform_face(&st.x[1], &st.d[1], false);
 }
 
-__device__ __forceinline__ Pair weno_nonuniform_pair(const double v[7], double r1, double l1, double ds,
-                                                     double h_fore, double h_back)
+template <typename S>
+__device__ __forceinline__ PairT<S> weno_nonuniform_pair(const S v[7], S r1, S l1, double ds, S h_fore, S h_back)
 {
     return weno_nonuniform_pair(v[0], v[1], v[2], v[3], v[4], v[5], v[6], r1, l1, ds, h_fore, h_back);
 }
@@ -297,17 +307,18 @@ __device__ __forceinline__ Pair weno_nonuniform_pair(const double v[7], double r
 // ---------------------------------------------------------------------------
 // Godunov Hamiltonian residual, re_step.cu:382-392
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double godunov_step(bool inside, double xR, double xL, double yF, double yB,
-                                               double zU, double zD, double dt)
+template <typename S>
+__device__ __forceinline__ S godunov_step(bool inside, S xR, S xL, S yF, S yB, S zU, S zD, S dt)
 {
-    double g;
+    const S z(0.0);
+    S g;
     if (inside)
-        g = dmax(sq(dmin(0.0, xL)), sq(dmax(0.0, xR))) + dmax(sq(dmin(0.0, yB)), sq(dmax(0.0, yF)))
-            + dmax(sq(dmin(0.0, zD)), sq(dmax(0.0, zU)));
+        g = dmax(sq(dmin(z, xL)), sq(dmax(z, xR))) + dmax(sq(dmin(z, yB)), sq(dmax(z, yF)))
+            + dmax(sq(dmin(z, zD)), sq(dmax(z, zU)));
     else
-        g = dmax(sq(dmax(0.0, xL)), sq(dmin(0.0, xR))) + dmax(sq(dmax(0.0, yB)), sq(dmin(0.0, yF)))
-            + dmax(sq(dmax(0.0, zD)), sq(dmin(0.0, zU)));
-    const double s = (sqrt(g) - 1) * dt;
+        g = dmax(sq(dmax(z, xL)), sq(dmin(z, xR))) + dmax(sq(dmax(z, yB)), sq(dmin(z, yF)))
+            + dmax(sq(dmax(z, zD)), sq(dmin(z, zU)));
+    const S s = (sqrt(g) - 1.0) * dt;
     return inside ? -s : s;
 }
 

@@ -212,4 +212,33 @@ hipError_t launch_batch_curl_velocity(const BatchView& B, T* vx, T* vy, T* vz, c
 template <typename T>
 hipError_t launch_batch_ellipsoid(const BatchView& B, T* F, const double* axes, const double* lo, hipStream_t s);
 
+// ---- discrete adjoint and tangent (vls_adjoint.hip), FP64, one grid ----------
+// stage Jacobians: J = 21 rows of n (d out_i / d fs along the three lines
+// through i, row a*7 + offset+3), JU = 3 rows of n (d out_i / d u_a,i),
+// JP = 7 rows of lv.cap (d out / d (xpr..zpd, dt) of the listed points)
+hipError_t launch_adj_advect_jac(double* J, double* JU, const double* fs, const double* vx, const double* vy,
+                                 const double* vz, double dt, const Geom& g, int stage, hipStream_t s);
+hipError_t launch_adj_reinit_jac(double* J, double* JP, const double* fs, const uint8_t* flags, const ListView& lv,
+                                 const Geom& g, int stage, hipStream_t s);
+// lam_in = beta*lam_in + J^T lam;  out = J x
+hipError_t launch_adj_transpose(double* lam_in, const double* J, const double* lam, const Geom& g, double beta,
+                                hipStream_t s);
+hipError_t launch_adj_apply(double* out, const double* J, const double* x, const Geom& g, hipStream_t s);
+hipError_t launch_adj_fill(double* y, double v, int64_t n, hipStream_t s);
+hipError_t launch_adj_axpby(double* y, const double* x, double alpha, double beta, int64_t n, hipStream_t s);
+hipError_t launch_adj_velocity_grad(double* gx, double* gy, double* gz, const double* JU, const double* lam,
+                                    int64_t n, hipStream_t s);
+hipError_t launch_adj_velocity_tangent(double* out, const double* JU, const double* ux, const double* uy,
+                                       const double* uz, int64_t n, hipStream_t s);
+hipError_t launch_adj_param_grad(double* lamP, const double* JP, const double* lam, const ListView& lv,
+                                 hipStream_t s);
+hipError_t launch_adj_param_tangent(double* out, const double* JP, const double* Pdot, const ListView& lv,
+                                    hipStream_t s);
+// distances of the listed points as functions of F0: adjoint (atomic adds
+// into lamF0) and tangent (Pdot, 7 rows of lv.cap)
+hipError_t launch_adj_dist(double* lamF0, const double* F0, const double* lamP, const ListView& lv, const Geom& g,
+                           hipStream_t s);
+hipError_t launch_tan_dist(double* Pdot, const double* F0, const double* F0dot, const ListView& lv, const Geom& g,
+                           hipStream_t s);
+
 }  // namespace vls

@@ -27,8 +27,10 @@ cpu_baseline: the CPU oracle (OpenMP, FP64) on one full pipeline step of the
 same workload, timed on this host.
 
 Beside the main line: config1 (64^3 ENO2 forward-Euler reinitialization with
-its error measures) and config3 (the pipeline at 256^3 with a 32-mode
-velocity, FP32 bulk vs FP64, 200 steps each, with the FP32 run's errors).
+its error measures), config3 (the pipeline at 256^3 with a 32-mode velocity,
+FP32 bulk vs FP64, 200 steps each, with the FP32 run's errors) and config4
+(512 vesicles of uneven sizes batched, FP32 bulk vs FP64 batched vs the plain
+per-vesicle FP64 path).
 """
 import argparse
 import json
@@ -53,6 +55,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--precision-steps", type=int, default=200,
                     help="steps of the 256^3 FP32-bulk vs FP64 comparison (0 skips it)")
+    ap.add_argument("--batch-steps", type=int, default=10,
+                    help="timed steps of the 512-vesicle batch comparison (0 skips it)")
     ap.add_argument("--probe", choices=["extend"], default=None,
                     help="run only a few extension stages and a 256 MiB device copy (for rocprofv3 --pmc)")
     return ap.parse_args()
@@ -184,6 +188,24 @@ def config3(steps):
     return r
 
 
+def config4(steps):
+    """Config 4 beside the main line: 512 vesicles on cubes of edge
+    {32,48,64,80,96} (seeded), one 8-mode curl velocity each, advanced
+    together; FP32 bulk and FP64 through the batched persistent kernels
+    (graph-captured steps, autotuned tiles), and the plain FP64 path (one
+    object per vesicle) on every 16th vesicle; `steps` timed steps each (the
+    rate does not depend on the step count; the workload's 100 steps are not
+    all run)."""
+    import importlib
+
+    bm = importlib.import_module("2018_7_vesicle_amd.batch")
+    r = bm.batch_comparison(bm.BatchConfig(), steps=steps, sample_every=16)
+    r["workload"] = (f"512 vesicles, edges {{32,48,64,80,96}} on [-1,1]^3, ellipsoid semi-axes U[0.3,0.8], "
+                     f"8-mode curl velocity (seed 2018+v), WENO5-RK3 advection CFL 0.4, 40-iteration extension, "
+                     f"reinit 100 iterations every 5 steps; {steps} steps timed")
+    return r
+
+
 def main():
     args = parse()
     if args.probe == "extend":
@@ -270,6 +292,8 @@ def main():
     out["config1"] = config1(vls, not args.no_cpu)
     if args.precision_steps > 0:
         out["config3"] = config3(args.precision_steps)
+    if args.batch_steps > 0:
+        out["config4"] = config4(args.batch_steps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg)
     print(json.dumps(out), flush=True)

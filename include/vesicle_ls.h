@@ -308,6 +308,33 @@ int vls_batch_autotune_f32(vls_batch_t b, const float *F, int iterations, int *t
 int vls_sdf3_autotune(vls_sdf3_t h, const double *F, int iterations, int *tile_rows, vls_stream_t stream);
 int vls_sdf3_autotune_f32(vls_sdf3_t h, const float *F, int iterations, int *tile_rows, vls_stream_t stream);
 
+/* ------------------------------------------------------------------------ */
+/* 6. discrete adjoint and tangent (FP64, one grid; no counterpart in the   */
+/*    reference, which has no derivatives)                                  */
+/* ------------------------------------------------------------------------ */
+/* Exact derivatives of the discrete maps computed by vls_sdf3_advect_rk3
+ * (one step) and by set_distance + vls_sdf3_weno5_rk3_reinit (the
+ * reinitialization of F0 with F0's own distances and mask): WENO weights
+ * differentiated exactly, upwind sides, sign tests, Godunov branches and
+ * frozen points fixed as the forward pass takes them, subcell distances
+ * differentiated through their quintic root.  Boxes need >= 4 points per axis.
+ *
+ * advect: lam_in = (dF1/dF)^T lam_out; g{x,y,z} += (dF1/du_{x,y,z})^T lam_out
+ * (pointwise, accumulated: zero them first).  lam_in may alias lam_out.
+ * Tangent: Fout = F1, Fdot_out = (dF1/dF) Fdot + sum_a (dF1/du_a) udot_a. */
+int vls_sdf3_adjoint_advect_rk3(vls_sdf3_t h, const double *F, const double *vx, const double *vy, const double *vz,
+                                double dt, const double *lam_out, double *lam_in, double *gx, double *gy, double *gz,
+                                vls_stream_t stream);
+int vls_sdf3_tangent_advect_rk3(vls_sdf3_t h, const double *F, const double *Fdot, const double *vx,
+                                const double *vy, const double *vz, const double *udx, const double *udy,
+                                const double *udz, double dt, double *Fout, double *Fdot_out, vls_stream_t stream);
+/* reinit: lam_in = (dR/dF0)^T lam_out; tangent: Fout = R(F0), Fdot_out = (dR/dF0) F0dot.
+ * Both recompute R and leave the handle with F0's distances and mask. */
+int vls_sdf3_adjoint_reinit(vls_sdf3_t h, const double *F0, int iterations, const double *lam_out, double *lam_in,
+                            vls_stream_t stream);
+int vls_sdf3_tangent_reinit(vls_sdf3_t h, const double *F0, const double *F0dot, int iterations, double *Fout,
+                            double *Fdot_out, vls_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,177 @@
+"""Discrete adjoint and tangent kernels (vls_sdf3_adjoint_*, vls_sdf3_tangent_*).
+
+Per operator (one TVD-RK3 advection step; the reinitialization map
+setDistance + mask + WENO5-RK3 iterations):
+  * the tangent against FP64 central differences of the forward entry
+    points (the same code the pipeline runs);
+  * the adjoint against the tangent through the dot-product identity
+    <lam, J v> = <J^T lam, v>.
+End to end: the shape-matching gradient with checkpointing, wrapped in a
+torch.autograd.Function (here only, as a harness), against central
+differences on every velocity coefficient and against the tangent-mode
+directional derivatives on 8 random directions.
+"""
+import ctypes
+import importlib
+
+import numpy as np
+import pytest
+import torch
+
+from golden import cases
+
+vls = importlib.import_module("2018_7_vesicle_amd")
+adj = importlib.import_module("2018_7_vesicle_amd.adjoint")
+
+pytestmark = pytest.mark.gpu
+
+P = ctypes.c_void_p
+
+
+def _p(t):
+    return P(t.data_ptr())
+
+
+def _setup(n=24, seed=3):
+    g = cases.Grid(np.linspace(-1, 1, n), np.linspace(-1.05, 0.95, n + 2), np.linspace(-0.9, 1.1, n - 1))
+    G = vls.GD3(g.xv, g.yv, g.zv)
+    r = np.sqrt(g.X ** 2 + g.Y ** 2 + g.Z ** 2)
+    F = (r - 0.55) * (1 + 0.15 * np.sin(3 * g.X) * np.cos(2 * g.Y))
+    u = cases.smooth_velocity(g)
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    rng = np.random.default_rng(seed)
+    rand = lambda: T(rng.normal(size=F.shape))  # noqa: E731
+
+    def smooth():
+        """a smooth random field (a few random low modes): a direction in which
+        central differences stay in the WENO weights' linear range"""
+        out = np.zeros(F.shape)
+        for _ in range(4):
+            k = rng.normal(size=3) * 2
+            out += rng.normal() * np.cos(k[0] * g.X + k[1] * g.Y + k[2] * g.Z + rng.uniform(0, 6.3))
+        return T(out)
+
+    return G, T(F), [T(c) for c in u], rand, smooth
+
+
+def _advect(m, F, u, dt):
+    out = F.clone()
+    m.advect_(out, *u, dt, 1)
+    return out
+
+
+def _reinit(m, F, K):
+    out = F.clone()
+    m.F = out
+    m.setDistance()
+    m.reinitialize_(out, K)
+    return out
+
+
+def _tangent_advect(m, F, Fd, u, ud, dt):
+    Fo, Fdo = torch.empty_like(F), torch.empty_like(F)
+    rc = vls.lib().vls_sdf3_tangent_advect_rk3(m._h, _p(F), _p(Fd), *[_p(c) for c in u], *[_p(c) for c in ud],
+                                               ctypes.c_double(dt), _p(Fo), _p(Fdo), None)
+    assert rc == 0
+    return Fo, Fdo
+
+
+def test_advection_tangent_and_adjoint():
+    G, F, u, rand, smooth = _setup()
+    m = vls.SDF3(G, F)
+    dt = 0.4 * G.Dx / m.maxSpeed(*u)
+    # dot-product identity on random fields
+    Fd, ud = rand(), [rand() for _ in range(3)]
+    Fo, Fdo = _tangent_advect(m, F, Fd, u, ud, dt)
+    assert torch.equal(Fo, _advect(m, F, u, dt))  # the tangent's forward is the forward
+    lam = rand()
+    lin = torch.empty_like(F)
+    gu = [torch.zeros_like(F) for _ in range(3)]
+    vls.lib().vls_sdf3_adjoint_advect_rk3(m._h, _p(F), *[_p(c) for c in u], ctypes.c_double(dt), _p(lam), _p(lin),
+                                          *[_p(g) for g in gu], None)
+    lhs = float((lam * Fdo).sum())
+    rhs = float((lin * Fd).sum()) + sum(float((g * d).sum()) for g, d in zip(gu, ud))
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs), (lhs, rhs)
+    # central differences of the forward map along a smooth direction: the
+    # error falls with h^2 (two step sizes) down to 1e-8 of the tangent
+    Fd, ud = smooth(), [smooth() for _ in range(3)]
+    _, Fdo = _tangent_advect(m, F, Fd, u, ud, dt)
+    errs = []
+    for h in (1e-4, 1e-6):
+        fp = _advect(m, F + h * Fd, [c + h * d for c, d in zip(u, ud)], dt)
+        fm = _advect(m, F - h * Fd, [c - h * d for c, d in zip(u, ud)], dt)
+        errs.append(float(((fp - fm) / (2 * h) - Fdo).norm() / Fdo.norm()))
+    assert errs[1] < 1e-8, errs
+
+
+def _tangent_reinit(m, F, Fd, K):
+    Fo, Fdo = torch.empty_like(F), torch.empty_like(F)
+    assert vls.lib().vls_sdf3_tangent_reinit(m._h, _p(F), _p(Fd), K, _p(Fo), _p(Fdo), None) == 0
+    return Fo, Fdo
+
+
+@pytest.mark.parametrize("K", [1, 4])
+def test_reinit_tangent_and_adjoint(K):
+    G, F, u, rand, smooth = _setup(seed=5 + K)
+    m = vls.SDF3(G, F)
+    Fd = rand()
+    Fo, Fdo = _tangent_reinit(m, F, Fd, K)
+    assert torch.equal(Fo, _reinit(m, F, K))
+    lam = rand()
+    lin = torch.empty_like(F)
+    assert vls.lib().vls_sdf3_adjoint_reinit(m._h, _p(F), K, _p(lam), _p(lin), None) == 0
+    lhs = float((lam * Fdo).sum())
+    rhs = float((lin * Fd).sum())
+    assert abs(lhs - rhs) <= 1e-11 * abs(lhs), (lhs, rhs)
+    # central differences along a smooth direction
+    Fd = smooth()
+    _, Fdo = _tangent_reinit(m, F, Fd, K)
+    errs = []
+    for h in (1e-5, 1e-7):
+        fd = (_reinit(m, F + h * Fd, K) - _reinit(m, F - h * Fd, K)) / (2 * h)
+        errs.append(float((fd - Fdo).norm() / Fdo.norm()))
+    assert errs[1] < 1e-7, errs
+
+
+class AdjointLoss(torch.autograd.Function):
+    """L(c) of adjoint.ShapeMatching with its hand-written adjoint as backward"""
+
+    @staticmethod
+    def forward(ctx, c, problem):
+        L, g = problem.gradient(c.detach().cpu().numpy())
+        ctx.save_for_backward(torch.from_numpy(g))
+        return torch.tensor(L, dtype=torch.float64)
+
+    @staticmethod
+    def backward(ctx, grad):
+        (g,) = ctx.saved_tensors
+        return grad * g, None
+
+
+def test_shape_matching_gradient_fd_and_jvp():
+    cfg = adj.AdjointConfig(n=32, steps=20, reinit_every=10, reinit_iters=4, ckpt_every=5)
+    prob = adj.ShapeMatching(cfg)
+    c = torch.from_numpy(prob.c0.copy()).requires_grad_(True)
+    L = AdjointLoss.apply(c, prob)
+    L.backward()
+    g = c.grad.numpy()
+    assert abs(float(L) - prob.loss(prob.c0)) <= 1e-15 * abs(float(L))
+    # central differences, h = 1e-6, every coefficient
+    h = 1e-6
+    fd = np.zeros_like(g)
+    for a in range(3):
+        for k in range(cfg.modes):
+            cp = prob.c0.copy()
+            cp[a, k] += h
+            cm = prob.c0.copy()
+            cm[a, k] -= h
+            fd[a, k] = (prob.loss(cp) - prob.loss(cm)) / (2 * h)
+    rel = np.linalg.norm(g - fd) / np.linalg.norm(fd)
+    assert rel < 1e-6, rel
+    # tangent-mode directional derivatives
+    rng = np.random.default_rng(11)
+    for _ in range(8):
+        d = rng.normal(size=g.shape)
+        Lj, dj = prob.jvp(prob.c0, d)
+        assert abs(Lj - float(L)) <= 1e-15 * abs(float(L))
+        assert abs(dj - float((g * d).sum())) <= 1e-9 * abs(dj), (dj, float((g * d).sum()))
