@@ -28,9 +28,10 @@ same workload, timed on this host.
 
 Beside the main line: config1 (64^3 ENO2 forward-Euler reinitialization with
 its error measures), config3 (the pipeline at 256^3 with a 32-mode velocity,
-FP32 bulk vs FP64, 200 steps each, with the FP32 run's errors) and config4
+FP32 bulk vs FP64, 200 steps each, with the FP32 run's errors), config4
 (512 vesicles of uneven sizes batched, FP32 bulk vs FP64 batched vs the plain
-per-vesicle FP64 path).
+per-vesicle FP64 path) and config5 (the 96^3 adjoint gradient against finite
+differences and tangent-mode derivatives).
 """
 import argparse
 import json
@@ -57,6 +58,7 @@ def parse():
                     help="steps of the 256^3 FP32-bulk vs FP64 comparison (0 skips it)")
     ap.add_argument("--batch-steps", type=int, default=10,
                     help="timed steps of the 512-vesicle batch comparison (0 skips it)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the 96^3 adjoint gradient check")
     ap.add_argument("--probe", choices=["extend"], default=None,
                     help="run only a few extension stages and a 256 MiB device copy (for rocprofv3 --pmc)")
     return ap.parse_args()
@@ -206,6 +208,25 @@ def config4(steps):
     return r
 
 
+def config5():
+    """Config 5 beside the main line: the 96^3 shape-matching problem (200
+    WENO5-RK3 advection steps, reinitialization every 10, velocity from 3x64
+    Fourier coefficients, cosine-Heaviside mismatch to an ellipsoid); the
+    adjoint gradient (checkpoint every 20 steps) against FP64 central
+    differences on all 192 coefficients (h = 1e-6, and h = 1e-8, a step that
+    stays clear of the scheme's branch thresholds) and tangent-mode
+    directional derivatives on 8 directions."""
+    import importlib
+
+    adj = importlib.import_module("2018_7_vesicle_amd.adjoint")
+    cfg = adj.AdjointConfig()
+    r = adj.check_gradient(adj.ShapeMatching(cfg), hs=(1e-6, 1e-8), directions=8)
+    r["workload"] = (f"{cfg.n}^3 on [-1,1]^3, {cfg.steps} steps, reinit ({cfg.reinit_iters} iterations) every "
+                     f"{cfg.reinit_every}, 3x{cfg.modes} coefficients ~ N(0,{cfg.coef_std}^2), checkpoint every "
+                     f"{cfg.ckpt_every} steps, FP64")
+    return r
+
+
 def main():
     args = parse()
     if args.probe == "extend":
@@ -294,6 +315,8 @@ def main():
         out["config3"] = config3(args.precision_steps)
     if args.batch_steps > 0:
         out["config4"] = config4(args.batch_steps)
+    if not args.no_adjoint:
+        out["config5"] = config5()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg)
     print(json.dumps(out), flush=True)
