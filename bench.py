@@ -43,6 +43,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 B_ALG_EXTEND_FP64 = 37.3  # bytes per point per FP64 WENO5 extension stage
+B_ALG_EXTEND_FP32 = 22.7  # the same with float C and velocity, FP64 phi
 HBM_PEAK_GBS = 8000.0
 
 
@@ -59,8 +60,13 @@ def parse():
     ap.add_argument("--batch-steps", type=int, default=10,
                     help="timed steps of the 512-vesicle batch comparison (0 skips it)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 96^3 adjoint gradient check")
-    ap.add_argument("--probe", choices=["extend"], default=None,
-                    help="run only a few extension stages and a 256 MiB device copy (for rocprofv3 --pmc)")
+    ap.add_argument("--probe", choices=["extend", "batch"], default=None,
+                    help="extend: a few extension stages and a 256 MiB device copy (for rocprofv3 --pmc); "
+                         "batch: a few steps of the 512-vesicle FP32 batch (for rocprofv3 --kernel-trace)")
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64",
+                    help="field type of the main pipeline line (fp32: FP32 bulk)")
+    ap.add_argument("--main-only", action="store_true",
+                    help="only the main line: no config1/3/4/5 and no CPU baseline")
     return ap.parse_args()
 
 
@@ -86,6 +92,23 @@ def probe_extend(args):
     dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"probe": "extend", "n": args.n, "stage_launches": 9, "copy_bytes": 8 << 25}), flush=True)
+    return 0
+
+
+def probe_batch(args):
+    """A short batch run for kernel traces: the 512-vesicle FP32 workload
+    (autotuned), both step graphs captured, then 3 steps."""
+    import importlib
+
+    import torch
+
+    bm = importlib.import_module("2018_7_vesicle_amd.batch")
+    p = bm.BatchPipeline(bm.BatchConfig())
+    p.warm()
+    upd = p.run(3)
+    torch.cuda.synchronize()
+    print(json.dumps({"probe": "batch", "points": p.batch.total, "updates": upd, "tile_rows": p.batch.tile_rows,
+                      "blocks": p.batch.blocks}), flush=True)
     return 0
 
 
@@ -231,6 +254,8 @@ def main():
     args = parse()
     if args.probe == "extend":
         return probe_extend(args)
+    if args.probe == "batch":
+        return probe_batch(args)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported by this build "
               "(the multi-GPU z-slab decomposition is a design note, not code)", file=sys.stderr)
@@ -242,7 +267,7 @@ def main():
     vls = importlib.import_module("2018_7_vesicle_amd")
     pl = vls.pipeline
     torch.cuda.set_device(0)
-    cfg = pl.PipelineConfig(n=args.n)
+    cfg = pl.PipelineConfig(n=args.n, precision=args.precision)
     p = pl.Pipeline(cfg)
     n = p.grid.NumElt
     if args.warmup:
@@ -268,11 +293,14 @@ def main():
         torch.cuda.synchronize()
         setup_ms.append(e0.elapsed_time(e1))
     stage_s = (sum(ext_ms) / len(ext_ms) - sum(setup_ms) / len(setup_ms)) / (3 * cfg.ext_iters) * 1e-3
-    achieved = B_ALG_EXTEND_FP64 * n / stage_s / 1e9
+    b_alg = B_ALG_EXTEND_FP64 if args.precision == "fp64" else B_ALG_EXTEND_FP32
+    achieved = b_alg * n / stage_s / 1e9
 
     av = p.area_volume.cpu().numpy()
+    fp64 = args.precision == "fp64"
     out = {
-        "metric": "WENO grid-pt updates/s (FP64 vesicle-step pipeline)",
+        "metric": "WENO grid-pt updates/s (FP64 vesicle-step pipeline)" if fp64 else
+                  "WENO grid-pt updates/s (FP32-bulk vesicle-step pipeline)",
         "value": updates / wall,
         "unit": "grid-pt updates/s",
         "n_gpus": 1,
@@ -282,7 +310,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if fp64 else "f32 bulk, f64 interface and reductions",
         "data": "synthetic",
         "config": {
             "workload": f"config-2 vesicle step: {cfg.n}^3 on [{cfg.lo},{cfg.hi}]^3, ellipsoid {cfg.axes}, "
@@ -294,15 +322,16 @@ def main():
             "updates_per_step_avg": updates / args.steps / n,
         },
         "roofline": {
-            "kernel": "FP64 WENO5-RK3 extension stage (k_extend_bulk + k_extend_iface launches)",
+            "kernel": ("FP64" if fp64 else "FP32-bulk") +
+                      " WENO5-RK3 extension stage (k_extend_bulk + k_extend_iface launches)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic_from_profiles(),
+            "traffic": traffic_from_profiles() if (fp64 and args.n == 128) else None,
             "avg_launch_us": stage_s * 1e6,
-            "bytes_per_point": B_ALG_EXTEND_FP64,
+            "bytes_per_point": b_alg,
         },
         "diagnostics": {
             "area_first_last": [float(av[0, 0]), float(av[-1, 0])],
@@ -310,15 +339,16 @@ def main():
             "dt": p.dt,
         },
     }
-    out["config1"] = config1(vls, not args.no_cpu)
-    if args.precision_steps > 0:
-        out["config3"] = config3(args.precision_steps)
-    if args.batch_steps > 0:
-        out["config4"] = config4(args.batch_steps)
-    if not args.no_adjoint:
-        out["config5"] = config5()
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(cfg)
+    if not args.main_only:
+        out["config1"] = config1(vls, not args.no_cpu)
+        if args.precision_steps > 0:
+            out["config3"] = config3(args.precision_steps)
+        if args.batch_steps > 0:
+            out["config4"] = config4(args.batch_steps)
+        if not args.no_adjoint:
+            out["config5"] = config5()
+        if not args.no_cpu:
+            out["cpu_baseline"] = cpu_baseline(cfg)
     print(json.dumps(out), flush=True)
     return 0
 
