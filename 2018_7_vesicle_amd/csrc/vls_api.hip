@@ -85,7 +85,38 @@ Dist6c mk6c(const double* a, const double* b, const double* c, const double* d, 
 // is keyed by them; it keeps the 8 most recently used graphs per handle.  The
 // captured work has no memset nodes: counters are cleared by kernels.
 // ---------------------------------------------------------------------------
-int g_graphs = 1;  // vls_set_graphs
+int g_graphs = 1;   // vls_set_graphs
+int g_overlap = 0;  // vls_set_overlap (off: measured no gain, see vls_set_overlap)
+
+// the side stream and events of a handle's overlapped interface passes,
+// created on first use; nullptr when overlap is off
+const Fork* fork_of(Fork& f)
+{
+    if (!g_overlap) return nullptr;
+    if (!f.side) {
+        if (hipStreamCreateWithFlags(&f.side, hipStreamNonBlocking) != hipSuccess) {
+            f.side = nullptr;
+            return nullptr;
+        }
+        if (hipEventCreateWithFlags(&f.fork, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&f.join, hipEventDisableTiming) != hipSuccess) {
+            (void)hipStreamDestroy(f.side);
+            f.side = nullptr;
+            return nullptr;
+        }
+    }
+    return &f;
+}
+
+void destroy_fork(Fork& f)
+{
+    if (f.side) {
+        (void)hipStreamDestroy(f.side);
+        (void)hipEventDestroy(f.fork);
+        (void)hipEventDestroy(f.join);
+        f.side = nullptr;
+    }
+}
 
 struct GraphKey {
     uint64_t v[16] = {};
@@ -138,9 +169,11 @@ struct GraphCache {
 // run `issue(stream)` through the cache: replay the graph of `key`, or
 // capture issue's launches into one and replay it
 template <class Issue>
-int with_graph(GraphCache& gc, const GraphKey& key, hipStream_t s, Issue&& issue)
+int with_graph(GraphCache& gc, const GraphKey& key0, hipStream_t s, Issue&& issue)
 {
     if (!g_graphs) return issue(s);
+    GraphKey key = key0;
+    key.v[15] = (uint64_t)g_overlap;  // the graph's shape depends on it
     for (auto& e : gc.entries)
         if (e.key == key) {
             e.used = ++gc.tick;
@@ -206,6 +239,7 @@ struct vls_sdf3 {
     GraphCache graphs;
     double* adj = nullptr;  // adjoint/tangent workspace, grown on demand
     size_t adj_len = 0;
+    Fork fork{nullptr, nullptr, nullptr};
 };
 
 namespace {
@@ -259,10 +293,11 @@ int extend_loop(vls_sdf3* h, T* C, int iterations, const DSet& ds, hipStream_t s
     T* t1 = tmp<T>(h, 1);
     const T *vx = vel<T>(h, 0), *vy = vel<T>(h, 1), *vz = vel<T>(h, 2);
     VLS_HIP(launch_list_interp(h->lcp, ds.lv, (const T*)C, g, s));
+    const Fork* fk = fork_of(h->fork);
     for (int it = 0; it < iterations; ++it) {
-        VLS_HIP(launch_extend_stage<T>(t0, C, nullptr, vx, vy, vz, ds.flags, ds.lv, h->lcp, h->lvel, g, 0, s));
-        VLS_HIP(launch_extend_stage<T>(t1, t0, C, vx, vy, vz, ds.flags, ds.lv, h->lcp, h->lvel, g, 1, s));
-        VLS_HIP(launch_extend_stage<T>(C, t1, C, vx, vy, vz, ds.flags, ds.lv, h->lcp, h->lvel, g, 2, s));
+        VLS_HIP(launch_extend_stage<T>(t0, C, nullptr, vx, vy, vz, ds.flags, ds.lv, h->lcp, h->lvel, g, 0, s, fk));
+        VLS_HIP(launch_extend_stage<T>(t1, t0, C, vx, vy, vz, ds.flags, ds.lv, h->lcp, h->lvel, g, 1, s, fk));
+        VLS_HIP(launch_extend_stage<T>(C, t1, C, vx, vy, vz, ds.flags, ds.lv, h->lcp, h->lvel, g, 2, s, fk));
     }
     return VLS_OK;
 }
@@ -472,6 +507,13 @@ int vls_set_graphs(int enable)
 {
     const int prev = g_graphs;
     g_graphs = enable ? 1 : 0;
+    return prev;
+}
+
+int vls_set_overlap(int enable)
+{
+    const int prev = g_overlap;
+    g_overlap = enable ? 1 : 0;
     return prev;
 }
 const char* vls_version(void) { return "vesicle_ls 0.1 (gfx950, FP64)"; }
@@ -705,6 +747,7 @@ int vls_sdf3_destroy(vls_sdf3_t h)
 {
     if (!h) return VLS_OK;
     (void)hipFree(h->adj);
+    destroy_fork(h->fork);
     hipError_t e1 = hipFree(h->pool), e2 = hipFree(h->flag_pool), e3 = hipFree(h->int_pool);
     delete h;
     if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) return fail(VLS_ERR_HIP, "hipFree failed");
@@ -882,6 +925,7 @@ struct vls_batch {
     double* v[3] = {nullptr, nullptr, nullptr};
     double* partial = nullptr;  // per-tile partial sums (4 per tile of either shape)
     GraphCache graphs;
+    Fork fork{nullptr, nullptr, nullptr};
 
     BatchView view() const { return BatchView{d_grids, d_tile_grid, d_offs, ngrids, ntiles, blocks, tr}; }
     ~vls_batch()
@@ -893,6 +937,7 @@ struct vls_batch {
         (void)hipFree(flags);
         (void)hipFree(int_pool);
         (void)hipFree(partial);
+        destroy_fork(fork);
     }
 };
 
@@ -987,10 +1032,12 @@ int batch_extend(vls_batch* b, const T* F, T* C, int iterations, hipStream_t s)
     VLS_HIP(launch_batch_extension_velocity<T>(B, vx, vy, vz, F, b->flags, b->lv, 1e-6, s));
     VLS_HIP(launch_batch_list_gather_vel<T>(b->lvel, b->lv, vx, vy, vz, s));
     VLS_HIP(launch_batch_list_interp<T>(B, b->lcp, b->lv, C, s));
+    const Fork* fk = fork_of(b->fork);
     for (int it = 0; it < iterations; ++it) {
-        VLS_HIP(launch_batch_extend_stage<T>(B, t0, C, nullptr, vx, vy, vz, b->flags, b->lv, b->lcp, b->lvel, 0, s));
-        VLS_HIP(launch_batch_extend_stage<T>(B, t1, t0, C, vx, vy, vz, b->flags, b->lv, b->lcp, b->lvel, 1, s));
-        VLS_HIP(launch_batch_extend_stage<T>(B, C, t1, C, vx, vy, vz, b->flags, b->lv, b->lcp, b->lvel, 2, s));
+        VLS_HIP(launch_batch_extend_stage<T>(B, t0, C, nullptr, vx, vy, vz, b->flags, b->lv, b->lcp, b->lvel, 0, s,
+                                             fk));
+        VLS_HIP(launch_batch_extend_stage<T>(B, t1, t0, C, vx, vy, vz, b->flags, b->lv, b->lcp, b->lvel, 1, s, fk));
+        VLS_HIP(launch_batch_extend_stage<T>(B, C, t1, C, vx, vy, vz, b->flags, b->lv, b->lcp, b->lvel, 2, s, fk));
     }
     return VLS_OK;
 }

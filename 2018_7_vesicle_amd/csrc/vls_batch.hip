@@ -603,8 +603,20 @@ hipError_t launch_batch_reinit_stage(const BatchView& B, T* out, const T* fs, co
 template <typename T>
 hipError_t launch_batch_extend_stage(const BatchView& B, T* out, const T* cs, const T* cn, const T* vx,
                                      const T* vy, const T* vz, const uint8_t* flags, const ListView& lv,
-                                     const double* lcp, const double* lvel, int stage, hipStream_t s)
+                                     const double* lcp, const double* lvel, int stage, hipStream_t s,
+                                     const Fork* fk)
 {
+    if (fk && fk->side) {
+        hipError_t e;
+        if ((e = hipEventRecord(fk->fork, s)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(fk->side, fk->fork, 0)) != hipSuccess) return e;
+        kb_extend_iface<T><<<list_blocks(B), 256, 0, fk->side>>>(out, cs, cn, ilist(lv), lcp, lvel, B, stage);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        TR_LAUNCH(kb_extend_bulk, T, B.blocks, out, cs, cn, vx, vy, vz, flags, B, stage);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if ((e = hipEventRecord(fk->join, fk->side)) != hipSuccess) return e;
+        return hipStreamWaitEvent(s, fk->join, 0);
+    }
     TR_LAUNCH(kb_extend_bulk, T, B.blocks, out, cs, cn, vx, vy, vz, flags, B, stage);
     kb_extend_iface<T><<<list_blocks(B), 256, 0, s>>>(out, cs, cn, ilist(lv), lcp, lvel, B, stage);
     return hipGetLastError();
@@ -691,7 +703,7 @@ hipError_t launch_batch_ellipsoid(const BatchView& B, T* F, const double* axes, 
                                                      const ListView&, int, int*, int*, int, hipStream_t);           \
     template hipError_t launch_batch_extend_stage<T>(const BatchView&, T*, const T*, const T*, const T*, const T*,  \
                                                      const T*, const uint8_t*, const ListView&, const double*,      \
-                                                     const double*, int, hipStream_t);                              \
+                                                     const double*, int, hipStream_t, const Fork*);                 \
     template hipError_t launch_batch_advect_stage<T>(const BatchView&, T*, const T*, const T*, const T*, const T*,  \
                                                      const T*, const double*, int, hipStream_t);                    \
     template hipError_t launch_batch_extension_velocity<T>(const BatchView&, T*, T*, T*, const T*, const uint8_t*,  \

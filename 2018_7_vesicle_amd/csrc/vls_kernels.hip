@@ -931,15 +931,29 @@ hipError_t launch_extend_step(double* step, const double* deltat, const double* 
 template <typename T>
 hipError_t launch_extend_stage(T* out, const T* cs, const T* cn, const T* vx, const T* vy, const T* vz,
                                const uint8_t* flags, const ListView& lv, const double* lcp, const double* lvel,
-                               const Geom& g, int stage, hipStream_t s)
+                               const Geom& g, int stage, hipStream_t s, const Fork* fk)
 {
+    const IList il{lv.idx, lv.count, lv.dist, lv.dt, lv.cap};
+    if (fk && fk->side) {
+        hipError_t e;
+        if ((e = hipEventRecord(fk->fork, s)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(fk->side, fk->fork, 0)) != hipSuccess) return e;
+        k_extend_iface<T><<<dim3(iface_blocks(g)), dim3(256), 0, fk->side>>>(out, cs, cn, il, lcp, lvel, box_of(g),
+                                                                              inv_of(g), stage);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
     if (g.tr == 32)
         k_extend_bulk<32, T><<<dim3(bulk_blocks(g)), dim3(256), 0, s>>>(out, cs, cn, vx, vy, vz, flags, box_of(g),
                                                                          inv_of(g), stage);
     else
         k_extend_bulk<64, T><<<dim3(bulk_blocks(g)), dim3(256), 0, s>>>(out, cs, cn, vx, vy, vz, flags, box_of(g),
                                                                          inv_of(g), stage);
-    const IList il{lv.idx, lv.count, lv.dist, lv.dt, lv.cap};
+    if (fk && fk->side) {
+        hipError_t e;
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if ((e = hipEventRecord(fk->join, fk->side)) != hipSuccess) return e;
+        return hipStreamWaitEvent(s, fk->join, 0);
+    }
     k_extend_iface<T><<<dim3(iface_blocks(g)), dim3(256), 0, s>>>(out, cs, cn, il, lcp, lvel, box_of(g), inv_of(g),
                                                                    stage);
     return hipGetLastError();
@@ -1046,7 +1060,7 @@ hipError_t launch_copy(T* dst, const T* src, int64_t n, hipStream_t s)
     template hipError_t launch_extension_velocity<T>(T*, T*, T*, const T*, double, const Geom&, hipStream_t);       \
     template hipError_t launch_extend_stage<T>(T*, const T*, const T*, const T*, const T*, const T*,               \
                                                const uint8_t*, const ListView&, const double*, const double*,       \
-                                               const Geom&, int, hipStream_t);                                      \
+                                               const Geom&, int, hipStream_t, const Fork*);                         \
     template hipError_t launch_advect_stage<T>(T*, const T*, const T*, const T*, const T*, const T*, double,        \
                                                const Geom&, int, hipStream_t);                                      \
     template hipError_t launch_normal_velocity<T>(T*, const T*, const T*, const T*, const T*, const Geom&,          \
