@@ -45,6 +45,8 @@ int check_geom(int rows, int cols, int pges, double dx, double dy, double dz)
     if (rows <= 0 || cols <= 0 || pges <= 0) return fail(VLS_ERR_INVALID, "box %dx%dx%d is empty", rows, cols, pges);
     if (!(dx > 0) || !(dy > 0) || !(dz > 0)) return fail(VLS_ERR_INVALID, "spacings must be positive");
     if ((int64_t)rows * cols * pges > ((int64_t)1 << 40)) return fail(VLS_ERR_INVALID, "box too large");
+    // the stage kernels address 7 pages with 32-bit byte offsets
+    if ((int64_t)rows * cols >= ((int64_t)1 << 26)) return fail(VLS_ERR_INVALID, "a page holds 2^26 points or more");
     if (pges > 65535 || (cols + 3) / 4 > 65535) return fail(VLS_ERR_INVALID, "box exceeds the launch grid");
     return VLS_OK;
 }

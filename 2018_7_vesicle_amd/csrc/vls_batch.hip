@@ -163,11 +163,15 @@ __global__ void __launch_bounds__(256) kb_reinit_bulk(T* __restrict__ out, const
         const BGrid G = B.grids[B.tile_grid[tile]];
         const Box b = box_of(G);
         const Tile t = tile_at<R>(b, tile - G.tile0);
-        fill_tile<R>(lds, fs + G.off, b, t);
-        const idx_t ind = t.valid ? lin(b, t.r, t.c, t.p) : 0;
-        const bool defer = reinit_bulk_point<R>(out + G.off, lds, fs + G.off, shift(fn, G.off), flags + G.off, b, t,
-                                             ind, inv_of(G), stage);
-        wave_append(defer, (int)(G.off + ind), over, &over_count[seq & 1]);
+        TileRegs<R, T> g;
+        load_tile<R>(g, fs + G.off, b, t);
+        BulkIn<T, false> q;
+        bulk_load(q, fs + G.off, shift(fn, G.off), (const T*)nullptr, (const T*)nullptr, (const T*)nullptr, b, t,
+                  stage);
+        store_tile<R>(lds, g);
+        __syncthreads();
+        const bool defer = reinit_bulk_point<R>(out + G.off, lds, q, flags + G.off, t, inv_of(G), stage);
+        wave_append(defer, (int)(G.off + t.page + t.in), over, &over_count[seq & 1]);
         __syncthreads();
     }
 }
@@ -208,9 +212,13 @@ __global__ void __launch_bounds__(256) kb_extend_bulk(T* __restrict__ out, const
         const BGrid G = B.grids[B.tile_grid[tile]];
         const Box b = box_of(G);
         const Tile t = tile_at<R>(b, tile - G.tile0);
-        fill_tile<R>(lds, cs + G.off, b, t);
-        extend_bulk_point<R>(out + G.off, lds, cs + G.off, shift(cn, G.off), vx + G.off, vy + G.off, vz + G.off,
-                          flags + G.off, b, t, inv_of(G), stage);
+        TileRegs<R, T> g;
+        load_tile<R>(g, cs + G.off, b, t);
+        BulkIn<T, true> q;
+        bulk_load(q, cs + G.off, shift(cn, G.off), vx + G.off, vy + G.off, vz + G.off, b, t, stage);
+        store_tile<R>(lds, g);
+        __syncthreads();
+        extend_bulk_point<R>(out + G.off, lds, q, flags + G.off, t, inv_of(G), stage);
         __syncthreads();
     }
 }
@@ -249,9 +257,13 @@ __global__ void __launch_bounds__(256) kb_advect_stage(T* __restrict__ out, cons
         const BGrid G = B.grids[g];
         const Box b = box_of(G);
         const Tile t = tile_at<R>(b, tile - G.tile0);
-        fill_tile<R>(lds, fs + G.off, b, t);
-        advect_point<R>(out + G.off, lds, fs + G.off, shift(fn, G.off), vx + G.off, vy + G.off, vz + G.off, dt[g], b, t,
-                     inv_of(G), stage);
+        TileRegs<R, T> tv;
+        load_tile<R>(tv, fs + G.off, b, t);
+        BulkIn<T, true> q;
+        bulk_load(q, fs + G.off, shift(fn, G.off), vx + G.off, vy + G.off, vz + G.off, b, t, stage);
+        store_tile<R>(lds, tv);
+        __syncthreads();
+        advect_point<R>(out + G.off, lds, q, dt[g], t, inv_of(G), stage);
         __syncthreads();
     }
 }

@@ -160,10 +160,14 @@ __global__ void __launch_bounds__(256) k_reinit_bulk(T* __restrict__ out, const 
     __shared__ T lds[Shape<R>::N];
     if (blockIdx.x == 0 && threadIdx.x == 0) over_count[(seq + 1) & 1] = 0;
     const Tile t = tile_of<R>(b);
-    fill_tile<R>(lds, fs, b, t);
-    const idx_t ind = t.valid ? lin(b, t.r, t.c, t.p) : 0;
-    const bool defer = reinit_bulk_point<R>(out, lds, fs, fn, flags, b, t, ind, invs, stage);
-    wave_append(defer, (int)ind, over, &over_count[seq & 1]);
+    TileRegs<R, T> g;
+    load_tile<R>(g, fs, b, t);
+    BulkIn<T, false> q;
+    bulk_load(q, fs, fn, (const T*)nullptr, (const T*)nullptr, (const T*)nullptr, b, t, stage);
+    store_tile<R>(lds, g);
+    __syncthreads();
+    const bool defer = reinit_bulk_point<R>(out, lds, q, flags, t, invs, stage);
+    wave_append(defer, (int)(t.page + t.in), over, &over_count[seq & 1]);
 }
 
 
@@ -411,8 +415,13 @@ __global__ void __launch_bounds__(256) k_extend_bulk(T* __restrict__ out, const 
 {
     __shared__ T lds[Shape<R>::N];
     const Tile t = tile_of<R>(b);
-    fill_tile<R>(lds, cs, b, t);
-    extend_bulk_point<R>(out, lds, cs, cn, vx, vy, vz, flags, b, t, inv, stage);
+    TileRegs<R, T> g;
+    load_tile<R>(g, cs, b, t);
+    BulkIn<T, true> q;
+    bulk_load(q, cs, cn, vx, vy, vz, b, t, stage);
+    store_tile<R>(lds, g);
+    __syncthreads();
+    extend_bulk_point<R>(out, lds, q, flags, t, inv, stage);
 }
 
 // fused WENO5-RK3 extension stage, interface list: three lanes per point
@@ -476,8 +485,13 @@ __global__ void __launch_bounds__(256) k_advect_stage(T* __restrict__ out, const
 {
     __shared__ T lds[Shape<R>::N];
     const Tile t = tile_of<R>(b);
-    fill_tile<R>(lds, fs, b, t);
-    advect_point<R>(out, lds, fs, fn, vx, vy, vz, dt, b, t, inv, stage);
+    TileRegs<R, T> g;
+    load_tile<R>(g, fs, b, t);
+    BulkIn<T, true> q;
+    bulk_load(q, fs, fn, vx, vy, vz, b, t, stage);
+    store_tile<R>(lds, g);
+    __syncthreads();
+    advect_point<R>(out, lds, q, dt, t, inv, stage);
 }
 
 __global__ void __launch_bounds__(256) k_weno_derivative(Dist6 out, const double* __restrict__ lsf, Box b)

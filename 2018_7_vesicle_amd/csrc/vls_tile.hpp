@@ -163,7 +163,24 @@ struct Tile {
     int r, c;       // this thread's point
     bool valid;
     int base;       // LDS index of this thread's point
+    idx_t page;     // element offset of page p in the box (uniform)
+    unsigned in;    // element offset of the point in its page, c * rows + r
 };
+
+// element `off` of a page: the uniform 64-bit page base plus a 32-bit byte
+// offset per lane, so a load or store is one instruction with a scalar base
+// (offsets reach 7 pages of doubles: pages hold fewer than 2^26 points,
+// vls_sdf3_create and vls_batch_create refuse larger ones)
+template <typename T>
+__device__ __forceinline__ T ldp(const T* __restrict__ base, unsigned off)
+{
+    return *(const T*)((const char*)base + off * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void stp(T* __restrict__ base, unsigned off, T v)
+{
+    *(T*)((char*)base + off * (unsigned)sizeof(T)) = v;
+}
 
 // Blocks are dispatched round-robin over the 8 XCDs (block i to XCD i % 8),
 // each with its own L2.  Give every XCD a contiguous range of tiles instead,
@@ -203,6 +220,8 @@ __device__ __forceinline__ Tile tile_at(const Box& b, int blk)
     t.c = t.c0 + tx;
     t.valid = t.r < b.rows && t.c < b.cols;
     t.base = (tx + HALO) * S::LR + ty + HALO;
+    t.page = (idx_t)t.p * b.rows * b.cols;
+    t.in = (unsigned)t.c * (unsigned)b.rows + (unsigned)t.r;
     return t;
 }
 
@@ -213,17 +232,68 @@ __device__ __forceinline__ Tile tile_of(const Box& b)
     return tile_at<R>(b, xcd_remap((int)blockIdx.x, (int)gridDim.x));
 }
 
-// stage the tile of page t.p of f (with clamped halo) in lds[Shape<R>::N],
-// column-major (LR rows per column); ends with a barrier
+// Staging a tile of page t.p of f with its clamped halo in lds[Shape<R>::N],
+// column-major (LR rows per column).  load_tile issues the global loads into
+// registers and store_tile writes them to LDS, so a kernel can issue every
+// other load of its points in between and wait for memory once.  Each thread
+// loads one row of the first TR staged rows in every TC-th staged column (the
+// column offsets are wave-uniform for R = 64), and threads below 6*LC one
+// element of the last 6 staged rows.
+template <int R, typename T>
+struct TileRegs {
+    static constexpr int NK = (Shape<R>::LC + Shape<R>::TC - 1) / Shape<R>::TC;
+    T m[NK];
+    T s;
+};
+
+template <int R>
+__device__ __forceinline__ int tile_tx()
+{
+    const int tx = (int)threadIdx.x / R;
+    return (R % 64 == 0) ? __builtin_amdgcn_readfirstlane(tx) : tx;
+}
+
+template <int R, typename T>
+__device__ __forceinline__ void load_tile(TileRegs<R, T>& g, const T* __restrict__ f, const Box& b, const Tile& t)
+{
+    using S = Shape<R>;
+    const T* fp = f + t.page;
+    const int ty = (int)threadIdx.x % R, tx = tile_tx<R>();
+    const unsigned row = (unsigned)clampi(t.r0 - HALO + ty, b.rows - 1);
+#pragma unroll
+    for (int k = 0; k < TileRegs<R, T>::NK; ++k) {
+        const int lc = tx + S::TC * k;
+        if (lc < S::LC) g.m[k] = ldp(fp, (unsigned)clampi(t.c0 - HALO + lc, b.cols - 1) * (unsigned)b.rows + row);
+    }
+    const int e = (int)threadIdx.x;
+    if (e < 2 * HALO * S::LC) {
+        const int lc = e / (2 * HALO), lr = S::TR + e % (2 * HALO);
+        g.s = ldp(fp, (unsigned)clampi(t.c0 - HALO + lc, b.cols - 1) * (unsigned)b.rows +
+                          (unsigned)clampi(t.r0 - HALO + lr, b.rows - 1));
+    }
+}
+
+template <int R, typename T>
+__device__ __forceinline__ void store_tile(T* lds, const TileRegs<R, T>& g)
+{
+    using S = Shape<R>;
+    const int ty = (int)threadIdx.x % R, tx = tile_tx<R>();
+#pragma unroll
+    for (int k = 0; k < TileRegs<R, T>::NK; ++k) {
+        const int lc = tx + S::TC * k;
+        if (lc < S::LC) lds[lc * S::LR + ty] = g.m[k];
+    }
+    const int e = (int)threadIdx.x;
+    if (e < 2 * HALO * S::LC) lds[(e / (2 * HALO)) * S::LR + S::TR + e % (2 * HALO)] = g.s;
+}
+
+// stage and wait (ends with a barrier)
 template <int R, typename T>
 __device__ __forceinline__ void fill_tile(T* lds, const T* __restrict__ f, const Box& b, const Tile& t)
 {
-    using S = Shape<R>;
-    const T* fp = f + (idx_t)t.p * b.rows * b.cols;
-    for (int e = (int)threadIdx.x; e < S::N; e += 256) {
-        const int lc = e / S::LR, lr = e - lc * S::LR;
-        lds[e] = fp[(idx_t)clampi(t.c0 + lc - HALO, b.cols - 1) * b.rows + clampi(t.r0 + lr - HALO, b.rows - 1)];
-    }
+    TileRegs<R, T> g;
+    load_tile<R>(g, f, b, t);
+    store_tile<R>(lds, g);
     __syncthreads();
 }
 
@@ -241,14 +311,25 @@ __device__ __forceinline__ void tile_line_x(const T* lds, int base, T v[7])
     for (int k = 0; k < 7; ++k) v[k] = lds[base + (k - 3) * Shape<R>::LR];
 }
 
-// z-taps: page offsets are the same for the whole block
+// z-taps of a valid point: pages p-3..p+3 (clamped), addressed from the
+// lowest one with 32-bit offsets; the centre v[3] is the staged tile's
+// (tile_line_z)
 template <typename T>
-__device__ __forceinline__ void tile_line_z(const T* __restrict__ f, const Box& b, const Tile& t, T v[7])
+__device__ __forceinline__ void load_zline(T v[7], const T* __restrict__ f, const Box& b, const Tile& t)
 {
-    const idx_t plane = (idx_t)b.rows * b.cols;
-    const idx_t inplane = (idx_t)t.c * b.rows + t.r;
+    const unsigned plane = (unsigned)b.rows * (unsigned)b.cols;
+    const int lo = clampi(t.p - 3, b.pges - 1);
+    const T* f0 = f + (idx_t)lo * plane;
 #pragma unroll
-    for (int k = 0; k < 7; ++k) v[k] = f[(idx_t)clampi(t.p + k - 3, b.pges - 1) * plane + inplane];
+    for (int k = 0; k < 7; ++k)
+        if (k != 3) v[k] = ldp(f0, t.in + (unsigned)(clampi(t.p + k - 3, b.pges - 1) - lo) * plane);
+}
+
+template <typename T>
+__device__ __forceinline__ void tile_line_z(const T* lds, int base, const T z[7], T v[7])
+{
+#pragma unroll
+    for (int k = 0; k < 7; ++k) v[k] = (k == 3) ? lds[base] : z[k];
 }
 
 // Interface work is spread over three lanes per point, one per axis (21
@@ -363,35 +444,78 @@ __device__ __forceinline__ void block_reduce_store(KSum (&k)[NQ], double* __rest
 // result bit for bit either way.
 // ---------------------------------------------------------------------------
 
+// Per-point inputs of the bulk stage kernels, loaded (bulk_load) before the
+// tile's LDS barrier so that a block waits for memory once: the z-taps of the
+// stage field, the velocity (extension, advection) and the RK stage's start
+// value.  The point's flags are loaded after the barrier by the bodies below,
+// which evaluate every thread and use the flags only for the Godunov branch
+// and the store: loads issued before a barrier cannot be made to wait on a
+// load after it.
+template <typename T, bool VEL>
+struct BulkIn {
+    T z[7];
+    T vel[VEL ? 3 : 1];
+    T fn;
+};
+
+template <typename T, bool VEL>
+__device__ __forceinline__ void bulk_load(BulkIn<T, VEL>& q, const T* __restrict__ fs, const T* __restrict__ fn,
+                                          const T* __restrict__ vx, const T* __restrict__ vy,
+                                          const T* __restrict__ vz, const Box& b, const Tile& t, int stage)
+{
+    q.fn = (T)0;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) q.z[k] = (T)0;
+#pragma unroll
+    for (int a = 0; a < (VEL ? 3 : 1); ++a) q.vel[a] = (T)0;
+    if (!t.valid) return;
+    if (VEL) {
+        q.vel[0] = ldp(vx + t.page, t.in);
+        q.vel[1] = ldp(vy + t.page, t.in);
+        q.vel[2] = ldp(vz + t.page, t.in);
+    }
+    // branch-free: at stage 0 (no start value) the point's own stage value
+    // is read and dropped
+    const T f0 = ldp((stage != 0 ? fn : fs) + t.page, t.in);
+    q.fn = (stage != 0) ? f0 : (T)0;
+    load_zline(q.z, fs, b, t);
+}
+
+// the point's flags; FLAG_SUBCELL (no bulk store) outside the box.  Without
+// branches (a thread outside the box reads its page's first byte), so the
+// load is waited for where the value is used, not here.
+__device__ __forceinline__ uint8_t point_flags(const uint8_t* __restrict__ flags, const Tile& t)
+{
+    const uint8_t f = ldp(flags + t.page, t.valid ? t.in : 0u);
+    return t.valid ? f : (uint8_t)FLAG_SUBCELL;
+}
+
 // WENO5-RK3 reinitialization stage, bulk point of a staged tile: writes out
 // and returns false, or returns true (defer to the interface pass) for a point
 // next to a sign change of the stage field.  Points with subcell distances and
 // points outside the box write nothing and return false.
 template <int R, typename T>
-__device__ __forceinline__ bool reinit_bulk_point(T* __restrict__ out, const T* lds, const T* __restrict__ fs,
-                                                  const T* __restrict__ fn, const uint8_t* __restrict__ flags,
-                                                  const Box& b, const Tile& t, idx_t ind, const Inv& invs, int stage)
+__device__ __forceinline__ bool reinit_bulk_point(T* __restrict__ out, const T* lds, const BulkIn<T, false>& q,
+                                                  const uint8_t* __restrict__ flags, const Tile& t, const Inv& invs,
+                                                  int stage)
 {
-    const uint8_t fl = t.valid ? flags[ind] : (uint8_t)FLAG_SUBCELL;
-    bool defer = false;
-    if (!(fl & FLAG_SUBCELL)) {
-        T fo[3], bk[3];
+    const uint8_t fl = point_flags(flags, t);
+    bool cross = false;
+    T fo[3], bk[3];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            T v[7];
-            if (a == 0) tile_line_x<R>(lds, t.base, v);
-            else if (a == 1) tile_line_y(lds, t.base, v);
-            else tile_line_z(fs, b, t, v);
-            defer = defer || (sign_change(v[2], v[3]) || sign_change(v[3], v[4]));
-            weno5_pair(fo[a], bk[a], v, (T)invs.i[a]);
-        }
-        if (!defer) {
-            const T step = godunov_step((fl & FLAG_MASK) != 0, fo[0], bk[0], fo[1], bk[1], fo[2], bk[2],
-                                        (T)invs.dt_bulk);
-            out[ind] = rk_combine(stage, lds[t.base], step, (stage == 0) ? (T)0 : fn[ind]);
-        }
+    for (int a = 0; a < 3; ++a) {
+        T v[7];
+        if (a == 0) tile_line_x<R>(lds, t.base, v);
+        else if (a == 1) tile_line_y(lds, t.base, v);
+        else tile_line_z(lds, t.base, q.z, v);
+        cross = cross || (sign_change(v[2], v[3]) || sign_change(v[3], v[4]));
+        weno5_pair(fo[a], bk[a], v, (T)invs.i[a]);
     }
-    return defer;
+    const T step = godunov_step((fl & FLAG_MASK) != 0, fo[0], bk[0], fo[1], bk[1], fo[2], bk[2], (T)invs.dt_bulk);
+    const T res = rk_combine(stage, lds[t.base], step, q.fn);
+    if (fl & FLAG_SUBCELL) return false;
+    if (!cross) stp(out + t.page, t.in, res);
+    return cross;
 }
 
 // one lane (point, axis) of the reinitialization interface pass
@@ -454,25 +578,21 @@ __device__ __forceinline__ void reinit_iface_finish(T* __restrict__ out, const T
 // WENO5-RK3 extension stage, bulk point of a staged tile (points with subcell
 // distances are left to the interface pass)
 template <int R, typename T>
-__device__ __forceinline__ void extend_bulk_point(T* __restrict__ out, const T* lds, const T* __restrict__ cs,
-                                                  const T* __restrict__ cn, const T* __restrict__ vx,
-                                                  const T* __restrict__ vy, const T* __restrict__ vz,
-                                                  const uint8_t* __restrict__ flags, const Box& b, const Tile& t,
-                                                  const Inv& inv, int stage)
+__device__ __forceinline__ void extend_bulk_point(T* __restrict__ out, const T* lds, const BulkIn<T, true>& q,
+                                                  const uint8_t* __restrict__ flags, const Tile& t, const Inv& inv,
+                                                  int stage)
 {
-    if (!t.valid) return;
-    const idx_t ind = lin(b, t.r, t.c, t.p);
-    if (flags[ind] & FLAG_SUBCELL) return;
-    const T vel[3] = {vx[ind], vy[ind], vz[ind]};
+    const uint8_t fl = point_flags(flags, t);
     T acc = 0;
     T v[7];
     tile_line_x<R>(lds, t.base, v);
-    acc = upwind_axis(acc, vel[0], v, (T)inv.i[0]);
+    acc = upwind_axis(acc, q.vel[0], v, (T)inv.i[0]);
     tile_line_y(lds, t.base, v);
-    acc = upwind_axis(acc, vel[1], v, (T)inv.i[1]);
-    tile_line_z(cs, b, t, v);
-    acc = upwind_axis(acc, vel[2], v, (T)inv.i[2]);
-    out[ind] = rk_combine(stage, lds[t.base], acc * (T)inv.dt_bulk, (stage == 0) ? (T)0 : cn[ind]);
+    acc = upwind_axis(acc, q.vel[1], v, (T)inv.i[1]);
+    tile_line_z(lds, t.base, q.z, v);
+    acc = upwind_axis(acc, q.vel[2], v, (T)inv.i[2]);
+    const T res = rk_combine(stage, lds[t.base], acc * (T)inv.dt_bulk, q.fn);
+    if (!(fl & FLAG_SUBCELL)) stp(out + t.page, t.in, res);
 }
 
 // one lane (point, axis) of the extension interface pass
@@ -529,25 +649,20 @@ __device__ __forceinline__ void extend_iface_finish(T* __restrict__ out, const T
 
 // TVD-RK3 advection stage, point of a staged tile
 template <int R, typename T>
-__device__ __forceinline__ void advect_point(T* __restrict__ out, const T* lds, const T* __restrict__ fs,
-                                             const T* __restrict__ fn, const T* __restrict__ vx,
-                                             const T* __restrict__ vy, const T* __restrict__ vz, double dt,
-                                             const Box& b, const Tile& t, const Inv& inv, int stage)
+__device__ __forceinline__ void advect_point(T* __restrict__ out, const T* lds, const BulkIn<T, true>& q,
+                                             double dt, const Tile& t, const Inv& inv, int stage)
 {
-    if (!t.valid) return;
-    const idx_t ind = lin(b, t.r, t.c, t.p);
-    const T vel[3] = {vx[ind], vy[ind], vz[ind]};
     T acc = 0;
     T v[7];
     tile_line_x<R>(lds, t.base, v);
-    acc = upwind_axis(acc, vel[0], v, (T)inv.i[0]);
+    acc = upwind_axis(acc, q.vel[0], v, (T)inv.i[0]);
     tile_line_y(lds, t.base, v);
-    acc = upwind_axis(acc, vel[1], v, (T)inv.i[1]);
-    tile_line_z(fs, b, t, v);
-    acc = upwind_axis(acc, vel[2], v, (T)inv.i[2]);
-    const T fnv = (stage == 0) ? (T)0 : fn[ind];
+    acc = upwind_axis(acc, q.vel[1], v, (T)inv.i[1]);
+    tile_line_z(lds, t.base, q.z, v);
+    acc = upwind_axis(acc, q.vel[2], v, (T)inv.i[2]);
     const T step = (T)dt * acc;
-    out[ind] = rk_combine(stage, lds[t.base], step, fnv);
+    const T res = rk_combine(stage, lds[t.base], step, q.fn);
+    if (t.valid) stp(out + t.page, t.in, res);
 }
 
 }  // namespace vls

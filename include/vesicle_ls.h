@@ -114,7 +114,8 @@ typedef struct vls_sdf3 *vls_sdf3_t;
 /* SD.SDF3(grid, x, y, z, F) with grid = SD.GD3 (+SD/@GD3/GD3.m:86-105,
  * +SD/@SDF3/SDF3.m:11-15): allocates the object's device state for a
  * rows x cols x pges box with spacings dx, dy, dz.  No limit on rows (the
- * reference's launch shape needs rows <= 256, GPUInitialize.m:7). */
+ * reference's launch shape needs rows <= 256, GPUInitialize.m:7); a page
+ * (rows x cols) holds fewer than 2^26 points. */
 int vls_sdf3_create(vls_sdf3_t *out, int rows, int cols, int pges, double dx, double dy, double dz);
 int vls_sdf3_destroy(vls_sdf3_t h);
 /* tile shape of the fused bulk stage kernels: 64 (64 x 4 points of a page,
