@@ -64,6 +64,8 @@ SIGNATURES = {
     "vls_batch_set_tile_rows": [P, I],
     "vls_batch_get_tile_rows": [P, P],
     "vls_sdf3_set_tile_rows": [P, I],
+    "vls_sdf3_set_zmarch_depth": [P, I],
+    "vls_sdf3_get_launch": [P, P, P],
     "vls_batch_autotune": [P, P, I, P, P, P],
     "vls_batch_autotune_f32": [P, P, I, P, P, P],
     "vls_sdf3_autotune": [P, P, I, P, P],

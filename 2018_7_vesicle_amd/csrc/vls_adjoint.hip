@@ -449,7 +449,9 @@ static inline Box abox(const Geom& g) { return Box{g.rows, g.cols, g.pges, g.dx,
 static inline Inv ainv(const Geom& g)
 {
     const double m = std::min(std::min(g.dx, g.dx), std::min(std::min(g.dy, g.dy), std::min(g.dz, g.dz)));
-    return Inv{{1.0 / g.dx, 1.0 / g.dy, 1.0 / g.dz}, 0.3 * m};
+    Inv v{{1.0 / g.dx, 1.0 / g.dy, 1.0 / g.dz}, 0.3 * m, {}};
+    set_jp(v.jf, g.dx, g.dy, g.dz);
+    return v;
 }
 static inline IList alist(const ListView& lv) { return IList{lv.idx, lv.count, lv.dist, lv.dt, lv.cap}; }
 

@@ -51,6 +51,18 @@ int check_geom(int rows, int cols, int pges, double dx, double dy, double dz)
     return VLS_OK;
 }
 
+// compute units of the current device (256 on MI355X)
+int cu_count()
+{
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
+            cus = prop.multiProcessorCount;
+    }
+    return cus;
+}
+
 int check_num(int num_ele, int rows, int cols, int pges)
 {
     if ((int64_t)num_ele != (int64_t)rows * cols * pges)
@@ -745,6 +757,23 @@ int vls_sdf3_set_tile_rows(vls_sdf3_t h, int tile_rows)
     return VLS_OK;
 }
 
+int vls_sdf3_set_zmarch_depth(vls_sdf3_t h, int depth)
+{
+    VLS_REQUIRE(h, "null handle");
+    VLS_REQUIRE(depth >= 0, "depth must be 0 (by the box size) or positive");
+    h->g.zd = depth;
+    h->graphs.clear();
+    return VLS_OK;
+}
+
+int vls_sdf3_get_launch(vls_sdf3_t h, int* tile_rows, int* depth)
+{
+    VLS_REQUIRE(h, "null handle");
+    if (tile_rows) *tile_rows = h->g.tr;
+    if (depth) *depth = vls::zmarch_depth(h->g);
+    return VLS_OK;
+}
+
 int vls_sdf3_destroy(vls_sdf3_t h)
 {
     if (!h) return VLS_OK;
@@ -951,13 +980,7 @@ namespace {
 // to a multiple of 8)
 int default_blocks(int ntiles)
 {
-    int dev = 0, cus = 256;
-    if (hipGetDevice(&dev) == hipSuccess) {
-        hipDeviceProp_t prop;
-        if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
-            cus = prop.multiProcessorCount;
-    }
-    int b = 8 * cus;
+    int b = 8 * cu_count();
     const int cap = (ntiles + 7) / 8 * 8;
     if (b > cap) b = cap;
     return b < 8 ? 8 : b;
@@ -1112,6 +1135,7 @@ int vls_batch_create(vls_batch_t* out, int ngrids, const int* rows, const int* c
         G.inv[2] = 1.0 / dz[g];
         const double m = std::min(std::min(dx[g], dx[g]), std::min(std::min(dy[g], dy[g]), std::min(dz[g], dz[g])));
         G.dt_bulk = 0.3 * m;
+        set_jp(G.jf, dx[g], dy[g], dz[g]);
         b->grids.push_back(G);
         b->offs[(size_t)g] = off;
         off += (int64_t)rows[g] * cols[g] * pges[g];
@@ -1329,12 +1353,7 @@ struct EventTimer {
 template <typename T>
 int batch_autotune(vls_batch* b, const T* F, int iterations, hipStream_t s, int* tr_out, int* blocks_out)
 {
-    int cus = 256, dev = 0;
-    if (hipGetDevice(&dev) == hipSuccess) {
-        hipDeviceProp_t prop;
-        if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
-            cus = prop.multiProcessorCount;
-    }
+    const int cus = cu_count();
     T* C = nullptr;
     if (hipMalloc((void**)&C, sizeof(T) * (size_t)b->total) != hipSuccess)
         return fail(VLS_ERR_ALLOC, "hipMalloc of the autotuning field failed");
@@ -1391,30 +1410,39 @@ int sdf3_autotune(vls_sdf3* h, const T* F, int iterations, hipStream_t s, int* t
     const DSet ms = main_set(h);
     T *vx = vel<T>(h, 0), *vy = vel<T>(h, 1), *vz = vel<T>(h, 2);
     float best = 1e30f;
-    int best_tr = h->g.tr;
+    int best_tr = h->g.tr, best_zd = h->g.zd;
     const int trs[2] = {64, 32};
     int rc;
     for (int i = 0; i < 2; ++i) {
         h->g.tr = trs[i];
-        const Geom& g = h->g;
-        VLS_HIP(hipMemcpyAsync(C, F, sizeof(T) * (size_t)g.n(), hipMemcpyDeviceToDevice, s));
-        if ((rc = set_distance_into<T>(h, 6, ms, h->min_dist, F, (const T*)nullptr, s))) return rc;
-        VLS_HIP(launch_upwind_normal<T>(vx, vy, vz, F, constify(h->d), g, s));
-        VLS_HIP(launch_extension_velocity<T>(vx, vy, vz, F, 1e-6, g, s));
-        VLS_HIP(launch_list_gather_vel<T>(h->lvel, ms.lv, vx, vy, vz, g, s));
-        if ((rc = extend_loop<T>(h, C, 1, ms, s))) return rc;
-        VLS_HIP(hipEventRecord(ev.a, s));
-        if ((rc = extend_loop<T>(h, C, iterations, ms, s))) return rc;
-        VLS_HIP(hipEventRecord(ev.b, s));
-        VLS_HIP(hipEventSynchronize(ev.b));
-        float ms_ = 0;
-        VLS_HIP(hipEventElapsedTime(&ms_, ev.a, ev.b));
-        if (ms_ < best) {
-            best = ms_;
-            best_tr = trs[i];
+        h->g.zd = 0;
+        const int d0 = zmarch_depth(h->g);
+        const int depths[3] = {d0, d0 > 1 ? d0 / 2 : 0, 2 * d0 <= h->g.pges ? 2 * d0 : 0};
+        for (int j = 0; j < 3; ++j) {
+            if (depths[j] <= 0) continue;
+            h->g.zd = depths[j];
+            const Geom& g = h->g;
+            VLS_HIP(hipMemcpyAsync(C, F, sizeof(T) * (size_t)g.n(), hipMemcpyDeviceToDevice, s));
+            if ((rc = set_distance_into<T>(h, 6, ms, h->min_dist, F, (const T*)nullptr, s))) return rc;
+            VLS_HIP(launch_upwind_normal<T>(vx, vy, vz, F, constify(h->d), g, s));
+            VLS_HIP(launch_extension_velocity<T>(vx, vy, vz, F, 1e-6, g, s));
+            VLS_HIP(launch_list_gather_vel<T>(h->lvel, ms.lv, vx, vy, vz, g, s));
+            if ((rc = extend_loop<T>(h, C, 1, ms, s))) return rc;
+            VLS_HIP(hipEventRecord(ev.a, s));
+            if ((rc = extend_loop<T>(h, C, iterations, ms, s))) return rc;
+            VLS_HIP(hipEventRecord(ev.b, s));
+            VLS_HIP(hipEventSynchronize(ev.b));
+            float ms_ = 0;
+            VLS_HIP(hipEventElapsedTime(&ms_, ev.a, ev.b));
+            if (ms_ < best) {
+                best = ms_;
+                best_tr = trs[i];
+                best_zd = depths[j];
+            }
         }
     }
     h->g.tr = best_tr;
+    h->g.zd = best_zd;
     h->graphs.clear();
     if (tr_out) *tr_out = best_tr;
     return VLS_OK;

@@ -25,32 +25,20 @@ __device__ __forceinline__ Inv inv_of(const BGrid& G)
     v.i[1] = G.inv[1];
     v.i[2] = G.inv[2];
     v.dt_bulk = G.dt_bulk;
+    for (int a = 0; a < 3; ++a) v.jf[a] = G.jf[a];
     return v;
 }
 
-// the tiles a block of a persistent launch visits: XCD x (= blockIdx.x % 8,
-// the hardware's round-robin) takes the contiguous range
-// [x*chunk, (x+1)*chunk) and its blocks stride through it
-struct TileRange {
-    int first, end, step;
-};
-
-__device__ __forceinline__ TileRange tile_range(int ntiles)
+// the tile-th tile of a batch: its grid's box, spacings and offset
+template <int R, typename T, bool VEL>
+__device__ __forceinline__ void batch_tile(Staged<R, T, VEL>& st, const BatchView& B, int tile)
 {
-    TileRange r;
-    const int nb = (int)gridDim.x;
-    if ((nb & 7) == 0) {
-        const int x = (int)(blockIdx.x & 7), j = (int)(blockIdx.x >> 3), per = nb >> 3;
-        const int chunk = (ntiles + 7) >> 3;
-        r.first = x * chunk + j;
-        r.end = min(ntiles, (x + 1) * chunk);
-        r.step = per;
-    } else {
-        r.first = (int)blockIdx.x;
-        r.end = ntiles;
-        r.step = nb;
-    }
-    return r;
+    st.g = B.tile_grid[tile];
+    const BGrid G = B.grids[st.g];
+    st.b = box_of(G);
+    st.t = tile_at<R>(st.b, tile - G.tile0);
+    st.inv = inv_of(G);
+    st.off = G.off;
 }
 
 // grid of a packed element index: offs[g] <= gind < offs[g+1]
@@ -156,24 +144,17 @@ __global__ void __launch_bounds__(256) kb_reinit_bulk(T* __restrict__ out, const
                                                       BatchView B, int stage, int* __restrict__ over,
                                                       int* __restrict__ over_count, int seq)
 {
-    __shared__ T lds[Shape<R>::N];
+    __shared__ T lds[2][Shape<R>::N];
     if (blockIdx.x == 0 && threadIdx.x == 0) over_count[(seq + 1) & 1] = 0;
+    using St = Staged<R, T, false>;
     const TileRange tr = tile_range(B.ntiles);
-    for (int tile = tr.first; tile < tr.end; tile += tr.step) {
-        const BGrid G = B.grids[B.tile_grid[tile]];
-        const Box b = box_of(G);
-        const Tile t = tile_at<R>(b, tile - G.tile0);
-        TileRegs<R, T> g;
-        load_tile<R>(g, fs + G.off, b, t);
-        BulkIn<T, false> q;
-        bulk_load(q, fs + G.off, shift(fn, G.off), (const T*)nullptr, (const T*)nullptr, (const T*)nullptr, b, t,
-                  stage);
-        store_tile<R>(lds, g);
-        __syncthreads();
-        const bool defer = reinit_bulk_point<R>(out + G.off, lds, q, flags + G.off, t, inv_of(G), stage);
-        wave_append(defer, (int)(G.off + t.page + t.in), over, &over_count[seq & 1]);
-        __syncthreads();
-    }
+    tile_pipeline<R, T, false>(
+        lds[0], lds[1], tr.first, tr.end, tr.step, fs, fn, nullptr, nullptr, nullptr, flags, stage,
+        [&](int tile, St& st) { batch_tile<R>(st, B, tile); },
+        [&](const T* l, const St& st) {
+            const bool defer = reinit_bulk_point<R>(out + st.off, l, st.q, st.t, st.inv, stage);
+            wave_append(defer, (int)(st.off + st.t.page + st.t.in), over, &over_count[seq & 1]);
+        });
 }
 
 template <typename T>
@@ -206,21 +187,13 @@ __global__ void __launch_bounds__(256) kb_extend_bulk(T* __restrict__ out, const
                                                       const T* __restrict__ vy, const T* __restrict__ vz,
                                                       const uint8_t* __restrict__ flags, BatchView B, int stage)
 {
-    __shared__ T lds[Shape<R>::N];
+    __shared__ T lds[2][Shape<R>::N];
+    using St = Staged<R, T, true>;
     const TileRange tr = tile_range(B.ntiles);
-    for (int tile = tr.first; tile < tr.end; tile += tr.step) {
-        const BGrid G = B.grids[B.tile_grid[tile]];
-        const Box b = box_of(G);
-        const Tile t = tile_at<R>(b, tile - G.tile0);
-        TileRegs<R, T> g;
-        load_tile<R>(g, cs + G.off, b, t);
-        BulkIn<T, true> q;
-        bulk_load(q, cs + G.off, shift(cn, G.off), vx + G.off, vy + G.off, vz + G.off, b, t, stage);
-        store_tile<R>(lds, g);
-        __syncthreads();
-        extend_bulk_point<R>(out + G.off, lds, q, flags + G.off, t, inv_of(G), stage);
-        __syncthreads();
-    }
+    tile_pipeline<R, T, true>(
+        lds[0], lds[1], tr.first, tr.end, tr.step, cs, cn, vx, vy, vz, flags, stage,
+        [&](int tile, St& st) { batch_tile<R>(st, B, tile); },
+        [&](const T* l, const St& st) { extend_bulk_point<R>(out + st.off, l, st.q, st.t, st.inv, stage); });
 }
 
 template <typename T>
@@ -250,22 +223,13 @@ __global__ void __launch_bounds__(256) kb_advect_stage(T* __restrict__ out, cons
                                                        const T* __restrict__ vy, const T* __restrict__ vz,
                                                        const double* __restrict__ dt, BatchView B, int stage)
 {
-    __shared__ T lds[Shape<R>::N];
+    __shared__ T lds[2][Shape<R>::N];
+    using St = Staged<R, T, true>;
     const TileRange tr = tile_range(B.ntiles);
-    for (int tile = tr.first; tile < tr.end; tile += tr.step) {
-        const int g = B.tile_grid[tile];
-        const BGrid G = B.grids[g];
-        const Box b = box_of(G);
-        const Tile t = tile_at<R>(b, tile - G.tile0);
-        TileRegs<R, T> tv;
-        load_tile<R>(tv, fs + G.off, b, t);
-        BulkIn<T, true> q;
-        bulk_load(q, fs + G.off, shift(fn, G.off), vx + G.off, vy + G.off, vz + G.off, b, t, stage);
-        store_tile<R>(lds, tv);
-        __syncthreads();
-        advect_point<R>(out + G.off, lds, q, dt[g], t, inv_of(G), stage);
-        __syncthreads();
-    }
+    tile_pipeline<R, T, true>(
+        lds[0], lds[1], tr.first, tr.end, tr.step, fs, fn, vx, vy, vz, nullptr, stage,
+        [&](int tile, St& st) { batch_tile<R>(st, B, tile); },
+        [&](const T* l, const St& st) { advect_point<R>(out + st.off, l, st.q, dt[st.g], st.t, st.inv, stage); });
 }
 
 // ---------------------------------------------------------------------------

@@ -11,6 +11,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "vls_kernels.hpp"
+
 namespace vls {
 
 typedef int64_t idx_t;
@@ -120,55 +122,83 @@ __device__ __forceinline__ void weno5_pair(S& d_fore, S& d_back, const S p[7], d
 }
 
 // ---------------------------------------------------------------------------
-// FP32 bulk arithmetic: the same WENO5 and upwinding in single precision for
-// the points away from the interface.  The final division of the weights is a
-// hardware reciprocal (v_rcp_f32, 1 ulp) times the numerator.  The products
-// a_j a_k ~ S^4 would overflow FP32 for S > ~4e9, so each S_k is clamped at
-// 2^32 first: a stencil that rough has a relative weight below 1e-19 next to
-// any smooth one, and the clamp only matters when all three stencils exceed
-// it (divided differences above ~1e4 throughout), where the weights fall
-// back to the linear ones.
+// FP32 bulk arithmetic, for the points away from the interface: the same
+// WENO5 derivatives in the Jiang-Peng form (G.-S. Jiang, D. Peng, SIAM J. Sci.
+// Comput. 21 (2000), eq. 2.5-2.8), algebraically identical to the
+// stencil-weight form of re_step.cu:11-37 and cheaper to evaluate.  With
+// nodal values p[0..6] (p[3] the point), undivided second differences
+// D_j = p_j - 2 p_{j+1} + p_{j+2} and the fourth-order central part
+// C = p_1 - p_5 + 8 (p_4 - p_2):
+//   D-  = (C - Q(D0, D1, D2, D3)) / (12 ds),   D+ = (C + Q(D4, D3, D2, D1)) / (12 ds),
+//   Q(a,b,c,d) = [4 w0 (a - 2b + c) + (2 w2 - W)(b - 2c + d)] / W,
+//   w0 = s1^2 s2^2, w1 = 6 s0^2 s2^2, w2 = 3 s0^2 s1^2, W = w0 + w1 + w2,
+//   s0 = (a-b)^2 + 3/13 (a-3b)^2 + e, s1 = (b-c)^2 + 3/13 (b+c)^2 + e,
+//   s2 = (c-d)^2 + 3/13 (3c-d)^2 + e,  e = 12/13 * 1e-6 * ds^2,
+// where s_k = 12 ds^2/13 * (S_k + 1e-6) of the reference's smoothness
+// indicators S_k of divided differences, so the weights are the reference's.
+// The weights are homogeneous of degree 0 in the s_k: they are scaled by a
+// power of two that puts the largest in [0.5, 1), which keeps every product
+// in FP32 range for any data (the products only underflow, harmlessly, for
+// indicator ratios beyond 1e19, where W is floored at 1e-30 and Q -> 0).
+// The final division is a hardware reciprocal (v_rcp_f32, 1 ulp).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float fmaxf_(float a, float b) { return (a < b) ? b : a; }
 __device__ __forceinline__ float fminf_(float a, float b) { return (a < b) ? a : b; }
 
-__device__ __forceinline__ float weno5_onesided(float v1, float v2, float v3, float v4, float v5)
+// the correction Q(a, b, c, d) of one side
+__device__ __forceinline__ float jp_q(float a, float b, float c, float d, float e)
 {
-    const float c13 = 1.f / 3.f, c76 = 7.f / 6.f, c116 = 11.f / 6.f, c16 = 1.f / 6.f, c56 = 5.f / 6.f,
-                c1312 = 13.f / 12.f;
-    const float phi1 = fmaf(c116, v3, fmaf(-c76, v2, c13 * v1));
-    const float phi2 = fmaf(c13, v4, fmaf(c56, v3, -c16 * v2));
-    const float phi3 = fmaf(-c16, v5, fmaf(c56, v4, c13 * v3));
-    const float d1 = fmaf(-2.f, v2, v1) + v3, e1 = fmaf(3.f, v3, fmaf(-4.f, v2, v1));
-    const float d2 = fmaf(-2.f, v3, v2) + v4, e2 = v2 - v4;
-    const float d3 = fmaf(-2.f, v4, v3) + v5, e3 = fmaf(3.f, v3, fmaf(-4.f, v4, v5));
-    const float smax = 4294967296.f;
-    const float s1 = fminf_(fmaf(c1312 * d1, d1, 0.25f * e1 * e1), smax) + 1e-6f;
-    const float s2 = fminf_(fmaf(c1312 * d2, d2, 0.25f * e2 * e2), smax) + 1e-6f;
-    const float s3 = fminf_(fmaf(c1312 * d3, d3, 0.25f * e3 * e3), smax) + 1e-6f;
-    const float a1 = s1 * s1, a2 = s2 * s2, a3 = s3 * s3;
-    const float w1 = 0.1f * (a2 * a3), w2 = 0.6f * (a1 * a3), w3 = 0.3f * (a1 * a2);
-    return fmaf(w1, phi1, fmaf(w2, phi2, w3 * phi3)) * __builtin_amdgcn_rcpf(w1 + w2 + w3);
+    const float t0 = a - b, t1 = b - c, t2 = c - d;
+    const float y0 = fmaf(-3.f, b, a), y1 = b + c, y2 = fmaf(3.f, c, -d);
+    const float k = 3.f / 13.f;
+    float s0 = fmaf(t0, t0, fmaf(k * y0, y0, e));
+    float s1 = fmaf(t1, t1, fmaf(k * y1, y1, e));
+    float s2 = fmaf(t2, t2, fmaf(k * y2, y2, e));
+    const int ex = __builtin_amdgcn_frexp_expf(__builtin_fmaxf(s0, __builtin_fmaxf(s1, s2)));
+    s0 = __builtin_amdgcn_ldexpf(s0, -ex);
+    s1 = __builtin_amdgcn_ldexpf(s1, -ex);
+    s2 = __builtin_amdgcn_ldexpf(s2, -ex);
+    const float a0 = s0 * s0, a1 = s1 * s1, a2 = s2 * s2;
+    const float w0 = a1 * a2, w1 = 6.f * (a0 * a2), w2 = 3.f * (a0 * a1);
+    const float W = (w0 + w1) + w2;
+    const float num = fmaf(4.f * w0, t0 - t1, fmaf(2.f, w2, -W) * (t1 - t2));
+    return num * __builtin_amdgcn_rcpf(__builtin_fmaxf(W, 1e-30f));
 }
 
-__device__ __forceinline__ float upwind_axis(float acc, float vel, const float p[7], float inv_ds)
+// second differences D1, D2, D3 and the central part C of p[0..6]
+struct JPc {
+    float d1, d2, d3, c;
+};
+__device__ __forceinline__ JPc jp_centre(const float p[7])
 {
-    float v[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) v[k] = (p[k + 1] - p[k]) * inv_ds;
+    JPc r;
+    r.d1 = fmaf(-2.f, p[2], p[1]) + p[3];
+    r.d2 = fmaf(-2.f, p[3], p[2]) + p[4];
+    r.d3 = fmaf(-2.f, p[4], p[3]) + p[5];
+    r.c = fmaf(8.f, p[4] - p[2], p[1] - p[5]);
+    return r;
+}
+
+// Upwind term of one axis added to acc: vel * (vel < 0 ? D+ : D-), as
+// extend_step.cu:121-123 / advection_step.cu:51-53 (min(0,vel) D+ + max(0,vel) D-).
+// With kv = vel/(12 ds): vel * D-+ = kv C - |kv| Q on the upwind side.
+__device__ __forceinline__ float upwind_axis(float acc, float vel, const float p[7], JPf j)
+{
+    const JPc m = jp_centre(p);
     const bool fore = vel < 0;
-    const float d = weno5_onesided(fore ? v[5] : v[0], fore ? v[4] : v[1], fore ? v[3] : v[2],
-                                    fore ? v[2] : v[3], fore ? v[1] : v[4]);
-    return acc + (fore ? vel : fmaxf_(0.f, vel)) * d;
+    const float d0 = fmaf(-2.f, p[1], p[0]) + p[2], d4 = fmaf(-2.f, p[5], p[4]) + p[6];
+    const float q = jp_q(fore ? d4 : d0, fore ? m.d3 : m.d1, m.d2, fore ? m.d1 : m.d3, j.e);
+    const float kv = vel * j.k;
+    return fmaf(-__builtin_fabsf(kv), q, fmaf(kv, m.c, acc));
 }
 
-__device__ __forceinline__ void weno5_pair(float& d_fore, float& d_back, const float p[7], float inv_ds)
+// back/fore pair (re_step.cu:260-267)
+__device__ __forceinline__ void weno5_pair(float& d_fore, float& d_back, const float p[7], JPf j)
 {
-    float v[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) v[k] = (p[k + 1] - p[k]) * inv_ds;
-    d_back = weno5_onesided(v[0], v[1], v[2], v[3], v[4]);
-    d_fore = weno5_onesided(v[5], v[4], v[3], v[2], v[1]);
+    const JPc m = jp_centre(p);
+    const float d0 = fmaf(-2.f, p[1], p[0]) + p[2], d4 = fmaf(-2.f, p[5], p[4]) + p[6];
+    d_back = (m.c - jp_q(d0, m.d1, m.d2, m.d3, j.e)) * j.k;
+    d_fore = (m.c + jp_q(d4, m.d3, m.d2, m.d1, j.e)) * j.k;
 }
 
 // a*b < 0 without forming the product (an FP32 product of two tiny values

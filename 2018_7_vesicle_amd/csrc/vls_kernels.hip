@@ -155,19 +155,21 @@ template <int R, typename T>
 __global__ void __launch_bounds__(256) k_reinit_bulk(T* __restrict__ out, const T* __restrict__ fs,
                                                      const T* __restrict__ fn, const uint8_t* __restrict__ flags,
                                                      Box b, Inv invs, int stage, int* __restrict__ over,
-                                                     int* __restrict__ over_count, int seq)
+                                                     int* __restrict__ over_count, int seq, int depth)
 {
-    __shared__ T lds[Shape<R>::N];
+    __shared__ T lds[2][Shape<R>::N];
     if (blockIdx.x == 0 && threadIdx.x == 0) over_count[(seq + 1) & 1] = 0;
-    const Tile t = tile_of<R>(b);
-    TileRegs<R, T> g;
-    load_tile<R>(g, fs, b, t);
-    BulkIn<T, false> q;
-    bulk_load(q, fs, fn, (const T*)nullptr, (const T*)nullptr, (const T*)nullptr, b, t, stage);
-    store_tile<R>(lds, g);
-    __syncthreads();
-    const bool defer = reinit_bulk_point<R>(out, lds, q, flags, t, invs, stage);
-    wave_append(defer, (int)(t.page + t.in), over, &over_count[seq & 1]);
+    int r0, c0, p0, p1;
+    zmarch_share<R>(b, depth, r0, c0, p0, p1);
+    const idx_t plane = (idx_t)b.rows * b.cols;
+    zmarch<R, T, false>(lds[0], lds[1], b, r0, c0, p0, p1, fs, fn, nullptr, nullptr, nullptr, flags, stage,
+                        [&](const T* l, const ZCol<R>& zc, const T w[7], const ZStep<R, T, false>& st, int p) {
+                            bool cross;
+                            const T res = reinit_value<R>(l, zc.base, w, st.fn, st.fl, invs, stage, cross);
+                            const bool bulk = !(st.fl & FLAG_SUBCELL);
+                            if (bulk && !cross) stp(out + (idx_t)p * plane, zc.in, res);
+                            wave_append(bulk && cross, (int)((idx_t)p * plane + zc.in), over, &over_count[seq & 1]);
+                        });
 }
 
 
@@ -411,17 +413,18 @@ template <int R, typename T>
 __global__ void __launch_bounds__(256) k_extend_bulk(T* __restrict__ out, const T* __restrict__ cs,
                                                      const T* __restrict__ cn, const T* __restrict__ vx,
                                                      const T* __restrict__ vy, const T* __restrict__ vz,
-                                                     const uint8_t* __restrict__ flags, Box b, Inv inv, int stage)
+                                                     const uint8_t* __restrict__ flags, Box b, Inv inv, int stage,
+                                                     int depth)
 {
-    __shared__ T lds[Shape<R>::N];
-    const Tile t = tile_of<R>(b);
-    TileRegs<R, T> g;
-    load_tile<R>(g, cs, b, t);
-    BulkIn<T, true> q;
-    bulk_load(q, cs, cn, vx, vy, vz, b, t, stage);
-    store_tile<R>(lds, g);
-    __syncthreads();
-    extend_bulk_point<R>(out, lds, q, flags, t, inv, stage);
+    __shared__ T lds[2][Shape<R>::N];
+    int r0, c0, p0, p1;
+    zmarch_share<R>(b, depth, r0, c0, p0, p1);
+    const idx_t plane = (idx_t)b.rows * b.cols;
+    zmarch<R, T, true>(lds[0], lds[1], b, r0, c0, p0, p1, cs, cn, vx, vy, vz, flags, stage,
+                       [&](const T* l, const ZCol<R>& zc, const T w[7], const ZStep<R, T, true>& st, int p) {
+                           const T res = upwind_value<R>(l, zc.base, w, st.vel, st.fn, (T)inv.dt_bulk, inv, stage);
+                           if (!(st.fl & FLAG_SUBCELL)) stp(out + (idx_t)p * plane, zc.in, res);
+                       });
 }
 
 // fused WENO5-RK3 extension stage, interface list: three lanes per point
@@ -481,17 +484,17 @@ template <int R, typename T>
 __global__ void __launch_bounds__(256) k_advect_stage(T* __restrict__ out, const T* __restrict__ fs,
                                                       const T* __restrict__ fn, const T* __restrict__ vx,
                                                       const T* __restrict__ vy, const T* __restrict__ vz,
-                                                      double dt, Box b, Inv inv, int stage)
+                                                      double dt, Box b, Inv inv, int stage, int depth)
 {
-    __shared__ T lds[Shape<R>::N];
-    const Tile t = tile_of<R>(b);
-    TileRegs<R, T> g;
-    load_tile<R>(g, fs, b, t);
-    BulkIn<T, true> q;
-    bulk_load(q, fs, fn, vx, vy, vz, b, t, stage);
-    store_tile<R>(lds, g);
-    __syncthreads();
-    advect_point<R>(out, lds, q, dt, t, inv, stage);
+    __shared__ T lds[2][Shape<R>::N];
+    int r0, c0, p0, p1;
+    zmarch_share<R>(b, depth, r0, c0, p0, p1);
+    const idx_t plane = (idx_t)b.rows * b.cols;
+    zmarch<R, T, true>(lds[0], lds[1], b, r0, c0, p0, p1, fs, fn, vx, vy, vz, nullptr, stage,
+                       [&](const T* l, const ZCol<R>& zc, const T w[7], const ZStep<R, T, true>& st, int p) {
+                           const T res = upwind_value<R>(l, zc.base, w, st.vel, st.fn, (T)dt, inv, stage);
+                           if (zc.valid) stp(out + (idx_t)p * plane, zc.in, res);
+                       });
 }
 
 __global__ void __launch_bounds__(256) k_weno_derivative(Dist6 out, const double* __restrict__ lsf, Box b)
@@ -780,7 +783,9 @@ static inline Box box_of(const Geom& g) { return Box{g.rows, g.cols, g.pges, g.d
 static inline Inv inv_of(const Geom& g)
 {
     const double m = std::min(std::min(g.dx, g.dx), std::min(std::min(g.dy, g.dy), std::min(g.dz, g.dz)));
-    return Inv{{1.0 / g.dx, 1.0 / g.dy, 1.0 / g.dz}, 0.3 * m};
+    Inv v{{1.0 / g.dx, 1.0 / g.dy, 1.0 / g.dz}, 0.3 * m, {}};
+    set_jp(v.jf, g.dx, g.dy, g.dz);
+    return v;
 }
 
 // one thread per point: 64 rows x 4 cols per block, one z-page per grid.z
@@ -841,9 +846,23 @@ int iface_blocks(const Geom& g)
     return (int)(b < 16 ? 16 : (b > 4096 ? 4096 : b));
 }
 
-static inline int bulk_blocks(const Geom& g)
+// pages per block of the z-marching bulk stage launches: g.zd, or by
+// default the power of two (at most pges) that gives about 8 blocks per CU
+// of a 256-CU device
+int zmarch_depth(const Geom& g)
 {
-    return g.tr == 32 ? box_tiles<32>(g.rows, g.cols, g.pges) : box_tiles<64>(g.rows, g.cols, g.pges);
+    if (g.zd > 0) return g.zd < g.pges ? g.zd : g.pges;
+    const int64_t cols_ = g.tr == 32 ? (int64_t)tiles_per_col<32>(g.rows) * tiles_per_row<32>(g.cols)
+                                     : (int64_t)tiles_per_col<64>(g.rows) * tiles_per_row<64>(g.cols);
+    int d = 1;
+    while (2 * d <= g.pges && cols_ * g.pges / (2 * d) >= 2048) d *= 2;
+    return d;
+}
+
+static inline int bulk_blocks(const Geom& g, int depth)
+{
+    return g.tr == 32 ? zmarch_blocks<32>(g.rows, g.cols, g.pges, depth)
+                      : zmarch_blocks<64>(g.rows, g.cols, g.pges, depth);
 }
 
 template <typename T>
@@ -870,12 +889,13 @@ template <typename T>
 hipError_t launch_reinit_stage(T* out, const T* fs, const T* fn, const uint8_t* flags, const ListView& lv,
                                const Geom& g, int stage, int* over, int* over_count, int seq, hipStream_t s)
 {
+    const int zd = zmarch_depth(g);
     if (g.tr == 32)
-        k_reinit_bulk<32, T><<<dim3(bulk_blocks(g)), dim3(256), 0, s>>>(out, fs, fn, flags, box_of(g), inv_of(g),
-                                                                         stage, over, over_count, seq);
+        k_reinit_bulk<32, T><<<dim3(bulk_blocks(g, zd)), dim3(256), 0, s>>>(out, fs, fn, flags, box_of(g), inv_of(g),
+                                                                             stage, over, over_count, seq, zd);
     else
-        k_reinit_bulk<64, T><<<dim3(bulk_blocks(g)), dim3(256), 0, s>>>(out, fs, fn, flags, box_of(g), inv_of(g),
-                                                                         stage, over, over_count, seq);
+        k_reinit_bulk<64, T><<<dim3(bulk_blocks(g, zd)), dim3(256), 0, s>>>(out, fs, fn, flags, box_of(g), inv_of(g),
+                                                                             stage, over, over_count, seq, zd);
     const IList il{lv.idx, lv.count, lv.dist, lv.dt, lv.cap};
     k_reinit_iface<T><<<dim3(iface_blocks(g)), dim3(256), 0, s>>>(out, fs, fn, flags, il, box_of(g), inv_of(g),
                                                                    stage, (const int*)over, (const int*)over_count,
@@ -956,12 +976,13 @@ hipError_t launch_extend_stage(T* out, const T* cs, const T* cn, const T* vx, co
                                                                               inv_of(g), stage);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
+    const int zd = zmarch_depth(g);
     if (g.tr == 32)
-        k_extend_bulk<32, T><<<dim3(bulk_blocks(g)), dim3(256), 0, s>>>(out, cs, cn, vx, vy, vz, flags, box_of(g),
-                                                                         inv_of(g), stage);
+        k_extend_bulk<32, T><<<dim3(bulk_blocks(g, zd)), dim3(256), 0, s>>>(out, cs, cn, vx, vy, vz, flags,
+                                                                             box_of(g), inv_of(g), stage, zd);
     else
-        k_extend_bulk<64, T><<<dim3(bulk_blocks(g)), dim3(256), 0, s>>>(out, cs, cn, vx, vy, vz, flags, box_of(g),
-                                                                         inv_of(g), stage);
+        k_extend_bulk<64, T><<<dim3(bulk_blocks(g, zd)), dim3(256), 0, s>>>(out, cs, cn, vx, vy, vz, flags,
+                                                                             box_of(g), inv_of(g), stage, zd);
     if (fk && fk->side) {
         hipError_t e;
         if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -983,12 +1004,13 @@ template <typename T>
 hipError_t launch_advect_stage(T* out, const T* fs, const T* fn, const T* vx, const T* vy, const T* vz, double dt,
                                const Geom& g, int stage, hipStream_t s)
 {
+    const int zd = zmarch_depth(g);
     if (g.tr == 32)
-        k_advect_stage<32, T><<<dim3(bulk_blocks(g)), dim3(256), 0, s>>>(out, fs, fn, vx, vy, vz, dt, box_of(g),
-                                                                          inv_of(g), stage);
+        k_advect_stage<32, T><<<dim3(bulk_blocks(g, zd)), dim3(256), 0, s>>>(out, fs, fn, vx, vy, vz, dt, box_of(g),
+                                                                              inv_of(g), stage, zd);
     else
-        k_advect_stage<64, T><<<dim3(bulk_blocks(g)), dim3(256), 0, s>>>(out, fs, fn, vx, vy, vz, dt, box_of(g),
-                                                                          inv_of(g), stage);
+        k_advect_stage<64, T><<<dim3(bulk_blocks(g, zd)), dim3(256), 0, s>>>(out, fs, fn, vx, vy, vz, dt, box_of(g),
+                                                                              inv_of(g), stage, zd);
     return hipGetLastError();
 }
 

@@ -50,7 +50,8 @@ struct ListView {
 struct Geom {
     int rows, cols, pges;
     double dx, dy, dz;
-    int tr = 64;  // rows per tile of the bulk stage kernels: 64 or 32
+    int tr = 64;      // rows per tile of the bulk stage kernels: 64 or 32
+    int zd = 0;       // pages per block of the z-marching bulk stage kernels (0: by the box size)
     int64_t n() const { return (int64_t)rows * cols * pges; }
 };
 
@@ -68,6 +69,8 @@ hipError_t launch_min_dist_flags(double* min_dist, uint8_t* flags, const Dist6c&
                                  const Geom& g, int* list, int* list_count, hipStream_t s);
 hipError_t launch_zero_ints(int* a, int n, hipStream_t s);
 int iface_blocks(const Geom& g);
+// pages per block of the z-marching bulk stage kernels (g.zd, or chosen by the box size)
+int zmarch_depth(const Geom& g);
 template <typename T>
 hipError_t launch_set_mask_bit(uint8_t* flags, const T* F, const Geom& g, hipStream_t s);
 hipError_t launch_re_step(double* step, const double* lsf, const uint8_t* mask, const double* deltat,
@@ -143,6 +146,21 @@ hipError_t launch_copy(T* dst, const T* src, int64_t n, hipStream_t s);
 // field of a batch is the grids' column-major arrays back to back) and its
 // first tile in the batch's tile list (tiles of tr rows x 256/tr cols x 1
 // page, page-major).
+// per-axis constants of the FP32 bulk WENO5 form (vls_device.hpp):
+// k = 1/(12 ds), e = 12/13 * 1e-6 * ds^2
+struct JPf {
+    float k, e;
+};
+
+__host__ __device__ inline void set_jp(JPf jf[3], double dx, double dy, double dz)
+{
+    const double sp[3] = {dx, dy, dz};
+    for (int a = 0; a < 3; ++a) {
+        jf[a].k = (float)(1.0 / (12.0 * sp[a]));
+        jf[a].e = (float)(12.0 / 13.0 * 1e-6 * sp[a] * sp[a]);
+    }
+}
+
 struct BGrid {
     int rows, cols, pges;
     int ntiles;       // tiles of this grid
@@ -151,6 +169,7 @@ struct BGrid {
     double dx, dy, dz;
     double inv[3];    // 1/dx, 1/dy, 1/dz
     double dt_bulk;   // 0.3*min(dx, dy, dz)
+    JPf jf[3];        // the FP32 bulk form's per-axis constants (set_jp)
 };
 
 // device view of a batch

@@ -29,7 +29,13 @@ __device__ __forceinline__ idx_t lin(const Box& b, int r, int c, int p)
 struct Inv {
     double i[3];
     double dt_bulk;
+    JPf jf[3];  // the FP32 bulk form's per-axis constants
 };
+
+// the per-axis argument of the WENO5 helpers: 1/ds for FP64, the FP32 form's
+// constants for FP32
+__device__ __forceinline__ double axis_arg(const Inv& v, int a, double) { return v.i[a]; }
+__device__ __forceinline__ JPf axis_arg(const Inv& v, int a, float) { return v.jf[a]; }
 
 // RK combinations in the order the MATLAB wrappers evaluate them
 // (+SD/@SDF3/WENO5RK3Reinitialization.m:20-37), without fused multiply-adds.
@@ -223,6 +229,31 @@ __device__ __forceinline__ Tile tile_at(const Box& b, int blk)
     t.page = (idx_t)t.p * b.rows * b.cols;
     t.in = (unsigned)t.c * (unsigned)b.rows + (unsigned)t.r;
     return t;
+}
+
+// the tiles a block of a persistent launch visits: XCD x (= blockIdx.x % 8,
+// the hardware's round-robin) takes the contiguous range
+// [x*chunk, (x+1)*chunk) and its blocks stride through it
+struct TileRange {
+    int first, end, step;
+};
+
+__device__ __forceinline__ TileRange tile_range(int ntiles)
+{
+    TileRange r;
+    const int nb = (int)gridDim.x;
+    if ((nb & 7) == 0) {
+        const int x = (int)(blockIdx.x & 7), j = (int)(blockIdx.x >> 3), per = nb >> 3;
+        const int chunk = (ntiles + 7) >> 3;
+        r.first = x * chunk + j;
+        r.end = min(ntiles, (x + 1) * chunk);
+        r.step = per;
+    } else {
+        r.first = (int)blockIdx.x;
+        r.end = ntiles;
+        r.step = nb;
+    }
+    return r;
 }
 
 // this block's tile of a one-block-per-tile launch
@@ -444,76 +475,294 @@ __device__ __forceinline__ void block_reduce_store(KSum (&k)[NQ], double* __rest
 // result bit for bit either way.
 // ---------------------------------------------------------------------------
 
-// Per-point inputs of the bulk stage kernels, loaded (bulk_load) before the
-// tile's LDS barrier so that a block waits for memory once: the z-taps of the
-// stage field, the velocity (extension, advection) and the RK stage's start
-// value.  The point's flags are loaded after the barrier by the bodies below,
-// which evaluate every thread and use the flags only for the Godunov branch
-// and the store: loads issued before a barrier cannot be made to wait on a
-// load after it.
+// Per-point inputs of the bulk stage kernels: the z-taps of the stage field,
+// the velocity (extension, advection), the RK stage's start value and the
+// point's flags (FLAG_SUBCELL outside the box).  bulk_load issues the loads
+// without branches on loaded values; the bodies below evaluate every thread
+// and use the flags only for the Godunov branch and the store, so no load
+// waits for another.
 template <typename T, bool VEL>
 struct BulkIn {
     T z[7];
     T vel[VEL ? 3 : 1];
     T fn;
+    uint8_t fl;
 };
 
 template <typename T, bool VEL>
 __device__ __forceinline__ void bulk_load(BulkIn<T, VEL>& q, const T* __restrict__ fs, const T* __restrict__ fn,
                                           const T* __restrict__ vx, const T* __restrict__ vy,
-                                          const T* __restrict__ vz, const Box& b, const Tile& t, int stage)
+                                          const T* __restrict__ vz, const uint8_t* __restrict__ flags,
+                                          const Box& b, const Tile& t, int stage)
 {
-    q.fn = (T)0;
-#pragma unroll
-    for (int k = 0; k < 7; ++k) q.z[k] = (T)0;
-#pragma unroll
-    for (int a = 0; a < (VEL ? 3 : 1); ++a) q.vel[a] = (T)0;
-    if (!t.valid) return;
+    // a thread outside the box reads its page's first point and drops it
+    const unsigned in = t.valid ? t.in : 0u;
+    const uint8_t f = VEL && !flags ? (uint8_t)0 : ldp(flags + t.page, in);
+    q.fl = t.valid ? f : (uint8_t)FLAG_SUBCELL;
     if (VEL) {
-        q.vel[0] = ldp(vx + t.page, t.in);
-        q.vel[1] = ldp(vy + t.page, t.in);
-        q.vel[2] = ldp(vz + t.page, t.in);
+        q.vel[0] = ldp(vx + t.page, in);
+        q.vel[1] = ldp(vy + t.page, in);
+        q.vel[2] = ldp(vz + t.page, in);
     }
-    // branch-free: at stage 0 (no start value) the point's own stage value
-    // is read and dropped
-    const T f0 = ldp((stage != 0 ? fn : fs) + t.page, t.in);
+    // at stage 0 (no start value) the point's own stage value is read and dropped
+    const T f0 = ldp((stage != 0 ? fn : fs) + t.page, in);
     q.fn = (stage != 0) ? f0 : (T)0;
-    load_zline(q.z, fs, b, t);
+    Tile tz = t;
+    tz.in = in;
+    load_zline(q.z, fs, b, tz);
 }
 
-// the point's flags; FLAG_SUBCELL (no bulk store) outside the box.  Without
-// branches (a thread outside the box reads its page's first byte), so the
-// load is waited for where the value is used, not here.
-__device__ __forceinline__ uint8_t point_flags(const uint8_t* __restrict__ flags, const Tile& t)
+// One tile of a pipelined walk (tile_pipeline): where it lies and its loads.
+template <int R, typename T, bool VEL>
+struct Staged {
+    Tile t;
+    Box b;
+    Inv inv;
+    idx_t off;  // the tile's grid in packed fields (0 for a single grid)
+    int g;      // the tile's grid (0 for a single grid)
+    TileRegs<R, T> reg;
+    BulkIn<T, VEL> q;
+};
+
+// the tile-th tile of a single grid
+template <int R, typename T, bool VEL>
+__device__ __forceinline__ void single_grid_tile(Staged<R, T, VEL>& st, const Box& b, const Inv& inv, int tile)
 {
-    const uint8_t f = ldp(flags + t.page, t.valid ? t.in : 0u);
-    return t.valid ? f : (uint8_t)FLAG_SUBCELL;
+    st.t = tile_at<R>(b, tile);
+    st.b = b;
+    st.inv = inv;
+    st.off = 0;
+    st.g = 0;
 }
 
-// WENO5-RK3 reinitialization stage, bulk point of a staged tile: writes out
-// and returns false, or returns true (defer to the interface pass) for a point
-// next to a sign change of the stage field.  Points with subcell distances and
-// points outside the box write nothing and return false.
+// Software-pipelined walk of a block over its tiles first, first + step, ...
+// below end: the loads of the next tile are issued before the current tile is
+// evaluated, so memory latency overlaps arithmetic.  Staged tiles alternate
+// between two LDS buffers and two register sets (the loop is unrolled by two,
+// so no registers are copied), with one barrier per tile: a wave that stages
+// tile i+1 has passed tile i's barrier, which every wave reaches only after
+// it finished evaluating tile i-1 from the other buffer.  locate(tile, s)
+// fills s.t, s.b, s.inv, s.off and s.g; body(lds, s) evaluates a staged tile.
+template <int R, typename T, bool VEL, class Locate>
+__device__ __forceinline__ void stage_tile(Staged<R, T, VEL>& s, int tile, const T* __restrict__ fs,
+                                           const T* __restrict__ fn, const T* __restrict__ vx,
+                                           const T* __restrict__ vy, const T* __restrict__ vz,
+                                           const uint8_t* __restrict__ flags, int stage, Locate locate)
+{
+    locate(tile, s);
+    load_tile<R>(s.reg, fs + s.off, s.b, s.t);
+    bulk_load(s.q, fs + s.off, fn ? fn + s.off : fn, VEL ? vx + s.off : vx, VEL ? vy + s.off : vy,
+              VEL ? vz + s.off : vz, flags ? flags + s.off : flags, s.b, s.t, stage);
+}
+
+template <int R, typename T, bool VEL, class Locate, class Body>
+__device__ __forceinline__ void tile_pipeline(T* lds0, T* lds1, int first, int end, int step,
+                                              const T* __restrict__ fs, const T* __restrict__ fn,
+                                              const T* __restrict__ vx, const T* __restrict__ vy,
+                                              const T* __restrict__ vz, const uint8_t* __restrict__ flags,
+                                              int stage, Locate locate, Body body)
+{
+    if (first >= end) return;
+    Staged<R, T, VEL> sa, sb;
+    stage_tile(sa, first, fs, fn, vx, vy, vz, flags, stage, locate);
+    for (int tile = first;;) {
+        store_tile<R>(lds0, sa.reg);
+        __syncthreads();
+        tile += step;
+        if (tile < end) stage_tile(sb, tile, fs, fn, vx, vy, vz, flags, stage, locate);
+        body(lds0, sa);
+        if (tile >= end) break;
+        store_tile<R>(lds1, sb.reg);
+        __syncthreads();
+        tile += step;
+        if (tile < end) stage_tile(sa, tile, fs, fn, vx, vy, vz, flags, stage, locate);
+        body(lds1, sb);
+        if (tile >= end) break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// z-marching walk (single grid): a block takes one tile column (TR rows x TC
+// cols of every page) over pages [p0, p1).  The z-taps of a point are a
+// register window over pages p-3..p+3 that shifts by one page per step, so a
+// step loads one new z-value per point; the tile and the point's other inputs
+// of page p+1 are loaded while page p is evaluated (two LDS buffers, one
+// barrier per page).  In-page offsets are computed once per block; moving to
+// the next page only moves the (scalar) page base.
+// ---------------------------------------------------------------------------
+template <int R>
+struct ZCol {
+    unsigned m[TileRegs<R, double>::NK];  // in-page offsets of the tile-row loads
+    unsigned s;                           // in-page offset of the strip load
+    unsigned in;                          // the point's in-page offset (0 outside the box)
+    int base;                             // the point's LDS index
+    bool valid;                           // the point is in the box
+};
+
+template <int R>
+__device__ __forceinline__ ZCol<R> zcol(const Box& b, int r0, int c0)
+{
+    using S = Shape<R>;
+    ZCol<R> z;
+    const int ty = (int)threadIdx.x % R, tx = tile_tx<R>();
+    const unsigned rows = (unsigned)b.rows;
+    const unsigned row = (unsigned)clampi(r0 - HALO + ty, b.rows - 1);
+#pragma unroll
+    for (int k = 0; k < TileRegs<R, double>::NK; ++k) {
+        const int lc = tx + S::TC * k;
+        z.m[k] = (lc < S::LC) ? (unsigned)clampi(c0 - HALO + lc, b.cols - 1) * rows + row : 0u;
+    }
+    const int e = (int)threadIdx.x;
+    z.s = (e < 2 * HALO * S::LC) ? (unsigned)clampi(c0 - HALO + e / (2 * HALO), b.cols - 1) * rows +
+                                       (unsigned)clampi(r0 - HALO + S::TR + e % (2 * HALO), b.rows - 1)
+                                 : 0u;
+    const int r = r0 + ty, c = c0 + tx;
+    z.valid = r < b.rows && c < b.cols;
+    z.in = z.valid ? (unsigned)c * rows + (unsigned)r : 0u;
+    z.base = (tx + HALO) * S::LR + ty + HALO;
+    return z;
+}
+
+// the loads of one page: the staged tile, the point's velocity, start value
+// and flags, and the z-value the window takes in at this page
+template <int R, typename T, bool VEL>
+struct ZStep {
+    TileRegs<R, T> reg;
+    T vel[VEL ? 3 : 1];
+    T fn, w;
+    uint8_t fl;
+};
+
+template <int R, typename T, bool VEL>
+__device__ __forceinline__ void zstep_load(ZStep<R, T, VEL>& s, const ZCol<R>& zc, int p, const Box& b, idx_t plane,
+                                           const T* __restrict__ fs, const T* __restrict__ fn,
+                                           const T* __restrict__ vx, const T* __restrict__ vy,
+                                           const T* __restrict__ vz, const uint8_t* __restrict__ flags, int stage)
+{
+    using S = Shape<R>;
+    const idx_t po = (idx_t)p * plane;
+    const T* fp = fs + po;
+    const int tx = tile_tx<R>();
+#pragma unroll
+    for (int k = 0; k < TileRegs<R, T>::NK; ++k)
+        if (tx + S::TC * k < S::LC) s.reg.m[k] = ldp(fp, zc.m[k]);
+    if ((int)threadIdx.x < 2 * HALO * S::LC) s.reg.s = ldp(fp, zc.s);
+    s.w = ldp(fs + (idx_t)min(p + 3, b.pges - 1) * plane, zc.in);
+    if (VEL) {
+        s.vel[0] = ldp(vx + po, zc.in);
+        s.vel[1] = ldp(vy + po, zc.in);
+        s.vel[2] = ldp(vz + po, zc.in);
+    }
+    // at stage 0 (no start value) the point's own value is read and dropped
+    const T f0 = ldp((stage != 0 ? fn : fs) + po, zc.in);
+    s.fn = (stage != 0) ? f0 : (T)0;
+    const uint8_t f = VEL && !flags ? (uint8_t)0 : ldp(flags + po, zc.in);
+    s.fl = zc.valid ? f : (uint8_t)FLAG_SUBCELL;
+}
+
+template <typename T>
+__device__ __forceinline__ void window_shift(T w[7], T in)
+{
+#pragma unroll
+    for (int k = 0; k < 6; ++k) w[k] = w[k + 1];
+    w[6] = in;
+}
+
+// body(lds, zc, w, step, p) evaluates page p: x- and y-taps from lds, z-taps w
+template <int R, typename T, bool VEL, class Body>
+__device__ __forceinline__ void zmarch(T* lds0, T* lds1, const Box& b, int r0, int c0, int p0, int p1,
+                                       const T* __restrict__ fs, const T* __restrict__ fn,
+                                       const T* __restrict__ vx, const T* __restrict__ vy,
+                                       const T* __restrict__ vz, const uint8_t* __restrict__ flags, int stage,
+                                       Body body)
+{
+    if (p0 >= p1) return;
+    const ZCol<R> zc = zcol<R>(b, r0, c0);
+    const idx_t plane = (idx_t)b.rows * b.cols;
+    // the window of page p0 - 1 (pages p0-4 .. p0+2, clamped)
+    T w[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) w[k] = ldp(fs + (idx_t)clampi(p0 - 4 + k, b.pges - 1) * plane, zc.in);
+    ZStep<R, T, VEL> sa, sb;
+    zstep_load(sa, zc, p0, b, plane, fs, fn, vx, vy, vz, flags, stage);
+    for (int p = p0;;) {
+        window_shift(w, sa.w);
+        store_tile<R>(lds0, sa.reg);
+        __syncthreads();
+        if (p + 1 < p1) zstep_load(sb, zc, p + 1, b, plane, fs, fn, vx, vy, vz, flags, stage);
+        body(lds0, zc, w, sa, p);
+        if (++p >= p1) break;
+        window_shift(w, sb.w);
+        store_tile<R>(lds1, sb.reg);
+        __syncthreads();
+        if (p + 1 < p1) zstep_load(sa, zc, p + 1, b, plane, fs, fn, vx, vy, vz, flags, stage);
+        body(lds1, zc, w, sb, p);
+        if (++p >= p1) break;
+    }
+}
+
+// the z-march share of block blockIdx.x of a launch of zmarch_blocks(...)
+// blocks: tile column (r0, c0) and pages [p0, p1).  Consecutive blocks of
+// one XCD take neighbouring columns, which share their halos in its L2.
+template <int R>
+__device__ __forceinline__ void zmarch_share(const Box& b, int depth, int& r0, int& c0, int& p0, int& p1)
+{
+    const int ntr = tiles_per_col<R>(b.rows), nct = ntr * tiles_per_row<R>(b.cols);
+    const int gi = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int ch = gi / nct, ct = gi - ch * nct;
+    const int cb = ct / ntr;
+    r0 = (ct - cb * ntr) * R;
+    c0 = cb * Shape<R>::TC;
+    p0 = ch * depth;
+    p1 = min(p0 + depth, b.pges);
+}
+
+// blocks of a z-march launch with `depth` pages per block
+template <int R>
+__host__ __device__ inline int zmarch_blocks(int rows, int cols, int pges, int depth)
+{
+    return tiles_per_col<R>(rows) * tiles_per_row<R>(cols) * ((pges + depth - 1) / depth);
+}
+
+// WENO5-RK3 reinitialization stage at a bulk point: x- and y-taps from the
+// staged tile (lds, the point at index base), z-taps z[0..6] (z[3] the point),
+// the start value fn and flags fl.  Returns the new value; cross is set for a
+// point next to a sign change of the stage field (it is left to the interface
+// pass).
 template <int R, typename T>
-__device__ __forceinline__ bool reinit_bulk_point(T* __restrict__ out, const T* lds, const BulkIn<T, false>& q,
-                                                  const uint8_t* __restrict__ flags, const Tile& t, const Inv& invs,
-                                                  int stage)
+__device__ __forceinline__ T reinit_value(const T* lds, int base, const T z[7], T fn, uint8_t fl, const Inv& invs,
+                                          int stage, bool& cross)
 {
-    const uint8_t fl = point_flags(flags, t);
-    bool cross = false;
+    cross = false;
     T fo[3], bk[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         T v[7];
-        if (a == 0) tile_line_x<R>(lds, t.base, v);
-        else if (a == 1) tile_line_y(lds, t.base, v);
-        else tile_line_z(lds, t.base, q.z, v);
+        if (a == 0) tile_line_x<R>(lds, base, v);
+        else if (a == 1) tile_line_y(lds, base, v);
+        else
+#pragma unroll
+            for (int k = 0; k < 7; ++k) v[k] = z[k];
         cross = cross || (sign_change(v[2], v[3]) || sign_change(v[3], v[4]));
-        weno5_pair(fo[a], bk[a], v, (T)invs.i[a]);
+        weno5_pair(fo[a], bk[a], v, axis_arg(invs, a, T()));
     }
     const T step = godunov_step((fl & FLAG_MASK) != 0, fo[0], bk[0], fo[1], bk[1], fo[2], bk[2], (T)invs.dt_bulk);
-    const T res = rk_combine(stage, lds[t.base], step, q.fn);
-    if (fl & FLAG_SUBCELL) return false;
+    return rk_combine(stage, z[3], step, fn);
+}
+
+// the same on a staged tile of tile_pipeline: writes out and returns false,
+// or returns true (defer to the interface pass) for a point next to a sign
+// change.  Points with subcell distances and points outside the box write
+// nothing and return false.
+template <int R, typename T>
+__device__ __forceinline__ bool reinit_bulk_point(T* __restrict__ out, const T* lds, const BulkIn<T, false>& q,
+                                                  const Tile& t, const Inv& invs, int stage)
+{
+    T z[7];
+    tile_line_z(lds, t.base, q.z, z);
+    bool cross;
+    const T res = reinit_value<R>(lds, t.base, z, q.fn, q.fl, invs, stage, cross);
+    if (q.fl & FLAG_SUBCELL) return false;
     if (!cross) stp(out + t.page, t.in, res);
     return cross;
 }
@@ -575,24 +824,33 @@ __device__ __forceinline__ void reinit_iface_finish(T* __restrict__ out, const T
     }
 }
 
-// WENO5-RK3 extension stage, bulk point of a staged tile (points with subcell
-// distances are left to the interface pass)
+// upwind WENO5 rate of the extension and advection stages at a bulk point
+// (x- and y-taps from the staged tile, z-taps z[0..6]) and the RK
+// combination with the stage's time step: acc * scale is the step
 template <int R, typename T>
-__device__ __forceinline__ void extend_bulk_point(T* __restrict__ out, const T* lds, const BulkIn<T, true>& q,
-                                                  const uint8_t* __restrict__ flags, const Tile& t, const Inv& inv,
-                                                  int stage)
+__device__ __forceinline__ T upwind_value(const T* lds, int base, const T z[7], const T vel[3], T fn, T scale,
+                                          const Inv& inv, int stage)
 {
-    const uint8_t fl = point_flags(flags, t);
     T acc = 0;
     T v[7];
-    tile_line_x<R>(lds, t.base, v);
-    acc = upwind_axis(acc, q.vel[0], v, (T)inv.i[0]);
-    tile_line_y(lds, t.base, v);
-    acc = upwind_axis(acc, q.vel[1], v, (T)inv.i[1]);
-    tile_line_z(lds, t.base, q.z, v);
-    acc = upwind_axis(acc, q.vel[2], v, (T)inv.i[2]);
-    const T res = rk_combine(stage, lds[t.base], acc * (T)inv.dt_bulk, q.fn);
-    if (!(fl & FLAG_SUBCELL)) stp(out + t.page, t.in, res);
+    tile_line_x<R>(lds, base, v);
+    acc = upwind_axis(acc, vel[0], v, axis_arg(inv, 0, T()));
+    tile_line_y(lds, base, v);
+    acc = upwind_axis(acc, vel[1], v, axis_arg(inv, 1, T()));
+    acc = upwind_axis(acc, vel[2], z, axis_arg(inv, 2, T()));
+    return rk_combine(stage, z[3], acc * scale, fn);
+}
+
+// WENO5-RK3 extension stage, bulk point of a staged tile of tile_pipeline
+// (points with subcell distances are left to the interface pass)
+template <int R, typename T>
+__device__ __forceinline__ void extend_bulk_point(T* __restrict__ out, const T* lds, const BulkIn<T, true>& q,
+                                                  const Tile& t, const Inv& inv, int stage)
+{
+    T z[7];
+    tile_line_z(lds, t.base, q.z, z);
+    const T res = upwind_value<R>(lds, t.base, z, q.vel, q.fn, (T)inv.dt_bulk, inv, stage);
+    if (!(q.fl & FLAG_SUBCELL)) stp(out + t.page, t.in, res);
 }
 
 // one lane (point, axis) of the extension interface pass
@@ -647,22 +905,15 @@ __device__ __forceinline__ void extend_iface_finish(T* __restrict__ out, const T
     }
 }
 
-// TVD-RK3 advection stage, point of a staged tile
+// TVD-RK3 advection stage, point of a staged tile of tile_pipeline
 template <int R, typename T>
 __device__ __forceinline__ void advect_point(T* __restrict__ out, const T* lds, const BulkIn<T, true>& q,
                                              double dt, const Tile& t, const Inv& inv, int stage)
 {
-    T acc = 0;
-    T v[7];
-    tile_line_x<R>(lds, t.base, v);
-    acc = upwind_axis(acc, q.vel[0], v, (T)inv.i[0]);
-    tile_line_y(lds, t.base, v);
-    acc = upwind_axis(acc, q.vel[1], v, (T)inv.i[1]);
-    tile_line_z(lds, t.base, q.z, v);
-    acc = upwind_axis(acc, q.vel[2], v, (T)inv.i[2]);
-    const T step = (T)dt * acc;
-    const T res = rk_combine(stage, lds[t.base], step, q.fn);
-    if (t.valid) stp(out + t.page, t.in, res);
+    T z[7];
+    tile_line_z(lds, t.base, q.z, z);
+    const T res = upwind_value<R>(lds, t.base, z, q.vel, q.fn, (T)dt, inv, stage);
+    if (!(q.fl & FLAG_SUBCELL)) stp(out + t.page, t.in, res);
 }
 
 }  // namespace vls

@@ -123,9 +123,20 @@ class SDF3:
         """tile shape of the bulk stage kernels: 64 (64 x 4, default) or 32 (32 x 8)"""
         call("vls_sdf3_set_tile_rows", self._h, int(tile_rows))
 
+    def setZmarchDepth(self, depth):
+        """pages per block of the z-marching bulk stage kernels (0: chosen by the box size)"""
+        call("vls_sdf3_set_zmarch_depth", self._h, int(depth))
+
+    def launch(self):
+        """(tile rows, z-march depth) the bulk stage kernels launch with"""
+        tr, zd = ctypes.c_int(), ctypes.c_int()
+        call("vls_sdf3_get_launch", self._h, ctypes.byref(tr), ctypes.byref(zd))
+        return tr.value, zd.value
+
     def autotune(self, iterations=2):
-        """time extension stages off self.F's interface with each tile shape and keep
-        the fastest; leaves the distances of self.F set.  Returns the tile rows."""
+        """time extension stages off self.F's interface with each tile shape and
+        z-march depth and keep the fastest; leaves the distances of self.F set.
+        Returns the tile rows."""
         tr = ctypes.c_int()
         call(self._fn("vls_sdf3_autotune"), self._h, self._check(self.F), int(iterations), ctypes.byref(tr),
              _stream())

@@ -121,6 +121,12 @@ int vls_sdf3_destroy(vls_sdf3_t h);
 /* tile shape of the fused bulk stage kernels: 64 (64 x 4 points of a page,
  * the default) or 32 (32 x 8); results do not depend on it */
 int vls_sdf3_set_tile_rows(vls_sdf3_t h, int tile_rows);
+/* pages per block of the z-marching bulk stage kernels: each block walks one
+ * tile column over `depth` consecutive pages (0, the default: chosen by the
+ * box size, about 8 blocks per CU); results do not depend on it */
+int vls_sdf3_set_zmarch_depth(vls_sdf3_t h, int depth);
+/* the tile rows and z-march depth the bulk stage kernels launch with */
+int vls_sdf3_get_launch(vls_sdf3_t h, int *tile_rows, int *depth);
 
 /* setDistance (+SD/@SDF3/setDistance.m:3-21): subcell distances of F with the
  * sixth-order locator, and min_dist.  Stored in the handle. */
@@ -308,7 +314,9 @@ int vls_batch_step_f32(vls_batch_t b, float *F, float *Vn, const float *vx, cons
 /* ------------------------------------------------------------------------ */
 /* Times `iterations` extension iterations of a scratch copy of F, off F's
  * interface, for each candidate launch shape and keeps the fastest: tile rows
- * 64 or 32 and, for a batch, 2/4/8/16 persistent blocks per CU.  Results of
+ * 64 or 32 and, for a batch, 2/4/8/16 persistent blocks per CU, for a single
+ * grid the z-march depth chosen by the box size, half or twice it (read it
+ * back with vls_sdf3_get_launch).  Results of
  * every method are the same for every shape.  Synchronizes.  The handle is
  * left with F's distances (as after set_distance(F)); a batch's cached graphs
  * are dropped.  tile_rows and blocks (may be null) receive the choice. */

@@ -290,6 +290,33 @@ def test_tile_shapes_give_same_results(dtype):
         assert torch.equal(a, c)
 
 
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_zmarch_depth_gives_same_results(dtype):
+    """single grid: the pages per block of the z-marching bulk kernels (1, 3,
+    all 29, and the default) do not change any result"""
+    g = cases.Grid(np.linspace(-1, 1, 37), np.linspace(-1, 1, 45), np.linspace(-1, 1, 29))
+    G = vls.GD3(g.xv, g.yv, g.zv)
+    F, C, u = _fields(g, 2)
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda", dtype)  # noqa: E731
+    outs = []
+    for tr, depth in [(64, 0), (64, 1), (64, 3), (64, 29), (32, 5)]:
+        m = vls.SDF3(G, T(F))
+        m.setTileRows(tr)
+        m.setZmarchDepth(depth)
+        got = m.launch()
+        assert got[0] == tr and (depth == 0 or got[1] == depth)
+        m.setDistance()
+        Fr = m.WENO5RK3Reinitialization(m.F, 3)
+        m.F = Fr
+        m.setDistance()
+        Cn = m.WENO5RK3Extend(T(C), 3)
+        Fa = m.WENO5RK3Advection(Fr, *[T(c) for c in u], 0.01, 2)
+        outs.append((Fr, Cn, Fa))
+    for o in outs[1:]:
+        for a, b in zip(o, outs[0]):
+            assert torch.equal(a, b)
+
+
 def test_autotune_keeps_results():
     b, grids, Gs, F, C, u = _setup(torch.float32)
     ref_F, ref_C = F.clone(), C.clone()
