@@ -68,7 +68,7 @@ __global__ void __launch_bounds__(256) ka_advect_jac(double* __restrict__ J, dou
 #pragma unroll
         for (int k = 0; k < 7; ++k) q[k] = ad::dvar<8>(v[k], k);
         // the term of axis a: coef(u_a) * D_a (upwind_axis), u_a is input 7
-        const D8 t = upwind_axis(D8(0.0), ad::dvar<8>(vel[a], 7), q, inv.i[a]);
+        const D8 t = upwind_axis(D8(0.0), ad::dvar<8>(vel[a], 7), q, inv.jd[a]);
 #pragma unroll
         for (int k = 0; k < 7; ++k) {
             // out = c (fs - dt * sum_a t_a) + e fn
@@ -90,7 +90,7 @@ __global__ void __launch_bounds__(256) ka_advect_jac(double* __restrict__ J, dou
 // axis pair (fore, back) on duals of the 7 line values (inputs 0..6) and
 // r1, l1 (inputs 7, 8), with reinit_axis's branch
 __device__ __forceinline__ void reinit_axis_dual(D9& fo, D9& bk, const double v[7], double r1, double l1, double ds,
-                                                 double inv_ds)
+                                                 JPd jp)
 {
     D9 q[7];
 #pragma unroll
@@ -100,7 +100,7 @@ __device__ __forceinline__ void reinit_axis_dual(D9& fo, D9& bk, const double v[
         fo = pr.fore;
         bk = pr.back;
     } else {
-        weno5_pair(fo, bk, q, inv_ds);
+        weno5_pair(fo, bk, q, jp);
     }
 }
 
@@ -126,7 +126,7 @@ __device__ __forceinline__ void reinit_point_jac(double* __restrict__ J, double*
 #pragma unroll 1
     for (int a = 0; a < 3; ++a) {
         line7(fs, r, c, p, a, b, v[a]);
-        reinit_axis_dual(fo[a], bk[a], v[a], r1[a], l1[a], sp[a], inv.i[a]);
+        reinit_axis_dual(fo[a], bk[a], v[a], r1[a], l1[a], sp[a], inv.jd[a]);
     }
     // Godunov step on duals of its 7 arguments (fore/back per axis, dt)
     const D7 g = godunov_step(inside, ad::dvar<7>(fo[0].v, 0), ad::dvar<7>(bk[0].v, 1), ad::dvar<7>(fo[1].v, 2),
@@ -449,8 +449,8 @@ static inline Box abox(const Geom& g) { return Box{g.rows, g.cols, g.pges, g.dx,
 static inline Inv ainv(const Geom& g)
 {
     const double m = std::min(std::min(g.dx, g.dx), std::min(std::min(g.dy, g.dy), std::min(g.dz, g.dz)));
-    Inv v{{1.0 / g.dx, 1.0 / g.dy, 1.0 / g.dz}, 0.3 * m, {}};
-    set_jp(v.jf, g.dx, g.dy, g.dz);
+    Inv v{{1.0 / g.dx, 1.0 / g.dy, 1.0 / g.dz}, 0.3 * m, {}, {}};
+    set_jp(v.jd, v.jf, g.dx, g.dy, g.dz);
     return v;
 }
 static inline IList alist(const ListView& lv) { return IList{lv.idx, lv.count, lv.dist, lv.dt, lv.cap}; }

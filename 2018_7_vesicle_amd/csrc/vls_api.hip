@@ -1135,7 +1135,7 @@ int vls_batch_create(vls_batch_t* out, int ngrids, const int* rows, const int* c
         G.inv[2] = 1.0 / dz[g];
         const double m = std::min(std::min(dx[g], dx[g]), std::min(std::min(dy[g], dy[g]), std::min(dz[g], dz[g])));
         G.dt_bulk = 0.3 * m;
-        set_jp(G.jf, dx[g], dy[g], dz[g]);
+        set_jp(G.jd, G.jf, dx[g], dy[g], dz[g]);
         b->grids.push_back(G);
         b->offs[(size_t)g] = off;
         off += (int64_t)rows[g] * cols[g] * pges[g];

@@ -25,7 +25,10 @@ __device__ __forceinline__ Inv inv_of(const BGrid& G)
     v.i[1] = G.inv[1];
     v.i[2] = G.inv[2];
     v.dt_bulk = G.dt_bulk;
-    for (int a = 0; a < 3; ++a) v.jf[a] = G.jf[a];
+    for (int a = 0; a < 3; ++a) {
+        v.jd[a] = G.jd[a];
+        v.jf[a] = G.jf[a];
+    }
     return v;
 }
 
@@ -269,7 +272,7 @@ __global__ void __launch_bounds__(256) kb_ext_vel_bulk(T* __restrict__ vx, T* __
         for (int a = 0; a < 3; ++a) {
             double v[7], fo, bk;
             line7(F + G.off, t.r, t.c, t.p, a, b, v);
-            weno5_pair(fo, bk, v, 1.0 / sp[a]);
+            weno5_pair(fo, bk, v, jpd(sp[a]));
             n[a] = (fabs(v[4]) < fabs(v[2])) ? fo : bk;
         }
         store_extension_velocity(vx, vy, vz, F, (idx_t)(G.off + ind), n, floor_);
@@ -294,7 +297,7 @@ __global__ void __launch_bounds__(256) kb_ext_vel_list(T* __restrict__ vx, T* __
             double v[7], fo, bk;
             line7(F + G.off, r, c, p, a, b, v);
             extend_axis(fo, bk, v, il.dist[(size_t)(2 * a) * il.cap + j], il.dist[(size_t)(2 * a + 1) * il.cap + j],
-                        sp[a], 1.0 / sp[a], 0.0, 0.0);
+                        sp[a], jpd(sp[a]), 0.0, 0.0);
             n[a] = (fabs(v[4]) < fabs(v[2])) ? fo : bk;
         }
         store_extension_velocity(vx, vy, vz, F, (idx_t)gind, n, floor_);

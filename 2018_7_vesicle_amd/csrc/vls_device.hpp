@@ -63,32 +63,56 @@ __device__ __forceinline__ void line7(const T* __restrict__ f, int r, int c, int
 }
 
 // ---------------------------------------------------------------------------
-// Uniform WENO5 (Jiang-Peng), 1_3_WENO5RK3_Reinitialization/re_step.cu:11-37.
-// alpha_k = c_k/(S_k+eps)^2 and omega_k = alpha_k/sum are folded into one
-// division: sum_k c_k phi_k prod_{j!=k} a_j / sum_k c_k prod_{j!=k} a_j with
-// a_k = (S_k+eps)^2.  For |v| < 1e30 nothing overflows (a_j a_k ~ v^8).
-// The library is compiled without floating-point contraction; the fused
-// multiply-adds below are explicit, so the result is one fixed IEEE sequence
-// (the CPU oracle states the same sequence).
+// Uniform WENO5 derivatives of re_step.cu:11-37 (same text in
+// 2_2_WENO5RK3_Extend/shared_utilities.cup:19-45, advection_step.cu:10-28,
+// nonoscillating_interpolant.cu:31-57) in the Jiang-Peng form (G.-S. Jiang,
+// D. Peng, SIAM J. Sci. Comput. 21 (2000), eq. 2.5-2.8), which is
+// algebraically identical to the reference's stencil-weight form and cheaper
+// to evaluate.  With nodal values p[0..6] (p[3] the point), undivided second
+// differences D_j = p_j - 2 p_{j+1} + p_{j+2} and the fourth-order central
+// part C = p_1 - p_5 + 8 (p_4 - p_2):
+//   D-  = (C - Q(D0, D1, D2, D3)) / (12 ds),   D+ = (C + Q(D4, D3, D2, D1)) / (12 ds),
+//   Q(a,b,c,d) = [4 w0 (a - 2b + c) + (2 w2 - W)(b - 2c + d)] / W,
+//   w0 = s1^2 s2^2, w1 = 6 s0^2 s2^2, w2 = 3 s0^2 s1^2, W = w0 + w1 + w2,
+//   s0 = (a-b)^2 + 3/13 (a-3b)^2 + e, s1 = (b-c)^2 + 3/13 (b+c)^2 + e,
+//   s2 = (c-d)^2 + 3/13 (3c-d)^2 + e,  e = 12/13 * 1e-6 * ds^2,
+// where s_k = 12 ds^2/13 * (S_k + 1e-6) of the reference's smoothness
+// indicators S_k of divided differences, so the weights are the reference's.
+// The FP64 form below is the exact IEEE sequence of the CPU oracle
+// (or_weno_pair; the library is compiled without floating-point contraction
+// and the fused multiply-adds are explicit); templates on the scalar S also
+// evaluate it on dual numbers for the adjoint.  Nothing overflows while the
+// second differences stay below about 1e70 (W ~ D^8).
 // ---------------------------------------------------------------------------
 template <typename S>
-__device__ __forceinline__ S weno5_onesided(S v1, S v2, S v3, S v4, S v5)
+__device__ __forceinline__ S jp_q(S a, S b, S c, S d, double e)
 {
-    const double c13 = 1. / 3., c76 = 7. / 6., c116 = 11. / 6., c16 = 1. / 6., c56 = 5. / 6., c1312 = 13. / 12.;
-    const S phi1 = fma(c116, v3, fma(-c76, v2, c13 * v1));
-    const S phi2 = fma(c13, v4, fma(c56, v3, -c16 * v2));
-    const S phi3 = fma(-c16, v5, fma(c56, v4, c13 * v3));
-    const S d1 = fma(-2.0, v2, v1) + v3, e1 = fma(3.0, v3, fma(-4.0, v2, v1));
-    const S d2 = fma(-2.0, v3, v2) + v4, e2 = v2 - v4;
-    const S d3 = fma(-2.0, v4, v3) + v5, e3 = fma(3.0, v3, fma(-4.0, v4, v5));
-    const S S1 = fma(c1312 * d1, d1, 0.25 * e1 * e1);
-    const S S2 = fma(c1312 * d2, d2, 0.25 * e2 * e2);
-    const S S3 = fma(c1312 * d3, d3, 0.25 * e3 * e3);
-    const double eps = 1e-6;
-    const S s1 = S1 + eps, s2 = S2 + eps, s3 = S3 + eps;
-    const S a1 = s1 * s1, a2 = s2 * s2, a3 = s3 * s3;
-    const S w1 = 0.1 * (a2 * a3), w2 = 0.6 * (a1 * a3), w3 = 0.3 * (a1 * a2);
-    return fma(w1, phi1, fma(w2, phi2, w3 * phi3)) / (w1 + w2 + w3);
+    const double k = 3.0 / 13.0;
+    const S t0 = a - b, t1 = b - c, t2 = c - d;
+    const S y0 = fma(-3.0, b, a), y1 = b + c, y2 = fma(3.0, c, -d);
+    const S s0 = fma(t0, t0, fma(k * y0, y0, S(e)));
+    const S s1 = fma(t1, t1, fma(k * y1, y1, S(e)));
+    const S s2 = fma(t2, t2, fma(k * y2, y2, S(e)));
+    const S a0 = s0 * s0, a1 = s1 * s1, a2 = s2 * s2;
+    const S w0 = a1 * a2, w1 = 6.0 * (a0 * a2), w2 = 3.0 * (a0 * a1);
+    const S W = (w0 + w1) + w2;
+    return fma(4.0 * w0, t0 - t1, fma(2.0, w2, -W) * (t1 - t2)) / W;
+}
+
+// the second differences D0..D4 and the central part C of p[0..6]
+template <typename S>
+struct JPcd {
+    S d[5];
+    S c;
+};
+template <typename S>
+__device__ __forceinline__ JPcd<S> jp_centre(const S p[7])
+{
+    JPcd<S> m;
+#pragma unroll
+    for (int j = 0; j < 5; ++j) m.d[j] = fma(-2.0, p[j + 1], p[j]) + p[j + 2];
+    m.c = fma(8.0, p[4] - p[2], p[1] - p[5]);
+    return m;
 }
 
 // Upwind term of one axis, min(0,vel)*D+ + max(0,vel)*D- (extend_step.cu:121-123,
@@ -99,48 +123,32 @@ __device__ __forceinline__ S weno5_onesided(S v1, S v2, S v3, S v4, S v5)
 // acc is never -0 (it starts at +0 and x + -x rounds to +0), so the result
 // is bit-identical to evaluating both derivatives.
 template <typename S>
-__device__ __forceinline__ S upwind_axis(S acc, S vel, const S p[7], double inv_ds)
+__device__ __forceinline__ S upwind_axis(S acc, S vel, const S p[7], JPd j)
 {
-    S v[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) v[k] = (p[k + 1] - p[k]) * inv_ds;
+    const JPcd<S> m = jp_centre(p);
     const bool fore = val(vel) < 0;
-    const S d = weno5_onesided(fore ? v[5] : v[0], fore ? v[4] : v[1], fore ? v[3] : v[2], fore ? v[2] : v[3],
-                               fore ? v[1] : v[4]);
+    const S q = jp_q(fore ? m.d[4] : m.d[0], fore ? m.d[3] : m.d[1], m.d[2], fore ? m.d[1] : m.d[3], j.e);
+    const S d = (fore ? m.c + q : m.c - q) * j.k;
     return acc + (fore ? vel : dmax(S(0.0), vel)) * d;
 }
 
-// back/fore pair from 7 nodal values (re_step.cu:260-267), 1/ds precomputed
+// back/fore pair (re_step.cu:260-267)
 template <typename S>
-__device__ __forceinline__ void weno5_pair(S& d_fore, S& d_back, const S p[7], double inv_ds)
+__device__ __forceinline__ void weno5_pair(S& d_fore, S& d_back, const S p[7], JPd j)
 {
-    S v[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) v[k] = (p[k + 1] - p[k]) * inv_ds;
-    d_back = weno5_onesided(v[0], v[1], v[2], v[3], v[4]);
-    d_fore = weno5_onesided(v[5], v[4], v[3], v[2], v[1]);
+    const JPcd<S> m = jp_centre(p);
+    d_back = (m.c - jp_q(m.d[0], m.d[1], m.d[2], m.d[3], j.e)) * j.k;
+    d_fore = (m.c + jp_q(m.d[4], m.d[3], m.d[2], m.d[1], j.e)) * j.k;
 }
 
 // ---------------------------------------------------------------------------
 // FP32 bulk arithmetic, for the points away from the interface: the same
-// WENO5 derivatives in the Jiang-Peng form (G.-S. Jiang, D. Peng, SIAM J. Sci.
-// Comput. 21 (2000), eq. 2.5-2.8), algebraically identical to the
-// stencil-weight form of re_step.cu:11-37 and cheaper to evaluate.  With
-// nodal values p[0..6] (p[3] the point), undivided second differences
-// D_j = p_j - 2 p_{j+1} + p_{j+2} and the fourth-order central part
-// C = p_1 - p_5 + 8 (p_4 - p_2):
-//   D-  = (C - Q(D0, D1, D2, D3)) / (12 ds),   D+ = (C + Q(D4, D3, D2, D1)) / (12 ds),
-//   Q(a,b,c,d) = [4 w0 (a - 2b + c) + (2 w2 - W)(b - 2c + d)] / W,
-//   w0 = s1^2 s2^2, w1 = 6 s0^2 s2^2, w2 = 3 s0^2 s1^2, W = w0 + w1 + w2,
-//   s0 = (a-b)^2 + 3/13 (a-3b)^2 + e, s1 = (b-c)^2 + 3/13 (b+c)^2 + e,
-//   s2 = (c-d)^2 + 3/13 (3c-d)^2 + e,  e = 12/13 * 1e-6 * ds^2,
-// where s_k = 12 ds^2/13 * (S_k + 1e-6) of the reference's smoothness
-// indicators S_k of divided differences, so the weights are the reference's.
-// The weights are homogeneous of degree 0 in the s_k: they are scaled by a
-// power of two that puts the largest in [0.5, 1), which keeps every product
-// in FP32 range for any data (the products only underflow, harmlessly, for
-// indicator ratios beyond 1e19, where W is floored at 1e-30 and Q -> 0).
-// The final division is a hardware reciprocal (v_rcp_f32, 1 ulp).
+// Jiang-Peng form in single precision.  The weights are homogeneous of degree
+// 0 in the s_k: they are scaled by a power of two that puts the largest in
+// [0.5, 1), which keeps every product in FP32 range for any data (products
+// only underflow, harmlessly, for indicator ratios beyond 1e19, where W is
+// floored at 1e-30 and Q -> 0).  The final division is a hardware reciprocal
+// (v_rcp_f32, 1 ulp).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float fmaxf_(float a, float b) { return (a < b) ? b : a; }
 __device__ __forceinline__ float fminf_(float a, float b) { return (a < b) ? a : b; }

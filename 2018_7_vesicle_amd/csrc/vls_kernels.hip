@@ -117,7 +117,7 @@ __device__ __forceinline__ double reinit_residual(const double* __restrict__ f, 
     for (int a = 0; a < 3; ++a) {
         double v[7];
         line7(f, r, c, p, a, b, v);
-        reinit_axis(fo[a], bk[a], v, q.r[a], q.l[a], sp[a], 1.0 / sp[a]);
+        reinit_axis(fo[a], bk[a], v, q.r[a], q.l[a], sp[a], jpd(sp[a]));
     }
     return godunov_step(inside, fo[0], bk[0], fo[1], bk[1], fo[2], bk[2], dt);
 }
@@ -329,7 +329,7 @@ __global__ void __launch_bounds__(256) k_upwind_normal(T* __restrict__ nx, T* __
     for (int a = 0; a < 3; ++a) {
         double v[7], fo, bk;
         line7(lsf, r, c, p, a, b, v);
-        extend_axis(fo, bk, v, d.r[a][ind], d.l[a][ind], sp[a], 1.0 / sp[a], 0.0, 0.0);
+        extend_axis(fo, bk, v, d.r[a][ind], d.l[a][ind], sp[a], jpd(sp[a]), 0.0, 0.0);
         out[a][ind] = (T)((fabs(v[4]) < fabs(v[2])) ? fo : bk);
     }
 }
@@ -383,7 +383,7 @@ __device__ __forceinline__ double extend_rate(const double* __restrict__ C, int 
         line7(C, r, c, p, a, b, v);
         double hf = 0.0, hb = 0.0;
         if (subcell) { hf = cp.r[a][ind]; hb = cp.l[a][ind]; }
-        extend_axis(fo, bk, v, q.r[a], q.l[a], sp[a], 1.0 / sp[a], hf, hb);
+        extend_axis(fo, bk, v, q.r[a], q.l[a], sp[a], jpd(sp[a]), hf, hb);
         // extend_step.cu:121-123 sums x, y, z terms left to right
         acc = acc + dmin(0.0, vel[a]) * fo;
         acc = acc + dmax(0.0, vel[a]) * bk;
@@ -462,7 +462,7 @@ __device__ __forceinline__ double advect_rate(const double* __restrict__ f, int 
     for (int a = 0; a < 3; ++a) {
         double v[7];
         line7(f, r, c, p, a, b, v);
-        acc = upwind_axis(acc, vel[a], v, inv.i[a]);
+        acc = upwind_axis(acc, vel[a], v, inv.jd[a]);
     }
     return acc;
 }
@@ -507,7 +507,7 @@ __global__ void __launch_bounds__(256) k_weno_derivative(Dist6 out, const double
     for (int a = 0; a < 3; ++a) {
         double v[7], fo, bk;
         line7(lsf, r, c, p, a, b, v);
-        weno5_pair(fo, bk, v, 1.0 / sp[a]);
+        weno5_pair(fo, bk, v, jpd(sp[a]));
         out.l[a][ind] = bk;  // back_x, back_y, back_z
         out.r[a][ind] = fo;  // fore_x, fore_y, fore_z
     }
@@ -783,8 +783,8 @@ static inline Box box_of(const Geom& g) { return Box{g.rows, g.cols, g.pges, g.d
 static inline Inv inv_of(const Geom& g)
 {
     const double m = std::min(std::min(g.dx, g.dx), std::min(std::min(g.dy, g.dy), std::min(g.dz, g.dz)));
-    Inv v{{1.0 / g.dx, 1.0 / g.dy, 1.0 / g.dz}, 0.3 * m, {}};
-    set_jp(v.jf, g.dx, g.dy, g.dz);
+    Inv v{{1.0 / g.dx, 1.0 / g.dy, 1.0 / g.dz}, 0.3 * m, {}, {}};
+    set_jp(v.jd, v.jf, g.dx, g.dy, g.dz);
     return v;
 }
 

@@ -146,18 +146,25 @@ hipError_t launch_copy(T* dst, const T* src, int64_t n, hipStream_t s);
 // field of a batch is the grids' column-major arrays back to back) and its
 // first tile in the batch's tile list (tiles of tr rows x 256/tr cols x 1
 // page, page-major).
-// per-axis constants of the FP32 bulk WENO5 form (vls_device.hpp):
-// k = 1/(12 ds), e = 12/13 * 1e-6 * ds^2
+// per-axis constants of the Jiang-Peng form of the uniform WENO5 derivative
+// (vls_device.hpp): k = 1/(12 ds), e = 12/13 * 1e-6 * ds^2, in FP64 (the
+// expressions the CPU oracle evaluates) and rounded to FP32
+struct JPd {
+    double k, e;
+};
 struct JPf {
     float k, e;
 };
 
-__host__ __device__ inline void set_jp(JPf jf[3], double dx, double dy, double dz)
+__host__ __device__ inline JPd jpd(double ds) { return JPd{1.0 / (12.0 * ds), 12.0 / 13.0 * 1e-6 * ds * ds}; }
+
+__host__ __device__ inline void set_jp(JPd jd[3], JPf jf[3], double dx, double dy, double dz)
 {
     const double sp[3] = {dx, dy, dz};
     for (int a = 0; a < 3; ++a) {
-        jf[a].k = (float)(1.0 / (12.0 * sp[a]));
-        jf[a].e = (float)(12.0 / 13.0 * 1e-6 * sp[a] * sp[a]);
+        jd[a] = jpd(sp[a]);
+        jf[a].k = (float)jd[a].k;
+        jf[a].e = (float)jd[a].e;
     }
 }
 
@@ -169,7 +176,8 @@ struct BGrid {
     double dx, dy, dz;
     double inv[3];    // 1/dx, 1/dy, 1/dz
     double dt_bulk;   // 0.3*min(dx, dy, dz)
-    JPf jf[3];        // the FP32 bulk form's per-axis constants (set_jp)
+    JPd jd[3];        // the WENO5 form's per-axis constants (set_jp)
+    JPf jf[3];
 };
 
 // device view of a batch

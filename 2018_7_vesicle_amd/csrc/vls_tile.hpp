@@ -29,12 +29,12 @@ __device__ __forceinline__ idx_t lin(const Box& b, int r, int c, int p)
 struct Inv {
     double i[3];
     double dt_bulk;
-    JPf jf[3];  // the FP32 bulk form's per-axis constants
+    JPd jd[3];  // per-axis constants of the WENO5 form (set_jp)
+    JPf jf[3];
 };
 
-// the per-axis argument of the WENO5 helpers: 1/ds for FP64, the FP32 form's
-// constants for FP32
-__device__ __forceinline__ double axis_arg(const Inv& v, int a, double) { return v.i[a]; }
+// the per-axis constants of the WENO5 helpers for FP64 and FP32 fields
+__device__ __forceinline__ JPd axis_arg(const Inv& v, int a, double) { return v.jd[a]; }
 __device__ __forceinline__ JPf axis_arg(const Inv& v, int a, float) { return v.jf[a]; }
 
 // RK combinations in the order the MATLAB wrappers evaluate them
@@ -122,14 +122,14 @@ __device__ __forceinline__ double min6(const Dists& q)
 
 // per-axis derivative pair with the sign-change interface test (re_step.cu:248-269)
 __device__ __forceinline__ void reinit_axis(double& fore, double& back, const double v[7], double r1, double l1,
-                                            double ds, double inv_ds)
+                                            double ds, JPd jp)
 {
     if (v[2] * v[3] < 0 || v[3] * v[4] < 0) {
         const Pair q = weno_nonuniform_pair(v, r1, l1, ds, 0.0, 0.0);
         fore = q.fore;
         back = q.back;
     } else {
-        weno5_pair(fore, back, v, inv_ds);
+        weno5_pair(fore, back, v, jp);
     }
 }
 
@@ -390,14 +390,14 @@ __device__ __forceinline__ int axis_lane_stride() { return (int)(gridDim.x * (bl
 
 // per-axis pair with the distance interface test (shared_utilities.cup:256-278)
 __device__ __forceinline__ void extend_axis(double& fore, double& back, const double v[7], double r1, double l1,
-                                            double ds, double inv_ds, double h_fore, double h_back)
+                                            double ds, JPd jp, double h_fore, double h_back)
 {
     if (r1 < ds || l1 < ds) {
         const Pair q = weno_nonuniform_pair(v, r1, l1, ds, h_fore, h_back);
         fore = q.fore;
         back = q.back;
     } else {
-        weno5_pair(fore, back, v, inv_ds);
+        weno5_pair(fore, back, v, jp);
     }
 }
 
@@ -799,7 +799,7 @@ __device__ __forceinline__ ReinitLane reinit_iface_lane(const T* __restrict__ fs
     if (q.dt >= 0) {
         double v[7], fo, bk;
         line7(fs, r, c, p, a, b, v);
-        reinit_axis(fo, bk, v, r1, l1, sp[a], inv.i[a]);
+        reinit_axis(fo, bk, v, r1, l1, sp[a], inv.jd[a]);
         q.m = q.inside ? dmax(sq(dmin(0.0, bk)), sq(dmax(0.0, fo))) : dmax(sq(dmax(0.0, bk)), sq(dmin(0.0, fo)));
     }
     return q;
@@ -887,7 +887,7 @@ __device__ __forceinline__ ExtendLane extend_iface_lane(const T* __restrict__ cs
                                                   lcp[(size_t)(2 * a + 1) * il.cap + slot], !fore);
             q.t = 0.0 + (fore ? vel : dmax(0.0, vel)) * dv;
         } else {
-            q.t = upwind_axis(0.0, vel, v, inv.i[a]);
+            q.t = upwind_axis(0.0, vel, v, inv.jd[a]);
         }
     }
     return q;

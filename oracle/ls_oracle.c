@@ -260,17 +260,16 @@ void or_boundary_correction(double *xpr, double *xpl, double *ypf, double *ypb, 
 /* WENO5 derivatives                                                          */
 /* ------------------------------------------------------------------------- */
 
-/* Jiang-Peng WENO5 one-sided derivative from 5 divided differences,
+/* The stencil-weight WENO5 one-sided derivative from 5 divided differences
+ * (a check of or_weno_pair below, which the algorithms use),
  * CUDA_Code/1_3_WENO5RK3_Reinitialization/re_step.cu:11-37 (same text in
  * 2_2_WENO5RK3_Extend/shared_utilities.cup:19-45 and
  * 5_0_Nonoscillating_Reconstruction/nonoscillating_interpolant.cu:31-57).
  * The reference's alpha_k = c_k/(S_k+eps)^2, omega_k = alpha_k/sum(alpha),
  * d = sum omega_k phi_k is evaluated here as the algebraically identical
  * d = sum_k c_k phi_k prod_{j!=k} a_j / sum_k c_k prod_{j!=k} a_j with
- * a_k = (S_k+eps)^2 and explicit fused multiply-adds: the exact IEEE sequence
- * the product's kernels use, so GPU and oracle can be compared bit for bit.
- * The reference's own order is or_weno_onesided_reference below; the two are
- * compared in tests/test_oracle_math.py. */
+ * a_k = (S_k+eps)^2.  The reference's own order is or_weno_onesided_reference
+ * below; both are compared with or_weno_pair in tests/test_oracle_math.py. */
 double or_weno_onesided(double v1, double v2, double v3, double v4, double v5)
 {
     const double c13 = 1. / 3., c76 = 7. / 6., c116 = 11. / 6., c16 = 1. / 6., c56 = 5. / 6., c1312 = 13. / 12.;
@@ -401,14 +400,49 @@ static void weno_nonuniform_pair(double *d_fore, double *d_back, const double p[
     *d_back = weno_nonuniform_face(&x[0], &d[0], 1);
 }
 
-/* re_step.cu:260-267; the divided differences multiply by 1/ds */
+/* The uniform WENO5 pair of re_step.cu:260-267 (re_step.cu:11-37 on the
+ * divided differences of p) in the Jiang-Peng form (G.-S. Jiang, D. Peng,
+ * SIAM J. Sci. Comput. 21 (2000), eq. 2.5-2.8): with undivided second
+ * differences D_j = p_j - 2 p_{j+1} + p_{j+2} and the fourth-order central
+ * part C = p_1 - p_5 + 8 (p_4 - p_2),
+ *   d_back = (C - Q(D0,D1,D2,D3)) / (12 ds),  d_fore = (C + Q(D4,D3,D2,D1)) / (12 ds),
+ *   Q(a,b,c,d) = [4 w0 (a-2b+c) + (2 w2 - W)(b-2c+d)] / W,
+ *   w0 = s1^2 s2^2, w1 = 6 s0^2 s2^2, w2 = 3 s0^2 s1^2, W = w0 + w1 + w2,
+ *   s0 = (a-b)^2 + 3/13 (a-3b)^2 + e, s1 = (b-c)^2 + 3/13 (b+c)^2 + e,
+ *   s2 = (c-d)^2 + 3/13 (3c-d)^2 + e,  e = 12/13 * 1e-6 * ds^2.
+ * s_k = 12 ds^2/13 * (S_k + 1e-6) for the reference's smoothness indicators
+ * S_k of the divided differences, so the weights are the reference's and the
+ * result is algebraically the reference's; it is the exact IEEE sequence the
+ * product's FP64 kernels evaluate.  The stencil-weight form
+ * (or_weno_onesided, or_weno_onesided_reference) is checked against it in
+ * tests/test_oracle_math.py. */
+static inline double jp_q(double a, double b, double c, double d, double e)
+{
+    const double k = 3.0 / 13.0;
+    const double t0 = a - b, t1 = b - c, t2 = c - d;
+    const double y0 = fma(-3.0, b, a), y1 = b + c, y2 = fma(3.0, c, -d);
+    const double s0 = fma(t0, t0, fma(k * y0, y0, e));
+    const double s1 = fma(t1, t1, fma(k * y1, y1, e));
+    const double s2 = fma(t2, t2, fma(k * y2, y2, e));
+    const double a0 = s0 * s0, a1 = s1 * s1, a2 = s2 * s2;
+    const double w0 = a1 * a2, w1 = 6.0 * (a0 * a2), w2 = 3.0 * (a0 * a1);
+    const double W = (w0 + w1) + w2;
+    return fma(4.0 * w0, t0 - t1, fma(2.0, w2, -W) * (t1 - t2)) / W;
+}
+
+void or_weno_pair(double *d_fore, double *d_back, const double p[7], double ds)
+{
+    const double k = 1.0 / (12.0 * ds), e = 12.0 / 13.0 * 1e-6 * ds * ds;
+    double D[5];
+    for (int j = 0; j < 5; ++j) D[j] = fma(-2.0, p[j + 1], p[j]) + p[j + 2];
+    const double c = fma(8.0, p[4] - p[2], p[1] - p[5]);
+    *d_back = (c - jp_q(D[0], D[1], D[2], D[3], e)) * k;
+    *d_fore = (c + jp_q(D[4], D[3], D[2], D[1], e)) * k;
+}
+
 static inline void weno_uniform_pair(double *d_fore, double *d_back, const double p[7], double ds)
 {
-    double v[6];
-    const double inv_ds = 1.0 / ds;
-    for (int k = 0; k < 6; ++k) v[k] = (p[k + 1] - p[k]) * inv_ds;
-    *d_back = or_weno_onesided(v[0], v[1], v[2], v[3], v[4]);
-    *d_fore = or_weno_onesided(v[5], v[4], v[3], v[2], v[1]);
+    or_weno_pair(d_fore, d_back, p, ds);
 }
 
 /* Exported for the tests: the per-axis derivative pair of re_step

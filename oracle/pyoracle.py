@@ -131,6 +131,14 @@ def weno_onesided(v1, v2, v3, v4, v5):
     return f(_D(v1), _D(v2), _D(v3), _D(v4), _D(v5))
 
 
+def weno_pair(p7, ds):
+    """(d_fore, d_back) of the uniform WENO5 pair (or_weno_pair, Jiang-Peng form)"""
+    p = np.ascontiguousarray(p7, dtype=np.float64)
+    df, db = _D(), _D()
+    lib().or_weno_pair(ctypes.byref(df), ctypes.byref(db), _p(p), _D(ds))
+    return df.value, db.value
+
+
 def six_distance(v0, v1, v2, v3, v4, v5, s):
     f = lib().or_six_distance
     f.restype = _D

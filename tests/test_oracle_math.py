@@ -66,6 +66,58 @@ def test_nonuniform_weno_on_uniform_stencil_is_weno5():
     assert abs(db - po.weno_onesided(*v[:5])) < 1e-14
 
 
+@pytest.mark.parametrize("trial", range(5))
+def test_weno5_pair_exact_for_cubics(trial):
+    a, b, c, d = RNG.normal(size=4)
+    h = RNG.uniform(0.01, 0.2)
+    x = (np.arange(7) - 3) * h
+    p = a + b * x + c * x ** 2 + d * x ** 3
+    scale = abs(a) + abs(b) + abs(c) + abs(d)
+    df, db = po.weno_pair(p, h)
+    assert abs(df - b) <= 1e-12 * scale and abs(db - b) <= 1e-12 * scale
+
+
+def _pair_cases(n):
+    """nodal lines at scales from smooth (indicators far below the 1e-6 of the
+    weights) through eps-dominated to rough, with kinks and random data"""
+    out = []
+    for _ in range(n):
+        h = RNG.uniform(0.005, 0.2)
+        x = (np.arange(7) - 3) * h
+        kind = RNG.integers(4)
+        if kind == 0:  # linear plus noise of amplitude 10^k * h (k = -6..0)
+            p = RNG.normal() + RNG.normal() * x + 10.0 ** RNG.integers(-6, 1) * h * RNG.uniform(-1, 1, 7)
+        elif kind == 1:  # kink
+            s = RNG.integers(0, 7)
+            p = np.where(np.arange(7) < s, RNG.normal() * x, RNG.normal() * x) + RNG.normal()
+        elif kind == 2:  # smooth
+            p = np.sin(RNG.normal() * 3 * x + RNG.normal()) * 10.0 ** RNG.integers(-3, 3)
+        else:  # random
+            p = RNG.normal(size=7) * 10.0 ** RNG.integers(-3, 3)
+        out.append((p, h))
+    return out
+
+
+def test_weno5_pair_matches_both_stencil_weight_forms():
+    """the Jiang-Peng pair of the algorithms against the reference's own
+    evaluation order and the single-division form, on the same divided
+    differences: a dropped term, a wrong index or a wrong eps scaling shows"""
+    import ctypes
+
+    ref = po.lib().or_weno_onesided_reference
+    ref.restype = ctypes.c_double
+    for p, h in _pair_cases(400):
+        v = np.diff(p) / h
+        df, db = po.weno_pair(p, h)
+        scale = max(1.0, np.abs(v).max())
+        want_b = ref(*[ctypes.c_double(t) for t in v[:5]])
+        want_f = ref(*[ctypes.c_double(t) for t in (v[5], v[4], v[3], v[2], v[1])])
+        assert abs(db - want_b) <= 1e-13 * scale, (p, h, db, want_b)
+        assert abs(df - want_f) <= 1e-13 * scale, (p, h, df, want_f)
+        assert abs(db - po.weno_onesided(*v[:5])) <= 1e-13 * scale
+        assert abs(df - po.weno_onesided(v[5], v[4], v[3], v[2], v[1])) <= 1e-13 * scale
+
+
 def test_weno5_single_division_form_matches_reference_order():
     import ctypes
 
