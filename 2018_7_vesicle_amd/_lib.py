@@ -63,6 +63,8 @@ SIGNATURES = {
     "vls_batch_get_blocks": [P, P],
     "vls_batch_set_tile_rows": [P, I],
     "vls_batch_get_tile_rows": [P, P],
+    "vls_batch_set_zmarch_depth": [P, I],
+    "vls_batch_get_zmarch_depth": [P, P],
     "vls_sdf3_set_tile_rows": [P, I],
     "vls_sdf3_set_zmarch_depth": [P, I],
     "vls_sdf3_get_launch": [P, P, P],

@@ -112,9 +112,21 @@ class Batch:
     def tile_rows(self, value):
         call("vls_batch_set_tile_rows", self._h, int(value))
 
+    @property
+    def zmarch_depth(self):
+        """pages per work item of the z-marching bulk stage kernels"""
+        d = ctypes.c_int()
+        call("vls_batch_get_zmarch_depth", self._h, ctypes.byref(d))
+        return d.value
+
+    @zmarch_depth.setter
+    def zmarch_depth(self, value):
+        call("vls_batch_set_zmarch_depth", self._h, int(value))
+
     def autotune(self, F, iterations=2):
-        """pick the fastest tile shape and persistent block count for extension
-        stages off F's interface; returns (tile_rows, blocks).  Synchronizes."""
+        """pick the fastest tile shape and z-march depth for extension stages off
+        F's interface (see zmarch_depth); returns (tile_rows, blocks).
+        Synchronizes."""
         tr, nb = ctypes.c_int(), ctypes.c_int()
         call("vls_batch_autotune" + self._sfx, self._h, self._f(F), int(iterations), ctypes.byref(tr),
              ctypes.byref(nb), _stream())
@@ -276,7 +288,7 @@ class BatchPipeline:
         b.curl_velocity(torch.from_numpy(modes).to(dev), lo, *self.u)
         b.set_distance(self.F)
         b.reinitialize(self.F, cfg.initial_reinit)
-        self.tuned = b.autotune(self.F) if cfg.autotune else None
+        self.tuned = (*b.autotune(self.F), b.zmarch_depth) if cfg.autotune else None
         self.dt = b.cfl_dt(*self.u, cfg.cfl)
         self.Vn = b.zeros()
         self._av = torch.zeros((b.ngrids, 2), dtype=torch.float64, device=b.device)  # the step's output row

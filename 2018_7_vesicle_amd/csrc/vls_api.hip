@@ -939,6 +939,9 @@ struct vls_batch {
     int64_t total = 0;
     int blocks = 0;  // persistent blocks per tile-loop launch
     int tr = 64;     // rows per tile
+    int zd = 8;      // pages per work item of the z-marching bulk stage kernels
+    int nitems = 0;
+    ZItem* d_items = nullptr;
     std::vector<BGrid> grids;
     std::vector<int64_t> offs;
     BGrid* d_grids = nullptr;
@@ -958,9 +961,13 @@ struct vls_batch {
     GraphCache graphs;
     Fork fork{nullptr, nullptr, nullptr};
 
-    BatchView view() const { return BatchView{d_grids, d_tile_grid, d_offs, ngrids, ntiles, blocks, tr}; }
+    BatchView view() const
+    {
+        return BatchView{d_grids, d_tile_grid, d_offs, ngrids, ntiles, blocks, tr, d_items, nitems};
+    }
     ~vls_batch()
     {
+        (void)hipFree(d_items);
         (void)hipFree(d_grids);
         (void)hipFree(d_tile_grid);
         (void)hipFree(d_offs);
@@ -999,13 +1006,32 @@ int build_tiles(vls_batch* b, int tr)
         if (tiles >= ((int64_t)1 << 31)) return fail(VLS_ERR_INVALID, "too many tiles");
         tile_grid.insert(tile_grid.end(), (size_t)G.ntiles, g);
     }
+    // z-march work list: per grid, per chunk of zd pages, the tile columns
+    // (neighbouring columns adjacent, so consecutive blocks share halos)
+    std::vector<ZItem> items;
+    const int tc = 256 / tr;
+    for (int g = 0; g < b->ngrids; ++g) {
+        const BGrid& G = b->grids[(size_t)g];
+        const int ntr = (G.rows + tr - 1) / tr, ntc = (G.cols + tc - 1) / tc;
+        for (int p0 = 0; p0 < G.pges; p0 += b->zd)
+            for (int cb = 0; cb < ntc; ++cb)
+                for (int rb = 0; rb < ntr; ++rb)
+                    items.push_back(ZItem{g, rb * tr, cb * tc, p0, std::min(p0 + b->zd, G.pges)});
+        if (items.size() >= ((size_t)1 << 31)) return fail(VLS_ERR_INVALID, "too many work items");
+    }
     (void)hipFree(b->d_tile_grid);
     b->d_tile_grid = nullptr;
+    (void)hipFree(b->d_items);
+    b->d_items = nullptr;
     if (hipMalloc((void**)&b->d_tile_grid, sizeof(int) * (size_t)tiles) != hipSuccess)
         return fail(VLS_ERR_ALLOC, "hipMalloc of the tile list failed");
+    if (hipMalloc((void**)&b->d_items, sizeof(ZItem) * items.size()) != hipSuccess)
+        return fail(VLS_ERR_ALLOC, "hipMalloc of the work list failed");
     VLS_HIP(hipMemcpy(b->d_tile_grid, tile_grid.data(), sizeof(int) * (size_t)tiles, hipMemcpyHostToDevice));
+    VLS_HIP(hipMemcpy(b->d_items, items.data(), sizeof(ZItem) * items.size(), hipMemcpyHostToDevice));
     VLS_HIP(hipMemcpy(b->d_grids, b->grids.data(), sizeof(BGrid) * (size_t)b->ngrids, hipMemcpyHostToDevice));
     b->ntiles = (int)tiles;
+    b->nitems = (int)items.size();
     b->tr = tr;
     return VLS_OK;
 }
@@ -1242,6 +1268,26 @@ int vls_batch_get_tile_rows(vls_batch_t b, int* tile_rows)
     return VLS_OK;
 }
 
+int vls_batch_set_zmarch_depth(vls_batch_t b, int depth)
+{
+    VLS_REQUIRE(b, "null handle");
+    VLS_REQUIRE(depth > 0, "depth must be positive");
+    if (depth == b->zd) return VLS_OK;
+    VLS_HIP(hipDeviceSynchronize());  // no launch of this batch may still read the old work list
+    b->zd = depth;
+    const int rc = build_tiles(b, b->tr);
+    if (rc) return rc;
+    b->graphs.clear();
+    return VLS_OK;
+}
+
+int vls_batch_get_zmarch_depth(vls_batch_t b, int* depth)
+{
+    VLS_REQUIRE(b && depth, "null argument");
+    *depth = b->zd;
+    return VLS_OK;
+}
+
 #define VLS_BATCH_ENTRIES(T, SFX)                                                                                   \
     int vls_batch_set_distance##SFX(vls_batch_t b, const T* F, vls_stream_t stream)                                \
     {                                                                                                              \
@@ -1353,21 +1399,21 @@ struct EventTimer {
 template <typename T>
 int batch_autotune(vls_batch* b, const T* F, int iterations, hipStream_t s, int* tr_out, int* blocks_out)
 {
-    const int cus = cu_count();
     T* C = nullptr;
     if (hipMalloc((void**)&C, sizeof(T) * (size_t)b->total) != hipSuccess)
         return fail(VLS_ERR_ALLOC, "hipMalloc of the autotuning field failed");
     EventTimer ev;
     int rc = VLS_OK;
     float best = 1e30f;
-    int best_tr = b->tr, best_blocks = b->blocks;
+    int best_tr = b->tr, best_zd = b->zd;
     const int trs[2] = {64, 32};
-    const int per_cu[4] = {2, 4, 8, 16};
+    const int depths[3] = {4, 8, 16};
     if (hipEventCreate(&ev.a) != hipSuccess || hipEventCreate(&ev.b) != hipSuccess) rc = fail(VLS_ERR_HIP, "hipEventCreate");
     for (int i = 0; i < 2 && rc == VLS_OK; ++i) {
-        if ((rc = build_tiles(b, trs[i]))) break;
-        for (int j = 0; j < 4 && rc == VLS_OK; ++j) {
-            b->blocks = std::max(8, std::min(per_cu[j] * cus, (b->ntiles + 7) / 8 * 8) / 8 * 8);
+        for (int j = 0; j < 3 && rc == VLS_OK; ++j) {
+            b->zd = depths[j];
+            if ((rc = build_tiles(b, trs[i]))) break;
+            b->blocks = default_blocks(b->ntiles);
             if (hipMemcpyAsync(C, F, sizeof(T) * (size_t)b->total, hipMemcpyDeviceToDevice, s) != hipSuccess) {
                 rc = fail(VLS_ERR_HIP, "hipMemcpyAsync");
                 break;
@@ -1385,15 +1431,17 @@ int batch_autotune(vls_batch* b, const T* F, int iterations, hipStream_t s, int*
             if (ms < best) {
                 best = ms;
                 best_tr = trs[i];
-                best_blocks = b->blocks;
+                best_zd = depths[j];
             }
         }
     }
     (void)hipStreamSynchronize(s);
     (void)hipFree(C);
     if (rc) return rc;
+    b->zd = best_zd;
     if ((rc = build_tiles(b, best_tr))) return rc;
-    b->blocks = best_blocks;
+    b->blocks = default_blocks(b->ntiles);
+    const int best_blocks = b->blocks;
     b->graphs.clear();
     if (tr_out) *tr_out = best_tr;
     if (blocks_out) *blocks_out = best_blocks;

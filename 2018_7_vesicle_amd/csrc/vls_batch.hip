@@ -32,18 +32,6 @@ __device__ __forceinline__ Inv inv_of(const BGrid& G)
     return v;
 }
 
-// the tile-th tile of a batch: its grid's box, spacings and offset
-template <int R, typename T, bool VEL>
-__device__ __forceinline__ void batch_tile(Staged<R, T, VEL>& st, const BatchView& B, int tile)
-{
-    st.g = B.tile_grid[tile];
-    const BGrid G = B.grids[st.g];
-    st.b = box_of(G);
-    st.t = tile_at<R>(st.b, tile - G.tile0);
-    st.inv = inv_of(G);
-    st.off = G.off;
-}
-
 // grid of a packed element index: offs[g] <= gind < offs[g+1]
 __device__ __forceinline__ int grid_of(const BatchView& B, int64_t gind)
 {
@@ -141,6 +129,9 @@ __global__ void kb_set_mask_bit(uint8_t* __restrict__ flags, const T* __restrict
 // ---------------------------------------------------------------------------
 // fused stage kernels
 // ---------------------------------------------------------------------------
+// z-marching bulk stage kernels: one block per work item (a tile column of
+// one grid over a range of its pages), the single-grid kernels' walk and
+// per-point bodies with the item's grid
 template <int R, typename T>
 __global__ void __launch_bounds__(256) kb_reinit_bulk(T* __restrict__ out, const T* __restrict__ fs,
                                                       const T* __restrict__ fn, const uint8_t* __restrict__ flags,
@@ -149,15 +140,21 @@ __global__ void __launch_bounds__(256) kb_reinit_bulk(T* __restrict__ out, const
 {
     __shared__ T lds[2][Shape<R>::N];
     if (blockIdx.x == 0 && threadIdx.x == 0) over_count[(seq + 1) & 1] = 0;
-    using St = Staged<R, T, false>;
-    const TileRange tr = tile_range(B.ntiles);
-    tile_pipeline<R, T, false>(
-        lds[0], lds[1], tr.first, tr.end, tr.step, fs, fn, nullptr, nullptr, nullptr, flags, stage,
-        [&](int tile, St& st) { batch_tile<R>(st, B, tile); },
-        [&](const T* l, const St& st) {
-            const bool defer = reinit_bulk_point<R>(out + st.off, l, st.q, st.t, st.inv, stage);
-            wave_append(defer, (int)(st.off + st.t.page + st.t.in), over, &over_count[seq & 1]);
-        });
+    const ZItem it = B.items[xcd_remap((int)blockIdx.x, (int)gridDim.x)];
+    const BGrid G = B.grids[it.g];
+    const Box b = box_of(G);
+    const Inv invs = inv_of(G);
+    const idx_t off = G.off, plane = (idx_t)b.rows * b.cols;
+    zmarch<R, T, false>(lds[0], lds[1], b, it.r0, it.c0, it.p0, it.p1, fs + off, shift(fn, off), nullptr, nullptr,
+                        nullptr, flags + off, stage,
+                        [&](const T* l, const ZCol<R>& zc, const T w[7], const ZStep<R, T, false>& st, int p) {
+                            bool cross;
+                            const T res = reinit_value<R>(l, zc.base, w, st.fn, st.fl, invs, stage, cross);
+                            const bool bulk = !(st.fl & FLAG_SUBCELL);
+                            const idx_t ind = (idx_t)p * plane + zc.in;
+                            if (bulk && !cross) out[off + ind] = res;
+                            wave_append(bulk && cross, (int)(off + ind), over, &over_count[seq & 1]);
+                        });
 }
 
 template <typename T>
@@ -191,12 +188,18 @@ __global__ void __launch_bounds__(256) kb_extend_bulk(T* __restrict__ out, const
                                                       const uint8_t* __restrict__ flags, BatchView B, int stage)
 {
     __shared__ T lds[2][Shape<R>::N];
-    using St = Staged<R, T, true>;
-    const TileRange tr = tile_range(B.ntiles);
-    tile_pipeline<R, T, true>(
-        lds[0], lds[1], tr.first, tr.end, tr.step, cs, cn, vx, vy, vz, flags, stage,
-        [&](int tile, St& st) { batch_tile<R>(st, B, tile); },
-        [&](const T* l, const St& st) { extend_bulk_point<R>(out + st.off, l, st.q, st.t, st.inv, stage); });
+    const ZItem it = B.items[xcd_remap((int)blockIdx.x, (int)gridDim.x)];
+    const BGrid G = B.grids[it.g];
+    const Box b = box_of(G);
+    const Inv inv = inv_of(G);
+    const idx_t off = G.off, plane = (idx_t)b.rows * b.cols;
+    T* o = out + off;
+    zmarch<R, T, true>(lds[0], lds[1], b, it.r0, it.c0, it.p0, it.p1, cs + off, shift(cn, off), vx + off, vy + off,
+                       vz + off, flags + off, stage,
+                       [&](const T* l, const ZCol<R>& zc, const T w[7], const ZStep<R, T, true>& st, int p) {
+                           const T res = upwind_value<R>(l, zc.base, w, st.vel, st.fn, (T)inv.dt_bulk, inv, stage);
+                           if (!(st.fl & FLAG_SUBCELL)) stp(o + (idx_t)p * plane, zc.in, res);
+                       });
 }
 
 template <typename T>
@@ -227,12 +230,19 @@ __global__ void __launch_bounds__(256) kb_advect_stage(T* __restrict__ out, cons
                                                        const double* __restrict__ dt, BatchView B, int stage)
 {
     __shared__ T lds[2][Shape<R>::N];
-    using St = Staged<R, T, true>;
-    const TileRange tr = tile_range(B.ntiles);
-    tile_pipeline<R, T, true>(
-        lds[0], lds[1], tr.first, tr.end, tr.step, fs, fn, vx, vy, vz, nullptr, stage,
-        [&](int tile, St& st) { batch_tile<R>(st, B, tile); },
-        [&](const T* l, const St& st) { advect_point<R>(out + st.off, l, st.q, dt[st.g], st.t, st.inv, stage); });
+    const ZItem it = B.items[xcd_remap((int)blockIdx.x, (int)gridDim.x)];
+    const BGrid G = B.grids[it.g];
+    const Box b = box_of(G);
+    const Inv inv = inv_of(G);
+    const T dtg = (T)dt[it.g];
+    const idx_t off = G.off, plane = (idx_t)b.rows * b.cols;
+    T* o = out + off;
+    zmarch<R, T, true>(lds[0], lds[1], b, it.r0, it.c0, it.p0, it.p1, fs + off, shift(fn, off), vx + off, vy + off,
+                       vz + off, nullptr, stage,
+                       [&](const T* l, const ZCol<R>& zc, const T w[7], const ZStep<R, T, true>& st, int p) {
+                           const T res = upwind_value<R>(l, zc.base, w, st.vel, st.fn, dtg, inv, stage);
+                           if (zc.valid) stp(o + (idx_t)p * plane, zc.in, res);
+                       });
 }
 
 // ---------------------------------------------------------------------------
@@ -573,7 +583,7 @@ hipError_t launch_batch_reinit_stage(const BatchView& B, T* out, const T* fs, co
                                      const ListView& lv, int stage, int* over, int* over_count, int seq,
                                      hipStream_t s)
 {
-    TR_LAUNCH(kb_reinit_bulk, T, B.blocks, out, fs, fn, flags, B, stage, over, over_count, seq);
+    TR_LAUNCH(kb_reinit_bulk, T, B.nitems, out, fs, fn, flags, B, stage, over, over_count, seq);
     kb_reinit_iface<T><<<list_blocks(B), 256, 0, s>>>(out, fs, fn, flags, ilist(lv), B, stage, (const int*)over,
                                                       (const int*)over_count, seq);
     return hipGetLastError();
@@ -591,12 +601,12 @@ hipError_t launch_batch_extend_stage(const BatchView& B, T* out, const T* cs, co
         if ((e = hipStreamWaitEvent(fk->side, fk->fork, 0)) != hipSuccess) return e;
         kb_extend_iface<T><<<list_blocks(B), 256, 0, fk->side>>>(out, cs, cn, ilist(lv), lcp, lvel, B, stage);
         if ((e = hipGetLastError()) != hipSuccess) return e;
-        TR_LAUNCH(kb_extend_bulk, T, B.blocks, out, cs, cn, vx, vy, vz, flags, B, stage);
+        TR_LAUNCH(kb_extend_bulk, T, B.nitems, out, cs, cn, vx, vy, vz, flags, B, stage);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if ((e = hipEventRecord(fk->join, fk->side)) != hipSuccess) return e;
         return hipStreamWaitEvent(s, fk->join, 0);
     }
-    TR_LAUNCH(kb_extend_bulk, T, B.blocks, out, cs, cn, vx, vy, vz, flags, B, stage);
+    TR_LAUNCH(kb_extend_bulk, T, B.nitems, out, cs, cn, vx, vy, vz, flags, B, stage);
     kb_extend_iface<T><<<list_blocks(B), 256, 0, s>>>(out, cs, cn, ilist(lv), lcp, lvel, B, stage);
     return hipGetLastError();
 }
@@ -605,7 +615,7 @@ template <typename T>
 hipError_t launch_batch_advect_stage(const BatchView& B, T* out, const T* fs, const T* fn, const T* vx,
                                      const T* vy, const T* vz, const double* dt, int stage, hipStream_t s)
 {
-    TR_LAUNCH(kb_advect_stage, T, B.blocks, out, fs, fn, vx, vy, vz, dt, B, stage);
+    TR_LAUNCH(kb_advect_stage, T, B.nitems, out, fs, fn, vx, vy, vz, dt, B, stage);
     return hipGetLastError();
 }
 

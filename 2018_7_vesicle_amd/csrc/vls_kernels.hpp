@@ -181,6 +181,12 @@ struct BGrid {
 };
 
 // device view of a batch
+// one block's share of a z-marching batch launch: tile column (r0, c0) of
+// grid g over pages [p0, p1)
+struct ZItem {
+    int g, r0, c0, p0, p1;
+};
+
 struct BatchView {
     const BGrid* grids;    // ngrids descriptors
     const int* tile_grid;  // grid of each tile
@@ -188,6 +194,8 @@ struct BatchView {
     int ngrids, ntiles;
     int blocks;            // persistent blocks of a tile-loop launch (multiple of 8)
     int tr;                // rows per tile: 64 or 32
+    const ZItem* items;    // z-march work list of the bulk stage kernels
+    int nitems;
 };
 
 // tiles of one grid with `tr` rows per tile (tr x 256/tr columns)

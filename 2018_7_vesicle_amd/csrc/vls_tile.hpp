@@ -256,20 +256,13 @@ __device__ __forceinline__ TileRange tile_range(int ntiles)
     return r;
 }
 
-// this block's tile of a one-block-per-tile launch
-template <int R>
-__device__ __forceinline__ Tile tile_of(const Box& b)
-{
-    return tile_at<R>(b, xcd_remap((int)blockIdx.x, (int)gridDim.x));
-}
-
-// Staging a tile of page t.p of f with its clamped halo in lds[Shape<R>::N],
-// column-major (LR rows per column).  load_tile issues the global loads into
-// registers and store_tile writes them to LDS, so a kernel can issue every
-// other load of its points in between and wait for memory once.  Each thread
-// loads one row of the first TR staged rows in every TC-th staged column (the
-// column offsets are wave-uniform for R = 64), and threads below 6*LC one
-// element of the last 6 staged rows.
+// A staged tile: one page's values of a tile with its clamped 3-point halo
+// in lds[Shape<R>::N], column-major (LR rows per column).  The loads go to
+// registers first (zstep_load) and store_tile writes them to LDS, so a
+// kernel issues every other load of its points in between and waits for
+// memory once.  Each thread loads one row of the first TR staged rows in
+// every TC-th staged column (the column offsets are wave-uniform for R = 64),
+// and threads below 6*LC one element of the last 6 staged rows.
 template <int R, typename T>
 struct TileRegs {
     static constexpr int NK = (Shape<R>::LC + Shape<R>::TC - 1) / Shape<R>::TC;
@@ -282,26 +275,6 @@ __device__ __forceinline__ int tile_tx()
 {
     const int tx = (int)threadIdx.x / R;
     return (R % 64 == 0) ? __builtin_amdgcn_readfirstlane(tx) : tx;
-}
-
-template <int R, typename T>
-__device__ __forceinline__ void load_tile(TileRegs<R, T>& g, const T* __restrict__ f, const Box& b, const Tile& t)
-{
-    using S = Shape<R>;
-    const T* fp = f + t.page;
-    const int ty = (int)threadIdx.x % R, tx = tile_tx<R>();
-    const unsigned row = (unsigned)clampi(t.r0 - HALO + ty, b.rows - 1);
-#pragma unroll
-    for (int k = 0; k < TileRegs<R, T>::NK; ++k) {
-        const int lc = tx + S::TC * k;
-        if (lc < S::LC) g.m[k] = ldp(fp, (unsigned)clampi(t.c0 - HALO + lc, b.cols - 1) * (unsigned)b.rows + row);
-    }
-    const int e = (int)threadIdx.x;
-    if (e < 2 * HALO * S::LC) {
-        const int lc = e / (2 * HALO), lr = S::TR + e % (2 * HALO);
-        g.s = ldp(fp, (unsigned)clampi(t.c0 - HALO + lc, b.cols - 1) * (unsigned)b.rows +
-                          (unsigned)clampi(t.r0 - HALO + lr, b.rows - 1));
-    }
 }
 
 template <int R, typename T>
@@ -318,16 +291,6 @@ __device__ __forceinline__ void store_tile(T* lds, const TileRegs<R, T>& g)
     if (e < 2 * HALO * S::LC) lds[(e / (2 * HALO)) * S::LR + S::TR + e % (2 * HALO)] = g.s;
 }
 
-// stage and wait (ends with a barrier)
-template <int R, typename T>
-__device__ __forceinline__ void fill_tile(T* lds, const T* __restrict__ f, const Box& b, const Tile& t)
-{
-    TileRegs<R, T> g;
-    load_tile<R>(g, f, b, t);
-    store_tile<R>(lds, g);
-    __syncthreads();
-}
-
 template <typename T>
 __device__ __forceinline__ void tile_line_y(const T* lds, int base, T v[7])
 {
@@ -340,27 +303,6 @@ __device__ __forceinline__ void tile_line_x(const T* lds, int base, T v[7])
 {
 #pragma unroll
     for (int k = 0; k < 7; ++k) v[k] = lds[base + (k - 3) * Shape<R>::LR];
-}
-
-// z-taps of a valid point: pages p-3..p+3 (clamped), addressed from the
-// lowest one with 32-bit offsets; the centre v[3] is the staged tile's
-// (tile_line_z)
-template <typename T>
-__device__ __forceinline__ void load_zline(T v[7], const T* __restrict__ f, const Box& b, const Tile& t)
-{
-    const unsigned plane = (unsigned)b.rows * (unsigned)b.cols;
-    const int lo = clampi(t.p - 3, b.pges - 1);
-    const T* f0 = f + (idx_t)lo * plane;
-#pragma unroll
-    for (int k = 0; k < 7; ++k)
-        if (k != 3) v[k] = ldp(f0, t.in + (unsigned)(clampi(t.p + k - 3, b.pges - 1) - lo) * plane);
-}
-
-template <typename T>
-__device__ __forceinline__ void tile_line_z(const T* lds, int base, const T z[7], T v[7])
-{
-#pragma unroll
-    for (int k = 0; k < 7; ++k) v[k] = (k == 3) ? lds[base] : z[k];
 }
 
 // Interface work is spread over three lanes per point, one per axis (21
@@ -474,112 +416,6 @@ __device__ __forceinline__ void block_reduce_store(KSum (&k)[NQ], double* __rest
 // grid (offset into the packed fields for a batch), so a grid gives the same
 // result bit for bit either way.
 // ---------------------------------------------------------------------------
-
-// Per-point inputs of the bulk stage kernels: the z-taps of the stage field,
-// the velocity (extension, advection), the RK stage's start value and the
-// point's flags (FLAG_SUBCELL outside the box).  bulk_load issues the loads
-// without branches on loaded values; the bodies below evaluate every thread
-// and use the flags only for the Godunov branch and the store, so no load
-// waits for another.
-template <typename T, bool VEL>
-struct BulkIn {
-    T z[7];
-    T vel[VEL ? 3 : 1];
-    T fn;
-    uint8_t fl;
-};
-
-template <typename T, bool VEL>
-__device__ __forceinline__ void bulk_load(BulkIn<T, VEL>& q, const T* __restrict__ fs, const T* __restrict__ fn,
-                                          const T* __restrict__ vx, const T* __restrict__ vy,
-                                          const T* __restrict__ vz, const uint8_t* __restrict__ flags,
-                                          const Box& b, const Tile& t, int stage)
-{
-    // a thread outside the box reads its page's first point and drops it
-    const unsigned in = t.valid ? t.in : 0u;
-    const uint8_t f = VEL && !flags ? (uint8_t)0 : ldp(flags + t.page, in);
-    q.fl = t.valid ? f : (uint8_t)FLAG_SUBCELL;
-    if (VEL) {
-        q.vel[0] = ldp(vx + t.page, in);
-        q.vel[1] = ldp(vy + t.page, in);
-        q.vel[2] = ldp(vz + t.page, in);
-    }
-    // at stage 0 (no start value) the point's own stage value is read and dropped
-    const T f0 = ldp((stage != 0 ? fn : fs) + t.page, in);
-    q.fn = (stage != 0) ? f0 : (T)0;
-    Tile tz = t;
-    tz.in = in;
-    load_zline(q.z, fs, b, tz);
-}
-
-// One tile of a pipelined walk (tile_pipeline): where it lies and its loads.
-template <int R, typename T, bool VEL>
-struct Staged {
-    Tile t;
-    Box b;
-    Inv inv;
-    idx_t off;  // the tile's grid in packed fields (0 for a single grid)
-    int g;      // the tile's grid (0 for a single grid)
-    TileRegs<R, T> reg;
-    BulkIn<T, VEL> q;
-};
-
-// the tile-th tile of a single grid
-template <int R, typename T, bool VEL>
-__device__ __forceinline__ void single_grid_tile(Staged<R, T, VEL>& st, const Box& b, const Inv& inv, int tile)
-{
-    st.t = tile_at<R>(b, tile);
-    st.b = b;
-    st.inv = inv;
-    st.off = 0;
-    st.g = 0;
-}
-
-// Software-pipelined walk of a block over its tiles first, first + step, ...
-// below end: the loads of the next tile are issued before the current tile is
-// evaluated, so memory latency overlaps arithmetic.  Staged tiles alternate
-// between two LDS buffers and two register sets (the loop is unrolled by two,
-// so no registers are copied), with one barrier per tile: a wave that stages
-// tile i+1 has passed tile i's barrier, which every wave reaches only after
-// it finished evaluating tile i-1 from the other buffer.  locate(tile, s)
-// fills s.t, s.b, s.inv, s.off and s.g; body(lds, s) evaluates a staged tile.
-template <int R, typename T, bool VEL, class Locate>
-__device__ __forceinline__ void stage_tile(Staged<R, T, VEL>& s, int tile, const T* __restrict__ fs,
-                                           const T* __restrict__ fn, const T* __restrict__ vx,
-                                           const T* __restrict__ vy, const T* __restrict__ vz,
-                                           const uint8_t* __restrict__ flags, int stage, Locate locate)
-{
-    locate(tile, s);
-    load_tile<R>(s.reg, fs + s.off, s.b, s.t);
-    bulk_load(s.q, fs + s.off, fn ? fn + s.off : fn, VEL ? vx + s.off : vx, VEL ? vy + s.off : vy,
-              VEL ? vz + s.off : vz, flags ? flags + s.off : flags, s.b, s.t, stage);
-}
-
-template <int R, typename T, bool VEL, class Locate, class Body>
-__device__ __forceinline__ void tile_pipeline(T* lds0, T* lds1, int first, int end, int step,
-                                              const T* __restrict__ fs, const T* __restrict__ fn,
-                                              const T* __restrict__ vx, const T* __restrict__ vy,
-                                              const T* __restrict__ vz, const uint8_t* __restrict__ flags,
-                                              int stage, Locate locate, Body body)
-{
-    if (first >= end) return;
-    Staged<R, T, VEL> sa, sb;
-    stage_tile(sa, first, fs, fn, vx, vy, vz, flags, stage, locate);
-    for (int tile = first;;) {
-        store_tile<R>(lds0, sa.reg);
-        __syncthreads();
-        tile += step;
-        if (tile < end) stage_tile(sb, tile, fs, fn, vx, vy, vz, flags, stage, locate);
-        body(lds0, sa);
-        if (tile >= end) break;
-        store_tile<R>(lds1, sb.reg);
-        __syncthreads();
-        tile += step;
-        if (tile < end) stage_tile(sa, tile, fs, fn, vx, vy, vz, flags, stage, locate);
-        body(lds1, sb);
-        if (tile >= end) break;
-    }
-}
 
 // ---------------------------------------------------------------------------
 // z-marching walk (single grid): a block takes one tile column (TR rows x TC
@@ -750,23 +586,6 @@ __device__ __forceinline__ T reinit_value(const T* lds, int base, const T z[7], 
     return rk_combine(stage, z[3], step, fn);
 }
 
-// the same on a staged tile of tile_pipeline: writes out and returns false,
-// or returns true (defer to the interface pass) for a point next to a sign
-// change.  Points with subcell distances and points outside the box write
-// nothing and return false.
-template <int R, typename T>
-__device__ __forceinline__ bool reinit_bulk_point(T* __restrict__ out, const T* lds, const BulkIn<T, false>& q,
-                                                  const Tile& t, const Inv& invs, int stage)
-{
-    T z[7];
-    tile_line_z(lds, t.base, q.z, z);
-    bool cross;
-    const T res = reinit_value<R>(lds, t.base, z, q.fn, q.fl, invs, stage, cross);
-    if (q.fl & FLAG_SUBCELL) return false;
-    if (!cross) stp(out + t.page, t.in, res);
-    return cross;
-}
-
 // one lane (point, axis) of the reinitialization interface pass
 struct ReinitLane {
     double m = 0.0;   // this axis's Godunov term
@@ -841,18 +660,6 @@ __device__ __forceinline__ T upwind_value(const T* lds, int base, const T z[7], 
     return rk_combine(stage, z[3], acc * scale, fn);
 }
 
-// WENO5-RK3 extension stage, bulk point of a staged tile of tile_pipeline
-// (points with subcell distances are left to the interface pass)
-template <int R, typename T>
-__device__ __forceinline__ void extend_bulk_point(T* __restrict__ out, const T* lds, const BulkIn<T, true>& q,
-                                                  const Tile& t, const Inv& inv, int stage)
-{
-    T z[7];
-    tile_line_z(lds, t.base, q.z, z);
-    const T res = upwind_value<R>(lds, t.base, z, q.vel, q.fn, (T)inv.dt_bulk, inv, stage);
-    if (!(q.fl & FLAG_SUBCELL)) stp(out + t.page, t.in, res);
-}
-
 // one lane (point, axis) of the extension interface pass
 struct ExtendLane {
     double t = 0.0;    // this axis's term coef * derivative
@@ -903,17 +710,6 @@ __device__ __forceinline__ void extend_iface_finish(T* __restrict__ out, const T
         const double step = (q.dt >= 0) ? ((q.t + ty) + tz) * q.dt : 0.0;
         out[q.gind] = (T)rk_combine(stage, (double)cs[q.gind], step, (stage == 0) ? 0.0 : (double)cn[q.gind]);
     }
-}
-
-// TVD-RK3 advection stage, point of a staged tile of tile_pipeline
-template <int R, typename T>
-__device__ __forceinline__ void advect_point(T* __restrict__ out, const T* lds, const BulkIn<T, true>& q,
-                                             double dt, const Tile& t, const Inv& inv, int stage)
-{
-    T z[7];
-    tile_line_z(lds, t.base, q.z, z);
-    const T res = upwind_value<R>(lds, t.base, z, q.vel, q.fn, (T)dt, inv, stage);
-    if (!(q.fl & FLAG_SUBCELL)) stp(out + t.page, t.in, res);
 }
 
 }  // namespace vls

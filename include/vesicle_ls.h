@@ -262,6 +262,11 @@ int vls_batch_get_blocks(vls_batch_t b, int *blocks);
  * the default is the one with fewer tiles; cached graphs are dropped */
 int vls_batch_set_tile_rows(vls_batch_t b, int tile_rows);
 int vls_batch_get_tile_rows(vls_batch_t b, int *tile_rows);
+/* pages per work item of the batch's z-marching bulk stage kernels (each
+ * block walks one tile column of one grid over `depth` pages; default 8);
+ * results do not depend on it.  Synchronizes the device. */
+int vls_batch_set_zmarch_depth(vls_batch_t b, int depth);
+int vls_batch_get_zmarch_depth(vls_batch_t b, int *depth);
 
 int vls_batch_set_distance(vls_batch_t b, const double *F, vls_stream_t stream);
 int vls_batch_weno5_rk3_reinit(vls_batch_t b, double *F, int iterations, vls_stream_t stream);
@@ -314,9 +319,10 @@ int vls_batch_step_f32(vls_batch_t b, float *F, float *Vn, const float *vx, cons
 /* ------------------------------------------------------------------------ */
 /* Times `iterations` extension iterations of a scratch copy of F, off F's
  * interface, for each candidate launch shape and keeps the fastest: tile rows
- * 64 or 32 and, for a batch, 2/4/8/16 persistent blocks per CU, for a single
- * grid the z-march depth chosen by the box size, half or twice it (read it
- * back with vls_sdf3_get_launch).  Results of
+ * 64 or 32 and the z-march depth (a batch: 4, 8 or 16 pages; a single grid:
+ * the depth chosen by the box size, half or twice it; read back with
+ * vls_batch_get_zmarch_depth / vls_sdf3_get_launch).  A batch's persistent
+ * blocks are left at 8 per CU.  Results of
  * every method are the same for every shape.  Synchronizes.  The handle is
  * left with F's distances (as after set_distance(F)); a batch's cached graphs
  * are dropped.  tile_rows and blocks (may be null) receive the choice. */

@@ -122,10 +122,14 @@ def test_batch_step_equals_single_grid_sequence(dtype):
 
 
 def test_persistent_launch_shape_does_not_change_results():
+    """persistent block count of the tile-loop kernels and z-march depth of
+    the bulk stage kernels"""
     outs = []
-    for blocks in (8, 64, 0):
+    for blocks, depth in ((8, 8), (64, 1), (0, 5), (0, 40)):
         b, grids, Gs, F, C, u = _setup(torch.float32)
         b.blocks = blocks
+        b.zmarch_depth = depth
+        assert b.zmarch_depth == depth
         b.set_distance(F)
         b.reinitialize(F, 3)
         b.extend(F, C, 3)
