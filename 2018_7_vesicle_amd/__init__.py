@@ -10,6 +10,6 @@ from .sdf3 import (GD3, SDF3, TOOLBOX_FIELDS, feval_advection_step, feval_bounda
                    feval_boundary_interpolant, feval_boundary_location, feval_eno2_re_step, feval_extend_step,
                    feval_re_step, feval_set_calculus_toolbox, feval_upwind_normal, feval_weno_derivative)
 from . import pipeline  # noqa: F401
-from .batch import Batch, BatchConfig, BatchPipeline, set_graphs, set_overlap  # noqa: F401
+from .batch import Batch, BatchConfig, BatchPipeline, set_graphs  # noqa: F401
 
 __version__ = "0.1.0"

@@ -194,14 +194,6 @@ class Batch:
              self._d(av_row, 2 * self.ngrids), _stream())
 
 
-def set_overlap(enable):
-    """run each extension stage's interface pass on a side stream beside its
-    bulk pass (on) or after it (off); returns the previous setting"""
-    from ._lib import lib
-
-    return bool(lib().vls_set_overlap(1 if enable else 0))
-
-
 def set_graphs(enable):
     """turn hipGraph capture of the wrapper-level loops on or off; returns the previous setting"""
     from ._lib import lib

@@ -100,37 +100,6 @@ Dist6c mk6c(const double* a, const double* b, const double* c, const double* d, 
 // captured work has no memset nodes: counters are cleared by kernels.
 // ---------------------------------------------------------------------------
 int g_graphs = 1;   // vls_set_graphs
-int g_overlap = 0;  // vls_set_overlap (off: measured no gain, see vls_set_overlap)
-
-// the side stream and events of a handle's overlapped interface passes,
-// created on first use; nullptr when overlap is off
-const Fork* fork_of(Fork& f)
-{
-    if (!g_overlap) return nullptr;
-    if (!f.side) {
-        if (hipStreamCreateWithFlags(&f.side, hipStreamNonBlocking) != hipSuccess) {
-            f.side = nullptr;
-            return nullptr;
-        }
-        if (hipEventCreateWithFlags(&f.fork, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&f.join, hipEventDisableTiming) != hipSuccess) {
-            (void)hipStreamDestroy(f.side);
-            f.side = nullptr;
-            return nullptr;
-        }
-    }
-    return &f;
-}
-
-void destroy_fork(Fork& f)
-{
-    if (f.side) {
-        (void)hipStreamDestroy(f.side);
-        (void)hipEventDestroy(f.fork);
-        (void)hipEventDestroy(f.join);
-        f.side = nullptr;
-    }
-}
 
 struct GraphKey {
     uint64_t v[16] = {};
@@ -187,7 +156,6 @@ int with_graph(GraphCache& gc, const GraphKey& key0, hipStream_t s, Issue&& issu
 {
     if (!g_graphs) return issue(s);
     GraphKey key = key0;
-    key.v[15] = (uint64_t)g_overlap;  // the graph's shape depends on it
     for (auto& e : gc.entries)
         if (e.key == key) {
             e.used = ++gc.tick;
@@ -253,7 +221,6 @@ struct vls_sdf3 {
     GraphCache graphs;
     double* adj = nullptr;  // adjoint/tangent workspace, grown on demand
     size_t adj_len = 0;
-    Fork fork{nullptr, nullptr, nullptr};
 };
 
 namespace {
@@ -307,11 +274,10 @@ int extend_loop(vls_sdf3* h, T* C, int iterations, const DSet& ds, hipStream_t s
     T* t1 = tmp<T>(h, 1);
     const T *vx = vel<T>(h, 0), *vy = vel<T>(h, 1), *vz = vel<T>(h, 2);
     VLS_HIP(launch_list_interp(h->lcp, ds.lv, (const T*)C, g, s));
-    const Fork* fk = fork_of(h->fork);
     for (int it = 0; it < iterations; ++it) {
-        VLS_HIP(launch_extend_stage<T>(t0, C, nullptr, vx, vy, vz, ds.flags, ds.lv, h->lcp, h->lvel, g, 0, s, fk));
-        VLS_HIP(launch_extend_stage<T>(t1, t0, C, vx, vy, vz, ds.flags, ds.lv, h->lcp, h->lvel, g, 1, s, fk));
-        VLS_HIP(launch_extend_stage<T>(C, t1, C, vx, vy, vz, ds.flags, ds.lv, h->lcp, h->lvel, g, 2, s, fk));
+        VLS_HIP(launch_extend_stage<T>(t0, C, nullptr, vx, vy, vz, ds.flags, ds.lv, h->lcp, h->lvel, g, 0, s));
+        VLS_HIP(launch_extend_stage<T>(t1, t0, C, vx, vy, vz, ds.flags, ds.lv, h->lcp, h->lvel, g, 1, s));
+        VLS_HIP(launch_extend_stage<T>(C, t1, C, vx, vy, vz, ds.flags, ds.lv, h->lcp, h->lvel, g, 2, s));
     }
     return VLS_OK;
 }
@@ -524,12 +490,6 @@ int vls_set_graphs(int enable)
     return prev;
 }
 
-int vls_set_overlap(int enable)
-{
-    const int prev = g_overlap;
-    g_overlap = enable ? 1 : 0;
-    return prev;
-}
 const char* vls_version(void) { return "vesicle_ls 0.1 (gfx950, FP64)"; }
 
 // ---------------------------------------------------------------------------
@@ -778,7 +738,6 @@ int vls_sdf3_destroy(vls_sdf3_t h)
 {
     if (!h) return VLS_OK;
     (void)hipFree(h->adj);
-    destroy_fork(h->fork);
     hipError_t e1 = hipFree(h->pool), e2 = hipFree(h->flag_pool), e3 = hipFree(h->int_pool);
     delete h;
     if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) return fail(VLS_ERR_HIP, "hipFree failed");
@@ -952,18 +911,18 @@ struct vls_batch {
     int* int_pool = nullptr;
     ListView lv{};
     int* over = nullptr;
-    int* counts = nullptr;  // [0] list count, [4..5] deferred counts
+    int* list_grid = nullptr;  // grid of each interface-list entry
+    int* counts = nullptr;     // [0] list count, [4..5] deferred counts
     double* lcp = nullptr;
     double* lvel = nullptr;
     double* t[2] = {nullptr, nullptr};
     double* v[3] = {nullptr, nullptr, nullptr};
     double* partial = nullptr;  // per-tile partial sums (4 per tile of either shape)
     GraphCache graphs;
-    Fork fork{nullptr, nullptr, nullptr};
 
     BatchView view() const
     {
-        return BatchView{d_grids, d_tile_grid, d_offs, ngrids, ntiles, blocks, tr, d_items, nitems};
+        return BatchView{d_grids, d_tile_grid, d_offs, ngrids, ntiles, blocks, tr, d_items, nitems, list_grid};
     }
     ~vls_batch()
     {
@@ -975,7 +934,6 @@ struct vls_batch {
         (void)hipFree(flags);
         (void)hipFree(int_pool);
         (void)hipFree(partial);
-        destroy_fork(fork);
     }
 };
 
@@ -1083,12 +1041,10 @@ int batch_extend(vls_batch* b, const T* F, T* C, int iterations, hipStream_t s)
     VLS_HIP(launch_batch_extension_velocity<T>(B, vx, vy, vz, F, b->flags, b->lv, 1e-6, s));
     VLS_HIP(launch_batch_list_gather_vel<T>(b->lvel, b->lv, vx, vy, vz, s));
     VLS_HIP(launch_batch_list_interp<T>(B, b->lcp, b->lv, C, s));
-    const Fork* fk = fork_of(b->fork);
     for (int it = 0; it < iterations; ++it) {
-        VLS_HIP(launch_batch_extend_stage<T>(B, t0, C, nullptr, vx, vy, vz, b->flags, b->lv, b->lcp, b->lvel, 0, s,
-                                             fk));
-        VLS_HIP(launch_batch_extend_stage<T>(B, t1, t0, C, vx, vy, vz, b->flags, b->lv, b->lcp, b->lvel, 1, s, fk));
-        VLS_HIP(launch_batch_extend_stage<T>(B, C, t1, C, vx, vy, vz, b->flags, b->lv, b->lcp, b->lvel, 2, s, fk));
+        VLS_HIP(launch_batch_extend_stage<T>(B, t0, C, nullptr, vx, vy, vz, b->flags, b->lv, b->lcp, b->lvel, 0, s));
+        VLS_HIP(launch_batch_extend_stage<T>(B, t1, t0, C, vx, vy, vz, b->flags, b->lv, b->lcp, b->lvel, 1, s));
+        VLS_HIP(launch_batch_extend_stage<T>(B, C, t1, C, vx, vy, vz, b->flags, b->lv, b->lcp, b->lvel, 2, s));
     }
     return VLS_OK;
 }
@@ -1181,7 +1137,7 @@ int vls_batch_create(vls_batch_t* out, int ngrids, const int* rows, const int* c
     if (hipMalloc((void**)&b->pool, sizeof(double) * dbl) != hipSuccess ||
         hipMalloc((void**)&b->partial, sizeof(double) * ((size_t)max_tiles * 4 + 8)) != hipSuccess ||
         hipMalloc((void**)&b->flags, (size_t)n) != hipSuccess ||
-        hipMalloc((void**)&b->int_pool, sizeof(int) * ((size_t)n * 2 + 16)) != hipSuccess ||
+        hipMalloc((void**)&b->int_pool, sizeof(int) * ((size_t)n * 3 + 16)) != hipSuccess ||
         hipMalloc((void**)&b->d_grids, sizeof(BGrid) * (size_t)ngrids) != hipSuccess ||
         hipMalloc((void**)&b->d_offs, sizeof(int64_t) * ((size_t)ngrids + 1)) != hipSuccess) {
         delete b;
@@ -1210,7 +1166,8 @@ int vls_batch_create(vls_batch_t* out, int ngrids, const int* rows, const int* c
     for (int a = 0; a < 3; ++a) { b->v[a] = p; p += n; }
     b->lv.idx = b->int_pool;
     b->over = b->int_pool + n;
-    b->counts = b->int_pool + 2 * n;
+    b->list_grid = b->int_pool + 2 * n;
+    b->counts = b->int_pool + 3 * n;
     b->lv.count = b->counts;
     b->lv.cap = (int)n;
     *out = b;

@@ -69,7 +69,8 @@ __global__ void __launch_bounds__(256) kb_build_list(const T* __restrict__ F, co
 {
     const TileRange tr = tile_range(B.ntiles);
     for (int tile = tr.first; tile < tr.end; tile += tr.step) {
-        const BGrid G = B.grids[B.tile_grid[tile]];
+        const int g = B.tile_grid[tile];
+        const BGrid G = B.grids[g];
         const Box b = box_of(G);
         const Tile t = tile_at<R>(b, tile - G.tile0);
         const T* f = F + G.off;
@@ -106,6 +107,7 @@ __global__ void __launch_bounds__(256) kb_build_list(const T* __restrict__ F, co
         const int slot = wave_append_slot(sub, il.count);
         if (sub) {
             il.idx[slot] = (int)(G.off + ind);
+            B.list_grid[slot] = g;
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 il.dist[(size_t)(2 * a) * il.cap + slot] = q.r[a];
@@ -129,66 +131,107 @@ __global__ void kb_set_mask_bit(uint8_t* __restrict__ flags, const T* __restrict
 // ---------------------------------------------------------------------------
 // fused stage kernels
 // ---------------------------------------------------------------------------
-// z-marching bulk stage kernels: one block per work item (a tile column of
-// one grid over a range of its pages), the single-grid kernels' walk and
-// per-point bodies with the item's grid
+// Fused stage kernels (one launch per stage, see k_reinit_stage and
+// k_extend_stage): the first niface blocks walk the interface list, the
+// others take one z-march work item each (a tile column of one grid over a
+// range of its pages) with the single-grid kernels' walk and per-point
+// bodies, called with the item's grid.
 template <int R, typename T>
-__global__ void __launch_bounds__(256) kb_reinit_bulk(T* __restrict__ out, const T* __restrict__ fs,
-                                                      const T* __restrict__ fn, const uint8_t* __restrict__ flags,
-                                                      BatchView B, int stage, int* __restrict__ over,
-                                                      int* __restrict__ over_count, int seq)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(stage_waves<T>()))) kb_reinit_stage(T* __restrict__ out, const T* __restrict__ fs,
+                                                       const T* __restrict__ fn, const uint8_t* __restrict__ flags,
+                                                       IList il, BatchView B, int stage, int* __restrict__ over,
+                                                       int* __restrict__ over_count, int seq, int niface)
 {
     __shared__ T lds[2][Shape<R>::N];
-    if (blockIdx.x == 0 && threadIdx.x == 0) over_count[(seq + 1) & 1] = 0;
-    const ZItem it = B.items[xcd_remap((int)blockIdx.x, (int)gridDim.x)];
+    const int blk = (int)blockIdx.x;
+    if (blk == 0 && threadIdx.x == 0) over_count[(seq + 1) & 1] = 0;
+    if (blk < niface) {
+        const int total = *il.count;
+        for (int j0 = 0; j0 < total; j0 += axis_lane_stride(niface)) {
+            const AxisLane al = axis_lane(j0, blk);
+            const bool active = al.point >= 0 && al.point < total;
+            ReinitLane q;
+            if (active) {
+                const int64_t gind = il.idx[al.point];
+                const BGrid G = B.grids[B.list_grid[al.point]];
+                q = reinit_iface_lane(fs + G.off, flags + G.off, box_of(G), inv_of(G), il, al.point, gind - G.off,
+                                      gind, al.axis);
+            }
+            reinit_iface_finish(out, fs, fn, q, al, active, stage);
+        }
+        return;
+    }
+    const ZItem it = B.items[xcd_remap(blk - niface, (int)gridDim.x - niface)];
     const BGrid G = B.grids[it.g];
     const Box b = box_of(G);
     const Inv invs = inv_of(G);
     const idx_t off = G.off, plane = (idx_t)b.rows * b.cols;
+    T* o = out + off;
     zmarch<R, T, false>(lds[0], lds[1], b, it.r0, it.c0, it.p0, it.p1, fs + off, shift(fn, off), nullptr, nullptr,
                         nullptr, flags + off, stage,
                         [&](const T* l, const ZCol<R>& zc, const T w[7], const ZStep<R, T, false>& st, int p) {
                             bool cross;
                             const T res = reinit_value<R>(l, zc.base, w, st.fn, st.fl, invs, stage, cross);
                             const bool bulk = !(st.fl & FLAG_SUBCELL);
-                            const idx_t ind = (idx_t)p * plane + zc.in;
-                            if (bulk && !cross) out[off + ind] = res;
-                            wave_append(bulk && cross, (int)(off + ind), over, &over_count[seq & 1]);
+                            if (bulk && !cross) stp(o + (idx_t)p * plane, zc.in, res);
+                            wave_append(bulk && cross, (int)(off + (idx_t)p * plane + zc.in), over,
+                                        &over_count[seq & 1]);
                         });
 }
 
+// launch 2 of 2: the deferred bulk points (see k_reinit_deferred)
 template <typename T>
-__global__ void __launch_bounds__(256) kb_reinit_iface(T* __restrict__ out, const T* __restrict__ fs,
-                                                       const T* __restrict__ fn, const uint8_t* __restrict__ flags,
-                                                       IList il, BatchView B, int stage,
-                                                       const int* __restrict__ over,
-                                                       const int* __restrict__ over_count, int seq)
+__global__ void __launch_bounds__(256) kb_reinit_deferred(T* __restrict__ out, const T* __restrict__ fs,
+                                                          const T* __restrict__ fn, const uint8_t* __restrict__ flags,
+                                                          IList il, BatchView B, int stage,
+                                                          const int* __restrict__ over,
+                                                          const int* __restrict__ over_count, int seq)
 {
-    const int n1 = *il.count, n2 = over_count[seq & 1];
-    const int total = n1 + n2;
+    const int total = over_count[seq & 1];
     for (int j0 = 0; j0 < total; j0 += axis_lane_stride()) {
         const AxisLane al = axis_lane(j0);
         const bool active = al.point >= 0 && al.point < total;
         ReinitLane q;
         if (active) {
-            const bool listed = al.point < n1;
-            const int64_t gind = listed ? il.idx[al.point] : over[al.point - n1];
+            const int64_t gind = over[al.point];
             const BGrid G = B.grids[grid_of(B, gind)];
-            q = reinit_iface_lane(fs + G.off, flags + G.off, box_of(G), inv_of(G), il, listed ? al.point : -1,
-                                  gind - G.off, gind, al.axis);
+            q = reinit_iface_lane(fs + G.off, flags + G.off, box_of(G), inv_of(G), il, -1, gind - G.off, gind,
+                                  al.axis);
         }
         reinit_iface_finish(out, fs, fn, q, al, active, stage);
     }
 }
 
+// fused extension stage (one launch): the first niface blocks walk the
+// interface list (three lanes per point), the others take the z-march work
+// items; see k_extend_stage
 template <int R, typename T>
-__global__ void __launch_bounds__(256) kb_extend_bulk(T* __restrict__ out, const T* __restrict__ cs,
-                                                      const T* __restrict__ cn, const T* __restrict__ vx,
-                                                      const T* __restrict__ vy, const T* __restrict__ vz,
-                                                      const uint8_t* __restrict__ flags, BatchView B, int stage)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(stage_waves<T>()))) kb_extend_stage(T* __restrict__ out, const T* __restrict__ cs,
+                                                       const T* __restrict__ cn, const T* __restrict__ vx,
+                                                       const T* __restrict__ vy, const T* __restrict__ vz,
+                                                       const uint8_t* __restrict__ flags, IList il,
+                                                       const double* __restrict__ lcp, const double* __restrict__ lv,
+                                                       BatchView B, int stage, int niface)
 {
     __shared__ T lds[2][Shape<R>::N];
-    const ZItem it = B.items[xcd_remap((int)blockIdx.x, (int)gridDim.x)];
+    const int blk = (int)blockIdx.x;
+    if (blk < niface) {
+        const int total = *il.count;
+        for (int j0 = 0; j0 < total; j0 += axis_lane_stride(niface)) {
+            const AxisLane al = axis_lane(j0, blk);
+            const bool active = al.point >= 0 && al.point < total;
+            ExtendLane q;
+            if (active) {
+                const int64_t gind = il.idx[al.point];
+                const BGrid G = B.grids[B.list_grid[al.point]];
+                q = extend_iface_lane(cs + G.off, box_of(G), inv_of(G), il, lcp, lv, al.point, gind - G.off, gind,
+                                      al.axis);
+            }
+            extend_iface_finish(out, cs, cn, q, al, active, stage);
+        }
+        return;
+    }
+    const ZItem it = B.items[xcd_remap(blk - niface, (int)gridDim.x - niface)];
     const BGrid G = B.grids[it.g];
     const Box b = box_of(G);
     const Inv inv = inv_of(G);
@@ -200,27 +243,6 @@ __global__ void __launch_bounds__(256) kb_extend_bulk(T* __restrict__ out, const
                            const T res = upwind_value<R>(l, zc.base, w, st.vel, st.fn, (T)inv.dt_bulk, inv, stage);
                            if (!(st.fl & FLAG_SUBCELL)) stp(o + (idx_t)p * plane, zc.in, res);
                        });
-}
-
-template <typename T>
-__global__ void __launch_bounds__(256) kb_extend_iface(T* __restrict__ out, const T* __restrict__ cs,
-                                                       const T* __restrict__ cn, IList il,
-                                                       const double* __restrict__ lcp, const double* __restrict__ lv,
-                                                       BatchView B, int stage)
-{
-    const int total = *il.count;
-    for (int j0 = 0; j0 < total; j0 += axis_lane_stride()) {
-        const AxisLane al = axis_lane(j0);
-        const bool active = al.point >= 0 && al.point < total;
-        ExtendLane q;
-        if (active) {
-            const int64_t gind = il.idx[al.point];
-            const BGrid G = B.grids[grid_of(B, gind)];
-            q = extend_iface_lane(cs + G.off, box_of(G), inv_of(G), il, lcp, lv, al.point, gind - G.off, gind,
-                                  al.axis);
-        }
-        extend_iface_finish(out, cs, cn, q, al, active, stage);
-    }
 }
 
 template <int R, typename T>
@@ -296,7 +318,7 @@ __global__ void __launch_bounds__(256) kb_ext_vel_list(T* __restrict__ vx, T* __
     const int cnt = *il.count;
     for (int j = (int)(blockIdx.x * 256 + threadIdx.x); j < cnt; j += (int)(gridDim.x * 256)) {
         const int64_t gind = il.idx[j];
-        const BGrid G = B.grids[grid_of(B, gind)];
+        const BGrid G = B.grids[B.list_grid[j]];
         const Box b = box_of(G);
         int r, c, p;
         coords_of(b, gind - G.off, r, c, p);
@@ -325,7 +347,7 @@ __global__ void __launch_bounds__(256) kb_list_interp(double* __restrict__ cp, I
         if (!(al.point >= 0 && al.point < total)) continue;
         const int a = al.axis, j = al.point;
         const int64_t gind = il.idx[j];
-        const BGrid G = B.grids[grid_of(B, gind)];
+        const BGrid G = B.grids[B.list_grid[j]];
         const Box b = box_of(G);
         const double sp[3] = {b.dx, b.dy, b.dz};
         const double rf = il.dist[(size_t)(2 * a) * il.cap + j], lb = il.dist[(size_t)(2 * a + 1) * il.cap + j];
@@ -561,6 +583,23 @@ static inline int list_blocks(const BatchView& B)
     return b < 16 ? 16 : (b > 4096 ? 4096 : b);
 }
 
+// interface-list blocks of a fused stage launch: two per CU at most (see
+// fused_iface_blocks of the single-grid launches)
+static int fused_list_blocks(const BatchView& B)
+{
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        hipDeviceProp_t prop;
+        cus = (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+               prop.multiProcessorCount > 0)
+                  ? prop.multiProcessorCount
+                  : 256;
+    }
+    const int b = list_blocks(B);
+    return b < 2 * cus ? b : 2 * cus;
+}
+
 static inline IList ilist(const ListView& lv) { return IList{lv.idx, lv.count, lv.dist, lv.dt, lv.cap}; }
 
 template <typename T>
@@ -583,31 +622,21 @@ hipError_t launch_batch_reinit_stage(const BatchView& B, T* out, const T* fs, co
                                      const ListView& lv, int stage, int* over, int* over_count, int seq,
                                      hipStream_t s)
 {
-    TR_LAUNCH(kb_reinit_bulk, T, B.nitems, out, fs, fn, flags, B, stage, over, over_count, seq);
-    kb_reinit_iface<T><<<list_blocks(B), 256, 0, s>>>(out, fs, fn, flags, ilist(lv), B, stage, (const int*)over,
-                                                      (const int*)over_count, seq);
+    const int ni = fused_list_blocks(B);
+    TR_LAUNCH(kb_reinit_stage, T, B.nitems + ni, out, fs, fn, flags, ilist(lv), B, stage, over, over_count, seq, ni);
+    const int nd = ni / 10 < 8 ? 8 : ni / 10;
+    kb_reinit_deferred<T><<<nd, 256, 0, s>>>(out, fs, fn, flags, ilist(lv), B, stage, (const int*)over,
+                                               (const int*)over_count, seq);
     return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_batch_extend_stage(const BatchView& B, T* out, const T* cs, const T* cn, const T* vx,
                                      const T* vy, const T* vz, const uint8_t* flags, const ListView& lv,
-                                     const double* lcp, const double* lvel, int stage, hipStream_t s,
-                                     const Fork* fk)
+                                     const double* lcp, const double* lvel, int stage, hipStream_t s)
 {
-    if (fk && fk->side) {
-        hipError_t e;
-        if ((e = hipEventRecord(fk->fork, s)) != hipSuccess) return e;
-        if ((e = hipStreamWaitEvent(fk->side, fk->fork, 0)) != hipSuccess) return e;
-        kb_extend_iface<T><<<list_blocks(B), 256, 0, fk->side>>>(out, cs, cn, ilist(lv), lcp, lvel, B, stage);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-        TR_LAUNCH(kb_extend_bulk, T, B.nitems, out, cs, cn, vx, vy, vz, flags, B, stage);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-        if ((e = hipEventRecord(fk->join, fk->side)) != hipSuccess) return e;
-        return hipStreamWaitEvent(s, fk->join, 0);
-    }
-    TR_LAUNCH(kb_extend_bulk, T, B.nitems, out, cs, cn, vx, vy, vz, flags, B, stage);
-    kb_extend_iface<T><<<list_blocks(B), 256, 0, s>>>(out, cs, cn, ilist(lv), lcp, lvel, B, stage);
+    const int ni = fused_list_blocks(B);
+    TR_LAUNCH(kb_extend_stage, T, B.nitems + ni, out, cs, cn, vx, vy, vz, flags, ilist(lv), lcp, lvel, B, stage, ni);
     return hipGetLastError();
 }
 
@@ -692,7 +721,7 @@ hipError_t launch_batch_ellipsoid(const BatchView& B, T* F, const double* axes, 
                                                      const ListView&, int, int*, int*, int, hipStream_t);           \
     template hipError_t launch_batch_extend_stage<T>(const BatchView&, T*, const T*, const T*, const T*, const T*,  \
                                                      const T*, const uint8_t*, const ListView&, const double*,      \
-                                                     const double*, int, hipStream_t, const Fork*);                 \
+                                                     const double*, int, hipStream_t);                              \
     template hipError_t launch_batch_advect_stage<T>(const BatchView&, T*, const T*, const T*, const T*, const T*,  \
                                                      const T*, const double*, int, hipStream_t);                    \
     template hipError_t launch_batch_extension_velocity<T>(const BatchView&, T*, T*, T*, const T*, const uint8_t*,  \

@@ -313,12 +313,14 @@ __device__ __forceinline__ NonuniformStencil<S> nonuniform_stencil(S p0, S p1, S
     return st;
 }
 
-// weno_nonuniform (re_step.cu:79-243), both faces.  Not inlined: it runs only
-// at interface nodes; scalar arguments and a returned struct keep the
-// caller's stencil in registers.
+// weno_nonuniform (re_step.cu:79-243), both faces.  The _inl forms are for
+// kernels whose register budget is capped (the fused stage kernels, where
+// the interface lanes run in blocks of their own); the others call the
+// out-of-line forms: they run only at interface nodes, and scalar arguments
+// and a returned struct keep the caller's stencil in registers.
 template <typename S>
-__device__ __noinline__ PairT<S> weno_nonuniform_pair(S p0, S p1, S p2, S p3, S p4, S p5, S p6, S r1, S l1,
-                                                      double ds, S h_fore, S h_back)
+__device__ __forceinline__ PairT<S> weno_nonuniform_pair_inl(S p0, S p1, S p2, S p3, S p4, S p5, S p6, S r1, S l1,
+                                                             double ds, S h_fore, S h_back)
 {
     const NonuniformStencil<S> st = nonuniform_stencil(p0, p1, p2, p3, p4, p5, p6, r1, l1, ds, h_fore, h_back);
     PairT<S> out;
@@ -326,14 +328,26 @@ __device__ __noinline__ PairT<S> weno_nonuniform_pair(S p0, S p1, S p2, S p3, S 
     out.back = weno_nonuniform_face(&st.x[0], &st.d[0], true);
     return out;
 }
+template <typename S>
+__device__ __noinline__ PairT<S> weno_nonuniform_pair(S p0, S p1, S p2, S p3, S p4, S p5, S p6, S r1, S l1,
+                                                      double ds, S h_fore, S h_back)
+{
+    return weno_nonuniform_pair_inl(p0, p1, p2, p3, p4, p5, p6, r1, l1, ds, h_fore, h_back);
+}
 
 // one face only: d_fore (back == false) or d_back
+template <typename S>
+__device__ __forceinline__ S weno_nonuniform_one_inl(S p0, S p1, S p2, S p3, S p4, S p5, S p6, S r1, S l1,
+                                                     double ds, S h_fore, S h_back, bool back)
+{
+    const NonuniformStencil<S> st = nonuniform_stencil(p0, p1, p2, p3, p4, p5, p6, r1, l1, ds, h_fore, h_back);
+    return back ? weno_nonuniform_face(&st.x[0], &st.d[0], true) : weno_nonuniform_face(&st.x[1], &st.d[1], false);
+}
 template <typename S>
 __device__ __noinline__ S weno_nonuniform_one(S p0, S p1, S p2, S p3, S p4, S p5, S p6, S r1, S l1, double ds,
                                               S h_fore, S h_back, bool back)
 {
-    const NonuniformStencil<S> st = nonuniform_stencil(p0, p1, p2, p3, p4, p5, p6, r1, l1, ds, h_fore, h_back);
-    return back ? weno_nonuniform_face(&st.x[0], &st.d[0], true) : weno_nonuniform_face(&st.x[1], &st.d[1], false);
+    return weno_nonuniform_one_inl(p0, p1, p2, p3, p4, p5, p6, r1, l1, ds, h_fore, h_back, back);
 }
 
 template <typename S>

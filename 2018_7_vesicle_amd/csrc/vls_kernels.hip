@@ -146,21 +146,40 @@ __global__ void __launch_bounds__(256) k_re_step(double* __restrict__ step, cons
 
 
 
-// WENO5-RK3 reinitialization stage, launch 1 of 2: the bulk.  A point with
-// subcell distances is left to launch 2; so is a point whose stage field
-// changes sign next to it although it has none (it needs the nonuniform
-// stencil): those are appended to `over`, counted in over_count[seq & 1].
-// The other slot is cleared here for the next stage.
+// Fused WENO5-RK3 reinitialization stage, launch 1 of 2: the first niface
+// blocks walk the interface list with three lanes per point (the full
+// residual with subcell distances), the others z-march over every point
+// without subcell distances.  The list blocks are dispatched first, so their
+// long dependent chains (the nonuniform WENO divides ~30 times per face) run
+// beside the bulk instead of after it.  A bulk point whose stage field
+// changes sign beside it needs the nonuniform stencil: it is appended to
+// `over`, counted in over_count[seq & 1], for launch 2; the other slot is
+// cleared here for the next stage.
 template <int R, typename T>
-__global__ void __launch_bounds__(256) k_reinit_bulk(T* __restrict__ out, const T* __restrict__ fs,
-                                                     const T* __restrict__ fn, const uint8_t* __restrict__ flags,
-                                                     Box b, Inv invs, int stage, int* __restrict__ over,
-                                                     int* __restrict__ over_count, int seq, int depth)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(stage_waves<T>()))) k_reinit_stage(T* __restrict__ out, const T* __restrict__ fs,
+                                                      const T* __restrict__ fn, const uint8_t* __restrict__ flags,
+                                                      IList il, Box b, Inv invs, int stage, int* __restrict__ over,
+                                                      int* __restrict__ over_count, int seq, int depth, int niface)
 {
     __shared__ T lds[2][Shape<R>::N];
-    if (blockIdx.x == 0 && threadIdx.x == 0) over_count[(seq + 1) & 1] = 0;
+    const int blk = (int)blockIdx.x;
+    if (blk == 0 && threadIdx.x == 0) over_count[(seq + 1) & 1] = 0;
+    if (blk < niface) {
+        const int total = *il.count;
+        for (int j0 = 0; j0 < total; j0 += axis_lane_stride(niface)) {
+            const AxisLane al = axis_lane(j0, blk);
+            const bool active = al.point >= 0 && al.point < total;
+            ReinitLane q;
+            if (active) {
+                const idx_t ind = (idx_t)il.idx[al.point];
+                q = reinit_iface_lane(fs, flags, b, invs, il, al.point, ind, ind, al.axis);
+            }
+            reinit_iface_finish(out, fs, fn, q, al, active, stage);
+        }
+        return;
+    }
     int r0, c0, p0, p1;
-    zmarch_share<R>(b, depth, r0, c0, p0, p1);
+    zmarch_share<R>(b, depth, blk - niface, (int)gridDim.x - niface, r0, c0, p0, p1);
     const idx_t plane = (idx_t)b.rows * b.cols;
     zmarch<R, T, false>(lds[0], lds[1], b, r0, c0, p0, p1, fs, fn, nullptr, nullptr, nullptr, flags, stage,
                         [&](const T* l, const ZCol<R>& zc, const T w[7], const ZStep<R, T, false>& st, int p) {
@@ -172,26 +191,23 @@ __global__ void __launch_bounds__(256) k_reinit_bulk(T* __restrict__ out, const 
                         });
 }
 
-
-// launch 2 of 2: the interface list, then the deferred bulk points (no
-// subcell distances: spacing distances and the bulk pseudo-time step)
+// launch 2 of 2: the deferred bulk points, three lanes per point, with the
+// interface pass's arithmetic (spacing distances, the bulk pseudo-time step)
 template <typename T>
-__global__ void __launch_bounds__(256) k_reinit_iface(T* __restrict__ out, const T* __restrict__ fs,
-                                                      const T* __restrict__ fn,
-                                                      const uint8_t* __restrict__ flags, IList il, Box b, Inv inv,
-                                                      int stage, const int* __restrict__ over,
-                                                      const int* __restrict__ over_count, int seq)
+__global__ void __launch_bounds__(256) k_reinit_deferred(T* __restrict__ out, const T* __restrict__ fs,
+                                                         const T* __restrict__ fn, const uint8_t* __restrict__ flags,
+                                                         IList il, Box b, Inv inv, int stage,
+                                                         const int* __restrict__ over,
+                                                         const int* __restrict__ over_count, int seq)
 {
-    const int n1 = *il.count, n2 = over_count[seq & 1];
-    const int total = n1 + n2;
+    const int total = over_count[seq & 1];
     for (int j0 = 0; j0 < total; j0 += axis_lane_stride()) {
         const AxisLane al = axis_lane(j0);
         const bool active = al.point >= 0 && al.point < total;
         ReinitLane q;
         if (active) {
-            const bool listed = al.point < n1;
-            const idx_t ind = (idx_t)(listed ? il.idx[al.point] : over[al.point - n1]);
-            q = reinit_iface_lane(fs, flags, b, inv, il, listed ? al.point : -1, ind, ind, al.axis);
+            const idx_t ind = (idx_t)over[al.point];
+            q = reinit_iface_lane(fs, flags, b, inv, il, -1, ind, ind, al.axis);
         }
         reinit_iface_finish(out, fs, fn, q, al, active, stage);
     }
@@ -408,46 +424,46 @@ __global__ void __launch_bounds__(256) k_extend_step(double* __restrict__ step, 
     step[ind] = extend_rate(C, r, c, p, ind, b, q, vel, cp, true) * deltat[ind];
 }
 
-// fused WENO5-RK3 extension stage, bulk: every point without subcell distances
+// Fused WENO5-RK3 extension stage, one launch: the first niface blocks walk
+// the interface list with three lanes per point (see AxisLane; each lane
+// evaluates the one upwind derivative its axis needs, on the nonuniform
+// stencil where the axis crosses; lcp, lv: interface values and velocity in
+// list order), the others z-march over every point without subcell
+// distances.  The list blocks are dispatched first; their long dependent
+// chains (the nonuniform WENO divides ~30 times) then run beside the bulk
+// blocks instead of after them.  The two sets of points are disjoint.
 template <int R, typename T>
-__global__ void __launch_bounds__(256) k_extend_bulk(T* __restrict__ out, const T* __restrict__ cs,
-                                                     const T* __restrict__ cn, const T* __restrict__ vx,
-                                                     const T* __restrict__ vy, const T* __restrict__ vz,
-                                                     const uint8_t* __restrict__ flags, Box b, Inv inv, int stage,
-                                                     int depth)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(stage_waves<T>()))) k_extend_stage(T* __restrict__ out, const T* __restrict__ cs,
+                                                      const T* __restrict__ cn, const T* __restrict__ vx,
+                                                      const T* __restrict__ vy, const T* __restrict__ vz,
+                                                      const uint8_t* __restrict__ flags, IList il,
+                                                      const double* __restrict__ lcp, const double* __restrict__ lv,
+                                                      Box b, Inv inv, int stage, int depth, int niface)
 {
     __shared__ T lds[2][Shape<R>::N];
+    const int blk = (int)blockIdx.x;
+    if (blk < niface) {
+        const int total = *il.count;
+        for (int j0 = 0; j0 < total; j0 += axis_lane_stride(niface)) {
+            const AxisLane al = axis_lane(j0, blk);
+            const bool active = al.point >= 0 && al.point < total;
+            ExtendLane q;
+            if (active) {
+                const idx_t ind = (idx_t)il.idx[al.point];
+                q = extend_iface_lane(cs, b, inv, il, lcp, lv, al.point, ind, ind, al.axis);
+            }
+            extend_iface_finish(out, cs, cn, q, al, active, stage);
+        }
+        return;
+    }
     int r0, c0, p0, p1;
-    zmarch_share<R>(b, depth, r0, c0, p0, p1);
+    zmarch_share<R>(b, depth, blk - niface, (int)gridDim.x - niface, r0, c0, p0, p1);
     const idx_t plane = (idx_t)b.rows * b.cols;
     zmarch<R, T, true>(lds[0], lds[1], b, r0, c0, p0, p1, cs, cn, vx, vy, vz, flags, stage,
                        [&](const T* l, const ZCol<R>& zc, const T w[7], const ZStep<R, T, true>& st, int p) {
                            const T res = upwind_value<R>(l, zc.base, w, st.vel, st.fn, (T)inv.dt_bulk, inv, stage);
                            if (!(st.fl & FLAG_SUBCELL)) stp(out + (idx_t)p * plane, zc.in, res);
                        });
-}
-
-// fused WENO5-RK3 extension stage, interface list: three lanes per point
-// (see AxisLane); each lane evaluates the one upwind derivative its axis
-// needs (see upwind_axis), on the nonuniform stencil where the axis crosses.
-// lcp, lv: interface values and velocity in list order.
-template <typename T>
-__global__ void __launch_bounds__(256) k_extend_iface(T* __restrict__ out, const T* __restrict__ cs,
-                                                      const T* __restrict__ cn, IList il,
-                                                      const double* __restrict__ lcp, const double* __restrict__ lv,
-                                                      Box b, Inv inv, int stage)
-{
-    const int total = *il.count;
-    for (int j0 = 0; j0 < total; j0 += axis_lane_stride()) {
-        const AxisLane al = axis_lane(j0);
-        const bool active = al.point >= 0 && al.point < total;
-        ExtendLane q;
-        if (active) {
-            const idx_t ind = (idx_t)il.idx[al.point];
-            q = extend_iface_lane(cs, b, inv, il, lcp, lv, al.point, ind, ind, al.axis);
-        }
-        extend_iface_finish(out, cs, cn, q, al, active, stage);
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -488,7 +504,7 @@ __global__ void __launch_bounds__(256) k_advect_stage(T* __restrict__ out, const
 {
     __shared__ T lds[2][Shape<R>::N];
     int r0, c0, p0, p1;
-    zmarch_share<R>(b, depth, r0, c0, p0, p1);
+    zmarch_share<R>(b, depth, (int)blockIdx.x, (int)gridDim.x, r0, c0, p0, p1);
     const idx_t plane = (idx_t)b.rows * b.cols;
     zmarch<R, T, true>(lds[0], lds[1], b, r0, c0, p0, p1, fs, fn, vx, vy, vz, nullptr, stage,
                        [&](const T* l, const ZCol<R>& zc, const T w[7], const ZStep<R, T, true>& st, int p) {
@@ -846,6 +862,38 @@ int iface_blocks(const Geom& g)
     return (int)(b < 16 ? 16 : (b > 4096 ? 4096 : b));
 }
 
+// compute units of the current device, read once
+static int device_cus()
+{
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        hipDeviceProp_t prop;
+        cus = (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+               prop.multiProcessorCount > 0)
+                  ? prop.multiProcessorCount
+                  : 256;
+    }
+    return cus;
+}
+
+// interface-list blocks of a fused stage launch: one per CU at most, so the
+// list lanes (long dependent chains) hold a fifth to a quarter of the block
+// slots and the bulk blocks the rest
+static int fused_iface_blocks(const Geom& g)
+{
+    const int b = iface_blocks(g), c = device_cus();
+    return b < c ? b : c;
+}
+
+// blocks of the deferred-point launch: the deferred points are few (points
+// the interface reached during the reinitialization), a tenth of a list
+int deferred_blocks(const Geom& g)
+{
+    const int b = iface_blocks(g) / 10;
+    return b < 8 ? 8 : b;
+}
+
 // pages per block of the z-marching bulk stage launches: g.zd, or by
 // default the largest power of two up to 16 (and pges) that still gives at
 // least 2048 blocks (8 per CU of a 256-CU device); measured on MI355X,
@@ -890,17 +938,17 @@ template <typename T>
 hipError_t launch_reinit_stage(T* out, const T* fs, const T* fn, const uint8_t* flags, const ListView& lv,
                                const Geom& g, int stage, int* over, int* over_count, int seq, hipStream_t s)
 {
-    const int zd = zmarch_depth(g);
-    if (g.tr == 32)
-        k_reinit_bulk<32, T><<<dim3(bulk_blocks(g, zd)), dim3(256), 0, s>>>(out, fs, fn, flags, box_of(g), inv_of(g),
-                                                                             stage, over, over_count, seq, zd);
-    else
-        k_reinit_bulk<64, T><<<dim3(bulk_blocks(g, zd)), dim3(256), 0, s>>>(out, fs, fn, flags, box_of(g), inv_of(g),
-                                                                             stage, over, over_count, seq, zd);
     const IList il{lv.idx, lv.count, lv.dist, lv.dt, lv.cap};
-    k_reinit_iface<T><<<dim3(iface_blocks(g)), dim3(256), 0, s>>>(out, fs, fn, flags, il, box_of(g), inv_of(g),
-                                                                   stage, (const int*)over, (const int*)over_count,
-                                                                   seq);
+    const int zd = zmarch_depth(g), ni = fused_iface_blocks(g);
+    const dim3 grid(bulk_blocks(g, zd) + ni);
+    if (g.tr == 32)
+        k_reinit_stage<32, T><<<grid, dim3(256), 0, s>>>(out, fs, fn, flags, il, box_of(g), inv_of(g), stage, over,
+                                                         over_count, seq, zd, ni);
+    else
+        k_reinit_stage<64, T><<<grid, dim3(256), 0, s>>>(out, fs, fn, flags, il, box_of(g), inv_of(g), stage, over,
+                                                         over_count, seq, zd, ni);
+    k_reinit_deferred<T><<<dim3(deferred_blocks(g)), dim3(256), 0, s>>>(out, fs, fn, flags, il, box_of(g), inv_of(g),
+                                                                         stage, over, over_count, seq);
     return hipGetLastError();
 }
 
@@ -966,32 +1014,17 @@ hipError_t launch_extend_step(double* step, const double* deltat, const double* 
 template <typename T>
 hipError_t launch_extend_stage(T* out, const T* cs, const T* cn, const T* vx, const T* vy, const T* vz,
                                const uint8_t* flags, const ListView& lv, const double* lcp, const double* lvel,
-                               const Geom& g, int stage, hipStream_t s, const Fork* fk)
+                               const Geom& g, int stage, hipStream_t s)
 {
     const IList il{lv.idx, lv.count, lv.dist, lv.dt, lv.cap};
-    if (fk && fk->side) {
-        hipError_t e;
-        if ((e = hipEventRecord(fk->fork, s)) != hipSuccess) return e;
-        if ((e = hipStreamWaitEvent(fk->side, fk->fork, 0)) != hipSuccess) return e;
-        k_extend_iface<T><<<dim3(iface_blocks(g)), dim3(256), 0, fk->side>>>(out, cs, cn, il, lcp, lvel, box_of(g),
-                                                                              inv_of(g), stage);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-    }
-    const int zd = zmarch_depth(g);
+    const int zd = zmarch_depth(g), ni = fused_iface_blocks(g);
+    const dim3 grid(bulk_blocks(g, zd) + ni);
     if (g.tr == 32)
-        k_extend_bulk<32, T><<<dim3(bulk_blocks(g, zd)), dim3(256), 0, s>>>(out, cs, cn, vx, vy, vz, flags,
-                                                                             box_of(g), inv_of(g), stage, zd);
+        k_extend_stage<32, T><<<grid, dim3(256), 0, s>>>(out, cs, cn, vx, vy, vz, flags, il, lcp, lvel, box_of(g),
+                                                         inv_of(g), stage, zd, ni);
     else
-        k_extend_bulk<64, T><<<dim3(bulk_blocks(g, zd)), dim3(256), 0, s>>>(out, cs, cn, vx, vy, vz, flags,
-                                                                             box_of(g), inv_of(g), stage, zd);
-    if (fk && fk->side) {
-        hipError_t e;
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-        if ((e = hipEventRecord(fk->join, fk->side)) != hipSuccess) return e;
-        return hipStreamWaitEvent(s, fk->join, 0);
-    }
-    k_extend_iface<T><<<dim3(iface_blocks(g)), dim3(256), 0, s>>>(out, cs, cn, il, lcp, lvel, box_of(g), inv_of(g),
-                                                                   stage);
+        k_extend_stage<64, T><<<grid, dim3(256), 0, s>>>(out, cs, cn, vx, vy, vz, flags, il, lcp, lvel, box_of(g),
+                                                         inv_of(g), stage, zd, ni);
     return hipGetLastError();
 }
 
@@ -1097,7 +1130,7 @@ hipError_t launch_copy(T* dst, const T* src, int64_t n, hipStream_t s)
     template hipError_t launch_extension_velocity<T>(T*, T*, T*, const T*, double, const Geom&, hipStream_t);       \
     template hipError_t launch_extend_stage<T>(T*, const T*, const T*, const T*, const T*, const T*,               \
                                                const uint8_t*, const ListView&, const double*, const double*,       \
-                                               const Geom&, int, hipStream_t, const Fork*);                         \
+                                               const Geom&, int, hipStream_t);                                      \
     template hipError_t launch_advect_stage<T>(T*, const T*, const T*, const T*, const T*, const T*, double,        \
                                                const Geom&, int, hipStream_t);                                      \
     template hipError_t launch_normal_velocity<T>(T*, const T*, const T*, const T*, const T*, const Geom&,          \

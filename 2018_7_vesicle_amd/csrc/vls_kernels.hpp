@@ -103,18 +103,10 @@ hipError_t launch_boundary_interpolant(const Dist6& cp, const Dist6c& d, const d
 hipError_t launch_extend_step(double* step, const double* deltat, const double* C, const double* vx,
                               const double* vy, const double* vz, const Dist6c& d, const Dist6c& cp,
                               const Geom& g, hipStream_t s);
-// An optional side stream for the interface pass of an extension stage: the
-// interface launch goes to `side`, forked from the main stream with `fork`
-// and joined back with `join`, so it can run beside the bulk launch (the two
-// passes write disjoint points and read only the stage input).
-struct Fork {
-    hipStream_t side;
-    hipEvent_t fork, join;
-};
 template <typename T>
 hipError_t launch_extend_stage(T* out, const T* cs, const T* cn, const T* vx, const T* vy, const T* vz,
                                const uint8_t* flags, const ListView& lv, const double* lcp, const double* lvel,
-                               const Geom& g, int stage, hipStream_t s, const Fork* fk = nullptr);
+                               const Geom& g, int stage, hipStream_t s);
 hipError_t launch_advection_step(double* astep, const double* vx, const double* vy, const double* vz,
                                  const double* lsf, const Geom& g, hipStream_t s);
 template <typename T>
@@ -196,6 +188,7 @@ struct BatchView {
     int tr;                // rows per tile: 64 or 32
     const ZItem* items;    // z-march work list of the bulk stage kernels
     int nitems;
+    int* list_grid;        // grid of each interface-list entry (written with the list)
 };
 
 // tiles of one grid with `tr` rows per tile (tr x 256/tr columns)
@@ -220,8 +213,7 @@ hipError_t launch_batch_reinit_stage(const BatchView& B, T* out, const T* fs, co
 template <typename T>
 hipError_t launch_batch_extend_stage(const BatchView& B, T* out, const T* cs, const T* cn, const T* vx,
                                      const T* vy, const T* vz, const uint8_t* flags, const ListView& lv,
-                                     const double* lcp, const double* lvel, int stage, hipStream_t s,
-                                     const Fork* fk = nullptr);
+                                     const double* lcp, const double* lvel, int stage, hipStream_t s);
 // dt: one step per grid (device)
 template <typename T>
 hipError_t launch_batch_advect_stage(const BatchView& B, T* out, const T* fs, const T* fn, const T* vx,

@@ -120,12 +120,16 @@ __device__ __forceinline__ double min6(const Dists& q)
     return dmin(dmin(q.r[0], q.l[0]), dmin(dmin(q.r[1], q.l[1]), dmin(q.r[2], q.l[2])));
 }
 
-// per-axis derivative pair with the sign-change interface test (re_step.cu:248-269)
+// per-axis derivative pair with the sign-change interface test (re_step.cu:248-269);
+// INL: the nonuniform face inline (see weno_nonuniform_pair_inl)
+template <bool INL = false>
 __device__ __forceinline__ void reinit_axis(double& fore, double& back, const double v[7], double r1, double l1,
                                             double ds, JPd jp)
 {
     if (v[2] * v[3] < 0 || v[3] * v[4] < 0) {
-        const Pair q = weno_nonuniform_pair(v, r1, l1, ds, 0.0, 0.0);
+        Pair q;
+        if constexpr (INL) q = weno_nonuniform_pair_inl(v[0], v[1], v[2], v[3], v[4], v[5], v[6], r1, l1, ds, 0.0, 0.0);
+        else q = weno_nonuniform_pair(v, r1, l1, ds, 0.0, 0.0);
         fore = q.fore;
         back = q.back;
     } else {
@@ -133,13 +137,15 @@ __device__ __forceinline__ void reinit_axis(double& fore, double& back, const do
     }
 }
 
+// (row, col, page) of element ind of a grid; 32-bit division (a grid holds
+// fewer than 2^31 points: interface lists store int indices)
 __device__ __forceinline__ void coords_of(const Box& b, idx_t ind, int& r, int& c, int& p)
 {
-    const idx_t plane = (idx_t)b.rows * b.cols;
-    p = (int)(ind / plane);
-    const int rem = (int)(ind - (idx_t)p * plane);
-    c = rem / b.rows;
-    r = rem - c * b.rows;
+    const unsigned u = (unsigned)ind, rows = (unsigned)b.rows, plane = rows * (unsigned)b.cols;
+    const unsigned pp = u / plane, rem = u - pp * plane, cc = rem / rows;
+    p = (int)pp;
+    c = (int)cc;
+    r = (int)(rem - cc * rows);
 }
 
 __device__ __forceinline__ void load_dists(Dists& q, const Dist6c& d, idx_t ind)
@@ -316,19 +322,22 @@ struct AxisLane {
     int base;   // lane of the group's x member
 };
 
-__device__ __forceinline__ AxisLane axis_lane(int j0)
+// the lane's group in sweep j0 of block blk (of a 256-thread launch)
+__device__ __forceinline__ AxisLane axis_lane(int j0, int blk)
 {
     const int lane = (int)(threadIdx.x & 63);
-    const int wave = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    const int wave = blk * 4 + (int)(threadIdx.x >> 6);
     AxisLane a;
     a.axis = lane % 3;
     a.base = lane - a.axis;
     a.point = (lane < 63) ? j0 + wave * 21 + lane / 3 : -1;
     return a;
 }
+__device__ __forceinline__ AxisLane axis_lane(int j0) { return axis_lane(j0, (int)blockIdx.x); }
 
-// stride, in points, between the groups a grid visits per sweep
-__device__ __forceinline__ int axis_lane_stride() { return (int)(gridDim.x * (blockDim.x >> 6)) * 21; }
+// stride, in points, between the groups nblk blocks visit per sweep
+__device__ __forceinline__ int axis_lane_stride(int nblk) { return nblk * 4 * 21; }
+__device__ __forceinline__ int axis_lane_stride() { return axis_lane_stride((int)gridDim.x); }
 
 // per-axis pair with the distance interface test (shared_utilities.cup:256-278)
 __device__ __forceinline__ void extend_axis(double& fore, double& back, const double v[7], double r1, double l1,
@@ -416,6 +425,15 @@ __device__ __forceinline__ void block_reduce_store(KSum (&k)[NQ], double* __rest
 // grid (offset into the packed fields for a batch), so a grid gives the same
 // result bit for bit either way.
 // ---------------------------------------------------------------------------
+
+// Waves per SIMD the fused stage kernels are compiled for: their bulk
+// z-march fits 128 VGPRs in FP64 and 96 in FP32; the interface lanes (the
+// nonuniform WENO, inlined) are fitted into the same budget.
+template <typename T>
+constexpr int stage_waves()
+{
+    return sizeof(T) == 8 ? 4 : 5;
+}
 
 // ---------------------------------------------------------------------------
 // z-marching walk (single grid): a block takes one tile column (TR rows x TC
@@ -537,14 +555,15 @@ __device__ __forceinline__ void zmarch(T* lds0, T* lds1, const Box& b, int r0, i
     }
 }
 
-// the z-march share of block blockIdx.x of a launch of zmarch_blocks(...)
-// blocks: tile column (r0, c0) and pages [p0, p1).  Consecutive blocks of
-// one XCD take neighbouring columns, which share their halos in its L2.
+// the z-march share of block blk of the nblk = zmarch_blocks(...) blocks of a
+// launch: tile column (r0, c0) and pages [p0, p1).  Consecutive blocks of one
+// XCD take neighbouring columns, which share their halos in its L2.
 template <int R>
-__device__ __forceinline__ void zmarch_share(const Box& b, int depth, int& r0, int& c0, int& p0, int& p1)
+__device__ __forceinline__ void zmarch_share(const Box& b, int depth, int blk, int nblk, int& r0, int& c0, int& p0,
+                                             int& p1)
 {
     const int ntr = tiles_per_col<R>(b.rows), nct = ntr * tiles_per_row<R>(b.cols);
-    const int gi = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int gi = xcd_remap(blk, nblk);
     const int ch = gi / nct, ct = gi - ch * nct;
     const int cb = ct / ntr;
     r0 = (ct - cb * ntr) * R;
@@ -618,7 +637,7 @@ __device__ __forceinline__ ReinitLane reinit_iface_lane(const T* __restrict__ fs
     if (q.dt >= 0) {
         double v[7], fo, bk;
         line7(fs, r, c, p, a, b, v);
-        reinit_axis(fo, bk, v, r1, l1, sp[a], inv.jd[a]);
+        reinit_axis<true>(fo, bk, v, r1, l1, sp[a], inv.jd[a]);
         q.m = q.inside ? dmax(sq(dmin(0.0, bk)), sq(dmax(0.0, fo))) : dmax(sq(dmax(0.0, bk)), sq(dmin(0.0, fo)));
     }
     return q;
@@ -689,7 +708,7 @@ __device__ __forceinline__ ExtendLane extend_iface_lane(const T* __restrict__ cs
         const double l1 = il.dist[(size_t)(2 * a + 1) * il.cap + slot];
         if (r1 < sp[a] || l1 < sp[a]) {
             const bool fore = vel < 0;
-            const double dv = weno_nonuniform_one(v[0], v[1], v[2], v[3], v[4], v[5], v[6], r1, l1, sp[a],
+            const double dv = weno_nonuniform_one_inl(v[0], v[1], v[2], v[3], v[4], v[5], v[6], r1, l1, sp[a],
                                                   lcp[(size_t)(2 * a) * il.cap + slot],
                                                   lcp[(size_t)(2 * a + 1) * il.cap + slot], !fore);
             q.t = 0.0 + (fore ? vel : dmax(0.0, vel)) * dv;

@@ -223,15 +223,6 @@ int vls_sdf3_normal_velocity_f32(vls_sdf3_t h, const float *F, const float *vx, 
  * previous setting. */
 int vls_set_graphs(int enable);
 
-/* Overlap of an extension stage's interface pass with its bulk pass (no
- * counterpart in the reference): with it on (the default), the interface
- * launch of every extension stage goes to a side stream of the handle,
- * forked from and joined back to the caller's stream (inside a captured
- * graph: two parallel branches).  Results do not change.  Off by default:
- * on MI355X the branches did not run concurrently (40 extension iterations
- * at 128^3 FP64: 4.60 ms serial, 5.19 ms forked).  Returns the previous
- * setting. */
-int vls_set_overlap(int enable);
 
 /* ------------------------------------------------------------------------ */
 /* 4. batches of grids                                                      */

@@ -194,47 +194,6 @@ def test_single_grid_graph_replays_equal_eager():
         vls.set_graphs(prev)
 
 
-@pytest.mark.parametrize("graphs", [False, True])
-def test_forked_interface_pass_keeps_results(graphs):
-    """the extension stage's interface launch on a side stream (vls_set_overlap)
-    gives the serial result, for a batch step and a single-grid extension,
-    eager and over three graph replays"""
-    b, grids, Gs, F, C, u = _setup(torch.float64)
-    dt = b.cfl_dt(*u, 0.4)
-    b.set_distance(F)
-    F0 = F.clone()
-    g = cases.linspace_grid(28)
-    m = vls.SDF3(vls.GD3(g.xv, g.yv, g.zv), torch.from_numpy(_fields(g, 0)[0]).cuda())
-    m.setDistance()
-    C1 = torch.from_numpy(_fields(g, 0)[1]).cuda()
-    prev_g = vls.set_graphs(graphs)
-    prev_o = vls.set_overlap(False)
-    try:
-        Fe, Ve = F0.clone(), b.zeros()
-        ave = torch.zeros((b.ngrids, 2), dtype=torch.float64, device="cuda")
-        b.step(Fe, Ve, *u, dt, 4, 3, ave)
-        ce = m.WENO5RK3Extend(C1, 4)
-        torch.cuda.synchronize()
-        vls.set_overlap(True)
-        Fo, Vo, co = F0.clone(), b.zeros(), C1.clone()  # stable buffers: the graphs replay
-        avo = torch.zeros((b.ngrids, 2), dtype=torch.float64, device="cuda")
-        for rep in range(3):
-            Fo.copy_(F0)
-            Vo.zero_()
-            avo.zero_()
-            co.copy_(C1)
-            b.step(Fo, Vo, *u, dt, 4, 3, avo)
-            m.extend_(co, 4)
-            junk = (Fo * 2).sum()  # eager work between calls
-            torch.cuda.synchronize()
-            assert torch.isfinite(junk)
-            assert torch.equal(Fo, Fe) and torch.equal(Vo, Ve) and torch.equal(avo, ave), f"call {rep}"
-            assert torch.equal(co, ce), f"call {rep}"
-    finally:
-        vls.set_overlap(prev_o)
-        vls.set_graphs(prev_g)
-
-
 def test_synthetic_inputs_match_numpy():
     cfg = bm.BatchConfig(vesicles=3, edges=(24, 32))
     edges, axes, modes = bm.batch_inputs(cfg)
