@@ -12,8 +12,9 @@ ellipsoid (0.8,0.5,0.5), seeded 16-mode curl velocity, CFL 0.4 (see
 value = WENO grid-point updates per second: (operator-stage passes counted by
 pipeline.updates_per_step) x points / wall time of the K timed steps.
 
-roofline: the dominant kernel is the fused WENO5-RK3 extension stage (a bulk
-launch and an interface-list launch).  Its average duration is measured with HIP events around the
+roofline: the dominant kernel is the fused WENO5-RK3 extension stage
+(k_extend_stage: interface-list blocks and z-marching bulk blocks in one
+launch).  Its average duration is measured with HIP events around the
 extension calls inside the timed steps, minus the extension's per-call set-up
 (upwind normal, velocity, interface values: timed separately with 0
 iterations), divided by the stage launches.  achieved = 37.3 B/point (the
@@ -323,7 +324,7 @@ def main():
         },
         "roofline": {
             "kernel": ("FP64" if fp64 else "FP32-bulk") +
-                      " WENO5-RK3 extension stage (k_extend_bulk + k_extend_iface launches)",
+                      " WENO5-RK3 extension stage (k_extend_stage launches)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
