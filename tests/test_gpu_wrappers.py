@@ -166,3 +166,41 @@ def test_pipeline_small():
     exact(N(p.Vn), Vn, "pipeline Vn")
     got_av = N(p.area_volume)
     assert np.all(np.abs(got_av - av) <= 1e-13 * np.abs(av)), (got_av, av)
+
+
+@pytest.mark.parametrize("shape", [(24, 20, 2), (30, 26, 4), (9, 5, 11), (70, 3, 8)])
+def test_thin_and_small_boxes(shape):
+    """z-marching edge cases: fewer pages than the z-window (clamped at both
+    ends at once), a box smaller than one tile, partial tiles in rows and
+    columns; reinitialization, extension and advection against the oracle"""
+    rows, cols, pges = shape
+    g = cases.Grid(np.linspace(-1.0, 1.0, cols), np.linspace(-0.9, 1.0, rows), np.linspace(-1.1, 0.9, pges))
+    G = vls.GD3(g.xv, g.yv, g.zv)
+    F = np.ascontiguousarray(np.sqrt(g.X ** 2 + g.Y ** 2 + (0.3 * g.Z) ** 2) - 0.55)
+    m = vls.SDF3(G, T(F))
+    m.setDistance()
+    got = N(m.WENO5RK3Reinitialization(T(F), 3))
+    d6, md = po.set_distance(F, *g.spacing)
+    exact(got, po.weno5_rk3_reinit(F, 3, d6, md, *g.spacing), f"reinit {shape}")
+    C = cases.smooth_scalar(g)
+    got = N(m.WENO5RK3Extend(T(C), 3))
+    exact(got, po.weno5_rk3_extend(C, 3, F, d6, md, *g.spacing), f"extend {shape}")
+    u3 = cases.smooth_velocity(g)
+    dt = 0.4 * g.dx / max(np.abs(u).max() for u in u3)
+    got = N(m.WENO5RK3Advection(T(F), *[T(u) for u in u3], dt, 2))
+    exact(got, po.advect_rk3(F, 2, dt, u3, *g.spacing), f"advection {shape}")
+
+
+def test_field_without_interface():
+    """no sign change anywhere: an empty interface list in the fused launches"""
+    g = cases.Grid(np.linspace(-1, 1, 34), np.linspace(-1, 1, 30), np.linspace(-1, 1, 26))
+    G = vls.GD3(g.xv, g.yv, g.zv)
+    F = np.ascontiguousarray(np.sqrt(g.X ** 2 + g.Y ** 2 + g.Z ** 2) + 0.2)
+    m = vls.SDF3(G, T(F))
+    m.setDistance()
+    got = N(m.WENO5RK3Reinitialization(T(F), 4))
+    d6, md = po.set_distance(F, *g.spacing)
+    exact(got, po.weno5_rk3_reinit(F, 4, d6, md, *g.spacing), "reinit")
+    C = cases.smooth_scalar(g)
+    got = N(m.WENO5RK3Extend(T(C), 4))
+    exact(got, po.weno5_rk3_extend(C, 4, F, d6, md, *g.spacing), "extend")
