@@ -32,12 +32,23 @@ SIGNATURES = {
     "vls_advection_step": [P] * 5 + [I] * 3 + [D] * 3 + [P],
     "vls_weno_derivative": [P] * 7 + [I] * 3 + [D] * 3 + [P],
     "vls_set_calculus_toolbox": [P] * 3 + [I] * 3 + [D] * 4 + [I, P],
+    "vls_cubic_boundary_correction": [P] * 7 + [I] * 4 + [D] * 3 + [P],
+    "vls_wenork3_re_step": [P] * 10 + [I] * 3 + [D] * 3 + [I, P],
+    "vls_enork2_upwind_normal": [P] * 10 + [I] * 3 + [D] * 3 + [I, P],
+    "vls_enork2_boundary_interpolate": [P] * 13 + [I] * 3 + [D] * 3 + [I, P],
+    "vls_enork2_extend_step": [P] * 18 + [I] * 3 + [D] * 3 + [I, P],
+    "vls_wenork3_upwind_normal": [P] * 10 + [I] * 3 + [D] * 3 + [I, P],
+    "vls_wenork3_boundary_interpolate": [P] * 13 + [I] * 3 + [D] * 3 + [I, P],
+    "vls_wenork3_extend_step": [P] * 18 + [I] * 3 + [D] * 3 + [I, P],
     "vls_sdf3_create": [P, I, I, I, D, D, D],
     "vls_sdf3_destroy": [P],
     "vls_sdf3_set_distance": [P, P, P],
     "vls_sdf3_distances": [P, P],
     "vls_sdf3_weno5_rk3_reinit": [P, P, I, P],
     "vls_sdf3_eno2_reinit": [P, P, I, I, D, P],
+    "vls_sdf3_wenork3_reinit": [P, P, I, P],
+    "vls_sdf3_enork2_extend": [P, P, P, I, P],
+    "vls_sdf3_wenork3_extend": [P, P, P, I, P],
     "vls_sdf3_weno5_rk3_extend": [P, P, P, I, P],
     "vls_sdf3_surface_redistance": [P, P, P, I, I, P],
     "vls_sdf3_advect_rk3": [P, P, P, P, P, D, I, P],

@@ -602,6 +602,141 @@ int vls_extend_step(double* step, const double* deltat, const double* lsf, const
     return VLS_OK;
 }
 
+// ---- WENO-RK3 / ENO-RK2 variants: drop-in kernels -----------------------------
+int vls_cubic_boundary_correction(double* xpr, double* xpl, double* ypf, double* ypb, double* zpu, double* zpd,
+                                  const double* lsf, int num_ele, int rows, int cols, int pges, double dx,
+                                  double dy, double dz, vls_stream_t stream)
+{
+    return distances_dropin(3, xpr, xpl, ypf, ypb, zpu, zpd, lsf, num_ele, rows, cols, pges, dx, dy, dz, stream);
+}
+
+int vls_wenork3_re_step(double* step, const double* lsf, const uint8_t* mask, const double* deltat,
+                        const double* xpr, const double* xpl, const double* ypf, const double* ypb,
+                        const double* zpu, const double* zpd, int rows, int cols, int pges, double dx, double dy,
+                        double dz, int num_ele, vls_stream_t stream)
+{
+    int rc = check_geom(rows, cols, pges, dx, dy, dz);
+    if (rc) return rc;
+    if ((rc = check_num(num_ele, rows, cols, pges))) return rc;
+    VLS_REQUIRE(!any_null(step, lsf, mask, deltat, xpr, xpl, ypf, ypb, zpu, zpd), "null pointer");
+    VLS_REQUIRE(step != lsf, "step must not alias lsf");
+    const Geom g{rows, cols, pges, dx, dy, dz};
+    VLS_HIP(launch_wenork3_re_step(step, lsf, mask, deltat, mk6c(xpr, xpl, ypf, ypb, zpu, zpd), g, S(stream)));
+    return VLS_OK;
+}
+
+static int variant_normal_dropin(int weno, double* nx, double* ny, double* nz, const double* lsf, const double* xpr,
+                                 const double* xpl, const double* ypf, const double* ypb, const double* zpu,
+                                 const double* zpd, int rows, int cols, int pges, double dx, double dy, double dz,
+                                 int num_ele, vls_stream_t stream)
+{
+    int rc = check_geom(rows, cols, pges, dx, dy, dz);
+    if (rc) return rc;
+    if ((rc = check_num(num_ele, rows, cols, pges))) return rc;
+    VLS_REQUIRE(!any_null(nx, ny, nz, lsf, xpr, xpl, ypf, ypb, zpu, zpd), "null pointer");
+    const Geom g{rows, cols, pges, dx, dy, dz};
+    const Dist6c d = mk6c(xpr, xpl, ypf, ypb, zpu, zpd);
+    if (weno) VLS_HIP(launch_wenork3_upwind_normal(nx, ny, nz, lsf, d, g, S(stream)));
+    else VLS_HIP(launch_enork2_upwind_normal(nx, ny, nz, lsf, d, g, S(stream)));
+    return VLS_OK;
+}
+
+int vls_enork2_upwind_normal(double* nx, double* ny, double* nz, const double* lsf, const double* xpr,
+                             const double* xpl, const double* ypf, const double* ypb, const double* zpu,
+                             const double* zpd, int rows, int cols, int pges, double dx, double dy, double dz,
+                             int num_ele, vls_stream_t stream)
+{
+    return variant_normal_dropin(0, nx, ny, nz, lsf, xpr, xpl, ypf, ypb, zpu, zpd, rows, cols, pges, dx, dy, dz,
+                                 num_ele, stream);
+}
+
+int vls_wenork3_upwind_normal(double* nx, double* ny, double* nz, const double* lsf, const double* xpr,
+                              const double* xpl, const double* ypf, const double* ypb, const double* zpu,
+                              const double* zpd, int rows, int cols, int pges, double dx, double dy, double dz,
+                              int num_ele, vls_stream_t stream)
+{
+    return variant_normal_dropin(1, nx, ny, nz, lsf, xpr, xpl, ypf, ypb, zpu, zpd, rows, cols, pges, dx, dy, dz,
+                                 num_ele, stream);
+}
+
+static int variant_interpolate_dropin(int weno, double* cpr, double* cpl, double* cpf, double* cpb, double* cpu,
+                                      double* cpd, const double* xpr, const double* xpl, const double* ypf,
+                                      const double* ypb, const double* zpu, const double* zpd, const double* lsf,
+                                      int rows, int cols, int pges, double dx, double dy, double dz, int num_ele,
+                                      vls_stream_t stream)
+{
+    int rc = check_geom(rows, cols, pges, dx, dy, dz);
+    if (rc) return rc;
+    if ((rc = check_num(num_ele, rows, cols, pges))) return rc;
+    VLS_REQUIRE(!any_null(cpr, cpl, cpf, cpb, cpu, cpd, xpr, xpl, ypf, ypb, zpu, zpd, lsf), "null pointer");
+    const Geom g{rows, cols, pges, dx, dy, dz};
+    const Dist6 cp = mk6(cpr, cpl, cpf, cpb, cpu, cpd);
+    const Dist6c d = mk6c(xpr, xpl, ypf, ypb, zpu, zpd);
+    if (weno) VLS_HIP(launch_wenork3_boundary_interpolate(cp, d, lsf, g, S(stream)));
+    else VLS_HIP(launch_enork2_boundary_interpolate(cp, d, lsf, g, S(stream)));
+    return VLS_OK;
+}
+
+int vls_enork2_boundary_interpolate(double* cpr, double* cpl, double* cpf, double* cpb, double* cpu, double* cpd,
+                                    const double* xpr, const double* xpl, const double* ypf, const double* ypb,
+                                    const double* zpu, const double* zpd, const double* lsf, int rows, int cols,
+                                    int pges, double dx, double dy, double dz, int num_ele, vls_stream_t stream)
+{
+    return variant_interpolate_dropin(0, cpr, cpl, cpf, cpb, cpu, cpd, xpr, xpl, ypf, ypb, zpu, zpd, lsf, rows,
+                                      cols, pges, dx, dy, dz, num_ele, stream);
+}
+
+int vls_wenork3_boundary_interpolate(double* cpr, double* cpl, double* cpf, double* cpb, double* cpu, double* cpd,
+                                     const double* xpr, const double* xpl, const double* ypf, const double* ypb,
+                                     const double* zpu, const double* zpd, const double* lsf, int rows, int cols,
+                                     int pges, double dx, double dy, double dz, int num_ele, vls_stream_t stream)
+{
+    return variant_interpolate_dropin(1, cpr, cpl, cpf, cpb, cpu, cpd, xpr, xpl, ypf, ypb, zpu, zpd, lsf, rows,
+                                      cols, pges, dx, dy, dz, num_ele, stream);
+}
+
+static int variant_extend_dropin(int weno, double* step, const double* deltat, const double* lsf, const double* vx,
+                                 const double* vy, const double* vz, const double* xpr, const double* xpl,
+                                 const double* ypf, const double* ypb, const double* zpu, const double* zpd,
+                                 const double* cpr, const double* cpl, const double* cpf, const double* cpb,
+                                 const double* cpu, const double* cpd, int rows, int cols, int pges, double dx,
+                                 double dy, double dz, int num_ele, vls_stream_t stream)
+{
+    int rc = check_geom(rows, cols, pges, dx, dy, dz);
+    if (rc) return rc;
+    if ((rc = check_num(num_ele, rows, cols, pges))) return rc;
+    VLS_REQUIRE(!any_null(step, deltat, lsf, vx, vy, vz, xpr, xpl, ypf, ypb, zpu, zpd, cpr, cpl, cpf, cpb, cpu, cpd),
+                "null pointer");
+    VLS_REQUIRE(step != lsf, "step must not alias lsf");
+    const Geom g{rows, cols, pges, dx, dy, dz};
+    const Dist6c d = mk6c(xpr, xpl, ypf, ypb, zpu, zpd), cp = mk6c(cpr, cpl, cpf, cpb, cpu, cpd);
+    if (weno) VLS_HIP(launch_wenork3_extend_step(step, deltat, lsf, vx, vy, vz, d, cp, g, S(stream)));
+    else VLS_HIP(launch_enork2_extend_step(step, deltat, lsf, vx, vy, vz, d, cp, g, S(stream)));
+    return VLS_OK;
+}
+
+int vls_enork2_extend_step(double* step, const double* deltat, const double* lsf, const double* vx, const double* vy,
+                           const double* vz, const double* xpr, const double* xpl, const double* ypf,
+                           const double* ypb, const double* zpu, const double* zpd, const double* cpr,
+                           const double* cpl, const double* cpf, const double* cpb, const double* cpu,
+                           const double* cpd, int rows, int cols, int pges, double dx, double dy, double dz,
+                           int num_ele, vls_stream_t stream)
+{
+    return variant_extend_dropin(0, step, deltat, lsf, vx, vy, vz, xpr, xpl, ypf, ypb, zpu, zpd, cpr, cpl, cpf, cpb,
+                                 cpu, cpd, rows, cols, pges, dx, dy, dz, num_ele, stream);
+}
+
+int vls_wenork3_extend_step(double* step, const double* deltat, const double* lsf, const double* vx,
+                            const double* vy, const double* vz, const double* xpr, const double* xpl,
+                            const double* ypf, const double* ypb, const double* zpu, const double* zpd,
+                            const double* cpr, const double* cpl, const double* cpf, const double* cpb,
+                            const double* cpu, const double* cpd, int rows, int cols, int pges, double dx, double dy,
+                            double dz, int num_ele, vls_stream_t stream)
+{
+    return variant_extend_dropin(1, step, deltat, lsf, vx, vy, vz, xpr, xpl, ypf, ypb, zpu, zpd, cpr, cpl, cpf, cpb,
+                                 cpu, cpd, rows, cols, pges, dx, dy, dz, num_ele, stream);
+}
+
 int vls_advection_step(double* astep, const double* vx, const double* vy, const double* vz, const double* lsf,
                        int rows, int cols, int pges, double dx, double dy, double dz, vls_stream_t stream)
 {
@@ -802,6 +937,76 @@ int vls_sdf3_eno2_reinit(vls_sdf3_t h, double* F, int iterations, int rk2, doubl
     }
     if (cur != F) VLS_HIP(launch_copy<double>(F, cur, g.n(), s));
     return VLS_OK;
+}
+
+// WENORK3Reinitialization (+SD/@SDF3/WENORK3Reinitialization.m:3-61): cubic
+// distances of the argument (:17-21), mask = F<0, dt = 0.3*min (:23-26), RK3
+// (:30-57); the handle's auxiliary distance set holds the distances
+int vls_sdf3_wenork3_reinit(vls_sdf3_t h, double* F, int iterations, vls_stream_t stream)
+{
+    VLS_REQUIRE(h && F, "null argument");
+    VLS_REQUIRE(iterations >= 0, "negative iteration count");
+    const hipStream_t s = S(stream);
+    const Geom& g = h->g;
+    int rc = set_distance_into<double>(h, 3, aux_set(h), nullptr, F, F, s);
+    if (rc) return rc;
+    const Dist6c dc = constify(h->ad);
+    double *t0 = h->t[0], *t1 = h->t[1];
+    for (int it = 0; it < iterations; ++it) {
+        VLS_HIP(launch_wenork3_reinit_stage(t0, F, nullptr, h->aflags, dc, g, 0, s));
+        VLS_HIP(launch_wenork3_reinit_stage(t1, t0, F, h->aflags, dc, g, 1, s));
+        VLS_HIP(launch_wenork3_reinit_stage(F, t1, F, h->aflags, dc, g, 2, s));
+    }
+    return VLS_OK;
+}
+
+// ENORK2Extend (+SD/@SDF3/ENORK2Extend.m:9-81) and WENORK3Extend
+// (+SD/@SDF3/WENORK3Extend.m:9-98): distances of F (quadratic ENO or cubic),
+// v = sign(F) grad/max(|grad|,1e-6) from the variant's upwind normal, the
+// interface values of C once, then Heun RK2 or Shu-Osher RK3 steps of C in
+// place.  The auxiliary distance set, h->v and the full 6-row interface-value
+// block h->lcp are the work arrays.
+static int variant_extend(vls_sdf3* h, const double* F, double* C, int iterations, int weno, hipStream_t s)
+{
+    VLS_REQUIRE(h && F && C, "null argument");
+    VLS_REQUIRE(iterations >= 0, "negative iteration count");
+    VLS_REQUIRE((const void*)F != (const void*)C, "C must not alias F");
+    const Geom& g = h->g;
+    const int64_t n = g.n();
+    int rc = set_distance_into<double>(h, weno ? 3 : 2, aux_set(h), nullptr, F, nullptr, s);
+    if (rc) return rc;
+    const Dist6c dc = constify(h->ad);
+    double *vx = h->v[0], *vy = h->v[1], *vz = h->v[2];
+    if (weno) VLS_HIP(launch_wenork3_upwind_normal(vx, vy, vz, F, dc, g, s));
+    else VLS_HIP(launch_enork2_upwind_normal(vx, vy, vz, F, dc, g, s));
+    VLS_HIP(launch_extension_velocity<double>(vx, vy, vz, F, 1e-6, g, s));
+    double* q = h->lcp;
+    const Dist6 cp = mk6(q, q + n, q + 2 * n, q + 3 * n, q + 4 * n, q + 5 * n);
+    if (weno) VLS_HIP(launch_wenork3_boundary_interpolate(cp, dc, C, g, s));
+    else VLS_HIP(launch_enork2_boundary_interpolate(cp, dc, C, g, s));
+    const Dist6c cpc = constify(cp);
+    double *t0 = h->t[0], *t1 = h->t[1];
+    for (int it = 0; it < iterations; ++it) {
+        if (weno) {
+            VLS_HIP(launch_wenork3_extend_stage(t0, C, nullptr, vx, vy, vz, dc, cpc, g, 0, s));
+            VLS_HIP(launch_wenork3_extend_stage(t1, t0, C, vx, vy, vz, dc, cpc, g, 1, s));
+            VLS_HIP(launch_wenork3_extend_stage(C, t1, C, vx, vy, vz, dc, cpc, g, 2, s));
+        } else {
+            VLS_HIP(launch_enork2_extend_stage(t0, C, nullptr, vx, vy, vz, dc, cpc, g, 0, s));
+            VLS_HIP(launch_enork2_extend_stage(C, t0, C, vx, vy, vz, dc, cpc, g, 1, s));
+        }
+    }
+    return VLS_OK;
+}
+
+int vls_sdf3_enork2_extend(vls_sdf3_t h, const double* F, double* C, int iterations, vls_stream_t stream)
+{
+    return variant_extend(h, F, C, iterations, 0, S(stream));
+}
+
+int vls_sdf3_wenork3_extend(vls_sdf3_t h, const double* F, double* C, int iterations, vls_stream_t stream)
+{
+    return variant_extend(h, F, C, iterations, 1, S(stream));
 }
 
 int vls_sdf3_weno5_rk3_extend(vls_sdf3_t h, const double* F, double* C, int iterations, vls_stream_t stream)

@@ -428,6 +428,34 @@ __device__ __forceinline__ double eno_quadratic_distance(double v1, double v0, d
     return ds * v0 / (v0 - v2);
 }
 
+// Cubic crossing distance of the WENO-RK3 variants (cubic_distance,
+// 1_1_WENORK3_Reinitialization/weno_rk3_reinitialization.cu:74-117): the
+// cubic through (0,v0),(s,v1),(2s,v2),(3s,v3) with v1*v2 < 0, Newton from the
+// linear guess (at most 50 steps, until |f/d| <= 1e-14), the root clamped to
+// [s, 2s]; next to a kink the quadratic ENO distance (sp, :55-69).  The
+// reference's pow(s,2) and pow(s,3) are s*s and s*s*s; f/d is formed once.
+__device__ __forceinline__ double cubic_distance(double v0, double v1, double v2, double v3, double s)
+{
+#pragma clang fp contract(off)
+    if (v0 * v1 < 0 || v2 * v3 < 0) return eno_quadratic_distance(v0, v1, v2, v3, s);
+    const double c0 = v0;
+    const double c1 = (3.0 * (v1 - v0) - 3.0 / 2.0 * (v2 - v0) + 1.0 / 3.0 * (v3 - v0)) / s;
+    const double c2 = (-5.0 / 2.0 * (v1 - v0) + 2.0 * (v2 - v0) - 1.0 / 2.0 * (v3 - v0)) / (s * s);
+    const double c3 = (1.0 / 2.0 * (v1 - v0) - 1.0 / 2.0 * (v2 - v0) + 1.0 / 6.0 * (v3 - v0)) / (s * s * s);
+    double xc = s + s * v1 / (v1 - v2);
+    double diff = 1;
+    for (int iter = 0; diff > 1e-14 && iter < 50; ++iter) {
+        const double f = c0 + c1 * xc + c2 * xc * xc + c3 * xc * xc * xc;
+        const double d = c1 + 2 * c2 * xc + 3 * c3 * xc * xc;
+        const double q = f / d;
+        diff = fabs(q);
+        xc = xc - q;
+    }
+    xc = dmax(xc, s);
+    xc = dmin(xc, 2 * s);
+    return xc - s;
+}
+
 // Lagrange -> monomial coefficients of the quintic through (x_i, y_i)
 // (Poly<5>::setInterpolation, shared_utilities.hpp:57-103).  The monomial form
 // is ill-conditioned (~1e-11 relative near the data), so this and the
@@ -566,6 +594,164 @@ __device__ __noinline__ Pair sixth_interp(double dis_f, double dis_b, double s, 
         out.back = poly5_eval(coef, 3 * s - dis_b);
     }
     return out;
+}
+
+// ---------------------------------------------------------------------------
+// WENO-RK3 and ENO-RK2 variants of the reinitialization and extension
+// (1_1_WENORK3_Reinitialization, 2_0_ENORK2_Extend, 2_1_WENORK3_Extend)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double min_abs(double x, double y) { return (fabs(x) < fabs(y)) ? x : y; }
+
+// select_stencil (weno_rk3_reinitialization.cu:149-184, with the interface
+// values h_fore/h_back of wenork3_extend.cu:88-123): nodes x[0..6] = x3m..x3
+// (x[3] = 0 the point) and values h[0..6] from p[0..6], the crossing taking
+// the place of the first node on a side whose distance is below ds
+__device__ __forceinline__ void cubic_stencil(double x[7], double h[7], const double p[7], double r1, double l1,
+                                              double ds, double h_fore, double h_back)
+{
+    x[3] = 0.0;
+    h[3] = p[3];
+    if (r1 < ds) {
+        x[4] = r1; x[5] = ds; x[6] = 2 * ds;
+        h[4] = h_fore; h[5] = p[4]; h[6] = p[5];
+    } else {
+        x[4] = ds; x[5] = 2 * ds; x[6] = 3 * ds;
+        h[4] = p[4]; h[5] = p[5]; h[6] = p[6];
+    }
+    if (l1 < ds) {
+        x[2] = -l1; x[1] = -ds; x[0] = -2 * ds;
+        h[2] = h_back; h[1] = p[2]; h[0] = p[1];
+    } else {
+        x[2] = -ds; x[1] = -2 * ds; x[0] = -3 * ds;
+        h[2] = p[2]; h[1] = p[1]; h[0] = p[0];
+    }
+}
+
+// ENO_cubic_derivative (weno_rk3_reinitialization.cu:190-222, the same text in
+// wenork3_extend.cu:129-164): cubic ENO one-sided derivatives at x[3] on the
+// nonuniform stencil, from Newton divided differences
+__device__ __forceinline__ void eno_cubic_pair(double& d_fore, double& d_back, const double x[7], const double h[7])
+{
+#pragma clang fp contract(off)
+    const double d1_25 = (h[6] - h[5]) / (x[6] - x[5]);
+    const double d1_15 = (h[5] - h[4]) / (x[5] - x[4]);
+    const double d1_05 = (h[4] - h[3]) / (x[4] - x[3]);
+    const double d1_m05 = (h[3] - h[2]) / (x[3] - x[2]);
+    const double d1_m15 = (h[2] - h[1]) / (x[2] - x[1]);
+    const double d1_m25 = (h[1] - h[0]) / (x[1] - x[0]);
+    const double d2_2 = (d1_25 - d1_15) / (x[6] - x[4]);
+    const double d2_1 = (d1_15 - d1_05) / (x[5] - x[3]);
+    const double d2_0 = (d1_05 - d1_m05) / (x[4] - x[2]);
+    const double d2_m1 = (d1_m05 - d1_m15) / (x[3] - x[1]);
+    const double d2_m2 = (d1_m15 - d1_m25) / (x[2] - x[0]);
+    const double d3_15 = (d2_2 - d2_1) / (x[6] - x[3]);
+    const double d3_05 = (d2_1 - d2_0) / (x[5] - x[2]);
+    const double d3_m05 = (d2_0 - d2_m1) / (x[4] - x[1]);
+    const double d3_m15 = (d2_m1 - d2_m2) / (x[3] - x[0]);
+    const double a1 = (x[3] - x[2]) * (x[3] - x[1]) * min_abs(d3_m05, d3_m15);
+    const double a2 = (x[3] - x[2]) * (x[3] - x[4]) * min_abs(d3_m05, d3_05);
+    const double a = (fabs(d2_m1) < fabs(d2_0)) ? a1 : a2;
+    const double b1 = (x[3] - x[2]) * (x[3] - x[4]) * min_abs(d3_m05, d3_05);
+    const double b2 = (x[3] - x[4]) * (x[3] - x[5]) * min_abs(d3_05, d3_15);
+    const double b = (fabs(d2_0) < fabs(d2_1)) ? b1 : b2;
+    d_back = d1_m05 + min_mod(d2_m1, d2_0) * (x[3] - x[2]) + a;
+    d_fore = d1_05 + min_mod(d2_0, d2_1) * (x[3] - x[4]) + b;
+}
+
+// weno_derivative_boundary of the WENO-RK3 variants: cubic ENO on the
+// interface stencil where `cross`, else the uniform WENO5 pair (the same
+// weno_onesided_derivative as re_step.cu:11-37, in the Jiang-Peng form).
+// The reinitialization tests a sign change of p (weno_rk3_reinitialization.cu:228-247,
+// interface values 0), the extension a subcell distance (wenork3_extend.cu:167-190).
+__device__ __forceinline__ void wenork3_pair(double& fore, double& back, const double p[7], double r1, double l1,
+                                             double ds, JPd jp, bool cross, double h_fore, double h_back)
+{
+    if (cross) {
+        double x[7], h[7];
+        cubic_stencil(x, h, p, r1, l1, ds, h_fore, h_back);
+        eno_cubic_pair(fore, back, x, h);
+    } else {
+        weno5_pair(fore, back, p, jp);
+    }
+}
+
+// eno_derivative_field (enork2_extend.cu:56-78): ENO2 one-sided derivatives
+// with the interface values v_fore/v_back where a distance differs from ds
+__device__ __forceinline__ void eno2_field_pair(double& sR, double& sL, const double p[7], double dis_f, double dis_b,
+                                                double ds, double v_fore, double v_back)
+{
+#pragma clang fp contract(off)
+    const double v4 = p[1], v1 = p[2], v0 = p[3], v2 = p[4], v3 = p[5];
+    const double p2 = v1 - 2.0 * v0 + v2;
+    const double p2r = v0 - 2.0 * v2 + v3;
+    double p2m = 0.5 * min_mod(p2, p2r) / (ds * ds);
+    const double v_f = (dis_f == ds) ? v2 : v_fore;
+    sR = (v_f - v0) / dis_f - dis_f * p2m;
+    const double p2l = v0 - 2.0 * v1 + v4;
+    p2m = 0.5 * min_mod(p2, p2l) / (ds * ds);
+    const double v_b = (dis_b == ds) ? v1 : v_back;
+    sL = (v0 - v_b) / dis_b + dis_b * p2m;
+}
+
+// ENO2 interface values (cubic_eno_interp, enork2_extend.cu:144-170): the
+// quadratic ENO interpolant through the two nodes around a crossing, at the
+// crossing, for a side whose distance differs from ds (0 otherwise)
+__device__ __forceinline__ void eno2_interface_values(double& c_fore, double& c_back, double dis_f, double dis_b,
+                                                      double ds, const double p[7])
+{
+#pragma clang fp contract(off)
+    const double v4 = p[1], v1 = p[2], v0 = p[3], v2 = p[4], v3 = p[5];
+    c_fore = 0;
+    c_back = 0;
+    if (dis_f != ds) {
+        const double p2 = v2 - 2.0 * v0 + v1;
+        const double p2r = v3 - 2.0 * v2 + v0;
+        const double c2 = 0.5 * min_mod(p2, p2r) / (ds * ds);
+        const double c1 = (v2 - v0) / ds;
+        const double c0 = (v2 + v0) / 2 - c2 * (ds * ds) / 4;
+        const double x = dis_f - ds / 2;
+        c_fore = c2 * x * x + c1 * x + c0;
+    }
+    if (dis_b != ds) {
+        const double p2 = v2 - 2.0 * v0 + v1;
+        const double p2l = v0 - 2.0 * v1 + v4;
+        const double c2 = 0.5 * min_mod(p2, p2l) / (ds * ds);
+        const double c1 = (v0 - v1) / ds;
+        const double c0 = (v1 + v0) / 2 - c2 * (ds * ds) / 4;
+        const double x = ds / 2 - dis_b;
+        c_back = c2 * x * x + c1 * x + c0;
+    }
+}
+
+// cubic interface values (cubic_interp, wenork3_extend.cu:310-329): the cubic
+// through the four nodes around a crossing (cubic_interp_coefficient, :297-306),
+// at the crossing, for a side whose distance is below ds (0 otherwise).  The
+// reference's pow(xc,2), pow(xc,3) are xc*xc and xc*xc*xc.
+__device__ __forceinline__ void cubic_coefficients(double c[4], double v0, double v1, double v2, double v3, double s)
+{
+#pragma clang fp contract(off)
+    c[0] = v0;
+    c[1] = (3.0 * (v1 - v0) - 3.0 / 2.0 * (v2 - v0) + 1.0 / 3.0 * (v3 - v0)) / s;
+    c[2] = (-5.0 / 2.0 * (v1 - v0) + 2.0 * (v2 - v0) - 1.0 / 2.0 * (v3 - v0)) / (s * s);
+    c[3] = (1.0 / 2.0 * (v1 - v0) - 1.0 / 2.0 * (v2 - v0) + 1.0 / 6.0 * (v3 - v0)) / (s * s * s);
+}
+__device__ __forceinline__ void cubic_interface_values(double& c_fore, double& c_back, double dis_f, double dis_b,
+                                                       double ds, const double p[7])
+{
+#pragma clang fp contract(off)
+    c_fore = 0.;
+    c_back = 0.;
+    double c[4];
+    if (dis_f < ds) {
+        cubic_coefficients(c, p[2], p[3], p[4], p[5], ds);
+        const double xc = ds + dis_f;
+        c_fore = c[0] + c[1] * xc + c[2] * (xc * xc) + c[3] * (xc * xc * xc);
+    }
+    if (dis_b < ds) {
+        cubic_coefficients(c, p[1], p[2], p[3], p[4], ds);
+        const double xc = 2 * ds - dis_b;
+        c_back = c[0] + c[1] * xc + c[2] * (xc * xc) + c[3] * (xc * xc * xc);
+    }
 }
 
 }  // namespace vls

@@ -34,8 +34,10 @@ __global__ void k_fill_spacing(Dist6 d, Box b, idx_t n, int* __restrict__ list_c
     d.r[2][i] = b.dz; d.l[2][i] = b.dz;
 }
 
-// boundary_location (order 6, 1_3_WENO5RK3_Reinitialization/boundary_location.cu:84-132)
-// and boundary_correction (order 2, 1_0_ENORK2_Reinitialization/boundary_correction.cu:103-145).
+// boundary_location (order 6, 1_3_WENO5RK3_Reinitialization/boundary_location.cu:84-132),
+// boundary_correction (order 2, 1_0_ENORK2_Reinitialization/boundary_correction.cu:103-145)
+// and the cubic boundary_correction (order 3,
+// 1_1_WENORK3_Reinitialization/weno_rk3_reinitialization.cu:251-292).
 // One writer per element: r-side at ind, l-side at the +1 neighbour.
 template <int ORDER, typename T>
 __global__ void __launch_bounds__(256) k_boundary_location(Dist6 d, const T* __restrict__ lsf, Box b)
@@ -57,8 +59,9 @@ __global__ void __launch_bounds__(256) k_boundary_location(Dist6 d, const T* __r
         double v[7];
         line7(lsf, r, c, p, axis, b, v);
         const double ds = sp[axis];
-        const double dist = (ORDER == 2) ? eno_quadratic_distance(v[2], f0, f2, v[5], ds)
-                                         : six_distance(v[1], v[2], f0, f2, v[5], v[6], ds);
+        const double dist = (ORDER == 2)   ? eno_quadratic_distance(v[2], f0, f2, v[5], ds)
+                            : (ORDER == 3) ? cubic_distance(v[2], f0, f2, v[5], ds)
+                                           : six_distance(v[1], v[2], f0, f2, v[5], v[6], ds);
         d.r[axis][ind] = dist;
         d.l[axis][nb] = ds - dist;
     }
@@ -833,6 +836,7 @@ hipError_t launch_boundary_location(int order, const Dist6& d, const T* lsf, con
     dim3 grid, block;
     point_grid(g, grid, block);
     if (order == 2) k_boundary_location<2, T><<<grid, block, 0, s>>>(d, lsf, box_of(g));
+    else if (order == 3) k_boundary_location<3, T><<<grid, block, 0, s>>>(d, lsf, box_of(g));
     else k_boundary_location<6, T><<<grid, block, 0, s>>>(d, lsf, box_of(g));
     return hipGetLastError();
 }

@@ -98,6 +98,34 @@ template <typename T>
 hipError_t launch_upwind_normal(T* nx, T* ny, T* nz, const T* lsf, const Dist6c& d, const Geom& g, hipStream_t s);
 template <typename T>
 hipError_t launch_extension_velocity(T* vx, T* vy, T* vz, const T* F, double floor_, const Geom& g, hipStream_t s);
+
+// WENO-RK3 / ENO-RK2 variants (vls_variants.hip), FP64.  Drop-in kernels:
+hipError_t launch_wenork3_re_step(double* step, const double* lsf, const uint8_t* mask, const double* deltat,
+                                  const Dist6c& d, const Geom& g, hipStream_t s);
+hipError_t launch_enork2_upwind_normal(double* nx, double* ny, double* nz, const double* lsf, const Dist6c& d,
+                                       const Geom& g, hipStream_t s);
+hipError_t launch_enork2_boundary_interpolate(const Dist6& cp, const Dist6c& d, const double* lsf, const Geom& g,
+                                              hipStream_t s);
+hipError_t launch_enork2_extend_step(double* step, const double* deltat, const double* lsf, const double* vx,
+                                     const double* vy, const double* vz, const Dist6c& d, const Dist6c& cp,
+                                     const Geom& g, hipStream_t s);
+hipError_t launch_wenork3_upwind_normal(double* nx, double* ny, double* nz, const double* lsf, const Dist6c& d,
+                                        const Geom& g, hipStream_t s);
+hipError_t launch_wenork3_boundary_interpolate(const Dist6& cp, const Dist6c& d, const double* lsf, const Geom& g,
+                                               hipStream_t s);
+hipError_t launch_wenork3_extend_step(double* step, const double* deltat, const double* lsf, const double* vx,
+                                      const double* vy, const double* vz, const Dist6c& d, const Dist6c& cp,
+                                      const Geom& g, hipStream_t s);
+// fused stages of the wrapper loops (operator + RK combination; dt = 0.3*min
+// of the six distances d; cp the interface values)
+hipError_t launch_wenork3_reinit_stage(double* out, const double* fs, const double* fn, const uint8_t* flags,
+                                       const Dist6c& d, const Geom& g, int stage, hipStream_t s);
+hipError_t launch_wenork3_extend_stage(double* out, const double* cs, const double* cn, const double* vx,
+                                       const double* vy, const double* vz, const Dist6c& d, const Dist6c& cp,
+                                       const Geom& g, int stage, hipStream_t s);
+hipError_t launch_enork2_extend_stage(double* out, const double* cs, const double* cn, const double* vx,
+                                      const double* vy, const double* vz, const Dist6c& d, const Dist6c& cp,
+                                      const Geom& g, int heun2, hipStream_t s);
 hipError_t launch_boundary_interpolant(const Dist6& cp, const Dist6c& d, const double* C, const Geom& g,
                                        hipStream_t s);
 hipError_t launch_extend_step(double* step, const double* deltat, const double* C, const double* vx,

@@ -195,10 +195,36 @@ class SDF3:
              _stream())
         return NewF
 
+    def WENORK3Reinitialization(self, F, iteration):
+        """+SD/@SDF3/WENORK3Reinitialization.m:3-61: cubic distances of F, mask = F<0,
+        RK3 of the WENO-RK3 re_step.  FP64 only."""
+        if self.dtype != torch.float64:
+            raise VLSError("WENORK3Reinitialization is FP64 only")
+        NewF = F.clone()
+        call("vls_sdf3_wenork3_reinit", self._h, self._check(NewF), int(iteration), _stream())
+        return NewF
+
     # -- extension and surface redistance -------------------------------------
     def WENO5RK3Extend(self, C, iteration):
         """+SD/@SDF3/WENO5RK3Extend.m:9-76 (self.F and its last setDistance define the interface)."""
         return self.extend_(C.clone(), iteration)
+
+    def ENORK2Extend(self, C, iteration):
+        """+SD/@SDF3/ENORK2Extend.m:9-81: ENO2 extension of C off the zero set of self.F
+        (its own quadratic-ENO distances), Heun RK2.  FP64 only."""
+        return self._variant_extend("vls_sdf3_enork2_extend", "ENORK2Extend", C, iteration)
+
+    def WENORK3Extend(self, C, iteration):
+        """+SD/@SDF3/WENORK3Extend.m:9-98: WENO-RK3 extension of C off the zero set of
+        self.F (cubic distances and interface values), Shu-Osher RK3.  FP64 only."""
+        return self._variant_extend("vls_sdf3_wenork3_extend", "WENORK3Extend", C, iteration)
+
+    def _variant_extend(self, fn, what, C, iteration):
+        if self.dtype != torch.float64:
+            raise VLSError(f"{what} is FP64 only")
+        NewC = C.clone()
+        call(fn, self._h, self._check(self.F), self._check(NewC), int(iteration), _stream())
+        return NewC
 
     def WENO5RK3ClosetPointSurfaceRedistance(self, A, iter1, iter2):
         """+SD/@SDF3/WENO5RK3ClosetPointSurfaceRedistance.m:4-157"""
@@ -353,6 +379,65 @@ def feval_extend_step(step, deltat, lsf, vx, vy, vz, xpr, xpl, ypf, ypb, zpu, zp
                                                  cpl, cpf, cpb, cpu, cpd)], rows, cols, pges, dx, dy, dz, num_ele,
          _stream())
     return step
+
+
+# WENO-RK3 / ENO-RK2 variants (cubic_boundary_correction, WENORK3_*, ENORK2_* of
+# +SD/@SDF3/GPUInitialize.m:32-42, 64-102)
+def feval_cubic_boundary_correction(xpr, xpl, ypf, ypb, zpu, zpd, lsf, num_ele, rows, cols, pges, dx, dy, dz):
+    call("vls_cubic_boundary_correction", *[_ptr(t) for t in (xpr, xpl, ypf, ypb, zpu, zpd, lsf)], num_ele, rows,
+         cols, pges, dx, dy, dz, _stream())
+    return xpr, xpl, ypf, ypb, zpu, zpd
+
+
+def feval_wenork3_re_step(step, lsf, mask, deltat, xpr, xpl, ypf, ypb, zpu, zpd, rows, cols, pges, dx, dy, dz,
+                          num_ele):
+    call("vls_wenork3_re_step", _ptr(step), _ptr(lsf), _ptr(mask, torch.uint8), _ptr(deltat),
+         *[_ptr(t) for t in (xpr, xpl, ypf, ypb, zpu, zpd)], rows, cols, pges, dx, dy, dz, num_ele, _stream())
+    return step
+
+
+def _variant_normal(name, nx, ny, nz, lsf, xpr, xpl, ypf, ypb, zpu, zpd, rows, cols, pges, dx, dy, dz, num_ele):
+    call(name, *[_ptr(t) for t in (nx, ny, nz, lsf, xpr, xpl, ypf, ypb, zpu, zpd)], rows, cols, pges, dx, dy, dz,
+         num_ele, _stream())
+    return nx, ny, nz
+
+
+def feval_enork2_upwind_normal(*args):
+    return _variant_normal("vls_enork2_upwind_normal", *args)
+
+
+def feval_wenork3_upwind_normal(*args):
+    return _variant_normal("vls_wenork3_upwind_normal", *args)
+
+
+def _variant_interpolate(name, cpr, cpl, cpf, cpb, cpu, cpd, xpr, xpl, ypf, ypb, zpu, zpd, lsf, rows, cols, pges,
+                         dx, dy, dz, num_ele):
+    call(name, *[_ptr(t) for t in (cpr, cpl, cpf, cpb, cpu, cpd, xpr, xpl, ypf, ypb, zpu, zpd, lsf)], rows, cols,
+         pges, dx, dy, dz, num_ele, _stream())
+    return cpr, cpl, cpf, cpb, cpu, cpd
+
+
+def feval_enork2_boundary_interpolate(*args):
+    return _variant_interpolate("vls_enork2_boundary_interpolate", *args)
+
+
+def feval_wenork3_boundary_interpolate(*args):
+    return _variant_interpolate("vls_wenork3_boundary_interpolate", *args)
+
+
+def _variant_extend_step(name, step, deltat, lsf, vx, vy, vz, xpr, xpl, ypf, ypb, zpu, zpd, cpr, cpl, cpf, cpb,
+                         cpu, cpd, rows, cols, pges, dx, dy, dz, num_ele):
+    call(name, *[_ptr(t) for t in (step, deltat, lsf, vx, vy, vz, xpr, xpl, ypf, ypb, zpu, zpd, cpr, cpl, cpf, cpb,
+                                   cpu, cpd)], rows, cols, pges, dx, dy, dz, num_ele, _stream())
+    return step
+
+
+def feval_enork2_extend_step(*args):
+    return _variant_extend_step("vls_enork2_extend_step", *args)
+
+
+def feval_wenork3_extend_step(*args):
+    return _variant_extend_step("vls_wenork3_extend_step", *args)
 
 
 def feval_advection_step(astep, vx, vy, vz, lsf, rows, cols, pges, dx, dy, dz):

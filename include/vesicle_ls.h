@@ -90,6 +90,67 @@ int vls_extend_step(double *step, const double *deltat, const double *lsf, const
                     const double *cpb, const double *cpu, const double *cpd, int rows, int cols, int pges,
                     double dx, double dy, double dz, int num_ele, vls_stream_t stream);
 
+/* WENO-RK3 / ENO-RK2 variants.  boundary_correction (cubic):
+ * CUDA_Code/1_1_WENORK3_Reinitialization/weno_rk3_reinitialization.cu:251-252 (bound
+ * as cubic_boundary_correction, +SD/@SDF3/GPUInitialize.m:32-36).  Cubic
+ * subcell distances (Newton on the cubic through 4 nodes, quadratic ENO next to
+ * a kink); in/out convention of vls_boundary_location. */
+int vls_cubic_boundary_correction(double *xpr, double *xpl, double *ypf, double *ypb, double *zpu, double *zpd,
+                                  const double *lsf, int num_ele, int rows, int cols, int pges, double dx,
+                                  double dy, double dz, vls_stream_t stream);
+
+/* re_step (WENO-RK3): weno_rk3_reinitialization.cu:295-296 (WENORK3_reinitialization_step,
+ * GPUInitialize.m:38-42).  Cubic ENO next to a sign change, WENO5 elsewhere;
+ * frozen where a distance is below 1e-6 dx. */
+int vls_wenork3_re_step(double *step, const double *lsf, const uint8_t *mask, const double *deltat,
+                        const double *xpr, const double *xpl, const double *ypf, const double *ypb,
+                        const double *zpu, const double *zpd, int rows, int cols, int pges, double dx, double dy,
+                        double dz, int num_ele, vls_stream_t stream);
+
+/* upwind_normal (ENO-RK2 extension): CUDA_Code/2_0_ENORK2_Extend/enork2_extend.cu:102-103
+ * (ENORK2_upwind_normal, GPUInitialize.m:64-68). */
+int vls_enork2_upwind_normal(double *nx, double *ny, double *nz, const double *lsf, const double *xpr,
+                             const double *xpl, const double *ypf, const double *ypb, const double *zpu,
+                             const double *zpd, int rows, int cols, int pges, double dx, double dy, double dz,
+                             int num_ele, vls_stream_t stream);
+
+/* boundary_interpolate (ENO-RK2): enork2_extend.cu:174-175 (ENORK2_boundary_interpolate,
+ * GPUInitialize.m:76-80).  Quadratic ENO values of lsf at the crossings; 0 elsewhere. */
+int vls_enork2_boundary_interpolate(double *cpr, double *cpl, double *cpf, double *cpb, double *cpu, double *cpd,
+                                    const double *xpr, const double *xpl, const double *ypf, const double *ypb,
+                                    const double *zpu, const double *zpd, const double *lsf, int rows, int cols,
+                                    int pges, double dx, double dy, double dz, int num_ele, vls_stream_t stream);
+
+/* extend_step (ENO-RK2): enork2_extend.cu:213-214 (ENORK2_extend_step, GPUInitialize.m:70-74). */
+int vls_enork2_extend_step(double *step, const double *deltat, const double *lsf, const double *vx, const double *vy,
+                           const double *vz, const double *xpr, const double *xpl, const double *ypf,
+                           const double *ypb, const double *zpu, const double *zpd, const double *cpr,
+                           const double *cpl, const double *cpf, const double *cpb, const double *cpu,
+                           const double *cpd, int rows, int cols, int pges, double dx, double dy, double dz,
+                           int num_ele, vls_stream_t stream);
+
+/* upwind_normal (WENO-RK3 extension): CUDA_Code/2_1_WENORK3_Extend/wenork3_extend.cu:202-203
+ * (WENORK3_upwind_normal, GPUInitialize.m:86-90). */
+int vls_wenork3_upwind_normal(double *nx, double *ny, double *nz, const double *lsf, const double *xpr,
+                              const double *xpl, const double *ypf, const double *ypb, const double *zpu,
+                              const double *zpd, int rows, int cols, int pges, double dx, double dy, double dz,
+                              int num_ele, vls_stream_t stream);
+
+/* boundary_interpolate (WENO-RK3): wenork3_extend.cu:331-332 (WENORK3_boundary_interpolate,
+ * GPUInitialize.m:98-102).  Cubic values of lsf at the crossings; 0 elsewhere. */
+int vls_wenork3_boundary_interpolate(double *cpr, double *cpl, double *cpf, double *cpb, double *cpu, double *cpd,
+                                     const double *xpr, const double *xpl, const double *ypf, const double *ypb,
+                                     const double *zpu, const double *zpd, const double *lsf, int rows, int cols,
+                                     int pges, double dx, double dy, double dz, int num_ele, vls_stream_t stream);
+
+/* extend_step (WENO-RK3): wenork3_extend.cu:372-373 (WENORK3_extend_step, GPUInitialize.m:92-96). */
+int vls_wenork3_extend_step(double *step, const double *deltat, const double *lsf, const double *vx,
+                            const double *vy, const double *vz, const double *xpr, const double *xpl,
+                            const double *ypf, const double *ypb, const double *zpu, const double *zpd,
+                            const double *cpr, const double *cpl, const double *cpf, const double *cpb,
+                            const double *cpu, const double *cpd, int rows, int cols, int pges, double dx, double dy,
+                            double dz, int num_ele, vls_stream_t stream);
+
 /* advection_step: CUDA_Code/4_0_Calculus_ToolBox/advection_step.cu:30-31 (v.grad(lsf), no dt). */
 int vls_advection_step(double *astep, const double *vx, const double *vy, const double *vz, const double *lsf,
                        int rows, int cols, int pges, double dx, double dy, double dz, vls_stream_t stream);
@@ -143,6 +204,21 @@ int vls_sdf3_weno5_rk3_reinit(vls_sdf3_t h, double *F, int iterations, vls_strea
  * distances of the argument F, dt = cfl*min_dist.  rk2 = 1 is the reference's
  * Heun RK2 with cfl 0.3; rk2 = 0 is forward Euler. */
 int vls_sdf3_eno2_reinit(vls_sdf3_t h, double *F, int iterations, int rk2, double cfl, vls_stream_t stream);
+
+/* WENORK3Reinitialization (+SD/@SDF3/WENORK3Reinitialization.m:3-61): F in
+ * place; cubic distances of the argument F, mask = F<0, dt = 0.3*min
+ * distance, Shu-Osher RK3 of the WENO-RK3 re_step. */
+int vls_sdf3_wenork3_reinit(vls_sdf3_t h, double *F, int iterations, vls_stream_t stream);
+
+/* ENORK2Extend (+SD/@SDF3/ENORK2Extend.m:9-81): extends C (in place) off the
+ * zero set of F with ENO2 derivatives, quadratic-ENO distances and interface
+ * values of F and C, Heun RK2. */
+int vls_sdf3_enork2_extend(vls_sdf3_t h, const double *F, double *C, int iterations, vls_stream_t stream);
+
+/* WENORK3Extend (+SD/@SDF3/WENORK3Extend.m:9-98): extends C (in place) off the
+ * zero set of F with cubic distances and interface values, cubic ENO next to
+ * the interface and WENO5 elsewhere, Shu-Osher RK3. */
+int vls_sdf3_wenork3_extend(vls_sdf3_t h, const double *F, double *C, int iterations, vls_stream_t stream);
 
 /* WENO5RK3Extend (+SD/@SDF3/WENO5RK3Extend.m:9-76): extends C (in place) off
  * the zero set of F, which must be the field of the last set_distance. */

@@ -203,8 +203,37 @@ double or_six_distance(double v0, double v1, double v2, double v3, double v4, do
     return xc - 2 * s;
 }
 
-/* Kernel boundary_location (boundary_location.cu:84-132) and, when order==2,
- * boundary_correction (CUDA_Code/1_0_ENORK2_Reinitialization/boundary_correction.cu:103-145).
+/* cubic crossing distance of the WENO-RK3 variants:
+ * CUDA_Code/1_1_WENORK3_Reinitialization/weno_rk3_reinitialization.cu:74-117.  The cubic
+ * through (0,v0),(s,v1),(2s,v2),(3s,v3), v1*v2<0, Newton from the linear
+ * guess until |f/d| <= 1e-14 (at most 50 steps), clamped to [s,2s]; quadratic
+ * ENO (sp, :55-69, argument order (v0,v1,v2,v3)) next to a kink.  pow(s,2),
+ * pow(s,3) are s*s and s*s*s; f/d is formed once per step. */
+double or_cubic_distance(double v0, double v1, double v2, double v3, double s)
+{
+    if (v0 * v1 < 0 || v2 * v3 < 0) return or_eno_quadratic_distance(v0, v1, v2, v3, s);
+    double c0 = v0;
+    double c1 = (3.0 * (v1 - v0) - 3.0 / 2.0 * (v2 - v0) + 1.0 / 3.0 * (v3 - v0)) / s;
+    double c2 = (-5.0 / 2.0 * (v1 - v0) + 2.0 * (v2 - v0) - 1.0 / 2.0 * (v3 - v0)) / (s * s);
+    double c3 = (1.0 / 2.0 * (v1 - v0) - 1.0 / 2.0 * (v2 - v0) + 1.0 / 6.0 * (v3 - v0)) / (s * s * s);
+    double xc = s + s * v1 / (v1 - v2);
+    double diff = 1;
+    for (int iter = 0; diff > 1e-14 && iter < 50; ++iter) {
+        double f = c0 + c1 * xc + c2 * xc * xc + c3 * xc * xc * xc;
+        double d = c1 + 2 * c2 * xc + 3 * c3 * xc * xc;
+        double q = f / d;
+        diff = fabs(q);
+        xc = xc - q;
+    }
+    xc = dmax2(xc, s);
+    xc = dmin2(xc, 2 * s);
+    return xc - s;
+}
+
+/* Kernel boundary_location (boundary_location.cu:84-132), when order==2
+ * boundary_correction (CUDA_Code/1_0_ENORK2_Reinitialization/boundary_correction.cu:103-145),
+ * when order==3 the cubic boundary_correction
+ * (CUDA_Code/1_1_WENORK3_Reinitialization/weno_rk3_reinitialization.cu:251-292).
  * The six distance arrays are in/out: the caller pre-fills them with dx/dy/dz
  * (+SD/@SDF3/setDistance.m:5-10).  Each crossing writes the right-hand value
  * at ind and the left-hand value of the neighbour, exactly as the reference
@@ -231,9 +260,9 @@ static void distances_impl(int order, double *xpr, double *xpl, double *ypf, dou
                     if (axis == 0) cc += 1; else if (axis == 1) rr += 1; else pp += 1;
                     idx_t nb = at(rr, cc, pp, rows, cols, pges);
                     double ds = spacing[axis];
-                    double d = (order == 2)
-                                   ? or_eno_quadratic_distance(v[2], f0, f2, v[5], ds)
-                                   : or_six_distance(v[1], v[2], f0, f2, v[5], v[6], ds);
+                    double d = (order == 2)   ? or_eno_quadratic_distance(v[2], f0, f2, v[5], ds)
+                               : (order == 3) ? or_cubic_distance(v[2], f0, f2, v[5], ds)
+                                              : or_six_distance(v[1], v[2], f0, f2, v[5], v[6], ds);
                     right_out[axis][ind] = d;
                     left_out[axis][nb] = ds - d;
                 }
@@ -254,6 +283,14 @@ void or_boundary_correction(double *xpr, double *xpl, double *ypf, double *ypb, 
 {
     (void)num_ele;
     distances_impl(2, xpr, xpl, ypf, ypb, zpu, zpd, lsf, rows, cols, pges, dx, dy, dz);
+}
+
+void or_cubic_boundary_correction(double *xpr, double *xpl, double *ypf, double *ypb, double *zpu,
+                                  double *zpd, const double *lsf, int num_ele, int rows, int cols,
+                                  int pges, double dx, double dy, double dz)
+{
+    (void)num_ele;
+    distances_impl(3, xpr, xpl, ypf, ypb, zpu, zpd, lsf, rows, cols, pges, dx, dy, dz);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -1085,4 +1122,427 @@ void or_normal_velocity(double *out, const double *F, const double *vx, const do
                 double g = dmax2(sqrt(fx * fx + fy * fy + fz * fz), 1e-14);
                 out[ind] = vx[ind] * (fx / g) + vy[ind] * (fy / g) + vz[ind] * (fz / g);
             }
+}
+
+/* ------------------------------------------------------------------------- */
+/* WENO-RK3 / ENO-RK2 variants of the reinitialization and the extension     */
+/* (CUDA_Code/1_1_WENORK3_Reinitialization, 2_0_ENORK2_Extend,               */
+/* 2_1_WENORK3_Extend; +SD/@SDF3/WENORK3Reinitialization.m, ENORK2Extend.m,   */
+/* WENORK3Extend.m)                                                          */
+/* ------------------------------------------------------------------------- */
+static inline double min_abs(double x, double y) { return (fabs(x) < fabs(y)) ? x : y; }
+
+/* select_stencil: weno_rk3_reinitialization.cu:149-184 (interface value 0),
+ * wenork3_extend.cu:88-123 (interface values h_fore/h_back).  x[0..6] =
+ * x3m..x3, h[0..6] the values; p[0..6] = p1..p7. */
+static void cubic_stencil(double x[7], double h[7], const double p[7], double r1, double l1, double ds,
+                          double h_fore, double h_back)
+{
+    x[3] = 0.0;
+    h[3] = p[3];
+    if (r1 < ds) {
+        x[4] = r1; x[5] = ds; x[6] = 2 * ds;
+        h[4] = h_fore; h[5] = p[4]; h[6] = p[5];
+    } else {
+        x[4] = ds; x[5] = 2 * ds; x[6] = 3 * ds;
+        h[4] = p[4]; h[5] = p[5]; h[6] = p[6];
+    }
+    if (l1 < ds) {
+        x[2] = -l1; x[1] = -ds; x[0] = -2 * ds;
+        h[2] = h_back; h[1] = p[2]; h[0] = p[1];
+    } else {
+        x[2] = -ds; x[1] = -2 * ds; x[0] = -3 * ds;
+        h[2] = p[2]; h[1] = p[1]; h[0] = p[0];
+    }
+}
+
+/* ENO_cubic_derivative: weno_rk3_reinitialization.cu:190-222 (the same text in
+ * wenork3_extend.cu:129-164).  Exported for the tests. */
+void or_eno_cubic_pair(double *d_fore, double *d_back, const double x[7], const double h[7])
+{
+    double d1_25 = (h[6] - h[5]) / (x[6] - x[5]);
+    double d1_15 = (h[5] - h[4]) / (x[5] - x[4]);
+    double d1_05 = (h[4] - h[3]) / (x[4] - x[3]);
+    double d1_m05 = (h[3] - h[2]) / (x[3] - x[2]);
+    double d1_m15 = (h[2] - h[1]) / (x[2] - x[1]);
+    double d1_m25 = (h[1] - h[0]) / (x[1] - x[0]);
+    double d2_2 = (d1_25 - d1_15) / (x[6] - x[4]);
+    double d2_1 = (d1_15 - d1_05) / (x[5] - x[3]);
+    double d2_0 = (d1_05 - d1_m05) / (x[4] - x[2]);
+    double d2_m1 = (d1_m05 - d1_m15) / (x[3] - x[1]);
+    double d2_m2 = (d1_m15 - d1_m25) / (x[2] - x[0]);
+    double d3_15 = (d2_2 - d2_1) / (x[6] - x[3]);
+    double d3_05 = (d2_1 - d2_0) / (x[5] - x[2]);
+    double d3_m05 = (d2_0 - d2_m1) / (x[4] - x[1]);
+    double d3_m15 = (d2_m1 - d2_m2) / (x[3] - x[0]);
+    double a1 = (x[3] - x[2]) * (x[3] - x[1]) * min_abs(d3_m05, d3_m15);
+    double a2 = (x[3] - x[2]) * (x[3] - x[4]) * min_abs(d3_m05, d3_05);
+    double a = (fabs(d2_m1) < fabs(d2_0)) ? a1 : a2;
+    double b1 = (x[3] - x[2]) * (x[3] - x[4]) * min_abs(d3_m05, d3_05);
+    double b2 = (x[3] - x[4]) * (x[3] - x[5]) * min_abs(d3_05, d3_15);
+    double b = (fabs(d2_0) < fabs(d2_1)) ? b1 : b2;
+    *d_back = d1_m05 + min_mod(d2_m1, d2_0) * (x[3] - x[2]) + a;
+    *d_fore = d1_05 + min_mod(d2_0, d2_1) * (x[3] - x[4]) + b;
+}
+
+/* weno_derivative_boundary of the variants: rule 0 = the reinitialization's
+ * sign-change test with interface value 0 (weno_rk3_reinitialization.cu:228-247),
+ * rule 1 = the extension's subcell-distance test with the interface values
+ * v_fore/v_back (wenork3_extend.cu:167-190).  The uniform branch is the WENO5
+ * pair of re_step.cu:11-37 (or_weno_pair). */
+void or_wenork3_derivative_boundary(double *d_fore, double *d_back, const double p[7], double r1,
+                                    double l1, double ds, double v_fore, double v_back, int rule)
+{
+    int cross = rule == 0 ? (p[2] * p[3] < 0 || p[3] * p[4] < 0) : (r1 < ds || l1 < ds);
+    if (cross) {
+        double x[7], h[7];
+        if (rule == 0) { v_fore = 0.0; v_back = 0.0; }
+        cubic_stencil(x, h, p, r1, l1, ds, v_fore, v_back);
+        or_eno_cubic_pair(d_fore, d_back, x, h);
+    } else {
+        weno_uniform_pair(d_fore, d_back, p, ds);
+    }
+}
+
+/* Kernel re_step (WENO-RK3): weno_rk3_reinitialization.cu:295-407 */
+void or_wenork3_re_step(double *step, const double *lsf, const uint8_t *mask, const double *deltat,
+                        SIX_DIST_ARGS, int rows, int cols, int pges, double dx, double dy, double dz,
+                        int num_ele)
+{
+    (void)num_ele;
+    const double *R[3] = {xpr, ypf, zpu}, *L[3] = {xpl, ypb, zpd};
+    const double sp[3] = {dx, dy, dz};
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int p = 0; p < pges; ++p)
+        for (int c = 0; c < cols; ++c)
+            for (int r = 0; r < rows; ++r) {
+                idx_t ind = at(r, c, p, rows, cols, pges);
+                double eps = 1e-6 * dx;
+                if (xpr[ind] < eps || xpl[ind] < eps || ypf[ind] < eps || ypb[ind] < eps ||
+                    zpu[ind] < eps || zpd[ind] < eps) {
+                    step[ind] = 0;
+                    continue;
+                }
+                double fore[3], back[3];
+                for (int a = 0; a < 3; ++a) {
+                    double v[7];
+                    line7(lsf, r, c, p, a, rows, cols, pges, v);
+                    or_wenork3_derivative_boundary(&fore[a], &back[a], v, R[a][ind], L[a][ind], sp[a],
+                                                   0.0, 0.0, 0);
+                }
+                step[ind] = godunov_step(mask[ind] != 0, fore[0], back[0], fore[1], back[1],
+                                         fore[2], back[2], deltat[ind]);
+            }
+}
+
+/* the upwind combination of the extension steps (enork2_extend.cu:262-264,
+ * wenork3_extend.cu:490-492), summed left to right */
+static inline double upwind_sum(const double fo[3], const double bk[3], double vx, double vy, double vz,
+                                double dt)
+{
+    return (dmin2(0.0, vx) * fo[0] + dmax2(0.0, vx) * bk[0] + dmin2(0.0, vy) * fo[1] +
+            dmax2(0.0, vy) * bk[1] + dmin2(0.0, vz) * fo[2] + dmax2(0.0, vz) * bk[2]) * dt;
+}
+
+/* eno_derivative_field: enork2_extend.cu:56-78 */
+static void eno2_field_pair(double *sR, double *sL, const double p[7], double dis_f, double dis_b,
+                            double ds, double v_fore, double v_back)
+{
+    double v4 = p[1], v1 = p[2], v0 = p[3], v2 = p[4], v3 = p[5];
+    double p2 = v1 - 2.0 * v0 + v2;
+    double p2r = v0 - 2.0 * v2 + v3;
+    double p2m = 0.5 * min_mod(p2, p2r) / (ds * ds);
+    double v_f = (dis_f == ds) ? v2 : v_fore;
+    *sR = (v_f - v0) / dis_f - dis_f * p2m;
+    double p2l = v0 - 2.0 * v1 + v4;
+    p2m = 0.5 * min_mod(p2, p2l) / (ds * ds);
+    double v_b = (dis_b == ds) ? v1 : v_back;
+    *sL = (v0 - v_b) / dis_b + dis_b * p2m;
+}
+
+/* Kernel upwind_normal (ENO-RK2): enork2_extend.cu:102-138 with
+ * upwind_normal_point :82-99 */
+void or_enork2_upwind_normal(double *nx, double *ny, double *nz, const double *lsf, SIX_DIST_ARGS,
+                             int rows, int cols, int pges, double dx, double dy, double dz, int num_ele)
+{
+    (void)num_ele;
+    const double *R[3] = {xpr, ypf, zpu}, *L[3] = {xpl, ypb, zpd};
+    double *out[3] = {nx, ny, nz};
+    const double sp[3] = {dx, dy, dz};
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int p = 0; p < pges; ++p)
+        for (int c = 0; c < cols; ++c)
+            for (int r = 0; r < rows; ++r) {
+                idx_t ind = at(r, c, p, rows, cols, pges);
+                for (int a = 0; a < 3; ++a) {
+                    double v[7], sR, sL;
+                    line7(lsf, r, c, p, a, rows, cols, pges, v);
+                    double pr = R[a][ind], pl = L[a][ind];
+                    eno2_pair(&sR, &sL, v[1], v[2], v[3], v[4], v[5], pr, pl, sp[a]);
+                    double vr = (pr == sp[a]) ? v[4] : 0, vl = (pl == sp[a]) ? v[2] : 0;
+                    out[a][ind] = (fabs(vr) < fabs(vl)) ? sR : sL;
+                }
+            }
+}
+
+/* cubic_eno_interp: enork2_extend.cu:144-170 */
+static void eno2_interface_values(double *c_fore, double *c_back, double dis_f, double dis_b, double ds,
+                                  const double p[7])
+{
+    double v4 = p[1], v1 = p[2], v0 = p[3], v2 = p[4], v3 = p[5];
+    *c_fore = 0;
+    *c_back = 0;
+    if (dis_f != ds) {
+        double p2 = v2 - 2.0 * v0 + v1;
+        double p2r = v3 - 2.0 * v2 + v0;
+        double c2 = 0.5 * min_mod(p2, p2r) / (ds * ds);
+        double c1 = (v2 - v0) / ds;
+        double c0 = (v2 + v0) / 2 - c2 * (ds * ds) / 4;
+        double x = dis_f - ds / 2;
+        *c_fore = c2 * x * x + c1 * x + c0;
+    }
+    if (dis_b != ds) {
+        double p2 = v2 - 2.0 * v0 + v1;
+        double p2l = v0 - 2.0 * v1 + v4;
+        double c2 = 0.5 * min_mod(p2, p2l) / (ds * ds);
+        double c1 = (v0 - v1) / ds;
+        double c0 = (v1 + v0) / 2 - c2 * (ds * ds) / 4;
+        double x = ds / 2 - dis_b;
+        *c_back = c2 * x * x + c1 * x + c0;
+    }
+}
+
+/* cubic_interp_coefficient (wenork3_extend.cu:297-306) and cubic_interp
+ * (:310-329); pow(xc,2), pow(xc,3) are xc*xc and xc*xc*xc */
+static void cubic_coefficients(double c[4], double v0, double v1, double v2, double v3, double s)
+{
+    c[0] = v0;
+    c[1] = (3.0 * (v1 - v0) - 3.0 / 2.0 * (v2 - v0) + 1.0 / 3.0 * (v3 - v0)) / s;
+    c[2] = (-5.0 / 2.0 * (v1 - v0) + 2.0 * (v2 - v0) - 1.0 / 2.0 * (v3 - v0)) / (s * s);
+    c[3] = (1.0 / 2.0 * (v1 - v0) - 1.0 / 2.0 * (v2 - v0) + 1.0 / 6.0 * (v3 - v0)) / (s * s * s);
+}
+static void cubic_interface_values(double *c_fore, double *c_back, double dis_f, double dis_b, double ds,
+                                   const double p[7])
+{
+    double c[4];
+    *c_fore = 0.;
+    *c_back = 0.;
+    if (dis_f < ds) {
+        cubic_coefficients(c, p[2], p[3], p[4], p[5], ds);
+        double xc = ds + dis_f;
+        *c_fore = c[0] + c[1] * xc + c[2] * (xc * xc) + c[3] * (xc * xc * xc);
+    }
+    if (dis_b < ds) {
+        cubic_coefficients(c, p[1], p[2], p[3], p[4], ds);
+        double xc = 2 * ds - dis_b;
+        *c_back = c[0] + c[1] * xc + c[2] * (xc * xc) + c[3] * (xc * xc * xc);
+    }
+}
+
+/* Kernels boundary_interpolate: enork2_extend.cu:174-210 (weno == 0) and
+ * wenork3_extend.cu:331-368 (weno == 1) */
+static void variant_interpolate(int weno, double *cpr, double *cpl, double *cpf, double *cpb, double *cpu,
+                                double *cpd, SIX_DIST_ARGS, const double *lsf, int rows, int cols, int pges,
+                                double dx, double dy, double dz)
+{
+    const double *R[3] = {xpr, ypf, zpu}, *L[3] = {xpl, ypb, zpd};
+    double *CF[3] = {cpr, cpf, cpu}, *CB[3] = {cpl, cpb, cpd};
+    const double sp[3] = {dx, dy, dz};
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int p = 0; p < pges; ++p)
+        for (int c = 0; c < cols; ++c)
+            for (int r = 0; r < rows; ++r) {
+                idx_t ind = at(r, c, p, rows, cols, pges);
+                for (int a = 0; a < 3; ++a) {
+                    double v[7];
+                    line7(lsf, r, c, p, a, rows, cols, pges, v);
+                    if (weno) cubic_interface_values(&CF[a][ind], &CB[a][ind], R[a][ind], L[a][ind], sp[a], v);
+                    else eno2_interface_values(&CF[a][ind], &CB[a][ind], R[a][ind], L[a][ind], sp[a], v);
+                }
+            }
+}
+
+void or_enork2_boundary_interpolate(double *cpr, double *cpl, double *cpf, double *cpb, double *cpu,
+                                    double *cpd, SIX_DIST_ARGS, const double *lsf, int rows, int cols,
+                                    int pges, double dx, double dy, double dz, int num_ele)
+{
+    (void)num_ele;
+    variant_interpolate(0, cpr, cpl, cpf, cpb, cpu, cpd, xpr, xpl, ypf, ypb, zpu, zpd, lsf, rows, cols,
+                        pges, dx, dy, dz);
+}
+
+void or_wenork3_boundary_interpolate(double *cpr, double *cpl, double *cpf, double *cpb, double *cpu,
+                                     double *cpd, SIX_DIST_ARGS, const double *lsf, int rows, int cols,
+                                     int pges, double dx, double dy, double dz, int num_ele)
+{
+    (void)num_ele;
+    variant_interpolate(1, cpr, cpl, cpf, cpb, cpu, cpd, xpr, xpl, ypf, ypb, zpu, zpd, lsf, rows, cols,
+                        pges, dx, dy, dz);
+}
+
+/* Kernels extend_step: enork2_extend.cu:213-266 (weno == 0, no frozen test)
+ * and wenork3_extend.cu:372-495 (weno == 1, frozen below 1e-6 dx) */
+static void variant_extend_step(int weno, double *step, const double *deltat, const double *lsf,
+                                const double *vx, const double *vy, const double *vz, SIX_DIST_ARGS,
+                                const double *cpr, const double *cpl, const double *cpf, const double *cpb,
+                                const double *cpu, const double *cpd, int rows, int cols, int pges, double dx,
+                                double dy, double dz)
+{
+    const double *R[3] = {xpr, ypf, zpu}, *L[3] = {xpl, ypb, zpd};
+    const double *CF[3] = {cpr, cpf, cpu}, *CB[3] = {cpl, cpb, cpd};
+    const double sp[3] = {dx, dy, dz};
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int p = 0; p < pges; ++p)
+        for (int c = 0; c < cols; ++c)
+            for (int r = 0; r < rows; ++r) {
+                idx_t ind = at(r, c, p, rows, cols, pges);
+                double eps = 1e-6 * dx;
+                if (weno && (xpr[ind] < eps || xpl[ind] < eps || ypf[ind] < eps || ypb[ind] < eps ||
+                             zpu[ind] < eps || zpd[ind] < eps)) {
+                    step[ind] = 0;
+                    continue;
+                }
+                double fo[3], bk[3];
+                for (int a = 0; a < 3; ++a) {
+                    double v[7];
+                    line7(lsf, r, c, p, a, rows, cols, pges, v);
+                    if (weno)
+                        or_wenork3_derivative_boundary(&fo[a], &bk[a], v, R[a][ind], L[a][ind], sp[a],
+                                                       CF[a][ind], CB[a][ind], 1);
+                    else
+                        eno2_field_pair(&fo[a], &bk[a], v, R[a][ind], L[a][ind], sp[a], CF[a][ind],
+                                        CB[a][ind]);
+                }
+                step[ind] = upwind_sum(fo, bk, vx[ind], vy[ind], vz[ind], deltat[ind]);
+            }
+}
+
+void or_enork2_extend_step(double *step, const double *deltat, const double *lsf, const double *vx,
+                           const double *vy, const double *vz, SIX_DIST_ARGS, const double *cpr,
+                           const double *cpl, const double *cpf, const double *cpb, const double *cpu,
+                           const double *cpd, int rows, int cols, int pges, double dx, double dy,
+                           double dz, int num_ele)
+{
+    (void)num_ele;
+    variant_extend_step(0, step, deltat, lsf, vx, vy, vz, xpr, xpl, ypf, ypb, zpu, zpd, cpr, cpl, cpf, cpb,
+                        cpu, cpd, rows, cols, pges, dx, dy, dz);
+}
+
+void or_wenork3_extend_step(double *step, const double *deltat, const double *lsf, const double *vx,
+                            const double *vy, const double *vz, SIX_DIST_ARGS, const double *cpr,
+                            const double *cpl, const double *cpf, const double *cpb, const double *cpu,
+                            const double *cpd, int rows, int cols, int pges, double dx, double dy,
+                            double dz, int num_ele)
+{
+    (void)num_ele;
+    variant_extend_step(1, step, deltat, lsf, vx, vy, vz, xpr, xpl, ypf, ypb, zpu, zpd, cpr, cpl, cpf, cpb,
+                        cpu, cpd, rows, cols, pges, dx, dy, dz);
+}
+
+/* Kernel upwind_normal (WENO-RK3): wenork3_extend.cu:202-292 with
+ * upwind_normal_point :193-199 (interface values 0, distance test) */
+void or_wenork3_upwind_normal(double *nx, double *ny, double *nz, const double *lsf, SIX_DIST_ARGS,
+                              int rows, int cols, int pges, double dx, double dy, double dz, int num_ele)
+{
+    (void)num_ele;
+    const double *R[3] = {xpr, ypf, zpu}, *L[3] = {xpl, ypb, zpd};
+    double *out[3] = {nx, ny, nz};
+    const double sp[3] = {dx, dy, dz};
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int p = 0; p < pges; ++p)
+        for (int c = 0; c < cols; ++c)
+            for (int r = 0; r < rows; ++r) {
+                idx_t ind = at(r, c, p, rows, cols, pges);
+                for (int a = 0; a < 3; ++a) {
+                    double v[7], df, db;
+                    line7(lsf, r, c, p, a, rows, cols, pges, v);
+                    or_wenork3_derivative_boundary(&df, &db, v, R[a][ind], L[a][ind], sp[a], 0.0, 0.0, 1);
+                    out[a][ind] = (fabs(v[4]) < fabs(v[2])) ? df : db;
+                }
+            }
+}
+
+/* WENORK3Reinitialization: +SD/@SDF3/WENORK3Reinitialization.m:3-61.  mask =
+ * F<0, cubic distances of the argument, deltat = 0.3*min, RK3 (:30-57) */
+void or_wenork3_reinit(double *F, int iterations, int rows, int cols, int pges, double dx, double dy,
+                       double dz)
+{
+    idx_t n = (idx_t)rows * cols * pges;
+    uint8_t *mask = (uint8_t *)malloc((size_t)n);
+    double *d6 = dalloc(6 * n), *md = dalloc(n), *dt = dalloc(n), *step = dalloc(n), *f1 = dalloc(n),
+           *f2 = dalloc(n);
+    double *xpr = d6, *xpl = d6 + n, *ypf = d6 + 2 * n, *ypb = d6 + 3 * n, *zpu = d6 + 4 * n, *zpd = d6 + 5 * n;
+    for (idx_t i = 0; i < n; ++i) mask[i] = F[i] < 0;
+    or_set_distance(3, F, xpr, xpl, ypf, ypb, zpu, zpd, md, rows, cols, pges, dx, dy, dz);
+    for (idx_t i = 0; i < n; ++i) dt[i] = 0.3 * md[i];
+#define RS(src) or_wenork3_re_step(step, src, mask, dt, xpr, xpl, ypf, ypb, zpu, zpd, rows, cols, pges, dx, \
+                                   dy, dz, (int)n)
+    for (int it = 0; it < iterations; ++it) {
+        RS(F);
+        sub_into(f1, F, step, n);
+        RS(f1);
+        rk3_combine_stage2(f2, F, f1, step, n);
+        RS(f2);
+        rk3_combine_stage3(F, f2, step, n);
+    }
+#undef RS
+    free(mask); free(d6); free(md); free(dt); free(step); free(f1); free(f2);
+}
+
+/* ENORK2Extend (+SD/@SDF3/ENORK2Extend.m:9-81, weno == 0: quadratic-ENO
+ * distances, flat gradients floored at 1e-6, Heun RK2 :68-79) and
+ * WENORK3Extend (+SD/@SDF3/WENORK3Extend.m:9-98, weno == 1: cubic distances,
+ * max(|grad|,1e-6), RK3 :69-96).  C is extended in place off the zero set of F. */
+static void variant_extend(int weno, double *C, int iterations, const double *F, int rows, int cols, int pges,
+                           double dx, double dy, double dz)
+{
+    idx_t n = (idx_t)rows * cols * pges;
+    double *d6 = dalloc(6 * n), *md = dalloc(n), *dt = dalloc(n), *v = dalloc(3 * n), *cp = dalloc(6 * n),
+           *step = dalloc(n), *c1 = dalloc(n), *c2 = dalloc(n);
+    double *xpr = d6, *xpl = d6 + n, *ypf = d6 + 2 * n, *ypb = d6 + 3 * n, *zpu = d6 + 4 * n, *zpd = d6 + 5 * n;
+    double *vx = v, *vy = v + n, *vz = v + 2 * n;
+    or_set_distance(weno ? 3 : 2, F, xpr, xpl, ypf, ypb, zpu, zpd, md, rows, cols, pges, dx, dy, dz);
+    if (weno) or_wenork3_upwind_normal(vx, vy, vz, F, xpr, xpl, ypf, ypb, zpu, zpd, rows, cols, pges, dx, dy, dz, (int)n);
+    else or_enork2_upwind_normal(vx, vy, vz, F, xpr, xpl, ypf, ypb, zpu, zpd, rows, cols, pges, dx, dy, dz, (int)n);
+    for (idx_t i = 0; i < n; ++i) {
+        double g = dmax2(sqrt(sq(vx[i]) + sq(vy[i]) + sq(vz[i])), 1e-6);
+        double s = msign(F[i]);
+        vx[i] = s * (vx[i] / g);
+        vy[i] = s * (vy[i] / g);
+        vz[i] = s * (vz[i] / g);
+    }
+    variant_interpolate(weno, cp, cp + n, cp + 2 * n, cp + 3 * n, cp + 4 * n, cp + 5 * n, xpr, xpl, ypf, ypb,
+                        zpu, zpd, C, rows, cols, pges, dx, dy, dz);
+    for (idx_t i = 0; i < n; ++i) dt[i] = 0.3 * md[i];
+#define EXT(src) variant_extend_step(weno, step, dt, src, vx, vy, vz, xpr, xpl, ypf, ypb, zpu, zpd, cp, cp + n, \
+                                     cp + 2 * n, cp + 3 * n, cp + 4 * n, cp + 5 * n, rows, cols, pges, dx, dy, dz)
+    for (int it = 0; it < iterations; ++it) {
+        if (weno) {
+            EXT(C);
+            sub_into(c1, C, step, n);
+            EXT(c1);
+            rk3_combine_stage2(c2, C, c1, step, n);
+            EXT(c2);
+            rk3_combine_stage3(C, c2, step, n);
+        } else {
+            EXT(C);
+            sub_into(c1, C, step, n);
+            EXT(c1);
+            for (idx_t i = 0; i < n; ++i) C[i] = (C[i] + c1[i] - step[i]) / 2;
+        }
+    }
+#undef EXT
+    free(d6); free(md); free(dt); free(v); free(cp); free(step); free(c1); free(c2);
+}
+
+void or_enork2_extend(double *C, int iterations, const double *F, int rows, int cols, int pges, double dx,
+                      double dy, double dz)
+{
+    variant_extend(0, C, iterations, F, rows, cols, pges, dx, dy, dz);
+}
+
+void or_wenork3_extend(double *C, int iterations, const double *F, int rows, int cols, int pges, double dx,
+                       double dy, double dz)
+{
+    variant_extend(1, C, iterations, F, rows, cols, pges, dx, dy, dz);
 }

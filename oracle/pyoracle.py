@@ -55,7 +55,8 @@ def num_threads():
 
 def boundary_location(d6, lsf, dx, dy, dz, order=6):
     rows, cols, pges = _dims(lsf)
-    fn = lib().or_boundary_location if order == 6 else lib().or_boundary_correction
+    fn = {6: lib().or_boundary_location, 2: lib().or_boundary_correction,
+          3: lib().or_cubic_boundary_correction}[order]
     fn(*[_p(a) for a in d6], _p(lsf), _I(lsf.size), _I(rows), _I(cols), _I(pges), _D(dx), _D(dy), _D(dz))
 
 
@@ -66,7 +67,7 @@ def prefilled_distances(shape, dx, dy, dz):
 def re_step(lsf, mask, deltat, d6, dx, dy, dz, scheme="weno5"):
     rows, cols, pges = _dims(lsf)
     step = np.zeros_like(lsf)
-    fn = lib().or_re_step if scheme == "weno5" else lib().or_eno2_re_step
+    fn = {"weno5": lib().or_re_step, "eno2": lib().or_eno2_re_step, "wenork3": lib().or_wenork3_re_step}[scheme]
     fn(_p(step), _p(lsf), _p(np.ascontiguousarray(mask, dtype=np.uint8)), _p(deltat), *[_p(a) for a in d6],
        _I(rows), _I(cols), _I(pges), _D(dx), _D(dy), _D(dz), _I(lsf.size))
     return step
@@ -93,6 +94,34 @@ def extend_step(deltat, C, v3, d6, cp6, dx, dy, dz):
     step = np.zeros_like(C)
     lib().or_extend_step(_p(step), _p(deltat), _p(C), *[_p(a) for a in v3], *[_p(a) for a in d6],
                          *[_p(a) for a in cp6], _I(rows), _I(cols), _I(pges), _D(dx), _D(dy), _D(dz), _I(C.size))
+    return step
+
+
+def variant_upwind_normal(lsf, d6, dx, dy, dz, scheme):
+    """upwind_normal of the ENO-RK2 ("enork2") or WENO-RK3 ("wenork3") extension"""
+    rows, cols, pges = _dims(lsf)
+    out = [np.zeros_like(lsf) for _ in range(3)]
+    fn = {"enork2": lib().or_enork2_upwind_normal, "wenork3": lib().or_wenork3_upwind_normal}[scheme]
+    fn(*[_p(a) for a in out], _p(lsf), *[_p(a) for a in d6], _I(rows), _I(cols), _I(pges), _D(dx), _D(dy),
+       _D(dz), _I(lsf.size))
+    return out
+
+
+def variant_boundary_interpolate(d6, C, dx, dy, dz, scheme):
+    rows, cols, pges = _dims(C)
+    out = [np.zeros_like(C) for _ in range(6)]
+    fn = {"enork2": lib().or_enork2_boundary_interpolate, "wenork3": lib().or_wenork3_boundary_interpolate}[scheme]
+    fn(*[_p(a) for a in out], *[_p(a) for a in d6], _p(C), _I(rows), _I(cols), _I(pges), _D(dx), _D(dy), _D(dz),
+       _I(C.size))
+    return out
+
+
+def variant_extend_step(deltat, C, v3, d6, cp6, dx, dy, dz, scheme):
+    rows, cols, pges = _dims(C)
+    step = np.zeros_like(C)
+    fn = {"enork2": lib().or_enork2_extend_step, "wenork3": lib().or_wenork3_extend_step}[scheme]
+    fn(_p(step), _p(deltat), _p(C), *[_p(a) for a in v3], *[_p(a) for a in d6], *[_p(a) for a in cp6], _I(rows),
+       _I(cols), _I(pges), _D(dx), _D(dy), _D(dz), _I(C.size))
     return step
 
 
@@ -151,6 +180,29 @@ def eno_quadratic_distance(v1, v0, v2, v3, ds):
     return f(*[_D(x) for x in (v1, v0, v2, v3, ds)])
 
 
+def cubic_distance(v0, v1, v2, v3, s):
+    f = lib().or_cubic_distance
+    f.restype = _D
+    return f(*[_D(x) for x in (v0, v1, v2, v3, s)])
+
+
+def eno_cubic_pair(x7, h7):
+    """(d_fore, d_back) of ENO_cubic_derivative on nodes x7 (x7[3] the point) with values h7"""
+    x = np.ascontiguousarray(x7, dtype=np.float64)
+    h = np.ascontiguousarray(h7, dtype=np.float64)
+    df, db = _D(), _D()
+    lib().or_eno_cubic_pair(ctypes.byref(df), ctypes.byref(db), _p(x), _p(h))
+    return df.value, db.value
+
+
+def wenork3_derivative_boundary(p7, r1, l1, ds, v_fore=0.0, v_back=0.0, rule=0):
+    p = np.ascontiguousarray(p7, dtype=np.float64)
+    df, db = _D(), _D()
+    lib().or_wenork3_derivative_boundary(ctypes.byref(df), ctypes.byref(db), _p(p), _D(r1), _D(l1), _D(ds),
+                                         _D(v_fore), _D(v_back), _I(rule))
+    return df.value, db.value
+
+
 def weno_derivative_boundary(p7, r1, l1, ds, v_fore=0.0, v_back=0.0, rule=0):
     p = np.ascontiguousarray(p7, dtype=np.float64)
     df, db = _D(), _D()
@@ -184,6 +236,23 @@ def eno2_reinit(F, iterations, dx, dy, dz, rk2=True, cfl=0.3):
     lib().or_eno2_reinit(_p(F), _I(iterations), _I(1 if rk2 else 0), _D(cfl), _I(rows), _I(cols), _I(pges),
                          _D(dx), _D(dy), _D(dz))
     return F
+
+
+def wenork3_reinit(F, iterations, dx, dy, dz):
+    F = f64(F).copy()
+    rows, cols, pges = _dims(F)
+    lib().or_wenork3_reinit(_p(F), _I(iterations), _I(rows), _I(cols), _I(pges), _D(dx), _D(dy), _D(dz))
+    return F
+
+
+def variant_extend(C, iterations, F, dx, dy, dz, scheme):
+    """ENORK2Extend ("enork2") or WENORK3Extend ("wenork3") of C off the zero set of F"""
+    C = f64(C).copy()
+    F = f64(F)
+    rows, cols, pges = _dims(C)
+    fn = {"enork2": lib().or_enork2_extend, "wenork3": lib().or_wenork3_extend}[scheme]
+    fn(_p(C), _I(iterations), _p(F), _I(rows), _I(cols), _I(pges), _D(dx), _D(dy), _D(dz))
+    return C
 
 
 def extension_velocity(F, d6, floor, dx, dy, dz):

@@ -159,3 +159,29 @@ class RefKernels:
         args = [("p", t) for t in out18] + [("p", lsf), ("p", hprimal), ("i", rows), ("i", cols), ("i", pges),
                                             ("d", dx), ("d", dy), ("d", dz), ("d", ds), ("i", rows * cols * pges)]
         self.launch("set_calculus_toolbox.co", "set_calculus_toolbox", args, rows, cols, pges)
+
+    # ---- the WENO-RK3 / ENO-RK2 variants (GPUInitialize.m:32-42, 64-102) ------
+    def cubic_boundary_correction(self, d6, lsf, rows, cols, pges, dx, dy, dz):
+        args = [("p", t) for t in d6] + [("p", lsf), ("i", rows * cols * pges), ("i", rows), ("i", cols),
+                                         ("i", pges), ("d", dx), ("d", dy), ("d", dz)]
+        self.launch("weno_rk3_reinitialization.co", "boundary_correction", args, rows, cols, pges)
+
+    def wenork3_re_step(self, *a):
+        self._re_step("weno_rk3_reinitialization.co", *a)
+
+    def variant_upwind_normal(self, scheme, n3, lsf, d6, rows, cols, pges, dx, dy, dz):
+        args = [("p", t) for t in n3] + [("p", lsf)] + [("p", t) for t in d6] + [
+            ("i", rows), ("i", cols), ("i", pges), ("d", dx), ("d", dy), ("d", dz), ("i", rows * cols * pges)]
+        self.launch(scheme + "_extend.co", "upwind_normal", args, rows, cols, pges)
+
+    def variant_boundary_interpolate(self, scheme, cp6, d6, lsf, rows, cols, pges, dx, dy, dz):
+        args = [("p", t) for t in cp6] + [("p", t) for t in d6] + [("p", lsf), ("i", rows), ("i", cols),
+                                                                  ("i", pges), ("d", dx), ("d", dy), ("d", dz),
+                                                                  ("i", rows * cols * pges)]
+        self.launch(scheme + "_extend.co", "boundary_interpolate", args, rows, cols, pges)
+
+    def variant_extend_step(self, scheme, step, deltat, lsf, v3, d6, cp6, rows, cols, pges, dx, dy, dz):
+        args = [("p", step), ("p", deltat), ("p", lsf)] + [("p", t) for t in v3] + [("p", t) for t in d6] + [
+            ("p", t) for t in cp6] + [("i", rows), ("i", cols), ("i", pges), ("d", dx), ("d", dy), ("d", dz),
+                                      ("i", rows * cols * pges)]
+        self.launch(scheme + "_extend.co", "extend_step", args, rows, cols, pges)

@@ -180,6 +180,56 @@ def test_set_calculus_toolbox(case):
         exact(N(t), ref[name], name)
 
 
+def test_variant_distances_and_re_step(case):
+    """WENO-RK3 variant: cubic distances (cubic boundary_correction) and its re_step"""
+    g, F = case["g"], case["F"]
+    dx, dy, dz = g.spacing
+    rows, cols, pges = _dims(g)
+    ref = po.prefilled_distances(F.shape, dx, dy, dz)
+    po.boundary_location(ref, F, dx, dy, dz, order=3)
+    d6 = [T(a) for a in po.prefilled_distances(F.shape, dx, dy, dz)]
+    vls.feval_cubic_boundary_correction(*d6, T(F), F.size, rows, cols, pges, dx, dy, dz)
+    for k, (a, b) in enumerate(zip([N(t) for t in d6], ref)):
+        exact(a, b, f"cubic distance array {k}")
+    dt = 0.3 * np.min(np.stack(ref), axis=0)
+    Fs = F + 0.3 * g.dx * np.sin(3.0 * g.X + 2.0 * g.Y - g.Z)
+    for lsf in (F, Fs):
+        want = po.re_step(lsf, case["mask"], dt, ref, dx, dy, dz, scheme="wenork3")
+        step = torch.zeros_like(T(lsf))
+        vls.feval_wenork3_re_step(step, T(lsf), T(case["mask"], torch.uint8), T(dt), *[T(a) for a in ref], rows, cols,
+                                  pges, dx, dy, dz, F.size)
+        exact(N(step), want, "wenork3 re_step")
+
+
+@pytest.mark.parametrize("scheme,order", [("enork2", 2), ("wenork3", 3)])
+def test_variant_extension_kernels(case, scheme, order):
+    """ENO-RK2 / WENO-RK3 extension: upwind normal, interface values, extend_step"""
+    g, F = case["g"], case["F"]
+    dx, dy, dz = g.spacing
+    rows, cols, pges = _dims(g)
+    d6 = po.prefilled_distances(F.shape, dx, dy, dz)
+    po.boundary_location(d6, F, dx, dy, dz, order=order)
+    d6t = [T(a) for a in d6]
+    n_ref = po.variant_upwind_normal(F, d6, dx, dy, dz, scheme)
+    n_got = [torch.zeros_like(T(F)) for _ in range(3)]
+    getattr(vls, f"feval_{scheme}_upwind_normal")(*n_got, T(F), *d6t, rows, cols, pges, dx, dy, dz, F.size)
+    for a, b in zip(n_got, n_ref):
+        exact(N(a), b, f"{scheme} upwind_normal")
+    C = cases.smooth_scalar(g)
+    cp_ref = po.variant_boundary_interpolate(d6, C, dx, dy, dz, scheme)
+    cp = [torch.zeros_like(T(F)) for _ in range(6)]
+    getattr(vls, f"feval_{scheme}_boundary_interpolate")(*cp, *d6t, T(C), rows, cols, pges, dx, dy, dz, F.size)
+    for a, b in zip(cp, cp_ref):
+        exact(N(a), b, f"{scheme} boundary_interpolate")
+    v3 = cases.extension_velocity_from_normal(F, n_ref, 1e-6)
+    dt = 0.3 * np.min(np.stack(d6), axis=0)
+    want = po.variant_extend_step(dt, C, v3, d6, cp_ref, dx, dy, dz, scheme)
+    step = torch.zeros_like(T(F))
+    getattr(vls, f"feval_{scheme}_extend_step")(step, T(dt), T(C), *[T(a) for a in v3], *d6t,
+                                                *[T(a) for a in cp_ref], rows, cols, pges, dx, dy, dz, F.size)
+    exact(N(step), want, f"{scheme} extend_step")
+
+
 def test_invalid_arguments_raise():
     t = torch.zeros((4, 5, 6), dtype=torch.float64, device="cuda")
     with pytest.raises(vls.VLSError):

@@ -105,6 +105,57 @@ def test_extension_kernels(ref, case):
     close(N(s_got), N(s_ref), what="extend_step vs reference")
 
 
+def test_variant_kernels(ref, case):
+    """WENO-RK3 / ENO-RK2 variants against weno_rk3_reinitialization.cu,
+    enork2_extend.cu and wenork3_extend.cu"""
+    g, F = case
+    dx, dy, dz = g.spacing
+    r, c, p = g.rows, g.cols, g.pges
+    d_ref = [T(a) for a in po.prefilled_distances(F.shape, dx, dy, dz)]
+    ref.cubic_boundary_correction(d_ref, T(F), r, c, p, dx, dy, dz)
+    d_got = [T(a) for a in po.prefilled_distances(F.shape, dx, dy, dz)]
+    vls.feval_cubic_boundary_correction(*d_got, T(F), F.size, r, c, p, dx, dy, dz)
+    dr, dg = [N(t) for t in d_ref], [N(t) for t in d_got]
+    for k in range(6):
+        s = (dx, dx, dy, dy, dz, dz)[k]
+        assert np.array_equal(dg[k] < s, dr[k] < s)
+        assert np.abs(dg[k] - dr[k]).max() <= 1e-12 * s
+    mask = T((F < 0).astype(np.uint8), torch.uint8)
+    dt = T(0.3 * np.min(np.stack(dr), axis=0))
+    s_ref = torch.zeros_like(T(F))
+    ref.wenork3_re_step(s_ref, T(F), mask, dt, d_ref, r, c, p, dx, dy, dz)
+    s_got = torch.zeros_like(T(F))
+    vls.feval_wenork3_re_step(s_got, T(F), mask, dt, *d_ref, r, c, p, dx, dy, dz, F.size)
+    a, b = N(s_got), N(s_ref)
+    assert np.array_equal(a == 0, b == 0)
+    close(a, b, what="wenork3 re_step vs reference")
+    C = T(cases.smooth_scalar(g))
+    for scheme in ("enork2", "wenork3"):
+        d6 = d_ref
+        if scheme == "enork2":
+            d6 = [T(a) for a in po.prefilled_distances(F.shape, dx, dy, dz)]
+            ref.boundary_correction(d6, T(F), r, c, p, dx, dy, dz)
+        n_ref = [torch.zeros_like(T(F)) for _ in range(3)]
+        ref.variant_upwind_normal(scheme, n_ref, T(F), d6, r, c, p, dx, dy, dz)
+        n_got = [torch.zeros_like(T(F)) for _ in range(3)]
+        getattr(vls, f"feval_{scheme}_upwind_normal")(*n_got, T(F), *d6, r, c, p, dx, dy, dz, F.size)
+        for x, y in zip(n_got, n_ref):
+            close(N(x), N(y), what=f"{scheme} upwind_normal vs reference")
+        cp_ref = [torch.zeros_like(C) for _ in range(6)]
+        ref.variant_boundary_interpolate(scheme, cp_ref, d6, C, r, c, p, dx, dy, dz)
+        cp_got = [torch.zeros_like(C) for _ in range(6)]
+        getattr(vls, f"feval_{scheme}_boundary_interpolate")(*cp_got, *d6, C, r, c, p, dx, dy, dz, F.size)
+        for x, y in zip(cp_got, cp_ref):
+            close(N(x), N(y), rtol=1e-11, what=f"{scheme} boundary_interpolate vs reference")
+        v3 = [T(a) for a in cases.extension_velocity_from_normal(F, [N(t) for t in n_ref], 1e-6)]
+        dte = T(0.3 * np.min(np.stack([N(t) for t in d6]), axis=0))
+        s_ref = torch.zeros_like(C)
+        ref.variant_extend_step(scheme, s_ref, dte, C, v3, d6, cp_ref, r, c, p, dx, dy, dz)
+        s_got = torch.zeros_like(C)
+        getattr(vls, f"feval_{scheme}_extend_step")(s_got, dte, C, *v3, *d6, *cp_ref, r, c, p, dx, dy, dz, F.size)
+        close(N(s_got), N(s_ref), what=f"{scheme} extend_step vs reference")
+
+
 def test_advection_toolbox(ref, case):
     g, F = case
     dx, dy, dz = g.spacing

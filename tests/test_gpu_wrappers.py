@@ -110,6 +110,25 @@ def test_weno5_rk3_extend(setup):
     exact(got, ref, "WENO5RK3Extend")
 
 
+def test_wenork3_reinitialization(setup):
+    g, F, G = setup
+    m = vls.SDF3(G, T(F))
+    got = N(m.WENORK3Reinitialization(T(F), 6))
+    ref = po.wenork3_reinit(F, 6, *g.spacing)
+    exact(got, ref, "WENORK3Reinitialization")
+
+
+@pytest.mark.parametrize("scheme", ["enork2", "wenork3"])
+def test_variant_extend(setup, scheme):
+    g, F, G = setup
+    m = vls.SDF3(G, T(F))
+    C = cases.smooth_scalar(g)
+    fn = m.ENORK2Extend if scheme == "enork2" else m.WENORK3Extend
+    got = N(fn(T(C), 6))
+    ref = po.variant_extend(C, 6, F, *g.spacing, scheme)
+    exact(got, ref, f"{scheme} extension")
+
+
 def test_surface_redistance(setup):
     g, F, G = setup
     m = vls.SDF3(G, T(F))

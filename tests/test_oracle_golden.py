@@ -121,3 +121,56 @@ def test_set_calculus_toolbox(rec):
     got = po.set_calculus_toolbox(F, np.maximum(F, 0), dx, dy, dz, ds)
     for k, name in enumerate(po.TOOLBOX_FIELDS):
         _close(got[name], rec["toolbox"][k], rtol=1e-11)
+
+
+# --------------------------------------------------------------------------
+# WENO-RK3 / ENO-RK2 variants: vectors written by the reference's
+# weno_rk3_reinitialization.cu, enork2_extend.cu and wenork3_extend.cu
+# (tests/golden/make_golden_variants.py)
+# --------------------------------------------------------------------------
+VFILES = sorted(glob.glob(os.path.join(HERE, "golden", "variants_*.npz")))
+
+
+@pytest.fixture(scope="module", params=VFILES, ids=[os.path.basename(f)[9:-4] for f in VFILES])
+def vrec(request):
+    return _load(request.param)
+
+
+def test_variant_fixture_count():
+    assert len(VFILES) == 3
+
+
+def test_cubic_distances(vrec):
+    dx, dy, dz = _spacing(vrec)
+    d6 = po.prefilled_distances(vrec["F"].shape, dx, dy, dz)
+    po.boundary_location(d6, vrec["F"], dx, dy, dz, order=3)
+    for k in range(6):
+        s = (dx, dx, dy, dy, dz, dz)[k]
+        assert np.array_equal(d6[k] < s, vrec["d6_cubic"][k] < s)
+        assert np.abs(d6[k] - vrec["d6_cubic"][k]).max() <= 1e-12 * s
+
+
+def test_wenork3_re_step(vrec):
+    dx, dy, dz = _spacing(vrec)
+    for lsf, key in ((vrec["F"], "wenork3_re_step"), (vrec["F_stage"], "wenork3_re_step_stage")):
+        got = po.re_step(lsf, vrec["mask"], vrec["deltat_cubic"], list(vrec["d6_cubic"]), dx, dy, dz,
+                         scheme="wenork3")
+        assert np.array_equal(got == 0, vrec[key] == 0)
+        _close(got, vrec[key])
+
+
+@pytest.mark.parametrize("scheme,dkey,tkey", [("enork2", "d6_eno", "deltat_eno"),
+                                              ("wenork3", "d6_cubic", "deltat_cubic")])
+def test_variant_extension_kernels(vrec, scheme, dkey, tkey):
+    dx, dy, dz = _spacing(vrec)
+    d6 = list(vrec[dkey])
+    got = po.variant_upwind_normal(vrec["F"], d6, dx, dy, dz, scheme)
+    for a, b in zip(got, vrec[f"{scheme}_upwind_normal"]):
+        _close(a, b)
+    cp = po.variant_boundary_interpolate(d6, vrec["C"], dx, dy, dz, scheme)
+    for a, b in zip(cp, vrec[f"{scheme}_cp6"]):
+        assert np.array_equal(a == 0, b == 0)
+        _close(a, b, rtol=1e-11)
+    st = po.variant_extend_step(vrec[tkey], vrec["C"], list(vrec[f"{scheme}_v3"]), d6, list(vrec[f"{scheme}_cp6"]),
+                                dx, dy, dz, scheme)
+    _close(st, vrec[f"{scheme}_extend_step"])

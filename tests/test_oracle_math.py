@@ -345,3 +345,144 @@ def test_compensated_sums_against_fsum():
     ev = g.dx * g.dy * g.dz
     assert abs(a - math.fsum(delta.ravel()) * ev) <= 1e-14 * abs(a)
     assert abs(v - math.fsum((1 - heav).ravel()) * ev) <= 1e-14 * abs(v)
+
+
+# --------------------------------------------------------------------------
+# WENO-RK3 / ENO-RK2 variants (1_1_WENORK3_Reinitialization, 2_0_ENORK2_Extend,
+# 2_1_WENORK3_Extend and their +SD/@SDF3 wrappers)
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("trial", range(6))
+def test_cubic_distance_finds_the_root_of_a_cubic(trial):
+    # cubic through (0,v0),(s,v1),(2s,v2),(3s,v3) with its only root in
+    # (s, 2s): Newton converges to it (weno_rk3_reinitialization.cu:74-117)
+    s = RNG.uniform(0.02, 0.2)
+    root = RNG.uniform(1.05, 1.95) * s
+    a, k = RNG.uniform(0.2, 3.0), RNG.choice([-1.0, 1.0]) * RNG.uniform(0.5, 2.0)
+    f = lambda t: k * (t - root) * ((t - root) ** 2 + a * s * s)  # noqa: E731  (monotone: one real root)
+    xs = np.arange(4) * s
+    v = f(xs)
+    assert v[0] * v[1] >= 0 and v[2] * v[3] >= 0 and v[1] * v[2] < 0
+    assert abs(po.cubic_distance(*v, s) - (root - s)) < 1e-12 * s
+
+
+def test_cubic_distance_exact_for_linear_data():
+    s = 0.05
+    v = 0.12 - 1.9 * np.arange(4) * s  # root at 0.12/1.9 in (s, 2s)
+    assert abs(po.cubic_distance(*v, s) - (0.12 / 1.9 - s)) < 1e-15
+
+
+def test_cubic_distance_kink_falls_back_to_quadratic_eno():
+    s = 0.1
+    v = np.array([-0.2, 0.05, -0.05, 0.1])  # v0*v1 < 0: a kink behind the crossing
+    assert po.cubic_distance(*v, s) == po.eno_quadratic_distance(*v, s)
+
+
+def _nonuniform_stencil(h, r1, l1):
+    """x3m..x3 of select_stencil (weno_rk3_reinitialization.cu:149-184)"""
+    right = [r1, h, 2 * h] if r1 < h else [h, 2 * h, 3 * h]
+    left = [-l1, -h, -2 * h] if l1 < h else [-h, -2 * h, -3 * h]
+    return np.array(left[::-1] + [0.0] + right)
+
+
+@pytest.mark.parametrize("trial", range(6))
+def test_eno_cubic_derivative_exact_for_cubics(trial):
+    # ENO_cubic_derivative (weno_rk3_reinitialization.cu:190-222) is the
+    # derivative of a Newton cubic: exact for cubic data when the second
+    # divided differences keep one sign (min_mod keeps the quadratic term)
+    b, d = RNG.normal(size=2)
+    c = 4.0 + abs(RNG.normal())  # p'' = 2c + 6dx > 0 on the stencil
+    f = lambda t: 0.3 + b * t + c * t ** 2 + d * t ** 3  # noqa: E731
+    h = 0.1
+    r1, l1 = RNG.uniform(0.05, 0.95, size=2) * h
+    for rr, ll in ((r1, h), (h, l1), (r1, l1), (h, h)):
+        x = _nonuniform_stencil(h, rr, ll)
+        df, db = po.eno_cubic_pair(x, f(x))
+        assert abs(df - b) < 1e-12 and abs(db - b) < 1e-12
+
+
+def test_wenork3_derivative_boundary_branches():
+    h = 0.1
+    x = (np.arange(7) - 3) * h
+    p = 0.4 - 1.3 * x + 0.2 * x ** 2
+    # no sign change: the uniform WENO5 pair of re_step.cu:11-37
+    assert po.wenork3_derivative_boundary(p, h, h, h, rule=0) == po.weno_pair(p, h)
+    # a sign change beside the node: cubic ENO with value 0 at the crossing
+    q = x - 0.03
+    r1 = 0.03
+    df, db = po.wenork3_derivative_boundary(q, h, h, h, rule=0)
+    xs = _nonuniform_stencil(h, h, h)
+    assert (df, db) == po.eno_cubic_pair(xs, q)
+    df, db = po.wenork3_derivative_boundary(q, r1, h, h, rule=1, v_fore=0.0)
+    xs = _nonuniform_stencil(h, r1, h)
+    hs = np.array([q[0], q[1], q[2], q[3], 0.0, q[4], q[5]])
+    assert (df, db) == po.eno_cubic_pair(xs, hs)
+    assert abs(df - 1.0) < 1e-13 and abs(db - 1.0) < 1e-13
+
+
+def _axis_grid():
+    return cases.Grid(np.linspace(-1, 1, 18), np.linspace(-0.9, 0.9, 16), np.linspace(-1.1, 1.1, 14))
+
+
+@pytest.mark.parametrize("scheme,order,deg", [("enork2", 2, 2), ("wenork3", 3, 3)])
+def test_variant_interface_values_exact_for_polynomials(scheme, order, deg):
+    # boundary_interpolate: ENO2 quadratic (enork2_extend.cu:144-170) and
+    # cubic (wenork3_extend.cu:310-329) interpolants reproduce polynomials of
+    # their degree at the crossing
+    g = _axis_grid()
+    F = g.X - 0.013  # crossings between columns along x only
+    d6 = po.prefilled_distances(F.shape, *g.spacing)
+    po.boundary_location(d6, F, *g.spacing, order=order)
+    C = 0.2 + 0.7 * g.X - 0.4 * g.X ** 2 + (0.3 * g.X ** 3 if deg == 3 else 0.0) + g.Y
+    cp = po.variant_boundary_interpolate(d6, C, *g.spacing, scheme)
+    x_at = lambda xs: 0.2 + 0.7 * xs - 0.4 * xs ** 2 + (0.3 * xs ** 3 if deg == 3 else 0.0)  # noqa: E731
+    sel = d6[0] < g.dx
+    assert sel.sum() > 50
+    want = x_at(g.X[sel] + d6[0][sel]) + g.Y[sel]
+    assert np.abs(cp[0][sel] - want).max() < 1e-12
+    sel = d6[1] < g.dx
+    want = x_at(g.X[sel] - d6[1][sel]) + g.Y[sel]
+    assert np.abs(cp[1][sel] - want).max() < 1e-12
+    assert np.all(cp[2] == 0) and np.all(cp[4] == 0)
+
+
+@pytest.mark.parametrize("scheme", ["enork2", "wenork3"])
+def test_variant_extension_keeps_fields_constant_along_normals(scheme):
+    # F = x - x0: v = sign(F) e_x; a C that does not depend on x has zero
+    # x-derivatives, so every extension step is an exact zero.  Heun's
+    # (C + C - 0)/2 returns C exactly; the RK3 combinations 3/4 C + 1/4 C and
+    # 1/3 C + 2/3 C round to within an ulp or two of C.
+    g = _axis_grid()
+    F = g.X - 0.013
+    C = 0.3 + np.sin(2.0 * g.Y) * np.cos(g.Z)
+    out = po.variant_extend(C, 8, F, *g.spacing, scheme)
+    if scheme == "enork2":
+        assert np.array_equal(out, C)
+    else:
+        assert np.abs(out - C).max() <= 4 * np.spacing(np.abs(C).max())
+
+
+@pytest.mark.parametrize("scheme", ["enork2", "wenork3"])
+def test_variant_extension_of_z_off_a_sphere(scheme):
+    # as test_extension_of_z_off_a_sphere, with the ENO2 / WENO-RK3 schemes
+    errs = []
+    for n in (24, 32):
+        g, r, F = _sphere(n, 0.5)
+        G = po.variant_extend(g.Z.copy(), 100, F, *g.spacing, scheme)
+        band = np.abs(F) < 1.5 * g.ds
+        errs.append(np.abs(G - 0.5 * g.Z / r)[band].max())
+    assert errs[0] < 2e-2 and errs[1] < errs[0], errs
+
+
+def test_wenork3_reinit_keeps_planes_and_converges_on_spheres():
+    g = _axis_grid()
+    F = g.X - 0.013
+    out = po.wenork3_reinit(F, 5, *g.spacing)
+    assert np.abs(out - F).max() < 1e-13
+    errs = []
+    for n in (24, 32):
+        gs = cases.linspace_grid(n)
+        r = np.sqrt(gs.X ** 2 + gs.Y ** 2 + gs.Z ** 2)
+        Fs = po.wenork3_reinit(r ** 2 - 0.36, 100, *gs.spacing)
+        band = np.abs(r - 0.6) < 2 * gs.ds
+        errs.append(np.abs(Fs - (r - 0.6))[band].max())
+    assert errs[0] < 5e-3 and errs[1] < errs[0], errs
