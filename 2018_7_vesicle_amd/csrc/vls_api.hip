@@ -988,12 +988,12 @@ static int variant_extend(vls_sdf3* h, const double* F, double* C, int iteration
     double *t0 = h->t[0], *t1 = h->t[1];
     for (int it = 0; it < iterations; ++it) {
         if (weno) {
-            VLS_HIP(launch_wenork3_extend_stage(t0, C, nullptr, vx, vy, vz, dc, cpc, g, 0, s));
-            VLS_HIP(launch_wenork3_extend_stage(t1, t0, C, vx, vy, vz, dc, cpc, g, 1, s));
-            VLS_HIP(launch_wenork3_extend_stage(C, t1, C, vx, vy, vz, dc, cpc, g, 2, s));
+            VLS_HIP(launch_wenork3_extend_stage(t0, C, nullptr, vx, vy, vz, h->aflags, dc, cpc, g, 0, s));
+            VLS_HIP(launch_wenork3_extend_stage(t1, t0, C, vx, vy, vz, h->aflags, dc, cpc, g, 1, s));
+            VLS_HIP(launch_wenork3_extend_stage(C, t1, C, vx, vy, vz, h->aflags, dc, cpc, g, 2, s));
         } else {
-            VLS_HIP(launch_enork2_extend_stage(t0, C, nullptr, vx, vy, vz, dc, cpc, g, 0, s));
-            VLS_HIP(launch_enork2_extend_stage(C, t0, C, vx, vy, vz, dc, cpc, g, 1, s));
+            VLS_HIP(launch_enork2_extend_stage(t0, C, nullptr, vx, vy, vz, h->aflags, dc, cpc, g, 0, s));
+            VLS_HIP(launch_enork2_extend_stage(C, t0, C, vx, vy, vz, h->aflags, dc, cpc, g, 1, s));
         }
     }
     return VLS_OK;

@@ -69,7 +69,7 @@ __global__ void __launch_bounds__(256) k_boundary_location(Dist6 d, const T* __r
 
 
 // min_dist (setDistance.m:17-21), the per-point flags of the fused kernels and
-// the list of interface points (some distance below the spacing).
+// the list of interface points (some distance differs from the spacing).
 // mask_src may be null (no mask bit); list may be null.
 template <typename T>
 __global__ void k_min_dist_flags(double* __restrict__ min_dist, uint8_t* __restrict__ flags, Dist6c d,
@@ -81,7 +81,10 @@ __global__ void k_min_dist_flags(double* __restrict__ min_dist, uint8_t* __restr
     if (i < n) {
         const double xr = d.r[0][i], xl = d.l[0][i], yf = d.r[1][i], yb = d.l[1][i], zu = d.r[2][i], zd = d.l[2][i];
         if (min_dist) min_dist[i] = dmin(dmin(xr, xl), dmin(dmin(yf, yb), dmin(zu, zd)));
-        sub = xr < b.dx || xl < b.dx || yf < b.dy || yb < b.dy || zu < b.dz || zd < b.dz;
+        // a distance differs from the spacing (below it; the quadratic ENO
+        // locator is not clamped, so a root within rounding of the far node
+        // can land just above it, where the ENO2 schemes treat it as a crossing)
+        sub = xr != b.dx || xl != b.dx || yf != b.dy || yb != b.dy || zu != b.dz || zd != b.dz;
         if (flags) {
             uint8_t f = sub ? FLAG_SUBCELL : 0;
             if (mask_src && mask_src[i] < 0) f |= FLAG_MASK;

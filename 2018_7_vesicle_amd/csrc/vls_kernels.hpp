@@ -117,15 +117,16 @@ hipError_t launch_wenork3_extend_step(double* step, const double* deltat, const 
                                       const double* vy, const double* vz, const Dist6c& d, const Dist6c& cp,
                                       const Geom& g, hipStream_t s);
 // fused stages of the wrapper loops (operator + RK combination; dt = 0.3*min
-// of the six distances d; cp the interface values)
+// of the six distances d; cp the interface values; d and cp are read only
+// where flags has FLAG_SUBCELL)
 hipError_t launch_wenork3_reinit_stage(double* out, const double* fs, const double* fn, const uint8_t* flags,
                                        const Dist6c& d, const Geom& g, int stage, hipStream_t s);
 hipError_t launch_wenork3_extend_stage(double* out, const double* cs, const double* cn, const double* vx,
-                                       const double* vy, const double* vz, const Dist6c& d, const Dist6c& cp,
-                                       const Geom& g, int stage, hipStream_t s);
+                                       const double* vy, const double* vz, const uint8_t* flags, const Dist6c& d,
+                                       const Dist6c& cp, const Geom& g, int stage, hipStream_t s);
 hipError_t launch_enork2_extend_stage(double* out, const double* cs, const double* cn, const double* vx,
-                                      const double* vy, const double* vz, const Dist6c& d, const Dist6c& cp,
-                                      const Geom& g, int heun2, hipStream_t s);
+                                      const double* vy, const double* vz, const uint8_t* flags, const Dist6c& d,
+                                      const Dist6c& cp, const Geom& g, int heun2, hipStream_t s);
 hipError_t launch_boundary_interpolant(const Dist6& cp, const Dist6c& d, const double* C, const Geom& g,
                                        hipStream_t s);
 hipError_t launch_extend_step(double* step, const double* deltat, const double* C, const double* vx,

@@ -28,6 +28,20 @@ __device__ __forceinline__ void point_dists(Dists& q, const Dist6c& d, idx_t ind
     for (int a = 0; a < 3; ++a) { q.r[a] = d.r[a][ind]; q.l[a] = d.l[a][ind]; }
 }
 
+// the fused stages read the six distances (and interface values) only at a
+// point whose FLAG_SUBCELL says some distance differs from the spacing; every
+// other point has all six equal to the spacings, which is what it gets here
+__device__ __forceinline__ void point_dists(Dists& q, const Dist6c& d, idx_t ind, bool sub, const Box& b)
+{
+    if (sub) {
+        point_dists(q, d, ind);
+    } else {
+        q.r[0] = q.l[0] = b.dx;
+        q.r[1] = q.l[1] = b.dy;
+        q.r[2] = q.l[2] = b.dz;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // WENO-RK3 reinitialization step at a point (weno_rk3_reinitialization.cu:295-407):
 // frozen next to the interface (any distance below 1e-6 dx), else the Godunov
@@ -75,7 +89,10 @@ __device__ __forceinline__ double wenork3_extend_residual(const double* __restri
         double v[7];
         line7(f, r, c, p, a, b, v);
         const bool cross = q.r[a] < sp[a] || q.l[a] < sp[a];
-        wenork3_pair(fo[a], bk[a], v, q.r[a], q.l[a], sp[a], jpd(sp[a]), cross, cp.r[a][ind], cp.l[a][ind]);
+        // the interface values are read where the axis crosses (they are 0
+        // elsewhere, and unused)
+        const double hf = cross ? cp.r[a][ind] : 0.0, hb = cross ? cp.l[a][ind] : 0.0;
+        wenork3_pair(fo[a], bk[a], v, q.r[a], q.l[a], sp[a], jpd(sp[a]), cross, hf, hb);
     }
     return upwind_sum(fo, bk, vx, vy, vz, dt);
 }
@@ -91,7 +108,9 @@ __device__ __forceinline__ double enork2_extend_residual(const double* __restric
     for (int a = 0; a < 3; ++a) {
         double v[7];
         line7(f, r, c, p, a, b, v);
-        eno2_field_pair(fo[a], bk[a], v, q.r[a], q.l[a], sp[a], cp.r[a][ind], cp.l[a][ind]);
+        // an interface value is used only where its distance differs from ds
+        const double hf = (q.r[a] != sp[a]) ? cp.r[a][ind] : 0.0, hb = (q.l[a] != sp[a]) ? cp.l[a][ind] : 0.0;
+        eno2_field_pair(fo[a], bk[a], v, q.r[a], q.l[a], sp[a], hf, hb);
     }
     return upwind_sum(fo, bk, vx, vy, vz, dt);
 }
@@ -242,9 +261,10 @@ __global__ void __launch_bounds__(256) k_wenork3_reinit_stage(double* __restrict
     int r, c, p;
     if (!point_of_thread(b, r, c, p)) return;
     const idx_t ind = lin(b, r, c, p);
+    const uint8_t fl = flags[ind];
     Dists q;
-    point_dists(q, d, ind);
-    const double step = wenork3_reinit_residual(fs, r, c, p, b, q, (flags[ind] & FLAG_MASK) != 0, 0.3 * min6(q));
+    point_dists(q, d, ind, (fl & FLAG_SUBCELL) != 0, b);
+    const double step = wenork3_reinit_residual(fs, r, c, p, b, q, (fl & FLAG_MASK) != 0, 0.3 * min6(q));
     out[ind] = rk_combine(stage, fs[ind], step, (stage == 0) ? 0.0 : fn[ind]);
 }
 
@@ -253,14 +273,15 @@ __global__ void __launch_bounds__(256) k_wenork3_extend_stage(double* __restrict
                                                               const double* __restrict__ cn,
                                                               const double* __restrict__ vx,
                                                               const double* __restrict__ vy,
-                                                              const double* __restrict__ vz, Dist6c d, Dist6c cp,
+                                                              const double* __restrict__ vz, const uint8_t* __restrict__ flags, Dist6c d,
+                                                             Dist6c cp,
                                                               Box b, int stage)
 {
     int r, c, p;
     if (!point_of_thread(b, r, c, p)) return;
     const idx_t ind = lin(b, r, c, p);
     Dists q;
-    point_dists(q, d, ind);
+    point_dists(q, d, ind, (flags[ind] & FLAG_SUBCELL) != 0, b);
     const double step = wenork3_extend_residual(cs, r, c, p, b, q, cp, ind, vx[ind], vy[ind], vz[ind], 0.3 * min6(q));
     out[ind] = rk_combine(stage, cs[ind], step, (stage == 0) ? 0.0 : cn[ind]);
 }
@@ -271,7 +292,8 @@ __global__ void __launch_bounds__(256) k_enork2_extend_stage(double* __restrict_
                                                              const double* __restrict__ cn,
                                                              const double* __restrict__ vx,
                                                              const double* __restrict__ vy,
-                                                             const double* __restrict__ vz, Dist6c d, Dist6c cp,
+                                                             const double* __restrict__ vz, const uint8_t* __restrict__ flags, Dist6c d,
+                                                             Dist6c cp,
                                                              Box b, int heun2)
 {
 #pragma clang fp contract(off)
@@ -279,7 +301,7 @@ __global__ void __launch_bounds__(256) k_enork2_extend_stage(double* __restrict_
     if (!point_of_thread(b, r, c, p)) return;
     const idx_t ind = lin(b, r, c, p);
     Dists q;
-    point_dists(q, d, ind);
+    point_dists(q, d, ind, (flags[ind] & FLAG_SUBCELL) != 0, b);
     const double step = enork2_extend_residual(cs, r, c, p, b, q, cp, ind, vx[ind], vy[ind], vz[ind], 0.3 * min6(q));
     out[ind] = heun2 ? (cn[ind] + cs[ind] - step) / 2 : cs[ind] - step;
 }
@@ -346,17 +368,17 @@ hipError_t launch_wenork3_reinit_stage(double* out, const double* fs, const doub
 }
 
 hipError_t launch_wenork3_extend_stage(double* out, const double* cs, const double* cn, const double* vx,
-                                       const double* vy, const double* vz, const Dist6c& d, const Dist6c& cp,
-                                       const Geom& g, int stage, hipStream_t s)
+                                       const double* vy, const double* vz, const uint8_t* flags, const Dist6c& d,
+                                       const Dist6c& cp, const Geom& g, int stage, hipStream_t s)
 {
-    VLS_VARIANT_LAUNCH(k_wenork3_extend_stage, out, cs, cn, vx, vy, vz, d, cp, vbox(g), stage);
+    VLS_VARIANT_LAUNCH(k_wenork3_extend_stage, out, cs, cn, vx, vy, vz, flags, d, cp, vbox(g), stage);
 }
 
 hipError_t launch_enork2_extend_stage(double* out, const double* cs, const double* cn, const double* vx,
-                                      const double* vy, const double* vz, const Dist6c& d, const Dist6c& cp,
-                                      const Geom& g, int heun2, hipStream_t s)
+                                      const double* vy, const double* vz, const uint8_t* flags, const Dist6c& d,
+                                      const Dist6c& cp, const Geom& g, int heun2, hipStream_t s)
 {
-    VLS_VARIANT_LAUNCH(k_enork2_extend_stage, out, cs, cn, vx, vy, vz, d, cp, vbox(g), heun2);
+    VLS_VARIANT_LAUNCH(k_enork2_extend_stage, out, cs, cn, vx, vy, vz, flags, d, cp, vbox(g), heun2);
 }
 
 }  // namespace vls
