@@ -940,63 +940,76 @@ int vls_sdf3_eno2_reinit(vls_sdf3_t h, double* F, int iterations, int rk2, doubl
 }
 
 // WENORK3Reinitialization (+SD/@SDF3/WENORK3Reinitialization.m:3-61): cubic
-// distances of the argument (:17-21), mask = F<0, dt = 0.3*min (:23-26), RK3
-// (:30-57); the handle's auxiliary distance set holds the distances
+// distances of the argument (:17-21) into the auxiliary distance set, mask =
+// F<0, dt = 0.3*min (:23-26), RK3 (:30-57) on the fused stage kernels of the
+// WENO5 path with cubic ENO at the interface
 int vls_sdf3_wenork3_reinit(vls_sdf3_t h, double* F, int iterations, vls_stream_t stream)
 {
     VLS_REQUIRE(h && F, "null argument");
     VLS_REQUIRE(iterations >= 0, "negative iteration count");
-    const hipStream_t s = S(stream);
-    const Geom& g = h->g;
-    int rc = set_distance_into<double>(h, 3, aux_set(h), nullptr, F, F, s);
-    if (rc) return rc;
-    const Dist6c dc = constify(h->ad);
-    double *t0 = h->t[0], *t1 = h->t[1];
-    for (int it = 0; it < iterations; ++it) {
-        VLS_HIP(launch_wenork3_reinit_stage(t0, F, nullptr, h->aflags, dc, g, 0, s));
-        VLS_HIP(launch_wenork3_reinit_stage(t1, t0, F, h->aflags, dc, g, 1, s));
-        VLS_HIP(launch_wenork3_reinit_stage(F, t1, F, h->aflags, dc, g, 2, s));
-    }
-    return VLS_OK;
+    return with_graph(h->graphs, make_key(21, F, iterations), S(stream), [&](hipStream_t s) -> int {
+        const Geom& g = h->g;
+        const DSet as = aux_set(h);
+        int rc = set_distance_into<double>(h, 3, as, nullptr, F, F, s);
+        if (rc) return rc;
+        double *t0 = h->t[0], *t1 = h->t[1];
+        ReinitSeq rs;
+        if ((rc = reinit_begin(h, rs, s))) return rc;
+        for (int it = 0; it < iterations; ++it) {
+            VLS_HIP(launch_cubic_reinit_stage(t0, F, nullptr, as.flags, as.lv, g, 0, h->over, rs.counts, rs.seq++, s));
+            VLS_HIP(launch_cubic_reinit_stage(t1, t0, F, as.flags, as.lv, g, 1, h->over, rs.counts, rs.seq++, s));
+            VLS_HIP(launch_cubic_reinit_stage(F, t1, F, as.flags, as.lv, g, 2, h->over, rs.counts, rs.seq++, s));
+        }
+        return VLS_OK;
+    });
 }
 
 // ENORK2Extend (+SD/@SDF3/ENORK2Extend.m:9-81) and WENORK3Extend
-// (+SD/@SDF3/WENORK3Extend.m:9-98): distances of F (quadratic ENO or cubic),
-// v = sign(F) grad/max(|grad|,1e-6) from the variant's upwind normal, the
-// interface values of C once, then Heun RK2 or Shu-Osher RK3 steps of C in
-// place.  The auxiliary distance set, h->v and the full 6-row interface-value
-// block h->lcp are the work arrays.
+// (+SD/@SDF3/WENORK3Extend.m:9-98): distances of F (quadratic ENO or cubic)
+// into the auxiliary distance set, v = sign(F) grad/max(|grad|,1e-6) from the
+// variant's upwind normal, the interface values of C once, then Heun RK2 or
+// Shu-Osher RK3 steps of C in place.  ENO-RK2 runs one point kernel per stage
+// (full 6-row interface values in h->lcp); WENO-RK3 runs the fused stage
+// kernels of the WENO5 path with cubic ENO at the listed points (list-order
+// interface values and velocities in h->lcp, h->lvel).
 static int variant_extend(vls_sdf3* h, const double* F, double* C, int iterations, int weno, hipStream_t s)
 {
     VLS_REQUIRE(h && F && C, "null argument");
     VLS_REQUIRE(iterations >= 0, "negative iteration count");
     VLS_REQUIRE((const void*)F != (const void*)C, "C must not alias F");
-    const Geom& g = h->g;
-    const int64_t n = g.n();
-    int rc = set_distance_into<double>(h, weno ? 3 : 2, aux_set(h), nullptr, F, nullptr, s);
-    if (rc) return rc;
-    const Dist6c dc = constify(h->ad);
-    double *vx = h->v[0], *vy = h->v[1], *vz = h->v[2];
-    if (weno) VLS_HIP(launch_wenork3_upwind_normal(vx, vy, vz, F, dc, g, s));
-    else VLS_HIP(launch_enork2_upwind_normal(vx, vy, vz, F, dc, g, s));
-    VLS_HIP(launch_extension_velocity<double>(vx, vy, vz, F, 1e-6, g, s));
-    double* q = h->lcp;
-    const Dist6 cp = mk6(q, q + n, q + 2 * n, q + 3 * n, q + 4 * n, q + 5 * n);
-    if (weno) VLS_HIP(launch_wenork3_boundary_interpolate(cp, dc, C, g, s));
-    else VLS_HIP(launch_enork2_boundary_interpolate(cp, dc, C, g, s));
-    const Dist6c cpc = constify(cp);
-    double *t0 = h->t[0], *t1 = h->t[1];
-    for (int it = 0; it < iterations; ++it) {
+    return with_graph(h->graphs, make_key(22 + weno, F, C, iterations), s, [&](hipStream_t cs) -> int {
+        const Geom& g = h->g;
+        const int64_t n = g.n();
+        const DSet as = aux_set(h);
+        int rc = set_distance_into<double>(h, weno ? 3 : 2, as, nullptr, F, nullptr, cs);
+        if (rc) return rc;
+        const Dist6c dc = constify(h->ad);
+        double *vx = h->v[0], *vy = h->v[1], *vz = h->v[2];
+        if (weno) VLS_HIP(launch_wenork3_upwind_normal(vx, vy, vz, F, dc, g, cs));
+        else VLS_HIP(launch_enork2_upwind_normal(vx, vy, vz, F, dc, g, cs));
+        VLS_HIP(launch_extension_velocity<double>(vx, vy, vz, F, 1e-6, g, cs));
+        double *t0 = h->t[0], *t1 = h->t[1];
         if (weno) {
-            VLS_HIP(launch_wenork3_extend_stage(t0, C, nullptr, vx, vy, vz, h->aflags, dc, cpc, g, 0, s));
-            VLS_HIP(launch_wenork3_extend_stage(t1, t0, C, vx, vy, vz, h->aflags, dc, cpc, g, 1, s));
-            VLS_HIP(launch_wenork3_extend_stage(C, t1, C, vx, vy, vz, h->aflags, dc, cpc, g, 2, s));
-        } else {
-            VLS_HIP(launch_enork2_extend_stage(t0, C, nullptr, vx, vy, vz, h->aflags, dc, cpc, g, 0, s));
-            VLS_HIP(launch_enork2_extend_stage(C, t0, C, vx, vy, vz, h->aflags, dc, cpc, g, 1, s));
+            VLS_HIP(launch_list_gather_vel<double>(h->lvel, as.lv, vx, vy, vz, g, cs));
+            VLS_HIP(launch_list_interp_cubic(h->lcp, as.lv, C, g, cs));
+            for (int it = 0; it < iterations; ++it) {
+                VLS_HIP(launch_cubic_extend_stage(t0, C, nullptr, vx, vy, vz, as.flags, as.lv, h->lcp, h->lvel, g, 0,
+                                                  cs));
+                VLS_HIP(launch_cubic_extend_stage(t1, t0, C, vx, vy, vz, as.flags, as.lv, h->lcp, h->lvel, g, 1, cs));
+                VLS_HIP(launch_cubic_extend_stage(C, t1, C, vx, vy, vz, as.flags, as.lv, h->lcp, h->lvel, g, 2, cs));
+            }
+            return VLS_OK;
         }
-    }
-    return VLS_OK;
+        double* q = h->lcp;
+        const Dist6 cp = mk6(q, q + n, q + 2 * n, q + 3 * n, q + 4 * n, q + 5 * n);
+        VLS_HIP(launch_enork2_boundary_interpolate(cp, dc, C, g, cs));
+        const Dist6c cpc = constify(cp);
+        for (int it = 0; it < iterations; ++it) {
+            VLS_HIP(launch_enork2_extend_stage(t0, C, nullptr, vx, vy, vz, h->aflags, dc, cpc, g, 0, cs));
+            VLS_HIP(launch_enork2_extend_stage(C, t0, C, vx, vy, vz, h->aflags, dc, cpc, g, 1, cs));
+        }
+        return VLS_OK;
+    });
 }
 
 int vls_sdf3_enork2_extend(vls_sdf3_t h, const double* F, double* C, int iterations, vls_stream_t stream)

@@ -161,7 +161,7 @@ __global__ void __launch_bounds__(256) k_re_step(double* __restrict__ step, cons
 // changes sign beside it needs the nonuniform stencil: it is appended to
 // `over`, counted in over_count[seq & 1], for launch 2; the other slot is
 // cleared here for the next stage.
-template <int R, typename T>
+template <int R, typename T, int S = IFACE_WENO>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(stage_waves<T>()))) k_reinit_stage(T* __restrict__ out, const T* __restrict__ fs,
                                                       const T* __restrict__ fn, const uint8_t* __restrict__ flags,
                                                       IList il, Box b, Inv invs, int stage, int* __restrict__ over,
@@ -178,7 +178,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(stage_
             ReinitLane q;
             if (active) {
                 const idx_t ind = (idx_t)il.idx[al.point];
-                q = reinit_iface_lane(fs, flags, b, invs, il, al.point, ind, ind, al.axis);
+                q = reinit_iface_lane<T, S>(fs, flags, b, invs, il, al.point, ind, ind, al.axis);
             }
             reinit_iface_finish(out, fs, fn, q, al, active, stage);
         }
@@ -199,7 +199,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(stage_
 
 // launch 2 of 2: the deferred bulk points, three lanes per point, with the
 // interface pass's arithmetic (spacing distances, the bulk pseudo-time step)
-template <typename T>
+template <typename T, int S = IFACE_WENO>
 __global__ void __launch_bounds__(256) k_reinit_deferred(T* __restrict__ out, const T* __restrict__ fs,
                                                          const T* __restrict__ fn, const uint8_t* __restrict__ flags,
                                                          IList il, Box b, Inv inv, int stage,
@@ -213,7 +213,7 @@ __global__ void __launch_bounds__(256) k_reinit_deferred(T* __restrict__ out, co
         ReinitLane q;
         if (active) {
             const idx_t ind = (idx_t)over[al.point];
-            q = reinit_iface_lane(fs, flags, b, inv, il, -1, ind, ind, al.axis);
+            q = reinit_iface_lane<T, S>(fs, flags, b, inv, il, -1, ind, ind, al.axis);
         }
         reinit_iface_finish(out, fs, fn, q, al, active, stage);
     }
@@ -264,6 +264,31 @@ __global__ void __launch_bounds__(256) k_list_interp(double* __restrict__ cp, IL
         }
         cp[(size_t)(2 * a) * il.cap + j] = q.fore;
         cp[(size_t)(2 * a + 1) * il.cap + j] = q.back;
+    }
+}
+
+// the cubic interface values of the WENO-RK3 extension (cubic_interp,
+// wenork3_extend.cu:310-329) at the listed points, list order
+__global__ void __launch_bounds__(256) k_list_interp_cubic(double* __restrict__ cp, IList il,
+                                                           const double* __restrict__ C, Box b)
+{
+    const int total = *il.count;
+    const double sp[3] = {b.dx, b.dy, b.dz};
+    for (int j0 = 0; j0 < total; j0 += axis_lane_stride()) {
+        const AxisLane al = axis_lane(j0);
+        if (!(al.point >= 0 && al.point < total)) continue;
+        const int a = al.axis, j = al.point;
+        const double rf = il.dist[(size_t)(2 * a) * il.cap + j], lb = il.dist[(size_t)(2 * a + 1) * il.cap + j];
+        double cf = 0.0, cb = 0.0;
+        if (rf < sp[a] || lb < sp[a]) {
+            int r, c, p;
+            coords_of(b, (idx_t)il.idx[j], r, c, p);
+            double v[7];
+            line7(C, r, c, p, a, b, v);
+            cubic_interface_values(cf, cb, rf, lb, sp[a], v);
+        }
+        cp[(size_t)(2 * a) * il.cap + j] = cf;
+        cp[(size_t)(2 * a + 1) * il.cap + j] = cb;
     }
 }
 
@@ -438,7 +463,7 @@ __global__ void __launch_bounds__(256) k_extend_step(double* __restrict__ step, 
 // distances.  The list blocks are dispatched first; their long dependent
 // chains (the nonuniform WENO divides ~30 times) then run beside the bulk
 // blocks instead of after them.  The two sets of points are disjoint.
-template <int R, typename T>
+template <int R, typename T, int S = IFACE_WENO>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(stage_waves<T>()))) k_extend_stage(T* __restrict__ out, const T* __restrict__ cs,
                                                       const T* __restrict__ cn, const T* __restrict__ vx,
                                                       const T* __restrict__ vy, const T* __restrict__ vz,
@@ -456,7 +481,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(stage_
             ExtendLane q;
             if (active) {
                 const idx_t ind = (idx_t)il.idx[al.point];
-                q = extend_iface_lane(cs, b, inv, il, lcp, lv, al.point, ind, ind, al.axis);
+                q = extend_iface_lane<T, S>(cs, b, inv, il, lcp, lv, al.point, ind, ind, al.axis);
             }
             extend_iface_finish(out, cs, cn, q, al, active, stage);
         }
@@ -941,22 +966,36 @@ hipError_t launch_eno2_re_step(double* step, const double* lsf, const uint8_t* m
 }
 
 // bulk, then the interface list and the deferred bulk points, in stream order
-template <typename T>
-hipError_t launch_reinit_stage(T* out, const T* fs, const T* fn, const uint8_t* flags, const ListView& lv,
-                               const Geom& g, int stage, int* over, int* over_count, int seq, hipStream_t s)
+template <typename T, int S>
+static hipError_t reinit_stage_scheme(T* out, const T* fs, const T* fn, const uint8_t* flags, const ListView& lv,
+                                      const Geom& g, int stage, int* over, int* over_count, int seq, hipStream_t s)
 {
     const IList il{lv.idx, lv.count, lv.dist, lv.dt, lv.cap};
     const int zd = zmarch_depth(g), ni = fused_iface_blocks(g);
     const dim3 grid(bulk_blocks(g, zd) + ni);
     if (g.tr == 32)
-        k_reinit_stage<32, T><<<grid, dim3(256), 0, s>>>(out, fs, fn, flags, il, box_of(g), inv_of(g), stage, over,
-                                                         over_count, seq, zd, ni);
+        k_reinit_stage<32, T, S><<<grid, dim3(256), 0, s>>>(out, fs, fn, flags, il, box_of(g), inv_of(g), stage,
+                                                            over, over_count, seq, zd, ni);
     else
-        k_reinit_stage<64, T><<<grid, dim3(256), 0, s>>>(out, fs, fn, flags, il, box_of(g), inv_of(g), stage, over,
-                                                         over_count, seq, zd, ni);
-    k_reinit_deferred<T><<<dim3(deferred_blocks(g)), dim3(256), 0, s>>>(out, fs, fn, flags, il, box_of(g), inv_of(g),
-                                                                         stage, over, over_count, seq);
+        k_reinit_stage<64, T, S><<<grid, dim3(256), 0, s>>>(out, fs, fn, flags, il, box_of(g), inv_of(g), stage,
+                                                            over, over_count, seq, zd, ni);
+    k_reinit_deferred<T, S><<<dim3(deferred_blocks(g)), dim3(256), 0, s>>>(out, fs, fn, flags, il, box_of(g),
+                                                                            inv_of(g), stage, over, over_count, seq);
     return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_reinit_stage(T* out, const T* fs, const T* fn, const uint8_t* flags, const ListView& lv,
+                               const Geom& g, int stage, int* over, int* over_count, int seq, hipStream_t s)
+{
+    return reinit_stage_scheme<T, IFACE_WENO>(out, fs, fn, flags, lv, g, stage, over, over_count, seq, s);
+}
+
+hipError_t launch_cubic_reinit_stage(double* out, const double* fs, const double* fn, const uint8_t* flags,
+                                     const ListView& lv, const Geom& g, int stage, int* over, int* over_count,
+                                     int seq, hipStream_t s)
+{
+    return reinit_stage_scheme<double, IFACE_CUBIC>(out, fs, fn, flags, lv, g, stage, over, over_count, seq, s);
 }
 
 hipError_t launch_list_gather(const ListView& lv, const Dist6c& d, const Geom& g, hipStream_t s)
@@ -971,6 +1010,13 @@ hipError_t launch_list_interp(double* lcp, const ListView& lv, const T* C, const
 {
     const IList il{lv.idx, lv.count, lv.dist, lv.dt, lv.cap};
     k_list_interp<T><<<dim3(iface_blocks(g)), dim3(256), 0, s>>>(lcp, il, C, box_of(g));
+    return hipGetLastError();
+}
+
+hipError_t launch_list_interp_cubic(double* lcp, const ListView& lv, const double* C, const Geom& g, hipStream_t s)
+{
+    const IList il{lv.idx, lv.count, lv.dist, lv.dt, lv.cap};
+    k_list_interp_cubic<<<dim3(iface_blocks(g)), dim3(256), 0, s>>>(lcp, il, C, box_of(g));
     return hipGetLastError();
 }
 
@@ -1018,21 +1064,36 @@ hipError_t launch_extend_step(double* step, const double* deltat, const double* 
     VLS_POINT_LAUNCH(k_extend_step, step, deltat, C, vx, vy, vz, d, cp, box_of(g));
 }
 
-template <typename T>
-hipError_t launch_extend_stage(T* out, const T* cs, const T* cn, const T* vx, const T* vy, const T* vz,
-                               const uint8_t* flags, const ListView& lv, const double* lcp, const double* lvel,
-                               const Geom& g, int stage, hipStream_t s)
+template <typename T, int S>
+static hipError_t extend_stage_scheme(T* out, const T* cs, const T* cn, const T* vx, const T* vy, const T* vz,
+                                      const uint8_t* flags, const ListView& lv, const double* lcp,
+                                      const double* lvel, const Geom& g, int stage, hipStream_t s)
 {
     const IList il{lv.idx, lv.count, lv.dist, lv.dt, lv.cap};
     const int zd = zmarch_depth(g), ni = fused_iface_blocks(g);
     const dim3 grid(bulk_blocks(g, zd) + ni);
     if (g.tr == 32)
-        k_extend_stage<32, T><<<grid, dim3(256), 0, s>>>(out, cs, cn, vx, vy, vz, flags, il, lcp, lvel, box_of(g),
-                                                         inv_of(g), stage, zd, ni);
+        k_extend_stage<32, T, S><<<grid, dim3(256), 0, s>>>(out, cs, cn, vx, vy, vz, flags, il, lcp, lvel,
+                                                            box_of(g), inv_of(g), stage, zd, ni);
     else
-        k_extend_stage<64, T><<<grid, dim3(256), 0, s>>>(out, cs, cn, vx, vy, vz, flags, il, lcp, lvel, box_of(g),
-                                                         inv_of(g), stage, zd, ni);
+        k_extend_stage<64, T, S><<<grid, dim3(256), 0, s>>>(out, cs, cn, vx, vy, vz, flags, il, lcp, lvel,
+                                                            box_of(g), inv_of(g), stage, zd, ni);
     return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_extend_stage(T* out, const T* cs, const T* cn, const T* vx, const T* vy, const T* vz,
+                               const uint8_t* flags, const ListView& lv, const double* lcp, const double* lvel,
+                               const Geom& g, int stage, hipStream_t s)
+{
+    return extend_stage_scheme<T, IFACE_WENO>(out, cs, cn, vx, vy, vz, flags, lv, lcp, lvel, g, stage, s);
+}
+
+hipError_t launch_cubic_extend_stage(double* out, const double* cs, const double* cn, const double* vx,
+                                     const double* vy, const double* vz, const uint8_t* flags, const ListView& lv,
+                                     const double* lcp, const double* lvel, const Geom& g, int stage, hipStream_t s)
+{
+    return extend_stage_scheme<double, IFACE_CUBIC>(out, cs, cn, vx, vy, vz, flags, lv, lcp, lvel, g, stage, s);
 }
 
 hipError_t launch_advection_step(double* astep, const double* vx, const double* vy, const double* vz,

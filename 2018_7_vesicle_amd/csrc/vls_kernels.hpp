@@ -116,14 +116,19 @@ hipError_t launch_wenork3_boundary_interpolate(const Dist6& cp, const Dist6c& d,
 hipError_t launch_wenork3_extend_step(double* step, const double* deltat, const double* lsf, const double* vx,
                                       const double* vy, const double* vz, const Dist6c& d, const Dist6c& cp,
                                       const Geom& g, hipStream_t s);
-// fused stages of the wrapper loops (operator + RK combination; dt = 0.3*min
-// of the six distances d; cp the interface values; d and cp are read only
-// where flags has FLAG_SUBCELL)
-hipError_t launch_wenork3_reinit_stage(double* out, const double* fs, const double* fn, const uint8_t* flags,
-                                       const Dist6c& d, const Geom& g, int stage, hipStream_t s);
-hipError_t launch_wenork3_extend_stage(double* out, const double* cs, const double* cn, const double* vx,
-                                       const double* vy, const double* vz, const uint8_t* flags, const Dist6c& d,
-                                       const Dist6c& cp, const Geom& g, int stage, hipStream_t s);
+// the WENO-RK3 variants on the fused stage kernels of the WENO5 path (same
+// z-marching bulk; cubic ENO instead of the nonuniform WENO at the listed and
+// deferred points), FP64; lcp from launch_list_interp_cubic
+hipError_t launch_cubic_reinit_stage(double* out, const double* fs, const double* fn, const uint8_t* flags,
+                                     const ListView& lv, const Geom& g, int stage, int* over, int* over_count,
+                                     int seq, hipStream_t s);
+hipError_t launch_cubic_extend_stage(double* out, const double* cs, const double* cn, const double* vx,
+                                     const double* vy, const double* vz, const uint8_t* flags, const ListView& lv,
+                                     const double* lcp, const double* lvel, const Geom& g, int stage, hipStream_t s);
+hipError_t launch_list_interp_cubic(double* lcp, const ListView& lv, const double* C, const Geom& g, hipStream_t s);
+// fused stage of the ENO-RK2 extension loop (operator + Heun combination;
+// dt = 0.3*min of the six distances d; cp the interface values; d and cp are
+// read only where flags has FLAG_SUBCELL)
 hipError_t launch_enork2_extend_stage(double* out, const double* cs, const double* cn, const double* vx,
                                       const double* vy, const double* vz, const uint8_t* flags, const Dist6c& d,
                                       const Dist6c& cp, const Geom& g, int heun2, hipStream_t s);

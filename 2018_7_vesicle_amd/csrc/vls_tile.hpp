@@ -122,6 +122,20 @@ __device__ __forceinline__ double min6(const Dists& q)
 
 // per-axis derivative pair with the sign-change interface test (re_step.cu:248-269);
 // INL: the nonuniform face inline (see weno_nonuniform_pair_inl)
+// Interface schemes of the fused stage kernels: the WENO5 path's nonuniform
+// WENO (re_step.cu:79-243, shared_utilities.cup:49-278) or the WENO-RK3
+// variants' cubic ENO (weno_rk3_reinitialization.cu:149-247,
+// wenork3_extend.cu:88-190).  Away from the interface both use the same
+// uniform WENO5, so the bulk of a stage is shared.
+enum { IFACE_WENO = 0, IFACE_CUBIC = 1 };
+
+// per-axis pair with the sign-change interface test in the cubic-ENO scheme
+__device__ __forceinline__ void reinit_axis_cubic(double& fore, double& back, const double v[7], double r1,
+                                                  double l1, double ds, JPd jp)
+{
+    wenork3_pair(fore, back, v, r1, l1, ds, jp, v[2] * v[3] < 0 || v[3] * v[4] < 0, 0.0, 0.0);
+}
+
 template <bool INL = false>
 __device__ __forceinline__ void reinit_axis(double& fore, double& back, const double v[7], double r1, double l1,
                                             double ds, JPd jp)
@@ -616,7 +630,7 @@ struct ReinitLane {
 // slot: the point's position in the interface list, or -1 for a deferred
 // bulk point (spacing distances, bulk time step); ind: its index in the grid
 // (fs, flags of that grid); gind: its index in the output fields
-template <typename T>
+template <typename T, int SCHEME = IFACE_WENO>
 __device__ __forceinline__ ReinitLane reinit_iface_lane(const T* __restrict__ fs, const uint8_t* __restrict__ flags,
                                                         const Box& b, const Inv& inv, const IList& il, int slot,
                                                         idx_t ind, idx_t gind, int a)
@@ -637,7 +651,8 @@ __device__ __forceinline__ ReinitLane reinit_iface_lane(const T* __restrict__ fs
     if (q.dt >= 0) {
         double v[7], fo, bk;
         line7(fs, r, c, p, a, b, v);
-        reinit_axis<true>(fo, bk, v, r1, l1, sp[a], inv.jd[a]);
+        if constexpr (SCHEME == IFACE_CUBIC) reinit_axis_cubic(fo, bk, v, r1, l1, sp[a], inv.jd[a]);
+        else reinit_axis<true>(fo, bk, v, r1, l1, sp[a], inv.jd[a]);
         q.m = q.inside ? dmax(sq(dmin(0.0, bk)), sq(dmax(0.0, fo))) : dmax(sq(dmax(0.0, bk)), sq(dmin(0.0, fo)));
     }
     return q;
@@ -688,7 +703,7 @@ struct ExtendLane {
 
 // slot: position in the list (lcp, lv, il.dist, il.dt rows); ind: index in
 // the grid of cs; gind: index in the output fields
-template <typename T>
+template <typename T, int SCHEME = IFACE_WENO>
 __device__ __forceinline__ ExtendLane extend_iface_lane(const T* __restrict__ cs, const Box& b, const Inv& inv,
                                                         const IList& il, const double* __restrict__ lcp,
                                                         const double* __restrict__ lv, int slot, idx_t ind,
@@ -708,9 +723,16 @@ __device__ __forceinline__ ExtendLane extend_iface_lane(const T* __restrict__ cs
         const double l1 = il.dist[(size_t)(2 * a + 1) * il.cap + slot];
         if (r1 < sp[a] || l1 < sp[a]) {
             const bool fore = vel < 0;
-            const double dv = weno_nonuniform_one_inl(v[0], v[1], v[2], v[3], v[4], v[5], v[6], r1, l1, sp[a],
-                                                  lcp[(size_t)(2 * a) * il.cap + slot],
-                                                  lcp[(size_t)(2 * a + 1) * il.cap + slot], !fore);
+            const double hf = lcp[(size_t)(2 * a) * il.cap + slot], hb = lcp[(size_t)(2 * a + 1) * il.cap + slot];
+            double dv;
+            if constexpr (SCHEME == IFACE_CUBIC) {
+                double x[7], h[7], df, db;
+                cubic_stencil(x, h, v, r1, l1, sp[a], hf, hb);
+                eno_cubic_pair(df, db, x, h);
+                dv = fore ? df : db;
+            } else {
+                dv = weno_nonuniform_one_inl(v[0], v[1], v[2], v[3], v[4], v[5], v[6], r1, l1, sp[a], hf, hb, !fore);
+            }
             q.t = 0.0 + (fore ? vel : dmax(0.0, vel)) * dv;
         } else {
             q.t = upwind_axis(0.0, vel, v, inv.jd[a]);

@@ -8,10 +8,11 @@
 // 64 rows x 4 cols of one page per block, clamped neighbours.  Two families:
 //   * drop-in kernels with the argument meaning of the reference's entry
 //     points (re_step, upwind_normal, boundary_interpolate, extend_step);
-//   * fused stage kernels for the wrapper loops: the operator and its RK
-//     combination in one pass, the pseudo-time step 0.3*min of the six
-//     distances derived in the kernel (WENORK3Reinitialization.m:23-26,
-//     ENORK2Extend.m:61-64, WENORK3Extend.m:62-65).
+//   * the fused stage of the ENO-RK2 extension loop: the operator and its
+//     Heun combination in one pass, the pseudo-time step 0.3*min of the six
+//     distances derived in the kernel (ENORK2Extend.m:61-64).
+// The WENO-RK3 wrapper loops run on the fused stage kernels of the WENO5 path
+// (vls_kernels.hip) with the cubic-ENO interface scheme.
 // The cubic crossing distances (cubic boundary_correction) are the ORDER 3
 // case of k_boundary_location in vls_kernels.hip.
 #include "vls_tile.hpp"
@@ -250,42 +251,6 @@ __global__ void __launch_bounds__(256) k_wenork3_extend_step(double* __restrict_
 // fused stage kernels of the wrapper loops
 // ---------------------------------------------------------------------------
 
-// WENORK3Reinitialization.m:30-57: stage 0 F1 = F - L(F); stage 1
-// F05 = 3/4 F + 1/4 (F1 - L(F1)); stage 2 F = 1/3 F + 2/3 (F05 - L(F05));
-// fn = F (stages 1, 2); mask from the FLAG_MASK bit
-__global__ void __launch_bounds__(256) k_wenork3_reinit_stage(double* __restrict__ out, const double* __restrict__ fs,
-                                                              const double* __restrict__ fn,
-                                                              const uint8_t* __restrict__ flags, Dist6c d, Box b,
-                                                              int stage)
-{
-    int r, c, p;
-    if (!point_of_thread(b, r, c, p)) return;
-    const idx_t ind = lin(b, r, c, p);
-    const uint8_t fl = flags[ind];
-    Dists q;
-    point_dists(q, d, ind, (fl & FLAG_SUBCELL) != 0, b);
-    const double step = wenork3_reinit_residual(fs, r, c, p, b, q, (fl & FLAG_MASK) != 0, 0.3 * min6(q));
-    out[ind] = rk_combine(stage, fs[ind], step, (stage == 0) ? 0.0 : fn[ind]);
-}
-
-// WENORK3Extend.m:69-96, the same RK3 combinations as the reinitialization
-__global__ void __launch_bounds__(256) k_wenork3_extend_stage(double* __restrict__ out, const double* __restrict__ cs,
-                                                              const double* __restrict__ cn,
-                                                              const double* __restrict__ vx,
-                                                              const double* __restrict__ vy,
-                                                              const double* __restrict__ vz, const uint8_t* __restrict__ flags, Dist6c d,
-                                                             Dist6c cp,
-                                                              Box b, int stage)
-{
-    int r, c, p;
-    if (!point_of_thread(b, r, c, p)) return;
-    const idx_t ind = lin(b, r, c, p);
-    Dists q;
-    point_dists(q, d, ind, (flags[ind] & FLAG_SUBCELL) != 0, b);
-    const double step = wenork3_extend_residual(cs, r, c, p, b, q, cp, ind, vx[ind], vy[ind], vz[ind], 0.3 * min6(q));
-    out[ind] = rk_combine(stage, cs[ind], step, (stage == 0) ? 0.0 : cn[ind]);
-}
-
 // ENORK2Extend.m:68-79 (Heun): heun2 == 0: Ctmp = C - L(C); heun2 == 1:
 // C = (C + Ctmp - L(Ctmp)) / 2 with cs = Ctmp, cn = C
 __global__ void __launch_bounds__(256) k_enork2_extend_stage(double* __restrict__ out, const double* __restrict__ cs,
@@ -359,19 +324,6 @@ hipError_t launch_wenork3_extend_step(double* step, const double* deltat, const 
                                       const Geom& g, hipStream_t s)
 {
     VLS_VARIANT_LAUNCH(k_wenork3_extend_step, step, deltat, lsf, vx, vy, vz, d, cp, vbox(g));
-}
-
-hipError_t launch_wenork3_reinit_stage(double* out, const double* fs, const double* fn, const uint8_t* flags,
-                                       const Dist6c& d, const Geom& g, int stage, hipStream_t s)
-{
-    VLS_VARIANT_LAUNCH(k_wenork3_reinit_stage, out, fs, fn, flags, d, vbox(g), stage);
-}
-
-hipError_t launch_wenork3_extend_stage(double* out, const double* cs, const double* cn, const double* vx,
-                                       const double* vy, const double* vz, const uint8_t* flags, const Dist6c& d,
-                                       const Dist6c& cp, const Geom& g, int stage, hipStream_t s)
-{
-    VLS_VARIANT_LAUNCH(k_wenork3_extend_stage, out, cs, cn, vx, vy, vz, flags, d, cp, vbox(g), stage);
 }
 
 hipError_t launch_enork2_extend_stage(double* out, const double* cs, const double* cn, const double* vx,
