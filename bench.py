@@ -32,7 +32,8 @@ its error measures), config3 (the pipeline at 256^3 with a 32-mode velocity,
 FP32 bulk vs FP64, 200 steps each, with the FP32 run's errors), config4
 (512 vesicles of uneven sizes batched, FP32 bulk vs FP64 batched vs the plain
 per-vesicle FP64 path) and config5 (the 96^3 adjoint gradient against finite
-differences and tangent-mode derivatives).
+differences and tangent-mode derivatives), variants (the WENO-RK3 / ENO-RK2
+extensions and WENO-RK3 reinitialization at 96^3).
 """
 import argparse
 import json
@@ -61,9 +62,10 @@ def parse():
     ap.add_argument("--batch-steps", type=int, default=10,
                     help="timed steps of the 512-vesicle batch comparison (0 skips it)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 96^3 adjoint gradient check")
-    ap.add_argument("--probe", choices=["extend", "batch"], default=None,
+    ap.add_argument("--probe", choices=["extend", "batch", "variants"], default=None,
                     help="extend: a few extension stages and a 256 MiB device copy (for rocprofv3 --pmc); "
-                         "batch: a few steps of the 512-vesicle FP32 batch (for rocprofv3 --kernel-trace)")
+                         "batch: a few steps of the 512-vesicle FP32 batch (for rocprofv3 --kernel-trace); "
+                         "variants: the WENO-RK3 / ENO-RK2 calls of the variants line (for rocprofv3 --kernel-trace)")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64",
                     help="field type of the main pipeline line (fp32: FP32 bulk)")
     ap.add_argument("--main-only", action="store_true",
@@ -232,6 +234,58 @@ def config4(steps):
     return r
 
 
+def variants(with_cpu, iterations=100):
+    """Beside the main line: the WENO-RK3 and ENO-RK2 extensions and the
+    WENO-RK3 reinitialization that the vesicle drivers call every step
+    (src/7_0_paper/2_0_Vesicle/p00_test_singlephase_relax.m:76,84,115) on the drivers'
+    96^3 grid: an ellipsoid level set on [-1,1]^3, a smooth field extended off
+    it; ms per call (100 iterations) and stage updates per second, on the GPU
+    and (10 iterations, scaled) on the CPU oracle."""
+    import numpy as np
+    import torch
+
+    vls = __import__("importlib").import_module("2018_7_vesicle_amd")
+    n = 96
+    xv = np.linspace(-1.0, 1.0, n)
+    Z, X, Y = np.meshgrid(xv, xv, xv, indexing="ij")
+    F0 = np.ascontiguousarray(np.sqrt(X ** 2 / 0.7 ** 2 + Y ** 2 / 0.55 ** 2 + Z ** 2 / 0.45 ** 2) - 1.0)
+    C0 = np.ascontiguousarray(0.5 * X + Y * Z + 0.2 * np.sin(3.0 * Y))
+    G = vls.GD3(xv, xv, xv)
+    F = torch.from_numpy(F0).cuda()
+    C = torch.from_numpy(C0).cuda()
+    m = vls.SDF3(G, F)
+    calls = (("WENORK3Extend", 3, lambda k: m.WENORK3Extend(C, k), "wenork3"),
+             ("ENORK2Extend", 2, lambda k: m.ENORK2Extend(C, k), "enork2"),
+             ("WENORK3Reinitialization", 3, lambda k: m.WENORK3Reinitialization(F, k), None))
+    out = {"grid": n, "iterations": iterations}
+    for name, per_it, fn, scheme in calls:
+        fn(iterations)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(3):
+            fn(iterations)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 3
+        rec = {"gpu_ms": ms, "gpu_updates_per_s": per_it * iterations * n ** 3 / (ms * 1e-3)}
+        if with_cpu:
+            from oracle import pyoracle as po
+
+            k = 10
+            t0 = time.perf_counter()
+            if scheme is None:
+                po.wenork3_reinit(F0, k, G.Dx, G.Dy, G.Dz)
+            else:
+                po.variant_extend(C0, k, F0, G.Dx, G.Dy, G.Dz, scheme)
+            cpu_s = time.perf_counter() - t0
+            rec["cpu_updates_per_s"] = per_it * k * n ** 3 / cpu_s
+            rec["cpu_threads"] = po.num_threads()
+        out[name] = rec
+    return out
+
+
 def config5():
     """Config 5 beside the main line: the 96^3 shape-matching problem (200
     WENO5-RK3 advection steps, reinitialization every 10, velocity from 3x64
@@ -257,6 +311,9 @@ def main():
         return probe_extend(args)
     if args.probe == "batch":
         return probe_batch(args)
+    if args.probe == "variants":
+        print(json.dumps({"probe": "variants", **variants(False, iterations=20)}), flush=True)
+        return 0
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported by this build "
               "(the multi-GPU z-slab decomposition is a design note, not code)", file=sys.stderr)
@@ -348,6 +405,7 @@ def main():
             out["config4"] = config4(args.batch_steps)
         if not args.no_adjoint:
             out["config5"] = config5()
+        out["variants"] = variants(not args.no_cpu)
         if not args.no_cpu:
             out["cpu_baseline"] = cpu_baseline(cfg)
     print(json.dumps(out), flush=True)
