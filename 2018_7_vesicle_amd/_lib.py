@@ -32,6 +32,8 @@ SIGNATURES = {
     "vls_advection_step": [P] * 5 + [I] * 3 + [D] * 3 + [P],
     "vls_weno_derivative": [P] * 7 + [I] * 3 + [D] * 3 + [P],
     "vls_set_calculus_toolbox": [P] * 3 + [I] * 3 + [D] * 4 + [I, P],
+    "vls_auxi_set_calculus_toolbox": [P] * 4 + [I] * 3 + [D] * 4 + [I, P],
+    "vls_aux_calculus_toolbox": [P] * 3 + [I] * 3 + [D] * 4 + [P],
     "vls_cubic_boundary_correction": [P] * 7 + [I] * 4 + [D] * 3 + [P],
     "vls_wenork3_re_step": [P] * 10 + [I] * 3 + [D] * 3 + [I, P],
     "vls_enork2_upwind_normal": [P] * 10 + [I] * 3 + [D] * 3 + [I, P],

@@ -780,6 +780,52 @@ int vls_set_calculus_toolbox(double* const* out18, const double* lsf, const doub
     return VLS_OK;
 }
 
+int vls_auxi_set_calculus_toolbox(double* const* out27, const double* lsf, const double* ahprimal,
+                                  const double* const* fin13, int rows, int cols, int pges, double dx, double dy,
+                                  double dz, double ds, int num_ele, vls_stream_t stream)
+{
+    int rc = check_geom(rows, cols, pges, dx, dy, dz);
+    if (rc) return rc;
+    if ((rc = check_num(num_ele, rows, cols, pges))) return rc;
+    VLS_REQUIRE(!any_null(out27, lsf, ahprimal, fin13), "null pointer");
+    Fields27 o;
+    for (int k = 0; k < 27; ++k) {
+        VLS_REQUIRE(out27[k] != nullptr, "null output %d", k);
+        o.f[k] = out27[k];
+    }
+    Fields13c in;
+    for (int k = 0; k < 13; ++k) {
+        VLS_REQUIRE(fin13[k] != nullptr, "null input field %d", k);
+        in.f[k] = fin13[k];
+    }
+    const Geom g{rows, cols, pges, dx, dy, dz};
+    VLS_HIP(launch_auxi_set_calculus_toolbox(o, lsf, ahprimal, in, g, ds, S(stream)));
+    return VLS_OK;
+}
+
+int vls_aux_calculus_toolbox(double* const* out30, const double* A, const double* const* ftool14, int rows,
+                             int cols, int pges, double dx, double dy, double dz, double ds, vls_stream_t stream)
+{
+    int rc = check_geom(rows, cols, pges, dx, dy, dz);
+    if (rc) return rc;
+    VLS_REQUIRE(!any_null(out30, A, ftool14), "null pointer");
+    for (int k = 0; k < 30; ++k) VLS_REQUIRE(out30[k] != nullptr, "null output %d", k);
+    for (int k = 0; k < 14; ++k) VLS_REQUIRE(ftool14[k] != nullptr, "null input field %d", k);
+    const Geom g{rows, cols, pges, dx, dy, dz};
+    // out30 = {AHPrimal, the 27 kernel outputs, AFDiracDelta, ADiracDeltaDn}
+    VLS_HIP(launch_positive_part(out30[0], A, g.n(), S(stream)));
+    Fields27 o;
+    for (int k = 0; k < 27; ++k) o.f[k] = out30[1 + k];
+    Fields13c in;
+    for (int k = 0; k < 13; ++k) in.f[k] = ftool14[k];
+    VLS_HIP(launch_auxi_set_calculus_toolbox(o, A, out30[0], in, g, ds, S(stream)));
+    // out30 positions: AGradMag 4, C 5..7, NormCrossAF 8, AHeaviside 26, ADiracDelta 27;
+    // F's Heaviside is ftool14[13]
+    VLS_HIP(launch_aux_toolbox_tail(out30[28], out30[29], out30[26], ftool14[13], A, out30[5], out30[6], out30[7],
+                                    out30[8], out30[27], out30[4], g, S(stream)));
+    return VLS_OK;
+}
+
 // ---------------------------------------------------------------------------
 // 2. @SDF3 handle
 // ---------------------------------------------------------------------------

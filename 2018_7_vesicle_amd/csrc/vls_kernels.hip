@@ -661,6 +661,131 @@ __global__ void __launch_bounds__(256) k_set_calculus_toolbox(Fields18 o, const 
 }
 #undef T27
 
+// ---------------------------------------------------------------------------
+// auxiliary-field geometry (4_0_Calculus_ToolBox/auxi_set_calculus_toolbox.cu:9-199,
+// with norm and cross_product of shared_utilities.cup:7-19).  pow(g,k) is the
+// product g*...*g, as in the toolbox above.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void cross3(double& wx, double& wy, double& wz, double ux, double uy, double uz,
+                                       double vx, double vy, double vz)
+{
+#pragma clang fp contract(off)
+    wx = uy * vz - uz * vy;
+    wy = uz * vx - ux * vz;
+    wz = ux * vy - uy * vx;
+}
+__device__ __forceinline__ double norm3(double x, double y, double z)
+{
+    return dmax(sqrt(x * x + y * y + z * z), 1e-14);
+}
+
+__global__ void __launch_bounds__(256) k_auxi_set_calculus_toolbox(Fields27 o, const double* __restrict__ A,
+                                                                   const double* __restrict__ ahp, Fields13c fin,
+                                                                   Box b, double ds)
+{
+#pragma clang fp contract(off)
+    int r, c, p;
+    if (!point_of_thread(b, r, c, p)) return;
+    const idx_t ind = lin(b, r, c, p);
+    Box27 Ab, Pb;
+    load_box27(Ab, A, r, c, p, b);
+    load_box27(Pb, ahp, r, c, p, b);
+    const Toolbox a = toolbox_of(Ab, b, ds);  // Ax..Azx, |grad A| = norm, ALaplacian
+    o.f[0][ind] = a.fx; o.f[1][ind] = a.fy; o.f[2][ind] = a.fz; o.f[3][ind] = a.g;
+    const double fx = fin.f[0][ind], fy = fin.f[1][ind], fz = fin.f[2][ind], fg = fin.f[3][ind];
+    double cx, cy, cz;
+    cross3(cx, cy, cz, fx, fy, fz, a.fx, a.fy, a.fz);
+    const double ncr = norm3(cx, cy, cz);
+    o.f[4][ind] = cx; o.f[5][ind] = cy; o.f[6][ind] = cz; o.f[7][ind] = ncr;
+    const double tx = cx / ncr, ty = cy / ncr, tz = cz / ncr;
+    o.f[8][ind] = tx; o.f[9][ind] = ty; o.f[10][ind] = tz;
+    double nx, ny, nz;
+    cross3(nx, ny, nz, tx, ty, tz, fin.f[4][ind], fin.f[5][ind], fin.f[6][ind]);
+    o.f[11][ind] = nx; o.f[12][ind] = ny; o.f[13][ind] = nz;
+    o.f[14][ind] = a.fxx; o.f[15][ind] = a.fyy; o.f[16][ind] = a.fzz;
+    o.f[17][ind] = a.fxy; o.f[18][ind] = a.fyz; o.f[19][ind] = a.fzx; o.f[20][ind] = a.lap;
+    const double fxx = fin.f[7][ind], fyy = fin.f[8][ind], fzz = fin.f[9][ind];
+    const double fxy = fin.f[10][ind], fyz = fin.f[11][ind], fzx = fin.f[12][ind];
+    // geodesic curvature (auxi_set_calculus_toolbox.cu:115-131)
+    double vx = tx * fxx + ty * fxy + tz * fzx;
+    double vy = tx * fxy + ty * fyy + tz * fyz;
+    double vz = tx * fzx + ty * fyz + tz * fzz;
+    double w1x, w1y, w1z;
+    cross3(w1x, w1y, w1z, vx, vy, vz, a.fx, a.fy, a.fz);
+    vx = tx * a.fxx + ty * a.fxy + tz * a.fzx;
+    vy = tx * a.fxy + ty * a.fyy + tz * a.fyz;
+    vz = tx * a.fzx + ty * a.fyz + tz * a.fzz;
+    double w2x, w2y, w2z;
+    cross3(w2x, w2y, w2z, fx, fy, fz, vx, vy, vz);
+    o.f[21][ind] = (nx * (w1x + w2x) + ny * (w1y + w2y) + nz * (w1z + w2z)) / ncr;
+    // the shape operator of F's level sets, dN/dx (:135-145)
+    const double g3 = fg * fg * fg;
+    const double sx = fxx * fx + fxy * fy + fzx * fz;
+    const double sy = fxy * fx + fyy * fy + fyz * fz;
+    const double sz = fzx * fx + fyz * fy + fzz * fz;
+    const double Nxx = fxx / fg - fx * sx / g3, Nyx = fxy / fg - fy * sx / g3, Nzx = fzx / fg - fz * sx / g3;
+    const double Nxy = fxy / fg - fx * sy / g3, Nyy = fyy / fg - fy * sy / g3, Nzy = fyz / fg - fz * sy / g3;
+    const double Nxz = fzx / fg - fx * sz / g3, Nyz = fyz / fg - fy * sz / g3, Nzz = fzz / fg - fz * sz / g3;
+    vx = Nxx * tx + Nxy * ty + Nxz * tz;
+    vy = Nyx * tx + Nyy * ty + Nyz * tz;
+    vz = Nzx * tx + Nzy * ty + Nzz * tz;
+    o.f[22][ind] = -(tx * vx + ty * vy + tz * vz);
+    vx = Nxx * nx + Nxy * ny + Nxz * nz;
+    vy = Nyx * nx + Nyy * ny + Nyz * nz;
+    vz = Nzx * nx + Nzy * ny + Nzz * nz;
+    o.f[23][ind] = -(tx * vx + ty * vy + tz * vz);
+    o.f[24][ind] = -(nx * vx + ny * vy + nz * vz);
+    // Heaviside and delta of A from its primal (:166-197)
+    double H, delta;
+    towers_h_delta(H, delta, Pb, a, b);
+    o.f[25][ind] = H;
+    o.f[26][ind] = delta;
+}
+
+// periodic neighbour index of the @GD3 index maps (circshift, GD3.m:112-117)
+__device__ __forceinline__ int wrapi(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+
+// GPUAsetCalculusToolBox.m:55-62: AFDiracDelta = C.(grad H x grad AH)/|C|^2 and
+// ADiracDeltaDn = (Lap AH - ADiracDelta Lap A)/|grad A| with @GD3's periodic
+// Fx/Fy/Fz (GD3.m:218-240) and Laplacian (:344-346, Fxx :248-250)
+__global__ void __launch_bounds__(256) k_aux_toolbox_tail(double* __restrict__ afdd, double* __restrict__ addn,
+                                                          const double* __restrict__ ah, const double* __restrict__ h,
+                                                          const double* __restrict__ a, const double* __restrict__ cx,
+                                                          const double* __restrict__ cy, const double* __restrict__ cz,
+                                                          const double* __restrict__ ncross,
+                                                          const double* __restrict__ add,
+                                                          const double* __restrict__ agrad, Box b)
+{
+#pragma clang fp contract(off)
+    int r, c, p;
+    if (!point_of_thread(b, r, c, p)) return;
+    const idx_t ind = lin(b, r, c, p);
+    const idx_t xp = lin(b, r, wrapi(c + 1, b.cols), p), xm = lin(b, r, wrapi(c - 1, b.cols), p);
+    const idx_t yp = lin(b, wrapi(r + 1, b.rows), c, p), ym = lin(b, wrapi(r - 1, b.rows), c, p);
+    const idx_t zp = lin(b, r, c, wrapi(p + 1, b.pges)), zm = lin(b, r, c, wrapi(p - 1, b.pges));
+    const double h1x = (ah[xp] - ah[xm]) / (2 * b.dx), h1y = (ah[yp] - ah[ym]) / (2 * b.dy),
+                 h1z = (ah[zp] - ah[zm]) / (2 * b.dz);
+    const double h2x = (h[xp] - h[xm]) / (2 * b.dx), h2y = (h[yp] - h[ym]) / (2 * b.dy),
+                 h2z = (h[zp] - h[zm]) / (2 * b.dz);
+    double chx, chy, chz;
+    cross3(chx, chy, chz, h2x, h2y, h2z, h1x, h1y, h1z);
+    const double nc = ncross[ind];
+    afdd[ind] = (cx[ind] * chx + cy[ind] * chy + cz[ind] * chz) / (nc * nc);
+    const double lah = (ah[xp] - 2 * ah[ind] + ah[xm]) / (b.dx * b.dx) + (ah[yp] - 2 * ah[ind] + ah[ym]) / (b.dy * b.dy) +
+                       (ah[zp] - 2 * ah[ind] + ah[zm]) / (b.dz * b.dz);
+    const double la = (a[xp] - 2 * a[ind] + a[xm]) / (b.dx * b.dx) + (a[yp] - 2 * a[ind] + a[ym]) / (b.dy * b.dy) +
+                      (a[zp] - 2 * a[ind] + a[zm]) / (b.dz * b.dz);
+    addn[ind] = (lah - add[ind] * la) / agrad[ind];
+}
+
+// AHPrimal = max(A, 0) (GPUAsetCalculusToolBox.m:4); MATLAB's max ignores a NaN,
+// so a NaN of A gives 0
+__global__ void k_positive_part(double* __restrict__ out, const double* __restrict__ a, idx_t n)
+{
+    const idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = a[i] > 0.0 ? a[i] : 0.0;
+}
+
 // normal component of a velocity, u.N with N = grad F/max(|grad F|,1e-14) from
 // clamped central differences (set_calculus_toolbox.cu:123-136)
 template <typename T>
@@ -1135,6 +1260,25 @@ hipError_t launch_set_calculus_toolbox(const Fields18& o, const double* lsf, con
                                        double ds, hipStream_t s)
 {
     VLS_POINT_LAUNCH(k_set_calculus_toolbox, o, lsf, hp, box_of(g), ds);
+}
+
+hipError_t launch_auxi_set_calculus_toolbox(const Fields27& o, const double* lsf, const double* ahp,
+                                            const Fields13c& fin, const Geom& g, double ds, hipStream_t s)
+{
+    VLS_POINT_LAUNCH(k_auxi_set_calculus_toolbox, o, lsf, ahp, fin, box_of(g), ds);
+}
+
+hipError_t launch_aux_toolbox_tail(double* afdd, double* addn, const double* ah, const double* h, const double* a,
+                                   const double* cx, const double* cy, const double* cz, const double* ncross,
+                                   const double* add, const double* agrad, const Geom& g, hipStream_t s)
+{
+    VLS_POINT_LAUNCH(k_aux_toolbox_tail, afdd, addn, ah, h, a, cx, cy, cz, ncross, add, agrad, box_of(g));
+}
+
+hipError_t launch_positive_part(double* out, const double* a, int64_t n, hipStream_t s)
+{
+    k_positive_part<<<flat_grid(n), dim3(256), 0, s>>>(out, a, (idx_t)n);
+    return hipGetLastError();
 }
 
 // 1024 blocks x 256 threads: 4 blocks per CU on 256 CUs

@@ -22,6 +22,13 @@ struct Dist6c {
 struct Fields18 {
     double* f[18];
 };
+// the auxiliary-field toolbox: 27 outputs and the 13 toolbox fields of F it reads
+struct Fields27 {
+    double* f[27];
+};
+struct Fields13c {
+    const double* f[13];
+};
 
 // per-point flag bits of the fused stage kernels
 enum : uint8_t {
@@ -147,6 +154,18 @@ template <typename T>
 hipError_t launch_advect_stage(T* out, const T* fs, const T* fn, const T* vx, const T* vy, const T* vz, double dt,
                                const Geom& g, int stage, hipStream_t s);
 hipError_t launch_weno_derivative(const Dist6& out, const double* lsf, const Geom& g, hipStream_t s);
+// auxi_set_calculus_toolbox (4_0_Calculus_ToolBox/auxi_set_calculus_toolbox.cu:9-199): out
+// = Ax..ADiracDelta of the auxiliary field A (lsf) on the surface of F, whose
+// toolbox fields Fx, Fy, Fz, FGradMag, Nx, Ny, Nz, Fxx, Fyy, Fzz, Fxy, Fyz, Fzx
+// are fin; clamped neighbours
+hipError_t launch_auxi_set_calculus_toolbox(const Fields27& o, const double* lsf, const double* ahp,
+                                            const Fields13c& fin, const Geom& g, double ds, hipStream_t s);
+// the MATLAB tail of GPUAsetCalculusToolBox.m:55-62 with the periodic central
+// differences of @GD3 (GD3.m:112-132 index maps): AFDiracDelta and ADiracDeltaDn
+hipError_t launch_aux_toolbox_tail(double* afdd, double* addn, const double* ah, const double* h, const double* a,
+                                   const double* cx, const double* cy, const double* cz, const double* ncross,
+                                   const double* add, const double* agrad, const Geom& g, hipStream_t s);
+hipError_t launch_positive_part(double* out, const double* a, int64_t n, hipStream_t s);
 hipError_t launch_set_calculus_toolbox(const Fields18& o, const double* lsf, const double* hp, const Geom& g,
                                        double ds, hipStream_t s);
 // reductions: `partial` holds at least reduce_blocks()*4 doubles, `result` 2

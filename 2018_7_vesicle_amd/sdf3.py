@@ -26,10 +26,18 @@ __all__ = ["GD3", "SDF3", "VLSError"] + [
     "feval_boundary_location", "feval_boundary_correction", "feval_re_step", "feval_eno2_re_step",
     "feval_upwind_normal", "feval_boundary_interpolant", "feval_extend_step", "feval_advection_step",
     "feval_weno_derivative", "feval_set_calculus_toolbox", "TOOLBOX_FIELDS",
+    "feval_auxi_set_calculus_toolbox", "feval_aux_calculus_toolbox", "AUX_TOOLBOX_FIELDS", "AUX_METHOD_FIELDS",
 ]
 
 TOOLBOX_FIELDS = ["Fx", "Fy", "Fz", "FGradMag", "Nx", "Ny", "Nz", "Fxx", "Fyy", "Fzz", "Fxy", "Fyz", "Fzx",
                   "FLaplacian", "MeanCurvature", "GaussianCurvature", "Heaviside", "DiracDelta"]
+# auxi_set_calculus_toolbox's 27 outputs under the @SDF3 property names
+# (+SD/@SDF3/GPUAsetCalculusToolBox.m:40-43)
+AUX_TOOLBOX_FIELDS = ["Ax", "Ay", "Az", "AGradMag", "Cx", "Cy", "Cz", "NormCrossAF", "tx", "ty", "tz", "nx", "ny",
+                      "nz", "Axx", "Ayy", "Azz", "Axy", "Ayz", "Azx", "ALaplacian", "GeodesicCurvature",
+                      "NormalCurvature", "GeodesicTorsion", "BPerpendicular", "AHeaviside", "ADiracDelta"]
+# what GPUAsetCalculusToolBox.m sets, in vls_aux_calculus_toolbox's out30 order
+AUX_METHOD_FIELDS = ["AHPrimal"] + AUX_TOOLBOX_FIELDS + ["AFDiracDelta", "ADiracDeltaDn"]
 
 
 def _stream():
@@ -98,6 +106,11 @@ class SDF3:
         for name in TOOLBOX_FIELDS:
             setattr(self, name, None)
         self.HPrimal = None
+        # the auxiliary level set A (the @SDF3 property the AsetCalculusToolBox
+        # methods read) and its toolbox
+        self.A = None
+        for name in AUX_METHOD_FIELDS:
+            setattr(self, name, None)
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -252,6 +265,27 @@ class SDF3:
         outs = [g.zeros() for _ in TOOLBOX_FIELDS]
         feval_set_calculus_toolbox(outs, self.F, self.HPrimal, g.mrows, g.ncols, g.lshts, g.Dx, g.Dy, g.Dz, g.Ds)
         for name, t in zip(TOOLBOX_FIELDS, outs):
+            setattr(self, name, t)
+
+    def GPUAsetCalculusToolBox(self):
+        """+SD/@SDF3/GPUAsetCalculusToolBox.m:2-64: the geometry of the auxiliary
+        field A on the surface of F -- AHPrimal = max(A,0), the 27 fields of
+        auxi_set_calculus_toolbox, AFDiracDelta and ADiracDeltaDn (the last two
+        with @GD3's periodic differences).  Needs self.A and the toolbox of F
+        (GPUsetCalculusToolBox) first.  FP64 only."""
+        if self.dtype != torch.float64:
+            raise VLSError("GPUAsetCalculusToolBox is FP64 only")
+        if self.A is None:
+            raise VLSError("GPUAsetCalculusToolBox needs the auxiliary field A")
+        if self.Fx is None:
+            raise VLSError("GPUAsetCalculusToolBox needs GPUsetCalculusToolBox first")
+        g = self.GD3
+        outs = [g.zeros() for _ in AUX_METHOD_FIELDS]
+        fin = [getattr(self, n) for n in TOOLBOX_FIELDS[:13]] + [self.Heaviside]
+        for t in fin + [self.A]:
+            self._check(t)
+        feval_aux_calculus_toolbox(outs, self.A, fin, g.mrows, g.ncols, g.lshts, g.Dx, g.Dy, g.Dz, g.Ds)
+        for name, t in zip(AUX_METHOD_FIELDS, outs):
             setattr(self, name, t)
 
     def calAreaVolume(self):
@@ -450,6 +484,29 @@ def feval_weno_derivative(back_x, fore_x, back_y, fore_y, back_z, fore_z, lsf, r
     call("vls_weno_derivative", *[_ptr(t) for t in (back_x, fore_x, back_y, fore_y, back_z, fore_z, lsf)], rows,
          cols, pges, dx, dy, dz, _stream())
     return back_x, fore_x, back_y, fore_y, back_z, fore_z
+
+
+def feval_auxi_set_calculus_toolbox(out27, lsf, ahprimal, fin13, rows, cols, pges, dx, dy, dz, ds):
+    """feval(auxi_set_calculus_toolbox, ...) of +SD/@SDF3/GPUAsetCalculusToolBox.m:44-54:
+    out27 in AUX_TOOLBOX_FIELDS order; fin13 = Fx..Fzx of F (TOOLBOX_FIELDS[:13])."""
+    if len(out27) != 27 or len(fin13) != 13:
+        raise VLSError("auxi_set_calculus_toolbox writes 27 fields from 13 fields of F")
+    arr = (ctypes.c_void_p * 27)(*[_ptr(t).value for t in out27])
+    farr = (ctypes.c_void_p * 13)(*[_ptr(t).value for t in fin13])
+    call("vls_auxi_set_calculus_toolbox", arr, _ptr(lsf), _ptr(ahprimal), farr, rows, cols, pges, dx, dy, dz, ds,
+         rows * cols * pges, _stream())
+    return out27
+
+
+def feval_aux_calculus_toolbox(out30, A, ftool14, rows, cols, pges, dx, dy, dz, ds):
+    """GPUAsetCalculusToolBox.m:2-64 on caller buffers: out30 in AUX_METHOD_FIELDS
+    order; ftool14 = Fx..Fzx of F (TOOLBOX_FIELDS[:13]) and F's Heaviside."""
+    if len(out30) != 30 or len(ftool14) != 14:
+        raise VLSError("the auxiliary toolbox writes 30 fields from 14 fields of F")
+    arr = (ctypes.c_void_p * 30)(*[_ptr(t).value for t in out30])
+    farr = (ctypes.c_void_p * 14)(*[_ptr(t).value for t in ftool14])
+    call("vls_aux_calculus_toolbox", arr, _ptr(A), farr, rows, cols, pges, dx, dy, dz, ds, _stream())
+    return out30
 
 
 def feval_set_calculus_toolbox(out18, lsf, hprimal, rows, cols, pges, dx, dy, dz, ds):

@@ -33,7 +33,8 @@ FP32 bulk vs FP64, 200 steps each, with the FP32 run's errors), config4
 (512 vesicles of uneven sizes batched, FP32 bulk vs FP64 batched vs the plain
 per-vesicle FP64 path) and config5 (the 96^3 adjoint gradient against finite
 differences and tangent-mode derivatives), variants (the WENO-RK3 / ENO-RK2
-extensions and WENO-RK3 reinitialization at 96^3).
+extensions and WENO-RK3 reinitialization at 96^3), toolbox (the calculus
+toolboxes of F and of an auxiliary field A at 128^3).
 """
 import argparse
 import json
@@ -62,10 +63,11 @@ def parse():
     ap.add_argument("--batch-steps", type=int, default=10,
                     help="timed steps of the 512-vesicle batch comparison (0 skips it)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 96^3 adjoint gradient check")
-    ap.add_argument("--probe", choices=["extend", "batch", "variants"], default=None,
+    ap.add_argument("--probe", choices=["extend", "batch", "variants", "toolbox"], default=None,
                     help="extend: a few extension stages and a 256 MiB device copy (for rocprofv3 --pmc); "
                          "batch: a few steps of the 512-vesicle FP32 batch (for rocprofv3 --kernel-trace); "
-                         "variants: the WENO-RK3 / ENO-RK2 calls of the variants line (for rocprofv3 --kernel-trace)")
+                         "variants: the WENO-RK3 / ENO-RK2 calls of the variants line (for rocprofv3 --kernel-trace); "
+                         "toolbox: the calculus-toolbox calls of the toolbox line (for rocprofv3 --kernel-trace)")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64",
                     help="field type of the main pipeline line (fp32: FP32 bulk)")
     ap.add_argument("--main-only", action="store_true",
@@ -286,6 +288,62 @@ def variants(with_cpu, iterations=100):
     return out
 
 
+B_ALG_TOOLBOX = 160.0  # set_calculus_toolbox: F and HPrimal in, 18 fields out (FP64)
+B_ALG_AUX = 440.0  # GPUAsetCalculusToolBox: max(A,0) 16 + kernel (15 in, 27 out) 336 + tail (9 in, 2 out) 88
+
+
+def toolbox(with_cpu, n=128, reps=20):
+    """Beside the main line: the calculus toolboxes of a surface F and of an
+    auxiliary field A on it (GPUsetCalculusToolBox.m, GPUAsetCalculusToolBox.m)
+    at 128^3, on caller buffers (no allocation in the timed region): ms per call
+    and the algorithmic bytes per call over that time against HBM peak; the CPU
+    oracle on the same grid."""
+    import numpy as np
+    import torch
+
+    vls = __import__("importlib").import_module("2018_7_vesicle_amd")
+    xv = np.linspace(-1.0, 1.0, n)
+    Z, X, Y = np.meshgrid(xv, xv, xv, indexing="ij")
+    F0 = np.ascontiguousarray(np.sqrt(X ** 2 / 0.7 ** 2 + Y ** 2 / 0.55 ** 2 + Z ** 2 / 0.45 ** 2) - 1.0)
+    A0 = np.ascontiguousarray(0.6 * Z + 0.25 * X * Y - 0.15 * Y + 0.05)
+    dx = float(xv[1] - xv[0])
+    F = torch.from_numpy(F0).cuda()
+    A = torch.from_numpy(A0).cuda()
+    hp = torch.clamp(F, min=0.0).contiguous()
+    ft = [torch.empty_like(F) for _ in range(18)]
+    aux = [torch.empty_like(F) for _ in range(30)]
+    fin = ft[:13] + [ft[16]]
+    args = (n, n, n, dx, dx, dx, dx)
+    calls = (("set_calculus_toolbox", B_ALG_TOOLBOX, lambda: vls.feval_set_calculus_toolbox(ft, F, hp, *args)),
+             ("aux_calculus_toolbox", B_ALG_AUX, lambda: vls.feval_aux_calculus_toolbox(aux, A, fin, *args)))
+    out = {"grid": n, "reps": reps}
+    for name, b, fn in calls:
+        fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        gbs = b * n ** 3 / (ms * 1e-3) / 1e9
+        out[name] = {"gpu_ms": ms, "bytes_per_point": b, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
+    if with_cpu:
+        from oracle import pyoracle as po
+
+        t0 = time.perf_counter()
+        ftc = po.set_calculus_toolbox(F0, np.maximum(F0, 0), dx, dx, dx, dx)
+        t1 = time.perf_counter()
+        po.aux_calculus_toolbox(A0, ftc, dx, dx, dx, dx)
+        t2 = time.perf_counter()
+        out["set_calculus_toolbox"]["cpu_ms"] = (t1 - t0) * 1e3
+        out["aux_calculus_toolbox"]["cpu_ms"] = (t2 - t1) * 1e3
+        out["cpu_threads"] = po.num_threads()
+    return out
+
+
 def config5():
     """Config 5 beside the main line: the 96^3 shape-matching problem (200
     WENO5-RK3 advection steps, reinitialization every 10, velocity from 3x64
@@ -313,6 +371,9 @@ def main():
         return probe_batch(args)
     if args.probe == "variants":
         print(json.dumps({"probe": "variants", **variants(False, iterations=20)}), flush=True)
+        return 0
+    if args.probe == "toolbox":
+        print(json.dumps({"probe": "toolbox", **toolbox(False, reps=5)}), flush=True)
         return 0
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported by this build "
@@ -406,6 +467,7 @@ def main():
         if not args.no_adjoint:
             out["config5"] = config5()
         out["variants"] = variants(not args.no_cpu)
+        out["toolbox"] = toolbox(not args.no_cpu)
         if not args.no_cpu:
             out["cpu_baseline"] = cpu_baseline(cfg)
     print(json.dumps(out), flush=True)

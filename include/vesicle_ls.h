@@ -167,6 +167,27 @@ int vls_set_calculus_toolbox(double *const *out18, const double *lsf, const doub
                              int pges, double dx, double dy, double dz, double ds, int num_ele,
                              vls_stream_t stream);
 
+/* auxi_set_calculus_toolbox: CUDA_Code/4_0_Calculus_ToolBox/auxi_set_calculus_toolbox.cu:9-199,
+ * called by +SD/@SDF3/GPUAsetCalculusToolBox.m:40-54.  lsf is the auxiliary
+ * field A, ahprimal its primal max(A,0); fin13 = {Fx,Fy,Fz,FGradMag,Nx,Ny,Nz,
+ * Fxx,Fyy,Fzz,Fxy,Fyz,Fzx} of F (set_calculus_toolbox's first 13 outputs);
+ * out27 = {Ax,Ay,Az,AGradMag,Cx,Cy,Cz,NormCrossAF,tx,ty,tz,nx,ny,nz,Axx,Ayy,Azz,
+ * Axy,Ayz,Azx,ALaplacian,GeodesicCurvature,NormalCurvature,GeodesicTorsion,
+ * BPerpendicular,AHeaviside,ADiracDelta} (27 device pointers). */
+int vls_auxi_set_calculus_toolbox(double *const *out27, const double *lsf, const double *ahprimal,
+                                  const double *const *fin13, int rows, int cols, int pges, double dx, double dy,
+                                  double dz, double ds, int num_ele, vls_stream_t stream);
+
+/* GPUAsetCalculusToolBox: +SD/@SDF3/GPUAsetCalculusToolBox.m:2-64, the whole
+ * method: AHPrimal = max(A,0) (:4), the kernel above (:40-54), then
+ * AFDiracDelta = C.(grad Heaviside x grad AHeaviside)/NormCrossAF^2 (:57-61) and
+ * ADiracDeltaDn = (Lap AHeaviside - ADiracDelta Lap A)/AGradMag (:63) with
+ * @GD3's periodic central differences (+SD/@GD3/GD3.m:112-117,218-250,344-346).
+ * ftool14 = fin13 above followed by F's Heaviside; out30 = {AHPrimal, out27,
+ * AFDiracDelta, ADiracDeltaDn}. */
+int vls_aux_calculus_toolbox(double *const *out30, const double *A, const double *const *ftool14, int rows,
+                             int cols, int pges, double dx, double dy, double dz, double ds, vls_stream_t stream);
+
 /* ------------------------------------------------------------------------ */
 /* 2. the @SDF3 object: one handle, one call per MATLAB method              */
 /* ------------------------------------------------------------------------ */

@@ -797,6 +797,144 @@ void or_set_calculus_toolbox(double **out, const double *lsf, const double *hp, 
             }
 }
 
+static inline void cross3(double *w, double ux, double uy, double uz, double vx, double vy, double vz)
+{
+    /* cross_product, 4_0_Calculus_ToolBox/shared_utilities.cup:14-19 */
+    w[0] = uy * vz - uz * vy;
+    w[1] = uz * vx - ux * vz;
+    w[2] = ux * vy - uy * vx;
+}
+
+/* Kernel auxi_set_calculus_toolbox: 4_0_Calculus_ToolBox/auxi_set_calculus_toolbox.cu:9-199
+ * (norm = max(|v|,1e-14), shared_utilities.cup:7-11).  A is the auxiliary field
+ * (lsf), ahp its primal; fin[0..12] = Fx,Fy,Fz,FGradMag,Nx,Ny,Nz,Fxx,Fyy,Fzz,Fxy,
+ * Fyz,Fzx of F.  out[0..26] = Ax,Ay,Az,AGradMag,Cx,Cy,Cz,NormCrossAF,tx,ty,tz,
+ * nx,ny,nz,Axx,Ayy,Azz,Axy,Ayz,Azx,ALaplacian,GeodesicCurvature,NormalCurvature,
+ * GeodesicTorsion,BPerpendicular,AHeaviside,ADiracDelta.  pow(g,k) is written as
+ * the product g*...*g. */
+void or_auxi_set_calculus_toolbox(double **out, const double *A, const double *ahp, const double **fin,
+                                  int rows, int cols, int pges, double dx, double dy, double dz, double ds,
+                                  int num_ele)
+{
+    (void)num_ele;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int p = 0; p < pges; ++p)
+        for (int c = 0; c < cols; ++c)
+            for (int r = 0; r < rows; ++r) {
+#define A_(dr, dc, dp) A[at(r + (dr), c + (dc), p + (dp), rows, cols, pges)]
+#define P_(dr, dc, dp) ahp[at(r + (dr), c + (dc), p + (dp), rows, cols, pges)]
+                idx_t ind = at(r, c, p, rows, cols, pges);
+                double ax = (A_(0, 1, 0) - A_(0, -1, 0)) / (2 * dx);
+                double ay = (A_(1, 0, 0) - A_(-1, 0, 0)) / (2 * dy);
+                double az = (A_(0, 0, 1) - A_(0, 0, -1)) / (2 * dz);
+                double ag = dmax2(sqrt(ax * ax + ay * ay + az * az), 1e-14);
+                out[0][ind] = ax; out[1][ind] = ay; out[2][ind] = az; out[3][ind] = ag;
+                double fx = fin[0][ind], fy = fin[1][ind], fz = fin[2][ind], fg = fin[3][ind];
+                double C[3];
+                cross3(C, fx, fy, fz, ax, ay, az);
+                double nc = dmax2(sqrt(C[0] * C[0] + C[1] * C[1] + C[2] * C[2]), 1e-14);
+                out[4][ind] = C[0]; out[5][ind] = C[1]; out[6][ind] = C[2]; out[7][ind] = nc;
+                double t[3] = {C[0] / nc, C[1] / nc, C[2] / nc};
+                out[8][ind] = t[0]; out[9][ind] = t[1]; out[10][ind] = t[2];
+                double n[3];
+                cross3(n, t[0], t[1], t[2], fin[4][ind], fin[5][ind], fin[6][ind]);
+                out[11][ind] = n[0]; out[12][ind] = n[1]; out[13][ind] = n[2];
+                double a0 = A_(0, 0, 0);
+                double axx = (A_(0, 1, 0) - 2 * a0 + A_(0, -1, 0)) / (dx * dx);
+                double ayy = (A_(1, 0, 0) - 2 * a0 + A_(-1, 0, 0)) / (dy * dy);
+                double azz = (A_(0, 0, 1) - 2 * a0 + A_(0, 0, -1)) / (dz * dz);
+                double axy = (A_(1, 1, 0) + A_(-1, -1, 0) - A_(1, -1, 0) - A_(-1, 1, 0)) / (4 * ds * ds);
+                double ayz = (A_(1, 0, 1) + A_(-1, 0, -1) - A_(1, 0, -1) - A_(-1, 0, 1)) / (4 * ds * ds);
+                double azx = (A_(0, 1, 1) + A_(0, -1, -1) - A_(0, 1, -1) - A_(0, -1, 1)) / (4 * ds * ds);
+                double alap = axx + ayy + azz;
+                out[14][ind] = axx; out[15][ind] = ayy; out[16][ind] = azz;
+                out[17][ind] = axy; out[18][ind] = ayz; out[19][ind] = azx; out[20][ind] = alap;
+                double fxx = fin[7][ind], fyy = fin[8][ind], fzz = fin[9][ind];
+                double fxy = fin[10][ind], fyz = fin[11][ind], fzx = fin[12][ind];
+                /* geodesic curvature, :115-131: n.((H_F t) x grad A + grad F x (H_A t))/|C| */
+                double w1[3], w2[3];
+                cross3(w1, t[0] * fxx + t[1] * fxy + t[2] * fzx, t[0] * fxy + t[1] * fyy + t[2] * fyz,
+                       t[0] * fzx + t[1] * fyz + t[2] * fzz, ax, ay, az);
+                cross3(w2, fx, fy, fz, t[0] * axx + t[1] * axy + t[2] * azx, t[0] * axy + t[1] * ayy + t[2] * ayz,
+                       t[0] * azx + t[1] * ayz + t[2] * azz);
+                out[21][ind] = (n[0] * (w1[0] + w2[0]) + n[1] * (w1[1] + w2[1]) + n[2] * (w1[2] + w2[2])) / nc;
+                /* grad N = Hess F/|grad F| - grad F (Hess F grad F)^T/|grad F|^3, :135-145 */
+                double g3 = fg * fg * fg;
+                double H[3][3] = {{fxx, fxy, fzx}, {fxy, fyy, fyz}, {fzx, fyz, fzz}};
+                double gF[3] = {fx, fy, fz};
+                double Nm[3][3];
+                for (int j = 0; j < 3; ++j) {
+                    double sj = H[0][j] * fx + H[1][j] * fy + H[2][j] * fz;
+                    for (int i = 0; i < 3; ++i) Nm[i][j] = H[i][j] / fg - gF[i] * sj / g3;
+                }
+                double v[3];
+                for (int i = 0; i < 3; ++i) v[i] = Nm[i][0] * t[0] + Nm[i][1] * t[1] + Nm[i][2] * t[2];
+                out[22][ind] = -(t[0] * v[0] + t[1] * v[1] + t[2] * v[2]);
+                for (int i = 0; i < 3; ++i) v[i] = Nm[i][0] * n[0] + Nm[i][1] * n[1] + Nm[i][2] * n[2];
+                out[23][ind] = -(t[0] * v[0] + t[1] * v[1] + t[2] * v[2]);
+                out[24][ind] = -(n[0] * v[0] + n[1] * v[1] + n[2] * v[2]);
+                /* Heaviside and delta of A from its primal, :166-197 */
+                double px = (P_(0, 1, 0) - P_(0, -1, 0)) / (2 * dx);
+                double py = (P_(1, 0, 0) - P_(-1, 0, 0)) / (2 * dy);
+                double pz = (P_(0, 0, 1) - P_(0, 0, -1)) / (2 * dz);
+                double dot = px * ax + py * ay + pz * az;
+                out[25][ind] = dot / (ag * ag);
+                double p0 = P_(0, 0, 0);
+                double pxx = (P_(0, 1, 0) - 2 * p0 + P_(0, -1, 0)) / (dx * dx);
+                double pyy = (P_(1, 0, 0) - 2 * p0 + P_(-1, 0, 0)) / (dy * dy);
+                double pzz = (P_(0, 0, 1) - 2 * p0 + P_(0, 0, -1)) / (dz * dz);
+                out[26][ind] = (pxx + pyy + pzz) / (ag * ag) - dot * alap / (ag * ag * ag * ag);
+#undef A_
+#undef P_
+            }
+}
+
+static inline idx_t wrap_at(int r, int c, int p, int rows, int cols, int pges)
+{
+    r = (r + rows) % rows;
+    c = (c + cols) % cols;
+    p = (p + pges) % pges;
+    return (idx_t)p * rows * cols + (idx_t)c * rows + r;
+}
+
+/* +SD/@SDF3/GPUAsetCalculusToolBox.m:2-64 as a whole: AHPrimal = max(A,0) (:4),
+ * the kernel above (:40-54), AFDiracDelta (:57-61) and ADiracDeltaDn (:63) with
+ * @GD3's Gradient/Laplacian, whose neighbours wrap around the box (circshift
+ * index maps, +SD/@GD3/GD3.m:112-117; Fx :218-220, Fxx :248-250, Laplacian
+ * :344-346; CrossProduct/DotProduct :352-360).  ftool[0..12] as fin above,
+ * ftool[13] = Heaviside of F; out[0] = AHPrimal, out[1..27] the kernel's,
+ * out[28] = AFDiracDelta, out[29] = ADiracDeltaDn. */
+void or_aux_calculus_toolbox(double **out, const double *A, const double **ftool, int rows, int cols,
+                             int pges, double dx, double dy, double dz, double ds)
+{
+    idx_t n = (idx_t)rows * cols * pges;
+    for (idx_t i = 0; i < n; ++i) out[0][i] = A[i] > 0 ? A[i] : 0.0; /* MATLAB max ignores NaN */
+    or_auxi_set_calculus_toolbox(out + 1, A, out[0], ftool, rows, cols, pges, dx, dy, dz, ds, (int)n);
+    const double *AH = out[26], *Hf = ftool[13], *AG = out[4], *NC = out[8], *ADD = out[27];
+    const double *Cx = out[5], *Cy = out[6], *Cz = out[7];
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int p = 0; p < pges; ++p)
+        for (int c = 0; c < cols; ++c)
+            for (int r = 0; r < rows; ++r) {
+                idx_t ind = (idx_t)p * rows * cols + (idx_t)c * rows + r;
+                idx_t xp = wrap_at(r, c + 1, p, rows, cols, pges), xm = wrap_at(r, c - 1, p, rows, cols, pges);
+                idx_t yp = wrap_at(r + 1, c, p, rows, cols, pges), ym = wrap_at(r - 1, c, p, rows, cols, pges);
+                idx_t zp = wrap_at(r, c, p + 1, rows, cols, pges), zm = wrap_at(r, c, p - 1, rows, cols, pges);
+                double h1[3] = {(AH[xp] - AH[xm]) / (2 * dx), (AH[yp] - AH[ym]) / (2 * dy),
+                                (AH[zp] - AH[zm]) / (2 * dz)};
+                double h2[3] = {(Hf[xp] - Hf[xm]) / (2 * dx), (Hf[yp] - Hf[ym]) / (2 * dy),
+                                (Hf[zp] - Hf[zm]) / (2 * dz)};
+                double ch[3];
+                cross3(ch, h2[0], h2[1], h2[2], h1[0], h1[1], h1[2]);
+                out[28][ind] = (Cx[ind] * ch[0] + Cy[ind] * ch[1] + Cz[ind] * ch[2]) / (NC[ind] * NC[ind]);
+                double lah = (AH[xp] - 2 * AH[ind] + AH[xm]) / (dx * dx) + (AH[yp] - 2 * AH[ind] + AH[ym]) / (dy * dy)
+                             + (AH[zp] - 2 * AH[ind] + AH[zm]) / (dz * dz);
+                double la = (A[xp] - 2 * A[ind] + A[xm]) / (dx * dx) + (A[yp] - 2 * A[ind] + A[ym]) / (dy * dy)
+                            + (A[zp] - 2 * A[ind] + A[zm]) / (dz * dz);
+                out[29][ind] = (lah - ADD[ind] * la) / AG[ind];
+            }
+}
+
 /* ------------------------------------------------------------------------- */
 /* reductions                                                                 */
 /* ------------------------------------------------------------------------- */

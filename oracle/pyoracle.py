@@ -154,6 +154,37 @@ def set_calculus_toolbox(lsf, hprimal, dx, dy, dz, ds):
     return dict(zip(TOOLBOX_FIELDS, out))
 
 
+AUX_TOOLBOX_FIELDS = ["Ax", "Ay", "Az", "AGradMag", "Cx", "Cy", "Cz", "NormCrossAF", "tx", "ty", "tz", "nx", "ny",
+                      "nz", "Axx", "Ayy", "Azz", "Axy", "Ayz", "Azx", "ALaplacian", "GeodesicCurvature",
+                      "NormalCurvature", "GeodesicTorsion", "BPerpendicular", "AHeaviside", "ADiracDelta"]
+
+
+def auxi_set_calculus_toolbox(A, ahprimal, fin13, dx, dy, dz, ds):
+    """or_auxi_set_calculus_toolbox (auxi_set_calculus_toolbox.cu:9-199); fin13 =
+    Fx..Fzx of F in TOOLBOX_FIELDS order.  Returns a dict of the 27 fields."""
+    rows, cols, pges = _dims(A)
+    out = [np.zeros_like(A) for _ in range(27)]
+    fin = [np.ascontiguousarray(a, dtype=np.float64) for a in fin13]
+    arr = (_P * 27)(*[a.ctypes.data for a in out])
+    farr = (_P * 13)(*[a.ctypes.data for a in fin])
+    lib().or_auxi_set_calculus_toolbox(arr, _p(A), _p(ahprimal), farr, _I(rows), _I(cols), _I(pges), _D(dx),
+                                       _D(dy), _D(dz), _D(ds), _I(A.size))
+    return dict(zip(AUX_TOOLBOX_FIELDS, out))
+
+
+def aux_calculus_toolbox(A, ftool, dx, dy, dz, ds):
+    """or_aux_calculus_toolbox (+SD/@SDF3/GPUAsetCalculusToolBox.m:2-64); ftool is
+    set_calculus_toolbox's dict of F.  Returns AHPrimal, the 27 fields,
+    AFDiracDelta and ADiracDeltaDn by name."""
+    rows, cols, pges = _dims(A)
+    out = [np.zeros_like(A) for _ in range(30)]
+    fin = [np.ascontiguousarray(ftool[k], dtype=np.float64) for k in TOOLBOX_FIELDS[:13] + ["Heaviside"]]
+    arr = (_P * 30)(*[a.ctypes.data for a in out])
+    farr = (_P * 14)(*[a.ctypes.data for a in fin])
+    lib().or_aux_calculus_toolbox(arr, _p(A), farr, _I(rows), _I(cols), _I(pges), _D(dx), _D(dy), _D(dz), _D(ds))
+    return dict(zip(["AHPrimal"] + AUX_TOOLBOX_FIELDS + ["AFDiracDelta", "ADiracDeltaDn"], out))
+
+
 def weno_onesided(v1, v2, v3, v4, v5):
     f = lib().or_weno_onesided
     f.restype = _D

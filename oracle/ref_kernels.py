@@ -160,6 +160,12 @@ class RefKernels:
                                             ("d", dx), ("d", dy), ("d", dz), ("d", ds), ("i", rows * cols * pges)]
         self.launch("set_calculus_toolbox.co", "set_calculus_toolbox", args, rows, cols, pges)
 
+    def auxi_set_calculus_toolbox(self, out27, lsf, ahprimal, fin13, rows, cols, pges, dx, dy, dz, ds):
+        args = [("p", t) for t in out27] + [("p", lsf), ("p", ahprimal)] + [("p", t) for t in fin13] + [
+            ("i", rows), ("i", cols), ("i", pges), ("d", dx), ("d", dy), ("d", dz), ("d", ds),
+            ("i", rows * cols * pges)]
+        self.launch("auxi_set_calculus_toolbox.co", "auxi_set_calculus_toolbox", args, rows, cols, pges)
+
     # ---- the WENO-RK3 / ENO-RK2 variants (GPUInitialize.m:32-42, 64-102) ------
     def cubic_boundary_correction(self, d6, lsf, rows, cols, pges, dx, dy, dz):
         args = [("p", t) for t in d6] + [("p", lsf), ("i", rows * cols * pges), ("i", rows), ("i", cols),

@@ -180,6 +180,27 @@ def test_set_calculus_toolbox(case):
         exact(N(t), ref[name], name)
 
 
+def _aux_field(g):
+    return np.ascontiguousarray(0.6 * g.Z + 0.25 * g.X * g.Y - 0.15 * g.Y + 0.05)
+
+
+def test_auxi_set_calculus_toolbox(case):
+    """auxi_set_calculus_toolbox.cu: the 27 fields of an auxiliary field A on the
+    surface of F, from the toolbox of F"""
+    g, F = case["g"], case["F"]
+    dx, dy, dz = g.spacing
+    rows, cols, pges = _dims(g)
+    A = _aux_field(g)
+    ft = po.set_calculus_toolbox(F, np.maximum(F, 0), dx, dy, dz, g.ds)
+    fin = [ft[k] for k in po.TOOLBOX_FIELDS[:13]]
+    ref = po.auxi_set_calculus_toolbox(A, np.maximum(A, 0), fin, dx, dy, dz, g.ds)
+    out = [torch.zeros_like(T(F)) for _ in range(27)]
+    vls.feval_auxi_set_calculus_toolbox(out, T(A), T(np.maximum(A, 0)), [T(a) for a in fin], rows, cols, pges, dx,
+                                        dy, dz, g.ds)
+    for name, t in zip(po.AUX_TOOLBOX_FIELDS, out):
+        exact(N(t), ref[name], name)
+
+
 def test_variant_distances_and_re_step(case):
     """WENO-RK3 variant: cubic distances (cubic boundary_correction) and its re_step"""
     g, F = case["g"], case["F"]

@@ -173,3 +173,20 @@ def test_advection_toolbox(ref, case):
     vls.feval_set_calculus_toolbox(t_got, T(F), hp, r, c, p, dx, dy, dz, g.ds)
     for name, a, b in zip(vls.TOOLBOX_FIELDS, t_got, t_ref):
         close(N(a), N(b), rtol=1e-11, what=f"{name} vs reference")
+
+
+def test_aux_toolbox(ref, case):
+    """auxi_set_calculus_toolbox.cu, fed the reference's own toolbox of F"""
+    g, F = case
+    dx, dy, dz = g.spacing
+    r, c, p = g.rows, g.cols, g.pges
+    A = np.ascontiguousarray(0.6 * g.Z + 0.25 * g.X * g.Y - 0.15 * g.Y + 0.05)
+    ft = [torch.zeros_like(T(F)) for _ in range(18)]
+    ref.set_calculus_toolbox(ft, T(F), T(np.maximum(F, 0)), r, c, p, dx, dy, dz, g.ds)
+    ahp = T(np.maximum(A, 0))
+    a_ref = [torch.zeros_like(T(F)) for _ in range(27)]
+    ref.auxi_set_calculus_toolbox(a_ref, T(A), ahp, ft[:13], r, c, p, dx, dy, dz, g.ds)
+    a_got = [torch.zeros_like(T(F)) for _ in range(27)]
+    vls.feval_auxi_set_calculus_toolbox(a_got, T(A), ahp, ft[:13], r, c, p, dx, dy, dz, g.ds)
+    for name, a, b in zip(vls.AUX_TOOLBOX_FIELDS, a_got, a_ref):
+        close(N(a), N(b), rtol=1e-11, what=f"{name} vs reference")

@@ -169,6 +169,26 @@ def test_calculus_toolbox_object(setup):
         exact(N(getattr(m, name)), ref[name], name)
 
 
+def test_aux_calculus_toolbox_object(setup):
+    """GPUAsetCalculusToolBox.m as a whole: AHPrimal, the kernel's 27 fields and
+    the periodic-difference tail (AFDiracDelta, ADiracDeltaDn)"""
+    g, F, G = setup
+    m = vls.SDF3(G, T(F))
+    with pytest.raises(vls.VLSError):
+        m.GPUAsetCalculusToolBox()
+    A = np.ascontiguousarray(0.6 * g.Z + 0.25 * g.X * g.Y - 0.15 * g.Y + 0.05)
+    m.A = T(A)
+    with pytest.raises(vls.VLSError):
+        m.GPUAsetCalculusToolBox()
+    m.GPUsetCalculusToolBox()
+    m.GPUAsetCalculusToolBox()
+    ref = po.aux_calculus_toolbox(A, po.set_calculus_toolbox(F, np.maximum(F, 0), *g.spacing, g.ds), *g.spacing,
+                                  g.ds)
+    assert len(ref) == 30
+    for name, want in ref.items():
+        exact(N(getattr(m, name)), want, name)
+
+
 def test_pipeline_small():
     """two vesicle steps of the benchmark pipeline (one with reinit and surface
     redistance) at 32^3 against the oracle's restatement of the same pipeline"""

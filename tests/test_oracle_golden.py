@@ -174,3 +174,31 @@ def test_variant_extension_kernels(vrec, scheme, dkey, tkey):
     st = po.variant_extend_step(vrec[tkey], vrec["C"], list(vrec[f"{scheme}_v3"]), d6, list(vrec[f"{scheme}_cp6"]),
                                 dx, dy, dz, scheme)
     _close(st, vrec[f"{scheme}_extend_step"])
+
+
+# --------------------------------------------------------------------------
+# auxiliary-field toolbox: vectors written by the reference's
+# set_calculus_toolbox.cu and auxi_set_calculus_toolbox.cu
+# (tests/golden/make_golden_aux.py); the oracle is fed the reference's own
+# toolbox of F
+# --------------------------------------------------------------------------
+AFILES = sorted(glob.glob(os.path.join(HERE, "golden", "aux_*.npz")))
+
+
+def test_aux_fixture_count():
+    assert len(AFILES) == 3
+
+
+@pytest.mark.parametrize("path", AFILES, ids=[os.path.basename(f)[4:-4] for f in AFILES])
+def test_auxi_set_calculus_toolbox(path):
+    arec = _load(path)
+    dx, dy, dz = _spacing(arec)
+    ds = (dx + dy + dz) / 3
+    A = arec["A"]
+    got = po.auxi_set_calculus_toolbox(A, np.maximum(A, 0), list(arec["ftool"][:13]), dx, dy, dz, ds)
+    for k, name in enumerate(po.AUX_TOOLBOX_FIELDS):
+        _close(got[name], arec["aux"][k], rtol=1e-11)
+    # and the oracle's own toolbox of F is the reference's
+    ft = po.set_calculus_toolbox(arec["F"], np.maximum(arec["F"], 0), dx, dy, dz, ds)
+    for k, name in enumerate(po.TOOLBOX_FIELDS):
+        _close(ft[name], arec["ftool"][k], rtol=1e-11)

@@ -486,3 +486,156 @@ def test_wenork3_reinit_keeps_planes_and_converges_on_spheres():
         band = np.abs(r - 0.6) < 2 * gs.ds
         errs.append(np.abs(Fs - (r - 0.6))[band].max())
     assert errs[0] < 5e-3 and errs[1] < errs[0], errs
+
+
+# --------------------------------------------------------------------------
+# auxiliary-field geometry (4_0_Calculus_ToolBox/auxi_set_calculus_toolbox.cu,
+# +SD/@SDF3/GPUAsetCalculusToolBox.m)
+# --------------------------------------------------------------------------
+_CF = np.array([0.07, -0.04, 0.11])
+_HA = np.array([[0.6, -0.25, 0.0], [-0.25, 0.4, 0.15], [0.0, 0.15, 0.2]])
+_BA = np.array([0.1, -0.2, 1.0])
+
+
+def _quadratic_pair(g):
+    """F = |x-c|^2/2 - 0.2 and A = x.HA.x/2 + b.x: central differences of
+    quadratics are exact (equal spacings make the cross terms exact too)"""
+    P = np.stack([g.X, g.Y, g.Z], axis=-1)
+    F = 0.5 * ((P - _CF) ** 2).sum(-1) - 0.2
+    A = 0.5 * np.einsum("...i,ij,...j->...", P, _HA, P) + P @ _BA
+    return P, np.ascontiguousarray(F), np.ascontiguousarray(A)
+
+
+def _unit(v):
+    return v / np.linalg.norm(v, axis=-1, keepdims=True)
+
+
+def _frame(P):
+    gF = P - _CF
+    gA = P @ _HA + _BA
+    N = _unit(gF)
+    t = _unit(np.cross(gF, gA))
+    return N, t, np.cross(t, N)
+
+
+def test_aux_toolbox_frame_and_curvatures_match_the_analytic_curve():
+    # t = grad F x grad A/|.|, n = t x N; geodesic curvature n.dt/ds, normal
+    # curvature -t.dN/ds, geodesic torsion -t.D_n N and -n.D_n N, all by
+    # central differences of the analytic frame along the curve (eps = 1e-5)
+    g = cases.linspace_grid(20)
+    P, F, A = _quadratic_pair(g)
+    ft = po.set_calculus_toolbox(F, np.maximum(F, 0), *g.spacing, g.ds)
+    fin = [ft[k] for k in po.TOOLBOX_FIELDS[:13]]
+    out = po.auxi_set_calculus_toolbox(A, np.maximum(A, 0), fin, *g.spacing, g.ds)
+    inner = (slice(1, -1),) * 3
+    gA = P @ _HA + _BA
+    C = np.cross(P - _CF, gA)
+    sel = np.zeros(F.shape, bool)
+    sel[inner] = True
+    sel &= np.linalg.norm(C, axis=-1) > 0.05
+    assert sel.sum() > 2000
+    for k, name in enumerate(("Ax", "Ay", "Az")):
+        assert np.abs(out[name] - gA[..., k])[sel].max() < 1e-12
+    for name, v in zip(("Axx", "Ayy", "Azz", "Axy", "Ayz", "Azx"),
+                       (_HA[0, 0], _HA[1, 1], _HA[2, 2], _HA[0, 1], _HA[1, 2], _HA[2, 0])):
+        assert np.abs(out[name] - v)[sel].max() < 1e-10, name
+    assert np.abs(out["ALaplacian"] - np.trace(_HA))[sel].max() < 1e-10
+    for k, name in enumerate(("Cx", "Cy", "Cz")):
+        assert np.abs(out[name] - C[..., k])[sel].max() < 1e-12
+    assert np.abs(out["NormCrossAF"] - np.linalg.norm(C, axis=-1))[sel].max() < 1e-12
+    N, t, n = _frame(P)
+    for k in range(3):
+        assert np.abs(out["t" + "xyz"[k]] - t[..., k])[sel].max() < 1e-12
+        assert np.abs(out["n" + "xyz"[k]] - n[..., k])[sel].max() < 1e-12
+    eps = 1e-5
+
+    def along(fn, d):
+        return (fn(P + eps * d) - fn(P - eps * d)) / (2 * eps)
+
+    dt = along(lambda Q: _frame(Q)[1], t)
+    dNt = along(lambda Q: _frame(Q)[0], t)
+    dNn = along(lambda Q: _frame(Q)[0], n)
+    checks = {
+        "GeodesicCurvature": (n * dt).sum(-1),
+        "NormalCurvature": -(t * dNt).sum(-1),
+        "GeodesicTorsion": -(t * dNn).sum(-1),
+        "BPerpendicular": -(n * dNn).sum(-1),
+    }
+    for name, want in checks.items():
+        err = np.abs(out[name] - want)[sel].max()
+        assert err < 1e-7 * max(1.0, np.abs(want[sel]).max()), (name, err)
+    # on the sphere-like F: normal curvature -1/|x-c| for every tangent direction
+    assert np.abs(out["NormalCurvature"] + 1 / np.linalg.norm(P - _CF, axis=-1))[sel].max() < 1e-9
+
+
+def test_aux_heaviside_and_delta_away_from_the_aux_surface():
+    # Towers H/delta of A from max(A,0): 1/0 where the 7-point stencil is
+    # on the positive side, exactly 0/0 on the negative side
+    g = cases.linspace_grid(16)
+    A = np.ascontiguousarray(0.8 * g.X - 0.3 * g.Y + 0.5 * g.Z + 0.05)
+    Fq = np.ascontiguousarray(g.X ** 2 + g.Y ** 2 + g.Z ** 2 - 0.5)
+    ft = po.set_calculus_toolbox(Fq, np.maximum(Fq, 0), *g.spacing, g.ds)
+    out = po.auxi_set_calculus_toolbox(A, np.maximum(A, 0), [ft[k] for k in po.TOOLBOX_FIELDS[:13]], *g.spacing,
+                                       g.ds)
+    h = 1.01 * max(0.8 * g.dx, 0.3 * g.dy, 0.5 * g.dz) * 3
+    inner = np.zeros(A.shape, bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    pos, neg = inner & (A > h), inner & (A < -h)
+    assert pos.sum() > 300 and neg.sum() > 300
+    assert np.abs(out["AHeaviside"][pos] - 1).max() < 1e-13
+    assert np.abs(out["ADiracDelta"][pos]).max() < 1e-12
+    assert np.all(out["AHeaviside"][neg] == 0) and np.all(out["ADiracDelta"][neg] == 0)
+
+
+def _roll(f, shift, axis):
+    # circshift on the MATLAB dims (rows=y, cols=x, pges=z) of a (pges, cols, rows) array
+    return np.roll(f, shift, axis={"x": 1, "y": 2, "z": 0}[axis])
+
+
+def test_aux_toolbox_tail_is_gd3_periodic_differencing():
+    # AFDiracDelta / ADiracDeltaDn of GPUAsetCalculusToolBox.m:57-63 written
+    # with np.roll as @GD3's circshift maps (GD3.m:112-117): Field(oXo) is the
+    # right neighbour, wrapping at the box face
+    g = cases.golden_grids()["star"][0]
+    F = cases.star_phi0(g)
+    A = np.ascontiguousarray(0.7 * g.Z + 0.2 * g.X * g.Y - 0.1)
+    ft = po.set_calculus_toolbox(F, np.maximum(F, 0), *g.spacing, g.ds)
+    out = po.aux_calculus_toolbox(A, ft, *g.spacing, g.ds)
+    assert np.array_equal(out["AHPrimal"], np.maximum(A, 0))
+    dx, dy, dz = g.spacing
+
+    def grad(f):
+        return [(_roll(f, -1, "x") - _roll(f, 1, "x")) / (2 * dx), (_roll(f, -1, "y") - _roll(f, 1, "y")) / (2 * dy),
+                (_roll(f, -1, "z") - _roll(f, 1, "z")) / (2 * dz)]
+
+    def lap(f):
+        return ((_roll(f, -1, "x") - 2 * f + _roll(f, 1, "x")) / (dx * dx)
+                + (_roll(f, -1, "y") - 2 * f + _roll(f, 1, "y")) / (dy * dy)
+                + (_roll(f, -1, "z") - 2 * f + _roll(f, 1, "z")) / (dz * dz))
+
+    h1 = grad(out["AHeaviside"])
+    h2 = grad(ft["Heaviside"])
+    ch = [h2[1] * h1[2] - h2[2] * h1[1], h2[2] * h1[0] - h2[0] * h1[2], h2[0] * h1[1] - h2[1] * h1[0]]
+    afdd = (out["Cx"] * ch[0] + out["Cy"] * ch[1] + out["Cz"] * ch[2]) / out["NormCrossAF"] ** 2
+    addn = (lap(out["AHeaviside"]) - out["ADiracDelta"] * lap(A)) / out["AGradMag"]
+    assert np.array_equal(out["AFDiracDelta"], afdd)
+    assert np.array_equal(out["ADiracDeltaDn"], addn)
+    # the wrap is what makes the faces differ from a clamped stencil
+    assert np.abs(lap(A) - np.pad(lap(A)[1:-1, 1:-1, 1:-1], 1, mode="edge")).max() > 1.0
+
+
+def test_aux_dirac_delta_integrates_to_the_curve_length():
+    # AFDiracDelta ~ delta(F) delta(A): with F = |x| - R and A = z the
+    # integral is the length 2 pi R of the equator (|grad F x grad A| = 1 there)
+    R = 0.55
+    lens = []
+    for n in (40, 56):
+        g = cases.linspace_grid(n)
+        F = np.ascontiguousarray(np.sqrt(g.X ** 2 + g.Y ** 2 + g.Z ** 2) - R)
+        A = np.ascontiguousarray(g.Z + 0.013)
+        ft = po.set_calculus_toolbox(F, np.maximum(F, 0), *g.spacing, g.ds)
+        out = po.aux_calculus_toolbox(A, ft, *g.spacing, g.ds)
+        lens.append(out["AFDiracDelta"].sum() * g.dx * g.dy * g.dz)
+    want = 2 * np.pi * np.sqrt(R ** 2 - 0.013 ** 2)
+    errs = [abs(v - want) / want for v in lens]
+    assert errs[0] < 3e-3 and errs[1] < errs[0], (lens, want)
