@@ -6,7 +6,7 @@ name starts with a digit, so import it with
 importlib.import_module("2018_7_vesicle_amd").
 """
 from ._lib import LIB_PATH, VLSError, exported_names, lib  # noqa: F401
-from .sdf3 import (AUX_METHOD_FIELDS, AUX_TOOLBOX_FIELDS, GD3, SDF3, TOOLBOX_FIELDS, feval_advection_step, feval_boundary_correction,  # noqa: F401
+from .sdf3 import (AUX_METHOD_FIELDS, AUX_TOOLBOX_FIELDS, GD3, TENSION_PRESSURE_FIELDS, SDF3, TOOLBOX_FIELDS, feval_advection_step, feval_boundary_correction,  # noqa: F401
                    feval_boundary_interpolant, feval_boundary_location, feval_cubic_boundary_correction,
                    feval_eno2_re_step, feval_enork2_boundary_interpolate, feval_enork2_extend_step,
                    feval_enork2_upwind_normal, feval_extend_step, feval_re_step, feval_set_calculus_toolbox,

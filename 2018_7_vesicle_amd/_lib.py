@@ -34,6 +34,11 @@ SIGNATURES = {
     "vls_set_calculus_toolbox": [P] * 3 + [I] * 3 + [D] * 4 + [I, P],
     "vls_auxi_set_calculus_toolbox": [P] * 4 + [I] * 3 + [D] * 4 + [I, P],
     "vls_aux_calculus_toolbox": [P] * 3 + [I] * 3 + [D] * 4 + [P],
+    "vls_smooth_fft": [P] * 2 + [I] * 3 + [D] * 5 + [I, P],
+    "vls_smooth_gmres": [P] * 2 + [I] * 3 + [D] * 5 + [I, P],
+    "vls_smooth_operator": [P] * 2 + [I] * 3 + [D] * 5 + [I, P],
+    "vls_bending_speed": [P] * 3 + [I] * 3 + [D] * 5 + [P],
+    "vls_sdf3_tension_pressure": [P] * 7 + [D] * 4 + [P] * 3,
     "vls_cubic_boundary_correction": [P] * 7 + [I] * 4 + [D] * 3 + [P],
     "vls_wenork3_re_step": [P] * 10 + [I] * 3 + [D] * 3 + [I, P],
     "vls_enork2_upwind_normal": [P] * 10 + [I] * 3 + [D] * 3 + [I, P],

@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB_NAME = "libvesicle_ls.so"
 LIB_PATH = os.path.join(HERE, LIB_NAME)
-SOURCES = ["vls_kernels.hip", "vls_batch.hip", "vls_adjoint.hip", "vls_variants.hip", "vls_api.hip"]
+SOURCES = ["vls_kernels.hip", "vls_batch.hip", "vls_adjoint.hip", "vls_variants.hip", "vls_relax.hip", "vls_api.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
@@ -41,7 +41,8 @@ def build(verbose=False):
         subprocess.run(cmd, check=True)
         objs.append(obj)
     tmp = LIB_PATH + ".tmp"
-    cmd = [HIPCC, "-shared", "-o", tmp] + objs + ["--offload-arch=gfx950", "-fPIC", "-Wl,-rpath,/opt/rocm/lib"]
+    cmd = [HIPCC, "-shared", "-o", tmp] + objs + ["--offload-arch=gfx950", "-fPIC", "-Wl,-rpath,/opt/rocm/lib",
+                                                  "-L/opt/rocm/lib", "-lrocfft"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -3,6 +3,7 @@
 // caller's stream; nothing here synchronizes except the calls that return a
 // host scalar.
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -826,6 +827,56 @@ int vls_aux_calculus_toolbox(double* const* out30, const double* A, const double
     return VLS_OK;
 }
 
+static int smooth_entry(double* out, const double* rhs, int rows, int cols, int pges, double dx, double dy,
+                        double dz, double dt, double alpha, int diffusion, int exact, vls_stream_t stream)
+{
+    int rc = check_geom(rows, cols, pges, dx, dy, dz);
+    if (rc) return rc;
+    VLS_REQUIRE(!any_null(out, rhs), "null pointer");
+    VLS_REQUIRE(diffusion == 0 || diffusion == 1, "diffusion must be 0 (bilaplacian) or 1 (laplacian)");
+    VLS_REQUIRE(std::isfinite(dt) && std::isfinite(alpha), "dt and alpha must be finite");
+    const Geom g{rows, cols, pges, dx, dy, dz};
+    VLS_HIP(launch_smooth(out, rhs, g, alpha * dt, diffusion, exact, S(stream)));
+    return VLS_OK;
+}
+
+int vls_smooth_fft(double* out, const double* rhs, int rows, int cols, int pges, double dx, double dy, double dz,
+                   double dt, double alpha, int diffusion, vls_stream_t stream)
+{
+    return smooth_entry(out, rhs, rows, cols, pges, dx, dy, dz, dt, alpha, diffusion, 0, stream);
+}
+
+int vls_smooth_gmres(double* out, const double* rhs, int rows, int cols, int pges, double dx, double dy, double dz,
+                     double dt, double alpha, int diffusion, vls_stream_t stream)
+{
+    return smooth_entry(out, rhs, rows, cols, pges, dx, dy, dz, dt, alpha, diffusion, 1, stream);
+}
+
+int vls_smooth_operator(double* y, const double* x, int rows, int cols, int pges, double dx, double dy, double dz,
+                        double dt, double alpha, int diffusion, vls_stream_t stream)
+{
+    int rc = check_geom(rows, cols, pges, dx, dy, dz);
+    if (rc) return rc;
+    VLS_REQUIRE(!any_null(y, x), "null pointer");
+    VLS_REQUIRE(y != x, "the operator cannot be applied in place");
+    VLS_REQUIRE(diffusion == 0 || diffusion == 1, "diffusion must be 0 (bilaplacian) or 1 (laplacian)");
+    const Geom g{rows, cols, pges, dx, dy, dz};
+    VLS_HIP(launch_smooth_operator(y, x, g, alpha * dt, diffusion, S(stream)));
+    return VLS_OK;
+}
+
+int vls_bending_speed(double* out, const double* H, const double* G, int rows, int cols, int pges, double dx,
+                      double dy, double dz, double kappa, double sponc, vls_stream_t stream)
+{
+    int rc = check_geom(rows, cols, pges, dx, dy, dz);
+    if (rc) return rc;
+    VLS_REQUIRE(!any_null(out, H, G), "null pointer");
+    VLS_REQUIRE(out != H, "the bending speed cannot overwrite H (its Laplacian reads neighbours)");
+    const Geom g{rows, cols, pges, dx, dy, dz};
+    VLS_HIP(launch_bending_speed(out, H, G, g, kappa, sponc, S(stream)));
+    return VLS_OK;
+}
+
 // ---------------------------------------------------------------------------
 // 2. @SDF3 handle
 // ---------------------------------------------------------------------------
@@ -839,7 +890,7 @@ int vls_sdf3_create(vls_sdf3_t* out, int rows, int cols, int pges, double dx, do
     if (!h) return fail(VLS_ERR_ALLOC, "host allocation failed");
     h->g = Geom{rows, cols, pges, dx, dy, dz};
     const int64_t n = h->g.n();
-    const size_t bytes = sizeof(double) * ((size_t)n * kFields + (size_t)reduce_blocks() * 4 + 8);
+    const size_t bytes = sizeof(double) * ((size_t)n * kFields + (size_t)reduce_blocks() * 16 + 16);
     if (hipMalloc((void**)&h->pool, bytes) != hipSuccess) {
         delete h;
         return fail(VLS_ERR_ALLOC, "hipMalloc of %zu bytes failed", bytes);
@@ -875,7 +926,7 @@ int vls_sdf3_create(vls_sdf3_t* out, int rows, int cols, int pges, double dx, do
     h->lvel = p; p += 3 * n;
     for (int a = 0; a < 3; ++a) { h->v[a] = p; p += n; }
     for (int k = 0; k < 4; ++k) { h->t[k] = p; p += n; }
-    h->partial = p; p += (size_t)reduce_blocks() * 4;
+    h->partial = p; p += (size_t)reduce_blocks() * 16;
     h->result = p;
     h->flags = h->flag_pool;
     h->aflags = h->flag_pool + n;
@@ -1117,6 +1168,21 @@ int vls_sdf3_area_volume_towers(vls_sdf3_t h, const double* F, double* area, dou
 {
     VLS_REQUIRE(area && volume, "null argument");
     return area_volume<double>(h, 0, F, 0.0, nullptr, area, volume, S(stream));
+}
+
+int vls_sdf3_tension_pressure(vls_sdf3_t h, const double* F, const double* H, const double* Vb, const double* dirac,
+                              const double* gradmag, const double* heaviside, double area0, double volume0,
+                              double dt, double cfl, double* speed, double* out10, vls_stream_t stream)
+{
+    VLS_REQUIRE(h, "null handle");
+    VLS_REQUIRE(!any_null(F, H, Vb, dirac, gradmag, heaviside, speed, out10), "null pointer");
+    VLS_REQUIRE(std::isfinite(area0) && std::isfinite(volume0) && std::isfinite(dt) && std::isfinite(cfl),
+                "area0, volume0, dt and cfl must be finite");
+    VLS_REQUIRE(dt > 0 || cfl > 0, "give a time step dt > 0 or a CFL number cfl > 0");
+    const double ds = (h->g.dx + h->g.dy + h->g.dz) / 3;
+    VLS_HIP(launch_tension_pressure(out10, h->partial, speed, F, H, Vb, dirac, gradmag, heaviside, h->g, ds, area0,
+                                    volume0, dt, cfl, S(stream)));
+    return VLS_OK;
 }
 
 int vls_sdf3_area_volume_cosine(vls_sdf3_t h, const double* F, double eps, double* area, double* volume,

@@ -166,6 +166,21 @@ hipError_t launch_aux_toolbox_tail(double* afdd, double* addn, const double* ah,
                                    const double* cx, const double* cy, const double* cz, const double* ncross,
                                    const double* add, const double* agrad, const Geom& g, hipStream_t s);
 hipError_t launch_positive_part(double* out, const double* a, int64_t n, hipStream_t s);
+
+// vesicle relaxation step (vls_relax.hip): the @GD3 smoothers through rocFFT
+// (exact = 0: spectral symbol of smoothFFT; 1: the finite-difference system of
+// smoothGMRES solved exactly), c = alpha*dt; the operator itself; the bending
+// speed; surface integrals, Dt and the tension/pressure solve with the normal
+// speed (partial: reduce_blocks()*13 doubles; out10 on the device)
+hipError_t launch_smooth(double* out, const double* rhs, const Geom& g, double c, int diffusion, int exact,
+                         hipStream_t s);
+hipError_t launch_smooth_operator(double* y, const double* x, const Geom& g, double c, int diffusion, hipStream_t s);
+hipError_t launch_bending_speed(double* out, const double* H, const double* G, const Geom& g, double kappa,
+                                double sponc, hipStream_t s);
+hipError_t launch_tension_pressure(double* out10, double* partial, double* speed, const double* F, const double* H,
+                                   const double* Vb, const double* dirac, const double* gmag, const double* heav,
+                                   const Geom& g, double ds, double a0, double v0, double dt, double cfl,
+                                   hipStream_t s);
 hipError_t launch_set_calculus_toolbox(const Fields18& o, const double* lsf, const double* hp, const Geom& g,
                                        double ds, hipStream_t s);
 // reductions: `partial` holds at least reduce_blocks()*4 doubles, `result` 2

@@ -27,6 +27,7 @@ __all__ = ["GD3", "SDF3", "VLSError"] + [
     "feval_upwind_normal", "feval_boundary_interpolant", "feval_extend_step", "feval_advection_step",
     "feval_weno_derivative", "feval_set_calculus_toolbox", "TOOLBOX_FIELDS",
     "feval_auxi_set_calculus_toolbox", "feval_aux_calculus_toolbox", "AUX_TOOLBOX_FIELDS", "AUX_METHOD_FIELDS",
+    "TENSION_PRESSURE_FIELDS",
 ]
 
 TOOLBOX_FIELDS = ["Fx", "Fy", "Fz", "FGradMag", "Nx", "Ny", "Nz", "Fxx", "Fyy", "Fzz", "Fxy", "Fyz", "Fzx",
@@ -36,6 +37,8 @@ TOOLBOX_FIELDS = ["Fx", "Fy", "Fz", "FGradMag", "Nx", "Ny", "Nz", "Fxx", "Fyy", 
 AUX_TOOLBOX_FIELDS = ["Ax", "Ay", "Az", "AGradMag", "Cx", "Cy", "Cz", "NormCrossAF", "tx", "ty", "tz", "nx", "ny",
                       "nz", "Axx", "Ayy", "Azz", "Axy", "Ayz", "Azx", "ALaplacian", "GeodesicCurvature",
                       "NormalCurvature", "GeodesicTorsion", "BPerpendicular", "AHeaviside", "ADiracDelta"]
+# the device scalars of SDF3.tensionPressure, in order
+TENSION_PRESSURE_FIELDS = ["Tension", "Pressure", "Dt", "c11", "c12", "s1", "s2", "area", "volume", "vmax"]
 # what GPUAsetCalculusToolBox.m sets, in vls_aux_calculus_toolbox's out30 order
 AUX_METHOD_FIELDS = ["AHPrimal"] + AUX_TOOLBOX_FIELDS + ["AFDiracDelta", "ADiracDeltaDn"]
 
@@ -86,6 +89,51 @@ class GD3:
 
     def zeros(self, dtype=torch.float64):
         return torch.zeros(self.Size, dtype=dtype, device=self.device)
+
+    # the smoothers of the vesicle step (GD3.m:435-486), FP64, periodic box
+    def _field(self, t):
+        if not isinstance(t, torch.Tensor) or t.dtype != torch.float64 or tuple(t.shape) != self.Size:
+            raise VLSError(f"expected a float64 field of shape {self.Size}")
+        return _ptr(t)
+
+    def _smooth(self, name, rhs, Dt, Alpha, diffusion, out=None):
+        out = torch.empty_like(rhs) if out is None else out
+        call(name, self._field(out), self._field(rhs), self.mrows, self.ncols, self.lshts, self.Dx, self.Dy, self.Dz,
+             float(Dt), float(Alpha), int(diffusion), _stream())
+        return out
+
+    def smoothGMRES(self, rhs, Dt, Alpha, out=None):
+        """(Idt + Alpha*Dt*LBiLaplacian)^-1 rhs (GD3.m:435-451): the solution of the
+        system MATLAB's gmres solves to 1e-12, by the DFT that diagonalizes it."""
+        return self._smooth("vls_smooth_gmres", rhs, Dt, Alpha, 0, out)
+
+    def smoothFFT(self, rhs, Dt, Alpha, out=None):
+        """GD3.m:453-458: divide by the spectral symbol 1 + Alpha*Dt*|k|^4."""
+        return self._smooth("vls_smooth_fft", rhs, Dt, Alpha, 0, out)
+
+    def smoothDiffusionGMRES(self, rhs, Dt, Alpha, out=None):
+        """(Idt - Alpha*Dt*LLaplacian)^-1 rhs (GD3.m:461-477), as smoothGMRES."""
+        return self._smooth("vls_smooth_gmres", rhs, Dt, Alpha, 1, out)
+
+    def smoothDiffusionFFT(self, rhs, Dt, Alpha, out=None):
+        """GD3.m:479-484: divide by 1 + Alpha*Dt*|k|^2."""
+        return self._smooth("vls_smooth_fft", rhs, Dt, Alpha, 1, out)
+
+    def smoothOperator(self, x, Dt, Alpha, diffusion=False, out=None):
+        """Op*x of the smoothGMRES systems: (Idt + Alpha*Dt*LBiLaplacian) x, or
+        (Idt - Alpha*Dt*LLaplacian) x with diffusion (GD3.m:438, 464)."""
+        out = torch.empty_like(x) if out is None else out
+        call("vls_smooth_operator", self._field(out), self._field(x), self.mrows, self.ncols, self.lshts, self.Dx,
+             self.Dy, self.Dz, float(Dt), float(Alpha), int(bool(diffusion)), _stream())
+        return out
+
+    def bendingSpeed(self, H, G, Kappa=1.0, SponC=0.0, out=None):
+        """NormalSpeedBend of p00_test_singlephase_relax.m:79-83 from the mean
+        curvature H and Gaussian curvature G, periodic Laplacian."""
+        out = torch.empty_like(H) if out is None else out
+        call("vls_bending_speed", self._field(out), self._field(H), self._field(G), self.mrows, self.ncols,
+             self.lshts, self.Dx, self.Dy, self.Dz, float(Kappa), float(SponC), _stream())
+        return out
 
 
 class SDF3:
@@ -287,6 +335,27 @@ class SDF3:
         feval_aux_calculus_toolbox(outs, self.A, fin, g.mrows, g.ncols, g.lshts, g.Dx, g.Dy, g.Dz, g.Ds)
         for name, t in zip(AUX_METHOD_FIELDS, outs):
             setattr(self, name, t)
+
+    def tensionPressure(self, H, Vb, InitialArea, InitialVolume, Dt=None, CFLNumber=None, out=None):
+        """The tension/pressure step of p00_test_singlephase_relax.m:67-109 on the
+        toolbox of F (GPUsetCalculusToolBox first): H the extended mean
+        curvature, Vb the extended bending speed.  Dt = CFLNumber*Ds/max|Vb|
+        on |F|<2Ds unless Dt is given.  Returns (normalSpeed, info) with info a
+        device tensor {Tension, Pressure, Dt, c11, c12, s1, s2, area, volume,
+        max|Vb|}; nothing is read back to the host."""
+        if self.dtype != torch.float64:
+            raise VLSError("tensionPressure is FP64 only")
+        if self.DiracDelta is None:
+            raise VLSError("tensionPressure needs GPUsetCalculusToolBox first")
+        if (Dt is None or Dt <= 0) and (CFLNumber is None or CFLNumber <= 0):
+            raise VLSError("give Dt > 0 or CFLNumber > 0")
+        speed = self.GD3.zeros() if out is None else out
+        info = torch.zeros(10, dtype=torch.float64, device=self.F.device)
+        call("vls_sdf3_tension_pressure", self._h, self._check(self.F), self._check(H), self._check(Vb),
+             self._check(self.DiracDelta), self._check(self.FGradMag), self._check(self.Heaviside),
+             float(InitialArea), float(InitialVolume), float(Dt or 0.0), float(CFLNumber or 0.0),
+             self._check(speed), _ptr(info), _stream())
+        return speed, info
 
     def calAreaVolume(self):
         """(calArea, calVolume) of +SD/@SDF3/SDF3.m:73-80 with the Towers delta/H,

@@ -188,6 +188,39 @@ int vls_auxi_set_calculus_toolbox(double *const *out27, const double *lsf, const
 int vls_aux_calculus_toolbox(double *const *out30, const double *A, const double *const *ftool14, int rows,
                              int cols, int pges, double dx, double dy, double dz, double ds, vls_stream_t stream);
 
+/* The @GD3 smoothers of the vesicle step (+SD/@GD3/GD3.m:435-486), applied
+ * to rhs on the periodic box; out may alias rhs.  diffusion = 0: the
+ * bilaplacian smoothers, (Idt + alpha*dt*LBiLaplacian)^-1; diffusion = 1:
+ * (Idt - alpha*dt*LLaplacian)^-1.  rocFFT double-precision transforms; the
+ * first call for a grid size builds and keeps its plans and buffers.
+ *
+ * vls_smooth_fft: smoothFFT (GD3.m:453-458) / smoothDiffusionFFT (:479-484),
+ * real(ifftn(fftn(rhs) ./ (1 + alpha*dt*|k|^4 or |k|^2))) with @GD3's
+ * wavenumbers (GD3.m:195-207). */
+int vls_smooth_fft(double *out, const double *rhs, int rows, int cols, int pges, double dx, double dy, double dz,
+                   double dt, double alpha, int diffusion, vls_stream_t stream);
+
+/* vls_smooth_gmres: smoothGMRES (GD3.m:435-451) / smoothDiffusionGMRES
+ * (:461-477), which the drivers call on the normal speed
+ * (p00_test_singlephase_relax.m:110).  Returns the solution of the same
+ * finite-difference system that MATLAB's gmres solves to 1e-12: the periodic
+ * operator is circulant, so one pair of FFTs with its exact eigenvalues
+ * solves it to rounding. */
+int vls_smooth_gmres(double *out, const double *rhs, int rows, int cols, int pges, double dx, double dy,
+                     double dz, double dt, double alpha, int diffusion, vls_stream_t stream);
+
+/* y = Op*x with Op = obj.Idt + alpha*dt*obj.LBiLaplacian (GD3.m:438) or
+ * obj.Idt - alpha*dt*obj.LLaplacian (:464); y must not alias x.  The residual
+ * of a smoothing solve is rhs - Op*out. */
+int vls_smooth_operator(double *y, const double *x, int rows, int cols, int pges, double dx, double dy, double dz,
+                        double dt, double alpha, int diffusion, vls_stream_t stream);
+
+/* NormalSpeedBend = Kappa*(GD3.Laplacian(H) + 0.5*H.*(H.^2 - 4*G) + 2*SponC*G
+ * - 0.5*SponC^2*H) (p00_test_singlephase_relax.m:79-83); H the (extended) mean
+ * curvature, G the Gaussian curvature; periodic Laplacian.  out must not alias H. */
+int vls_bending_speed(double *out, const double *H, const double *G, int rows, int cols, int pges, double dx,
+                      double dy, double dz, double kappa, double sponc, vls_stream_t stream);
+
 /* ------------------------------------------------------------------------ */
 /* 2. the @SDF3 object: one handle, one call per MATLAB method              */
 /* ------------------------------------------------------------------------ */
@@ -266,6 +299,21 @@ int vls_sdf3_max_speed(vls_sdf3_t h, const double *vx, const double *vy, const d
  * Compensated FP64 sums; returns to the host (synchronizes). */
 int vls_sdf3_area_volume_towers(vls_sdf3_t h, const double *F, double *area, double *volume,
                                 vls_stream_t stream);
+
+/* The tension/pressure step of the vesicle driver (p00_test_singlephase_relax.m:
+ * 67-109): area = calArea, volume = calVolume (SDF3.m:73-80), the surface
+ * integrals c11 = surfaceIntegral(H.^2), c12 = surfaceIntegral(H) (SDF3.m:82-85),
+ * Dt = cfl*Ds/max|Vb| over |F|<2Ds when dt <= 0 (else dt), s1 =
+ * surfaceIntegral(H.*Vb) - (area0 - area)/Dt, s2 = surfaceIntegral(Vb) +
+ * (volume0 - volume)/Dt, [Tension; Pressure] = [c11 c12; c12 area0] \ [s1; s2]
+ * (partial pivoting), and speed = (Tension*H - Vb + Pressure).*gradmag.  H is
+ * the extended mean curvature, Vb the extended bending speed, dirac/gradmag/
+ * heaviside the toolbox fields of F.  out10 (device, 10 doubles) = {Tension,
+ * Pressure, Dt, c11, c12, s1, s2, area, volume, max|Vb|}.  Compensated sums in
+ * a fixed order; nothing returns to the host, so the call can be captured. */
+int vls_sdf3_tension_pressure(vls_sdf3_t h, const double *F, const double *H, const double *Vb, const double *dirac,
+                              const double *gradmag, const double *heaviside, double area0, double volume0,
+                              double dt, double cfl, double *speed, double *out10, vls_stream_t stream);
 
 /* Cosine-smoothed delta/H of half-width eps (src/0_0_setup/+SD/@SDF3/SDF3.m:59,126-136,
  * 178-187; the reference uses eps = 1.5*dx).  Returns to the host (synchronizes). */
