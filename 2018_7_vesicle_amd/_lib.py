@@ -34,6 +34,10 @@ SIGNATURES = {
     "vls_set_calculus_toolbox": [P] * 3 + [I] * 3 + [D] * 4 + [I, P],
     "vls_auxi_set_calculus_toolbox": [P] * 4 + [I] * 3 + [D] * 4 + [I, P],
     "vls_aux_calculus_toolbox": [P] * 3 + [I] * 3 + [D] * 4 + [P],
+    "vls_mat_count": [ctypes.c_char_p, P],
+    "vls_mat_info": [ctypes.c_char_p, I, ctypes.c_char_p, I, P, P, P],
+    "vls_mat_read": [ctypes.c_char_p, ctypes.c_char_p, P, ctypes.c_int64],
+    "vls_mat_write": [ctypes.c_char_p, I, P, P, P, P, I],
     "vls_smooth_fft": [P] * 2 + [I] * 3 + [D] * 5 + [I, P],
     "vls_smooth_gmres": [P] * 2 + [I] * 3 + [D] * 5 + [I, P],
     "vls_smooth_operator": [P] * 2 + [I] * 3 + [D] * 5 + [I, P],

@@ -17,6 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB_NAME = "libvesicle_ls.so"
 LIB_PATH = os.path.join(HERE, LIB_NAME)
 SOURCES = ["vls_kernels.hip", "vls_batch.hip", "vls_adjoint.hip", "vls_variants.hip", "vls_relax.hip", "vls_api.hip"]
+CXX_SOURCES = ["vls_matio.cpp"]  # host-only C++ (the MAT v5 reader/writer)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
@@ -40,9 +41,16 @@ def build(verbose=False):
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
         objs.append(obj)
+    for src in CXX_SOURCES:
+        obj = os.path.join(CSRC, src.replace(".cpp", ".o"))
+        cmd = ["g++", "-c", os.path.join(CSRC, src), "-o", obj, "-O2", "-std=c++17", "-fPIC", "-Wall"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        objs.append(obj)
     tmp = LIB_PATH + ".tmp"
     cmd = [HIPCC, "-shared", "-o", tmp] + objs + ["--offload-arch=gfx950", "-fPIC", "-Wl,-rpath,/opt/rocm/lib",
-                                                  "-L/opt/rocm/lib", "-lrocfft"]
+                                                  "-L/opt/rocm/lib", "-lrocfft", "-lz"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -41,6 +41,15 @@ int fail(int code, const char* fmt, ...)
 
 inline hipStream_t S(vls_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
 
+}  // namespace
+
+// the error text of vls_last_error() for the host-only sources (vls_matio.cpp)
+namespace vls {
+int api_fail(int code, const char* msg) { return fail(code, "%s", msg); }
+}  // namespace vls
+
+namespace {
+
 int check_geom(int rows, int cols, int pges, double dx, double dy, double dz)
 {
     if (rows <= 0 || cols <= 0 || pges <= 0) return fail(VLS_ERR_INVALID, "box %dx%dx%d is empty", rows, cols, pges);

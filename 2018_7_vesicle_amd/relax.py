@@ -1,0 +1,111 @@
+"""The single-phase vesicle relaxation step: the loop body of
+src/7_0_paper/2_0_Vesicle/p00_test_singlephase_relax.m:60-196 on the GPU.
+
+One step, in the driver's order:
+  1. GPUsetCalculusToolBox of F (mean and Gaussian curvature, Towers delta/H);
+  2. MeanCurvature = WENORK3Extend(MeanCurvature, ext_iters)          (:76)
+  3. NormalSpeedBend = Kappa*(Lap H + 0.5 H(H^2-4G) + 2 SponC G - 0.5 SponC^2 H),
+     then ENORK2Extend(., ext_iters)                                    (:79-84)
+  4. area, volume, Dt = CFLNumber*Ds/max|Vb| on |F|<2Ds, the 2x2 tension/
+     pressure solve and normalSpeed = (T H - Vb + P)|grad F|           (:67-109)
+  5. smoothGMRES(normalSpeed, Dt, 0.5), then ENORK2Extend(., ext_iters) (:110-112)
+  6. F = F - Dt*speed; setDistance                                      (:114-115)
+  7. every reinit_every steps: WENO5RK3Reinitialization(F, reinit_iters)  (:191-193)
+The energy surfaceIntegral((H - SponC)^2) of :118 is kept on the device per
+step.  Dt is read back once per step (the smoothing and the update need it
+as a scalar), as the driver's gather does.
+
+The toolbox writes into buffers allocated once; every operator is a HIP
+kernel of the library (the FFT through rocFFT).
+"""
+from dataclasses import dataclass
+
+import torch
+
+from ._lib import VLSError
+from .sdf3 import GD3, SDF3, TOOLBOX_FIELDS, feval_set_calculus_toolbox
+
+
+@dataclass
+class RelaxConfig:
+    cfl: float = 0.5          # CFLNumber of the 96^3 run (p00:7)
+    kappa: float = 1.0        # bending modulus (:46)
+    sponc: float = 0.0        # spontaneous curvature (:12)
+    alpha: float = 0.5        # smoothGMRES(normalSpeed, Dt, 0.5) (:110)
+    ext_iters: int = 100      # WENORK3Extend / ENORK2Extend iterations (:76, :84, :111)
+    reinit_every: int = 1     # ReinitializationRate of the 96^3 run (:7)
+    reinit_iters: int = 100   # WENO5RK3Reinitialization(map.F, 100) (:192)
+
+
+def oblate_ellipsoid(n, reduced_volume=0.6, ratio=0.35):
+    """+SD/+Shape/Ellipsoid.m:5-44 for TYPE 'O': xv = linspace(-1,1,n), yv = zv =
+    ((1-n)/2 : (n-1)/2)*dx, a = b = ratio*2, c = a*cq with cq interpolated
+    (interp1, linear) in the table of reduced volumes of the spheroids with
+    half axes (1,1,c), c = 0.01:0.01:0.99 (the oblate area is closed-form).
+    Returns (xv, yv, zv, F) with F of shape (n, n, n) = (pges, cols, rows)."""
+    import numpy as np
+
+    xv = np.linspace(-1.0, 1.0, n)
+    dx = xv[1] - xv[0]
+    yv = (np.arange(n) + (1 - n) / 2.0) * dx
+    zv = yv.copy()
+    cs = np.arange(1, 100) / 100.0
+    e = np.sqrt(1 - cs ** 2)
+    area = 2 * np.pi * (1 + cs ** 2 / e * np.arctanh(e))
+    vol = 4 * np.pi * cs / 3
+    rv = (3 * vol / 4 / np.pi) * (4 * np.pi / area) ** 1.5
+    cq = float(np.interp(reduced_volume, rv, cs))
+    a = ratio * 2.0
+    c = a * cq
+    Z, X, Y = np.meshgrid(zv, xv, yv, indexing="ij")
+    F = np.sqrt(X ** 2 / a ** 2 + Y ** 2 / a ** 2 + Z ** 2 / c ** 2) - 1
+    return xv, yv, zv, np.ascontiguousarray(F)
+
+
+class VesicleRelax:
+    """State of the driver: the SDF3 object, the initial area and volume, time."""
+
+    def __init__(self, grid: GD3, F: torch.Tensor, cfg: RelaxConfig = RelaxConfig()):
+        if F.dtype != torch.float64:
+            raise VLSError("the relaxation step is FP64")
+        self.cfg = cfg
+        self.G = grid
+        self.map = SDF3(grid, F.contiguous())
+        self.map.setDistance()
+        self._tool = [grid.zeros() for _ in TOOLBOX_FIELDS]
+        self._hp = grid.zeros()
+        self._toolbox()
+        self.InitialArea, self.InitialVolume = self.map.calAreaVolume()
+        self.time = 0.0
+        self.iteration = 1
+        self.info = None
+        self.energy = []
+
+    def _toolbox(self):
+        m, g = self.map, self.G
+        torch.clamp(m.F, min=0.0, out=self._hp)
+        feval_set_calculus_toolbox(self._tool, m.F, self._hp, g.mrows, g.ncols, g.lshts, g.Dx, g.Dy, g.Dz, g.Ds)
+        for name, t in zip(TOOLBOX_FIELDS, self._tool):
+            setattr(m, name, t)
+        m.HPrimal = self._hp
+
+    def step(self):
+        cfg, m, g = self.cfg, self.map, self.G
+        self.iteration += 1
+        self._toolbox()
+        H = m.WENORK3Extend(m.MeanCurvature, cfg.ext_iters)
+        Vb = m.ENORK2Extend(g.bendingSpeed(H, m.GaussianCurvature, cfg.kappa, cfg.sponc), cfg.ext_iters)
+        speed, info = m.tensionPressure(H, Vb, self.InitialArea, self.InitialVolume, CFLNumber=cfg.cfl)
+        dt = float(info[2])  # the one read-back of the step
+        g.smoothGMRES(speed, dt, cfg.alpha, out=speed)
+        speed = m.ENORK2Extend(speed, cfg.ext_iters)
+        m.F = m.F - dt * speed
+        m.setDistance()
+        band = m.F.abs() < 2 * g.Ds
+        hs = H - cfg.sponc
+        self.energy.append((hs * hs * m.DiracDelta * m.FGradMag)[band].sum() * g.Ds ** 3)
+        if cfg.reinit_every > 0 and self.iteration % cfg.reinit_every == 0:
+            m.F = m.WENO5RK3Reinitialization(m.F, cfg.reinit_iters)
+        self.time += dt
+        self.info = info
+        return info

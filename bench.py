@@ -34,7 +34,9 @@ FP32 bulk vs FP64, 200 steps each, with the FP32 run's errors), config4
 per-vesicle FP64 path) and config5 (the 96^3 adjoint gradient against finite
 differences and tangent-mode derivatives), variants (the WENO-RK3 / ENO-RK2
 extensions and WENO-RK3 reinitialization at 96^3), toolbox (the calculus
-toolboxes of F and of an auxiliary field A at 128^3).
+toolboxes of F and of an auxiliary field A at 128^3), relax (the single-phase
+vesicle relaxation step at 96^3: extensions, bending speed, tension/pressure
+solve, rocFFT smoothing, reinitialization).
 """
 import argparse
 import json
@@ -63,11 +65,12 @@ def parse():
     ap.add_argument("--batch-steps", type=int, default=10,
                     help="timed steps of the 512-vesicle batch comparison (0 skips it)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 96^3 adjoint gradient check")
-    ap.add_argument("--probe", choices=["extend", "batch", "variants", "toolbox"], default=None,
+    ap.add_argument("--probe", choices=["extend", "batch", "variants", "toolbox", "relax"], default=None,
                     help="extend: a few extension stages and a 256 MiB device copy (for rocprofv3 --pmc); "
                          "batch: a few steps of the 512-vesicle FP32 batch (for rocprofv3 --kernel-trace); "
                          "variants: the WENO-RK3 / ENO-RK2 calls of the variants line (for rocprofv3 --kernel-trace); "
-                         "toolbox: the calculus-toolbox calls of the toolbox line (for rocprofv3 --kernel-trace)")
+                         "toolbox: the calculus-toolbox calls of the toolbox line (for rocprofv3 --kernel-trace); "
+                         "relax: the vesicle relaxation steps of the relax line")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64",
                     help="field type of the main pipeline line (fp32: FP32 bulk)")
     ap.add_argument("--main-only", action="store_true",
@@ -344,6 +347,54 @@ def toolbox(with_cpu, n=128, reps=20):
     return out
 
 
+def relax(with_cpu, steps=5):
+    """Beside the main line: the single-phase vesicle relaxation step of
+    p00_test_singlephase_relax.m at its 96^3 setting (oblate ellipsoid of
+    reduced volume 0.6, CFL 0.5, 100 extension and reinitialization
+    iterations, reinitialization every step): toolbox, WENORK3Extend,
+    bending speed, ENORK2Extend, tension/pressure, smoothGMRES (rocFFT),
+    ENORK2Extend, update, setDistance, WENO5RK3Reinitialization.  ms per step
+    on the GPU (one warm-up step, then `steps`), and one step of the CPU
+    oracle's composition of the same operators."""
+    import numpy as np
+    import torch
+
+    vls = __import__("importlib").import_module("2018_7_vesicle_amd")
+    n = 96
+    xv, yv, zv, F0 = vls.relax.oblate_ellipsoid(n)
+    G = vls.GD3(xv, yv, zv)
+    m0 = vls.SDF3(G, torch.from_numpy(F0).cuda())
+    m0.setDistance()
+    F = m0.WENO5RK3Reinitialization(m0.F, 500)
+    cfg = vls.relax.RelaxConfig()
+    r = vls.relax.VesicleRelax(G, F, cfg)
+    r.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r.step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    info = dict(zip(vls.TENSION_PRESSURE_FIELDS, r.info.cpu().tolist()))
+    # pseudo-time stage updates per step: WENO-RK3 (3 stages) and two ENO-RK2
+    # (2 stages) extensions and the WENO5-RK3 reinitialization (3 stages)
+    upd = (3 + 2 + 2 + 3) * cfg.ext_iters * n ** 3
+    out = {"grid": n, "steps": steps, "gpu_ms_per_step": ms, "gpu_updates_per_s": upd / (ms * 1e-3),
+           "area_drift": info["area"] / r.InitialArea - 1, "volume_drift": info["volume"] / r.InitialVolume - 1,
+           "Tension": info["Tension"], "Pressure": info["Pressure"], "Dt": info["Dt"]}
+    if with_cpu:
+        from oracle import pyoracle as po
+        from oracle import relax_oracle as ro
+
+        Fc = F.cpu().numpy()
+        t0 = time.perf_counter()
+        ro.relax_step(Fc, r.InitialArea, r.InitialVolume, G.Dx, G.Dy, G.Dz, cfl=cfg.cfl)
+        cpu_s = time.perf_counter() - t0
+        out["cpu_ms_per_step"] = cpu_s * 1e3
+        out["cpu_threads"] = po.num_threads()
+    return out
+
+
 def config5():
     """Config 5 beside the main line: the 96^3 shape-matching problem (200
     WENO5-RK3 advection steps, reinitialization every 10, velocity from 3x64
@@ -371,6 +422,9 @@ def main():
         return probe_batch(args)
     if args.probe == "variants":
         print(json.dumps({"probe": "variants", **variants(False, iterations=20)}), flush=True)
+        return 0
+    if args.probe == "relax":
+        print(json.dumps({"probe": "relax", **relax(not args.no_cpu, steps=3)}), flush=True)
         return 0
     if args.probe == "toolbox":
         print(json.dumps({"probe": "toolbox", **toolbox(False, reps=5)}), flush=True)
@@ -468,6 +522,7 @@ def main():
             out["config5"] = config5()
         out["variants"] = variants(not args.no_cpu)
         out["toolbox"] = toolbox(not args.no_cpu)
+        out["relax"] = relax(not args.no_cpu)
         if not args.no_cpu:
             out["cpu_baseline"] = cpu_baseline(cfg)
     print(json.dumps(out), flush=True)

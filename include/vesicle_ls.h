@@ -221,6 +221,29 @@ int vls_smooth_operator(double *y, const double *x, int rows, int cols, int pges
 int vls_bending_speed(double *out, const double *H, const double *G, int rows, int cols, int pges, double dx,
                       double dy, double dz, double kappa, double sponc, vls_stream_t stream);
 
+/* MATLAB Level 5 MAT-files (the format of the reference's fixtures and run
+ * records, e.g. src/7_0_paper/2_0_Vesicle/pear.mat and the CFL*.mat files that
+ * p00_test_singlephase_relax.m:197-206 saves with MATLAB's save), for fixtures
+ * and restart.  Host memory only; little-endian files; compressed
+ * (miCOMPRESSED) and plain elements.  Only numeric arrays are listed; every
+ * numeric class reads as double in MATLAB's column-major order.
+ *
+ * vls_mat_count: number of numeric arrays in the file. */
+int vls_mat_count(const char *path, int *count);
+/* vls_mat_info: name (NUL-terminated, at most name_cap bytes), rank and sizes
+ * (dims8, up to 8) of the index-th numeric array; is_complex may be NULL. */
+int vls_mat_info(const char *path, int index, char *name, int name_cap, int *ndims, int64_t *dims8,
+                 int *is_complex);
+/* vls_mat_read: the real part of the array called name (the first numeric
+ * array when name is NULL) into out, which holds exactly count doubles. */
+int vls_mat_read(const char *path, const char *name, double *out, int64_t count);
+/* vls_mat_write: a new file with narrays double arrays (names 1-63
+ * characters, ndims[k] in 2..8, dims8 holding 8 sizes per array); compress =
+ * 1 stores each array zlib-compressed as MATLAB's default save does.  Written
+ * to path.tmp and renamed. */
+int vls_mat_write(const char *path, int narrays, const char *const *names, const double *const *data,
+                  const int *ndims, const int64_t *dims8, int compress);
+
 /* ------------------------------------------------------------------------ */
 /* 2. the @SDF3 object: one handle, one call per MATLAB method              */
 /* ------------------------------------------------------------------------ */

@@ -30,9 +30,6 @@ import math
 import numpy as np
 import scipy.sparse as sp
 
-_AX = {"y": 2, "x": 1, "z": 0}
-
-
 def circshift(f, shift_rows, shift_cols, shift_pges):
     """MATLAB circshift(f, [shift_rows shift_cols shift_pges]) on a (pges, cols, rows) array"""
     return np.roll(np.roll(np.roll(f, shift_rows, axis=2), shift_cols, axis=1), shift_pges, axis=0)
@@ -203,3 +200,29 @@ def tension_pressure(F, H, Vb, dirac, gmag, heav, ds, area0, volume0, dt=None, c
     speed = (T * H - Vb + P) * gmag
     return speed, dict(Tension=T, Pressure=P, Dt=dt, c11=c11, c12=c12, s1=s1, s2=s2, area=area, volume=volume,
                        vmax=vmax)
+
+
+def relax_step(F, area0, volume0, dx, dy, dz, cfl=0.5, kappa=1.0, sponc=0.0, alpha=0.5, ext_iters=100,
+               reinit=True, reinit_iters=100):
+    """one loop body of p00_test_singlephase_relax.m:60-196 on the CPU oracle:
+    toolbox, WENORK3Extend of the mean curvature, bending speed and its
+    ENORK2Extend, tension/pressure, smoothGMRES (its exact solution), the
+    ENORK2Extend of the smoothed speed, F - Dt*speed, setDistance and the
+    WENO5-RK3 reinitialization.  Returns (F, info)."""
+    from . import pyoracle as po
+
+    ds = (dx + dy + dz) / 3
+    ft = po.set_calculus_toolbox(F, np.maximum(F, 0), dx, dy, dz, ds)
+    H = po.variant_extend(ft["MeanCurvature"], ext_iters, F, dx, dy, dz, "wenork3")
+    Vb = po.variant_extend(bending_speed(H, ft["GaussianCurvature"], dx, dy, dz, kappa, sponc), ext_iters, F, dx, dy,
+                           dz, "enork2")
+    speed, info = tension_pressure(F, H, Vb, ft["DiracDelta"], ft["FGradMag"], ft["Heaviside"], ds, area0, volume0,
+                                   cfl=cfl)
+    dt = info["Dt"]
+    sm = smooth_exact(speed, dx, dy, dz, dt, alpha)
+    sm = po.variant_extend(sm, ext_iters, F, dx, dy, dz, "enork2")
+    F = F - dt * sm
+    d6, md = po.set_distance(F, dx, dy, dz)
+    if reinit:
+        F = po.weno5_rk3_reinit(F, reinit_iters, d6, md, dx, dy, dz)
+    return F, info

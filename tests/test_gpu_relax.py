@@ -124,3 +124,63 @@ def test_tension_pressure(use_cfl):
     for k, v in want.items():
         assert abs(got[k] - v) <= 1e-12 * max(abs(v), 1e-300), (k, got[k], v)
     assert rel(N(speed), want_speed) < 1e-12
+
+
+def test_relax_step_matches_the_oracle():
+    """two loop bodies of the single-phase driver at 32^3 (8 extension and
+    reinitialization iterations) against the oracle's composition of the same
+    operators"""
+    n = 32
+    xv = np.linspace(-1.0, 1.0, n)
+    g = cases.Grid(xv, xv, xv)
+    F0 = np.ascontiguousarray(np.sqrt(g.X ** 2 / 0.7 ** 2 + g.Y ** 2 / 0.55 ** 2 + g.Z ** 2 / 0.45 ** 2) - 1.0)
+    d6, md = po.set_distance(F0, *g.spacing)
+    F0 = po.weno5_rk3_reinit(F0, 20, d6, md, *g.spacing)
+    cfg = vls.relax.RelaxConfig(ext_iters=8, reinit_iters=8)
+    r = vls.relax.VesicleRelax(vls.GD3(xv, xv, xv), T(F0), cfg)
+    a0, v0 = po.area_volume_towers(F0, *g.spacing)
+    assert abs(r.InitialArea - a0) <= 1e-13 * a0 and abs(r.InitialVolume - v0) <= 1e-13 * v0
+    F = F0
+    for _ in range(2):
+        info = N(r.step())
+        F, want = ro.relax_step(F, r.InitialArea, r.InitialVolume, *g.spacing, cfl=cfg.cfl, ext_iters=8,
+                                reinit_iters=8)
+        got = dict(zip(vls.TENSION_PRESSURE_FIELDS, info.tolist()))
+        for k in ("Tension", "Pressure", "Dt", "area", "volume"):
+            assert abs(got[k] - want[k]) <= 1e-10 * abs(want[k]), (k, got[k], want[k])
+        assert np.abs(N(r.map.F) - F).max() < 1e-12
+
+
+@pytest.mark.parametrize("cfl,record,steps,tol", [(0.5, "CFL0_5", 4, 1e-6), (0.1, "CFL0_1", 2, 1e-9)],
+                         ids=["cfl0.5", "cfl0.1"])
+def test_relax_reproduces_the_reference_run(cfl, record, steps, tol):
+    """the single-phase relaxation at 96^3 (oblate ellipsoid of reduced volume
+    0.6, setDistance + 500 WENO5-RK3 reinitialization iterations, then the
+    loop body of p00_test_singlephase_relax.m) against the run the reference
+    recorded with MATLAB (tests/golden/<record>.mat, saved by
+    p00_test_singlephase_relax.m:197-216): time, pressure, tension, energy,
+    area and volume drift of the first steps.  The runs agree to 1e-12 in the
+    first step; rounding differences (MATLAB's gmres stops at 1e-12, its sums
+    run in another order) grow from there, and a grid point that crosses the
+    |F| < 2 Ds band edge changes max|Vb| and with it Dt, so only the first
+    steps are compared, at `tol`, and the first step at 1e-10."""
+    import os
+
+    rec = vls.matio.loadmat(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", record + ".mat"))
+    s = record[3:]
+    ref = {k: rec[k + s].ravel() for k in ("t", "P", "ten", "da", "dv", "ene")}
+    xv, yv, zv, F0 = vls.relax.oblate_ellipsoid(96)
+    G = vls.GD3(xv, yv, zv)
+    m0 = vls.SDF3(G, T(F0))
+    m0.setDistance()
+    F = m0.WENO5RK3Reinitialization(m0.F, 500)
+    r = vls.relax.VesicleRelax(G, F, vls.relax.RelaxConfig(cfl=cfl))
+    for i in range(steps):
+        info = dict(zip(vls.TENSION_PRESSURE_FIELDS, N(r.step()).tolist()))
+        got = {"t": r.time, "P": info["Pressure"], "ten": info["Tension"], "ene": float(r.energy[-1]),
+               "da": info["area"] / r.InitialArea - 1, "dv": info["volume"] / r.InitialVolume - 1}
+        bar = 1e-10 if i == 0 else tol
+        for k in ("t", "P", "ten", "ene"):
+            assert abs(got[k] - ref[k][i]) <= bar * abs(ref[k][i]), (i, k, got[k], ref[k][i])
+        for k in ("da", "dv"):  # drifts: relative to the area / volume
+            assert abs(got[k] - ref[k][i]) <= 10 * bar, (i, k, got[k], ref[k][i])
