@@ -250,6 +250,14 @@ struct FftPlan {
     rocfft_execution_info info = nullptr;
     void* work = nullptr;
     double2* buf = nullptr;
+    ~FftPlan()  // only plans that failed to build are destroyed; the cached ones live with the process
+    {
+        if (fwd) rocfft_plan_destroy(fwd);
+        if (inv) rocfft_plan_destroy(inv);
+        if (info) rocfft_execution_info_destroy(info);
+        if (work) (void)hipFree(work);
+        if (buf) (void)hipFree(buf);
+    }
 };
 
 std::mutex g_fft_mu;
@@ -289,7 +297,6 @@ hipError_t fft_plan(const Geom& g, FftPlan*& out)
     }
     if (wb > 0) {
         if ((e = hipMalloc(&p->work, wb)) != hipSuccess) {
-            (void)hipFree(p->buf);
             delete p;
             return e;
         }

@@ -35,6 +35,7 @@ class RelaxConfig:
     ext_iters: int = 100      # WENORK3Extend / ENORK2Extend iterations (:76, :84, :111)
     reinit_every: int = 1     # ReinitializationRate of the 96^3 run (:7)
     reinit_iters: int = 100   # WENO5RK3Reinitialization(map.F, 100) (:192)
+    autotune: bool = True     # pick the stage kernels' tile shape and z-march depth for this grid once
 
 
 def oblate_ellipsoid(n, reduced_volume=0.6, ratio=0.35):
@@ -71,7 +72,10 @@ class VesicleRelax:
         self.cfg = cfg
         self.G = grid
         self.map = SDF3(grid, F.contiguous())
-        self.map.setDistance()
+        if cfg.autotune:
+            self.map.autotune()  # leaves the distances of F set, as setDistance does
+        else:
+            self.map.setDistance()
         self._tool = [grid.zeros() for _ in TOOLBOX_FIELDS]
         self._hp = grid.zeros()
         self._toolbox()
