@@ -1,6 +1,6 @@
 """Write the golden vectors from the reference's own kernels.
 
-Runs on a GPU box with oracle/_ref built (`make -C oracle ref` in a checkout
+Runs on a machine with an MI355X and oracle/_ref built (`make -C oracle ref` in a checkout
 that has the reference sources).  For each case of cases.golden_grids() it
 feeds seeded/analytic inputs to the reference kernels (compiled unmodified
 from the reference's CUDA sources), in the order the reference's MATLAB

@@ -1,7 +1,7 @@
 """Write the golden vectors of the auxiliary-field toolbox from the reference's
 own kernels.
 
-Runs on a GPU box with oracle/_ref built (`make -C oracle ref` in a checkout
+Runs on a machine with an MI355X and oracle/_ref built (`make -C oracle ref` in a checkout
 that has the reference sources; the code objects are set_calculus_toolbox.co
 and auxi_set_calculus_toolbox.co).  For each case of cases.golden_grids() it
 runs set_calculus_toolbox on F and then auxi_set_calculus_toolbox on the

@@ -1,7 +1,7 @@
 """Write the golden vectors of the WENO-RK3 / ENO-RK2 variants from the
 reference's own kernels.
 
-Runs on a GPU box with oracle/_ref built (`make -C oracle ref` in a checkout
+Runs on a machine with an MI355X and oracle/_ref built (`make -C oracle ref` in a checkout
 that has the reference sources; the variant code objects are
 weno_rk3_reinitialization.co, enork2_extend.co and wenork3_extend.co).  For
 each case of cases.golden_grids() it runs the reference kernels in the order
