@@ -184,3 +184,30 @@ def test_relax_reproduces_the_reference_run(cfl, record, steps, tol):
             assert abs(got[k] - ref[k][i]) <= bar * abs(ref[k][i]), (i, k, got[k], ref[k][i])
         for k in ("da", "dv"):  # drifts: relative to the area / volume
             assert abs(got[k] - ref[k][i]) <= 10 * bar, (i, k, got[k], ref[k][i])
+
+
+@pytest.mark.parametrize("n,steps,tol,ttol", [(48, 1, 1e-9, 1e-6), (64, 3, 1e-9, 1e-9), (80, 2, 1e-10, 1e-10)],
+                         ids=["48", "64", "80"])
+def test_relax_reproduces_the_reference_grid_study(n, steps, tol, ttol):
+    """the reference's grid study of the same relaxation (CFL 0.1,
+    reinitialization every step; tests/golden/oblate<n>.mat, saved by
+    p00_test_singlephase_relax.m:260-290): time, pressure, tension and energy
+    of the first steps on 48^3, 64^3 and 80^3 grids.  At 48^3 the time step
+    (max|Vb| on the band) agrees to 2e-7 only: the maximum sits where the two
+    extensions differ most."""
+    import os
+
+    rec = vls.matio.loadmat(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", f"oblate{n}.mat"))
+    ref = {k: rec[f"{k}{n}"].ravel() for k in ("t", "P", "ten", "ene")}
+    xv, yv, zv, F0 = vls.relax.oblate_ellipsoid(n)
+    G = vls.GD3(xv, yv, zv)
+    m0 = vls.SDF3(G, T(F0))
+    m0.setDistance()
+    F = m0.WENO5RK3Reinitialization(m0.F, 500)
+    r = vls.relax.VesicleRelax(G, F, vls.relax.RelaxConfig(cfl=0.1))
+    for i in range(steps):
+        info = dict(zip(vls.TENSION_PRESSURE_FIELDS, N(r.step()).tolist()))
+        got = {"t": r.time, "P": info["Pressure"], "ten": info["Tension"], "ene": float(r.energy[-1])}
+        for k in ("t", "P", "ten", "ene"):
+            bar = ttol if k == "t" else tol
+            assert abs(got[k] - ref[k][i]) <= bar * abs(ref[k][i]), (n, i, k, got[k], ref[k][i])

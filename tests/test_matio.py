@@ -11,11 +11,12 @@ import scipy.io as sio
 
 vls = importlib.import_module("2018_7_vesicle_amd")
 HERE = os.path.dirname(os.path.abspath(__file__))
-RECORDS = [os.path.join(HERE, "golden", f) for f in ("CFL0_5.mat", "CFL0_1.mat")]
+RECORDS = [os.path.join(HERE, "golden", f) for f in ("CFL0_5.mat", "CFL0_1.mat", "oblate48.mat", "oblate64.mat",
+                                                     "oblate80.mat")]
 PEAR = "/root/reference/src/7_0_paper/2_0_Vesicle/pear.mat"
 
 
-@pytest.mark.parametrize("path", RECORDS, ids=["CFL0_5", "CFL0_1"])
+@pytest.mark.parametrize("path", RECORDS, ids=[os.path.basename(p)[:-4] for p in RECORDS])
 def test_reads_the_reference_run_records(path):
     got = vls.matio.loadmat(path)
     want = {k: v for k, v in sio.loadmat(path).items() if not k.startswith("__")}
