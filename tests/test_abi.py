@@ -58,6 +58,28 @@ def test_invalid_geometry_rejected_without_device_work():
     assert rc == -1 and b"null" in lib.vls_last_error()
 
 
+def test_relaxation_entries_reject_bad_arguments_without_device_work():
+    lib = vls.lib()
+    p, z = ctypes.c_void_p(16), ctypes.c_void_p()
+    # smoothers: null field, bad kind, non-finite dt, aliasing of the operator
+    assert lib.vls_smooth_gmres(z, p, 4, 4, 4, 0.1, 0.1, 0.1, 1e-3, 0.5, 0, None) == -1
+    assert b"null" in lib.vls_last_error()
+    assert lib.vls_smooth_fft(p, p, 4, 4, 4, 0.1, 0.1, 0.1, 1e-3, 0.5, 2, None) == -1
+    assert b"diffusion" in lib.vls_last_error()
+    assert lib.vls_smooth_fft(p, p, 4, 4, 4, 0.1, 0.1, 0.1, float("nan"), 0.5, 0, None) == -1
+    assert lib.vls_smooth_operator(p, p, 4, 4, 4, 0.1, 0.1, 0.1, 1e-3, 0.5, 0, None) == -1
+    assert b"in place" in lib.vls_last_error()
+    assert lib.vls_smooth_gmres(p, p, 0, 4, 4, 0.1, 0.1, 0.1, 1e-3, 0.5, 0, None) == -1
+    # bending speed may not overwrite H
+    assert lib.vls_bending_speed(p, p, ctypes.c_void_p(32), 4, 4, 4, 0.1, 0.1, 0.1, 1.0, 0.0, None) == -1
+    # tension/pressure needs a handle; toolbox entries need every field
+    assert lib.vls_sdf3_tension_pressure(z, p, p, p, p, p, p, 1.0, 1.0, 0.0, 0.5, p, p, None) == -1
+    out = (ctypes.c_void_p * 30)(*([16] * 29 + [0]))
+    fin = (ctypes.c_void_p * 14)(*([16] * 14))
+    assert lib.vls_aux_calculus_toolbox(out, p, fin, 4, 4, 4, 0.1, 0.1, 0.1, 0.1, None) == -1
+    assert b"null output 29" in lib.vls_last_error()
+
+
 def test_missing_library_fails_loudly(monkeypatch):
     lib_mod = importlib.import_module("2018_7_vesicle_amd._lib")
     monkeypatch.setattr(lib_mod, "_lib", None)
