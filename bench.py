@@ -428,6 +428,13 @@ def main():
         return 0
     if args.probe == "toolbox":
         print(json.dumps({"probe": "toolbox", **toolbox(False, reps=5)}), flush=True)
+        # a plain 256 MiB float64 copy last: calibrates FETCH_SIZE / WRITE_SIZE
+        import torch
+
+        src = torch.ones(1 << 25, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
         return 0
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported by this build "
