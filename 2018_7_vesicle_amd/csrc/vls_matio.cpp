@@ -154,7 +154,10 @@ bool parse_matrix(const uint8_t* p, const uint8_t* end, Array& a)
     a.name.assign((const char*)name.data, name.nbytes);
     if (!next_elem(p, end, a.real) || type_size(a.real.type) == 0) return false;
     int64_t count = 1;
-    for (int64_t d : a.dims) count *= d;
+    for (int64_t d : a.dims) {
+        if (d < 0 || (d > 0 && count > ((int64_t)1 << 40) / d)) return false;  // no element holds 2^40 values
+        count *= d;
+    }
     return (int64_t)a.real.nbytes == count * type_size(a.real.type);
 }
 
