@@ -114,3 +114,27 @@ def test_tension_pressure_solves_the_2x2_system():
     assert abs(r["Tension"] - T) < 1e-12 * abs(T) and abs(r["Pressure"] - P) < 1e-12 * abs(P)
     assert r["Dt"] == 0.05 * ds / np.abs(Vb[band]).max()
     assert np.abs(speed - (T * H - Vb + P) * gmag).max() < 1e-12 * np.abs(speed).max()
+
+
+def test_relax_step_reproduces_the_reference_grid_study():
+    """the oracle's composition of one relaxation step (48^3, CFL 0.1) against
+    the first entry of the reference's own record of that run
+    (tests/golden/oblate48.mat, p00_test_singlephase_relax.m:260-270): the
+    oblate ellipsoid, setDistance and 500 reinitialization iterations, then
+    one loop body.  Dt rests on max|Vb| over the band, which agrees to 2e-7."""
+    import importlib
+    import os
+
+    vls = importlib.import_module("2018_7_vesicle_amd")
+    from oracle import pyoracle as po
+
+    rec = vls.matio.loadmat(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "oblate48.mat"))
+    xv, yv, zv, F0 = vls.relax.oblate_ellipsoid(48)
+    dx, dy, dz = xv[1] - xv[0], yv[1] - yv[0], zv[1] - zv[0]
+    d6, md = po.set_distance(F0, dx, dy, dz)
+    F = po.weno5_rk3_reinit(F0, 500, d6, md, dx, dy, dz)
+    a0, v0 = po.area_volume_towers(F, dx, dy, dz)
+    _, info = ro.relax_step(F, a0, v0, dx, dy, dz, cfl=0.1)
+    assert abs(info["Pressure"] - rec["P48"][0, 0]) <= 1e-9 * abs(rec["P48"][0, 0])
+    assert abs(info["Tension"] - rec["ten48"][0, 0]) <= 1e-9 * abs(rec["ten48"][0, 0])
+    assert abs(info["Dt"] - rec["t48"][0, 0]) <= 1e-6 * rec["t48"][0, 0]
