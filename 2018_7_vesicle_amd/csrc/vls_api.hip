@@ -1776,8 +1776,20 @@ int sdf3_autotune(vls_sdf3* h, const T* F, int iterations, hipStream_t s, int* t
             if ((rc = extend_loop<T>(h, C, iterations, ms, s))) return rc;
             VLS_HIP(hipEventRecord(ev.b, s));
             VLS_HIP(hipEventSynchronize(ev.b));
-            float ms_ = 0;
+            float ms_ = 0, ms_r = 0;
             VLS_HIP(hipEventElapsedTime(&ms_, ev.a, ev.b));
+            // the reinitialization stages share the launch shape: time them on a copy of F
+            T* Fr = tmp<T>(h, 3);
+            VLS_HIP(launch_set_mask_bit<T>(h->flags, F, g, s));
+            VLS_HIP(hipMemcpyAsync(Fr, F, sizeof(T) * (size_t)g.n(), hipMemcpyDeviceToDevice, s));
+            if ((rc = reinit_loop<T>(h, Fr, 1, ms, s))) return rc;
+            VLS_HIP(hipMemcpyAsync(Fr, F, sizeof(T) * (size_t)g.n(), hipMemcpyDeviceToDevice, s));
+            VLS_HIP(hipEventRecord(ev.a, s));
+            if ((rc = reinit_loop<T>(h, Fr, iterations, ms, s))) return rc;
+            VLS_HIP(hipEventRecord(ev.b, s));
+            VLS_HIP(hipEventSynchronize(ev.b));
+            VLS_HIP(hipEventElapsedTime(&ms_r, ev.a, ev.b));
+            ms_ += ms_r;
             if (ms_ < best) {
                 best = ms_;
                 best_tr = trs[i];

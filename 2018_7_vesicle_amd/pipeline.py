@@ -53,6 +53,7 @@ class PipelineConfig:
     initial_reinit: int = 100
     eps_factor: float = 1.5
     precision: str = "fp64"  # or "fp32" (FP32 bulk)
+    autotune: bool = True  # pick the stage kernels' launch shape for the grid at set-up (results unchanged)
 
 
 def curl_velocity(X, Y, Z, modes, amp, kmax, seed):
@@ -129,6 +130,8 @@ class Pipeline:
             self.psi = self.psi.float()
         elif cfg.precision != "fp64":
             raise ValueError(f"precision must be fp64 or fp32, got {cfg.precision}")
+        if cfg.autotune:
+            self.map.autotune()
         self.Vn = self.grid.zeros(self.map.dtype)
         self._av = torch.zeros(2, dtype=torch.float64, device=dev)  # the step's area/volume output
         self.area_volume = None

@@ -195,9 +195,10 @@ class SDF3:
         return tr.value, zd.value
 
     def autotune(self, iterations=2):
-        """time extension stages off self.F's interface with each tile shape and
-        z-march depth and keep the fastest; leaves the distances of self.F set.
-        Returns the tile rows."""
+        """time extension and reinitialization stages off self.F's interface with
+        each tile shape and z-march depth and keep the fastest (the sum of both);
+        leaves the distances of self.F set.  Results do not depend on the
+        choice.  Returns the tile rows."""
         tr = ctypes.c_int()
         call(self._fn("vls_sdf3_autotune"), self._h, self._check(self.F), int(iterations), ctypes.byref(tr),
              _stream())
