@@ -477,9 +477,8 @@ int vls_batch_step_f32(vls_batch_t b, float *F, float *Vn, const float *vx, cons
 /* 5. autotuning of the launch shape (no counterpart in the reference)      */
 /* ------------------------------------------------------------------------ */
 /* Times `iterations` extension iterations of a scratch copy of F, off F's
- * interface (a single grid: plus as many reinitialization iterations of
- * another copy, the sum decides), for each candidate launch shape and keeps
- * the fastest: tile rows
+ * interface, plus as many reinitialization iterations of another copy, for
+ * each candidate launch shape and keeps the one with the smallest sum: tile rows
  * 64 or 32 and the z-march depth (a batch: 4, 8 or 16 pages; a single grid:
  * the depth chosen by the box size, half or twice it; read back with
  * vls_batch_get_zmarch_depth / vls_sdf3_get_launch).  A batch's persistent
