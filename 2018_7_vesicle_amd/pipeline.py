@@ -131,7 +131,7 @@ class Pipeline:
         elif cfg.precision != "fp64":
             raise ValueError(f"precision must be fp64 or fp32, got {cfg.precision}")
         if cfg.autotune:
-            self.map.autotune()
+            self.map.autotune(iterations=5)  # 5 timed iterations per candidate: a steadier choice
         self.Vn = self.grid.zeros(self.map.dtype)
         self._av = torch.zeros(2, dtype=torch.float64, device=dev)  # the step's area/volume output
         self.area_volume = None

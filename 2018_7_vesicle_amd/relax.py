@@ -73,7 +73,7 @@ class VesicleRelax:
         self.G = grid
         self.map = SDF3(grid, F.contiguous())
         if cfg.autotune:
-            self.map.autotune()  # leaves the distances of F set, as setDistance does
+            self.map.autotune(iterations=5)  # leaves the distances of F set, as setDistance does
         else:
             self.map.setDistance()
         self._tool = [grid.zeros() for _ in TOOLBOX_FIELDS]
