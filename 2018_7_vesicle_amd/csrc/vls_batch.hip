@@ -378,7 +378,7 @@ __global__ void __launch_bounds__(256) kb_list_gather_vel(double* __restrict__ l
 }
 
 // u.N, N = grad F/max(|grad F|,1e-14), clamped central differences
-// (set_calculus_toolbox.cu:123-136), as k_normal_velocity
+// (set_calculus_toolbox.cu:31-44), as k_normal_velocity
 template <int R, typename T>
 __global__ void __launch_bounds__(256) kb_normal_velocity(T* __restrict__ out, const T* __restrict__ F,
                                                           const T* __restrict__ vx, const T* __restrict__ vy,

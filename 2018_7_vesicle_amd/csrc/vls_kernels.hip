@@ -611,7 +611,7 @@ __device__ __forceinline__ Toolbox toolbox_of(const Box27& F, const Box& b, doub
     return t;
 }
 
-// Towers primal-function Heaviside and delta (set_calculus_toolbox.cu:183-198)
+// Towers primal-function Heaviside and delta (set_calculus_toolbox.cu:91-106)
 __device__ __forceinline__ void towers_h_delta(double& H, double& delta, const Box27& P, const Toolbox& t,
                                                const Box& b)
 {
@@ -787,7 +787,7 @@ __global__ void k_positive_part(double* __restrict__ out, const double* __restri
 }
 
 // normal component of a velocity, u.N with N = grad F/max(|grad F|,1e-14) from
-// clamped central differences (set_calculus_toolbox.cu:123-136)
+// clamped central differences (set_calculus_toolbox.cu:31-44)
 template <typename T>
 __global__ void __launch_bounds__(256) k_normal_velocity(T* __restrict__ out, const T* __restrict__ F,
                                                          const T* __restrict__ vx, const T* __restrict__ vy,

@@ -2,7 +2,7 @@
 
 * |grad phi| - 1 in a band |phi| < k*dx, with grad phi from the clamped
   central differences of set_calculus_toolbox (FGradMag,
-  src/7_0_paper/2_0_Vesicle/CUDA_Code/4_0_Calculus_ToolBox/set_calculus_toolbox.cu:123-127);
+  src/7_0_paper/2_0_Vesicle/CUDA_Code/4_0_Calculus_ToolBox/set_calculus_toolbox.cu:31-35);
 * interface crossings: where the zero set cuts grid lines, located by
   setDistance's sixth-order subcell locator (boundary_location.cu:84-132),
   as points in space;

@@ -76,14 +76,14 @@ int vls_upwind_normal(double *nx, double *ny, double *nz, const double *lsf, con
                       const double *ypf, const double *ypb, const double *zpu, const double *zpd, int rows,
                       int cols, int pges, double dx, double dy, double dz, int num_ele, vls_stream_t stream);
 
-/* boundary_interpolant: CUDA_Code/2_2_WENO5RK3_Extend/boundary_interpolant.cu:206-207.
+/* boundary_interpolant: CUDA_Code/2_2_WENO5RK3_Extend/boundary_interpolant.cu:102-103.
  * `lsf` is the field being extended (the reference passes C there). */
 int vls_boundary_interpolant(double *cpr, double *cpl, double *cpf, double *cpb, double *cpu, double *cpd,
                              const double *xpr, const double *xpl, const double *ypf, const double *ypb,
                              const double *zpu, const double *zpd, const double *lsf, int rows, int cols,
                              int pges, double dx, double dy, double dz, int num_ele, vls_stream_t stream);
 
-/* extend_step: CUDA_Code/2_2_WENO5RK3_Extend/extend_step.cu:262-263. */
+/* extend_step: CUDA_Code/2_2_WENO5RK3_Extend/extend_step.cu:10-11. */
 int vls_extend_step(double *step, const double *deltat, const double *lsf, const double *vx, const double *vy,
                     const double *vz, const double *xpr, const double *xpl, const double *ypf, const double *ypb,
                     const double *zpu, const double *zpd, const double *cpr, const double *cpl, const double *cpf,
@@ -160,7 +160,7 @@ int vls_weno_derivative(double *back_x, double *fore_x, double *back_y, double *
                         double *fore_z, const double *lsf, int rows, int cols, int pges, double dx, double dy,
                         double dz, vls_stream_t stream);
 
-/* set_calculus_toolbox: CUDA_Code/4_0_Calculus_ToolBox/set_calculus_toolbox.cu:101-102.
+/* set_calculus_toolbox: CUDA_Code/4_0_Calculus_ToolBox/set_calculus_toolbox.cu:9-10.
  * out18 = {Fx,Fy,Fz,FGradMag,Nx,Ny,Nz,Fxx,Fyy,Fzz,Fxy,Fyz,Fzx,FLaplacian,
  *          MeanCurvature,GaussianCurvature,Heaviside,DiracDelta} (18 device pointers). */
 int vls_set_calculus_toolbox(double *const *out18, const double *lsf, const double *hprimal, int rows, int cols,
@@ -349,7 +349,7 @@ int vls_sdf3_area_volume_device(vls_sdf3_t h, const double *F, int kind, double 
                                 vls_stream_t stream);
 
 /* Normal component u.N of a velocity with N = grad F / max(|grad F|, 1e-14)
- * from central differences (set_calculus_toolbox.cu:123-136): the field the
+ * from central differences (set_calculus_toolbox.cu:31-44): the field the
  * vesicle drivers extend off the interface (V/p00_test_singlephase_relax.m:76-84). */
 int vls_sdf3_normal_velocity(vls_sdf3_t h, const double *F, const double *vx, const double *vy, const double *vz,
                              double *out, vls_stream_t stream);

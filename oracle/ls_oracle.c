@@ -22,10 +22,13 @@
  *   - pow(x,2) is written x*x and Poly<n>'s pow(x,i) is repeated multiplication;
  *   - build with -ffp-contract=off so no fused multiply-adds are formed.
  *
- * Parity pins: tests/test_oracle_golden.py (against vectors written by the
- * reference's own kernels, compiled from its sources) and tests/test_oracle_math.py
- * (closed forms, invariants, brute force).  Each exported function is pinned by
- * at least one of them; see the per-function notes.
+ * Parity pins (no recorded outputs are needed): tests/test_oracle_math.py and
+ * tests/test_oracle_kats.py (closed forms, known answers on kinked, linear,
+ * polynomial and planar fields, orders of accuracy, invariants) and the
+ * reference's own recorded runs (tests/test_oracle_relax.py).
+ * tests/test_oracle_mutants.py plants a mistake in every function of this
+ * file and shows that one of those checks fails.  The reference's CUDA
+ * kernels are not built: they need nvcc, which this image lacks.
  */
 #include <math.h>
 #include <stdint.h>
@@ -1244,7 +1247,7 @@ void or_set_num_threads(int n)
 }
 
 /* u.N with N = grad F / max(|grad F|, 1e-14) from clamped central differences
- * (the normal of set_calculus_toolbox.cu:123-136); the field the drivers extend
+ * (the normal of set_calculus_toolbox.cu:31-44); the field the drivers extend
  * off the interface (p00_test_singlephase_relax.m:76-84). */
 void or_normal_velocity(double *out, const double *F, const double *vx, const double *vy, const double *vz,
                         int rows, int cols, int pges, double dx, double dy, double dz)

@@ -1,4 +1,4 @@
-"""Input builders for the golden vectors and the parity tests.
+"""Input builders of the parity tests.
 
 Fields are numpy float64 arrays of shape (pges, cols, rows) = (Nz, Nx, Ny), the
 reference's column-major [rows, cols, pges] layout (MATLAB meshgrid gives
@@ -61,9 +61,10 @@ def smooth_velocity(g):
     return [u, v, w]
 
 
-def golden_grids():
-    """Small grids with distinct spacings per axis so that a transposed axis
-    or a swapped spacing cannot pass."""
+def small_grids():
+    """Small grids with distinct spacings per axis, so that a transposed axis
+    or a swapped spacing cannot pass: a perturbed star, two merging spheres
+    with a kink, and a field with exact zeros on a node plane."""
     g1 = Grid(np.linspace(-1.0, 1.0, 13), np.linspace(-0.9, 0.9, 12), np.linspace(-1.1, 1.1, 11))
     g2 = Grid(np.linspace(-1.0, 1.0, 12), np.linspace(-1.0, 1.0, 12), np.linspace(-1.25, 1.25, 14))
     g3 = Grid(np.linspace(-1.0, 1.0, 11), np.linspace(-0.8, 0.8, 10), np.linspace(-1.0, 1.0, 9))

@@ -1,13 +1,13 @@
 """Parity of the HIP kernel-level drop-ins with the CPU oracle, through the C ABI.
 
-Inputs: the golden-fixture inputs (written by the reference's own kernels)
-and larger seeded cases with distinct spacings per axis.  Bar: bit-for-bit
+Inputs: small anisotropic grids (a perturbed star, two merging spheres with a
+kink, a field with exact zeros on a node plane) and larger seeded cases with
+distinct spacings per axis.  Bar: bit-for-bit
 equality.  The kernels are compiled without contraction and the oracle
 evaluates the same IEEE-754 sequence of +, -, *, /, sqrt and explicit fma, so
 any difference at all is a bug (1e-12 relative, is
 what the oracle itself meets against the reference's own kernels).
 """
-import glob
 import importlib
 import os
 
@@ -48,13 +48,8 @@ def exact(a, b, what=""):
                              f"({d.max() / scale:.1e} relative)")
 
 
-def _golden_inputs():
-    out = []
-    for path in sorted(glob.glob(os.path.join(HERE, "golden", "golden_*.npz"))):
-        z = np.load(path)
-        g = cases.Grid(z["xv"], z["yv"], z["zv"])
-        out.append((os.path.basename(path)[7:-4], g, z["F"]))
-    return out
+def _small_inputs():
+    return [(name, g, F) for name, (g, F) in sorted(cases.small_grids().items())]
 
 
 def _large_inputs():
@@ -63,7 +58,7 @@ def _large_inputs():
     return [("star44", g1, cases.star_phi0(g1)), ("kink48", g2, cases.merging_spheres(g2))]
 
 
-CASES = _golden_inputs() + _large_inputs()
+CASES = _small_inputs() + _large_inputs()
 
 
 @pytest.fixture(scope="module", params=CASES, ids=[c[0] for c in CASES])

@@ -1,6 +1,9 @@
-"""What the mathematics fixes about the oracle, for the parts no golden vector
-covers (the MATLAB wrappers: RK loops, extension, surface redistance,
-integrals) and as a second pin on the kernels.
+"""What the mathematics fixes about the oracle: closed forms, exactness on
+polynomials, convergence to known solutions and invariants, for the kernels
+and for the MATLAB wrappers (RK loops, extension, surface redistance,
+integrals).  test_oracle_kats.py adds known answers on kinked, linear and
+planar fields; test_oracle_mutants.py shows that a planted mistake in any
+oracle function fails one of these checks.
 
 Each check is chosen so that a plausible slip fails it: a wrong stencil
 offset or WENO weight breaks polynomial exactness; a sign slip in the
@@ -158,9 +161,11 @@ def test_quadratic_eno_distance_exact_for_quadratics():
     assert abs(d - 0.43 * s) < 1e-14
 
 
-def test_kink_falls_back_to_quadratic_eno():
+@pytest.mark.parametrize("v", [[-0.3, 0.2, -0.05, 0.05, 0.15, 0.25],   # v1*v2 < 0: a kink behind the crossing
+                               [-0.3, -0.2, -0.05, 0.05, -0.15, 0.25]])  # v3*v4 < 0: a kink beyond it
+def test_kink_falls_back_to_quadratic_eno(v):
     s = 0.1
-    v = np.array([-0.3, 0.2, -0.05, 0.05, 0.15, 0.25])  # v1*v2 < 0: a kink behind the crossing
+    v = np.array(v)
     assert po.six_distance(*v, s) == po.eno_quadratic_distance(v[1], v[2], v[3], v[4], s)
 
 
@@ -271,6 +276,25 @@ def test_surface_redistance_gives_geodesic_distance():
     assert err.max() < 0.15 * err0.max() and err.mean() < 5e-3, (err.max(), err.mean(), err0.max())
 
 
+@pytest.mark.parametrize("profile", ["linear", "exp"])
+def test_surface_redistance_keeps_the_zero_set(profile):
+    # A = z - z0 (or exp(z) - exp(z0)) with z0 between nodes: on the sphere
+    # r0 = 0.6 the geodesic distance to the circle A = 0 is r0 (elev - elev0).
+    # Next to that circle the result is accurate to ~1e-3 only when A's
+    # reinitialization uses A's own subcell distances and mask
+    # (WENO5RK3ClosetPointSurfaceRedistance.m:30-52).
+    g, r, F = _sphere(28, 0.6)
+    z0 = g.zv[14] + 0.37 * g.dz
+    A = g.Z - z0 if profile == "linear" else np.exp(g.Z) - np.exp(z0)
+    An = po.surface_redistance(np.ascontiguousarray(A), F, 20, 10, *g.spacing)
+    elev = np.arcsin(np.clip(g.Z / np.maximum(r, 1e-12), -1, 1))
+    want = 0.6 * (elev - np.arcsin(z0 / 0.6))
+    near = (np.abs(F) < 1.5 * g.ds) & (np.abs(want) < 2 * g.dz)
+    assert near.sum() > 100
+    err = np.abs(An - want)[near]
+    assert err.max() < 3e-3 and err.mean() < 1e-3, (err.max(), err.mean())
+
+
 # --------------------------------------------------------------------------
 # advection (advection_step.cu + the Shu-Osher RK3 of the wrappers)
 # --------------------------------------------------------------------------
@@ -314,6 +338,18 @@ def test_towers_area_volume_converge():
     assert errs[0][1] < 2e-4 and errs[1][1] < 2e-4, errs
 
 
+def test_towers_integrals_of_a_scaled_distance():
+    # the Towers delta of F integrates to the area weighted by 1/|grad F|
+    # (int delta(F) dV = int_Gamma dS/|grad F|): half the area for F = 2 d, with
+    # contributions from points with d in [Ds/2, Ds) that only the |F| < 2Ds
+    # mask of SDF3.m:75 keeps; H and the volume do not depend on the scale
+    g, r, _ = _sphere(48, 0.6)
+    F = np.ascontiguousarray(2.0 * (r - 0.6))
+    a, v = po.area_volume_towers(F, *g.spacing)
+    assert abs(a / (0.5 * 4 * math.pi * 0.36) - 1) < 1e-3
+    assert abs(v / (4 * math.pi * 0.216 / 3) - 1) < 2e-4
+
+
 def test_cosine_area_volume_match_closed_forms():
     # continuum integrals of the cosine-smoothed delta/H of half-width eps for
     # the exact distance of a sphere of radius R (derived, not in the reference):
@@ -332,7 +368,12 @@ def test_cosine_area_volume_match_closed_forms():
 
 def test_compensated_sums_against_fsum():
     g = cases.Grid(np.linspace(-1, 1, 30), np.linspace(-1, 1, 26), np.linspace(-1, 1, 22))
-    F = np.ascontiguousarray(RNG.normal(size=g.shape) * 0.01)  # every point inside the band
+    F = RNG.normal(size=g.shape) * 0.01  # every point inside the band but three
+    # two huge values (outside the band) give the points around them deltas
+    # some 1e15 times the others: summed in plain order the small terms that
+    # follow lose their low bits (about 2e-13 of the total)
+    F[5, 7, 8], F[5, 7, 10] = 1e14, -1e14
+    F = np.ascontiguousarray(F)
     eps = 0.05
     a, v = po.area_volume_cosine(F, *g.spacing, eps)
     # recompute the summands with numpy (periodic central differences) and sum exactly
@@ -371,9 +412,11 @@ def test_cubic_distance_exact_for_linear_data():
     assert abs(po.cubic_distance(*v, s) - (0.12 / 1.9 - s)) < 1e-15
 
 
-def test_cubic_distance_kink_falls_back_to_quadratic_eno():
+@pytest.mark.parametrize("v", [[-0.2, 0.05, -0.05, -0.1],  # v0*v1 < 0: a kink behind the crossing
+                               [0.2, 0.05, -0.05, 0.1]])   # v2*v3 < 0: a kink beyond it
+def test_cubic_distance_kink_falls_back_to_quadratic_eno(v):
     s = 0.1
-    v = np.array([-0.2, 0.05, -0.05, 0.1])  # v0*v1 < 0: a kink behind the crossing
+    v = np.array(v)
     assert po.cubic_distance(*v, s) == po.eno_quadratic_distance(*v, s)
 
 
@@ -596,7 +639,7 @@ def test_aux_toolbox_tail_is_gd3_periodic_differencing():
     # AFDiracDelta / ADiracDeltaDn of GPUAsetCalculusToolBox.m:57-63 written
     # with np.roll as @GD3's circshift maps (GD3.m:112-117): Field(oXo) is the
     # right neighbour, wrapping at the box face
-    g = cases.golden_grids()["star"][0]
+    g = cases.small_grids()["star"][0]
     F = cases.star_phi0(g)
     A = np.ascontiguousarray(0.7 * g.Z + 0.2 * g.X * g.Y - 0.1)
     ft = po.set_calculus_toolbox(F, np.maximum(F, 0), *g.spacing, g.ds)
