@@ -9,6 +9,7 @@ sequence (the CPU oracle states the same one).  The flag
 emitting packed FP32 instructions; the host pass warns that it does not know
 the feature, which is harmless.
 """
+import concurrent.futures
 import os
 import subprocess
 
@@ -33,21 +34,22 @@ FLAGS = [
 
 
 def build(verbose=False):
-    objs = []
+    jobs = []
     for src in SOURCES:
         obj = os.path.join(CSRC, src.replace(".hip", ".o"))
-        cmd = [HIPCC, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj] + FLAGS
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
-        objs.append(obj)
+        jobs.append((obj, [HIPCC, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj] + FLAGS))
     for src in CXX_SOURCES:
         obj = os.path.join(CSRC, src.replace(".cpp", ".o"))
-        cmd = ["g++", "-c", os.path.join(CSRC, src), "-o", obj, "-O2", "-std=c++17", "-fPIC", "-Wall"]
-        if verbose:
+        jobs.append((obj, ["g++", "-c", os.path.join(CSRC, src), "-o", obj, "-O2", "-std=c++17", "-fPIC", "-Wall"]))
+    if verbose:
+        for _, cmd in jobs:
             print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
-        objs.append(obj)
+    # the translation units are independent: compile them side by side
+    workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with concurrent.futures.ThreadPoolExecutor(workers) as ex:
+        for f in [ex.submit(subprocess.run, cmd, check=True) for _, cmd in jobs]:
+            f.result()
+    objs = [obj for obj, _ in jobs]
     tmp = LIB_PATH + ".tmp"
     cmd = [HIPCC, "-shared", "-o", tmp] + objs + ["--offload-arch=gfx950", "-fPIC", "-Wl,-rpath,/opt/rocm/lib",
                                                   "-L/opt/rocm/lib", "-lrocfft", "-lz"]

@@ -229,6 +229,7 @@ struct vls_sdf3 {
     double* partial = nullptr;
     double* result = nullptr;
     GraphCache graphs;
+    bool dist_ready = false;  // a set_distance has filled d, min_dist, flags and the list
     double* adj = nullptr;  // adjoint/tangent workspace, grown on demand
     size_t adj_len = 0;
 };
@@ -349,6 +350,7 @@ int sdf3_weno5_rk3_reinit(vls_sdf3* h, T* F, int iterations, hipStream_t s)
 {
     VLS_REQUIRE(h && F, "null argument");
     VLS_REQUIRE(iterations >= 0, "negative iteration count");
+    VLS_REQUIRE(h->dist_ready, "set_distance must run before the reinitialization (it reads its distances)");
     return with_graph(h->graphs, make_key(1, sizeof(T), F, iterations), s, [&](hipStream_t cs) -> int {
         VLS_HIP(launch_set_mask_bit<T>(h->flags, F, h->g, cs));
         return reinit_loop<T>(h, F, iterations, main_set(h), cs);
@@ -360,6 +362,7 @@ int sdf3_weno5_rk3_extend(vls_sdf3* h, const T* F, T* C, int iterations, hipStre
 {
     VLS_REQUIRE(h && F && C, "null argument");
     VLS_REQUIRE(iterations >= 0, "negative iteration count");
+    VLS_REQUIRE(h->dist_ready, "set_distance must run before the extension (it reads F's distances)");
     VLS_REQUIRE((const void*)F != (const void*)C, "C must not alias F");
     const Geom& g = h->g;
     const DSet ms = main_set(h);
@@ -915,7 +918,8 @@ int vls_sdf3_create(vls_sdf3_t* out, int rows, int cols, int pges, double dx, do
         delete h;
         return fail(VLS_ERR_ALLOC, "hipMalloc of point lists failed");
     }
-    if (hipMemset(h->int_pool, 0, sizeof(int) * ((size_t)n * 4 + 16)) != hipSuccess) {
+    if (hipMemset(h->int_pool, 0, sizeof(int) * ((size_t)n * 4 + 16)) != hipSuccess ||
+        hipMemset(h->flag_pool, 0, (size_t)n * 3) != hipSuccess) {
         (void)hipFree(h->pool);
         (void)hipFree(h->flag_pool);
         (void)hipFree(h->int_pool);
@@ -988,13 +992,17 @@ int vls_sdf3_destroy(vls_sdf3_t h)
 int vls_sdf3_set_distance(vls_sdf3_t h, const double* F, vls_stream_t stream)
 {
     VLS_REQUIRE(h && F, "null argument");
-    return set_distance_into<double>(h, 6, main_set(h), h->min_dist, F, nullptr, S(stream));
+    const int rc = set_distance_into<double>(h, 6, main_set(h), h->min_dist, F, nullptr, S(stream));
+    if (rc == VLS_OK) h->dist_ready = true;
+    return rc;
 }
 
 int vls_sdf3_set_distance_f32(vls_sdf3_t h, const float* F, vls_stream_t stream)
 {
     VLS_REQUIRE(h && F, "null argument");
-    return set_distance_into<float>(h, 6, main_set(h), h->min_dist, F, nullptr, S(stream));
+    const int rc = set_distance_into<float>(h, 6, main_set(h), h->min_dist, F, nullptr, S(stream));
+    if (rc == VLS_OK) h->dist_ready = true;
+    return rc;
 }
 
 int vls_sdf3_distances(vls_sdf3_t h, const double** out7)
@@ -1258,6 +1266,7 @@ struct vls_batch {
     double* v[3] = {nullptr, nullptr, nullptr};
     double* partial = nullptr;  // per-tile partial sums (4 per tile of either shape)
     GraphCache graphs;
+    bool list_ready = false;  // a set_distance (or step) has built the interface list
 
     BatchView view() const
     {
@@ -1484,7 +1493,8 @@ int vls_batch_create(vls_batch_t* out, int ngrids, const int* rows, const int* c
     }
     if (hipMemcpy(b->d_offs, b->offs.data(), sizeof(int64_t) * ((size_t)ngrids + 1), hipMemcpyHostToDevice) !=
             hipSuccess ||
-        hipMemset(b->int_pool, 0, sizeof(int) * ((size_t)n * 2 + 16)) != hipSuccess) {
+        hipMemset(b->int_pool, 0, sizeof(int) * ((size_t)n * 3 + 16)) != hipSuccess ||
+        hipMemset(b->flags, 0, (size_t)n) != hipSuccess) {
         delete b;
         return fail(VLS_ERR_HIP, "copying the batch descriptors failed");
     }
@@ -1588,12 +1598,15 @@ int vls_batch_get_zmarch_depth(vls_batch_t b, int* depth)
     int vls_batch_set_distance##SFX(vls_batch_t b, const T* F, vls_stream_t stream)                                \
     {                                                                                                              \
         VLS_REQUIRE(b && F, "null argument");                                                                      \
-        return batch_set_distance<T>(b, F, (const T*)nullptr, S(stream));                                          \
+        const int rc = batch_set_distance<T>(b, F, (const T*)nullptr, S(stream));                                  \
+        if (rc == VLS_OK) b->list_ready = true;                                                                    \
+        return rc;                                                                                                 \
     }                                                                                                              \
     int vls_batch_weno5_rk3_reinit##SFX(vls_batch_t b, T* F, int iterations, vls_stream_t stream)                  \
     {                                                                                                              \
         VLS_REQUIRE(b && F, "null argument");                                                                      \
         VLS_REQUIRE(iterations >= 0, "negative iteration count");                                                  \
+        VLS_REQUIRE(b->list_ready, "set_distance must run before the reinitialization");                           \
         return with_graph(b->graphs, make_key(11, sizeof(T), F, iterations), S(stream),                            \
                           [&](hipStream_t cs) -> int { return batch_reinit_loop<T>(b, F, iterations, cs); });      \
     }                                                                                                              \
@@ -1602,6 +1615,7 @@ int vls_batch_get_zmarch_depth(vls_batch_t b, int* depth)
         VLS_REQUIRE(b && F && C, "null argument");                                                                 \
         VLS_REQUIRE(iterations >= 0, "negative iteration count");                                                  \
         VLS_REQUIRE((const void*)F != (const void*)C, "C must not alias F");                                       \
+        VLS_REQUIRE(b->list_ready, "set_distance must run before the extension");                                  \
         return with_graph(b->graphs, make_key(12, sizeof(T), F, C, iterations), S(stream),                         \
                           [&](hipStream_t cs) -> int { return batch_extend<T>(b, F, C, iterations, cs); });        \
     }                                                                                                              \
@@ -1658,12 +1672,13 @@ int vls_batch_get_zmarch_depth(vls_batch_t b, int* depth)
         VLS_REQUIRE(ext_iters >= 0 && reinit_iters >= 0, "negative iteration count");                              \
         VLS_REQUIRE(eps_factor > 0, "eps_factor must be positive");                                                \
         VLS_REQUIRE((const void*)F != (const void*)Vn, "Vn must not alias F");                                     \
-        return with_graph(b->graphs,                                                                               \
-                          make_key(14, sizeof(T), F, Vn, vx, vy, vz, dt, ext_iters, reinit_iters, eps_factor, av), \
-                          S(stream), [&](hipStream_t cs) -> int {                                                  \
-                              return batch_step<T>(b, F, Vn, vx, vy, vz, dt, ext_iters, reinit_iters, eps_factor,  \
-                                                   av, cs);                                                        \
-                          });                                                                                      \
+        const int rc = with_graph(                                                                                 \
+            b->graphs, make_key(14, sizeof(T), F, Vn, vx, vy, vz, dt, ext_iters, reinit_iters, eps_factor, av),    \
+            S(stream), [&](hipStream_t cs) -> int {                                                                \
+                return batch_step<T>(b, F, Vn, vx, vy, vz, dt, ext_iters, reinit_iters, eps_factor, av, cs);       \
+            });                                                                                                    \
+        if (rc == VLS_OK) b->list_ready = true;                                                                    \
+        return rc;                                                                                                 \
     }
 
 VLS_BATCH_ENTRIES(double, )
@@ -1696,9 +1711,19 @@ struct EventTimer {
 template <typename T>
 int batch_autotune(vls_batch* b, const T* F, int iterations, hipStream_t s, int* tr_out, int* blocks_out)
 {
+    // candidates rebuild the work list the cached graphs point into: drop
+    // them first, and on any failure go back to the shape the batch had
+    b->graphs.clear();
+    const int tr0 = b->tr, zd0 = b->zd, blocks0 = b->blocks;
+    auto restore = [&](int rc) -> int {
+        b->zd = zd0;
+        if (build_tiles(b, tr0) == VLS_OK) b->blocks = blocks0;
+        b->graphs.clear();
+        return rc;
+    };
     T* C = nullptr;
     if (hipMalloc((void**)&C, sizeof(T) * (size_t)b->total) != hipSuccess)
-        return fail(VLS_ERR_ALLOC, "hipMalloc of the autotuning field failed");
+        return restore(fail(VLS_ERR_ALLOC, "hipMalloc of the autotuning field failed"));
     EventTimer ev;
     int rc = VLS_OK;
     float best = 1e30f;
@@ -1752,19 +1777,20 @@ int batch_autotune(vls_batch* b, const T* F, int iterations, hipStream_t s, int*
     }
     (void)hipStreamSynchronize(s);
     (void)hipFree(C);
-    if (rc) return rc;
+    if (rc) return restore(rc);
     b->zd = best_zd;
-    if ((rc = build_tiles(b, best_tr))) return rc;
+    if ((rc = build_tiles(b, best_tr))) return restore(rc);
     b->blocks = default_blocks(b->ntiles);
     const int best_blocks = b->blocks;
     b->graphs.clear();
+    b->list_ready = true;  // the last candidate ran set_distance on F
     if (tr_out) *tr_out = best_tr;
     if (blocks_out) *blocks_out = best_blocks;
     return VLS_OK;
 }
 
 template <typename T>
-int sdf3_autotune(vls_sdf3* h, const T* F, int iterations, hipStream_t s, int* tr_out)
+int sdf3_autotune_try(vls_sdf3* h, const T* F, int iterations, hipStream_t s, int* tr_out)
 {
     EventTimer ev;
     VLS_HIP(hipEventCreate(&ev.a));
@@ -1819,8 +1845,24 @@ int sdf3_autotune(vls_sdf3* h, const T* F, int iterations, hipStream_t s, int* t
     h->g.tr = best_tr;
     h->g.zd = best_zd;
     h->graphs.clear();
+    h->dist_ready = true;  // every candidate ran set_distance on F
     if (tr_out) *tr_out = best_tr;
     return VLS_OK;
+}
+
+// on a failure the handle keeps the shape it had (no cached graph survives)
+template <typename T>
+int sdf3_autotune(vls_sdf3* h, const T* F, int iterations, hipStream_t s, int* tr_out)
+{
+    const int tr0 = h->g.tr, zd0 = h->g.zd;
+    h->graphs.clear();
+    const int rc = sdf3_autotune_try<T>(h, F, iterations, s, tr_out);
+    if (rc != VLS_OK) {
+        h->g.tr = tr0;
+        h->g.zd = zd0;
+        h->graphs.clear();
+    }
+    return rc;
 }
 
 }  // namespace

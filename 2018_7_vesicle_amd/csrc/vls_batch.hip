@@ -167,12 +167,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(stage_
     const Inv invs = inv_of(G);
     const idx_t off = G.off, plane = (idx_t)b.rows * b.cols;
     T* o = out + off;
-    zmarch<R, T, false>(lds[0], lds[1], b, it.r0, it.c0, it.p0, it.p1, fs + off, shift(fn, off), nullptr, nullptr,
+    zmarch<R, T, false, true>(lds[0], lds[1], b, it.r0, it.c0, it.p0, it.p1, fs + off, shift(fn, off), nullptr, nullptr,
                         nullptr, flags + off, stage,
                         [&](const T* l, const ZCol<R>& zc, const T w[7], const ZStep<R, T, false>& st, int p) {
                             bool cross;
-                            const T res = reinit_value<R>(l, zc.base, w, st.fn, st.fl, invs, stage, cross);
-                            const bool bulk = !(st.fl & FLAG_SUBCELL);
+                            const T res = reinit_value<R>(l, zc.base, w, st.fn, st.flags(zc), invs, stage, cross);
+                            const bool bulk = !(st.flags(zc) & FLAG_SUBCELL);
                             if (bulk && !cross) stp(o + (idx_t)p * plane, zc.in, res);
                             wave_append(bulk && cross, (int)(off + (idx_t)p * plane + zc.in), over,
                                         &over_count[seq & 1]);
@@ -237,11 +237,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(stage_
     const Inv inv = inv_of(G);
     const idx_t off = G.off, plane = (idx_t)b.rows * b.cols;
     T* o = out + off;
-    zmarch<R, T, true>(lds[0], lds[1], b, it.r0, it.c0, it.p0, it.p1, cs + off, shift(cn, off), vx + off, vy + off,
+    zmarch<R, T, true, true>(lds[0], lds[1], b, it.r0, it.c0, it.p0, it.p1, cs + off, shift(cn, off), vx + off, vy + off,
                        vz + off, flags + off, stage,
                        [&](const T* l, const ZCol<R>& zc, const T w[7], const ZStep<R, T, true>& st, int p) {
                            const T res = upwind_value<R>(l, zc.base, w, st.vel, st.fn, (T)inv.dt_bulk, inv, stage);
-                           if (!(st.fl & FLAG_SUBCELL)) stp(o + (idx_t)p * plane, zc.in, res);
+                           if (!(st.flags(zc) & FLAG_SUBCELL)) stp(o + (idx_t)p * plane, zc.in, res);
                        });
 }
 
@@ -259,7 +259,7 @@ __global__ void __launch_bounds__(256) kb_advect_stage(T* __restrict__ out, cons
     const T dtg = (T)dt[it.g];
     const idx_t off = G.off, plane = (idx_t)b.rows * b.cols;
     T* o = out + off;
-    zmarch<R, T, true>(lds[0], lds[1], b, it.r0, it.c0, it.p0, it.p1, fs + off, shift(fn, off), vx + off, vy + off,
+    zmarch<R, T, true, false>(lds[0], lds[1], b, it.r0, it.c0, it.p0, it.p1, fs + off, shift(fn, off), vx + off, vy + off,
                        vz + off, nullptr, stage,
                        [&](const T* l, const ZCol<R>& zc, const T w[7], const ZStep<R, T, true>& st, int p) {
                            const T res = upwind_value<R>(l, zc.base, w, st.vel, st.fn, dtg, inv, stage);

@@ -210,8 +210,15 @@ __device__ __forceinline__ void weno5_pair(float& d_fore, float& d_back, const f
 }
 
 // a*b < 0 without forming the product (an FP32 product of two tiny values
-// can underflow to -0 and hide the sign change the FP64 test sees)
-__device__ __forceinline__ bool opposite(float a, float b) { return (a < 0.f) ? (b > 0.f) : (a > 0.f && b < 0.f); }
+// can underflow to -0 and hide the sign change the FP64 test sees), as one
+// value: min(a,b) < 0 < max(a,b) exactly when max(min(a,b), -max(a,b)) < 0
+// (a zero of either sign gives >= 0).  Computed without branches, so the
+// test costs a few VALU operations and no scalar mask arithmetic.
+__device__ __forceinline__ float opposite_key(float a, float b)
+{
+    return __builtin_fmaxf(__builtin_fminf(a, b), -__builtin_fmaxf(a, b));
+}
+__device__ __forceinline__ bool opposite(float a, float b) { return opposite_key(a, b) < 0.f; }
 
 // ---------------------------------------------------------------------------
 // Nonuniform three-stencil WENO at a node next to the interface:

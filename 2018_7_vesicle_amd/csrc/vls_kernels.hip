@@ -187,11 +187,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(stage_
     int r0, c0, p0, p1;
     zmarch_share<R>(b, depth, blk - niface, (int)gridDim.x - niface, r0, c0, p0, p1);
     const idx_t plane = (idx_t)b.rows * b.cols;
-    zmarch<R, T, false>(lds[0], lds[1], b, r0, c0, p0, p1, fs, fn, nullptr, nullptr, nullptr, flags, stage,
+    zmarch<R, T, false, true>(lds[0], lds[1], b, r0, c0, p0, p1, fs, fn, nullptr, nullptr, nullptr, flags, stage,
                         [&](const T* l, const ZCol<R>& zc, const T w[7], const ZStep<R, T, false>& st, int p) {
                             bool cross;
-                            const T res = reinit_value<R>(l, zc.base, w, st.fn, st.fl, invs, stage, cross);
-                            const bool bulk = !(st.fl & FLAG_SUBCELL);
+                            const T res = reinit_value<R>(l, zc.base, w, st.fn, st.flags(zc), invs, stage, cross);
+                            const bool bulk = !(st.flags(zc) & FLAG_SUBCELL);
                             if (bulk && !cross) stp(out + (idx_t)p * plane, zc.in, res);
                             wave_append(bulk && cross, (int)((idx_t)p * plane + zc.in), over, &over_count[seq & 1]);
                         });
@@ -490,10 +490,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(stage_
     int r0, c0, p0, p1;
     zmarch_share<R>(b, depth, blk - niface, (int)gridDim.x - niface, r0, c0, p0, p1);
     const idx_t plane = (idx_t)b.rows * b.cols;
-    zmarch<R, T, true>(lds[0], lds[1], b, r0, c0, p0, p1, cs, cn, vx, vy, vz, flags, stage,
+    zmarch<R, T, true, true>(lds[0], lds[1], b, r0, c0, p0, p1, cs, cn, vx, vy, vz, flags, stage,
                        [&](const T* l, const ZCol<R>& zc, const T w[7], const ZStep<R, T, true>& st, int p) {
                            const T res = upwind_value<R>(l, zc.base, w, st.vel, st.fn, (T)inv.dt_bulk, inv, stage);
-                           if (!(st.fl & FLAG_SUBCELL)) stp(out + (idx_t)p * plane, zc.in, res);
+                           if (!(st.flags(zc) & FLAG_SUBCELL)) stp(out + (idx_t)p * plane, zc.in, res);
                        });
 }
 
@@ -537,7 +537,7 @@ __global__ void __launch_bounds__(256) k_advect_stage(T* __restrict__ out, const
     int r0, c0, p0, p1;
     zmarch_share<R>(b, depth, (int)blockIdx.x, (int)gridDim.x, r0, c0, p0, p1);
     const idx_t plane = (idx_t)b.rows * b.cols;
-    zmarch<R, T, true>(lds[0], lds[1], b, r0, c0, p0, p1, fs, fn, vx, vy, vz, nullptr, stage,
+    zmarch<R, T, true, false>(lds[0], lds[1], b, r0, c0, p0, p1, fs, fn, vx, vy, vz, nullptr, stage,
                        [&](const T* l, const ZCol<R>& zc, const T w[7], const ZStep<R, T, true>& st, int p) {
                            const T res = upwind_value<R>(l, zc.base, w, st.vel, st.fn, (T)dt, inv, stage);
                            if (zc.valid) stp(out + (idx_t)p * plane, zc.in, res);

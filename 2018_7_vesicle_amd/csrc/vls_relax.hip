@@ -21,6 +21,7 @@
 #include <rocfft/rocfft.h>
 
 #include <map>
+#include <cstdint>
 #include <mutex>
 #include <tuple>
 
@@ -242,8 +243,10 @@ __global__ void k_normal_speed_tp(double* __restrict__ out, const double* __rest
 }
 
 // ---------------------------------------------------------------------------
-// rocFFT plans, one pair per (device, grid size), with the complex buffer the
-// transforms run in and the library's work buffer
+// rocFFT plans, one pair per (device, stream, grid size), with the complex
+// buffer the transforms run in and the library's work buffer.  Keying by the
+// stream gives calls on different streams buffers of their own, so they never
+// overwrite each other's data; calls on one stream are ordered by it.
 // ---------------------------------------------------------------------------
 struct FftPlan {
     rocfft_plan fwd = nullptr, inv = nullptr;
@@ -261,16 +264,16 @@ struct FftPlan {
 };
 
 std::mutex g_fft_mu;
-std::map<std::tuple<int, int, int, int>, FftPlan*> g_fft;
+std::map<std::tuple<int, uintptr_t, int, int, int>, FftPlan*> g_fft;
 bool g_rocfft_ready = false;
 
-hipError_t fft_plan(const Geom& g, FftPlan*& out)
+hipError_t fft_plan(const Geom& g, hipStream_t s, FftPlan*& out)
 {
     std::lock_guard<std::mutex> lock(g_fft_mu);
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return e;
-    const auto key = std::make_tuple(dev, g.rows, g.cols, g.pges);
+    const auto key = std::make_tuple(dev, (uintptr_t)s, g.rows, g.cols, g.pges);
     auto it = g_fft.find(key);
     if (it != g_fft.end()) { out = it->second; return hipSuccess; }
     if (!g_rocfft_ready) {
@@ -326,7 +329,7 @@ hipError_t launch_smooth(double* out, const double* rhs, const Geom& g, double c
                          hipStream_t s)
 {
     FftPlan* p = nullptr;
-    hipError_t e = fft_plan(g, p);
+    hipError_t e = fft_plan(g, s, p);
     if (e != hipSuccess) return e;
     const idx_t n = (idx_t)g.n();
     const dim3 flat((unsigned)((n + 255) / 256));

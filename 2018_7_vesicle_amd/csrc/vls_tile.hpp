@@ -56,9 +56,14 @@ __device__ __forceinline__ float rk_combine(int stage, float fs, float step, flo
     return fmaf(1.f / 3.f, fn, (2.f / 3.f) * t);
 }
 
-// the interface test of re_step.cu:260-267 on a pair of nodal values
-__device__ __forceinline__ bool sign_change(double a, double b) { return a * b < 0; }
-__device__ __forceinline__ bool sign_change(float a, float b) { return opposite(a, b); }
+// the interface test of re_step.cu:260-267 at the middle node of v[2..4]:
+// a sign change between it and either neighbour (FP64: the reference's
+// product test; FP32: the sign test of opposite_key, one compare for both)
+__device__ __forceinline__ bool sign_change3(const double v[7]) { return (v[2] * v[3] < 0) | (v[3] * v[4] < 0); }
+__device__ __forceinline__ bool sign_change3(const float v[7])
+{
+    return __builtin_fminf(opposite_key(v[2], v[3]), opposite_key(v[3], v[4])) < 0.f;
+}
 
 // append `take` lanes' values to list[*count ...] with one atomic per wave
 __device__ __forceinline__ void wave_append(bool take, int value, int* __restrict__ list, int* __restrict__ count)
@@ -492,40 +497,48 @@ __device__ __forceinline__ ZCol<R> zcol(const Box& b, int r0, int c0)
 }
 
 // the loads of one page: the staged tile, the point's velocity, start value
-// and flags, and the z-value the window takes in at this page
+// and flags, and the z-value the window takes in at this page.  The loaded
+// values are kept as loaded; nothing is computed from them until the page is
+// evaluated, so the loads of page p+1 stay in flight while page p is (an
+// operation on them here would make the compiler wait for every load first).
 template <int R, typename T, bool VEL>
 struct ZStep {
     TileRegs<R, T> reg;
     T vel[VEL ? 3 : 1];
-    T fn, w;
-    uint8_t fl;
+    T fn, w;  // fn: the start value (at stage 0 the point's own value, unused)
+    uint8_t fl;  // the point's flags as loaded
+    // the point's flags, FLAG_SUBCELL for a lane outside the box (never written)
+    __device__ __forceinline__ uint8_t flags(const ZCol<R>& zc) const { return zc.valid ? fl : (uint8_t)FLAG_SUBCELL; }
 };
 
-template <int R, typename T, bool VEL>
+// Every load is issued by every lane on every path: the lanes without a tile
+// element to stage load element 0 of the page (their offsets are 0), and the
+// march always loads a next page (the last page again at its end).  The
+// compiler's wait counts then do not depend on the path taken, so a page's
+// evaluation never waits for the loads of the page after it.  FL: the point
+// flags are loaded (else 0).
+template <int R, typename T, bool VEL, bool FL>
 __device__ __forceinline__ void zstep_load(ZStep<R, T, VEL>& s, const ZCol<R>& zc, int p, const Box& b, idx_t plane,
                                            const T* __restrict__ fs, const T* __restrict__ fn,
                                            const T* __restrict__ vx, const T* __restrict__ vy,
                                            const T* __restrict__ vz, const uint8_t* __restrict__ flags, int stage)
 {
-    using S = Shape<R>;
     const idx_t po = (idx_t)p * plane;
     const T* fp = fs + po;
-    const int tx = tile_tx<R>();
 #pragma unroll
-    for (int k = 0; k < TileRegs<R, T>::NK; ++k)
-        if (tx + S::TC * k < S::LC) s.reg.m[k] = ldp(fp, zc.m[k]);
-    if ((int)threadIdx.x < 2 * HALO * S::LC) s.reg.s = ldp(fp, zc.s);
+    for (int k = 0; k < TileRegs<R, T>::NK; ++k) s.reg.m[k] = ldp(fp, zc.m[k]);
+    s.reg.s = ldp(fp, zc.s);
     s.w = ldp(fs + (idx_t)min(p + 3, b.pges - 1) * plane, zc.in);
     if (VEL) {
         s.vel[0] = ldp(vx + po, zc.in);
         s.vel[1] = ldp(vy + po, zc.in);
         s.vel[2] = ldp(vz + po, zc.in);
     }
-    // at stage 0 (no start value) the point's own value is read and dropped
-    const T f0 = ldp((stage != 0 ? fn : fs) + po, zc.in);
-    s.fn = (stage != 0) ? f0 : (T)0;
-    const uint8_t f = VEL && !flags ? (uint8_t)0 : ldp(flags + po, zc.in);
-    s.fl = zc.valid ? f : (uint8_t)FLAG_SUBCELL;
+    // at stage 0 (no start value) the point's own value is read and not used
+    // (rk_combine does not read fn at stage 0)
+    s.fn = ldp((stage != 0 ? fn : fs) + po, zc.in);
+    if (FL) s.fl = ldp(flags + po, zc.in);
+    else s.fl = 0;
 }
 
 template <typename T>
@@ -537,7 +550,7 @@ __device__ __forceinline__ void window_shift(T w[7], T in)
 }
 
 // body(lds, zc, w, step, p) evaluates page p: x- and y-taps from lds, z-taps w
-template <int R, typename T, bool VEL, class Body>
+template <int R, typename T, bool VEL, bool FL, class Body>
 __device__ __forceinline__ void zmarch(T* lds0, T* lds1, const Box& b, int r0, int c0, int p0, int p1,
                                        const T* __restrict__ fs, const T* __restrict__ fn,
                                        const T* __restrict__ vx, const T* __restrict__ vy,
@@ -552,18 +565,18 @@ __device__ __forceinline__ void zmarch(T* lds0, T* lds1, const Box& b, int r0, i
 #pragma unroll
     for (int k = 0; k < 7; ++k) w[k] = ldp(fs + (idx_t)clampi(p0 - 4 + k, b.pges - 1) * plane, zc.in);
     ZStep<R, T, VEL> sa, sb;
-    zstep_load(sa, zc, p0, b, plane, fs, fn, vx, vy, vz, flags, stage);
+    zstep_load<R, T, VEL, FL>(sa, zc, p0, b, plane, fs, fn, vx, vy, vz, flags, stage);
     for (int p = p0;;) {
         window_shift(w, sa.w);
         store_tile<R>(lds0, sa.reg);
         __syncthreads();
-        if (p + 1 < p1) zstep_load(sb, zc, p + 1, b, plane, fs, fn, vx, vy, vz, flags, stage);
+        zstep_load<R, T, VEL, FL>(sb, zc, min(p + 1, p1 - 1), b, plane, fs, fn, vx, vy, vz, flags, stage);
         body(lds0, zc, w, sa, p);
         if (++p >= p1) break;
         window_shift(w, sb.w);
         store_tile<R>(lds1, sb.reg);
         __syncthreads();
-        if (p + 1 < p1) zstep_load(sa, zc, p + 1, b, plane, fs, fn, vx, vy, vz, flags, stage);
+        zstep_load<R, T, VEL, FL>(sa, zc, min(p + 1, p1 - 1), b, plane, fs, fn, vx, vy, vz, flags, stage);
         body(lds1, zc, w, sb, p);
         if (++p >= p1) break;
     }
@@ -612,7 +625,7 @@ __device__ __forceinline__ T reinit_value(const T* lds, int base, const T z[7], 
         else
 #pragma unroll
             for (int k = 0; k < 7; ++k) v[k] = z[k];
-        cross = cross || (sign_change(v[2], v[3]) || sign_change(v[3], v[4]));
+        cross |= sign_change3(v);
         weno5_pair(fo[a], bk[a], v, axis_arg(invs, a, T()));
     }
     const T step = godunov_step((fl & FLAG_MASK) != 0, fo[0], bk[0], fo[1], bk[1], fo[2], bk[2], (T)invs.dt_bulk);

@@ -65,8 +65,10 @@ def parse():
     ap.add_argument("--batch-steps", type=int, default=10,
                     help="timed steps of the 512-vesicle batch comparison (0 skips it)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 96^3 adjoint gradient check")
-    ap.add_argument("--probe", choices=["extend", "batch", "variants", "toolbox", "relax"], default=None,
+    ap.add_argument("--probe", choices=["extend", "stages", "batch", "variants", "toolbox", "relax"], default=None,
                     help="extend: a few extension stages and a 256 MiB device copy (for rocprofv3 --pmc); "
+                         "stages: 3 launches each of the extension, reinitialization and advection stage "
+                         "kernels at 128^3 FP64 and 256^3 FP32 (for rocprofv3 --pmc); "
                          "batch: a few steps of the 512-vesicle FP32 batch (for rocprofv3 --kernel-trace); "
                          "variants: the WENO-RK3 / ENO-RK2 calls of the variants line (for rocprofv3 --kernel-trace); "
                          "toolbox: the calculus-toolbox calls of the toolbox line (for rocprofv3 --kernel-trace); "
@@ -100,6 +102,47 @@ def probe_extend(args):
     dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"probe": "extend", "n": args.n, "stage_launches": 9, "copy_bytes": 8 << 25}), flush=True)
+    return 0
+
+
+def probe_stages(args):
+    """A short run for hardware counters of the three fused stage kernels: the
+    config-2 pipeline at 128^3 (FP64) and the config-3 pipeline at 256^3
+    (FP32 bulk) are set up (initial reinitialization and launch-shape
+    autotuning), then each runs one extension iteration, one reinitialization
+    iteration and one advection step (3 launches of each stage kernel), with a
+    synchronization between the phases.  A plain 256 MiB float64 copy last
+    calibrates FETCH_SIZE / WRITE_SIZE.  Kernel names tell the two grids apart
+    by their field type (k_*_stage<R, double> at 128^3, <R, float> at 256^3)."""
+    import importlib
+
+    import torch
+
+    vls = importlib.import_module("2018_7_vesicle_amd")
+    pl = vls.pipeline
+    out = {"probe": "stages"}
+    for n, prec, modes in ((128, "fp64", 16), (256, "fp32", 32)):
+        cfg = pl.PipelineConfig(n=n, precision=prec, modes=modes, initial_reinit=20)
+        p = pl.Pipeline(cfg)
+        m = p.map
+        m.setDistance()
+        m.normalVelocity(*p.u, p.Vn)
+        torch.cuda.synchronize()
+        m.extend_(p.Vn, 1)
+        torch.cuda.synchronize()
+        m.reinitialize_(m.F, 1)
+        torch.cuda.synchronize()
+        m.advect_(m.F, *p.u, p.dt, 1)
+        torch.cuda.synchronize()
+        out[f"{prec}_{n}"] = {"tile_rows": m.launch()[0], "zmarch_depth": m.launch()[1], "points": n ** 3}
+        del p, m
+        torch.cuda.empty_cache()
+    src = torch.ones(1 << 25, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    out["copy_bytes"] = 8 << 25
+    print(json.dumps(out), flush=True)
     return 0
 
 
@@ -420,6 +463,8 @@ def main():
         return probe_extend(args)
     if args.probe == "batch":
         return probe_batch(args)
+    if args.probe == "stages":
+        return probe_stages(args)
     if args.probe == "variants":
         print(json.dumps({"probe": "variants", **variants(False, iterations=20)}), flush=True)
         return 0

@@ -192,7 +192,9 @@ int vls_aux_calculus_toolbox(double *const *out30, const double *A, const double
  * to rhs on the periodic box; out may alias rhs.  diffusion = 0: the
  * bilaplacian smoothers, (Idt + alpha*dt*LBiLaplacian)^-1; diffusion = 1:
  * (Idt - alpha*dt*LLaplacian)^-1.  rocFFT double-precision transforms; the
- * first call for a grid size builds and keeps its plans and buffers.
+ * first call for a grid size on a stream builds and keeps plans and a complex
+ * buffer for that (grid size, stream) until the process ends, so calls on
+ * different streams never share a buffer; calls on one stream are ordered.
  *
  * vls_smooth_fft: smoothFFT (GD3.m:453-458) / smoothDiffusionFFT (:479-484),
  * real(ifftn(fftn(rhs) ./ (1 + alpha*dt*|k|^4 or |k|^2))) with @GD3's

@@ -295,3 +295,25 @@ def test_autotune_keeps_results():
     g = grids[3]
     m = vls.SDF3(Gs[3], b.grid_view(F, 3).clone())
     assert m.autotune(1) in (32, 64)
+
+
+def test_reinit_and_extend_need_set_distance_first():
+    # they read the interface list that set_distance builds; before it exists
+    # they refuse, with no device work, instead of reading uninitialized lists
+    b, grids, Gs, F, C, u = _setup(torch.float64)
+    with pytest.raises(vls.VLSError, match="set_distance"):
+        b.reinitialize(F, 2)
+    with pytest.raises(vls.VLSError, match="set_distance"):
+        b.extend(F, C, 2)
+    b.set_distance(F)
+    b.extend(F, C, 1)
+    b.reinitialize(F, 1)
+    torch.cuda.synchronize()
+    G = Gs[0]
+    g = grids[0]
+    m = vls.SDF3(G, torch.from_numpy(np.ascontiguousarray(_fields(g, 0)[0])).cuda())
+    with pytest.raises(vls.VLSError, match="set_distance"):
+        m.WENO5RK3Reinitialization(m.F, 2)
+    m.setDistance()
+    m.WENO5RK3Reinitialization(m.F, 1)
+    torch.cuda.synchronize()
