@@ -77,6 +77,7 @@ SIGNATURES = {
     "vls_sdf3_area_volume_device_f32": [P, P, I, D, P, P],
     "vls_sdf3_normal_velocity_f32": [P] * 7,
     "vls_set_graphs": [I],
+    "vls_probe_valu_rate": [I, PD, P],
     "vls_batch_create": [P, I, P, P, P, P, P, P],
     "vls_batch_destroy": [P],
     "vls_batch_layout": [P, P, P],

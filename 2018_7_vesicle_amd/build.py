@@ -17,7 +17,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB_NAME = "libvesicle_ls.so"
 LIB_PATH = os.path.join(HERE, LIB_NAME)
-SOURCES = ["vls_kernels.hip", "vls_batch.hip", "vls_adjoint.hip", "vls_variants.hip", "vls_relax.hip", "vls_api.hip"]
+SOURCES = ["vls_kernels.hip", "vls_batch.hip", "vls_adjoint.hip", "vls_variants.hip", "vls_relax.hip", "vls_api.hip",
+           "vls_probe.hip"]
 CXX_SOURCES = ["vls_matio.cpp"]  # host-only C++ (the MAT v5 reader/writer)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -143,12 +143,17 @@ __device__ __forceinline__ void weno5_pair(S& d_fore, S& d_back, const S p[7], J
 
 // ---------------------------------------------------------------------------
 // FP32 bulk arithmetic, for the points away from the interface: the same
-// Jiang-Peng form in single precision.  The weights are homogeneous of degree
-// 0 in the s_k: they are scaled by a power of two that puts the largest in
-// [0.5, 1), which keeps every product in FP32 range for any data (products
-// only underflow, harmlessly, for indicator ratios beyond 1e19, where W is
-// floored at 1e-30 and Q -> 0).  The final division is a hardware reciprocal
-// (v_rcp_f32, 1 ulp).
+// Jiang-Peng form in single precision, with the weights written through the
+// reciprocals of the indicators: w0 : w1 : w2 = s1^2 s2^2 : 6 s0^2 s2^2 :
+// 3 s0^2 s1^2 = 1/s0^2 : 6/s1^2 : 3/s2^2 (divide by s0^2 s1^2 s2^2).  Every
+// s_k >= e > 0, so the reciprocals are finite; their squares stay in FP32
+// range while the indicators lie in [1e-19, 1e19] (second differences up to
+// ~1e9), and below that the largest weights still dominate: a weight that
+// underflows is one that would not count.  W is floored at 1e-30 so that a
+// degenerate line gives Q = 0.  The divisions are hardware reciprocals
+// (v_rcp_f32, 1 ulp); the bulk is VALU-bound, and this form needs fewer
+// instructions than scaling the indicators by a power of two and forming
+// their fourth powers.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float fmaxf_(float a, float b) { return (a < b) ? b : a; }
 __device__ __forceinline__ float fminf_(float a, float b) { return (a < b) ? a : b; }
@@ -159,16 +164,11 @@ __device__ __forceinline__ float jp_q(float a, float b, float c, float d, float 
     const float t0 = a - b, t1 = b - c, t2 = c - d;
     const float y0 = fmaf(-3.f, b, a), y1 = b + c, y2 = fmaf(3.f, c, -d);
     const float k = 3.f / 13.f;
-    float s0 = fmaf(t0, t0, fmaf(k * y0, y0, e));
-    float s1 = fmaf(t1, t1, fmaf(k * y1, y1, e));
-    float s2 = fmaf(t2, t2, fmaf(k * y2, y2, e));
-    const int ex = __builtin_amdgcn_frexp_expf(__builtin_fmaxf(s0, __builtin_fmaxf(s1, s2)));
-    s0 = __builtin_amdgcn_ldexpf(s0, -ex);
-    s1 = __builtin_amdgcn_ldexpf(s1, -ex);
-    s2 = __builtin_amdgcn_ldexpf(s2, -ex);
-    const float a0 = s0 * s0, a1 = s1 * s1, a2 = s2 * s2;
-    const float w0 = a1 * a2, w1 = 6.f * (a0 * a2), w2 = 3.f * (a0 * a1);
-    const float W = (w0 + w1) + w2;
+    const float r0 = __builtin_amdgcn_rcpf(fmaf(t0, t0, fmaf(k * y0, y0, e)));
+    const float r1 = __builtin_amdgcn_rcpf(fmaf(t1, t1, fmaf(k * y1, y1, e)));
+    const float r2 = __builtin_amdgcn_rcpf(fmaf(t2, t2, fmaf(k * y2, y2, e)));
+    const float w0 = r0 * r0, w2 = (3.f * r2) * r2;
+    const float W = fmaf(6.f * r1, r1, w0 + w2);
     const float num = fmaf(4.f * w0, t0 - t1, fmaf(2.f, w2, -W) * (t1 - t2));
     return num * __builtin_amdgcn_rcpf(__builtin_fmaxf(W, 1e-30f));
 }
@@ -366,34 +366,40 @@ __device__ __forceinline__ PairT<S> weno_nonuniform_pair(const S v[7], S r1, S l
 // ---------------------------------------------------------------------------
 // Godunov Hamiltonian residual, re_step.cu:382-392
 // ---------------------------------------------------------------------------
+// Per axis the reference takes max(min(0,l)^2, max(0,r)^2) inside and
+// max(max(0,l)^2, min(0,r)^2) outside.  The larger square is the square of
+// the larger magnitude, so each is the square of one three-way extremum:
+// min(0, l, -r) inside and max(0, l, -r) outside.  Squaring depends only on
+// the magnitude, so the result is bit-identical to the reference's form with
+// a third of the (half-rate) min/max operations.
+template <typename S>
+__device__ __forceinline__ S godunov_axis(bool inside, S l, S r)
+{
+    const S z(0.0);
+    const S m = inside ? dmin(dmin(z, l), -r) : dmax(dmax(z, l), -r);
+    return m * m;
+}
 template <typename S>
 __device__ __forceinline__ S godunov_step(bool inside, S xR, S xL, S yF, S yB, S zU, S zD, S dt)
 {
-    const S z(0.0);
-    S g;
-    if (inside)
-        g = dmax(sq(dmin(z, xL)), sq(dmax(z, xR))) + dmax(sq(dmin(z, yB)), sq(dmax(z, yF)))
-            + dmax(sq(dmin(z, zD)), sq(dmax(z, zU)));
-    else
-        g = dmax(sq(dmax(z, xL)), sq(dmin(z, xR))) + dmax(sq(dmax(z, yB)), sq(dmin(z, yF)))
-            + dmax(sq(dmax(z, zD)), sq(dmin(z, zU)));
+    const S g = godunov_axis(inside, xL, xR) + godunov_axis(inside, yB, yF) + godunov_axis(inside, zD, zU);
     const S s = (sqrt(g) - 1.0) * dt;
     return inside ? -s : s;
 }
 
 __device__ __forceinline__ float fsq(float a) { return a * a; }
 
-// FP32 form of godunov_step (bulk points)
+// FP32 form of godunov_step (bulk points): the three-way extremum of
+// godunov_axis as one v_min3 / v_max3 per axis
+__device__ __forceinline__ float godunov_axis_f(bool inside, float l, float r)
+{
+    const float m = inside ? __builtin_fminf(__builtin_fminf(0.f, l), -r) : __builtin_fmaxf(__builtin_fmaxf(0.f, l), -r);
+    return m * m;
+}
 __device__ __forceinline__ float godunov_step(bool inside, float xR, float xL, float yF, float yB, float zU,
                                               float zD, float dt)
 {
-    float g;
-    if (inside)
-        g = fmaxf_(fsq(fminf_(0.f, xL)), fsq(fmaxf_(0.f, xR))) + fmaxf_(fsq(fminf_(0.f, yB)), fsq(fmaxf_(0.f, yF)))
-            + fmaxf_(fsq(fminf_(0.f, zD)), fsq(fmaxf_(0.f, zU)));
-    else
-        g = fmaxf_(fsq(fmaxf_(0.f, xL)), fsq(fminf_(0.f, xR))) + fmaxf_(fsq(fmaxf_(0.f, yB)), fsq(fminf_(0.f, yF)))
-            + fmaxf_(fsq(fmaxf_(0.f, zD)), fsq(fminf_(0.f, zU)));
+    const float g = godunov_axis_f(inside, xL, xR) + godunov_axis_f(inside, yB, yF) + godunov_axis_f(inside, zD, zU);
     const float s = (__builtin_sqrtf(g) - 1.f) * dt;
     return inside ? -s : s;
 }

@@ -666,7 +666,7 @@ __device__ __forceinline__ ReinitLane reinit_iface_lane(const T* __restrict__ fs
         line7(fs, r, c, p, a, b, v);
         if constexpr (SCHEME == IFACE_CUBIC) reinit_axis_cubic(fo, bk, v, r1, l1, sp[a], inv.jd[a]);
         else reinit_axis<true>(fo, bk, v, r1, l1, sp[a], inv.jd[a]);
-        q.m = q.inside ? dmax(sq(dmin(0.0, bk)), sq(dmax(0.0, fo))) : dmax(sq(dmax(0.0, bk)), sq(dmin(0.0, fo)));
+        q.m = godunov_axis(q.inside, bk, fo);
     }
     return q;
 }

@@ -1,42 +1,46 @@
-"""Benchmark: the full FP64 vesicle-step pipeline at 128^3 on one MI355X.
+"""Benchmark: the vesicle-step pipeline at 256^3, FP32 bulk beside plain FP64,
+on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--main-only]
 
-Workload (one "step"): WENO5 + TVD-RK3 advection of F and psi, setDistance,
-normal velocity and its 40-iteration WENO5-RK3 extension, every 5th step a
-100-iteration WENO5-RK3 reinitialization plus a (10,5) closest-point surface
-redistance, and the cosine delta/H area and volume; 128^3 on [-1.2,1.2]^3,
-ellipsoid (0.8,0.5,0.5), seeded 16-mode curl velocity, CFL 0.4 (see
-2018_7_vesicle_amd/pipeline.py).  Inputs are resident in HBM before timing.
+Workload (one "step", config 3 of the metric): WENO5 + TVD-RK3 advection of F
+and psi, setDistance, normal velocity and its 40-iteration WENO5-RK3
+extension, every 5th step a 100-iteration WENO5-RK3 reinitialization plus a
+(10,5) closest-point surface redistance, and the cosine delta/H area and
+volume; 256^3 on [-1.2,1.2]^3, ellipsoid (0.8,0.5,0.5), seeded 32-mode curl
+velocity, CFL 0.4 (2018_7_vesicle_amd/pipeline.py).  The line times it with
+FP32 bulk fields (value) and with FP64 fields (fp64.value), each after its
+own set-up and warm-up steps.  Inputs are resident in HBM before timing.
 
 value = WENO grid-point updates per second: (operator-stage passes counted by
 pipeline.updates_per_step) x points / wall time of the K timed steps.
 
-roofline: the dominant kernel is the fused WENO5-RK3 extension stage
-(k_extend_stage: interface-list blocks and z-marching bulk blocks in one
-launch).  Its average duration is measured with HIP events around the
-extension calls inside the timed steps, minus the extension's per-call set-up
-(upwind normal, velocity, interface values: timed separately with 0
-iterations), divided by the stage launches.  achieved = 37.3 B/point (the
-algorithmic bytes of one FP64 extension stage) x points / that duration.
+roofline: the dominant kernel of the FP32 pipeline is the fused WENO5-RK3
+extension stage (k_extend_stage<R, float>: interface-list blocks and
+z-marching bulk blocks in one launch).  Its average duration is measured with
+HIP events around the extension calls inside the timed steps (on the
+pipeline's stream, torch's current stream), minus the extension's per-call
+set-up (timed separately with 0 iterations), divided by the stage launches.
+achieved = 22.7 B/point (the algorithmic bytes of one FP32-bulk extension
+stage) x points / that duration.  traffic: HBM bytes per launch from
+rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of `bench.py --probe stages`,
+calibrated on a 256 MiB copy (profiles/stage_counters.json);
+roofline.valu: that launch's VALU instructions (SQ_INSTS_VALU of the same
+runs) per second against the issue rate this run measures
+(vls_probe_valu_rate).
 
-traffic: HBM bytes per stage from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs
-of `bench.py --probe extend`, calibrated on a 256 MiB copy
-(profiles/extend_stage_128_traffic.txt).
+cpu_baseline: the CPU oracle (OpenMP, FP64) on a bounded sample of the same
+workload on this host: one step without reinitialization at 256^3.
 
-cpu_baseline: the CPU oracle (OpenMP, FP64) on one full pipeline step of the
-same workload, timed on this host.
-
-Beside the main line: config1 (64^3 ENO2 forward-Euler reinitialization with
-its error measures), config3 (the pipeline at 256^3 with a 32-mode velocity,
-FP32 bulk vs FP64, 200 steps each, with the FP32 run's errors), config4
-(512 vesicles of uneven sizes batched, FP32 bulk vs FP64 batched vs the plain
-per-vesicle FP64 path) and config5 (the 96^3 adjoint gradient against finite
-differences and tangent-mode derivatives), variants (the WENO-RK3 / ENO-RK2
-extensions and WENO-RK3 reinitialization at 96^3), toolbox (the calculus
-toolboxes of F and of an auxiliary field A at 128^3), relax (the single-phase
-vesicle relaxation step at 96^3: extensions, bending speed, tension/pressure
-solve, rocFFT smoothing, reinitialization).
+Beside the main line: config2_128 (the FP64 pipeline at 128^3, the config-2
+workload), config1 (64^3 ENO2 forward-Euler reinitialization with its error
+measures), config3 (the 256^3 pipeline for 200 steps, FP32 bulk vs FP64, with
+the FP32 run's errors), config4 (512 vesicles of uneven sizes batched, FP32
+bulk vs FP64 batched vs the plain per-vesicle FP64 path), config5 (the 96^3
+adjoint gradient against finite differences and tangent-mode derivatives),
+variants (the WENO-RK3 / ENO-RK2 extensions and WENO-RK3 reinitialization at
+96^3), toolbox (the calculus toolboxes of F and of an auxiliary field A at
+128^3), relax (the single-phase vesicle relaxation step at 96^3).
 """
 import argparse
 import json
@@ -58,62 +62,41 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=5,
                     help="untimed steps first (5 covers one reinitialization step, so every graph is captured)")
-    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--n", type=int, default=256, help="grid edge of the main line")
+    ap.add_argument("--modes", type=int, default=32, help="velocity modes of the main line")
+    ap.add_argument("--pmc-summary", nargs="+", default=None, metavar="DB",
+                    help="print per-kernel averages of the counters in rocprofv3 --pmc result databases "
+                         "(the last three dispatches of each kernel) and exit")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--precision-steps", type=int, default=200,
                     help="steps of the 256^3 FP32-bulk vs FP64 comparison (0 skips it)")
     ap.add_argument("--batch-steps", type=int, default=10,
                     help="timed steps of the 512-vesicle batch comparison (0 skips it)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 96^3 adjoint gradient check")
-    ap.add_argument("--probe", choices=["extend", "stages", "batch", "variants", "toolbox", "relax"], default=None,
-                    help="extend: a few extension stages and a 256 MiB device copy (for rocprofv3 --pmc); "
+    ap.add_argument("--probe", choices=["stages", "valu-rate", "batch", "variants", "toolbox", "relax"], default=None,
+                    help="valu-rate: the VALU issue rate (FP32, FP64) of this GPU; "
                          "stages: 3 launches each of the extension, reinitialization and advection stage "
                          "kernels at 128^3 FP64 and 256^3 FP32 (for rocprofv3 --pmc); "
                          "batch: a few steps of the 512-vesicle FP32 batch (for rocprofv3 --kernel-trace); "
                          "variants: the WENO-RK3 / ENO-RK2 calls of the variants line (for rocprofv3 --kernel-trace); "
                          "toolbox: the calculus-toolbox calls of the toolbox line (for rocprofv3 --kernel-trace); "
                          "relax: the vesicle relaxation steps of the relax line")
-    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64",
-                    help="field type of the main pipeline line (fp32: FP32 bulk)")
     ap.add_argument("--main-only", action="store_true",
                     help="only the main line: no config1/3/4/5 and no CPU baseline")
     return ap.parse_args()
 
 
-def probe_extend(args):
-    """A short run for hardware counters: 3 iterations (9 stage launches) of the
-    128^3 extension, then a plain 256 MiB float64 device-to-device copy whose
-    counter readings calibrate FETCH_SIZE/WRITE_SIZE (2^25 doubles read and
-    written)."""
-    import importlib
-
-    import torch
-
-    vls = importlib.import_module("2018_7_vesicle_amd")
-    pl = vls.pipeline
-    cfg = pl.PipelineConfig(n=args.n, initial_reinit=10)
-    p = pl.Pipeline(cfg)
-    p.run(1)
-    torch.cuda.synchronize()
-    p.map.setDistance()
-    p.map.WENO5RK3Extend(p.Vn, 3)
-    src = torch.randn(1 << 25, dtype=torch.float64, device="cuda")
-    dst = torch.empty_like(src)
-    dst.copy_(src)
-    torch.cuda.synchronize()
-    print(json.dumps({"probe": "extend", "n": args.n, "stage_launches": 9, "copy_bytes": 8 << 25}), flush=True)
-    return 0
-
-
 def probe_stages(args):
     """A short run for hardware counters of the three fused stage kernels: the
-    config-2 pipeline at 128^3 (FP64) and the config-3 pipeline at 256^3
-    (FP32 bulk) are set up (initial reinitialization and launch-shape
+    config-3 pipeline at 256^3 (FP32 bulk) and the config-2 pipeline at 128^3
+    (FP64) are both set up first (initial reinitialization, launch-shape
     autotuning), then each runs one extension iteration, one reinitialization
     iteration and one advection step (3 launches of each stage kernel), with a
-    synchronization between the phases.  A plain 256 MiB float64 copy last
-    calibrates FETCH_SIZE / WRITE_SIZE.  Kernel names tell the two grids apart
-    by their field type (k_*_stage<R, double> at 128^3, <R, float> at 256^3)."""
+    synchronization between the phases; so the last three launches of every
+    stage kernel are the probe's own (k_*_stage<R, float> at 256^3,
+    <R, double> at 128^3).  Last, a 256 MiB float64 copy whose source was
+    pushed out of the caches by a 2 GiB fill: FETCH_SIZE and WRITE_SIZE of a
+    known number of bytes, to calibrate them."""
     import importlib
 
     import torch
@@ -121,23 +104,29 @@ def probe_stages(args):
     vls = importlib.import_module("2018_7_vesicle_amd")
     pl = vls.pipeline
     out = {"probe": "stages"}
-    for n, prec, modes in ((128, "fp64", 16), (256, "fp32", 32)):
-        cfg = pl.PipelineConfig(n=n, precision=prec, modes=modes, initial_reinit=20)
-        p = pl.Pipeline(cfg)
+    lines = []
+    for n, prec, modes in ((256, "fp32", 32), (128, "fp64", 16)):
+        p = pl.Pipeline(pl.PipelineConfig(n=n, precision=prec, modes=modes, initial_reinit=20))
+        p.map.setDistance()
+        p.map.normalVelocity(*p.u, p.Vn)
+        lines.append((f"{prec}_{n}", p))
+    torch.cuda.synchronize()
+    for key, p in lines:
         m = p.map
-        m.setDistance()
-        m.normalVelocity(*p.u, p.Vn)
-        torch.cuda.synchronize()
         m.extend_(p.Vn, 1)
         torch.cuda.synchronize()
         m.reinitialize_(m.F, 1)
         torch.cuda.synchronize()
         m.advect_(m.F, *p.u, p.dt, 1)
         torch.cuda.synchronize()
-        out[f"{prec}_{n}"] = {"tile_rows": m.launch()[0], "zmarch_depth": m.launch()[1], "points": n ** 3}
-        del p, m
-        torch.cuda.empty_cache()
+        tr, zd = m.launch()
+        out[key] = {"tile_rows": tr, "zmarch_depth": zd, "points": p.grid.NumElt}
+    del lines, p, m
+    torch.cuda.empty_cache()
     src = torch.ones(1 << 25, dtype=torch.float64, device="cuda")
+    flush = torch.empty(1 << 28, dtype=torch.float64, device="cuda")
+    flush.fill_(2.0)
+    torch.cuda.synchronize()
     dst = torch.empty_like(src)
     dst.copy_(src)
     torch.cuda.synchronize()
@@ -163,23 +152,152 @@ def probe_batch(args):
     return 0
 
 
-def traffic_from_profiles():
-    """HBM bytes per launch of the dominant kernel measured with rocprofv3 --pmc
-    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), if recorded."""
-    path = os.path.join(ROOT, "profiles", "extend_stage_128_traffic.json")
+def stage_counters(key):
+    """Counters of one stage launch recorded with rocprofv3 --pmc runs of
+    `bench.py --probe stages` (profiles/stage_counters.json): HBM bytes per
+    launch (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)
+    and VALU wave-instructions per launch, if recorded."""
+    path = os.path.join(ROOT, "profiles", "stage_counters.json")
     if not os.path.exists(path):
         return None
     try:
-        return float(json.load(open(path))["bytes_per_launch"])
+        return json.load(open(path))["stages"].get(key)
     except Exception:
         return None
 
 
+def valu_rate(vls, fp64):
+    """VALU wave-instructions per second this GPU issues (vls_probe_valu_rate)."""
+    import ctypes
+
+    v = ctypes.c_double()
+    vls._lib.call("vls_probe_valu_rate", 1 if fp64 else 0, ctypes.byref(v), None)
+    return v.value
+
+
+def pmc_summary(paths):
+    """Per kernel and counter, the average over the last three dispatches of
+    that kernel in rocprofv3 --pmc result databases of `bench.py --probe
+    stages` (the probe's own launches come after set-up and autotuning).
+    Prints one JSON object: the raw table, and per stage kernel (extension,
+    reinitialization, advection at 128^3 FP64 and 256^3 FP32) its VALU, SALU,
+    LDS and memory instructions per launch and its HBM bytes per launch,
+    FETCH_SIZE and WRITE_SIZE (KiB) scaled by what they read for the probe's
+    256 MiB copy (the largest elementwise kernel of the run)."""
+    import collections
+    import sqlite3
+
+    per = collections.defaultdict(lambda: collections.defaultdict(lambda: collections.defaultdict(float)))
+    for path in paths:
+        con = sqlite3.connect(path)
+        rows = con.execute("select kernel_name, counter_name, value, dispatch_id from counters_collection").fetchall()
+        for name, counter, value, dispatch in rows:
+            per[name.split("(")[0].replace("void ", "")][counter][(path, dispatch)] += float(value)
+    table = {}
+    for kname, counters in per.items():
+        table[kname] = {}
+        for counter, by_dispatch in counters.items():
+            keys = sorted(by_dispatch, key=lambda k: k[1])
+            last = [by_dispatch[k] for k in keys[-3:]]
+            table[kname][counter] = {"dispatches": len(keys), "last3_avg": sum(last) / len(last),
+                                     "max": max(by_dispatch.values()), "last_dispatch": keys[-1][1]}
+    copy_bytes = 256 * 1024.0  # KiB read and written by the calibration copy, the run's last copy kernel
+    calib = {}
+    for c in ("FETCH_SIZE", "WRITE_SIZE"):
+        last = None
+        for kname, counters in per.items():
+            if "copy" in kname and c in counters:
+                for key, value in counters[c].items():
+                    if last is None or key[1] > last[0]:
+                        last = (key[1], value)
+        calib[c] = copy_bytes / last[1] if last and last[1] > 0 else None
+    stages = {}
+    for kname, v in table.items():
+        for stage in ("extend", "reinit", "advect"):
+            if f"k_{stage}_stage<" not in kname:
+                continue
+            prec = "fp32" if "float" in kname else "fp64"
+            key = f"{stage}_{prec}_{256 if prec == 'fp32' else 128}"
+            last = max(c["last_dispatch"] for c in v.values())
+            if key in stages and stages[key]["last_dispatch"] > last:
+                continue  # another launch shape of this kernel ran later (the probe's own)
+            rec = {"kernel": kname, "last_dispatch": last}
+            for c, name in (("SQ_INSTS_VALU", "valu_insts"), ("SQ_INSTS_SALU", "salu_insts"),
+                            ("SQ_INSTS_LDS", "lds_insts"), ("SQ_INSTS_VMEM_RD", "vmem_rd_insts"),
+                            ("SQ_INSTS_VMEM_WR", "vmem_wr_insts"), ("SQ_WAVES", "waves"),
+                            ("FETCH_SIZE", "fetch_kib"), ("WRITE_SIZE", "write_kib")):
+                if c in v:
+                    rec[name] = v[c]["last3_avg"]
+            if "fetch_kib" in rec and "write_kib" in rec and calib["FETCH_SIZE"] and calib["WRITE_SIZE"]:
+                rec["traffic_bytes"] = 1024.0 * (rec["fetch_kib"] * calib["FETCH_SIZE"] +
+                                                 rec["write_kib"] * calib["WRITE_SIZE"])
+            stages[key] = rec
+    print(json.dumps({"dbs": paths, "calibration": calib, "stages": stages, "kernels": table}), flush=True)
+    return 0
+
+
+def pipeline_line(vls, n, modes, precision, steps, warmup):
+    """The vesicle-step pipeline at n^3 in one precision: set-up (initial
+    reinitialization, launch-shape autotuning), `warmup` untimed steps (every
+    graph captured), then `steps` timed steps with the extension calls
+    bracketed by HIP events; returns the rate and the extension stage's
+    average launch time."""
+    import torch
+
+    pl = vls.pipeline
+    cfg = pl.PipelineConfig(n=n, modes=modes, precision=precision)
+    p = pl.Pipeline(cfg)
+    if warmup:
+        p.run(warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    updates = p.run(steps, record_extension=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ext_ms = [a.elapsed_time(b) for a, b in p.ext_events]
+    setup_ms = []
+    scratch = p.Vn.clone()
+    p.map.extend_(scratch, 0)  # capture
+    for _ in range(5):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        p.map.extend_(scratch, 0)
+        e1.record()
+        torch.cuda.synchronize()
+        setup_ms.append(e0.elapsed_time(e1))
+    stage_us = (sum(ext_ms) / len(ext_ms) - sum(setup_ms) / len(setup_ms)) / (3 * cfg.ext_iters) * 1e3
+    av = p.area_volume.cpu().numpy()
+    n_pts = p.grid.NumElt
+    out = {"cfg": cfg, "points": n_pts, "value": updates / wall, "ms_per_step": wall / steps * 1e3,
+           "updates_per_step_avg": updates / steps / n_pts, "stage_us": stage_us,
+           "area": [float(av[0, 0]), float(av[-1, 0])], "volume": [float(av[0, 1]), float(av[-1, 1])],
+           "dt": p.dt, "launch": p.map.launch()}
+    del p
+    torch.cuda.empty_cache()
+    return out
+
+
+def config2_line(vls, steps, warmup):
+    """Config 2 beside the main line: the FP64 pipeline at 128^3 (16-mode
+    velocity), with its extension stage against the FP64 algorithmic bytes."""
+    r = pipeline_line(vls, 128, 16, "fp64", steps, warmup)
+    achieved = B_ALG_EXTEND_FP64 * r["points"] / (r["stage_us"] * 1e-6) / 1e9
+    counters = stage_counters("extend_fp64_128")
+    return {"value": r["value"], "unit": "grid-pt updates/s", "ms_per_step": r["ms_per_step"], "dtype": "f64",
+            "workload": "config-2 vesicle step, 128^3, 16-mode velocity, FP64",
+            "roofline": {"kernel": "FP64 WENO5-RK3 extension stage (k_extend_stage<R, double>)", "achieved": achieved,
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                         "avg_launch_us": r["stage_us"], "bytes_per_point": B_ALG_EXTEND_FP64,
+                         "traffic": counters.get("traffic_bytes") if counters else None}}
+
+
 def cpu_baseline(cfg):
-    """The CPU oracle (OpenMP, FP64, all host threads) on one full pipeline step
-    of the same workload: a step that also reinitializes and redistances
-    (advection of F and psi, setDistance, normal velocity, extension,
-    reinitialization, surface redistance, area/volume), timed as the GPU's."""
+    """The CPU oracle (OpenMP, FP64, all host threads) on a bounded sample of
+    the main line's workload: one step without reinitialization at its grid
+    (advection of F and psi, setDistance, normal velocity, the 40-iteration
+    extension, area/volume), on the same inputs, timed as the GPU's line (the
+    rate per stage pass does not depend on which passes a step holds)."""
     import importlib
 
     import numpy as np
@@ -193,7 +311,7 @@ def cpu_baseline(cfg):
     po.set_num_threads(threads)
     umax = max(float(np.abs(c).max()) for c in u)
     dt = cfg.cfl * dx / umax
-    step_index = cfg.reinit_every - 1
+    step_index = 0  # not a reinitializing step
     t0 = time.perf_counter()
     po.pipeline(F0, psi0, u, dt, cfg.eps_factor * dx, (dx, dx, dx), 1, cfg.ext_iters, cfg.reinit_every,
                 cfg.reinit_iters, cfg.redist_iters, 0, first_step=step_index)
@@ -204,7 +322,7 @@ def cpu_baseline(cfg):
         "unit": "grid-pt updates/s",
         "cores": po.num_threads(),
         "kind": "port",
-        "sample": f"one full pipeline step (with reinitialization and surface redistance) at {cfg.n}^3 FP64: "
+        "sample": f"one pipeline step without reinitialization at {cfg.n}^3 FP64 ({cfg.modes}-mode velocity): "
                   f"{passes} WENO stage passes in {t1 - t0:.1f} s",
     }
 
@@ -459,8 +577,13 @@ def config5():
 
 def main():
     args = parse()
-    if args.probe == "extend":
-        return probe_extend(args)
+    if args.probe == "valu-rate":
+        import importlib
+
+        vls = importlib.import_module("2018_7_vesicle_amd")
+        print(json.dumps({"probe": "valu-rate", "fp32_insts_per_s": valu_rate(vls, False),
+                          "fp64_insts_per_s": valu_rate(vls, True)}), flush=True)
+        return 0
     if args.probe == "batch":
         return probe_batch(args)
     if args.probe == "stages":
@@ -481,6 +604,8 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         return 0
+    if args.pmc_summary:
+        return pmc_summary(args.pmc_summary)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported by this build "
               "(the multi-GPU z-slab decomposition is a design note, not code)", file=sys.stderr)
@@ -490,81 +615,69 @@ def main():
     import torch
 
     vls = importlib.import_module("2018_7_vesicle_amd")
-    pl = vls.pipeline
     torch.cuda.set_device(0)
-    cfg = pl.PipelineConfig(n=args.n, precision=args.precision)
-    p = pl.Pipeline(cfg)
-    n = p.grid.NumElt
-    if args.warmup:
-        p.run(args.warmup)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    updates = p.run(args.steps, record_extension=True)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    wall = t1 - t0
-
-    # dominant kernel: extension stage duration
-    ext_ms = [a.elapsed_time(b) for a, b in p.ext_events]
-    setup_ms = []
-    scratch = p.Vn.clone()
-    p.map.extend_(scratch, 0)  # capture
-    for _ in range(5):
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record()
-        p.map.extend_(scratch, 0)
-        e1.record()
-        torch.cuda.synchronize()
-        setup_ms.append(e0.elapsed_time(e1))
-    stage_s = (sum(ext_ms) / len(ext_ms) - sum(setup_ms) / len(setup_ms)) / (3 * cfg.ext_iters) * 1e-3
-    b_alg = B_ALG_EXTEND_FP64 if args.precision == "fp64" else B_ALG_EXTEND_FP32
-    achieved = b_alg * n / stage_s / 1e9
-
-    av = p.area_volume.cpu().numpy()
-    fp64 = args.precision == "fp64"
+    # the metric's configuration: the vesicle step at 256^3 (32-mode velocity),
+    # FP32 bulk, beside the plain FP64 path on the same workload
+    r32 = pipeline_line(vls, args.n, args.modes, "fp32", args.steps, args.warmup)
+    r64 = pipeline_line(vls, args.n, args.modes, "fp64", args.steps, args.warmup)
+    cfg = r32["cfg"]
+    stage_us = r32["stage_us"]
+    achieved = B_ALG_EXTEND_FP32 * r32["points"] / (stage_us * 1e-6) / 1e9
+    counters = stage_counters(f"extend_fp32_{args.n}")
+    roofline = {
+        "kernel": "FP32-bulk WENO5-RK3 extension stage (k_extend_stage<R, float> launches)",
+        "bound": "hbm",
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": counters.get("traffic_bytes") if counters else None,
+        "avg_launch_us": stage_us,
+        "bytes_per_point": B_ALG_EXTEND_FP32,
+    }
+    if counters and counters.get("valu_insts"):
+        # VALU wave-instructions of one launch over the launch time, against the
+        # FP32 issue rate this GPU shows (vls_probe_valu_rate)
+        peak = valu_rate(vls, False)
+        rate = counters["valu_insts"] / (stage_us * 1e-6)
+        roofline["valu"] = {"insts_per_launch": counters["valu_insts"],
+                            "insts_per_point": counters["valu_insts"] * 64 / r32["points"],
+                            "achieved_per_s": rate, "peak_per_s": peak, "frac": rate / peak}
     out = {
-        "metric": "WENO grid-pt updates/s (FP64 vesicle-step pipeline)" if fp64 else
-                  "WENO grid-pt updates/s (FP32-bulk vesicle-step pipeline)",
-        "value": updates / wall,
+        "metric": "WENO grid-pt updates/s (FP32-bulk vesicle-step pipeline at 256^3, beside plain FP64)",
+        "value": r32["value"],
         "unit": "grid-pt updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": r32["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if fp64 else "f32 bulk, f64 interface and reductions",
+        "dtype": "f32 (bulk stencils; f64 subcell root solve, interface lanes and area/volume sums)",
         "data": "synthetic",
         "config": {
-            "workload": f"config-2 vesicle step: {cfg.n}^3 on [{cfg.lo},{cfg.hi}]^3, ellipsoid {cfg.axes}, "
+            "workload": f"config-3 vesicle step: {cfg.n}^3 on [{cfg.lo},{cfg.hi}]^3, ellipsoid {cfg.axes}, "
                         f"{cfg.modes}-mode curl velocity (seed {cfg.seed}), WENO5-RK3 advection CFL {cfg.cfl}, "
                         f"{cfg.ext_iters}-iteration extension, reinit {cfg.reinit_iters} iterations every "
-                        f"{cfg.reinit_every} steps, surface redistance {cfg.redist_iters}, cosine delta/H 1.5dx",
+                        f"{cfg.reinit_every} steps, surface redistance {cfg.redist_iters}, cosine delta/H 1.5dx; "
+                        "FP32 bulk",
             "grid": [cfg.n, cfg.n, cfg.n],
-            "points": n,
-            "updates_per_step_avg": updates / args.steps / n,
+            "points": r32["points"],
+            "updates_per_step_avg": r32["updates_per_step_avg"],
         },
-        "roofline": {
-            "kernel": ("FP64" if fp64 else "FP32-bulk") +
-                      " WENO5-RK3 extension stage (k_extend_stage launches)",
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic_from_profiles() if (fp64 and args.n == 128) else None,
-            "avg_launch_us": stage_s * 1e6,
-            "bytes_per_point": b_alg,
-        },
-        "diagnostics": {
-            "area_first_last": [float(av[0, 0]), float(av[-1, 0])],
-            "volume_first_last": [float(av[0, 1]), float(av[-1, 1])],
-            "dt": p.dt,
-        },
+        "fp64": {"value": r64["value"], "ms_per_step": r64["ms_per_step"], "extension_stage_us": r64["stage_us"],
+                 "dtype": "f64", "note": "the plain FP64 GPU path on the same workload and seed"},
+        "fp32_over_fp64": r32["value"] / r64["value"],
+        "roofline": roofline,
+        "diagnostics": {"area_first_last": r32["area"], "volume_first_last": r32["volume"], "dt": r32["dt"],
+                        "fp32_vs_fp64_area_rel_last": abs(r32["area"][1] / r64["area"][1] - 1),
+                        "fp32_vs_fp64_volume_rel_last": abs(r32["volume"][1] / r64["volume"][1] - 1)},
     }
+    del r32, r64
+    torch.cuda.empty_cache()
     if not args.main_only:
+        out["config2_128"] = config2_line(vls, args.steps, args.warmup)
         out["config1"] = config1(vls, not args.no_cpu)
         if args.precision_steps > 0:
             out["config3"] = config3(args.precision_steps)
@@ -579,7 +692,6 @@ def main():
             out["cpu_baseline"] = cpu_baseline(cfg)
     print(json.dumps(out), flush=True)
     return 0
-
 
 if __name__ == "__main__":
     sys.exit(main())

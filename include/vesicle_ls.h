@@ -393,6 +393,13 @@ int vls_sdf3_normal_velocity_f32(vls_sdf3_t h, const float *F, const float *vx, 
  * previous setting. */
 int vls_set_graphs(int enable);
 
+/* Measurement only, no counterpart in the reference: the vector-ALU issue
+ * rate of the device (wave64 instructions per second over all compute units)
+ * for independent FP32 (fp64 = 0) or FP64 fused multiply-add chains, timed
+ * with HIP events on `stream`; the benchmark reports a stage kernel's VALU
+ * instructions per second against it.  Allocates and frees a small buffer. */
+int vls_probe_valu_rate(int fp64, double *insts_per_s, vls_stream_t stream);
+
 
 /* ------------------------------------------------------------------------ */
 /* 4. batches of grids                                                      */
