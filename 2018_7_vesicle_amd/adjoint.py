@@ -166,6 +166,40 @@ class ShapeMatching:
             self._advance(F, i)
         return self.loss_and_seed(F)[0]
 
+    def loss_with_signature(self, c):
+        """L(c) and the branch signature of the forward: a hash of the sign
+        pattern of F after every advection step and after every
+        reinitialization.  Every jump of the discrete map sits where that
+        pattern changes: a subcell crossing appearing or vanishing, the
+        quadratic-ENO / quintic branch of the root solve (a kink is two sign
+        changes within a stencil), the inside/outside mask, the uniform /
+        nonuniform WENO switch of the reinitialization (a sign change beside
+        a node).  Two coefficient vectors with equal signatures lie on one
+        smooth piece of the map, up to the sign tests inside the
+        reinitialization's RK stages, which follow the same sign pattern."""
+        if not hasattr(self, "_hash_w"):
+            g = torch.Generator(device="cpu").manual_seed(20181207)
+            self._hash_w = torch.randint(0, 1 << 40, self.grid.Size, generator=g, dtype=torch.int64).to(
+                self.F0.device)
+        self._set_velocity(c)
+        F = self.F
+        F.copy_(self.F0)
+        n = self.cfg.steps + sum(1 for i in range(self.cfg.steps) if self._reinit_step(i))
+        sig = torch.zeros(n, dtype=torch.int64, device=F.device)
+        k = 0
+        m = self.map
+        for i in range(self.cfg.steps):
+            m.advect_(F, *self.u, self.dt, 1)
+            sig[k] = torch.sum(self._hash_w * (F < 0))
+            k += 1
+            if self._reinit_step(i):
+                m.F = F
+                m.setDistance()
+                m.reinitialize_(F, self.cfg.reinit_iters)
+                sig[k] = torch.sum(self._hash_w * (F < 0))
+                k += 1
+        return self.loss_and_seed(F)[0], sig.cpu().numpy()
+
     def gradient(self, c):
         """(L, dL/dc (3, modes)) by the adjoint with checkpointing"""
         cfg, m = self.cfg, self.map
@@ -223,6 +257,36 @@ class ShapeMatching:
                 Fd, Fd2 = Fd2, Fd
         L, seed = self.loss_and_seed(F)
         return L, float((seed * Fd).sum())
+
+
+def classified_gradient_check(prob, h=1e-6):
+    """The adjoint gradient against FP64 central differences with step h on
+    every coefficient, each coefficient classified by the branch signatures
+    (ShapeMatching.loss_with_signature) at c - h e_k, c and c + h e_k:
+    `crossing` marks the coefficients whose perturbation changes the
+    signature, i.e. whose difference quotient spans a jump of the discrete
+    map.  Returns the gradient, the central, forward and backward quotients
+    and the mask."""
+    c0 = prob.c0
+    L0, s0 = prob.loss_with_signature(c0)
+    L, g = prob.gradient(c0)
+    fd = np.zeros_like(g)
+    fw = np.zeros_like(g)
+    bw = np.zeros_like(g)
+    crossing = np.zeros(g.shape, dtype=bool)
+    for a in range(g.shape[0]):
+        for k in range(g.shape[1]):
+            cp, cm = c0.copy(), c0.copy()
+            cp[a, k] += h
+            cm[a, k] -= h
+            Lp, sp = prob.loss_with_signature(cp)
+            Lm, sm = prob.loss_with_signature(cm)
+            fd[a, k] = (Lp - Lm) / (2 * h)
+            fw[a, k] = (Lp - L0) / h
+            bw[a, k] = (L0 - Lm) / h
+            crossing[a, k] = not (np.array_equal(sp, s0) and np.array_equal(sm, s0))
+    return {"loss": L0, "loss_gradient_pass": L, "gradient": g, "fd": fd, "forward": fw, "backward": bw,
+            "crossing": crossing}
 
 
 def check_gradient(prob, hs=(1e-6,), directions=8, seed=7):

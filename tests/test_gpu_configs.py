@@ -5,8 +5,16 @@ ENO2 with the subcell fix, forward Euler, CFL 0.5, 100 iterations; reports
 max and mean of ||grad phi|-1| in |phi| < 3dx and the interface displacement
 from the sphere r = 0.6 (the zero set of phi0).
 
-Config 3 (small): the vesicle-step pipeline in FP32 bulk against the same
-pipeline in FP64 from one seed; the error bounds are stated in the test.
+Config 3: the vesicle-step pipeline in FP32 bulk against the same pipeline in
+FP64 from one seed, at 64^3 and at 256^3 (where one FP64 step is also checked
+against the oracle bit for bit); the error bounds are stated in the tests.
+
+Config 4: a batch of grids of edges 32..96, one batch step against the
+oracle: FP64 bit for bit, FP32 bulk within stated bounds.
+
+Config 5: the discrete adjoint at 96^3 against central differences, each
+mismatched coefficient classified as one whose difference quotient crosses a
+recorded branch of the forward map.
 """
 import importlib
 
@@ -61,3 +69,167 @@ def test_config3_precision_comparison_small():
     assert e["area_rel_drift_max"] < 1e-6 and e["volume_rel_drift_max"] < 1e-6, e
     c = r["fp64_rounded_start_vs_fp64"]["interface_displacement"]
     assert c["normal"]["max_dx"] <= d["normal"]["max_dx"]
+
+
+def test_config5_gradient_at_full_size():
+    """Config 5 as specified: 96^3, 200 WENO5-RK3 advection steps, 10
+    reinitialization iterations every 10 steps, checkpoints every 20 steps;
+    the adjoint gradient of all 192 coefficients against FP64 central
+    differences.
+
+    The discrete map is piecewise smooth: WENORK3SurfaceConservationLaw.m:10-29
+    composed with the reinitialization, whose weno_derivative_boundary
+    (re_step.cu:248-269) switches stencils on sign tests and whose root solve
+    (boundary_location.cu:53-80) switches between the quintic and quadratic
+    ENO at kinks.  On this workload its jumps (about 1e-7 of the loss) are
+    so dense in coefficient space that every +-1e-6 perturbation crosses one:
+    at the config's h = 1e-6 each coefficient's quotient misses the adjoint,
+    and each such coefficient is shown to cross a recorded branch (the sign
+    pattern of F through the run changes) with forward and backward
+    quotients that disagree (the jump).  The derivative of the piece that
+    holds c is then checked with h = 1e-8, on the coefficients whose
+    perturbation keeps the sign pattern and whose one-sided quotients agree
+    to 1e-4 (no jump within +-h): norm-wise to 2e-6 and each to 1e-4, the
+    rounding floor of the FP64 loss over h = 1e-8 (the loss is computed to
+    about 1e-14)."""
+    adj = importlib.import_module("2018_7_vesicle_amd.adjoint")
+    cfg = adj.AdjointConfig()
+    assert (cfg.n, cfg.steps, cfg.reinit_every, cfg.ckpt_every, cfg.modes) == (96, 200, 10, 20, 64)
+    prob = adj.ShapeMatching(cfg)
+    # the config's step
+    r = adj.classified_gradient_check(prob, h=1e-6)
+    g, fd, cross = r["gradient"], r["fd"], r["crossing"]
+    assert abs(r["loss_gradient_pass"] - r["loss"]) <= 1e-15 * abs(r["loss"])
+    scale = np.abs(g).max()
+    den = np.maximum(np.abs(g), 1e-3 * scale)
+    rel = np.abs(g - fd) / den
+    one_sided = np.abs(r["forward"] - r["backward"]) / den
+    mism = rel > 1e-6
+    print(f"config5 h=1e-6: {int(mism.sum())} of {mism.size} quotients miss by > 1e-6; "
+          f"{int(cross.sum())} cross a recorded branch")
+    assert np.all(cross[mism]), "a mismatched coefficient crosses no recorded branch"
+    assert np.all(one_sided[mism] > 1e-6)
+    assert np.all(rel[~cross] <= 1e-6)
+    # the derivative of the piece that holds c
+    r8 = adj.classified_gradient_check(prob, h=1e-8)
+    fd8, cross8 = r8["fd"], r8["crossing"]
+    # the same gradient (its last contraction is a library GEMM, whose split may reorder the sum)
+    assert np.allclose(r8["gradient"], g, rtol=1e-12, atol=1e-14 * scale)
+    one8 = np.abs(r8["forward"] - r8["backward"]) / den
+    clean = (~cross8) & (one8 < 1e-4)
+    rel8 = np.abs(g - fd8) / den
+    nw = float(np.linalg.norm((g - fd8)[clean]) / np.linalg.norm(fd8[clean]))
+    print(f"config5 h=1e-8: {int(clean.sum())} coefficients without a jump: norm-wise {nw:.2e}, "
+          f"max {float(rel8[clean].max()):.2e}")
+    assert clean.sum() >= 48
+    assert nw <= 2e-6
+    assert np.all(rel8[clean] <= 1e-4)
+    # tangent-mode directional derivatives on 8 random directions
+    rng = np.random.default_rng(7)
+    for _ in range(8):
+        d = rng.normal(size=g.shape)
+        _, dj = prob.jvp(prob.c0, d)
+        assert abs(dj - float((g * d).sum())) <= 1e-9 * abs(dj)
+
+
+def test_config3_at_full_size():
+    """Config 3 at its size: the vesicle-step pipeline at 256^3 with the
+    32-mode velocity.  (a) One step of the FP64 path against the CPU oracle,
+    bit for bit (area and volume to 1e-13: summation order); the step is one
+    without reinitialization, whose oracle run takes ~15 s on the host.
+    (b) FP32 bulk against FP64 from the same initial state over 10 steps
+    (two of them reinitializing; the workload's 200 steps run in bench.py):
+    interface displacement normal to the interface below 1e-4 dx, the same
+    crossings in both, area and volume within 1e-6 relative."""
+    pl = importlib.import_module("2018_7_vesicle_amd.pipeline")
+    cfg = pl.PipelineConfig(n=256, modes=32, initial_reinit=0, autotune=False)
+    p = pl.Pipeline(cfg)
+    xv, F0, psi0, u = pl.make_inputs(cfg)
+    p.run(1)
+    torch.cuda.synchronize()
+    dx = float(xv[1] - xv[0])
+    F, psi, Vn, av = po.pipeline(F0, psi0, u, p.dt, cfg.eps_factor * dx, (dx, dx, dx), 1, cfg.ext_iters,
+                                 cfg.reinit_every, cfg.reinit_iters, cfg.redist_iters, 0)
+    assert np.array_equal(p.map.F.cpu().numpy(), F)
+    assert np.array_equal(p.psi.cpu().numpy(), psi)
+    assert np.array_equal(p.Vn.cpu().numpy(), Vn)
+    got = p.area_volume.cpu().numpy()[0]
+    assert np.all(np.abs(got - np.asarray(av[0])) <= 1e-13 * np.abs(np.asarray(av[0]))), (got, av[0])
+    del p
+    torch.cuda.empty_cache()
+    r = diag.precision_comparison(256, 32, 10)
+    e = r["errors_fp32_vs_fp64"]
+    d = e["interface_displacement"]
+    print(f"config3 256^3, 10 steps: normal displacement max {d['normal']['max_dx']:.2e} dx "
+          f"p99 {d['normal']['p99_dx']:.2e} dx; area {e['area_rel_drift_max']:.2e} volume "
+          f"{e['volume_rel_drift_max']:.2e}; fp32/fp64 speed {r['fp32_speedup']:.2f}")
+    assert d["unmatched"] == 0 and d["crossings"] > 10000
+    assert d["normal"]["max_dx"] < 1e-4, d
+    assert e["area_rel_drift_max"] < 1e-6 and e["volume_rel_drift_max"] < 1e-6, e
+
+
+def test_config4_mixed_batch_against_the_oracle():
+    """Config 4's batch, one vesicle per edge of {32, 48, 64, 80, 96}
+    (ellipsoids with seeded semi-axes in [0.3, 0.8], seeded 8-mode curl
+    velocities on [-1,1]^3), advanced by one reinitializing batch step (TVD-RK3
+    advection, setDistance, u.N and its 20-iteration extension, 20
+    reinitialization iterations, cosine area/volume).  Each grid against the
+    CPU oracle's sequence of the same operators: FP64 bit for bit (area and
+    volume to 1e-13), FP32 bulk within 1e-4 dx in the band |F| < 3 dx, the
+    extended normal velocity within 2e-4 of its largest value there, area and
+    volume within 1e-6 relative (on MI355X the largest velocity error is 8.8e-5,
+    at edge 32; the bound on it is 2e-4)."""
+    bm = importlib.import_module("2018_7_vesicle_amd.batch")
+    edges = (32, 48, 64, 80, 96)
+    rng = np.random.default_rng(2018)
+    axes = rng.uniform(0.3, 0.8, size=(len(edges), 3))
+    modes = np.stack([bm.vesicle_modes(2018 + v, 8, 0.1, 4) for v in range(len(edges))])
+    spac = [(2.0 / (e - 1),) * 3 for e in edges]
+    ext_iters, reinit_iters = 20, 20
+
+    def run(dtype, start=None, dt=None):
+        b = bm.Batch([(e, e, e) for e in edges], spac, dtype=dtype)
+        lo = torch.full((b.ngrids, 3), -1.0, dtype=torch.float64, device=b.device)
+        u = [b.empty() for _ in range(3)]
+        b.curl_velocity(torch.from_numpy(modes).to(b.device), lo, *u)
+        if start is None:
+            F = b.empty()
+            b.ellipsoid(torch.from_numpy(axes).to(b.device), lo, F)
+            b.set_distance(F)
+            b.reinitialize(F, 20)
+        else:
+            F = b.pack([torch.from_numpy(f).to(b.device, dtype) for f in start])
+        dt = b.cfl_dt(*u, 0.4) if dt is None else torch.from_numpy(dt).to(b.device)
+        F_in = [b.grid_view(F, g).double().cpu().numpy() for g in range(b.ngrids)]
+        Vn = b.zeros()
+        av = torch.zeros((b.ngrids, 2), dtype=torch.float64, device=b.device)
+        b.step(F, Vn, *u, dt, ext_iters, reinit_iters, av, 1.5)
+        torch.cuda.synchronize()
+        out = dict(F_in=F_in, dt=dt.cpu().numpy(), av=av.cpu().numpy(),
+                   u=[[b.grid_view(c, g).double().cpu().numpy() for c in u] for g in range(b.ngrids)],
+                   F=[b.grid_view(F, g).double().cpu().numpy() for g in range(b.ngrids)],
+                   Vn=[b.grid_view(Vn, g).double().cpu().numpy() for g in range(b.ngrids)])
+        del b
+        return out
+
+    r64 = run(torch.float64)
+    r32 = run(torch.float32, start=r64["F_in"], dt=r64["dt"])
+    for g, e in enumerate(edges):
+        dx = 2.0 / (e - 1)
+        F0 = np.ascontiguousarray(r64["F_in"][g])
+        u3 = [np.ascontiguousarray(c) for c in r64["u"][g]]
+        F = po.advect_rk3(F0, 1, float(r64["dt"][g]), u3, dx, dx, dx)
+        d6, md = po.set_distance(F, dx, dx, dx)
+        Vn = po.weno5_rk3_extend(po.normal_velocity(F, u3, dx, dx, dx), ext_iters, F, d6, md, dx, dx, dx)
+        F = po.weno5_rk3_reinit(F, reinit_iters, d6, md, dx, dx, dx)
+        av = np.array(po.area_volume_cosine(F, dx, dx, dx, 1.5 * dx))
+        assert np.array_equal(r64["F"][g], F), f"edge {e}: F"
+        assert np.array_equal(r64["Vn"][g], Vn), f"edge {e}: Vn"
+        assert np.all(np.abs(r64["av"][g] - av) <= 1e-13 * np.abs(av)), (e, r64["av"][g], av)
+        # FP32 bulk from the same (rounded) state against the FP64 oracle
+        band = np.abs(F) < 3 * dx
+        eF = np.abs(r32["F"][g] - F)[band].max() / dx
+        eV = np.abs(r32["Vn"][g] - Vn)[band].max() / np.abs(Vn[band]).max()
+        eA = np.abs(r32["av"][g] / av - 1)
+        print(f"config4 edge {e}: fp32 |dF| {eF:.2e} dx, |dVn| {eV:.2e}, area/volume {eA[0]:.2e} {eA[1]:.2e}")
+        assert eF < 1e-4 and eV < 2e-4 and np.all(eA < 1e-6), (e, eF, eV, eA)
