@@ -113,8 +113,10 @@ def test_config5_gradient_at_full_size():
     # the derivative of the piece that holds c
     r8 = adj.classified_gradient_check(prob, h=1e-8)
     fd8, cross8 = r8["fd"], r8["crossing"]
-    # the same gradient (its last contraction is a library GEMM, whose split may reorder the sum)
-    assert np.allclose(r8["gradient"], g, rtol=1e-12, atol=1e-14 * scale)
+    # the same gradient up to summation order: the adjoint of the distances
+    # scatters with FP64 atomic adds, whose order varies between passes (two
+    # passes differ by about 1.5e-14 of the largest coefficient on MI355X)
+    assert np.allclose(r8["gradient"], g, rtol=0, atol=1e-13 * scale)
     one8 = np.abs(r8["forward"] - r8["backward"]) / den
     clean = (~cross8) & (one8 < 1e-4)
     rel8 = np.abs(g - fd8) / den
