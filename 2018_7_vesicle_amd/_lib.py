@@ -56,6 +56,7 @@ SIGNATURES = {
     "vls_sdf3_set_distance": [P, P, P],
     "vls_sdf3_distances": [P, P],
     "vls_sdf3_weno5_rk3_reinit": [P, P, I, P],
+    "vls_weno5_rk3_reinit_static": [P, I, I, I, I, D, D, D, P],
     "vls_sdf3_eno2_reinit": [P, P, I, I, D, P],
     "vls_sdf3_wenork3_reinit": [P, P, I, P],
     "vls_sdf3_enork2_extend": [P, P, P, I, P],

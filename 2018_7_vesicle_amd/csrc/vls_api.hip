@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -487,6 +488,42 @@ int area_volume(vls_sdf3* h, int kind, const T* F, double eps, double* dev_out, 
     if (rc) return rc;
     *area = r[0];
     *volume = r[1];
+    return VLS_OK;
+}
+
+// ---- work memory of the allocation-free reinitialization --------------------
+// One copy per GPU, part of the library's code object: the runtime places it
+// when it loads the code object on a device and it stays until the process
+// ends.  Per grid point: six distance fields, the list-order distances (six
+// rows) and pseudo-time steps, two RK stage fields (15 doubles), the
+// interface list and the deferred points (2 ints) and the flags (1 byte);
+// 16 counters; every piece starts on a 256-byte boundary.
+constexpr int kStaticPieces = 9;
+constexpr size_t kStaticBytes =
+    (size_t)VLS_STATIC_POINTS * (15 * sizeof(double) + 2 * sizeof(int) + 1) + 16 * sizeof(int) + kStaticPieces * 256;
+static __device__ __align__(256) unsigned char scratch[kStaticBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// the device address of the current device's copy of `scratch`, looked up
+// once per device (the array's name is a host-side stand-in in host code)
+int static_scratch(unsigned char** out)
+{
+    int dev = 0;
+    VLS_HIP(hipGetDevice(&dev));
+    VLS_REQUIRE(dev >= 0 && dev < kMaxDevices, "device %d beyond the %d the library tracks", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(VLS_ERR_HIP, "hipGetSymbolAddress of the static work memory on device %d: %s", dev,
+                        hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+    }
+    *out = g_scratch_ptr[dev];
     return VLS_OK;
 }
 
@@ -1023,6 +1060,47 @@ int vls_sdf3_weno5_rk3_reinit(vls_sdf3_t h, double* F, int iterations, vls_strea
 int vls_sdf3_weno5_rk3_reinit_f32(vls_sdf3_t h, float* F, int iterations, vls_stream_t stream)
 {
     return sdf3_weno5_rk3_reinit<float>(h, F, iterations, S(stream));
+}
+
+int vls_weno5_rk3_reinit_static(double* F, int iterations, int rows, int cols, int pges, double dx, double dy,
+                                double dz, vls_stream_t stream)
+{
+    int rc = check_geom(rows, cols, pges, dx, dy, dz);
+    if (rc) return rc;
+    VLS_REQUIRE(F, "null argument");
+    VLS_REQUIRE(iterations >= 0, "negative iteration count");
+    const int64_t n = (int64_t)rows * cols * pges;
+    VLS_REQUIRE(n <= VLS_STATIC_POINTS, "box %dx%dx%d has %lld points, more than the %d the static work memory holds",
+                rows, cols, pges, (long long)n, VLS_STATIC_POINTS);
+    unsigned char* base = nullptr;
+    if ((rc = static_scratch(&base))) return rc;
+    // the handle's fields this path uses, carved from the static memory
+    vls_sdf3 h;
+    h.g = Geom{rows, cols, pges, dx, dy, dz};
+    size_t used = 0;
+    auto take = [&](size_t bytes) {
+        unsigned char* p = base + used;
+        used += (bytes + 255) / 256 * 256;
+        return p;
+    };
+    double* dp = reinterpret_cast<double*>(take(sizeof(double) * 6 * n));
+    h.d = carve6(dp, n);
+    h.ldist[0] = reinterpret_cast<double*>(take(sizeof(double) * 6 * n));
+    h.ldt[0] = reinterpret_cast<double*>(take(sizeof(double) * n));
+    h.t[0] = reinterpret_cast<double*>(take(sizeof(double) * n));
+    h.t[1] = reinterpret_cast<double*>(take(sizeof(double) * n));
+    h.list = reinterpret_cast<int*>(take(sizeof(int) * n));
+    h.over = reinterpret_cast<int*>(take(sizeof(int) * n));
+    h.counts = reinterpret_cast<int*>(take(sizeof(int) * 16));
+    h.flags = take((size_t)n);
+    VLS_REQUIRE(used <= kStaticBytes, "static work memory layout overflows (%zu > %zu bytes)", used, kStaticBytes);
+    const hipStream_t s = S(stream);
+    // the counters keep the previous call's values: clear them (a kernel, so
+    // that a captured call records no memset node)
+    VLS_HIP(launch_zero_ints(h.counts, 16, s));
+    if ((rc = set_distance_into<double>(&h, 6, main_set(&h), nullptr, F, nullptr, s))) return rc;
+    VLS_HIP(launch_set_mask_bit<double>(h.flags, F, h.g, s));
+    return reinit_loop<double>(&h, F, iterations, main_set(&h), s);
 }
 
 int vls_sdf3_eno2_reinit(vls_sdf3_t h, double* F, int iterations, int rk2, double cfl, vls_stream_t stream)

@@ -385,6 +385,22 @@ int vls_sdf3_area_volume_device_f32(vls_sdf3_t h, const float *F, int kind, doub
 int vls_sdf3_normal_velocity_f32(vls_sdf3_t h, const float *F, const float *vx, const float *vy, const float *vz,
                                  float *out, vls_stream_t stream);
 
+/* Allocation-free WENO5-RK3 reinitialization: no counterpart in the
+ * reference.  It is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same result, bit for bit, as vls_sdf3_create + vls_sdf3_set_distance(F) +
+ * vls_sdf3_weno5_rk3_reinit(F, iterations) on a fresh handle: the distances
+ * of F, then `iterations` RK3 iterations of F in place, on `stream`.  Its work
+ * memory is one static array per GPU (about 129 bytes per point for
+ * VLS_STATIC_POINTS points, 129 MiB, held by each GPU that loads the library
+ * until the process ends); a box of more points is rejected (VLS_ERR_INVALID).
+ * It calls no hipMalloc/hipFree and records no memset.  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (other streams or
+ * threads) race and must be serialized by the caller. */
+#define VLS_STATIC_POINTS (1 << 20)
+int vls_weno5_rk3_reinit_static(double *F, int iterations, int rows, int cols, int pges, double dx, double dy,
+                                double dz, vls_stream_t stream);
+
 /* Graph capture of the wrapper-level loops (no counterpart in the reference,
  * which launches every kernel from MATLAB): with it on (the default), a
  * reinitialization, extension, surface redistance, advection or batch step

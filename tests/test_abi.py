@@ -86,3 +86,13 @@ def test_missing_library_fails_loudly(monkeypatch):
     monkeypatch.setattr(lib_mod, "LIB_PATH", "/nonexistent/libvesicle_ls.so")
     with pytest.raises(lib_mod.VLSError):
         lib_mod.lib()
+
+
+def test_static_reinit_rejects_a_box_beyond_its_memory_without_device_work():
+    lib = vls.lib()
+    p = ctypes.c_void_p(16)
+    # VLS_STATIC_POINTS = 2^20 points; 128 x 128 x 65 is one page more
+    assert lib.vls_weno5_rk3_reinit_static(p, 2, 128, 128, 65, 0.1, 0.1, 0.1, None) == -1
+    assert b"static work memory" in lib.vls_last_error()
+    assert lib.vls_weno5_rk3_reinit_static(ctypes.c_void_p(), 2, 8, 8, 8, 0.1, 0.1, 0.1, None) == -1
+    assert lib.vls_weno5_rk3_reinit_static(p, -1, 8, 8, 8, 0.1, 0.1, 0.1, None) == -1
