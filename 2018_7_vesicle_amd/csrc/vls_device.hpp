@@ -149,28 +149,39 @@ __device__ __forceinline__ void weno5_pair(S& d_fore, S& d_back, const S p[7], J
 // s_k >= e > 0, so the reciprocals are finite; their squares stay in FP32
 // range while the indicators lie in [1e-19, 1e19] (second differences up to
 // ~1e9), and below that the largest weights still dominate: a weight that
-// underflows is one that would not count.  W is floored at 1e-30 so that a
-// degenerate line gives Q = 0.  The divisions are hardware reciprocals
-// (v_rcp_f32, 1 ulp); the bulk is VALU-bound, and this form needs fewer
-// instructions than scaling the indicators by a power of two and forming
-// their fourth powers.
+// underflows is one that would not count.  The divisions are hardware
+// reciprocals (v_rcp_f32, 1 ulp); the bulk is VALU-bound, and this form needs
+// fewer instructions than scaling the indicators by a power of two and
+// forming their fourth powers.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float fmaxf_(float a, float b) { return (a < b) ? b : a; }
 __device__ __forceinline__ float fminf_(float a, float b) { return (a < b) ? a : b; }
 
-// the correction Q(a, b, c, d) of one side
+// the correction Q(a, b, c, d) of one side.  The indicators enter the weights
+// only through their ratios, so they are taken divided by 1 + 3/13 (the floor
+// with them: the FP32 e of the axis constants is e * 13/16) and completed to
+// squares:
+//   s0 = (a - 11/8 b)^2 + 39/64 b^2 + e',  s1 = (b - 5/8 c)^2 + 39/64 c^2 + e',
+//   s2 = (d - 11/8 c)^2 + 39/64 c^2 + e'
+// (s1 and s2 share 39/64 c^2 + e').  With x_k = 1/s_k^2 the weights are
+// x0 : 6 x1 : 3 x2, and with A = a - 2b + c, B = b - 2c + d the correction is
+//   Q = [4 x0 A + (6 x2 - W) B] / W = (4 x0 A + 6 x2 B) / W - B,  W = x0 + 6 x1 + 3 x2.
+// W is floored at 1e-30 so that the quotient stays finite; W comes near that
+// floor only when every indicator exceeds ~1e15 (second differences beyond
+// ~1e7, far outside level-set data).
 __device__ __forceinline__ float jp_q(float a, float b, float c, float d, float e)
 {
-    const float t0 = a - b, t1 = b - c, t2 = c - d;
-    const float y0 = fmaf(-3.f, b, a), y1 = b + c, y2 = fmaf(3.f, c, -d);
-    const float k = 3.f / 13.f;
-    const float r0 = __builtin_amdgcn_rcpf(fmaf(t0, t0, fmaf(k * y0, y0, e)));
-    const float r1 = __builtin_amdgcn_rcpf(fmaf(t1, t1, fmaf(k * y1, y1, e)));
-    const float r2 = __builtin_amdgcn_rcpf(fmaf(t2, t2, fmaf(k * y2, y2, e)));
-    const float w0 = r0 * r0, w2 = (3.f * r2) * r2;
-    const float W = fmaf(6.f * r1, r1, w0 + w2);
-    const float num = fmaf(4.f * w0, t0 - t1, fmaf(2.f, w2, -W) * (t1 - t2));
-    return num * __builtin_amdgcn_rcpf(__builtin_fmaxf(W, 1e-30f));
+    const float ib = fmaf(0.609375f * b, b, e);
+    const float ic = fmaf(0.609375f * c, c, e);
+    const float u0 = fmaf(-1.375f, b, a), u1 = fmaf(-0.625f, c, b), u2 = fmaf(-1.375f, c, d);
+    const float r0 = __builtin_amdgcn_rcpf(fmaf(u0, u0, ib));
+    const float r1 = __builtin_amdgcn_rcpf(fmaf(u1, u1, ic));
+    const float r2 = __builtin_amdgcn_rcpf(fmaf(u2, u2, ic));
+    const float x0 = r0 * r0, x2 = r2 * r2;
+    const float W = fmaf(6.f * r1, r1, fmaf(3.f, x2, x0));
+    const float A = fmaf(-2.f, b, a) + c, B = fmaf(-2.f, c, b) + d;
+    const float P = fmaf(4.f * x0, A, (6.f * x2) * B);
+    return fmaf(P, __builtin_amdgcn_rcpf(__builtin_fmaxf(W, 1e-30f)), -B);
 }
 
 // second differences D1, D2, D3 and the central part C of p[0..6]

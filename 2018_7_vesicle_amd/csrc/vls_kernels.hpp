@@ -208,7 +208,8 @@ hipError_t launch_copy(T* dst, const T* src, int64_t n, hipStream_t s);
 // page, page-major).
 // per-axis constants of the Jiang-Peng form of the uniform WENO5 derivative
 // (vls_device.hpp): k = 1/(12 ds), e = 12/13 * 1e-6 * ds^2, in FP64 (the
-// expressions the CPU oracle evaluates) and rounded to FP32
+// expressions the CPU oracle evaluates); the FP32 form takes its indicators
+// divided by 16/13, so its floor is e * 13/16, rounded to FP32 with k
 struct JPd {
     double k, e;
 };
@@ -224,7 +225,7 @@ __host__ __device__ inline void set_jp(JPd jd[3], JPf jf[3], double dx, double d
     for (int a = 0; a < 3; ++a) {
         jd[a] = jpd(sp[a]);
         jf[a].k = (float)jd[a].k;
-        jf[a].e = (float)jd[a].e;
+        jf[a].e = (float)(jd[a].e * (13.0 / 16.0));
     }
 }
 
