@@ -230,7 +230,8 @@ struct vls_sdf3 {
     double* partial = nullptr;
     double* result = nullptr;
     GraphCache graphs;
-    bool dist_ready = false;  // a set_distance has filled d, min_dist, flags and the list
+    bool dist_ready = false;  // a set_distance has filled flags and the list
+    bool dense_ready = false;  // d and min_dist hold the last set_distance's values (else built on demand)
     double* adj = nullptr;  // adjoint/tangent workspace, grown on demand
     size_t adj_len = 0;
 };
@@ -333,15 +334,18 @@ int reinit_loop(vls_sdf3* h, T* F, int iterations, const DSet& ds, hipStream_t s
     return rc;
 }
 
+// the distances of F into the set `ds`: flags and interface list always;
+// the dense arrays (six distances, and min_dist if given) only if `dense`
+// (the kernels of the WENO5 path read the list; the ENO2 and cubic variants
+// read the dense arrays).  The candidate points go through h->over and
+// h->counts[3], free outside a reinitialization call.
 template <typename T>
 int set_distance_into(vls_sdf3* h, int order, const DSet& ds, double* min_dist, const T* F, const T* mask_src,
-                      hipStream_t s)
+                      bool dense, hipStream_t s)
 {
-    const Geom& g = h->g;
-    VLS_HIP(launch_fill_spacing(ds.d, g, ds.lv.count, s));
-    VLS_HIP(launch_boundary_location<T>(order, ds.d, F, g, s));
-    VLS_HIP(launch_min_dist_flags<T>(min_dist, ds.flags, constify(ds.d), mask_src, g, ds.lv.idx, ds.lv.count, s));
-    VLS_HIP(launch_list_gather(ds.lv, constify(ds.d), g, s));
+    const Dist6 d = dense ? ds.d : Dist6{};
+    VLS_HIP(launch_set_distance<T>(order, d, dense ? min_dist : nullptr, ds.flags, mask_src, F, ds.lv, h->over,
+                                   h->counts + 3, h->g, s));
     return VLS_OK;
 }
 
@@ -369,8 +373,7 @@ int sdf3_weno5_rk3_extend(vls_sdf3* h, const T* F, T* C, int iterations, hipStre
     const DSet ms = main_set(h);
     T *vx = vel<T>(h, 0), *vy = vel<T>(h, 1), *vz = vel<T>(h, 2);
     return with_graph(h->graphs, make_key(2, sizeof(T), F, C, iterations), s, [&](hipStream_t cs) -> int {
-        VLS_HIP(launch_upwind_normal<T>(vx, vy, vz, F, constify(h->d), g, cs));
-        VLS_HIP(launch_extension_velocity<T>(vx, vy, vz, F, 1e-6, g, cs));
+        VLS_HIP(launch_extension_normal<T>(vx, vy, vz, F, h->flags, ms.lv, 1e-6, g, cs));
         VLS_HIP(launch_list_gather_vel<T>(h->lvel, ms.lv, vx, vy, vz, g, cs));
         return extend_loop<T>(h, C, iterations, ms, cs);
     });
@@ -396,12 +399,11 @@ int surface_redistance_issue(vls_sdf3* h, const T* F, T* A, int iter1, int iter2
     int rc;
     // A's distances, deltat and mask (WENO5RK3ClosetPointSurfaceRedistance.m:8-25)
     const DSet as = aux_set(h), fs = f_set(h);
-    if ((rc = set_distance_into<T>(h, 6, as, nullptr, A, A, s))) return rc;
+    if ((rc = set_distance_into<T>(h, 6, as, nullptr, A, A, false, s))) return rc;
     // F's distances and the extension velocity with floor 1e-14 (:30-74)
-    if ((rc = set_distance_into<T>(h, 6, fs, nullptr, F, nullptr, s))) return rc;
+    if ((rc = set_distance_into<T>(h, 6, fs, nullptr, F, nullptr, false, s))) return rc;
     T *vx = vel<T>(h, 0), *vy = vel<T>(h, 1), *vz = vel<T>(h, 2);
-    VLS_HIP(launch_upwind_normal<T>(vx, vy, vz, F, constify(h->fd), g, s));
-    VLS_HIP(launch_extension_velocity<T>(vx, vy, vz, F, 1e-14, g, s));
+    VLS_HIP(launch_extension_normal<T>(vx, vy, vz, F, fs.flags, fs.lv, 1e-14, g, s));
     VLS_HIP(launch_list_gather_vel<T>(h->lvel, fs.lv, vx, vy, vz, g, s));
     T* a1 = tmp<T>(h, 2);
     T* a05 = tmp<T>(h, 3);
@@ -494,13 +496,13 @@ int area_volume(vls_sdf3* h, int kind, const T* F, double eps, double* dev_out, 
 // ---- work memory of the allocation-free reinitialization --------------------
 // One copy per GPU, part of the library's code object: the runtime places it
 // when it loads the code object on a device and it stays until the process
-// ends.  Per grid point: six distance fields, the list-order distances (six
-// rows) and pseudo-time steps, two RK stage fields (15 doubles), the
-// interface list and the deferred points (2 ints) and the flags (1 byte);
-// 16 counters; every piece starts on a 256-byte boundary.
-constexpr int kStaticPieces = 9;
+// ends.  Per grid point: the list-order distances (six rows) and
+// pseudo-time steps, two RK stage fields (9 doubles), the interface list and
+// the deferred points (2 ints) and the flags (1 byte); 16 counters; every
+// piece starts on a 256-byte boundary.
+constexpr int kStaticPieces = 8;
 constexpr size_t kStaticBytes =
-    (size_t)VLS_STATIC_POINTS * (15 * sizeof(double) + 2 * sizeof(int) + 1) + 16 * sizeof(int) + kStaticPieces * 256;
+    (size_t)VLS_STATIC_POINTS * (9 * sizeof(double) + 2 * sizeof(int) + 1) + 16 * sizeof(int) + kStaticPieces * 256;
 static __device__ __align__(256) unsigned char scratch[kStaticBytes];
 
 constexpr int kMaxDevices = 64;
@@ -1029,22 +1031,33 @@ int vls_sdf3_destroy(vls_sdf3_t h)
 int vls_sdf3_set_distance(vls_sdf3_t h, const double* F, vls_stream_t stream)
 {
     VLS_REQUIRE(h && F, "null argument");
-    const int rc = set_distance_into<double>(h, 6, main_set(h), h->min_dist, F, nullptr, S(stream));
+    const int rc = set_distance_into<double>(h, 6, main_set(h), h->min_dist, F, nullptr, false, S(stream));
     if (rc == VLS_OK) h->dist_ready = true;
+    h->dense_ready = false;
     return rc;
 }
 
 int vls_sdf3_set_distance_f32(vls_sdf3_t h, const float* F, vls_stream_t stream)
 {
     VLS_REQUIRE(h && F, "null argument");
-    const int rc = set_distance_into<float>(h, 6, main_set(h), h->min_dist, F, nullptr, S(stream));
+    const int rc = set_distance_into<float>(h, 6, main_set(h), h->min_dist, F, nullptr, false, S(stream));
     if (rc == VLS_OK) h->dist_ready = true;
+    h->dense_ready = false;
     return rc;
 }
 
 int vls_sdf3_distances(vls_sdf3_t h, const double** out7)
 {
     VLS_REQUIRE(h && out7, "null argument");
+    if (h->dist_ready && !h->dense_ready) {
+        // the dense arrays of the last set_distance, from its list, after
+        // all queued work, waited for: the pointers are then ready for any
+        // stream
+        VLS_HIP(hipDeviceSynchronize());
+        VLS_HIP(launch_fill_dense(h->d, h->min_dist, main_set(h).lv, h->g, nullptr));
+        VLS_HIP(hipStreamSynchronize(nullptr));
+        h->dense_ready = true;
+    }
     out7[0] = h->d.r[0]; out7[1] = h->d.l[0];
     out7[2] = h->d.r[1]; out7[3] = h->d.l[1];
     out7[4] = h->d.r[2]; out7[5] = h->d.l[2];
@@ -1083,8 +1096,6 @@ int vls_weno5_rk3_reinit_static(double* F, int iterations, int rows, int cols, i
         used += (bytes + 255) / 256 * 256;
         return p;
     };
-    double* dp = reinterpret_cast<double*>(take(sizeof(double) * 6 * n));
-    h.d = carve6(dp, n);
     h.ldist[0] = reinterpret_cast<double*>(take(sizeof(double) * 6 * n));
     h.ldt[0] = reinterpret_cast<double*>(take(sizeof(double) * n));
     h.t[0] = reinterpret_cast<double*>(take(sizeof(double) * n));
@@ -1098,7 +1109,7 @@ int vls_weno5_rk3_reinit_static(double* F, int iterations, int rows, int cols, i
     // the counters keep the previous call's values: clear them (a kernel, so
     // that a captured call records no memset node)
     VLS_HIP(launch_zero_ints(h.counts, 16, s));
-    if ((rc = set_distance_into<double>(&h, 6, main_set(&h), nullptr, F, nullptr, s))) return rc;
+    if ((rc = set_distance_into<double>(&h, 6, main_set(&h), nullptr, F, nullptr, false, s))) return rc;
     VLS_HIP(launch_set_mask_bit<double>(h.flags, F, h.g, s));
     return reinit_loop<double>(&h, F, iterations, main_set(&h), s);
 }
@@ -1110,7 +1121,7 @@ int vls_sdf3_eno2_reinit(vls_sdf3_t h, double* F, int iterations, int rk2, doubl
     const hipStream_t s = S(stream);
     const Geom& g = h->g;
     // ENO2 distances of the argument and mask = F<0 (ENORK2Reinitialization.m:7-25)
-    int rc = set_distance_into<double>(h, 2, aux_set(h), nullptr, F, F, s);
+    int rc = set_distance_into<double>(h, 2, aux_set(h), nullptr, F, F, true, s);
     if (rc) return rc;
     const Dist6c dc = constify(h->ad);
     if (rk2) {
@@ -1142,7 +1153,7 @@ int vls_sdf3_wenork3_reinit(vls_sdf3_t h, double* F, int iterations, vls_stream_
     return with_graph(h->graphs, make_key(21, F, iterations), S(stream), [&](hipStream_t s) -> int {
         const Geom& g = h->g;
         const DSet as = aux_set(h);
-        int rc = set_distance_into<double>(h, 3, as, nullptr, F, F, s);
+        int rc = set_distance_into<double>(h, 3, as, nullptr, F, F, false, s);
         if (rc) return rc;
         double *t0 = h->t[0], *t1 = h->t[1];
         ReinitSeq rs;
@@ -1173,7 +1184,7 @@ static int variant_extend(vls_sdf3* h, const double* F, double* C, int iteration
         const Geom& g = h->g;
         const int64_t n = g.n();
         const DSet as = aux_set(h);
-        int rc = set_distance_into<double>(h, weno ? 3 : 2, as, nullptr, F, nullptr, cs);
+        int rc = set_distance_into<double>(h, weno ? 3 : 2, as, nullptr, F, nullptr, true, cs);
         if (rc) return rc;
         const Dist6c dc = constify(h->ad);
         double *vx = h->v[0], *vy = h->v[1], *vz = h->v[2];
@@ -1890,9 +1901,9 @@ int sdf3_autotune_try(vls_sdf3* h, const T* F, int iterations, hipStream_t s, in
             h->g.zd = depths[j];
             const Geom& g = h->g;
             VLS_HIP(hipMemcpyAsync(C, F, sizeof(T) * (size_t)g.n(), hipMemcpyDeviceToDevice, s));
-            if ((rc = set_distance_into<T>(h, 6, ms, h->min_dist, F, (const T*)nullptr, s))) return rc;
-            VLS_HIP(launch_upwind_normal<T>(vx, vy, vz, F, constify(h->d), g, s));
-            VLS_HIP(launch_extension_velocity<T>(vx, vy, vz, F, 1e-6, g, s));
+            if ((rc = set_distance_into<T>(h, 6, ms, h->min_dist, F, (const T*)nullptr, false, s))) return rc;
+            h->dense_ready = false;
+            VLS_HIP(launch_extension_normal<T>(vx, vy, vz, F, ms.flags, ms.lv, 1e-6, g, s));
             VLS_HIP(launch_list_gather_vel<T>(h->lvel, ms.lv, vx, vy, vz, g, s));
             if ((rc = extend_loop<T>(h, C, 1, ms, s))) return rc;
             VLS_HIP(hipEventRecord(ev.a, s));
@@ -2112,7 +2123,8 @@ int vls_sdf3_adjoint_reinit(vls_sdf3_t h, const double* F0, int iterations, cons
     double* T1 = T0 + (size_t)K * n;
     const hipStream_t s = S(stream);
     const DSet ms = main_set(h);
-    if ((rc = set_distance_into<double>(h, 6, ms, h->min_dist, F0, (const double*)nullptr, s))) return rc;
+    if ((rc = set_distance_into<double>(h, 6, ms, h->min_dist, F0, (const double*)nullptr, false, s))) return rc;
+    h->dense_ready = false;
     VLS_HIP(launch_set_mask_bit<double>(h->flags, F0, g, s));
     VLS_HIP(launch_copy<double>(X, F0, n, s));
     ReinitSeq rs;
@@ -2171,7 +2183,8 @@ int vls_sdf3_tangent_reinit(vls_sdf3_t h, const double* F0, const double* F0dot,
     double* Xd = X + n;
     const hipStream_t s = S(stream);
     const DSet ms = main_set(h);
-    if ((rc = set_distance_into<double>(h, 6, ms, h->min_dist, F0, (const double*)nullptr, s))) return rc;
+    if ((rc = set_distance_into<double>(h, 6, ms, h->min_dist, F0, (const double*)nullptr, false, s))) return rc;
+    h->dense_ready = false;
     VLS_HIP(launch_set_mask_bit<double>(h->flags, F0, g, s));
     VLS_HIP(launch_adj_fill(Pd, 0.0, 7 * n, s));
     VLS_HIP(launch_tan_dist(Pd, F0, F0dot, ms.lv, g, s));

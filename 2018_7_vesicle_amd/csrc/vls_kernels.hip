@@ -24,16 +24,6 @@ namespace vls {
 // ---------------------------------------------------------------------------
 // setDistance: fill, crossing distances, min_dist + flags
 // ---------------------------------------------------------------------------
-__global__ void k_fill_spacing(Dist6 d, Box b, idx_t n, int* __restrict__ list_count)
-{
-    const idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0 && list_count) *list_count = 0;
-    if (i >= n) return;
-    d.r[0][i] = b.dx; d.l[0][i] = b.dx;
-    d.r[1][i] = b.dy; d.l[1][i] = b.dy;
-    d.r[2][i] = b.dz; d.l[2][i] = b.dz;
-}
-
 // boundary_location (order 6, 1_3_WENO5RK3_Reinitialization/boundary_location.cu:84-132),
 // boundary_correction (order 2, 1_0_ENORK2_Reinitialization/boundary_correction.cu:103-145)
 // and the cubic boundary_correction (order 3,
@@ -68,30 +58,176 @@ __global__ void __launch_bounds__(256) k_boundary_location(Dist6 d, const T* __r
 }
 
 
-// min_dist (setDistance.m:17-21), the per-point flags of the fused kernels and
-// the list of interface points (some distance differs from the spacing).
-// mask_src may be null (no mask bit); list may be null.
+// setDistance (setDistance.m:3-21) in two passes.  The first visits every
+// point: it tests the crossings with the six neighbours (F(i) F(i+-1) < 0 on
+// an axis, the locators' own tests), writes the flags without FLAG_SUBCELL
+// (FLAG_MASK where mask_src < 0) and appends the points with a crossing to
+// `cand`.  The second visits those points only: it locates their crossings,
+// and the points where a distance then differs from the spacing get
+// FLAG_SUBCELL and enter the interface list, in list order, with their six
+// distances and pseudo-time step (the interface kernels read them
+// coalesced).  The dense arrays (six distances and min_dist of every point,
+// the MATLAB object's properties) are written only where asked for (d.r[0]
+// set): spacings by the first pass, located distances by the second.
+//
+// the six neighbour values of a point (clamped at the box faces)
 template <typename T>
-__global__ void k_min_dist_flags(double* __restrict__ min_dist, uint8_t* __restrict__ flags, Dist6c d,
-                                 const T* __restrict__ mask_src, Box b, idx_t n, int* __restrict__ list,
-                                 int* __restrict__ list_count)
+__device__ __forceinline__ void neighbours6(const T* __restrict__ lsf, const Box& b, idx_t ind, int r, int c, int p,
+                                            double up[3], double dn[3])
 {
-    const idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool sub = false;
-    if (i < n) {
-        const double xr = d.r[0][i], xl = d.l[0][i], yf = d.r[1][i], yb = d.l[1][i], zu = d.r[2][i], zd = d.l[2][i];
-        if (min_dist) min_dist[i] = dmin(dmin(xr, xl), dmin(dmin(yf, yb), dmin(zu, zd)));
-        // a distance differs from the spacing (below it; the quadratic ENO
-        // locator is not clamped, so a root within rounding of the far node
-        // can land just above it, where the ENO2 schemes treat it as a crossing)
-        sub = xr != b.dx || xl != b.dx || yf != b.dy || yb != b.dy || zu != b.dz || zd != b.dz;
-        if (flags) {
-            uint8_t f = sub ? FLAG_SUBCELL : 0;
-            if (mask_src && mask_src[i] < 0) f |= FLAG_MASK;
-            flags[i] = f;
+    const idx_t rows = (idx_t)b.rows, plane = rows * b.cols;
+    up[0] = (double)lsf[ind + (idx_t)(clampi(c + 1, b.cols - 1) - c) * rows];
+    dn[0] = (double)lsf[ind + (idx_t)(clampi(c - 1, b.cols - 1) - c) * rows];
+    up[1] = (double)lsf[ind + (idx_t)(clampi(r + 1, b.rows - 1) - r)];
+    dn[1] = (double)lsf[ind + (idx_t)(clampi(r - 1, b.rows - 1) - r)];
+    up[2] = (double)lsf[ind + (idx_t)(clampi(p + 1, b.pges - 1) - p) * plane];
+    dn[2] = (double)lsf[ind + (idx_t)(clampi(p - 1, b.pges - 1) - p) * plane];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_crossings(Dist6 d, double* __restrict__ min_dist,
+                                                   uint8_t* __restrict__ flags, const T* __restrict__ mask_src,
+                                                   const T* __restrict__ lsf, Box b, int* __restrict__ cand,
+                                                   int* __restrict__ cand_count)
+{
+    int r, c, p;
+    const bool in = point_of_thread(b, r, c, p);
+    bool cross = false;
+    idx_t ind = 0;
+    if (in) {
+        ind = lin(b, r, c, p);
+        const double f0 = (double)lsf[ind];
+        double up[3], dn[3];
+        neighbours6(lsf, b, ind, r, c, p, up, dn);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) cross |= (f0 * up[a] < 0) | (dn[a] * f0 < 0);
+        if (flags) flags[ind] = (mask_src && mask_src[ind] < 0) ? (uint8_t)FLAG_MASK : (uint8_t)0;
+        if (d.r[0]) {
+            d.r[0][ind] = b.dx; d.l[0][ind] = b.dx;
+            d.r[1][ind] = b.dy; d.l[1][ind] = b.dy;
+            d.r[2][ind] = b.dz; d.l[2][ind] = b.dz;
+            if (min_dist) min_dist[ind] = dmin(dmin(b.dx, b.dx), dmin(dmin(b.dy, b.dy), dmin(b.dz, b.dz)));
         }
     }
-    if (list) wave_append(sub, (int)i, list, list_count);
+    wave_append(cross, (int)ind, cand, cand_count);
+}
+
+// the second pass, one lane per candidate.  The r-side distance on an axis is
+// the locator at the point (crossing with the +1 neighbour); the l-side one
+// is ds minus the locator of the -1 neighbour's crossing, evaluated here
+// from that neighbour's stencil (the point's own line shifted by one,
+// clamped the same way), so it is the value k_boundary_location stores from
+// the neighbour's thread.  FLAG_SUBCELL marks a distance that differs from
+// the spacing: below it, or, for the quadratic ENO locator, which is not
+// clamped, a root within rounding of the far node that lands just above it,
+// where the ENO2 schemes treat it as a crossing.
+template <int ORDER, typename T>
+__global__ void __launch_bounds__(256) k_crossing_distances(Dist6 d, double* __restrict__ min_dist,
+                                                            uint8_t* __restrict__ flags, const T* __restrict__ lsf,
+                                                            Box b, const int* __restrict__ cand,
+                                                            const int* __restrict__ cand_count, ListView lv)
+{
+    const int total = *cand_count;
+    const double sp[3] = {b.dx, b.dy, b.dz};
+    // whole waves per sweep, so that the list append sees every lane
+    for (int j0 = (int)(blockIdx.x * 256); j0 < total; j0 += (int)(gridDim.x * 256)) {
+        const int j = j0 + (int)threadIdx.x;
+        bool sub = false;
+        idx_t ind = 0;
+        Dists q;
+        if (j < total) {
+            ind = (idx_t)cand[j];
+            int r, c, p;
+            coords_of(b, ind, r, c, p);
+            const idx_t plane = (idx_t)b.rows * b.cols;
+            const double f0 = (double)lsf[ind];
+#pragma unroll 1
+            for (int a = 0; a < 3; ++a) {
+                const double ds = sp[a];
+                q.r[a] = ds;
+                q.l[a] = ds;
+                const int pos = a == 0 ? c : (a == 1 ? r : p);
+                const int last = a == 0 ? b.cols - 1 : (a == 1 ? b.rows - 1 : b.pges - 1);
+                const idx_t stride = a == 0 ? (idx_t)b.rows : (a == 1 ? (idx_t)1 : plane);
+                const idx_t base = ind - (idx_t)pos * stride;
+                double v[8];  // the line at offsets -4..+3, clamped
+#pragma unroll
+                for (int s = -4; s <= 3; ++s) v[s + 4] = (double)lsf[base + (idx_t)clampi(pos + s, last) * stride];
+                const double fu = v[5], fd = v[3];
+                // the locator at the point: its line is v[1..7], its f2 = fu
+                if (f0 * fu < 0)
+                    q.r[a] = (ORDER == 2)   ? eno_quadratic_distance(v[3], f0, fu, v[6], ds)
+                             : (ORDER == 3) ? cubic_distance(v[3], f0, fu, v[6], ds)
+                                            : six_distance(v[2], v[3], f0, fu, v[6], v[7], ds);
+                // the -1 neighbour's locator: its line is v[0..6], its f0 = fd, its f2 = f0
+                if (fd * f0 < 0)
+                    q.l[a] = ds - ((ORDER == 2)   ? eno_quadratic_distance(v[2], fd, f0, v[5], ds)
+                                   : (ORDER == 3) ? cubic_distance(v[2], fd, f0, v[5], ds)
+                                                  : six_distance(v[1], v[2], fd, f0, v[5], v[6], ds));
+            }
+            sub = q.r[0] != b.dx || q.l[0] != b.dx || q.r[1] != b.dy || q.l[1] != b.dy || q.r[2] != b.dz ||
+                  q.l[2] != b.dz;
+            if (sub) {
+                if (flags) flags[ind] |= (uint8_t)FLAG_SUBCELL;
+                if (d.r[0]) {
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        d.r[a][ind] = q.r[a];
+                        d.l[a][ind] = q.l[a];
+                    }
+                    if (min_dist) min_dist[ind] = min6(q);
+                }
+            }
+        }
+        if (!lv.idx) continue;
+        const int slot = wave_append_slot(sub, lv.count);
+        if (slot < 0) continue;
+        lv.idx[slot] = (int)ind;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            lv.dist[(size_t)(2 * a) * lv.cap + slot] = q.r[a];
+            lv.dist[(size_t)(2 * a + 1) * lv.cap + slot] = q.l[a];
+        }
+        lv.dt[slot] = frozen(q, b.dx) ? -1.0 : 0.3 * min6(q);
+    }
+}
+
+// the dense arrays of a lazily located distance set: spacings everywhere,
+// then the listed points' distances (every point with a distance that
+// differs from the spacing is listed, with all six)
+__global__ void k_fill_dense(Dist6 d, double* __restrict__ min_dist, Box b, idx_t n)
+{
+    const idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    d.r[0][i] = b.dx; d.l[0][i] = b.dx;
+    d.r[1][i] = b.dy; d.l[1][i] = b.dy;
+    d.r[2][i] = b.dz; d.l[2][i] = b.dz;
+    min_dist[i] = dmin(dmin(b.dx, b.dx), dmin(dmin(b.dy, b.dy), dmin(b.dz, b.dz)));
+}
+
+__global__ void __launch_bounds__(256) k_scatter_list(Dist6 d, double* __restrict__ min_dist, IList il)
+{
+    const int cnt = *il.count;
+    for (int j = (int)(blockIdx.x * 256 + threadIdx.x); j < cnt; j += (int)(gridDim.x * 256)) {
+        const idx_t ind = (idx_t)il.idx[j];
+        Dists q;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            q.r[a] = il.dist[(size_t)(2 * a) * il.cap + j];
+            q.l[a] = il.dist[(size_t)(2 * a + 1) * il.cap + j];
+            d.r[a][ind] = q.r[a];
+            d.l[a][ind] = q.l[a];
+        }
+        min_dist[ind] = min6(q);
+    }
+}
+
+__global__ void k_zero_pair(int* __restrict__ a, int* __restrict__ b)
+{
+    if (threadIdx.x == 0) {
+        *a = 0;
+        if (b) *b = 0;
+    }
 }
 
 __global__ void k_zero_ints(int* __restrict__ a, int n)
@@ -223,23 +359,6 @@ __global__ void __launch_bounds__(256) k_reinit_deferred(T* __restrict__ out, co
 // interface lists: per-point data in list order, so the interface kernels
 // read it coalesced
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_list_gather(IListW il, Dist6c d, Box b)
-{
-    const int cnt = *il.count;
-    for (int j = (int)(blockIdx.x * 256 + threadIdx.x); j < cnt; j += (int)(gridDim.x * 256)) {
-        const idx_t ind = (idx_t)il.idx[j];
-        Dists q;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            q.r[a] = d.r[a][ind];
-            q.l[a] = d.l[a][ind];
-            il.dist[(size_t)(2 * a) * il.cap + j] = q.r[a];
-            il.dist[(size_t)(2 * a + 1) * il.cap + j] = q.l[a];
-        }
-        il.dt[j] = frozen(q, b.dx) ? -1.0 : 0.3 * min6(q);
-    }
-}
-
 // boundary_interpolant (boundary_interpolant.cu:102-141) at the listed points
 // only (it is zero elsewhere), one lane per (point, axis), in list order:
 // cp[2a][j] = fore value, cp[2a+1][j] = back value
@@ -394,6 +513,83 @@ __global__ void k_extension_velocity(T* __restrict__ vx, T* __restrict__ vy, T* 
     vx[i] = (T)(s * (fx / g));
     vy[i] = (T)(s * (fy / g));
     vz[i] = (T)(s * (fz / g));
+}
+
+// The extension velocity (WENO5RK3Extend.m:19-31) in two launches over
+// disjoint points: per axis the component of upwind_normal.cu:4-104 (the
+// fore derivative where |F(+1)| < |F(-1)|, else the back one; only that side
+// is evaluated), rounded to T as upwind_normal stores it, then
+// v = sign(F) n / max(|n|, floor) as k_extension_velocity computes it.  The
+// first launch takes the points without a subcell distance (uniform WENO5 on
+// every axis), the second the listed points (the nonuniform form on an axis
+// with a subcell distance, read from the list).
+__device__ __forceinline__ double upwind_normal_axis(const double v[7], double ds, bool fore)
+{
+    const JPd j = jpd(ds);
+    const JPcd<double> m = jp_centre(v);
+    return fore ? (m.c + jp_q(m.d[4], m.d[3], m.d[2], m.d[1], j.e)) * j.k
+                : (m.c - jp_q(m.d[0], m.d[1], m.d[2], m.d[3], j.e)) * j.k;
+}
+
+template <typename T>
+__device__ __forceinline__ void store_extension_velocity(T* __restrict__ vx, T* __restrict__ vy,
+                                                         T* __restrict__ vz, idx_t ind, const double n[3], double f,
+                                                         double floor_)
+{
+    const double g = dmax(sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]), floor_);
+    const double sg = msign(f);
+    vx[ind] = (T)(sg * (n[0] / g));
+    vy[ind] = (T)(sg * (n[1] / g));
+    vz[ind] = (T)(sg * (n[2] / g));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_extension_normal(T* __restrict__ vx, T* __restrict__ vy,
+                                                          T* __restrict__ vz, const T* __restrict__ lsf,
+                                                          const uint8_t* __restrict__ flags, double floor_, Box b)
+{
+    int r, c, p;
+    if (!point_of_thread(b, r, c, p)) return;
+    const idx_t ind = lin(b, r, c, p);
+    if (flags[ind] & FLAG_SUBCELL) return;  // a listed point
+    const double sp[3] = {b.dx, b.dy, b.dz};
+    double n[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        double v[7];
+        line7(lsf, r, c, p, a, b, v);
+        n[a] = (double)(T)upwind_normal_axis(v, sp[a], fabs(v[4]) < fabs(v[2]));
+    }
+    store_extension_velocity(vx, vy, vz, ind, n, (double)lsf[ind], floor_);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_extension_normal_list(T* __restrict__ vx, T* __restrict__ vy,
+                                                               T* __restrict__ vz, const T* __restrict__ lsf,
+                                                               IList il, double floor_, Box b)
+{
+    const int cnt = *il.count;
+    const double sp[3] = {b.dx, b.dy, b.dz};
+    for (int j = (int)(blockIdx.x * 256 + threadIdx.x); j < cnt; j += (int)(gridDim.x * 256)) {
+        const idx_t ind = (idx_t)il.idx[j];
+        int r, c, p;
+        coords_of(b, ind, r, c, p);
+        double n[3];
+#pragma unroll 1
+        for (int a = 0; a < 3; ++a) {
+            double v[7];
+            line7(lsf, r, c, p, a, b, v);
+            const bool fore = fabs(v[4]) < fabs(v[2]);
+            const double ds = sp[a];
+            const double r1 = il.dist[(size_t)(2 * a) * il.cap + j], l1 = il.dist[(size_t)(2 * a + 1) * il.cap + j];
+            const double dv = (r1 < ds || l1 < ds)
+                                  ? weno_nonuniform_one(v[0], v[1], v[2], v[3], v[4], v[5], v[6], r1, l1, ds, 0.0,
+                                                        0.0, !fore)
+                                  : upwind_normal_axis(v, ds, fore);
+            n[a] = (double)(T)dv;
+        }
+        store_extension_velocity(vx, vy, vz, ind, n, (double)lsf[ind], floor_);
+    }
 }
 
 // boundary_interpolant: boundary_interpolant.cu:102-141
@@ -977,12 +1173,6 @@ static inline dim3 flat_grid(int64_t n) { return dim3((unsigned)((n + 255) / 256
         return hipGetLastError();                                       \
     } while (0)
 
-hipError_t launch_fill_spacing(const Dist6& d, const Geom& g, int* list_count, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_fill_spacing, flat_grid(g.n()), dim3(256), 0, s, d, box_of(g), (idx_t)g.n(), list_count);
-    return hipGetLastError();
-}
-
 template <typename T>
 hipError_t launch_boundary_location(int order, const Dist6& d, const T* lsf, const Geom& g, hipStream_t s)
 {
@@ -995,11 +1185,29 @@ hipError_t launch_boundary_location(int order, const Dist6& d, const T* lsf, con
 }
 
 template <typename T>
-hipError_t launch_min_dist_flags(double* min_dist, uint8_t* flags, const Dist6c& d, const T* mask_src,
-                                 const Geom& g, int* list, int* list_count, hipStream_t s)
+hipError_t launch_set_distance(int order, const Dist6& d, double* min_dist, uint8_t* flags, const T* mask_src,
+                               const T* lsf, const ListView& lv, int* cand, int* cand_count, const Geom& g,
+                               hipStream_t s)
 {
-    k_min_dist_flags<T><<<flat_grid(g.n()), dim3(256), 0, s>>>(min_dist, flags, d, mask_src, box_of(g),
-                                                                 (idx_t)g.n(), list, list_count);
+    dim3 grid, block;
+    point_grid(g, grid, block);
+    k_zero_pair<<<dim3(1), dim3(64), 0, s>>>(cand_count, lv.idx ? lv.count : nullptr);
+    k_crossings<T><<<grid, block, 0, s>>>(d, min_dist, flags, mask_src, lsf, box_of(g), cand, cand_count);
+    const dim3 cg(iface_blocks(g));
+    if (order == 2)
+        k_crossing_distances<2, T><<<cg, dim3(256), 0, s>>>(d, min_dist, flags, lsf, box_of(g), cand, cand_count, lv);
+    else if (order == 3)
+        k_crossing_distances<3, T><<<cg, dim3(256), 0, s>>>(d, min_dist, flags, lsf, box_of(g), cand, cand_count, lv);
+    else
+        k_crossing_distances<6, T><<<cg, dim3(256), 0, s>>>(d, min_dist, flags, lsf, box_of(g), cand, cand_count, lv);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_dense(const Dist6& d, double* min_dist, const ListView& lv, const Geom& g, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill_dense, flat_grid(g.n()), dim3(256), 0, s, d, min_dist, box_of(g), (idx_t)g.n());
+    const IList il{lv.idx, lv.count, lv.dist, lv.dt, lv.cap};
+    hipLaunchKernelGGL(k_scatter_list, dim3(iface_blocks(g)), dim3(256), 0, s, d, min_dist, il);
     return hipGetLastError();
 }
 
@@ -1123,13 +1331,6 @@ hipError_t launch_cubic_reinit_stage(double* out, const double* fs, const double
     return reinit_stage_scheme<double, IFACE_CUBIC>(out, fs, fn, flags, lv, g, stage, over, over_count, seq, s);
 }
 
-hipError_t launch_list_gather(const ListView& lv, const Dist6c& d, const Geom& g, hipStream_t s)
-{
-    const IListW il{lv.idx, lv.count, lv.dist, lv.dt, lv.cap};
-    hipLaunchKernelGGL(k_list_gather, dim3(iface_blocks(g)), dim3(256), 0, s, il, d, box_of(g));
-    return hipGetLastError();
-}
-
 template <typename T>
 hipError_t launch_list_interp(double* lcp, const ListView& lv, const T* C, const Geom& g, hipStream_t s)
 {
@@ -1166,6 +1367,18 @@ hipError_t launch_upwind_normal(T* nx, T* ny, T* nz, const T* lsf, const Dist6c&
     dim3 grid, block;
     point_grid(g, grid, block);
     k_upwind_normal<T><<<grid, block, 0, s>>>(nx, ny, nz, lsf, d, box_of(g));
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_extension_normal(T* vx, T* vy, T* vz, const T* F, const uint8_t* flags, const ListView& lv,
+                                   double floor_, const Geom& g, hipStream_t s)
+{
+    dim3 grid, block;
+    point_grid(g, grid, block);
+    k_extension_normal<T><<<grid, block, 0, s>>>(vx, vy, vz, F, flags, floor_, box_of(g));
+    const IList il{lv.idx, lv.count, lv.dist, lv.dt, lv.cap};
+    k_extension_normal_list<T><<<dim3(iface_blocks(g)), dim3(256), 0, s>>>(vx, vy, vz, F, il, floor_, box_of(g));
     return hipGetLastError();
 }
 
@@ -1330,9 +1543,9 @@ hipError_t launch_copy(T* dst, const T* src, int64_t n, hipStream_t s)
 // instantiations for FP64 fields and FP32 (bulk) fields
 #define VLS_INSTANTIATE(T)                                                                                          \
     template hipError_t launch_boundary_location<T>(int, const Dist6&, const T*, const Geom&, hipStream_t);         \
-    template hipError_t launch_min_dist_flags<T>(double*, uint8_t*, const Dist6c&, const T*, const Geom&, int*,     \
-                                                 int*, hipStream_t);                                                \
     template hipError_t launch_set_mask_bit<T>(uint8_t*, const T*, const Geom&, hipStream_t);                       \
+    template hipError_t launch_set_distance<T>(int, const Dist6&, double*, uint8_t*, const T*, const T*,            \
+                                               const ListView&, int*, int*, const Geom&, hipStream_t);              \
     template hipError_t launch_reinit_stage<T>(T*, const T*, const T*, const uint8_t*, const ListView&, const Geom&, \
                                                int, int*, int*, int, hipStream_t);                                  \
     template hipError_t launch_list_interp<T>(double*, const ListView&, const T*, const Geom&, hipStream_t);        \
@@ -1340,6 +1553,8 @@ hipError_t launch_copy(T* dst, const T* src, int64_t n, hipStream_t s)
                                                   const Geom&, hipStream_t);                                        \
     template hipError_t launch_upwind_normal<T>(T*, T*, T*, const T*, const Dist6c&, const Geom&, hipStream_t);     \
     template hipError_t launch_extension_velocity<T>(T*, T*, T*, const T*, double, const Geom&, hipStream_t);       \
+    template hipError_t launch_extension_normal<T>(T*, T*, T*, const T*, const uint8_t*, const ListView&, double,   \
+                                                   const Geom&, hipStream_t);                                       \
     template hipError_t launch_extend_stage<T>(T*, const T*, const T*, const T*, const T*, const T*,               \
                                                const uint8_t*, const ListView&, const double*, const double*,       \
                                                const Geom&, int, hipStream_t);                                      \

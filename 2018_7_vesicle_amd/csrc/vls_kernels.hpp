@@ -62,19 +62,26 @@ struct Geom {
     int64_t n() const { return (int64_t)rows * cols * pges; }
 };
 
-// pre-fill with the spacings; also zeroes *list_count (may be null)
-hipError_t launch_fill_spacing(const Dist6& d, const Geom& g, int* list_count, hipStream_t s);
 // Launchers templated on T take fields of T = double or float (FP32 bulk: the
 // stage kernels' bulk arithmetic is FP32; subcell distances, interface-list
 // arithmetic and reductions stay FP64).
 template <typename T>
 hipError_t launch_boundary_location(int order, const Dist6& d, const T* lsf, const Geom& g, hipStream_t s);
-// min_dist (may be null), flags (may be null) and the list of points with a
-// subcell distance (list may be null; *list_count must be 0 on entry)
-template <typename T>
-hipError_t launch_min_dist_flags(double* min_dist, uint8_t* flags, const Dist6c& d, const T* mask_src,
-                                 const Geom& g, int* list, int* list_count, hipStream_t s);
 hipError_t launch_zero_ints(int* a, int n, hipStream_t s);
+// setDistance: the flags (FLAG_SUBCELL where a distance differs from the
+// spacing, FLAG_MASK where mask_src < 0; mask_src may be null), the
+// interface list with its list-order distances and time steps (lv.idx null:
+// no list), and, if d.r[0] is set, the dense distance arrays and min_dist
+// (may be null).  cand (n ints) and *cand_count are work memory.  Four
+// launches: zero the counts, a pass over every point, a pass over the points
+// with a crossing.
+template <typename T>
+hipError_t launch_set_distance(int order, const Dist6& d, double* min_dist, uint8_t* flags, const T* mask_src,
+                               const T* lsf, const ListView& lv, int* cand, int* cand_count, const Geom& g,
+                               hipStream_t s);
+// the dense arrays (six distances, min_dist) of a distance set located
+// without them, from its interface list
+hipError_t launch_fill_dense(const Dist6& d, double* min_dist, const ListView& lv, const Geom& g, hipStream_t s);
 int iface_blocks(const Geom& g);
 // pages per block of the z-marching bulk stage kernels (g.zd, or chosen by the box size)
 int zmarch_depth(const Geom& g);
@@ -90,8 +97,6 @@ hipError_t launch_eno2_re_step(double* step, const double* lsf, const uint8_t* m
 template <typename T>
 hipError_t launch_reinit_stage(T* out, const T* fs, const T* fn, const uint8_t* flags, const ListView& lv,
                                const Geom& g, int stage, int* over, int* over_count, int seq, hipStream_t s);
-// fill the list-ordered distances and steps of `lv` from the full arrays
-hipError_t launch_list_gather(const ListView& lv, const Dist6c& d, const Geom& g, hipStream_t s);
 // interface values of C at the listed points, list order (6 rows of cap)
 template <typename T>
 hipError_t launch_list_interp(double* lcp, const ListView& lv, const T* C, const Geom& g, hipStream_t s);
@@ -103,6 +108,11 @@ hipError_t launch_eno2_stage(double* out, const double* fs, const double* fn, co
                              const Dist6c& d, const Geom& g, double cfl, int heun2, hipStream_t s);
 template <typename T>
 hipError_t launch_upwind_normal(T* nx, T* ny, T* nz, const T* lsf, const Dist6c& d, const Geom& g, hipStream_t s);
+// upwind normal and extension velocity of F (the two below, fused): the
+// points without FLAG_SUBCELL, then the listed ones (distances from the list)
+template <typename T>
+hipError_t launch_extension_normal(T* vx, T* vy, T* vz, const T* F, const uint8_t* flags, const ListView& lv,
+                                   double floor_, const Geom& g, hipStream_t s);
 template <typename T>
 hipError_t launch_extension_velocity(T* vx, T* vy, T* vz, const T* F, double floor_, const Geom& g, hipStream_t s);
 

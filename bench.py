@@ -67,6 +67,9 @@ def parse():
     ap.add_argument("--pmc-summary", nargs="+", default=None, metavar="DB",
                     help="print per-kernel averages of the counters in rocprofv3 --pmc result databases "
                          "(the last three dispatches of each kernel) and exit")
+    ap.add_argument("--trace-csv", default=None, metavar="CSV",
+                    help="with --pmc-summary: kernel trace (rocprofv3 --kernel-trace --output-format csv) of a "
+                         "run of the same probe, for launch times and VALU/HBM fractions")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--precision-steps", type=int, default=200,
                     help="steps of the 256^3 FP32-bulk vs FP64 comparison (0 skips it)")
@@ -93,8 +96,8 @@ def probe_stages(args):
     autotuning), then each runs one extension iteration, one reinitialization
     iteration and one advection step (3 launches of each stage kernel), with a
     synchronization between the phases; so the last three launches of every
-    stage kernel are the probe's own (k_*_stage<R, float> at 256^3,
-    <R, double> at 128^3).  Last, a 256 MiB float64 copy whose source was
+    stage kernel are the probe's own (k_*_stage<32, float> at 256^3 with 16
+    pages per block, <32, double> at 128^3 with 8).  Last, a 256 MiB float64 copy whose source was
     pushed out of the caches by a 2 GiB fill: FETCH_SIZE and WRITE_SIZE of a
     known number of bytes, to calibrate them."""
     import importlib
@@ -105,8 +108,12 @@ def probe_stages(args):
     pl = vls.pipeline
     out = {"probe": "stages"}
     lines = []
-    for n, prec, modes in ((256, "fp32", 32), (128, "fp64", 16)):
-        p = pl.Pipeline(pl.PipelineConfig(n=n, precision=prec, modes=modes, initial_reinit=20))
+    # a fixed launch shape (the one the autotuner picks most often for these
+    # grids), so that every profiled run launches the same kernels
+    for n, prec, modes, tr, zd in ((256, "fp32", 32, 32, 16), (128, "fp64", 16, 32, 8)):
+        p = pl.Pipeline(pl.PipelineConfig(n=n, precision=prec, modes=modes, initial_reinit=20, autotune=False))
+        p.map.setTileRows(tr)
+        p.map.setZmarchDepth(zd)
         p.map.setDistance()
         p.map.normalVelocity(*p.u, p.Vn)
         lines.append((f"{prec}_{n}", p))
@@ -175,15 +182,42 @@ def valu_rate(vls, fp64):
     return v.value
 
 
-def pmc_summary(paths):
+# algorithmic bytes per point of one stage launch (reads and writes of the
+# operator's fields), FP64 and FP32 bulk
+B_ALG_STAGE = {("extend", "fp64"): B_ALG_EXTEND_FP64, ("extend", "fp32"): B_ALG_EXTEND_FP32,
+               ("reinit", "fp64"): 29.3, ("reinit", "fp32"): 18.7,
+               ("advect", "fp64"): 45.3, ("advect", "fp32"): 22.7}
+CLOCK_HZ = 2.4e9  # MI355X peak engine clock
+CUS = 256
+
+
+def trace_durations(path):
+    """kernel name -> durations (us) of its dispatches in dispatch order, from a
+    rocprofv3 --kernel-trace --output-format csv kernel trace"""
+    import collections
+    import csv
+
+    rows = list(csv.DictReader(open(path)))
+    out = collections.defaultdict(list)
+    for r in sorted(rows, key=lambda r: int(r["Dispatch_Id"])):
+        name = r["Kernel_Name"].split("(")[0].replace("void ", "")
+        out[name].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+    return out
+
+
+def pmc_summary(paths, trace=None):
     """Per kernel and counter, the average over the last three dispatches of
     that kernel in rocprofv3 --pmc result databases of `bench.py --probe
-    stages` (the probe's own launches come after set-up and autotuning).
-    Prints one JSON object: the raw table, and per stage kernel (extension,
-    reinitialization, advection at 128^3 FP64 and 256^3 FP32) its VALU, SALU,
-    LDS and memory instructions per launch and its HBM bytes per launch,
-    FETCH_SIZE and WRITE_SIZE (KiB) scaled by what they read for the probe's
-    256 MiB copy (the largest elementwise kernel of the run)."""
+    stages` (the probe's own launches come after set-up).  Prints one JSON
+    object: the raw table, and per stage kernel (extension, reinitialization,
+    advection at 128^3 FP64 and 256^3 FP32) its VALU, SALU, LDS and memory
+    instructions per launch and its HBM bytes per launch, FETCH_SIZE and
+    WRITE_SIZE (KiB) scaled by what they read for the probe's 256 MiB copy
+    (the run's last copy kernel).  With `trace` (the kernel trace of a run of
+    the same probe without counters) each stage also gets its launch time
+    (last three dispatches), the VALU issue fraction (VALU wave-instructions
+    over one per CU per clock at 2.4 GHz) and the HBM fractions of its measured
+    traffic and of its algorithmic bytes (8 TB/s)."""
     import collections
     import sqlite3
 
@@ -232,7 +266,29 @@ def pmc_summary(paths):
                 rec["traffic_bytes"] = 1024.0 * (rec["fetch_kib"] * calib["FETCH_SIZE"] +
                                                  rec["write_kib"] * calib["WRITE_SIZE"])
             stages[key] = rec
-    print(json.dumps({"dbs": paths, "calibration": calib, "stages": stages, "kernels": table}), flush=True)
+    if trace:
+        durations = trace_durations(trace)
+        for key, rec in stages.items():
+            stage, prec, n = key.split("_")
+            d = durations.get(rec["kernel"])
+            if not d:
+                continue
+            us = sum(d[-3:]) / len(d[-3:])
+            pts = int(n) ** 3
+            rec["avg_us"] = us
+            rec["points"] = pts
+            rec["alg_bytes_per_point"] = B_ALG_STAGE[(stage, prec)]
+            rec["alg_hbm_frac"] = rec["alg_bytes_per_point"] * pts / (us * 1e-6) / (HBM_PEAK_GBS * 1e9)
+            if "valu_insts" in rec:
+                rec["valu_per_point"] = rec["valu_insts"] * 64 / pts
+                rec["valu_frac"] = rec["valu_insts"] / (CUS * CLOCK_HZ * us * 1e-6)
+            if "salu_insts" in rec:
+                rec["salu_frac"] = rec["salu_insts"] / (CUS * CLOCK_HZ * us * 1e-6)
+            if "traffic_bytes" in rec:
+                rec["traffic_bytes_per_point"] = rec["traffic_bytes"] / pts
+                rec["traffic_hbm_frac"] = rec["traffic_bytes"] / (us * 1e-6) / (HBM_PEAK_GBS * 1e9)
+    print(json.dumps({"dbs": paths, "trace": trace, "calibration": calib, "stages": stages, "kernels": table}),
+          flush=True)
     return 0
 
 
@@ -605,7 +661,7 @@ def main():
         torch.cuda.synchronize()
         return 0
     if args.pmc_summary:
-        return pmc_summary(args.pmc_summary)
+        return pmc_summary(args.pmc_summary, args.trace_csv)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported by this build "
               "(the multi-GPU z-slab decomposition is a design note, not code)", file=sys.stderr)

@@ -269,10 +269,16 @@ int vls_sdf3_set_zmarch_depth(vls_sdf3_t h, int depth);
 int vls_sdf3_get_launch(vls_sdf3_t h, int *tile_rows, int *depth);
 
 /* setDistance (+SD/@SDF3/setDistance.m:3-21): subcell distances of F with the
- * sixth-order locator, and min_dist.  Stored in the handle. */
+ * sixth-order locator, and min_dist.  Stored in the handle: the points with
+ * a subcell distance, with their six distances, in an interface list (what
+ * the reinitialization, extension and redistance read); the dense arrays are
+ * built from it when vls_sdf3_distances asks for them. */
 int vls_sdf3_set_distance(vls_sdf3_t h, const double *F, vls_stream_t stream);
 
-/* device pointers of the handle's xpr,xpl,ypf,ypb,zpu,zpd,min_dist (7 entries, read-only) */
+/* device pointers of the handle's xpr,xpl,ypf,ypb,zpu,zpd,min_dist (7 entries,
+ * read-only) as setDistance.m leaves them.  The first call after a
+ * set_distance builds them (spacings, then the listed points' distances)
+ * after a device-wide synchronization, and returns once they are written. */
 int vls_sdf3_distances(vls_sdf3_t h, const double **out7);
 
 /* WENO5RK3Reinitialization (+SD/@SDF3/WENO5RK3Reinitialization.m:3-43): F in
@@ -391,8 +397,8 @@ int vls_sdf3_normal_velocity_f32(vls_sdf3_t h, const float *F, const float *vx, 
  * Same result, bit for bit, as vls_sdf3_create + vls_sdf3_set_distance(F) +
  * vls_sdf3_weno5_rk3_reinit(F, iterations) on a fresh handle: the distances
  * of F, then `iterations` RK3 iterations of F in place, on `stream`.  Its work
- * memory is one static array per GPU (about 129 bytes per point for
- * VLS_STATIC_POINTS points, 129 MiB, held by each GPU that loads the library
+ * memory is one static array per GPU (about 81 bytes per point for
+ * VLS_STATIC_POINTS points, 81 MiB, held by each GPU that loads the library
  * until the process ends); a box of more points is rejected (VLS_ERR_INVALID).
  * It calls no hipMalloc/hipFree and records no memset.  Every call on one GPU
  * shares that GPU's array: calls that overlap on one GPU (other streams or
