@@ -79,6 +79,7 @@ SIGNATURES = {
     "vls_sdf3_normal_velocity_f32": [P] * 7,
     "vls_set_graphs": [I],
     "vls_probe_valu_rate": [I, PD, P],
+    "vls_probe_stream_rate": [ctypes.c_int64, PD, P],
     "vls_batch_create": [P, I, P, P, P, P, P, P],
     "vls_batch_destroy": [P],
     "vls_batch_layout": [P, P, P],
@@ -113,6 +114,12 @@ for _sfx in ("", "_f32"):
         "vls_batch_ellipsoid" + _sfx: [P, P, P, P, P],
         "vls_batch_step" + _sfx: [P, P, P, P, P, P, P, I, I, D, P, P],
     })
+
+SIGNATURES.update({
+    "vls_batch_wenork3_reinit": [P, P, I, P],
+    "vls_batch_wenork3_extend": [P, P, P, I, P],
+    "vls_batch_enork2_extend": [P, P, P, I, P],
+})
 
 _lib = None
 

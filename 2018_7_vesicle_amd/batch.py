@@ -154,6 +154,28 @@ class Batch:
     def extend(self, F, C, iterations):
         call("vls_batch_weno5_rk3_extend" + self._sfx, self._h, self._f(F), self._f(C), int(iterations), _stream())
 
+    # the WENO-RK3 / ENO-RK2 variants of every grid (FP64 batches; each
+    # computes its own distances of F, and leaves set_distance to be rerun
+    # before a WENO5 reinitialization or extension)
+    def _fp64(self, name):
+        if self.dtype != torch.float64:
+            raise VLSError(f"{name} runs on FP64 batches only")
+
+    def wenork3_reinitialize(self, F, iterations):
+        """WENORK3Reinitialization of every grid (+SD/@SDF3/WENORK3Reinitialization.m)"""
+        self._fp64("wenork3_reinitialize")
+        call("vls_batch_wenork3_reinit", self._h, self._f(F), int(iterations), _stream())
+
+    def wenork3_extend(self, F, C, iterations):
+        """WENORK3Extend of C off F on every grid (+SD/@SDF3/WENORK3Extend.m)"""
+        self._fp64("wenork3_extend")
+        call("vls_batch_wenork3_extend", self._h, self._f(F), self._f(C), int(iterations), _stream())
+
+    def enork2_extend(self, F, C, iterations):
+        """ENORK2Extend of C off F on every grid (+SD/@SDF3/ENORK2Extend.m)"""
+        self._fp64("enork2_extend")
+        call("vls_batch_enork2_extend", self._h, self._f(F), self._f(C), int(iterations), _stream())
+
     def advect(self, F, vx, vy, vz, dt, steps=1):
         call("vls_batch_advect_rk3" + self._sfx, self._h, self._f(F), self._f(vx), self._f(vy), self._f(vz),
              self._d(dt, self.ngrids), int(steps), _stream())
@@ -378,8 +400,9 @@ def batch_comparison(cfg, steps=10, sample_every=16):
     """Config 4: the batched workload in FP32 bulk (autotuned launch shape) and
     in FP64 (same batching), and the plain FP64 path (one object per vesicle)
     on every `sample_every`-th vesicle; rates in WENO updates/s, and the FP32
-    run's per-vesicle area/volume drift and interface displacement against the
-    FP64 run, both from the same initial state (the FP64 run's, rounded)."""
+    run's area/volume drift and interface displacement against the FP64 run
+    on every vesicle, both from the same initial state (the FP64 run's,
+    rounded)."""
     import dataclasses
     import time
 
@@ -415,7 +438,7 @@ def batch_comparison(cfg, steps=10, sample_every=16):
     av64, av32 = runs["fp64"]["av"], runs["fp32"]["av"]
     rel = ((av32 - av64).abs() / av64.abs()).amax(dim=0).cpu().numpy()
     disp = []
-    for v in sample[:8]:
+    for v in range(cfg.vesicles):
         G = _grid_of(cfg, b64.shapes[v][0])
         d = diagnostics._displacement_stats(G, b64.grid_view(runs["fp32"]["F"], v).double(),
                                             b64.grid_view(runs["fp64"]["F"], v))

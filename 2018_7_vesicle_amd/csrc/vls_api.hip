@@ -1446,7 +1446,7 @@ template <typename T>
 int batch_set_distance(vls_batch* b, const T* F, const T* mask_src, hipStream_t s)
 {
     VLS_HIP(launch_zero_ints(b->counts, 1, s));
-    VLS_HIP(launch_batch_build_list<T>(b->view(), F, mask_src, b->flags, b->lv, s));
+    VLS_HIP(launch_batch_build_list<T>(b->view(), F, mask_src, b->flags, b->lv, 6, s));
     return VLS_OK;
 }
 
@@ -1516,6 +1516,69 @@ int batch_step(vls_batch* b, T* F, T* Vn, const T* vx, const T* vy, const T* vz,
     if ((rc = batch_extend<T>(b, F, Vn, ext_iters, s))) return rc;
     if (reinit_iters > 0 && (rc = batch_reinit_loop<T>(b, F, reinit_iters, s))) return rc;
     VLS_HIP(launch_batch_area_volume_cosine<T>(b->view(), av, b->partial, F, eps_factor, s));
+    return VLS_OK;
+}
+
+// WENORK3Reinitialization of every grid (+SD/@SDF3/WENORK3Reinitialization.m:3-61):
+// cubic distances of F and mask F < 0 (:17-26), then Shu-Osher RK3 (:30-57)
+// on the fused stage launches with the cubic ENO at the listed and deferred
+// points, as vls_sdf3_wenork3_reinit does for one grid.  Leaves the list
+// holding the cubic distances.
+int batch_wenork3_reinit(vls_batch* b, double* F, int iterations, hipStream_t s)
+{
+    const BatchView B = b->view();
+    int* oc = b->counts + 4;
+    double* t0 = btmp<double>(b, 0);
+    double* t1 = btmp<double>(b, 1);
+    VLS_HIP(launch_zero_ints(b->counts, 1, s));
+    VLS_HIP(launch_batch_build_list<double>(B, F, F, b->flags, b->lv, 3, s));
+    VLS_HIP(launch_zero_ints(oc, 2, s));
+    int seq = 0;
+    for (int it = 0; it < iterations; ++it) {
+        VLS_HIP(launch_batch_reinit_stage<double>(B, t0, F, nullptr, b->flags, b->lv, 0, b->over, oc, seq++, s,
+                                                  IFACE_CUBIC));
+        VLS_HIP(launch_batch_reinit_stage<double>(B, t1, t0, F, b->flags, b->lv, 1, b->over, oc, seq++, s,
+                                                  IFACE_CUBIC));
+        VLS_HIP(launch_batch_reinit_stage<double>(B, F, t1, F, b->flags, b->lv, 2, b->over, oc, seq++, s,
+                                                  IFACE_CUBIC));
+    }
+    return VLS_OK;
+}
+
+// WENORK3Extend (weno = 1, +SD/@SDF3/WENORK3Extend.m:9-98) and ENORK2Extend
+// (weno = 0, ENORK2Extend.m:9-81) of C off F on every grid, as
+// variant_extend does for one grid: cubic or quadratic-ENO distances of F,
+// the variant's upwind normal made the extension velocity, the interface
+// values of C once, then Shu-Osher RK3 on the fused stage launches (cubic
+// ENO at the listed points) or Heun RK2 stages.  Leaves the list holding the
+// variant's distances.
+int batch_variant_extend(vls_batch* b, const double* F, double* C, int iterations, int weno, hipStream_t s)
+{
+    const BatchView B = b->view();
+    double *vx = bvel<double>(b, 0), *vy = bvel<double>(b, 1), *vz = bvel<double>(b, 2);
+    double* t0 = btmp<double>(b, 0);
+    double* t1 = btmp<double>(b, 1);
+    const int scheme = weno ? 1 : 2;  // the cubic ENO, the quadratic ENO
+    VLS_HIP(launch_zero_ints(b->counts, 1, s));
+    VLS_HIP(launch_batch_build_list<double>(B, F, nullptr, b->flags, b->lv, weno ? 3 : 2, s));
+    VLS_HIP(launch_batch_extension_velocity<double>(B, vx, vy, vz, F, b->flags, b->lv, 1e-6, s, scheme));
+    VLS_HIP(launch_batch_list_interp<double>(B, b->lcp, b->lv, C, s, scheme));
+    if (weno) {
+        VLS_HIP(launch_batch_list_gather_vel<double>(b->lvel, b->lv, vx, vy, vz, s));
+        for (int it = 0; it < iterations; ++it) {
+            VLS_HIP(launch_batch_extend_stage<double>(B, t0, C, nullptr, vx, vy, vz, b->flags, b->lv, b->lcp,
+                                                      b->lvel, 0, s, IFACE_CUBIC));
+            VLS_HIP(launch_batch_extend_stage<double>(B, t1, t0, C, vx, vy, vz, b->flags, b->lv, b->lcp, b->lvel, 1,
+                                                      s, IFACE_CUBIC));
+            VLS_HIP(launch_batch_extend_stage<double>(B, C, t1, C, vx, vy, vz, b->flags, b->lv, b->lcp, b->lvel, 2,
+                                                      s, IFACE_CUBIC));
+        }
+        return VLS_OK;
+    }
+    for (int it = 0; it < iterations; ++it) {
+        VLS_HIP(launch_batch_eno2_extend_stage(B, t0, C, nullptr, vx, vy, vz, b->flags, b->lv, b->lcp, 0, s));
+        VLS_HIP(launch_batch_eno2_extend_stage(B, C, t0, C, vx, vy, vz, b->flags, b->lv, b->lcp, 1, s));
+    }
     return VLS_OK;
 }
 
@@ -1681,6 +1744,38 @@ int vls_batch_get_zmarch_depth(vls_batch_t b, int* depth)
     VLS_REQUIRE(b && depth, "null argument");
     *depth = b->zd;
     return VLS_OK;
+}
+
+int vls_batch_wenork3_reinit(vls_batch_t b, double* F, int iterations, vls_stream_t stream)
+{
+    VLS_REQUIRE(b && F, "null argument");
+    VLS_REQUIRE(iterations >= 0, "negative iteration count");
+    const int rc = with_graph(b->graphs, make_key(14, F, iterations), S(stream),
+                              [&](hipStream_t cs) -> int { return batch_wenork3_reinit(b, F, iterations, cs); });
+    b->list_ready = false;  // the list holds the cubic distances now
+    return rc;
+}
+
+int vls_batch_wenork3_extend(vls_batch_t b, const double* F, double* C, int iterations, vls_stream_t stream)
+{
+    VLS_REQUIRE(b && F && C, "null argument");
+    VLS_REQUIRE(iterations >= 0, "negative iteration count");
+    VLS_REQUIRE((const void*)F != (const void*)C, "C must not alias F");
+    const int rc = with_graph(b->graphs, make_key(15, F, C, iterations), S(stream),
+                              [&](hipStream_t cs) -> int { return batch_variant_extend(b, F, C, iterations, 1, cs); });
+    b->list_ready = false;
+    return rc;
+}
+
+int vls_batch_enork2_extend(vls_batch_t b, const double* F, double* C, int iterations, vls_stream_t stream)
+{
+    VLS_REQUIRE(b && F && C, "null argument");
+    VLS_REQUIRE(iterations >= 0, "negative iteration count");
+    VLS_REQUIRE((const void*)F != (const void*)C, "C must not alias F");
+    const int rc = with_graph(b->graphs, make_key(16, F, C, iterations), S(stream),
+                              [&](hipStream_t cs) -> int { return batch_variant_extend(b, F, C, iterations, 0, cs); });
+    b->list_ready = false;
+    return rc;
 }
 
 #define VLS_BATCH_ENTRIES(T, SFX)                                                                                   \

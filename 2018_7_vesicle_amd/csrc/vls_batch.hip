@@ -12,6 +12,8 @@
 // the grid offsets.  Every per-point body is the single-grid kernels' own
 // (vls_tile.hpp), called with the pointers of the point's grid, so each grid
 // of a batch gets the single-grid result bit for bit.
+#include <type_traits>
+
 #include "vls_tile.hpp"
 
 namespace vls {
@@ -63,7 +65,10 @@ __device__ __forceinline__ T* shift(T* p, int64_t off)
 // stencil around i-1, which is what boundary_location writes into the l-side
 // array of i; no full distance arrays are stored.
 // ---------------------------------------------------------------------------
-template <int R, typename T>
+// ORDER: the locator, 6 (boundary_location), 2 (the quadratic-ENO
+// boundary_correction of the ENO-RK2 variants) or 3 (the cubic one of the
+// WENO-RK3 variants).
+template <int R, typename T, int ORDER = 6>
 __global__ void __launch_bounds__(256) kb_build_list(const T* __restrict__ F, const T* __restrict__ mask_src,
                                                      uint8_t* __restrict__ flags, ListView il, BatchView B)
 {
@@ -94,12 +99,20 @@ __global__ void __launch_bounds__(256) kb_build_list(const T* __restrict__ F, co
                 if (cr || cl) {
                     double v[7];
                     line7(f, t.r, t.c, t.p, a, b, v);
-                    if (cr) q.r[a] = six_distance(v[1], v[2], v[3], v[4], v[5], v[6], sp[a]);
-                    if (cl) q.l[a] = sp[a] - six_distance(v[0], v[1], v[2], v[3], v[4], v[5], sp[a]);
+                    if (cr)
+                        q.r[a] = (ORDER == 2)   ? eno_quadratic_distance(v[2], v[3], v[4], v[5], sp[a])
+                                 : (ORDER == 3) ? cubic_distance(v[2], v[3], v[4], v[5], sp[a])
+                                                : six_distance(v[1], v[2], v[3], v[4], v[5], v[6], sp[a]);
+                    if (cl)
+                        q.l[a] = sp[a] - ((ORDER == 2)   ? eno_quadratic_distance(v[1], v[2], v[3], v[4], sp[a])
+                                          : (ORDER == 3) ? cubic_distance(v[1], v[2], v[3], v[4], sp[a])
+                                                         : six_distance(v[0], v[1], v[2], v[3], v[4], v[5], sp[a]));
                 }
             }
-            sub = q.r[0] < b.dx || q.l[0] < b.dx || q.r[1] < b.dy || q.l[1] < b.dy || q.r[2] < b.dz ||
-                  q.l[2] < b.dz;
+            // a distance that differs from the spacing (the quadratic ENO
+            // locator is not clamped and can land just above it)
+            sub = q.r[0] != b.dx || q.l[0] != b.dx || q.r[1] != b.dy || q.l[1] != b.dy || q.r[2] != b.dz ||
+                  q.l[2] != b.dz;
             uint8_t fl = sub ? FLAG_SUBCELL : 0;
             if (mask_src && mask_src[G.off + ind] < 0) fl |= FLAG_MASK;
             flags[G.off + ind] = fl;
@@ -136,7 +149,7 @@ __global__ void kb_set_mask_bit(uint8_t* __restrict__ flags, const T* __restrict
 // others take one z-march work item each (a tile column of one grid over a
 // range of its pages) with the single-grid kernels' walk and per-point
 // bodies, called with the item's grid.
-template <int R, typename T>
+template <int R, typename T, int S = IFACE_WENO>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(stage_waves<T>()))) kb_reinit_stage(T* __restrict__ out, const T* __restrict__ fs,
                                                        const T* __restrict__ fn, const uint8_t* __restrict__ flags,
                                                        IList il, BatchView B, int stage, int* __restrict__ over,
@@ -154,8 +167,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(stage_
             if (active) {
                 const int64_t gind = il.idx[al.point];
                 const BGrid G = B.grids[B.list_grid[al.point]];
-                q = reinit_iface_lane(fs + G.off, flags + G.off, box_of(G), inv_of(G), il, al.point, gind - G.off,
-                                      gind, al.axis);
+                q = reinit_iface_lane<T, S>(fs + G.off, flags + G.off, box_of(G), inv_of(G), il, al.point,
+                                            gind - G.off, gind, al.axis);
             }
             reinit_iface_finish(out, fs, fn, q, al, active, stage);
         }
@@ -180,7 +193,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(stage_
 }
 
 // launch 2 of 2: the deferred bulk points (see k_reinit_deferred)
-template <typename T>
+template <typename T, int S = IFACE_WENO>
 __global__ void __launch_bounds__(256) kb_reinit_deferred(T* __restrict__ out, const T* __restrict__ fs,
                                                           const T* __restrict__ fn, const uint8_t* __restrict__ flags,
                                                           IList il, BatchView B, int stage,
@@ -195,8 +208,8 @@ __global__ void __launch_bounds__(256) kb_reinit_deferred(T* __restrict__ out, c
         if (active) {
             const int64_t gind = over[al.point];
             const BGrid G = B.grids[grid_of(B, gind)];
-            q = reinit_iface_lane(fs + G.off, flags + G.off, box_of(G), inv_of(G), il, -1, gind - G.off, gind,
-                                  al.axis);
+            q = reinit_iface_lane<T, S>(fs + G.off, flags + G.off, box_of(G), inv_of(G), il, -1, gind - G.off,
+                                        gind, al.axis);
         }
         reinit_iface_finish(out, fs, fn, q, al, active, stage);
     }
@@ -205,7 +218,7 @@ __global__ void __launch_bounds__(256) kb_reinit_deferred(T* __restrict__ out, c
 // fused extension stage (one launch): the first niface blocks walk the
 // interface list (three lanes per point), the others take the z-march work
 // items; see k_extend_stage
-template <int R, typename T>
+template <int R, typename T, int S = IFACE_WENO>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(stage_waves<T>()))) kb_extend_stage(T* __restrict__ out, const T* __restrict__ cs,
                                                        const T* __restrict__ cn, const T* __restrict__ vx,
                                                        const T* __restrict__ vy, const T* __restrict__ vz,
@@ -224,8 +237,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(stage_
             if (active) {
                 const int64_t gind = il.idx[al.point];
                 const BGrid G = B.grids[B.list_grid[al.point]];
-                q = extend_iface_lane(cs + G.off, box_of(G), inv_of(G), il, lcp, lv, al.point, gind - G.off, gind,
-                                      al.axis);
+                q = extend_iface_lane<T, S>(cs + G.off, box_of(G), inv_of(G), il, lcp, lv, al.point, gind - G.off,
+                                            gind, al.axis);
             }
             extend_iface_finish(out, cs, cn, q, al, active, stage);
         }
@@ -311,7 +324,40 @@ __global__ void __launch_bounds__(256) kb_ext_vel_bulk(T* __restrict__ vx, T* __
     }
 }
 
-template <typename T>
+// S: the interface scheme of the listed points' upwind normal: the
+// nonuniform WENO (upwind_normal.cu), the cubic ENO where the axis has a
+// subcell distance (wenork3_extend.cu:202-292, interface values 0), or ENO2
+// (enork2_extend.cu:102-138: the ENO2 derivative on the side whose node
+// value, 0 at a crossing, is smaller)
+// the ENO-RK2 upwind normal at the points without a subcell distance
+template <int R>
+__global__ void __launch_bounds__(256) kb_eno2_ext_vel_bulk(double* __restrict__ vx, double* __restrict__ vy,
+                                                            double* __restrict__ vz, const double* __restrict__ F,
+                                                            const uint8_t* __restrict__ flags, double floor_,
+                                                            BatchView B)
+{
+    const TileRange tr = tile_range(B.ntiles);
+    for (int tile = tr.first; tile < tr.end; tile += tr.step) {
+        const BGrid G = B.grids[B.tile_grid[tile]];
+        const Box b = box_of(G);
+        const Tile t = tile_at<R>(b, tile - G.tile0);
+        if (!t.valid) continue;
+        const idx_t ind = lin(b, t.r, t.c, t.p);
+        if (flags[G.off + ind] & FLAG_SUBCELL) continue;
+        const double sp[3] = {b.dx, b.dy, b.dz};
+        double n[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            double v[7], sR, sL;
+            line7(F + G.off, t.r, t.c, t.p, a, b, v);
+            eno2_pair(sR, sL, v, sp[a], sp[a], sp[a]);
+            n[a] = (fabs(v[4]) < fabs(v[2])) ? sR : sL;
+        }
+        store_extension_velocity(vx, vy, vz, F, (idx_t)(G.off + ind), n, floor_);
+    }
+}
+
+template <typename T, int S = IFACE_WENO>
 __global__ void __launch_bounds__(256) kb_ext_vel_list(T* __restrict__ vx, T* __restrict__ vy, T* __restrict__ vz,
                                                        const T* __restrict__ F, IList il, double floor_, BatchView B)
 {
@@ -328,16 +374,27 @@ __global__ void __launch_bounds__(256) kb_ext_vel_list(T* __restrict__ vx, T* __
         for (int a = 0; a < 3; ++a) {
             double v[7], fo, bk;
             line7(F + G.off, r, c, p, a, b, v);
-            extend_axis(fo, bk, v, il.dist[(size_t)(2 * a) * il.cap + j], il.dist[(size_t)(2 * a + 1) * il.cap + j],
-                        sp[a], jpd(sp[a]), 0.0, 0.0);
-            n[a] = (fabs(v[4]) < fabs(v[2])) ? fo : bk;
+            const double r1 = il.dist[(size_t)(2 * a) * il.cap + j], l1 = il.dist[(size_t)(2 * a + 1) * il.cap + j];
+            if constexpr (S == IFACE_ENO2) {
+                eno2_pair(fo, bk, v, r1, l1, sp[a]);
+                const double vr = (r1 == sp[a]) ? v[4] : 0.0, vl = (l1 == sp[a]) ? v[2] : 0.0;
+                n[a] = (fabs(vr) < fabs(vl)) ? fo : bk;
+            } else {
+                if constexpr (S == IFACE_CUBIC)
+                    wenork3_pair(fo, bk, v, r1, l1, sp[a], jpd(sp[a]), r1 < sp[a] || l1 < sp[a], 0.0, 0.0);
+                else
+                    extend_axis(fo, bk, v, r1, l1, sp[a], jpd(sp[a]), 0.0, 0.0);
+                n[a] = (fabs(v[4]) < fabs(v[2])) ? fo : bk;
+            }
         }
         store_extension_velocity(vx, vy, vz, F, (idx_t)gind, n, floor_);
     }
 }
 
-// interface values (boundary_interpolant.cu:102-141) at the listed points
-template <typename T>
+// interface values at the listed points: the quintic of
+// boundary_interpolant.cu:102-141, the cubic of wenork3_extend.cu:310-329 or
+// the quadratic ENO of enork2_extend.cu:144-170 (S)
+template <typename T, int S = IFACE_WENO>
 __global__ void __launch_bounds__(256) kb_list_interp(double* __restrict__ cp, IList il, const T* __restrict__ C,
                                                       BatchView B)
 {
@@ -352,15 +409,84 @@ __global__ void __launch_bounds__(256) kb_list_interp(double* __restrict__ cp, I
         const double sp[3] = {b.dx, b.dy, b.dz};
         const double rf = il.dist[(size_t)(2 * a) * il.cap + j], lb = il.dist[(size_t)(2 * a + 1) * il.cap + j];
         Pair q{0.0, 0.0};
-        if (rf < sp[a] || lb < sp[a]) {
+        if constexpr (S == IFACE_ENO2) {
             int r, c, p;
             coords_of(b, gind - G.off, r, c, p);
             double v[7];
             line7(C + G.off, r, c, p, a, b, v);
-            q = sixth_interp(rf, lb, sp[a], v[0], v[1], v[2], v[3], v[4], v[5], v[6]);
+            eno2_interface_values(q.fore, q.back, rf, lb, sp[a], v);
+        } else if (rf < sp[a] || lb < sp[a]) {
+            int r, c, p;
+            coords_of(b, gind - G.off, r, c, p);
+            double v[7];
+            line7(C + G.off, r, c, p, a, b, v);
+            if constexpr (S == IFACE_CUBIC) cubic_interface_values(q.fore, q.back, rf, lb, sp[a], v);
+            else q = sixth_interp(rf, lb, sp[a], v[0], v[1], v[2], v[3], v[4], v[5], v[6]);
         }
         cp[(size_t)(2 * a) * il.cap + j] = q.fore;
         cp[(size_t)(2 * a + 1) * il.cap + j] = q.back;
+    }
+}
+
+// ENO-RK2 extension stage (ENORK2Extend.m:68-79, Heun; heun2 == 0:
+// out = cs - L(cs), heun2 == 1: out = (cn + cs - L(cs)) / 2), as the
+// single-grid k_enork2_extend_stage: the points without a subcell distance in
+// a tile pass (spacings for distances, pseudo-time step 0.3 min of them), the
+// listed points in a list pass (distances, and interface values in list
+// order)
+template <int R>
+__global__ void __launch_bounds__(256) kb_eno2_extend_bulk(double* __restrict__ out, const double* __restrict__ cs,
+                                                           const double* __restrict__ cn,
+                                                           const double* __restrict__ vx,
+                                                           const double* __restrict__ vy,
+                                                           const double* __restrict__ vz,
+                                                           const uint8_t* __restrict__ flags, BatchView B, int heun2)
+{
+#pragma clang fp contract(off)
+    const TileRange tr = tile_range(B.ntiles);
+    for (int tile = tr.first; tile < tr.end; tile += tr.step) {
+        const BGrid G = B.grids[B.tile_grid[tile]];
+        const Box b = box_of(G);
+        const Tile t = tile_at<R>(b, tile - G.tile0);
+        if (!t.valid) continue;
+        const idx_t i = G.off + lin(b, t.r, t.c, t.p);
+        if (flags[i] & FLAG_SUBCELL) continue;
+        Dists q;
+        q.r[0] = q.l[0] = b.dx;
+        q.r[1] = q.l[1] = b.dy;
+        q.r[2] = q.l[2] = b.dz;
+        const double h0[3] = {0.0, 0.0, 0.0};
+        const double step = enork2_residual(cs + G.off, t.r, t.c, t.p, b, q, h0, h0, vx[i], vy[i], vz[i], 0.3 * min6(q));
+        out[i] = heun2 ? (cn[i] + cs[i] - step) / 2 : cs[i] - step;
+    }
+}
+
+__global__ void __launch_bounds__(256) kb_eno2_extend_list(double* __restrict__ out, const double* __restrict__ cs,
+                                                           const double* __restrict__ cn,
+                                                           const double* __restrict__ vx,
+                                                           const double* __restrict__ vy,
+                                                           const double* __restrict__ vz, IList il,
+                                                           const double* __restrict__ lcp, BatchView B, int heun2)
+{
+#pragma clang fp contract(off)
+    const int cnt = *il.count;
+    for (int j = (int)(blockIdx.x * 256 + threadIdx.x); j < cnt; j += (int)(gridDim.x * 256)) {
+        const int64_t i = il.idx[j];
+        const BGrid G = B.grids[B.list_grid[j]];
+        const Box b = box_of(G);
+        int r, c, p;
+        coords_of(b, i - G.off, r, c, p);
+        Dists q;
+        double hf[3], hb[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            q.r[a] = il.dist[(size_t)(2 * a) * il.cap + j];
+            q.l[a] = il.dist[(size_t)(2 * a + 1) * il.cap + j];
+            hf[a] = lcp[(size_t)(2 * a) * il.cap + j];
+            hb[a] = lcp[(size_t)(2 * a + 1) * il.cap + j];
+        }
+        const double step = enork2_residual(cs + G.off, r, c, p, b, q, hf, hb, vx[i], vy[i], vz[i], 0.3 * min6(q));
+        out[i] = heun2 ? (cn[i] + cs[i] - step) / 2 : cs[i] - step;
     }
 }
 
@@ -604,9 +730,17 @@ static inline IList ilist(const ListView& lv) { return IList{lv.idx, lv.count, l
 
 template <typename T>
 hipError_t launch_batch_build_list(const BatchView& B, const T* F, const T* mask_src, uint8_t* flags,
-                                   const ListView& lv, hipStream_t s)
+                                   const ListView& lv, int order, hipStream_t s)
 {
-    TR_LAUNCH(kb_build_list, T, B.blocks, F, mask_src, flags, lv, B);
+    if (order == 2) {
+        if (B.tr == 32) kb_build_list<32, T, 2><<<B.blocks, 256, 0, s>>>(F, mask_src, flags, lv, B);
+        else kb_build_list<64, T, 2><<<B.blocks, 256, 0, s>>>(F, mask_src, flags, lv, B);
+    } else if (order == 3) {
+        if (B.tr == 32) kb_build_list<32, T, 3><<<B.blocks, 256, 0, s>>>(F, mask_src, flags, lv, B);
+        else kb_build_list<64, T, 3><<<B.blocks, 256, 0, s>>>(F, mask_src, flags, lv, B);
+    } else {
+        TR_LAUNCH(kb_build_list, T, B.blocks, F, mask_src, flags, lv, B);
+    }
     return hipGetLastError();
 }
 
@@ -620,11 +754,25 @@ hipError_t launch_batch_set_mask_bit(uint8_t* flags, const T* F, int64_t n, hipS
 template <typename T>
 hipError_t launch_batch_reinit_stage(const BatchView& B, T* out, const T* fs, const T* fn, const uint8_t* flags,
                                      const ListView& lv, int stage, int* over, int* over_count, int seq,
-                                     hipStream_t s)
+                                     hipStream_t s, int scheme)
 {
     const int ni = fused_list_blocks(B);
-    TR_LAUNCH(kb_reinit_stage, T, B.nitems + ni, out, fs, fn, flags, ilist(lv), B, stage, over, over_count, seq, ni);
     const int nd = ni / 10 < 8 ? 8 : ni / 10;
+    if constexpr (std::is_same<T, double>::value) {
+        if (scheme == IFACE_CUBIC) {
+            if (B.tr == 32)
+                kb_reinit_stage<32, T, IFACE_CUBIC><<<B.nitems + ni, 256, 0, s>>>(out, fs, fn, flags, ilist(lv), B,
+                                                                                  stage, over, over_count, seq, ni);
+            else
+                kb_reinit_stage<64, T, IFACE_CUBIC><<<B.nitems + ni, 256, 0, s>>>(out, fs, fn, flags, ilist(lv), B,
+                                                                                  stage, over, over_count, seq, ni);
+            kb_reinit_deferred<T, IFACE_CUBIC><<<nd, 256, 0, s>>>(out, fs, fn, flags, ilist(lv), B, stage,
+                                                                    (const int*)over, (const int*)over_count, seq);
+            return hipGetLastError();
+        }
+    }
+    if (scheme != IFACE_WENO) return hipErrorInvalidValue;
+    TR_LAUNCH(kb_reinit_stage, T, B.nitems + ni, out, fs, fn, flags, ilist(lv), B, stage, over, over_count, seq, ni);
     kb_reinit_deferred<T><<<nd, 256, 0, s>>>(out, fs, fn, flags, ilist(lv), B, stage, (const int*)over,
                                                (const int*)over_count, seq);
     return hipGetLastError();
@@ -633,10 +781,32 @@ hipError_t launch_batch_reinit_stage(const BatchView& B, T* out, const T* fs, co
 template <typename T>
 hipError_t launch_batch_extend_stage(const BatchView& B, T* out, const T* cs, const T* cn, const T* vx,
                                      const T* vy, const T* vz, const uint8_t* flags, const ListView& lv,
-                                     const double* lcp, const double* lvel, int stage, hipStream_t s)
+                                     const double* lcp, const double* lvel, int stage, hipStream_t s, int scheme)
 {
     const int ni = fused_list_blocks(B);
+    if constexpr (std::is_same<T, double>::value) {
+        if (scheme == IFACE_CUBIC) {
+            if (B.tr == 32)
+                kb_extend_stage<32, T, IFACE_CUBIC><<<B.nitems + ni, 256, 0, s>>>(out, cs, cn, vx, vy, vz, flags,
+                                                                                  ilist(lv), lcp, lvel, B, stage, ni);
+            else
+                kb_extend_stage<64, T, IFACE_CUBIC><<<B.nitems + ni, 256, 0, s>>>(out, cs, cn, vx, vy, vz, flags,
+                                                                                  ilist(lv), lcp, lvel, B, stage, ni);
+            return hipGetLastError();
+        }
+    }
+    if (scheme != IFACE_WENO) return hipErrorInvalidValue;
     TR_LAUNCH(kb_extend_stage, T, B.nitems + ni, out, cs, cn, vx, vy, vz, flags, ilist(lv), lcp, lvel, B, stage, ni);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_eno2_extend_stage(const BatchView& B, double* out, const double* cs, const double* cn,
+                                          const double* vx, const double* vy, const double* vz, const uint8_t* flags,
+                                          const ListView& lv, const double* lcp, int heun2, hipStream_t s)
+{
+    if (B.tr == 32) kb_eno2_extend_bulk<32><<<B.blocks, 256, 0, s>>>(out, cs, cn, vx, vy, vz, flags, B, heun2);
+    else kb_eno2_extend_bulk<64><<<B.blocks, 256, 0, s>>>(out, cs, cn, vx, vy, vz, flags, B, heun2);
+    kb_eno2_extend_list<<<list_blocks(B), 256, 0, s>>>(out, cs, cn, vx, vy, vz, ilist(lv), lcp, B, heun2);
     return hipGetLastError();
 }
 
@@ -650,16 +820,43 @@ hipError_t launch_batch_advect_stage(const BatchView& B, T* out, const T* fs, co
 
 template <typename T>
 hipError_t launch_batch_extension_velocity(const BatchView& B, T* vx, T* vy, T* vz, const T* F,
-                                           const uint8_t* flags, const ListView& lv, double floor_, hipStream_t s)
+                                           const uint8_t* flags, const ListView& lv, double floor_, hipStream_t s,
+                                           int scheme)
 {
+    if constexpr (std::is_same<T, double>::value) {
+        if (scheme == IFACE_ENO2) {
+            if (B.tr == 32) kb_eno2_ext_vel_bulk<32><<<B.blocks, 256, 0, s>>>(vx, vy, vz, F, flags, floor_, B);
+            else kb_eno2_ext_vel_bulk<64><<<B.blocks, 256, 0, s>>>(vx, vy, vz, F, flags, floor_, B);
+            kb_ext_vel_list<T, IFACE_ENO2><<<list_blocks(B), 256, 0, s>>>(vx, vy, vz, F, ilist(lv), floor_, B);
+            return hipGetLastError();
+        }
+        if (scheme == IFACE_CUBIC) {
+            TR_LAUNCH(kb_ext_vel_bulk, T, B.blocks, vx, vy, vz, F, flags, floor_, B);
+            kb_ext_vel_list<T, IFACE_CUBIC><<<list_blocks(B), 256, 0, s>>>(vx, vy, vz, F, ilist(lv), floor_, B);
+            return hipGetLastError();
+        }
+    }
+    if (scheme != IFACE_WENO) return hipErrorInvalidValue;
     TR_LAUNCH(kb_ext_vel_bulk, T, B.blocks, vx, vy, vz, F, flags, floor_, B);
     kb_ext_vel_list<T><<<list_blocks(B), 256, 0, s>>>(vx, vy, vz, F, ilist(lv), floor_, B);
     return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_batch_list_interp(const BatchView& B, double* lcp, const ListView& lv, const T* C, hipStream_t s)
+hipError_t launch_batch_list_interp(const BatchView& B, double* lcp, const ListView& lv, const T* C, hipStream_t s,
+                                    int scheme)
 {
+    if constexpr (std::is_same<T, double>::value) {
+        if (scheme == IFACE_CUBIC) {
+            kb_list_interp<T, IFACE_CUBIC><<<list_blocks(B), 256, 0, s>>>(lcp, ilist(lv), C, B);
+            return hipGetLastError();
+        }
+        if (scheme == IFACE_ENO2) {
+            kb_list_interp<T, IFACE_ENO2><<<list_blocks(B), 256, 0, s>>>(lcp, ilist(lv), C, B);
+            return hipGetLastError();
+        }
+    }
+    if (scheme != IFACE_WENO) return hipErrorInvalidValue;
     kb_list_interp<T><<<list_blocks(B), 256, 0, s>>>(lcp, ilist(lv), C, B);
     return hipGetLastError();
 }
@@ -715,19 +912,19 @@ hipError_t launch_batch_ellipsoid(const BatchView& B, T* F, const double* axes, 
 
 #define VLS_BATCH_INSTANTIATE(T)                                                                                    \
     template hipError_t launch_batch_build_list<T>(const BatchView&, const T*, const T*, uint8_t*, const ListView&, \
-                                                   hipStream_t);                                                    \
+                                                   int, hipStream_t);                                               \
     template hipError_t launch_batch_set_mask_bit<T>(uint8_t*, const T*, int64_t, hipStream_t);                     \
     template hipError_t launch_batch_reinit_stage<T>(const BatchView&, T*, const T*, const T*, const uint8_t*,      \
-                                                     const ListView&, int, int*, int*, int, hipStream_t);           \
+                                                     const ListView&, int, int*, int*, int, hipStream_t, int);      \
     template hipError_t launch_batch_extend_stage<T>(const BatchView&, T*, const T*, const T*, const T*, const T*,  \
                                                      const T*, const uint8_t*, const ListView&, const double*,      \
-                                                     const double*, int, hipStream_t);                              \
+                                                     const double*, int, hipStream_t, int);                         \
     template hipError_t launch_batch_advect_stage<T>(const BatchView&, T*, const T*, const T*, const T*, const T*,  \
                                                      const T*, const double*, int, hipStream_t);                    \
     template hipError_t launch_batch_extension_velocity<T>(const BatchView&, T*, T*, T*, const T*, const uint8_t*,  \
-                                                           const ListView&, double, hipStream_t);                   \
+                                                           const ListView&, double, hipStream_t, int);              \
     template hipError_t launch_batch_list_interp<T>(const BatchView&, double*, const ListView&, const T*,           \
-                                                    hipStream_t);                                                   \
+                                                    hipStream_t, int);                                              \
     template hipError_t launch_batch_list_gather_vel<T>(double*, const ListView&, const T*, const T*, const T*,     \
                                                         hipStream_t);                                               \
     template hipError_t launch_batch_normal_velocity<T>(const BatchView&, T*, const T*, const T*, const T*,         \

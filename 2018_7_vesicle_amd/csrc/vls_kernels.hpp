@@ -54,6 +54,11 @@ struct ListView {
 };
 
 
+// interface schemes of the fused stage kernels and the list passes: the
+// nonuniform WENO of the WENO5 path, the cubic ENO of the WENO-RK3 variants,
+// the quadratic ENO of the ENO-RK2 extension (list passes of the batch only)
+enum { IFACE_WENO = 0, IFACE_CUBIC = 1, IFACE_ENO2 = 2 };
+
 struct Geom {
     int rows, cols, pges;
     double dx, dy, dz;
@@ -277,32 +282,47 @@ __host__ __device__ inline int grid_tiles(int rows, int cols, int pges, int tr)
     return ((rows + tr - 1) / tr) * ((cols + tc - 1) / tc) * pges;
 }
 
-// list building: subcell distances (sixth-order locator), flags (mask bit
-// from mask_src < 0 when mask_src is not null) and the interface list of
-// every grid, in one pass; *lv.count must be zero on entry
+// list building: subcell distances (the locator of `order`: 6, 2 for the
+// quadratic ENO, 3 for the cubic), flags (mask bit from mask_src < 0 when
+// mask_src is not null) and the interface list of every grid, in one pass;
+// *lv.count must be zero on entry
 template <typename T>
 hipError_t launch_batch_build_list(const BatchView& B, const T* F, const T* mask_src, uint8_t* flags,
-                                   const ListView& lv, hipStream_t s);
+                                   const ListView& lv, int order, hipStream_t s);
 template <typename T>
 hipError_t launch_batch_set_mask_bit(uint8_t* flags, const T* F, int64_t n, hipStream_t s);
+// `scheme` of the listed and deferred points: 0 the nonuniform WENO of the
+// WENO5 path, 1 the cubic ENO of the WENO-RK3 variants (FP64 only)
 template <typename T>
 hipError_t launch_batch_reinit_stage(const BatchView& B, T* out, const T* fs, const T* fn, const uint8_t* flags,
                                      const ListView& lv, int stage, int* over, int* over_count, int seq,
-                                     hipStream_t s);
+                                     hipStream_t s, int scheme = 0);
 template <typename T>
 hipError_t launch_batch_extend_stage(const BatchView& B, T* out, const T* cs, const T* cn, const T* vx,
                                      const T* vy, const T* vz, const uint8_t* flags, const ListView& lv,
-                                     const double* lcp, const double* lvel, int stage, hipStream_t s);
+                                     const double* lcp, const double* lvel, int stage, hipStream_t s,
+                                     int scheme = 0);
+// one ENO-RK2 extension stage of every grid (heun2: 0 first, 1 second Heun
+// stage); lcp: the ENO2 interface values of the listed points
+hipError_t launch_batch_eno2_extend_stage(const BatchView& B, double* out, const double* cs, const double* cn,
+                                          const double* vx, const double* vy, const double* vz, const uint8_t* flags,
+                                          const ListView& lv, const double* lcp, int heun2, hipStream_t s);
 // dt: one step per grid (device)
 template <typename T>
 hipError_t launch_batch_advect_stage(const BatchView& B, T* out, const T* fs, const T* fn, const T* vx,
                                      const T* vy, const T* vz, const double* dt, int stage, hipStream_t s);
-// upwind normal of F (subcell-aware) and v = sign(F) n/max(|n|, floor)
+// upwind normal of F (subcell-aware) and v = sign(F) n/max(|n|, floor);
+// scheme 0: WENO5 (upwind_normal.cu), 1: the cubic ENO of WENORK3Extend,
+// 2: the ENO2 of ENORK2Extend (1 and 2 FP64 only)
 template <typename T>
 hipError_t launch_batch_extension_velocity(const BatchView& B, T* vx, T* vy, T* vz, const T* F,
-                                           const uint8_t* flags, const ListView& lv, double floor_, hipStream_t s);
+                                           const uint8_t* flags, const ListView& lv, double floor_, hipStream_t s,
+                                           int scheme = 0);
+// interface values of C at the listed points (scheme as above: the quintic,
+// the cubic or the quadratic ENO interpolant)
 template <typename T>
-hipError_t launch_batch_list_interp(const BatchView& B, double* lcp, const ListView& lv, const T* C, hipStream_t s);
+hipError_t launch_batch_list_interp(const BatchView& B, double* lcp, const ListView& lv, const T* C, hipStream_t s,
+                                    int scheme = 0);
 template <typename T>
 hipError_t launch_batch_list_gather_vel(double* lvel, const ListView& lv, const T* vx, const T* vy, const T* vz,
                                         hipStream_t s);

@@ -127,12 +127,42 @@ __device__ __forceinline__ double min6(const Dists& q)
 
 // per-axis derivative pair with the sign-change interface test (re_step.cu:248-269);
 // INL: the nonuniform face inline (see weno_nonuniform_pair_inl)
-// Interface schemes of the fused stage kernels: the WENO5 path's nonuniform
-// WENO (re_step.cu:79-243, shared_utilities.cup:49-278) or the WENO-RK3
-// variants' cubic ENO (weno_rk3_reinitialization.cu:149-247,
-// wenork3_extend.cu:88-190).  Away from the interface both use the same
-// uniform WENO5, so the bulk of a stage is shared.
-enum { IFACE_WENO = 0, IFACE_CUBIC = 1 };
+// Interface schemes of the fused stage kernels (IFACE_WENO, IFACE_CUBIC of
+// vls_kernels.hpp): the WENO5 path's nonuniform WENO (re_step.cu:79-243,
+// shared_utilities.cup:49-278) or the WENO-RK3 variants' cubic ENO
+// (weno_rk3_reinitialization.cu:149-247, wenork3_extend.cu:88-190).  Away from
+// the interface both use the same uniform WENO5, so the bulk of a stage is
+// shared.
+
+// the upwind combination of extend_step.cu:121-123 (enork2_extend.cu:262-264,
+// wenork3_extend.cu:490-492), summed left to right, times dt
+__device__ __forceinline__ double upwind_sum(const double fo[3], const double bk[3], double vx, double vy, double vz,
+                                             double dt)
+{
+#pragma clang fp contract(off)
+    return (dmin(0.0, vx) * fo[0] + dmax(0.0, vx) * bk[0] + dmin(0.0, vy) * fo[1] + dmax(0.0, vy) * bk[1] +
+            dmin(0.0, vz) * fo[2] + dmax(0.0, vz) * bk[2]) *
+           dt;
+}
+
+// ENO-RK2 extension step at a point (enork2_extend.cu:213-266; no frozen
+// test) with the interface values of each axis given (hf, hb: fore and back;
+// each is used only where that side's distance differs from ds)
+__device__ __forceinline__ double enork2_residual(const double* __restrict__ f, int r, int c, int p, const Box& b,
+                                                  const Dists& q, const double hf[3], const double hb[3], double vx,
+                                                  double vy, double vz, double dt)
+{
+    const double sp[3] = {b.dx, b.dy, b.dz};
+    double fo[3], bk[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        double v[7];
+        line7(f, r, c, p, a, b, v);
+        eno2_field_pair(fo[a], bk[a], v, q.r[a], q.l[a], sp[a], (q.r[a] != sp[a]) ? hf[a] : 0.0,
+                        (q.l[a] != sp[a]) ? hb[a] : 0.0);
+    }
+    return upwind_sum(fo, bk, vx, vy, vz, dt);
+}
 
 // per-axis pair with the sign-change interface test in the cubic-ENO scheme
 __device__ __forceinline__ void reinit_axis_cubic(double& fore, double& back, const double v[7], double r1,

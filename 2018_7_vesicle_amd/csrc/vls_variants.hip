@@ -64,17 +64,6 @@ __device__ __forceinline__ double wenork3_reinit_residual(const double* __restri
     return godunov_step(inside, fo[0], bk[0], fo[1], bk[1], fo[2], bk[2], dt);
 }
 
-// the upwind combination of extend_step.cu:121-123 (enork2_extend.cu:262-264,
-// wenork3_extend.cu:490-492), summed left to right, times dt
-__device__ __forceinline__ double upwind_sum(const double fo[3], const double bk[3], double vx, double vy, double vz,
-                                             double dt)
-{
-#pragma clang fp contract(off)
-    return (dmin(0.0, vx) * fo[0] + dmax(0.0, vx) * bk[0] + dmin(0.0, vy) * fo[1] + dmax(0.0, vy) * bk[1] +
-            dmin(0.0, vz) * fo[2] + dmax(0.0, vz) * bk[2]) *
-           dt;
-}
-
 // WENO-RK3 extension step at a point (wenork3_extend.cu:372-495): frozen next
 // to the interface, else cubic ENO with the interface values cp where an axis
 // has a subcell distance, WENO5 elsewhere
@@ -99,21 +88,19 @@ __device__ __forceinline__ double wenork3_extend_residual(const double* __restri
 }
 
 // ENO-RK2 extension step at a point (enork2_extend.cu:213-266; no frozen test)
+// with the dense interface values cp
 __device__ __forceinline__ double enork2_extend_residual(const double* __restrict__ f, int r, int c, int p,
                                                          const Box& b, const Dists& q, const Dist6c& cp, idx_t ind,
                                                          double vx, double vy, double vz, double dt)
 {
     const double sp[3] = {b.dx, b.dy, b.dz};
-    double fo[3], bk[3];
+    double hf[3], hb[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        double v[7];
-        line7(f, r, c, p, a, b, v);
-        // an interface value is used only where its distance differs from ds
-        const double hf = (q.r[a] != sp[a]) ? cp.r[a][ind] : 0.0, hb = (q.l[a] != sp[a]) ? cp.l[a][ind] : 0.0;
-        eno2_field_pair(fo[a], bk[a], v, q.r[a], q.l[a], sp[a], hf, hb);
+        hf[a] = (q.r[a] != sp[a]) ? cp.r[a][ind] : 0.0;
+        hb[a] = (q.l[a] != sp[a]) ? cp.l[a][ind] : 0.0;
     }
-    return upwind_sum(fo, bk, vx, vy, vz, dt);
+    return enork2_residual(f, r, c, p, b, q, hf, hb, vx, vy, vz, dt);
 }
 
 }  // namespace

@@ -73,11 +73,12 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--precision-steps", type=int, default=200,
                     help="steps of the 256^3 FP32-bulk vs FP64 comparison (0 skips it)")
-    ap.add_argument("--batch-steps", type=int, default=10,
-                    help="timed steps of the 512-vesicle batch comparison (0 skips it)")
+    ap.add_argument("--batch-steps", type=int, default=100,
+                    help="timed steps of the 512-vesicle batch comparison (the workload's 100; 0 skips it)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 96^3 adjoint gradient check")
     ap.add_argument("--probe", choices=["stages", "valu-rate", "batch", "variants", "toolbox", "relax"], default=None,
-                    help="valu-rate: the VALU issue rate (FP32, FP64) of this GPU; "
+                    help="valu-rate: the VALU issue rate (FP32, FP64) of this GPU and the HBM rate of a five-read, "
+                         "one-write float stream; "
                          "stages: 3 launches each of the extension, reinitialization and advection stage "
                          "kernels at 128^3 FP64 and 256^3 FP32 (for rocprofv3 --pmc); "
                          "batch: a few steps of the 512-vesicle FP32 batch (for rocprofv3 --kernel-trace); "
@@ -203,6 +204,15 @@ def trace_durations(path):
         name = r["Kernel_Name"].split("(")[0].replace("void ", "")
         out[name].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
     return out
+
+
+def stream_rate(vls, points):
+    """HBM bytes per second of a five-read, one-write float stream (vls_probe_stream_rate)."""
+    import ctypes
+
+    v = ctypes.c_double()
+    vls._lib.call("vls_probe_stream_rate", ctypes.c_int64(points), ctypes.byref(v), None)
+    return v.value
 
 
 def pmc_summary(paths, trace=None):
@@ -442,10 +452,10 @@ def config4(steps):
     """Config 4 beside the main line: 512 vesicles on cubes of edge
     {32,48,64,80,96} (seeded), one 8-mode curl velocity each, advanced
     together; FP32 bulk and FP64 through the batched persistent kernels
-    (graph-captured steps, autotuned tiles), and the plain FP64 path (one
-    object per vesicle) on every 16th vesicle; `steps` timed steps each (the
-    rate does not depend on the step count; the workload's 100 steps are not
-    all run)."""
+    (graph-captured steps, autotuned tiles), `steps` timed steps each (by
+    default the workload's 100), and the plain FP64 path (one object per
+    vesicle) on every 16th vesicle for the same steps; the FP32 errors against
+    FP64 over all 512 vesicles."""
     import importlib
 
     bm = importlib.import_module("2018_7_vesicle_amd.batch")
@@ -505,6 +515,58 @@ def variants(with_cpu, iterations=100):
             rec["cpu_updates_per_s"] = per_it * k * n ** 3 / cpu_s
             rec["cpu_threads"] = po.num_threads()
         out[name] = rec
+    out["batched"] = variants_batched(iterations)
+    return out
+
+
+def variants_batched(iterations, vesicles=8):
+    """The same three calls on `vesicles` vesicles of 96^3 (ellipsoids with
+    seeded semi-axes in [0.35, 0.75]) at once through the batch entry points,
+    against one SDF3 object per vesicle called in turn: ms per call for all
+    vesicles, and the speedup."""
+    import numpy as np
+    import torch
+
+    vls = __import__("importlib").import_module("2018_7_vesicle_amd")
+    bm = __import__("importlib").import_module("2018_7_vesicle_amd.batch")
+    n = 96
+    xv = np.linspace(-1.0, 1.0, n)
+    Z, X, Y = np.meshgrid(xv, xv, xv, indexing="ij")
+    rng = np.random.default_rng(2018)
+    axes = rng.uniform(0.35, 0.75, size=(vesicles, 3))
+    Fs = [torch.from_numpy(np.ascontiguousarray(np.sqrt(X ** 2 / a ** 2 + Y ** 2 / b ** 2 + Z ** 2 / c ** 2) - 1.0))
+          .cuda() for a, b, c in axes]
+    C0 = torch.from_numpy(np.ascontiguousarray(0.5 * X + Y * Z + 0.2 * np.sin(3.0 * Y))).cuda()
+    G = vls.GD3(xv, xv, xv)
+    objs = [vls.SDF3(G, F) for F in Fs]
+    b = bm.Batch([(n, n, n)] * vesicles, [(G.Dx, G.Dy, G.Dz)] * vesicles, dtype=torch.float64)
+    F = b.pack(Fs)
+    C = b.pack([C0] * vesicles)
+    Fw = F.clone()
+    calls = (("WENORK3Extend", 3, lambda k: b.wenork3_extend(F, C, k), lambda m, k: m.WENORK3Extend(C0, k)),
+             ("ENORK2Extend", 2, lambda k: b.enork2_extend(F, C, k), lambda m, k: m.ENORK2Extend(C0, k)),
+             ("WENORK3Reinitialization", 3, lambda k: b.wenork3_reinitialize(Fw, k),
+              lambda m, k: m.WENORK3Reinitialization(m.F, k)))
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(3):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / 3
+
+    out = {"vesicles": vesicles, "grid": n, "iterations": iterations}
+    for name, per_it, batched, single in calls:
+        Fw.copy_(F)
+        ms_b = timed(lambda: batched(iterations))
+        ms_s = timed(lambda: [single(m, iterations) for m in objs])
+        out[name] = {"batched_ms": ms_b, "per_object_ms": ms_s, "speedup": ms_s / ms_b,
+                     "batched_updates_per_s": per_it * iterations * vesicles * n ** 3 / (ms_b * 1e-3)}
     return out
 
 
@@ -638,7 +700,9 @@ def main():
 
         vls = importlib.import_module("2018_7_vesicle_amd")
         print(json.dumps({"probe": "valu-rate", "fp32_insts_per_s": valu_rate(vls, False),
-                          "fp64_insts_per_s": valu_rate(vls, True)}), flush=True)
+                          "fp64_insts_per_s": valu_rate(vls, True),
+                          "stream5_bytes_per_s": {str(n): stream_rate(vls, n) for n in (1 << 22, 1 << 24, 1 << 26)}}),
+              flush=True)
         return 0
     if args.probe == "batch":
         return probe_batch(args)
@@ -691,6 +755,11 @@ def main():
         "avg_launch_us": stage_us,
         "bytes_per_point": B_ALG_EXTEND_FP32,
     }
+    # what a plain stream with the stage's access mix reaches on this GPU
+    # (five float arrays in, one out, 16.8 M points each: 403 MB per pass)
+    achievable = stream_rate(vls, 1 << 24) / 1e9
+    roofline["stream_GBs"] = achievable
+    roofline["frac_of_stream"] = achieved / achievable
     if counters and counters.get("valu_insts"):
         # VALU wave-instructions of one launch over the launch time, against the
         # FP32 issue rate this GPU shows (vls_probe_valu_rate)

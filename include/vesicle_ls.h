@@ -422,6 +422,13 @@ int vls_set_graphs(int enable);
  * instructions per second against it.  Allocates and frees a small buffer. */
 int vls_probe_valu_rate(int fp64, double *insts_per_s, vls_stream_t stream);
 
+/* Measurement only, no counterpart in the reference: the HBM rate (bytes per
+ * second) of a streaming kernel that reads five float arrays of `points`
+ * elements and writes one, the access mix of an FP32-bulk extension stage,
+ * timed with HIP events on `stream` (points >= 1024; choose the arrays larger
+ * than the last-level cache).  Allocates and frees its buffers. */
+int vls_probe_stream_rate(int64_t points, double *bytes_per_s, vls_stream_t stream);
+
 
 /* ------------------------------------------------------------------------ */
 /* 4. batches of grids                                                      */
@@ -503,6 +510,20 @@ int vls_batch_ellipsoid_f32(vls_batch_t b, const double *axes, const double *lo,
 int vls_batch_step_f32(vls_batch_t b, float *F, float *Vn, const float *vx, const float *vy, const float *vz,
                        const double *dt, int ext_iters, int reinit_iters, double eps_factor, double *av,
                        vls_stream_t stream);
+
+/* The WENO-RK3 / ENO-RK2 variants the relaxation driver calls every step
+ * (p00_test_singlephase_relax.m:76,84,115), on every grid of a batch at once,
+ * FP64: WENORK3Reinitialization (+SD/@SDF3/WENORK3Reinitialization.m:3-61),
+ * WENORK3Extend (WENORK3Extend.m:9-98) and ENORK2Extend (ENORK2Extend.m:9-81)
+ * of C off F.  Each computes the variant's own distances of F, as its MATLAB
+ * method does; every grid gets the single-grid result bit for bit
+ * (vls_sdf3_wenork3_reinit, vls_sdf3_wenork3_extend, vls_sdf3_enork2_extend).
+ * They leave the batch's interface list holding the variant's distances, so
+ * vls_batch_set_distance must run again before a WENO5 reinitialization or
+ * extension. */
+int vls_batch_wenork3_reinit(vls_batch_t b, double *F, int iterations, vls_stream_t stream);
+int vls_batch_wenork3_extend(vls_batch_t b, const double *F, double *C, int iterations, vls_stream_t stream);
+int vls_batch_enork2_extend(vls_batch_t b, const double *F, double *C, int iterations, vls_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
 /* 5. autotuning of the launch shape (no counterpart in the reference)      */

@@ -317,3 +317,31 @@ def test_reinit_and_extend_need_set_distance_first():
     m.setDistance()
     m.WENO5RK3Reinitialization(m.F, 1)
     torch.cuda.synchronize()
+
+
+def test_batch_variants_equal_single_grids():
+    """WENORK3Reinitialization, WENORK3Extend and ENORK2Extend on a batch of
+    uneven grids, every grid against the single-grid variant on the same data,
+    bit for bit (the single-grid variants are the oracle's results: the
+    variant tests); afterwards the WENO5 calls need set_distance again."""
+    b, grids, Gs, F, C, u = _setup(torch.float64)
+    Fr = F.clone()
+    b.wenork3_reinitialize(Fr, 7)
+    Cw = C.clone()
+    b.wenork3_extend(F, Cw, 6)
+    Ce = C.clone()
+    b.enork2_extend(F, Ce, 5)
+    torch.cuda.synchronize()
+    for gi, G in enumerate(Gs):
+        m = vls.SDF3(G, b.grid_view(F, gi).clone())
+        assert torch.equal(m.WENORK3Reinitialization(m.F, 7), b.grid_view(Fr, gi)), f"WENORK3 reinit grid {gi}"
+        c0 = b.grid_view(C, gi).clone()
+        assert torch.equal(m.WENORK3Extend(c0, 6), b.grid_view(Cw, gi)), f"WENORK3Extend grid {gi}"
+        assert torch.equal(m.ENORK2Extend(c0, 5), b.grid_view(Ce, gi)), f"ENORK2Extend grid {gi}"
+    with pytest.raises(vls.VLSError, match="set_distance"):
+        b.reinitialize(F, 1)
+    b.set_distance(F)
+    b.reinitialize(F, 1)
+    b32 = bm.Batch([(g.rows, g.cols, g.pges) for g in grids], [g.spacing for g in grids], dtype=torch.float32)
+    with pytest.raises(vls.VLSError, match="FP64"):
+        b32.wenork3_extend(b32.zeros(), b32.zeros(), 1)
