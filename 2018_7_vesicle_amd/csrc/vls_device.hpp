@@ -211,13 +211,45 @@ __device__ __forceinline__ float upwind_axis(float acc, float vel, const float p
     return fmaf(-__builtin_fabsf(kv), q, fmaf(kv, m.c, acc));
 }
 
-// back/fore pair (re_step.cu:260-267)
+// back/fore pair (re_step.cu:260-267).  The two corrections Q(D0,D1,D2,D3)
+// and Q(D4,D3,D2,D1) share the term 39/64 D2^2 + e' of their last two
+// indicators and B = D1 - 2 D2 + D3; each is jp_q's expression.
 __device__ __forceinline__ void weno5_pair(float& d_fore, float& d_back, const float p[7], JPf j)
 {
     const JPc m = jp_centre(p);
     const float d0 = fmaf(-2.f, p[1], p[0]) + p[2], d4 = fmaf(-2.f, p[5], p[4]) + p[6];
-    d_back = (m.c - jp_q(d0, m.d1, m.d2, m.d3, j.e)) * j.k;
-    d_fore = (m.c + jp_q(d4, m.d3, m.d2, m.d1, j.e)) * j.k;
+    const float d1 = m.d1, d2 = m.d2, d3 = m.d3, e = j.e;
+    const float ic = fmaf(0.609375f * d2, d2, e);
+    const float B = fmaf(-2.f, d2, d1) + d3;
+    // back: a = d0, b = d1, c = d2, d = d3
+    float qb, qf;
+    {
+        const float ib = fmaf(0.609375f * d1, d1, e);
+        const float u0 = fmaf(-1.375f, d1, d0), u1 = fmaf(-0.625f, d2, d1), u2 = fmaf(-1.375f, d2, d3);
+        const float r0 = __builtin_amdgcn_rcpf(fmaf(u0, u0, ib));
+        const float r1 = __builtin_amdgcn_rcpf(fmaf(u1, u1, ic));
+        const float r2 = __builtin_amdgcn_rcpf(fmaf(u2, u2, ic));
+        const float x0 = r0 * r0, x2 = r2 * r2;
+        const float W = fmaf(6.f * r1, r1, fmaf(3.f, x2, x0));
+        const float A = fmaf(-2.f, d1, d0) + d2;
+        const float P = fmaf(4.f * x0, A, (6.f * x2) * B);
+        qb = fmaf(P, __builtin_amdgcn_rcpf(__builtin_fmaxf(W, 1e-30f)), -B);
+    }
+    // fore: a = d4, b = d3, c = d2, d = d1 (its B is the same D3 - 2 D2 + D1)
+    {
+        const float ib = fmaf(0.609375f * d3, d3, e);
+        const float u0 = fmaf(-1.375f, d3, d4), u1 = fmaf(-0.625f, d2, d3), u2 = fmaf(-1.375f, d2, d1);
+        const float r0 = __builtin_amdgcn_rcpf(fmaf(u0, u0, ib));
+        const float r1 = __builtin_amdgcn_rcpf(fmaf(u1, u1, ic));
+        const float r2 = __builtin_amdgcn_rcpf(fmaf(u2, u2, ic));
+        const float x0 = r0 * r0, x2 = r2 * r2;
+        const float W = fmaf(6.f * r1, r1, fmaf(3.f, x2, x0));
+        const float A = fmaf(-2.f, d3, d4) + d2;
+        const float P = fmaf(4.f * x0, A, (6.f * x2) * B);
+        qf = fmaf(P, __builtin_amdgcn_rcpf(__builtin_fmaxf(W, 1e-30f)), -B);
+    }
+    d_back = (m.c - qb) * j.k;
+    d_fore = (m.c + qf) * j.k;
 }
 
 // a*b < 0 without forming the product (an FP32 product of two tiny values
