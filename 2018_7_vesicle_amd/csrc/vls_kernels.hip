@@ -112,6 +112,48 @@ __global__ void __launch_bounds__(256) k_crossings(Dist6 d, double* __restrict__
     wave_append(cross, (int)ind, cand, cand_count);
 }
 
+// The first pass on the z-marching walk of the stage kernels: the x- and
+// y-neighbours come from the staged tile, the z-neighbours from the register
+// window, so each value of F is read from memory about once.  mask: FLAG_MASK
+// where F < 0 (the only mask source the setDistance calls use is F itself).
+template <int R, typename T>
+__global__ void __launch_bounds__(256) k_crossings_z(Dist6 d, double* __restrict__ min_dist,
+                                                     uint8_t* __restrict__ flags, int mask,
+                                                     const T* __restrict__ lsf, Box b, int depth,
+                                                     int* __restrict__ cand, int* __restrict__ cand_count)
+{
+    __shared__ T lds[2][Shape<R>::N];
+    __shared__ int cbuf[1024], ccount, cbase;
+    BlockList<1024> bl{cbuf, &ccount, &cbase};
+    bl.init();
+    int r0, c0, p0, p1;
+    zmarch_share<R>(b, depth, (int)blockIdx.x, (int)gridDim.x, r0, c0, p0, p1);
+    const idx_t plane = (idx_t)b.rows * b.cols;
+    const double md = dmin(dmin(b.dx, b.dx), dmin(dmin(b.dy, b.dy), dmin(b.dz, b.dz)));
+    zmarch<R, T, false, false>(lds[0], lds[1], b, r0, c0, p0, p1, lsf, nullptr, nullptr, nullptr, nullptr, nullptr,
+                               0, [&](const T* l, const ZCol<R>& zc, const T w[7], const ZStep<R, T, false>&, int p) {
+                                   const double f0 = (double)w[3];
+                                   const double xu = (double)l[zc.base + Shape<R>::LR],
+                                                xd = (double)l[zc.base - Shape<R>::LR];
+                                   const double yu = (double)l[zc.base + 1], yd = (double)l[zc.base - 1];
+                                   const double zu = (double)w[4], zd = (double)w[2];
+                                   const bool cross = zc.valid && ((f0 * xu < 0) | (xd * f0 < 0) | (f0 * yu < 0) |
+                                                                   (yd * f0 < 0) | (f0 * zu < 0) | (zd * f0 < 0));
+                                   const idx_t ind = (idx_t)p * plane + zc.in;
+                                   if (zc.valid) {
+                                       if (flags) flags[ind] = (mask && w[3] < 0) ? (uint8_t)FLAG_MASK : (uint8_t)0;
+                                       if (d.r[0]) {
+                                           d.r[0][ind] = b.dx; d.l[0][ind] = b.dx;
+                                           d.r[1][ind] = b.dy; d.l[1][ind] = b.dy;
+                                           d.r[2][ind] = b.dz; d.l[2][ind] = b.dz;
+                                           if (min_dist) min_dist[ind] = md;
+                                       }
+                                   }
+                                   bl.push(cross, (int)ind, cand, cand_count);
+                               });
+    bl.flush(cand, cand_count);
+}
+
 // the second pass, one lane per candidate.  The r-side distance on an axis is
 // the locator at the point (crossing with the +1 neighbour); the l-side one
 // is ds minus the locator of the -1 neighbour's crossing, evaluated here
@@ -543,24 +585,39 @@ __device__ __forceinline__ void store_extension_velocity(T* __restrict__ vx, T* 
     vz[ind] = (T)(sg * (n[2] / g));
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) k_extension_normal(T* __restrict__ vx, T* __restrict__ vy,
-                                                          T* __restrict__ vz, const T* __restrict__ lsf,
-                                                          const uint8_t* __restrict__ flags, double floor_, Box b)
+// the first launch, on the z-marching walk of the stage kernels: x- and
+// y-taps from the staged tile, z-taps from the register window
+template <int R, typename T>
+__global__ void __launch_bounds__(256) k_extension_normal_z(T* __restrict__ vx, T* __restrict__ vy,
+                                                            T* __restrict__ vz, const T* __restrict__ lsf,
+                                                            const uint8_t* __restrict__ flags, double floor_, Box b,
+                                                            int depth)
 {
-    int r, c, p;
-    if (!point_of_thread(b, r, c, p)) return;
-    const idx_t ind = lin(b, r, c, p);
-    if (flags[ind] & FLAG_SUBCELL) return;  // a listed point
+    __shared__ T lds[2][Shape<R>::N];
+    int r0, c0, p0, p1;
+    zmarch_share<R>(b, depth, (int)blockIdx.x, (int)gridDim.x, r0, c0, p0, p1);
+    const idx_t plane = (idx_t)b.rows * b.cols;
     const double sp[3] = {b.dx, b.dy, b.dz};
-    double n[3];
+    zmarch<R, T, false, true>(lds[0], lds[1], b, r0, c0, p0, p1, lsf, nullptr, nullptr, nullptr, nullptr, flags, 0,
+                              [&](const T* l, const ZCol<R>& zc, const T w[7], const ZStep<R, T, false>& st, int p) {
+                                  if (st.flags(zc) & FLAG_SUBCELL) return;  // listed, or outside the box
+                                  double n[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        double v[7];
-        line7(lsf, r, c, p, a, b, v);
-        n[a] = (double)(T)upwind_normal_axis(v, sp[a], fabs(v[4]) < fabs(v[2]));
-    }
-    store_extension_velocity(vx, vy, vz, ind, n, (double)lsf[ind], floor_);
+                                  for (int a = 0; a < 3; ++a) {
+                                      T t[7];
+                                      if (a == 0) tile_line_x<R>(l, zc.base, t);
+                                      else if (a == 1) tile_line_y(l, zc.base, t);
+                                      else
+#pragma unroll
+                                          for (int k = 0; k < 7; ++k) t[k] = w[k];
+                                      double v[7];
+#pragma unroll
+                                      for (int k = 0; k < 7; ++k) v[k] = (double)t[k];
+                                      n[a] = (double)(T)upwind_normal_axis(v, sp[a], fabs(v[4]) < fabs(v[2]));
+                                  }
+                                  store_extension_velocity(vx, vy, vz, (idx_t)p * plane + zc.in, n, (double)w[3],
+                                                           floor_);
+                              });
 }
 
 template <typename T>
@@ -1184,6 +1241,8 @@ hipError_t launch_boundary_location(int order, const Dist6& d, const T* lsf, con
     return hipGetLastError();
 }
 
+static inline int bulk_blocks(const Geom& g, int depth);
+
 template <typename T>
 hipError_t launch_set_distance(int order, const Dist6& d, double* min_dist, uint8_t* flags, const T* mask_src,
                                const T* lsf, const ListView& lv, int* cand, int* cand_count, const Geom& g,
@@ -1192,7 +1251,17 @@ hipError_t launch_set_distance(int order, const Dist6& d, double* min_dist, uint
     dim3 grid, block;
     point_grid(g, grid, block);
     k_zero_pair<<<dim3(1), dim3(64), 0, s>>>(cand_count, lv.idx ? lv.count : nullptr);
-    k_crossings<T><<<grid, block, 0, s>>>(d, min_dist, flags, mask_src, lsf, box_of(g), cand, cand_count);
+    if (!mask_src || (const void*)mask_src == (const void*)lsf) {
+        const int zd = zmarch_depth(g), m = mask_src ? 1 : 0;
+        if (g.tr == 32)
+            k_crossings_z<32, T><<<dim3(bulk_blocks(g, zd)), dim3(256), 0, s>>>(d, min_dist, flags, m, lsf, box_of(g),
+                                                                               zd, cand, cand_count);
+        else
+            k_crossings_z<64, T><<<dim3(bulk_blocks(g, zd)), dim3(256), 0, s>>>(d, min_dist, flags, m, lsf, box_of(g),
+                                                                               zd, cand, cand_count);
+    } else {
+        k_crossings<T><<<grid, block, 0, s>>>(d, min_dist, flags, mask_src, lsf, box_of(g), cand, cand_count);
+    }
     const dim3 cg(iface_blocks(g));
     if (order == 2)
         k_crossing_distances<2, T><<<cg, dim3(256), 0, s>>>(d, min_dist, flags, lsf, box_of(g), cand, cand_count, lv);
@@ -1374,9 +1443,13 @@ template <typename T>
 hipError_t launch_extension_normal(T* vx, T* vy, T* vz, const T* F, const uint8_t* flags, const ListView& lv,
                                    double floor_, const Geom& g, hipStream_t s)
 {
-    dim3 grid, block;
-    point_grid(g, grid, block);
-    k_extension_normal<T><<<grid, block, 0, s>>>(vx, vy, vz, F, flags, floor_, box_of(g));
+    const int zd = zmarch_depth(g);
+    if (g.tr == 32)
+        k_extension_normal_z<32, T><<<dim3(bulk_blocks(g, zd)), dim3(256), 0, s>>>(vx, vy, vz, F, flags, floor_,
+                                                                                  box_of(g), zd);
+    else
+        k_extension_normal_z<64, T><<<dim3(bulk_blocks(g, zd)), dim3(256), 0, s>>>(vx, vy, vz, F, flags, floor_,
+                                                                                  box_of(g), zd);
     const IList il{lv.idx, lv.count, lv.dist, lv.dt, lv.cap};
     k_extension_normal_list<T><<<dim3(iface_blocks(g)), dim3(256), 0, s>>>(vx, vy, vz, F, il, floor_, box_of(g));
     return hipGetLastError();

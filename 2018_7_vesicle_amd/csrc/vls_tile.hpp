@@ -78,6 +78,51 @@ __device__ __forceinline__ void wave_append(bool take, int value, int* __restric
     if (take) list[base + __popcll(m & ((1ull << lane) - 1ull))] = value;
 }
 
+// Block-aggregated list append: the lanes of a block collect their entries
+// in LDS and the block moves them to the list with one global atomic per
+// flush.  A wave-aggregated append costs one global atomic per wave, and
+// when many waves append, those atomics serialize on the one counter.  Every
+// thread of the block calls push() the same number of times and flush() once
+// at the end; both contain block barriers.
+template <int CAP>
+struct BlockList {
+    int* buf;    // LDS, CAP entries
+    int* count;  // LDS
+    int* base;   // LDS
+    __device__ __forceinline__ void init()
+    {
+        if (threadIdx.x == 0) *count = 0;
+        __syncthreads();
+    }
+    __device__ __forceinline__ void flush(int* __restrict__ list, int* __restrict__ list_count)
+    {
+        __syncthreads();
+        const int n = *count;  // the same in every thread after the barrier
+        if (n > 0) {
+            if (threadIdx.x == 0) *base = atomicAdd(list_count, n);
+            __syncthreads();
+            const int b = *base;
+            for (int i = (int)threadIdx.x; i < n; i += (int)blockDim.x) list[b + i] = buf[i];
+            __syncthreads();
+            if (threadIdx.x == 0) *count = 0;
+            __syncthreads();
+        }
+    }
+    __device__ __forceinline__ void push(bool take, int value, int* __restrict__ list, int* __restrict__ list_count)
+    {
+        __syncthreads();  // the previous push's entries are in, so *count is final
+        if (*count > CAP - (int)blockDim.x) flush(list, list_count);
+        const unsigned long long m = __ballot(take);
+        if (m == 0) return;
+        const int lane = (int)__lane_id();
+        const int leader = __ffsll((long long)m) - 1;
+        int off = 0;
+        if (lane == leader) off = atomicAdd(count, __popcll(m));
+        off = __shfl(off, leader, 64);
+        if (take) buf[off + __popcll(m & ((1ull << lane) - 1ull))] = value;
+    }
+};
+
 // as wave_append, returning each taking lane's slot (-1 for the others)
 __device__ __forceinline__ int wave_append_slot(bool take, int* __restrict__ count)
 {
