@@ -1348,7 +1348,7 @@ struct vls_batch {
     ListView lv{};
     int* over = nullptr;
     int* list_grid = nullptr;  // grid of each interface-list entry
-    int* counts = nullptr;     // [0] list count, [4..5] deferred counts
+    int* counts = nullptr;     // [0] list count, [3] candidate count of the list build, [4..5] deferred counts
     double* lcp = nullptr;
     double* lvel = nullptr;
     double* t[2] = {nullptr, nullptr};
@@ -1445,8 +1445,8 @@ T* bvel(vls_batch* b, int a)
 template <typename T>
 int batch_set_distance(vls_batch* b, const T* F, const T* mask_src, hipStream_t s)
 {
-    VLS_HIP(launch_zero_ints(b->counts, 1, s));
-    VLS_HIP(launch_batch_build_list<T>(b->view(), F, mask_src, b->flags, b->lv, 6, s));
+    VLS_HIP(launch_zero_ints(b->counts, 4, s));  // the list count and the candidate count (counts[3])
+    VLS_HIP(launch_batch_build_list<T>(b->view(), F, mask_src, b->flags, b->lv, 6, b->over, b->counts + 3, s));
     return VLS_OK;
 }
 
@@ -1530,8 +1530,8 @@ int batch_wenork3_reinit(vls_batch* b, double* F, int iterations, hipStream_t s)
     int* oc = b->counts + 4;
     double* t0 = btmp<double>(b, 0);
     double* t1 = btmp<double>(b, 1);
-    VLS_HIP(launch_zero_ints(b->counts, 1, s));
-    VLS_HIP(launch_batch_build_list<double>(B, F, F, b->flags, b->lv, 3, s));
+    VLS_HIP(launch_zero_ints(b->counts, 4, s));
+    VLS_HIP(launch_batch_build_list<double>(B, F, F, b->flags, b->lv, 3, b->over, b->counts + 3, s));
     VLS_HIP(launch_zero_ints(oc, 2, s));
     int seq = 0;
     for (int it = 0; it < iterations; ++it) {
@@ -1559,8 +1559,9 @@ int batch_variant_extend(vls_batch* b, const double* F, double* C, int iteration
     double* t0 = btmp<double>(b, 0);
     double* t1 = btmp<double>(b, 1);
     const int scheme = weno ? 1 : 2;  // the cubic ENO, the quadratic ENO
-    VLS_HIP(launch_zero_ints(b->counts, 1, s));
-    VLS_HIP(launch_batch_build_list<double>(B, F, nullptr, b->flags, b->lv, weno ? 3 : 2, s));
+    VLS_HIP(launch_zero_ints(b->counts, 4, s));
+    VLS_HIP(launch_batch_build_list<double>(B, F, nullptr, b->flags, b->lv, weno ? 3 : 2, b->over, b->counts + 3,
+                                            s));
     VLS_HIP(launch_batch_extension_velocity<double>(B, vx, vy, vz, F, b->flags, b->lv, 1e-6, s, scheme));
     VLS_HIP(launch_batch_list_interp<double>(B, b->lcp, b->lv, C, s, scheme));
     if (weno) {

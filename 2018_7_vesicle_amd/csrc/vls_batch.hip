@@ -59,19 +59,25 @@ __device__ __forceinline__ T* shift(T* p, int64_t off)
 
 // ---------------------------------------------------------------------------
 // list building: setDistance (setDistance.m:5-31 with boundary_location.cu:84-132)
-// and the interface list in one pass.  Each point computes its own six
-// distances: the r-side from the crossing (i, i+1) on the stencil around i,
-// the l-side as spacing minus the distance of the crossing (i-1, i) on the
-// stencil around i-1, which is what boundary_location writes into the l-side
-// array of i; no full distance arrays are stored.
+// and the interface list, in two passes as for one grid (k_crossings,
+// k_crossing_distances): every point tests its six neighbours for a crossing
+// and writes its flags (mask bit only), the points with a crossing go to a
+// candidate list (block-aggregated); one lane per candidate then computes its
+// six distances (the r-side from the crossing (i, i+1) on the stencil around
+// i, the l-side as spacing minus the distance of the crossing (i-1, i) on the
+// stencil around i-1, which is what boundary_location writes into the
+// l-side array of i) and, where one differs from the spacing, sets
+// FLAG_SUBCELL and appends the point to the list with its distances, time
+// step and grid.  No full distance arrays are stored.
 // ---------------------------------------------------------------------------
-// ORDER: the locator, 6 (boundary_location), 2 (the quadratic-ENO
-// boundary_correction of the ENO-RK2 variants) or 3 (the cubic one of the
-// WENO-RK3 variants).
-template <int R, typename T, int ORDER = 6>
-__global__ void __launch_bounds__(256) kb_build_list(const T* __restrict__ F, const T* __restrict__ mask_src,
-                                                     uint8_t* __restrict__ flags, ListView il, BatchView B)
+template <int R, typename T>
+__global__ void __launch_bounds__(256) kb_crossings(const T* __restrict__ F, const T* __restrict__ mask_src,
+                                                    uint8_t* __restrict__ flags, BatchView B, int* __restrict__ cand,
+                                                    int* __restrict__ cand_count)
 {
+    __shared__ int cbuf[1024], ccount, cbase;
+    BlockList<1024> bl{cbuf, &ccount, &cbase};
+    bl.init();
     const TileRange tr = tile_range(B.ntiles);
     for (int tile = tr.first; tile < tr.end; tile += tr.step) {
         const int g = B.tile_grid[tile];
@@ -79,47 +85,75 @@ __global__ void __launch_bounds__(256) kb_build_list(const T* __restrict__ F, co
         const Box b = box_of(G);
         const Tile t = tile_at<R>(b, tile - G.tile0);
         const T* f = F + G.off;
-        const double sp[3] = {b.dx, b.dy, b.dz};
-        Dists q;
-        bool sub = false;
+        bool cross = false;
         idx_t ind = 0;
         if (t.valid) {
             ind = lin(b, t.r, t.c, t.p);
             const double f0 = (double)f[ind];
+            const double xu = (double)f[lin(b, t.r, min(t.c + 1, b.cols - 1), t.p)];
+            const double xd = (double)f[lin(b, t.r, max(t.c - 1, 0), t.p)];
+            const double yu = (double)f[lin(b, min(t.r + 1, b.rows - 1), t.c, t.p)];
+            const double yd = (double)f[lin(b, max(t.r - 1, 0), t.c, t.p)];
+            const double zu = (double)f[lin(b, t.r, t.c, min(t.p + 1, b.pges - 1))];
+            const double zd = (double)f[lin(b, t.r, t.c, max(t.p - 1, 0))];
+            cross = (f0 * xu < 0) | (xd * f0 < 0) | (f0 * yu < 0) | (yd * f0 < 0) | (f0 * zu < 0) | (zd * f0 < 0);
+            flags[G.off + ind] = (mask_src && mask_src[G.off + ind] < 0) ? (uint8_t)FLAG_MASK : (uint8_t)0;
+        }
+        bl.push(cross, (int)(G.off + ind), cand, cand_count);
+    }
+    bl.flush(cand, cand_count);
+}
+
+// ORDER: the locator, 6 (boundary_location), 2 (the quadratic-ENO
+// boundary_correction of the ENO-RK2 variants) or 3 (the cubic one of the
+// WENO-RK3 variants).
+template <typename T, int ORDER>
+__global__ void __launch_bounds__(256) kb_crossing_distances(const T* __restrict__ F, uint8_t* __restrict__ flags,
+                                                             ListView il, BatchView B, const int* __restrict__ cand,
+                                                             const int* __restrict__ cand_count)
+{
+    __shared__ int lcnt, lbase;
+    const int total = *cand_count;
+    for (int j0 = (int)(blockIdx.x * 256); j0 < total; j0 += (int)(gridDim.x * 256)) {
+        const int j = j0 + (int)threadIdx.x;
+        Dists q;
+        bool sub = false;
+        int64_t gind = 0;
+        int g = 0;
+        if (j < total) {
+            gind = cand[j];
+            g = grid_of(B, gind);
+            const BGrid G = B.grids[g];
+            const Box b = box_of(G);
+            int r, c, p;
+            coords_of(b, gind - G.off, r, c, p);
+            const T* f = F + G.off;
+            const double sp[3] = {b.dx, b.dy, b.dz};
 #pragma unroll 1
             for (int a = 0; a < 3; ++a) {
-                int rp = t.r, cp = t.c, pp = t.p, rm = t.r, cm = t.c, pm = t.p;
-                if (a == 0) { cp = min(t.c + 1, b.cols - 1); cm = max(t.c - 1, 0); }
-                else if (a == 1) { rp = min(t.r + 1, b.rows - 1); rm = max(t.r - 1, 0); }
-                else { pp = min(t.p + 1, b.pges - 1); pm = max(t.p - 1, 0); }
-                const double fp = (double)f[lin(b, rp, cp, pp)], fm = (double)f[lin(b, rm, cm, pm)];
+                double v[7];
+                line7(f, r, c, p, a, b, v);
                 q.r[a] = sp[a];
                 q.l[a] = sp[a];
-                const bool cr = f0 * fp < 0, cl = fm * f0 < 0;
-                if (cr || cl) {
-                    double v[7];
-                    line7(f, t.r, t.c, t.p, a, b, v);
-                    if (cr)
-                        q.r[a] = (ORDER == 2)   ? eno_quadratic_distance(v[2], v[3], v[4], v[5], sp[a])
-                                 : (ORDER == 3) ? cubic_distance(v[2], v[3], v[4], v[5], sp[a])
-                                                : six_distance(v[1], v[2], v[3], v[4], v[5], v[6], sp[a]);
-                    if (cl)
-                        q.l[a] = sp[a] - ((ORDER == 2)   ? eno_quadratic_distance(v[1], v[2], v[3], v[4], sp[a])
-                                          : (ORDER == 3) ? cubic_distance(v[1], v[2], v[3], v[4], sp[a])
-                                                         : six_distance(v[0], v[1], v[2], v[3], v[4], v[5], sp[a]));
-                }
+                if (v[3] * v[4] < 0)
+                    q.r[a] = (ORDER == 2)   ? eno_quadratic_distance(v[2], v[3], v[4], v[5], sp[a])
+                             : (ORDER == 3) ? cubic_distance(v[2], v[3], v[4], v[5], sp[a])
+                                            : six_distance(v[1], v[2], v[3], v[4], v[5], v[6], sp[a]);
+                if (v[2] * v[3] < 0)
+                    q.l[a] = sp[a] - ((ORDER == 2)   ? eno_quadratic_distance(v[1], v[2], v[3], v[4], sp[a])
+                                      : (ORDER == 3) ? cubic_distance(v[1], v[2], v[3], v[4], sp[a])
+                                                     : six_distance(v[0], v[1], v[2], v[3], v[4], v[5], sp[a]));
             }
             // a distance that differs from the spacing (the quadratic ENO
             // locator is not clamped and can land just above it)
             sub = q.r[0] != b.dx || q.l[0] != b.dx || q.r[1] != b.dy || q.l[1] != b.dy || q.r[2] != b.dz ||
                   q.l[2] != b.dz;
-            uint8_t fl = sub ? FLAG_SUBCELL : 0;
-            if (mask_src && mask_src[G.off + ind] < 0) fl |= FLAG_MASK;
-            flags[G.off + ind] = fl;
+            if (sub) flags[gind] |= (uint8_t)FLAG_SUBCELL;
         }
-        const int slot = wave_append_slot(sub, il.count);
-        if (sub) {
-            il.idx[slot] = (int)(G.off + ind);
+        const int slot = block_slot(sub, il.count, &lcnt, &lbase);
+        if (slot >= 0) {
+            const Box b = box_of(B.grids[g]);
+            il.idx[slot] = (int)gind;
             B.list_grid[slot] = g;
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
@@ -730,17 +764,13 @@ static inline IList ilist(const ListView& lv) { return IList{lv.idx, lv.count, l
 
 template <typename T>
 hipError_t launch_batch_build_list(const BatchView& B, const T* F, const T* mask_src, uint8_t* flags,
-                                   const ListView& lv, int order, hipStream_t s)
+                                   const ListView& lv, int order, int* cand, int* cand_count, hipStream_t s)
 {
-    if (order == 2) {
-        if (B.tr == 32) kb_build_list<32, T, 2><<<B.blocks, 256, 0, s>>>(F, mask_src, flags, lv, B);
-        else kb_build_list<64, T, 2><<<B.blocks, 256, 0, s>>>(F, mask_src, flags, lv, B);
-    } else if (order == 3) {
-        if (B.tr == 32) kb_build_list<32, T, 3><<<B.blocks, 256, 0, s>>>(F, mask_src, flags, lv, B);
-        else kb_build_list<64, T, 3><<<B.blocks, 256, 0, s>>>(F, mask_src, flags, lv, B);
-    } else {
-        TR_LAUNCH(kb_build_list, T, B.blocks, F, mask_src, flags, lv, B);
-    }
+    TR_LAUNCH(kb_crossings, T, B.blocks, F, mask_src, flags, B, cand, cand_count);
+    const int nb = list_blocks(B);
+    if (order == 2) kb_crossing_distances<T, 2><<<nb, 256, 0, s>>>(F, flags, lv, B, cand, cand_count);
+    else if (order == 3) kb_crossing_distances<T, 3><<<nb, 256, 0, s>>>(F, flags, lv, B, cand, cand_count);
+    else kb_crossing_distances<T, 6><<<nb, 256, 0, s>>>(F, flags, lv, B, cand, cand_count);
     return hipGetLastError();
 }
 
@@ -912,7 +942,7 @@ hipError_t launch_batch_ellipsoid(const BatchView& B, T* F, const double* axes, 
 
 #define VLS_BATCH_INSTANTIATE(T)                                                                                    \
     template hipError_t launch_batch_build_list<T>(const BatchView&, const T*, const T*, uint8_t*, const ListView&, \
-                                                   int, hipStream_t);                                               \
+                                                   int, int*, int*, hipStream_t);                                   \
     template hipError_t launch_batch_set_mask_bit<T>(uint8_t*, const T*, int64_t, hipStream_t);                     \
     template hipError_t launch_batch_reinit_stage<T>(const BatchView&, T*, const T*, const T*, const uint8_t*,      \
                                                      const ListView&, int, int*, int*, int, hipStream_t, int);      \

@@ -284,11 +284,12 @@ __host__ __device__ inline int grid_tiles(int rows, int cols, int pges, int tr)
 
 // list building: subcell distances (the locator of `order`: 6, 2 for the
 // quadratic ENO, 3 for the cubic), flags (mask bit from mask_src < 0 when
-// mask_src is not null) and the interface list of every grid, in one pass;
-// *lv.count must be zero on entry
+// mask_src is not null) and the interface list of every grid, in two passes;
+// cand (total ints) and *cand_count are work memory; *lv.count and
+// *cand_count must be zero on entry
 template <typename T>
 hipError_t launch_batch_build_list(const BatchView& B, const T* F, const T* mask_src, uint8_t* flags,
-                                   const ListView& lv, int order, hipStream_t s);
+                                   const ListView& lv, int order, int* cand, int* cand_count, hipStream_t s);
 template <typename T>
 hipError_t launch_batch_set_mask_bit(uint8_t* flags, const T* F, int64_t n, hipStream_t s);
 // `scheme` of the listed and deferred points: 0 the nonuniform WENO of the

@@ -123,6 +123,28 @@ struct BlockList {
     }
 };
 
+// Block-aggregated slot assignment: every thread of the block calls it; the
+// taking lanes get consecutive slots of the list counted by *count, with one
+// global atomic for the whole block (lds_cnt, lds_base: two LDS ints).
+__device__ __forceinline__ int block_slot(bool take, int* __restrict__ count, int* lds_cnt, int* lds_base)
+{
+    __syncthreads();  // the previous call's readers of *lds_base are done
+    if (threadIdx.x == 0) *lds_cnt = 0;
+    __syncthreads();
+    const unsigned long long m = __ballot(take);
+    const int lane = (int)__lane_id();
+    int off = 0;
+    if (m != 0) {
+        const int leader = __ffsll((long long)m) - 1;
+        if (lane == leader) off = atomicAdd(lds_cnt, __popcll(m));
+        off = __shfl(off, leader, 64) + __popcll(m & ((1ull << lane) - 1ull));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *lds_base = (*lds_cnt > 0) ? atomicAdd(count, *lds_cnt) : 0;
+    __syncthreads();
+    return take ? *lds_base + off : -1;
+}
+
 // as wave_append, returning each taking lane's slot (-1 for the others)
 __device__ __forceinline__ int wave_append_slot(bool take, int* __restrict__ count)
 {
