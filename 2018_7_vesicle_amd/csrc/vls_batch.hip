@@ -337,25 +337,40 @@ __global__ void __launch_bounds__(256) kb_ext_vel_bulk(T* __restrict__ vx, T* __
                                                        const T* __restrict__ F, const uint8_t* __restrict__ flags,
                                                        double floor_, BatchView B)
 {
-    const TileRange tr = tile_range(B.ntiles);
-    for (int tile = tr.first; tile < tr.end; tile += tr.step) {
-        const BGrid G = B.grids[B.tile_grid[tile]];
-        const Box b = box_of(G);
-        const Tile t = tile_at<R>(b, tile - G.tile0);
-        if (!t.valid) continue;
-        const idx_t ind = lin(b, t.r, t.c, t.p);
-        if (flags[G.off + ind] & FLAG_SUBCELL) continue;
-        const double sp[3] = {b.dx, b.dy, b.dz};
-        double n[3];
+    // one z-march work item per block (x- and y-taps from the staged tile,
+    // z-taps from the register window); only the side |F(+-1)| selects is
+    // evaluated, in the Jiang-Peng form of weno5_pair
+    __shared__ T lds[2][Shape<R>::N];
+    const ZItem it = B.items[xcd_remap((int)blockIdx.x, (int)gridDim.x)];
+    const BGrid G = B.grids[it.g];
+    const Box b = box_of(G);
+    const idx_t off = G.off, plane = (idx_t)b.rows * b.cols;
+    const double sp[3] = {b.dx, b.dy, b.dz};
+    zmarch<R, T, false, true>(lds[0], lds[1], b, it.r0, it.c0, it.p0, it.p1, F + off, nullptr, nullptr, nullptr,
+                              nullptr, flags + off, 0,
+                              [&](const T* l, const ZCol<R>& zc, const T w[7], const ZStep<R, T, false>& st, int p) {
+                                  if (st.flags(zc) & FLAG_SUBCELL) return;  // listed, or outside the box
+                                  double n[3];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            double v[7], fo, bk;
-            line7(F + G.off, t.r, t.c, t.p, a, b, v);
-            weno5_pair(fo, bk, v, jpd(sp[a]));
-            n[a] = (fabs(v[4]) < fabs(v[2])) ? fo : bk;
-        }
-        store_extension_velocity(vx, vy, vz, F, (idx_t)(G.off + ind), n, floor_);
-    }
+                                  for (int a = 0; a < 3; ++a) {
+                                      T t[7];
+                                      if (a == 0) tile_line_x<R>(l, zc.base, t);
+                                      else if (a == 1) tile_line_y(l, zc.base, t);
+                                      else
+#pragma unroll
+                                          for (int k = 0; k < 7; ++k) t[k] = w[k];
+                                      double v[7];
+#pragma unroll
+                                      for (int k = 0; k < 7; ++k) v[k] = (double)t[k];
+                                      const JPd j = jpd(sp[a]);
+                                      const JPcd<double> m = jp_centre(v);
+                                      n[a] = (fabs(v[4]) < fabs(v[2]))
+                                                 ? (m.c + jp_q(m.d[4], m.d[3], m.d[2], m.d[1], j.e)) * j.k
+                                                 : (m.c - jp_q(m.d[0], m.d[1], m.d[2], m.d[3], j.e)) * j.k;
+                                  }
+                                  store_extension_velocity(vx, vy, vz, F, (idx_t)(off + (idx_t)p * plane + zc.in), n,
+                                                           floor_);
+                              });
 }
 
 // S: the interface scheme of the listed points' upwind normal: the
@@ -861,13 +876,13 @@ hipError_t launch_batch_extension_velocity(const BatchView& B, T* vx, T* vy, T* 
             return hipGetLastError();
         }
         if (scheme == IFACE_CUBIC) {
-            TR_LAUNCH(kb_ext_vel_bulk, T, B.blocks, vx, vy, vz, F, flags, floor_, B);
+            TR_LAUNCH(kb_ext_vel_bulk, T, B.nitems, vx, vy, vz, F, flags, floor_, B);
             kb_ext_vel_list<T, IFACE_CUBIC><<<list_blocks(B), 256, 0, s>>>(vx, vy, vz, F, ilist(lv), floor_, B);
             return hipGetLastError();
         }
     }
     if (scheme != IFACE_WENO) return hipErrorInvalidValue;
-    TR_LAUNCH(kb_ext_vel_bulk, T, B.blocks, vx, vy, vz, F, flags, floor_, B);
+    TR_LAUNCH(kb_ext_vel_bulk, T, B.nitems, vx, vy, vz, F, flags, floor_, B);
     kb_ext_vel_list<T><<<list_blocks(B), 256, 0, s>>>(vx, vy, vz, F, ilist(lv), floor_, B);
     return hipGetLastError();
 }
