@@ -396,13 +396,14 @@ def per_grid_fp64(cfg, steps, vesicles):
     return updates / wall, wall, [o[0].F for o in objs]
 
 
-def batch_comparison(cfg, steps=10, sample_every=16):
+def batch_comparison(cfg, steps=10, sample_every=16, log=None):
     """Config 4: the batched workload in FP32 bulk (autotuned launch shape) and
     in FP64 (same batching), and the plain FP64 path (one object per vesicle)
     on every `sample_every`-th vesicle; rates in WENO updates/s, and the FP32
     run's area/volume drift and interface displacement against the FP64 run
     on every vesicle, both from the same initial state (the FP64 run's,
-    rounded)."""
+    rounded).  log(msg), if given, is called as each part ends."""
+    log = log or (lambda msg: None)
     import dataclasses
     import time
 
@@ -428,9 +429,11 @@ def batch_comparison(cfg, steps=10, sample_every=16):
         out[prec + "_batched"] = {"updates_per_s": upd / wall, "wall_s": wall, "points": p.batch.total,
                                   "tile_rows": p.batch.tile_rows, "blocks": p.batch.blocks}
         del p
+        log(f"{prec} batched run done ({wall:.1f} s)")
     b64 = runs["fp64"]["batch"]
     sample = list(range(0, cfg.vesicles, sample_every))
     rate, wall, _ = per_grid_fp64(cfg, steps, sample)
+    log(f"plain FP64 runs of {len(sample)} vesicles done ({wall:.1f} s)")
     out["fp64_per_vesicle"] = {"updates_per_s": rate, "wall_s": wall, "vesicles": len(sample)}
     out["speedup_fp32_batched_vs_fp64_per_vesicle"] = out["fp32_batched"]["updates_per_s"] / rate
     out["speedup_fp32_batched_vs_fp64_batched"] = (out["fp32_batched"]["updates_per_s"] /

@@ -459,7 +459,7 @@ def config4(steps):
     import importlib
 
     bm = importlib.import_module("2018_7_vesicle_amd.batch")
-    r = bm.batch_comparison(bm.BatchConfig(), steps=steps, sample_every=16)
+    r = bm.batch_comparison(bm.BatchConfig(), steps=steps, sample_every=16, log=progress)
     r["workload"] = (f"512 vesicles, edges {{32,48,64,80,96}} on [-1,1]^3, ellipsoid semi-axes U[0.3,0.8], "
                      f"8-mode curl velocity (seed 2018+v), WENO5-RK3 advection CFL 0.4, 40-iteration extension, "
                      f"reinit 100 iterations every 5 steps; {steps} steps timed")
@@ -693,6 +693,12 @@ def config5():
     return r
 
 
+def progress(msg):
+    """One line on stderr as each part of the run ends (the JSON line alone
+    goes to stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     args = parse()
     if args.probe == "valu-rate":
@@ -739,6 +745,7 @@ def main():
     # the metric's configuration: the vesicle step at 256^3 (32-mode velocity),
     # FP32 bulk, beside the plain FP64 path on the same workload
     r32 = pipeline_line(vls, args.n, args.modes, "fp32", args.steps, args.warmup)
+    progress(f"FP32 pipeline at {args.n}^3 done")
     r64 = pipeline_line(vls, args.n, args.modes, "fp64", args.steps, args.warmup)
     cfg = r32["cfg"]
     stage_us = r32["stage_us"]
@@ -801,20 +808,30 @@ def main():
     }
     del r32, r64
     torch.cuda.empty_cache()
+    progress(f"main line: FP32 {out['value'] / 1e9:.1f} G, FP64 {out['fp64']['value'] / 1e9:.1f} G updates/s")
     if not args.main_only:
         out["config2_128"] = config2_line(vls, args.steps, args.warmup)
+        progress("config2_128 done")
         out["config1"] = config1(vls, not args.no_cpu)
+        progress("config1 done")
         if args.precision_steps > 0:
             out["config3"] = config3(args.precision_steps)
+            progress("config3 done")
         if args.batch_steps > 0:
             out["config4"] = config4(args.batch_steps)
+            progress("config4 done")
         if not args.no_adjoint:
             out["config5"] = config5()
+            progress("config5 done")
         out["variants"] = variants(not args.no_cpu)
+        progress("variants done")
         out["toolbox"] = toolbox(not args.no_cpu)
+        progress("toolbox done")
         out["relax"] = relax(not args.no_cpu)
+        progress("relax done")
         if not args.no_cpu:
             out["cpu_baseline"] = cpu_baseline(cfg)
+            progress("cpu_baseline done")
     print(json.dumps(out), flush=True)
     return 0
 
