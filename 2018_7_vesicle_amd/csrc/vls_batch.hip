@@ -214,11 +214,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(stage_
     const Inv invs = inv_of(G);
     const idx_t off = G.off, plane = (idx_t)b.rows * b.cols;
     T* o = out + off;
+    ZFace<T> zf;
     zmarch<R, T, false, true>(lds[0], lds[1], b, it.r0, it.c0, it.p0, it.p1, fs + off, shift(fn, off), nullptr, nullptr,
                         nullptr, flags + off, stage,
                         [&](const T* l, const ZCol<R>& zc, const T w[7], const ZStep<R, T, false>& st, int p) {
                             bool cross;
-                            const T res = reinit_value<R>(l, zc.base, w, st.fn, st.flags(zc), invs, stage, cross);
+                            const T res = reinit_value<R>(l, zc.base, w, st.fn, st.flags(zc), invs, stage, cross,
+                                                          zf, p == it.p0);
                             const bool bulk = !(st.flags(zc) & FLAG_SUBCELL);
                             if (bulk && !cross) stp(o + (idx_t)p * plane, zc.in, res);
                             wave_append(bulk && cross, (int)(off + (idx_t)p * plane + zc.in), over,

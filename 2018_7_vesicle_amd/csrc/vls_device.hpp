@@ -264,6 +264,153 @@ __device__ __forceinline__ float opposite_key(float a, float b)
 __device__ __forceinline__ bool opposite(float a, float b) { return opposite_key(a, b) < 0.f; }
 
 // ---------------------------------------------------------------------------
+// z-axis WENO5 pairs along a z-march (one point per page, pages in order).
+// The back side of page p+1 and the fore side of page p take the same four
+// second differences in reverse order (E_{p-1}..E_{p+2}), so their outer
+// smoothness indicators coincide exactly (s0 of one is s2 of the other, the
+// same operations on the same operands) and so do the products of the FP64
+// weights; the back side of page p+1 takes them from page p.  The results
+// are bit for bit those of weno5_pair on the page's 7-point z-line.  ZFace
+// holds what page p leaves for page p+1; zface_start sets it up for the
+// block's first page from that page's z-line (the fore side of the page
+// before, whose result is not used).
+// ---------------------------------------------------------------------------
+template <typename T>
+struct ZFace;
+
+template <>
+struct ZFace<float> {
+    float d[4];    // D0..D3 of the next page (D1..D4 of this one)
+    float ic;      // 39/64 D2^2 + e' of the next page (this page's fore-side ib)
+    float x0, x2;  // back-side 1/s0^2, 1/s2^2 of the next page (this page's fore 1/s2^2, 1/s0^2)
+    float A;       // back-side A of the next page (this page's B)
+    float key;     // opposite_key of the next page's z-values 2 and 3
+};
+
+template <>
+struct ZFace<double> {
+    double d[4];        // D0..D3 of the next page
+    double w0, w1, w2;  // back-side weights of the next page
+    double tA, tB;      // back-side t0 - t1 and t1 - t2 of the next page
+    bool cross;         // v2 * v3 < 0 at the next page
+};
+
+// the fore side Q(d4, d3, d2, d1) of a page with B = d1 - 2 d2 + d3 and
+// ic = 39/64 d2^2 + e'; leaves the next page's back-side values in f
+__device__ __forceinline__ float zface_fore(ZFace<float>& f, float d1, float d2, float d3, float d4, float ic,
+                                            float B, float e)
+{
+    const float ib = fmaf(0.609375f * d3, d3, e);
+    const float u0 = fmaf(-1.375f, d3, d4), u1 = fmaf(-0.625f, d2, d3), u2 = fmaf(-1.375f, d2, d1);
+    const float r0 = __builtin_amdgcn_rcpf(fmaf(u0, u0, ib));
+    const float r1 = __builtin_amdgcn_rcpf(fmaf(u1, u1, ic));
+    const float r2 = __builtin_amdgcn_rcpf(fmaf(u2, u2, ic));
+    const float x0 = r0 * r0, x2 = r2 * r2;
+    const float W = fmaf(6.f * r1, r1, fmaf(3.f, x2, x0));
+    const float A = fmaf(-2.f, d3, d4) + d2;
+    const float P = fmaf(4.f * x0, A, (6.f * x2) * B);
+    f.d[0] = d1;
+    f.d[1] = d2;
+    f.d[2] = d3;
+    f.d[3] = d4;
+    f.ic = ib;
+    f.x0 = x2;
+    f.x2 = x0;
+    f.A = B;
+    return fmaf(P, __builtin_amdgcn_rcpf(__builtin_fmaxf(W, 1e-30f)), -B);
+}
+
+__device__ __forceinline__ void zface_start(ZFace<float>& f, const float p[7], JPf j)
+{
+    float d[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) d[k] = fmaf(-2.f, p[k + 1], p[k]) + p[k + 2];
+    const float ic = fmaf(0.609375f * d[1], d[1], j.e);
+    (void)zface_fore(f, d[0], d[1], d[2], d[3], ic, fmaf(-2.f, d[1], d[0]) + d[2], j.e);
+    f.key = opposite_key(p[2], p[3]);
+}
+
+// weno5_pair on the page's z-line p[0..6] with the faces of the page before;
+// cross: the sign test of sign_change3 on p[2..4]
+__device__ __forceinline__ void weno5_pair_z(float& d_fore, float& d_back, bool& cross, const float p[7],
+                                             ZFace<float>& f, JPf j)
+{
+    const float d1 = f.d[1], d2 = f.d[2], d3 = f.d[3];
+    const float d4 = fmaf(-2.f, p[5], p[4]) + p[6];
+    const float c = fmaf(8.f, p[4] - p[2], p[1] - p[5]);
+    const float ic = f.ic;
+    const float B = fmaf(-2.f, d2, d1) + d3;
+    // back: a = d0, b = d1, c = d2, d = d3; 1/s0^2, 1/s2^2 and A from the page before
+    float qb;
+    {
+        const float u1 = fmaf(-0.625f, d2, d1);
+        const float r1 = __builtin_amdgcn_rcpf(fmaf(u1, u1, ic));
+        const float W = fmaf(6.f * r1, r1, fmaf(3.f, f.x2, f.x0));
+        const float P = fmaf(4.f * f.x0, f.A, (6.f * f.x2) * B);
+        qb = fmaf(P, __builtin_amdgcn_rcpf(__builtin_fmaxf(W, 1e-30f)), -B);
+    }
+    const float key = opposite_key(p[3], p[4]);
+    cross = __builtin_fminf(f.key, key) < 0.f;
+    f.key = key;
+    const float qf = zface_fore(f, d1, d2, d3, d4, ic, B, j.e);
+    d_back = (c - qb) * j.k;
+    d_fore = (c + qf) * j.k;
+}
+
+// FP64: jp_q(d4, d3, d2, d1) (its exact operation sequence) with its weights
+// and differences left in f as the next page's back side
+__device__ __forceinline__ double zface_fore(ZFace<double>& f, double d1, double d2, double d3, double d4, double e)
+{
+    const double k = 3.0 / 13.0;
+    const double t0 = d4 - d3, t1 = d3 - d2, t2 = d2 - d1;
+    const double y0 = fma(-3.0, d3, d4), y1 = d3 + d2, y2 = fma(3.0, d2, -d1);
+    const double s0 = fma(t0, t0, fma(k * y0, y0, e));
+    const double s1 = fma(t1, t1, fma(k * y1, y1, e));
+    const double s2 = fma(t2, t2, fma(k * y2, y2, e));
+    const double a0 = s0 * s0, a1 = s1 * s1, a2 = s2 * s2;
+    const double w0 = a1 * a2, m01 = a0 * a1, w1 = 6.0 * (a0 * a2), w2 = 3.0 * m01;
+    const double W = (w0 + w1) + w2;
+    const double tA = t0 - t1, tB = t1 - t2;
+    f.d[0] = d1;
+    f.d[1] = d2;
+    f.d[2] = d3;
+    f.d[3] = d4;
+    // the next page's back side: s0, s1, s2 are this side's s2, s1, s0, so
+    // w0 = a1 a0, w1 = 6 (a2 a0), w2 = 3 (a2 a1), t0 - t1 = tB, t1 - t2 = tA
+    f.w0 = m01;
+    f.w1 = w1;
+    f.w2 = 3.0 * w0;
+    f.tA = tB;
+    f.tB = tA;
+    return fma(4.0 * w0, tA, fma(2.0, w2, -W) * tB) / W;
+}
+
+__device__ __forceinline__ void zface_start(ZFace<double>& f, const double p[7], JPd j)
+{
+    double d[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) d[k] = fma(-2.0, p[k + 1], p[k]) + p[k + 2];
+    (void)zface_fore(f, d[0], d[1], d[2], d[3], j.e);
+    f.cross = p[2] * p[3] < 0;
+}
+
+__device__ __forceinline__ void weno5_pair_z(double& d_fore, double& d_back, bool& cross, const double p[7],
+                                             ZFace<double>& f, JPd j)
+{
+    const double d1 = f.d[1], d2 = f.d[2], d3 = f.d[3];
+    const double d4 = fma(-2.0, p[5], p[4]) + p[6];
+    const double c = fma(8.0, p[4] - p[2], p[1] - p[5]);
+    const double W = (f.w0 + f.w1) + f.w2;
+    const double qb = fma(4.0 * f.w0, f.tA, fma(2.0, f.w2, -W) * f.tB) / W;
+    const bool up = p[3] * p[4] < 0;
+    cross = f.cross | up;
+    f.cross = up;
+    const double qf = zface_fore(f, d1, d2, d3, d4, j.e);
+    d_back = (c - qb) * j.k;
+    d_fore = (c + qf) * j.k;
+}
+
+// ---------------------------------------------------------------------------
 // Nonuniform three-stencil WENO at a node next to the interface:
 // re_step.cu:79-243 (weno_nonuniform), reconstructing at face a[2] (fore) or
 // a[3] (back) from first divided differences u[k] over [a[k], a[k+1]].

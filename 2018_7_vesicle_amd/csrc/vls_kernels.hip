@@ -365,10 +365,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(stage_
     int r0, c0, p0, p1;
     zmarch_share<R>(b, depth, blk - niface, (int)gridDim.x - niface, r0, c0, p0, p1);
     const idx_t plane = (idx_t)b.rows * b.cols;
+    ZFace<T> zf;
     zmarch<R, T, false, true>(lds[0], lds[1], b, r0, c0, p0, p1, fs, fn, nullptr, nullptr, nullptr, flags, stage,
                         [&](const T* l, const ZCol<R>& zc, const T w[7], const ZStep<R, T, false>& st, int p) {
                             bool cross;
-                            const T res = reinit_value<R>(l, zc.base, w, st.fn, st.flags(zc), invs, stage, cross);
+                            const T res = reinit_value<R>(l, zc.base, w, st.fn, st.flags(zc), invs, stage, cross,
+                                                          zf, p == p0);
                             const bool bulk = !(st.flags(zc) & FLAG_SUBCELL);
                             if (bulk && !cross) stp(out + (idx_t)p * plane, zc.in, res);
                             wave_append(bulk && cross, (int)((idx_t)p * plane + zc.in), over, &over_count[seq & 1]);

@@ -657,10 +657,12 @@ __device__ __forceinline__ void zmarch(T* lds0, T* lds1, const Box& b, int r0, i
     if (p0 >= p1) return;
     const ZCol<R> zc = zcol<R>(b, r0, c0);
     const idx_t plane = (idx_t)b.rows * b.cols;
-    // the window of page p0 - 1 (pages p0-4 .. p0+2, clamped)
+    // the window of page p0 - 1 (pages p0-4 .. p0+2, clamped); its first
+    // value leaves the window before any page reads it, so it is not loaded
     T w[7];
+    w[0] = T(0);
 #pragma unroll
-    for (int k = 0; k < 7; ++k) w[k] = ldp(fs + (idx_t)clampi(p0 - 4 + k, b.pges - 1) * plane, zc.in);
+    for (int k = 1; k < 7; ++k) w[k] = ldp(fs + (idx_t)clampi(p0 - 4 + k, b.pges - 1) * plane, zc.in);
     ZStep<R, T, VEL> sa, sb;
     zstep_load<R, T, VEL, FL>(sa, zc, p0, b, plane, fs, fn, vx, vy, vz, flags, stage);
     for (int p = p0;;) {
@@ -708,23 +710,26 @@ __host__ __device__ inline int zmarch_blocks(int rows, int cols, int pges, int d
 // the start value fn and flags fl.  Returns the new value; cross is set for a
 // point next to a sign change of the stage field (it is left to the interface
 // pass).
+// zf: the z-faces the march's previous page left (first: this is the
+// block's first page, zf is set up here)
 template <int R, typename T>
 __device__ __forceinline__ T reinit_value(const T* lds, int base, const T z[7], T fn, uint8_t fl, const Inv& invs,
-                                          int stage, bool& cross)
+                                          int stage, bool& cross, ZFace<T>& zf, bool first)
 {
     cross = false;
     T fo[3], bk[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
+    for (int a = 0; a < 2; ++a) {
         T v[7];
         if (a == 0) tile_line_x<R>(lds, base, v);
-        else if (a == 1) tile_line_y(lds, base, v);
-        else
-#pragma unroll
-            for (int k = 0; k < 7; ++k) v[k] = z[k];
+        else tile_line_y(lds, base, v);
         cross |= sign_change3(v);
         weno5_pair(fo[a], bk[a], v, axis_arg(invs, a, T()));
     }
+    if (first) zface_start(zf, z, axis_arg(invs, 2, T()));
+    bool cz;
+    weno5_pair_z(fo[2], bk[2], cz, z, zf, axis_arg(invs, 2, T()));
+    cross |= cz;
     const T step = godunov_step((fl & FLAG_MASK) != 0, fo[0], bk[0], fo[1], bk[1], fo[2], bk[2], (T)invs.dt_bulk);
     return rk_combine(stage, z[3], step, fn);
 }
