@@ -286,11 +286,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(stage_
     const Inv inv = inv_of(G);
     const idx_t off = G.off, plane = (idx_t)b.rows * b.cols;
     T* o = out + off;
-    zmarch<R, T, true, true>(lds[0], lds[1], b, it.r0, it.c0, it.p0, it.p1, cs + off, shift(cn, off), vx + off, vy + off,
-                       vz + off, flags + off, stage,
+    // the listed points carry a NaN x-velocity (kb_list_gather_vel)
+    zmarch<R, T, true, false>(lds[0], lds[1], b, it.r0, it.c0, it.p0, it.p1, cs + off, shift(cn, off), vx + off,
+                       vy + off, vz + off, nullptr, stage,
                        [&](const T* l, const ZCol<R>& zc, const T w[7], const ZStep<R, T, true>& st, int p) {
                            const T res = upwind_value<R>(l, zc.base, w, st.vel, st.fn, (T)inv.dt_bulk, inv, stage);
-                           if (!(st.flags(zc) & FLAG_SUBCELL)) stp(o + (idx_t)p * plane, zc.in, res);
+                           if (zc.valid && st.vel[0] == st.vel[0]) stp(o + (idx_t)p * plane, zc.in, res);
                        });
 }
 
@@ -542,7 +543,7 @@ __global__ void __launch_bounds__(256) kb_eno2_extend_list(double* __restrict__ 
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) kb_list_gather_vel(double* __restrict__ lv, IList il, const T* __restrict__ vx,
+__global__ void __launch_bounds__(256) kb_list_gather_vel(double* __restrict__ lv, IList il, T* __restrict__ vx,
                                                           const T* __restrict__ vy, const T* __restrict__ vz)
 {
     const int cnt = *il.count;
@@ -551,6 +552,7 @@ __global__ void __launch_bounds__(256) kb_list_gather_vel(double* __restrict__ l
         lv[j] = (double)vx[ind];
         lv[(size_t)il.cap + j] = (double)vy[ind];
         lv[(size_t)2 * il.cap + j] = (double)vz[ind];
+        vx[ind] = T(__builtin_nan(""));  // the bulk of the extension stages skips the listed points
     }
 }
 
@@ -909,7 +911,7 @@ hipError_t launch_batch_list_interp(const BatchView& B, double* lcp, const ListV
 }
 
 template <typename T>
-hipError_t launch_batch_list_gather_vel(double* lvel, const ListView& lv, const T* vx, const T* vy, const T* vz,
+hipError_t launch_batch_list_gather_vel(double* lvel, const ListView& lv, T* vx, const T* vy, const T* vz,
                                         hipStream_t s)
 {
     kb_list_gather_vel<T><<<1024, 256, 0, s>>>(lvel, ilist(lv), vx, vy, vz);
@@ -972,7 +974,7 @@ hipError_t launch_batch_ellipsoid(const BatchView& B, T* F, const double* axes, 
                                                            const ListView&, double, hipStream_t, int);              \
     template hipError_t launch_batch_list_interp<T>(const BatchView&, double*, const ListView&, const T*,           \
                                                     hipStream_t, int);                                              \
-    template hipError_t launch_batch_list_gather_vel<T>(double*, const ListView&, const T*, const T*, const T*,     \
+    template hipError_t launch_batch_list_gather_vel<T>(double*, const ListView&, T*, const T*, const T*,           \
                                                         hipStream_t);                                               \
     template hipError_t launch_batch_normal_velocity<T>(const BatchView&, T*, const T*, const T*, const T*,         \
                                                         const T*, hipStream_t);                                     \

@@ -456,7 +456,7 @@ __global__ void __launch_bounds__(256) k_list_interp_cubic(double* __restrict__ 
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_list_gather_vel(double* __restrict__ lv, IList il, const T* __restrict__ vx,
+__global__ void __launch_bounds__(256) k_list_gather_vel(double* __restrict__ lv, IList il, T* __restrict__ vx,
                                                          const T* __restrict__ vy, const T* __restrict__ vz)
 {
     const int cnt = *il.count;
@@ -465,6 +465,7 @@ __global__ void __launch_bounds__(256) k_list_gather_vel(double* __restrict__ lv
         lv[j] = (double)vx[ind];
         lv[(size_t)il.cap + j] = (double)vy[ind];
         lv[(size_t)2 * il.cap + j] = (double)vz[ind];
+        vx[ind] = T(__builtin_nan(""));  // the bulk of the extension stages skips the listed points
     }
 }
 
@@ -745,10 +746,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(stage_
     int r0, c0, p0, p1;
     zmarch_share<R>(b, depth, blk - niface, (int)gridDim.x - niface, r0, c0, p0, p1);
     const idx_t plane = (idx_t)b.rows * b.cols;
-    zmarch<R, T, true, true>(lds[0], lds[1], b, r0, c0, p0, p1, cs, cn, vx, vy, vz, flags, stage,
+    // the listed points carry a NaN x-velocity (k_list_gather_vel), so the
+    // bulk needs no flags: it skips them and the lanes outside the box
+    zmarch<R, T, true, false>(lds[0], lds[1], b, r0, c0, p0, p1, cs, cn, vx, vy, vz, nullptr, stage,
                        [&](const T* l, const ZCol<R>& zc, const T w[7], const ZStep<R, T, true>& st, int p) {
                            const T res = upwind_value<R>(l, zc.base, w, st.vel, st.fn, (T)inv.dt_bulk, inv, stage);
-                           if (!(st.flags(zc) & FLAG_SUBCELL)) stp(out + (idx_t)p * plane, zc.in, res);
+                           if (zc.valid && st.vel[0] == st.vel[0]) stp(out + (idx_t)p * plane, zc.in, res);
                        });
 }
 
@@ -1418,7 +1421,7 @@ hipError_t launch_list_interp_cubic(double* lcp, const ListView& lv, const doubl
 }
 
 template <typename T>
-hipError_t launch_list_gather_vel(double* lvel, const ListView& lv, const T* vx, const T* vy, const T* vz,
+hipError_t launch_list_gather_vel(double* lvel, const ListView& lv, T* vx, const T* vy, const T* vz,
                                   const Geom& g, hipStream_t s)
 {
     const IList il{lv.idx, lv.count, lv.dist, lv.dt, lv.cap};
@@ -1624,7 +1627,7 @@ hipError_t launch_copy(T* dst, const T* src, int64_t n, hipStream_t s)
     template hipError_t launch_reinit_stage<T>(T*, const T*, const T*, const uint8_t*, const ListView&, const Geom&, \
                                                int, int*, int*, int, hipStream_t);                                  \
     template hipError_t launch_list_interp<T>(double*, const ListView&, const T*, const Geom&, hipStream_t);        \
-    template hipError_t launch_list_gather_vel<T>(double*, const ListView&, const T*, const T*, const T*,          \
+    template hipError_t launch_list_gather_vel<T>(double*, const ListView&, T*, const T*, const T*,                \
                                                   const Geom&, hipStream_t);                                        \
     template hipError_t launch_upwind_normal<T>(T*, T*, T*, const T*, const Dist6c&, const Geom&, hipStream_t);     \
     template hipError_t launch_extension_velocity<T>(T*, T*, T*, const T*, double, const Geom&, hipStream_t);       \

@@ -107,7 +107,7 @@ template <typename T>
 hipError_t launch_list_interp(double* lcp, const ListView& lv, const T* C, const Geom& g, hipStream_t s);
 // velocity at the listed points, list order (3 rows of cap)
 template <typename T>
-hipError_t launch_list_gather_vel(double* lvel, const ListView& lv, const T* vx, const T* vy, const T* vz,
+hipError_t launch_list_gather_vel(double* lvel, const ListView& lv, T* vx, const T* vy, const T* vz,
                                   const Geom& g, hipStream_t s);
 hipError_t launch_eno2_stage(double* out, const double* fs, const double* fn, const uint8_t* flags,
                              const Dist6c& d, const Geom& g, double cfl, int heun2, hipStream_t s);
@@ -325,7 +325,7 @@ template <typename T>
 hipError_t launch_batch_list_interp(const BatchView& B, double* lcp, const ListView& lv, const T* C, hipStream_t s,
                                     int scheme = 0);
 template <typename T>
-hipError_t launch_batch_list_gather_vel(double* lvel, const ListView& lv, const T* vx, const T* vy, const T* vz,
+hipError_t launch_batch_list_gather_vel(double* lvel, const ListView& lv, T* vx, const T* vy, const T* vz,
                                         hipStream_t s);
 template <typename T>
 hipError_t launch_batch_normal_velocity(const BatchView& B, T* out, const T* F, const T* vx, const T* vy,
