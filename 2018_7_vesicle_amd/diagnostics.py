@@ -113,7 +113,8 @@ def _displacement_stats(grid, Fa, Fb):
     for name, normal in (("along_line", False), ("normal", True)):
         d, unmatched = crossing_displacements(grid, Fa, Fb, normal=normal)
         d = d / grid.Dx
-        out[name] = {"max_dx": float(d.max()), "p99_dx": float(np.percentile(d, 99)), "mean_dx": float(d.mean())}
+        out[name] = {"max_dx": float(d.max()), "p99_dx": float(np.percentile(d, 99)), "mean_dx": float(d.mean()),
+                     "count_above_1e-4_dx": int((d > 1e-4).sum())}
     out["crossings"] = int(len(d))
     out["unmatched"] = int(unmatched)
     return out

@@ -94,14 +94,15 @@ def parse():
 def probe_stages(args):
     """A short run for hardware counters of the three fused stage kernels: the
     config-3 pipeline at 256^3 (FP32 bulk) and the config-2 pipeline at 128^3
-    (FP64) are both set up first (initial reinitialization, launch-shape
-    autotuning), then each runs one extension iteration, one reinitialization
-    iteration and one advection step (3 launches of each stage kernel), with a
-    synchronization between the phases; so the last three launches of every
-    stage kernel are the probe's own (k_*_stage<32, float> at 256^3 with 16
-    pages per block, <32, double> at 128^3 with 8).  Last, a 256 MiB float64 copy whose source was
-    pushed out of the caches by a 2 GiB fill: FETCH_SIZE and WRITE_SIZE of a
-    known number of bytes, to calibrate them."""
+    (FP64) are both set up first (initial reinitialization, a fixed launch
+    shape: k_*_stage<32, float> at 256^3 with 16 pages per block, <32, double>
+    at 128^3 with 8), then each runs one extension iteration, one
+    reinitialization iteration and one advection step (3 launches of each
+    stage kernel), each twice, with a synchronization between the phases; so
+    the last three launches of every stage kernel are the probe's own, on warm
+    buffers.  Last, a 256 MiB float64 copy whose source was pushed out of the
+    caches by a 2 GiB fill: FETCH_SIZE and WRITE_SIZE of a known number of
+    bytes, to calibrate them."""
     import importlib
 
     import torch
@@ -122,12 +123,18 @@ def probe_stages(args):
     torch.cuda.synchronize()
     for key, p in lines:
         m = p.map
-        m.extend_(p.Vn, 1)
-        torch.cuda.synchronize()
-        m.reinitialize_(m.F, 1)
-        torch.cuda.synchronize()
-        m.advect_(m.F, *p.u, p.dt, 1)
-        torch.cuda.synchronize()
+        # each phase twice: the first launches after set-up run on cold
+        # buffers (the first extension stage took twice its time), so the
+        # profiled launches are those of the second pass
+        for _ in range(2):
+            m.extend_(p.Vn, 1)
+            torch.cuda.synchronize()
+        for _ in range(2):
+            m.reinitialize_(m.F, 1)
+            torch.cuda.synchronize()
+        for _ in range(2):
+            m.advect_(m.F, *p.u, p.dt, 1)
+            torch.cuda.synchronize()
         tr, zd = m.launch()
         out[key] = {"tile_rows": tr, "zmarch_depth": zd, "points": p.grid.NumElt}
     del lines, p, m
