@@ -90,6 +90,7 @@ SIGNATURES = {
     "vls_batch_set_zmarch_depth": [P, I],
     "vls_batch_get_zmarch_depth": [P, P],
     "vls_sdf3_set_tile_rows": [P, I],
+    "vls_sdf3_set_reinit_fp64_tail": [P, I],
     "vls_sdf3_set_zmarch_depth": [P, I],
     "vls_sdf3_get_launch": [P, P, P],
     "vls_batch_autotune": [P, P, I, P, P, P],

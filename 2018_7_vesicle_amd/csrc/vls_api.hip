@@ -231,6 +231,7 @@ struct vls_sdf3 {
     double* result = nullptr;
     GraphCache graphs;
     bool dist_ready = false;  // a set_distance has filled flags and the list
+    int fp64_tail = 0;        // FP64 iterations that end an FP32 reinitialization call
     bool dense_ready = false;  // d and min_dist hold the last set_distance's values (else built on demand)
     double* adj = nullptr;  // adjoint/tangent workspace, grown on demand
     size_t adj_len = 0;
@@ -349,6 +350,21 @@ int set_distance_into(vls_sdf3* h, int order, const DSet& ds, double* min_dist, 
     return VLS_OK;
 }
 
+// The last `tail` iterations of an FP32 reinitialization call in FP64: F is
+// widened into the fourth work field (an FP64 field of the handle), iterated
+// with the call's mask, flags and interface list (so the call is the same
+// algorithm as one call of `iterations` iterations), and rounded back.
+inline int reinit_tail_fp64(vls_sdf3* h, float* F, int tail, hipStream_t s)
+{
+    double* Fd = h->t[3];
+    VLS_HIP(launch_widen(Fd, F, h->g.n(), s));
+    const int rc = reinit_loop<double>(h, Fd, tail, main_set(h), s);
+    if (rc != VLS_OK) return rc;
+    VLS_HIP(launch_narrow(F, Fd, h->g.n(), s));
+    return VLS_OK;
+}
+inline int reinit_tail_fp64(vls_sdf3*, double*, int, hipStream_t) { return VLS_OK; }
+
 // ---- wrapper-level operations, for fields of T ------------------------------
 template <typename T>
 int sdf3_weno5_rk3_reinit(vls_sdf3* h, T* F, int iterations, hipStream_t s)
@@ -356,9 +372,13 @@ int sdf3_weno5_rk3_reinit(vls_sdf3* h, T* F, int iterations, hipStream_t s)
     VLS_REQUIRE(h && F, "null argument");
     VLS_REQUIRE(iterations >= 0, "negative iteration count");
     VLS_REQUIRE(h->dist_ready, "set_distance must run before the reinitialization (it reads its distances)");
-    return with_graph(h->graphs, make_key(1, sizeof(T), F, iterations), s, [&](hipStream_t cs) -> int {
+    // an FP32 call may end with FP64 iterations (vls_sdf3_set_reinit_fp64_tail)
+    const int tail = sizeof(T) == sizeof(float) ? std::min(h->fp64_tail, iterations) : 0;
+    return with_graph(h->graphs, make_key(1, sizeof(T), F, iterations, tail), s, [&](hipStream_t cs) -> int {
         VLS_HIP(launch_set_mask_bit<T>(h->flags, F, h->g, cs));
-        return reinit_loop<T>(h, F, iterations, main_set(h), cs);
+        int rc = reinit_loop<T>(h, F, iterations - tail, main_set(h), cs);
+        if (rc == VLS_OK && tail > 0) rc = reinit_tail_fp64(h, F, tail, cs);
+        return rc;
     });
 }
 
@@ -1073,6 +1093,14 @@ int vls_sdf3_weno5_rk3_reinit(vls_sdf3_t h, double* F, int iterations, vls_strea
 int vls_sdf3_weno5_rk3_reinit_f32(vls_sdf3_t h, float* F, int iterations, vls_stream_t stream)
 {
     return sdf3_weno5_rk3_reinit<float>(h, F, iterations, S(stream));
+}
+
+int vls_sdf3_set_reinit_fp64_tail(vls_sdf3_t h, int iterations)
+{
+    VLS_REQUIRE(h, "null handle");
+    VLS_REQUIRE(iterations >= 0, "negative iteration count");
+    h->fp64_tail = iterations;
+    return VLS_OK;
 }
 
 int vls_weno5_rk3_reinit_static(double* F, int iterations, int rows, int cols, int pges, double dx, double dy,

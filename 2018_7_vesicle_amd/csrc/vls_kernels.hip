@@ -1199,6 +1199,14 @@ __global__ void k_copy(T* __restrict__ dst, const T* __restrict__ src, idx_t n)
     if (i < n) dst[i] = src[i];
 }
 
+// dst = src converted (FP32 -> FP64 exactly, FP64 -> FP32 rounded to nearest)
+template <typename D, typename S>
+__global__ void k_convert(D* __restrict__ dst, const S* __restrict__ src, idx_t n)
+{
+    const idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (D)src[i];
+}
+
 template __global__ void k_reduce_final<2>(double*, const double*, int, double);
 
 }  // namespace vls
@@ -1615,6 +1623,18 @@ template <typename T>
 hipError_t launch_copy(T* dst, const T* src, int64_t n, hipStream_t s)
 {
     k_copy<T><<<flat_grid(n), dim3(256), 0, s>>>(dst, src, (idx_t)n);
+    return hipGetLastError();
+}
+
+hipError_t launch_widen(double* dst, const float* src, int64_t n, hipStream_t s)
+{
+    k_convert<double, float><<<flat_grid(n), dim3(256), 0, s>>>(dst, src, (idx_t)n);
+    return hipGetLastError();
+}
+
+hipError_t launch_narrow(float* dst, const double* src, int64_t n, hipStream_t s)
+{
+    k_convert<float, double><<<flat_grid(n), dim3(256), 0, s>>>(dst, src, (idx_t)n);
     return hipGetLastError();
 }
 

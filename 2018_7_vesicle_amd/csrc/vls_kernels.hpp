@@ -215,6 +215,8 @@ hipError_t launch_max_abs3(double* result, double* partial, const T* a, const T*
                            hipStream_t s);
 template <typename T>
 hipError_t launch_copy(T* dst, const T* src, int64_t n, hipStream_t s);
+hipError_t launch_widen(double* dst, const float* src, int64_t n, hipStream_t s);
+hipError_t launch_narrow(float* dst, const double* src, int64_t n, hipStream_t s);
 
 // ---- batches of grids (vls_batch.hip) -----------------------------------------
 // One grid of a batch: its box, its first element in the packed fields (every

@@ -53,6 +53,7 @@ class PipelineConfig:
     initial_reinit: int = 100
     eps_factor: float = 1.5
     precision: str = "fp64"  # or "fp32" (FP32 bulk)
+    reinit_fp64_tail: int = 0  # fp32: the last iterations of each reinitialization call run in FP64
     autotune: bool = True  # pick the stage kernels' launch shape for the grid at set-up (results unchanged)
 
 
@@ -128,6 +129,7 @@ class Pipeline:
             self.map = SDF3(self.grid, F)
             self.u = [c.float() for c in self.u]
             self.psi = self.psi.float()
+            self.map.setReinitFP64Tail(cfg.reinit_fp64_tail)
         elif cfg.precision != "fp64":
             raise ValueError(f"precision must be fp64 or fp32, got {cfg.precision}")
         if cfg.autotune:

@@ -188,6 +188,11 @@ class SDF3:
         """pages per block of the z-marching bulk stage kernels (0: chosen by the box size)"""
         call("vls_sdf3_set_zmarch_depth", self._h, int(depth))
 
+    def setReinitFP64Tail(self, iterations):
+        """FP32 objects: the last `iterations` iterations of each reinitialize_
+        call run in FP64 with the call's mask and distances (0: none)"""
+        call("vls_sdf3_set_reinit_fp64_tail", self._h, int(iterations))
+
     def launch(self):
         """(tile rows, z-march depth) the bulk stage kernels launch with"""
         tr, zd = ctypes.c_int(), ctypes.c_int()

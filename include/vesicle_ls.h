@@ -379,6 +379,15 @@ int vls_sdf3_normal_velocity(vls_sdf3_t h, const double *F, const double *vx, co
  * combination in FP32, storing float results. */
 int vls_sdf3_set_distance_f32(vls_sdf3_t h, const float *F, vls_stream_t stream);
 int vls_sdf3_weno5_rk3_reinit_f32(vls_sdf3_t h, float *F, int iterations, vls_stream_t stream);
+/* The last `iterations` iterations of every later vls_sdf3_weno5_rk3_reinit_f32
+ * call on this handle run in FP64 (no counterpart in the reference; default
+ * 0): F is widened into an FP64 work field of the handle, iterated with the
+ * call's mask, flags and interface list, and rounded back to float, so the
+ * call stays one call of its iteration count.  Over a 100-iteration call the
+ * FP32 iterations' rounding adds up where the distance function converges
+ * slowly; finishing in FP64 takes it out.  A count above a call's iteration
+ * count makes the whole call FP64.  Negative counts: VLS_ERR_INVALID. */
+int vls_sdf3_set_reinit_fp64_tail(vls_sdf3_t h, int iterations);
 int vls_sdf3_weno5_rk3_extend_f32(vls_sdf3_t h, const float *F, float *C, int iterations, vls_stream_t stream);
 int vls_sdf3_surface_redistance_f32(vls_sdf3_t h, const float *F, float *A, int iter1, int iter2,
                                     vls_stream_t stream);

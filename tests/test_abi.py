@@ -88,6 +88,12 @@ def test_missing_library_fails_loudly(monkeypatch):
         lib_mod.lib()
 
 
+def test_reinit_fp64_tail_setter_rejects_without_device_work():
+    lib = vls.lib()
+    assert lib.vls_sdf3_set_reinit_fp64_tail(ctypes.c_void_p(), 5) == -1
+    assert b"null handle" in lib.vls_last_error()
+
+
 def test_static_reinit_rejects_a_box_beyond_its_memory_without_device_work():
     lib = vls.lib()
     p = ctypes.c_void_p(16)

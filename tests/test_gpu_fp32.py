@@ -114,6 +114,40 @@ def test_reinit_f32_vs_oracle():
     assert np.mean((out < 0) != (ref < 0)) < 1e-4
 
 
+def test_reinit_f32_fp64_tail_of_the_whole_call_is_the_fp64_result():
+    """An FP64 tail as long as the call (vls_sdf3_set_reinit_fp64_tail): the
+    FP32 call returns the FP64 oracle's result rounded to float, bit for bit."""
+    g, G = _grid(40)
+    F0 = _f32(_wobbly_sphere(g))
+    d6, md = po.set_distance(F0, *g.spacing)
+    ref = po.weno5_rk3_reinit(F0, 20, d6, md, *g.spacing)
+    m = vls.SDF3(G, _cuda(F0, torch.float32))
+    m.setReinitFP64Tail(25)
+    m.setDistance()
+    out = m.WENO5RK3Reinitialization(m.F, 20).cpu().numpy()
+    assert np.array_equal(out, ref.astype(np.float32))
+
+
+def test_reinit_f32_fp64_tail_continues_the_fp32_iterations():
+    """A 30-iteration FP32 call with a 10-iteration FP64 tail is the 20 FP32
+    iterations followed by 10 FP64 iterations with the call's mask and
+    distances (those of the input): against the oracle continuing the FP32
+    result, bit for bit after rounding to float."""
+    g, G = _grid(40)
+    F0 = _f32(_wobbly_sphere(g))
+    d6, md = po.set_distance(F0, *g.spacing)
+    m = vls.SDF3(G, _cuda(F0, torch.float32))
+    m.setDistance()
+    a = m.WENO5RK3Reinitialization(m.F.clone(), 20).double().cpu().numpy()
+    # the oracle takes its mask from its input: the continuation is exact when
+    # the FP32 iterations kept every sign
+    assert np.array_equal(a < 0, F0 < 0)
+    ref = po.weno5_rk3_reinit(a, 10, d6, md, *g.spacing)
+    m.setReinitFP64Tail(10)
+    out = m.WENO5RK3Reinitialization(m.F, 30).cpu().numpy()
+    assert np.array_equal(out, ref.astype(np.float32))
+
+
 def test_extend_f32_vs_oracle():
     """40 WENO5-RK3 extension iterations off a sphere: within 1e-5 of max|C|."""
     g, G = _grid(40)
