@@ -222,9 +222,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(stage_
                             const T res = reinit_value<R>(l, zc.base, w, st.fn, st.flags(zc), invs, stage, cross,
                                                           zf, p == it.p0);
                             const bool bulk = !(st.flags(zc) & FLAG_SUBCELL);
-                            if (bulk && !cross) stp(o + (idx_t)p * plane, zc.in, res);
-                            wave_append(bulk && cross, (int)(off + (idx_t)p * plane + zc.in), over,
-                                        &over_count[seq & 1]);
+                            if constexpr (defers_crossings<T>()) {
+                                if (bulk && !cross) stp(o + (idx_t)p * plane, zc.in, res);
+                                wave_append(bulk && cross, (int)(off + (idx_t)p * plane + zc.in), over,
+                                            &over_count[seq & 1]);
+                            } else if (bulk) {
+                                stp(o + (idx_t)p * plane, zc.in, res);
+                            }
                         });
 }
 
@@ -822,8 +826,9 @@ hipError_t launch_batch_reinit_stage(const BatchView& B, T* out, const T* fs, co
     }
     if (scheme != IFACE_WENO) return hipErrorInvalidValue;
     TR_LAUNCH(kb_reinit_stage, T, B.nitems + ni, out, fs, fn, flags, ilist(lv), B, stage, over, over_count, seq, ni);
-    kb_reinit_deferred<T><<<nd, 256, 0, s>>>(out, fs, fn, flags, ilist(lv), B, stage, (const int*)over,
-                                               (const int*)over_count, seq);
+    if constexpr (defers_crossings<T>())
+        kb_reinit_deferred<T><<<nd, 256, 0, s>>>(out, fs, fn, flags, ilist(lv), B, stage, (const int*)over,
+                                                   (const int*)over_count, seq);
     return hipGetLastError();
 }
 

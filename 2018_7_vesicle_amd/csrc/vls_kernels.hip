@@ -372,8 +372,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(stage_
                             const T res = reinit_value<R>(l, zc.base, w, st.fn, st.flags(zc), invs, stage, cross,
                                                           zf, p == p0);
                             const bool bulk = !(st.flags(zc) & FLAG_SUBCELL);
-                            if (bulk && !cross) stp(out + (idx_t)p * plane, zc.in, res);
-                            wave_append(bulk && cross, (int)((idx_t)p * plane + zc.in), over, &over_count[seq & 1]);
+                            if constexpr (defers_crossings<T>()) {
+                                if (bulk && !cross) stp(out + (idx_t)p * plane, zc.in, res);
+                                wave_append(bulk && cross, (int)((idx_t)p * plane + zc.in), over,
+                                            &over_count[seq & 1]);
+                            } else if (bulk) {
+                                stp(out + (idx_t)p * plane, zc.in, res);
+                            }
                         });
 }
 
@@ -1380,7 +1385,7 @@ hipError_t launch_eno2_re_step(double* step, const double* lsf, const uint8_t* m
     VLS_POINT_LAUNCH(k_eno2_re_step, step, lsf, mask, deltat, d, box_of(g));
 }
 
-// bulk, then the interface list and the deferred bulk points, in stream order
+// bulk, then the interface list and the deferred bulk points (FP64), in stream order
 template <typename T, int S>
 static hipError_t reinit_stage_scheme(T* out, const T* fs, const T* fn, const uint8_t* flags, const ListView& lv,
                                       const Geom& g, int stage, int* over, int* over_count, int seq, hipStream_t s)
@@ -1394,8 +1399,10 @@ static hipError_t reinit_stage_scheme(T* out, const T* fs, const T* fn, const ui
     else
         k_reinit_stage<64, T, S><<<grid, dim3(256), 0, s>>>(out, fs, fn, flags, il, box_of(g), inv_of(g), stage,
                                                             over, over_count, seq, zd, ni);
-    k_reinit_deferred<T, S><<<dim3(deferred_blocks(g)), dim3(256), 0, s>>>(out, fs, fn, flags, il, box_of(g),
-                                                                            inv_of(g), stage, over, over_count, seq);
+    if constexpr (defers_crossings<T>())
+        k_reinit_deferred<T, S><<<dim3(deferred_blocks(g)), dim3(256), 0, s>>>(out, fs, fn, flags, il, box_of(g),
+                                                                                inv_of(g), stage, over, over_count,
+                                                                                seq);
     return hipGetLastError();
 }
 

@@ -705,6 +705,18 @@ __host__ __device__ inline int zmarch_blocks(int rows, int cols, int pges, int d
     return tiles_per_col<R>(rows) * tiles_per_row<R>(cols) * ((pges + depth - 1) / depth);
 }
 
+// Whether the reinitialization bulk defers a point next to a sign change of
+// the stage field to the interface path.  The reference evaluates such a
+// point with its nonuniform WENO at full spacings (re_step.cu:248-269: the
+// point has no subcell distance), which is the uniform WENO5 up to rounding
+// (the oracle tests hold the two within 1e-14 relative); FP64 defers so the
+// results stay the reference's bit for bit, FP32 evaluates them in its bulk.
+template <typename T>
+constexpr bool defers_crossings()
+{
+    return sizeof(T) == sizeof(double);
+}
+
 // WENO5-RK3 reinitialization stage at a bulk point: x- and y-taps from the
 // staged tile (lds, the point at index base), z-taps z[0..6] (z[3] the point),
 // the start value fn and flags fl.  Returns the new value; cross is set for a

@@ -69,6 +69,39 @@ def test_nonuniform_weno_on_uniform_stencil_is_weno5():
     assert abs(db - po.weno_onesided(*v[:5])) < 1e-14
 
 
+def test_nonuniform_weno_at_full_spacing_is_weno5_across_kinks():
+    """The reinitialization's interface path at a point with no subcell
+    distance (re_step.cu:248-269 with xpr = xpl = dx) is the uniform WENO5 up
+    to rounding, also where the stencil spans a kink of the distance function
+    or a thin sheet: the FP32 bulk evaluates such points with the uniform form
+    instead of deferring them."""
+    rng = np.random.default_rng(11)
+    h = 2.4 / 255
+    x = (np.arange(7) - 3) * h
+    worst = 0.0
+    checked = 0
+    for t in range(3000):
+        a = rng.uniform(-1, 1) * h
+        kind = t % 4
+        if kind == 0:
+            p = (x - a) * rng.uniform(0.2, 2)
+        elif kind == 1:
+            p = np.abs(x - a) - rng.uniform(0.05, 1.5) * h
+        elif kind == 2:
+            p = rng.uniform(0.05, 1.5) * h - np.abs(x - a)
+        else:
+            p = (x - a) + rng.uniform(-30, 30) * (x - a) ** 2 + rng.uniform(-300, 300) * (x - a) ** 3
+        p = p + rng.normal(0, 1e-3 * h, size=7)
+        if not (p[2] * p[3] < 0 or p[3] * p[4] < 0):
+            continue
+        checked += 1
+        fu, bu = po.weno_pair(p, h)
+        fn, bn = po.weno_derivative_boundary(p, h, h, h, rule=0)
+        worst = max(worst, abs(fn - fu) / max(abs(fu), 1.0), abs(bn - bu) / max(abs(bu), 1.0))
+    assert checked > 1000
+    assert worst < 1e-14, worst
+
+
 @pytest.mark.parametrize("trial", range(5))
 def test_weno5_pair_exact_for_cubics(trial):
     a, b, c, d = RNG.normal(size=4)
