@@ -335,10 +335,11 @@ __global__ void __launch_bounds__(256) k_re_step(double* __restrict__ step, cons
 // residual with subcell distances), the others z-march over every point
 // without subcell distances.  The list blocks are dispatched first, so their
 // long dependent chains (the nonuniform WENO divides ~30 times per face) run
-// beside the bulk instead of after it.  A bulk point whose stage field
-// changes sign beside it needs the nonuniform stencil: it is appended to
-// `over`, counted in over_count[seq & 1], for launch 2; the other slot is
-// cleared here for the next stage.
+// beside the bulk instead of after it.  In FP64 a bulk point whose stage
+// field changes sign beside it takes the reference's nonuniform stencil: it
+// is appended to `over`, counted in over_count[seq & 1], for launch 2; the
+// other slot is cleared here for the next stage.  FP32 evaluates such a
+// point in the bulk (defers_crossings) and has no launch 2.
 template <int R, typename T, int S = IFACE_WENO>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(stage_waves<T>()))) k_reinit_stage(T* __restrict__ out, const T* __restrict__ fs,
                                                       const T* __restrict__ fn, const uint8_t* __restrict__ flags,
