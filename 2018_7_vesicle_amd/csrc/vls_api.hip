@@ -289,9 +289,9 @@ int extend_loop(vls_sdf3* h, T* C, int iterations, const DSet& ds, hipStream_t s
     const T *vx = vel<T>(h, 0), *vy = vel<T>(h, 1), *vz = vel<T>(h, 2);
     VLS_HIP(launch_list_interp(h->lcp, ds.lv, (const T*)C, g, s));
     for (int it = 0; it < iterations; ++it) {
-        VLS_HIP(launch_extend_stage<T>(t0, C, nullptr, vx, vy, vz, ds.flags, ds.lv, h->lcp, h->lvel, g, 0, s));
-        VLS_HIP(launch_extend_stage<T>(t1, t0, C, vx, vy, vz, ds.flags, ds.lv, h->lcp, h->lvel, g, 1, s));
-        VLS_HIP(launch_extend_stage<T>(C, t1, C, vx, vy, vz, ds.flags, ds.lv, h->lcp, h->lvel, g, 2, s));
+        VLS_HIP(launch_extend_stage<T>(t0, C, nullptr, vx, vy, vz, ds.lv, h->lcp, h->lvel, g, 0, s));
+        VLS_HIP(launch_extend_stage<T>(t1, t0, C, vx, vy, vz, ds.lv, h->lcp, h->lvel, g, 1, s));
+        VLS_HIP(launch_extend_stage<T>(C, t1, C, vx, vy, vz, ds.lv, h->lcp, h->lvel, g, 2, s));
     }
     return VLS_OK;
 }
@@ -1224,10 +1224,10 @@ static int variant_extend(vls_sdf3* h, const double* F, double* C, int iteration
             VLS_HIP(launch_list_gather_vel<double>(h->lvel, as.lv, vx, vy, vz, g, cs));
             VLS_HIP(launch_list_interp_cubic(h->lcp, as.lv, C, g, cs));
             for (int it = 0; it < iterations; ++it) {
-                VLS_HIP(launch_cubic_extend_stage(t0, C, nullptr, vx, vy, vz, as.flags, as.lv, h->lcp, h->lvel, g, 0,
+                VLS_HIP(launch_cubic_extend_stage(t0, C, nullptr, vx, vy, vz, as.lv, h->lcp, h->lvel, g, 0,
                                                   cs));
-                VLS_HIP(launch_cubic_extend_stage(t1, t0, C, vx, vy, vz, as.flags, as.lv, h->lcp, h->lvel, g, 1, cs));
-                VLS_HIP(launch_cubic_extend_stage(C, t1, C, vx, vy, vz, as.flags, as.lv, h->lcp, h->lvel, g, 2, cs));
+                VLS_HIP(launch_cubic_extend_stage(t1, t0, C, vx, vy, vz, as.lv, h->lcp, h->lvel, g, 1, cs));
+                VLS_HIP(launch_cubic_extend_stage(C, t1, C, vx, vy, vz, as.lv, h->lcp, h->lvel, g, 2, cs));
             }
             return VLS_OK;
         }
@@ -1507,9 +1507,9 @@ int batch_extend(vls_batch* b, const T* F, T* C, int iterations, hipStream_t s)
     VLS_HIP(launch_batch_list_gather_vel<T>(b->lvel, b->lv, vx, vy, vz, s));
     VLS_HIP(launch_batch_list_interp<T>(B, b->lcp, b->lv, C, s));
     for (int it = 0; it < iterations; ++it) {
-        VLS_HIP(launch_batch_extend_stage<T>(B, t0, C, nullptr, vx, vy, vz, b->flags, b->lv, b->lcp, b->lvel, 0, s));
-        VLS_HIP(launch_batch_extend_stage<T>(B, t1, t0, C, vx, vy, vz, b->flags, b->lv, b->lcp, b->lvel, 1, s));
-        VLS_HIP(launch_batch_extend_stage<T>(B, C, t1, C, vx, vy, vz, b->flags, b->lv, b->lcp, b->lvel, 2, s));
+        VLS_HIP(launch_batch_extend_stage<T>(B, t0, C, nullptr, vx, vy, vz, b->lv, b->lcp, b->lvel, 0, s));
+        VLS_HIP(launch_batch_extend_stage<T>(B, t1, t0, C, vx, vy, vz, b->lv, b->lcp, b->lvel, 1, s));
+        VLS_HIP(launch_batch_extend_stage<T>(B, C, t1, C, vx, vy, vz, b->lv, b->lcp, b->lvel, 2, s));
     }
     return VLS_OK;
 }
@@ -1595,11 +1595,11 @@ int batch_variant_extend(vls_batch* b, const double* F, double* C, int iteration
     if (weno) {
         VLS_HIP(launch_batch_list_gather_vel<double>(b->lvel, b->lv, vx, vy, vz, s));
         for (int it = 0; it < iterations; ++it) {
-            VLS_HIP(launch_batch_extend_stage<double>(B, t0, C, nullptr, vx, vy, vz, b->flags, b->lv, b->lcp,
+            VLS_HIP(launch_batch_extend_stage<double>(B, t0, C, nullptr, vx, vy, vz, b->lv, b->lcp,
                                                       b->lvel, 0, s, IFACE_CUBIC));
-            VLS_HIP(launch_batch_extend_stage<double>(B, t1, t0, C, vx, vy, vz, b->flags, b->lv, b->lcp, b->lvel, 1,
+            VLS_HIP(launch_batch_extend_stage<double>(B, t1, t0, C, vx, vy, vz, b->lv, b->lcp, b->lvel, 1,
                                                       s, IFACE_CUBIC));
-            VLS_HIP(launch_batch_extend_stage<double>(B, C, t1, C, vx, vy, vz, b->flags, b->lv, b->lcp, b->lvel, 2,
+            VLS_HIP(launch_batch_extend_stage<double>(B, C, t1, C, vx, vy, vz, b->lv, b->lcp, b->lvel, 2,
                                                       s, IFACE_CUBIC));
         }
         return VLS_OK;

@@ -262,9 +262,9 @@ template <int R, typename T, int S = IFACE_WENO>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(stage_waves<T>()))) kb_extend_stage(T* __restrict__ out, const T* __restrict__ cs,
                                                        const T* __restrict__ cn, const T* __restrict__ vx,
                                                        const T* __restrict__ vy, const T* __restrict__ vz,
-                                                       const uint8_t* __restrict__ flags, IList il,
-                                                       const double* __restrict__ lcp, const double* __restrict__ lv,
-                                                       BatchView B, int stage, int niface)
+                                                       IList il, const double* __restrict__ lcp,
+                                                       const double* __restrict__ lv, BatchView B, int stage,
+                                                       int niface)
 {
     __shared__ T lds[2][Shape<R>::N];
     const int blk = (int)blockIdx.x;
@@ -834,23 +834,23 @@ hipError_t launch_batch_reinit_stage(const BatchView& B, T* out, const T* fs, co
 
 template <typename T>
 hipError_t launch_batch_extend_stage(const BatchView& B, T* out, const T* cs, const T* cn, const T* vx,
-                                     const T* vy, const T* vz, const uint8_t* flags, const ListView& lv,
-                                     const double* lcp, const double* lvel, int stage, hipStream_t s, int scheme)
+                                     const T* vy, const T* vz, const ListView& lv, const double* lcp,
+                                     const double* lvel, int stage, hipStream_t s, int scheme)
 {
     const int ni = fused_list_blocks(B);
     if constexpr (std::is_same<T, double>::value) {
         if (scheme == IFACE_CUBIC) {
             if (B.tr == 32)
-                kb_extend_stage<32, T, IFACE_CUBIC><<<B.nitems + ni, 256, 0, s>>>(out, cs, cn, vx, vy, vz, flags,
+                kb_extend_stage<32, T, IFACE_CUBIC><<<B.nitems + ni, 256, 0, s>>>(out, cs, cn, vx, vy, vz,
                                                                                   ilist(lv), lcp, lvel, B, stage, ni);
             else
-                kb_extend_stage<64, T, IFACE_CUBIC><<<B.nitems + ni, 256, 0, s>>>(out, cs, cn, vx, vy, vz, flags,
+                kb_extend_stage<64, T, IFACE_CUBIC><<<B.nitems + ni, 256, 0, s>>>(out, cs, cn, vx, vy, vz,
                                                                                   ilist(lv), lcp, lvel, B, stage, ni);
             return hipGetLastError();
         }
     }
     if (scheme != IFACE_WENO) return hipErrorInvalidValue;
-    TR_LAUNCH(kb_extend_stage, T, B.nitems + ni, out, cs, cn, vx, vy, vz, flags, ilist(lv), lcp, lvel, B, stage, ni);
+    TR_LAUNCH(kb_extend_stage, T, B.nitems + ni, out, cs, cn, vx, vy, vz, ilist(lv), lcp, lvel, B, stage, ni);
     return hipGetLastError();
 }
 
@@ -971,8 +971,8 @@ hipError_t launch_batch_ellipsoid(const BatchView& B, T* F, const double* axes, 
     template hipError_t launch_batch_reinit_stage<T>(const BatchView&, T*, const T*, const T*, const uint8_t*,      \
                                                      const ListView&, int, int*, int*, int, hipStream_t, int);      \
     template hipError_t launch_batch_extend_stage<T>(const BatchView&, T*, const T*, const T*, const T*, const T*,  \
-                                                     const T*, const uint8_t*, const ListView&, const double*,      \
-                                                     const double*, int, hipStream_t, int);                         \
+                                                     const T*, const ListView&, const double*, const double*, int,  \
+                                                     hipStream_t, int);                                             \
     template hipError_t launch_batch_advect_stage<T>(const BatchView&, T*, const T*, const T*, const T*, const T*,  \
                                                      const T*, const double*, int, hipStream_t);                    \
     template hipError_t launch_batch_extension_velocity<T>(const BatchView&, T*, T*, T*, const T*, const uint8_t*,  \

@@ -729,9 +729,9 @@ template <int R, typename T, int S = IFACE_WENO>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(stage_waves<T>()))) k_extend_stage(T* __restrict__ out, const T* __restrict__ cs,
                                                       const T* __restrict__ cn, const T* __restrict__ vx,
                                                       const T* __restrict__ vy, const T* __restrict__ vz,
-                                                      const uint8_t* __restrict__ flags, IList il,
-                                                      const double* __restrict__ lcp, const double* __restrict__ lv,
-                                                      Box b, Inv inv, int stage, int depth, int niface)
+                                                      IList il, const double* __restrict__ lcp,
+                                                      const double* __restrict__ lv, Box b, Inv inv, int stage,
+                                                      int depth, int niface)
 {
     __shared__ T lds[2][Shape<R>::N];
     const int blk = (int)blockIdx.x;
@@ -1498,34 +1498,34 @@ hipError_t launch_extend_step(double* step, const double* deltat, const double* 
 
 template <typename T, int S>
 static hipError_t extend_stage_scheme(T* out, const T* cs, const T* cn, const T* vx, const T* vy, const T* vz,
-                                      const uint8_t* flags, const ListView& lv, const double* lcp,
-                                      const double* lvel, const Geom& g, int stage, hipStream_t s)
+                                      const ListView& lv, const double* lcp, const double* lvel, const Geom& g,
+                                      int stage, hipStream_t s)
 {
     const IList il{lv.idx, lv.count, lv.dist, lv.dt, lv.cap};
     const int zd = zmarch_depth(g), ni = fused_iface_blocks(g);
     const dim3 grid(bulk_blocks(g, zd) + ni);
     if (g.tr == 32)
-        k_extend_stage<32, T, S><<<grid, dim3(256), 0, s>>>(out, cs, cn, vx, vy, vz, flags, il, lcp, lvel,
+        k_extend_stage<32, T, S><<<grid, dim3(256), 0, s>>>(out, cs, cn, vx, vy, vz, il, lcp, lvel,
                                                             box_of(g), inv_of(g), stage, zd, ni);
     else
-        k_extend_stage<64, T, S><<<grid, dim3(256), 0, s>>>(out, cs, cn, vx, vy, vz, flags, il, lcp, lvel,
+        k_extend_stage<64, T, S><<<grid, dim3(256), 0, s>>>(out, cs, cn, vx, vy, vz, il, lcp, lvel,
                                                             box_of(g), inv_of(g), stage, zd, ni);
     return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_extend_stage(T* out, const T* cs, const T* cn, const T* vx, const T* vy, const T* vz,
-                               const uint8_t* flags, const ListView& lv, const double* lcp, const double* lvel,
-                               const Geom& g, int stage, hipStream_t s)
+                               const ListView& lv, const double* lcp, const double* lvel, const Geom& g, int stage,
+                               hipStream_t s)
 {
-    return extend_stage_scheme<T, IFACE_WENO>(out, cs, cn, vx, vy, vz, flags, lv, lcp, lvel, g, stage, s);
+    return extend_stage_scheme<T, IFACE_WENO>(out, cs, cn, vx, vy, vz, lv, lcp, lvel, g, stage, s);
 }
 
 hipError_t launch_cubic_extend_stage(double* out, const double* cs, const double* cn, const double* vx,
-                                     const double* vy, const double* vz, const uint8_t* flags, const ListView& lv,
-                                     const double* lcp, const double* lvel, const Geom& g, int stage, hipStream_t s)
+                                     const double* vy, const double* vz, const ListView& lv, const double* lcp,
+                                     const double* lvel, const Geom& g, int stage, hipStream_t s)
 {
-    return extend_stage_scheme<double, IFACE_CUBIC>(out, cs, cn, vx, vy, vz, flags, lv, lcp, lvel, g, stage, s);
+    return extend_stage_scheme<double, IFACE_CUBIC>(out, cs, cn, vx, vy, vz, lv, lcp, lvel, g, stage, s);
 }
 
 hipError_t launch_advection_step(double* astep, const double* vx, const double* vy, const double* vz,
@@ -1662,8 +1662,8 @@ hipError_t launch_narrow(float* dst, const double* src, int64_t n, hipStream_t s
     template hipError_t launch_extension_normal<T>(T*, T*, T*, const T*, const uint8_t*, const ListView&, double,   \
                                                    const Geom&, hipStream_t);                                       \
     template hipError_t launch_extend_stage<T>(T*, const T*, const T*, const T*, const T*, const T*,               \
-                                               const uint8_t*, const ListView&, const double*, const double*,       \
-                                               const Geom&, int, hipStream_t);                                      \
+                                               const ListView&, const double*, const double*, const Geom&, int,      \
+                                               hipStream_t);                                                        \
     template hipError_t launch_advect_stage<T>(T*, const T*, const T*, const T*, const T*, const T*, double,        \
                                                const Geom&, int, hipStream_t);                                      \
     template hipError_t launch_normal_velocity<T>(T*, const T*, const T*, const T*, const T*, const Geom&,          \

@@ -145,8 +145,8 @@ hipError_t launch_cubic_reinit_stage(double* out, const double* fs, const double
                                      const ListView& lv, const Geom& g, int stage, int* over, int* over_count,
                                      int seq, hipStream_t s);
 hipError_t launch_cubic_extend_stage(double* out, const double* cs, const double* cn, const double* vx,
-                                     const double* vy, const double* vz, const uint8_t* flags, const ListView& lv,
-                                     const double* lcp, const double* lvel, const Geom& g, int stage, hipStream_t s);
+                                     const double* vy, const double* vz, const ListView& lv, const double* lcp,
+                                     const double* lvel, const Geom& g, int stage, hipStream_t s);
 hipError_t launch_list_interp_cubic(double* lcp, const ListView& lv, const double* C, const Geom& g, hipStream_t s);
 // fused stage of the ENO-RK2 extension loop (operator + Heun combination;
 // dt = 0.3*min of the six distances d; cp the interface values; d and cp are
@@ -161,8 +161,8 @@ hipError_t launch_extend_step(double* step, const double* deltat, const double* 
                               const Geom& g, hipStream_t s);
 template <typename T>
 hipError_t launch_extend_stage(T* out, const T* cs, const T* cn, const T* vx, const T* vy, const T* vz,
-                               const uint8_t* flags, const ListView& lv, const double* lcp, const double* lvel,
-                               const Geom& g, int stage, hipStream_t s);
+                               const ListView& lv, const double* lcp, const double* lvel, const Geom& g, int stage,
+                               hipStream_t s);
 hipError_t launch_advection_step(double* astep, const double* vx, const double* vy, const double* vz,
                                  const double* lsf, const Geom& g, hipStream_t s);
 template <typename T>
@@ -302,8 +302,8 @@ hipError_t launch_batch_reinit_stage(const BatchView& B, T* out, const T* fs, co
                                      hipStream_t s, int scheme = 0);
 template <typename T>
 hipError_t launch_batch_extend_stage(const BatchView& B, T* out, const T* cs, const T* cn, const T* vx,
-                                     const T* vy, const T* vz, const uint8_t* flags, const ListView& lv,
-                                     const double* lcp, const double* lvel, int stage, hipStream_t s,
+                                     const T* vy, const T* vz, const ListView& lv, const double* lcp,
+                                     const double* lvel, int stage, hipStream_t s,
                                      int scheme = 0);
 // one ENO-RK2 extension stage of every grid (heun2: 0 first, 1 second Heun
 // stage); lcp: the ENO2 interface values of the listed points
