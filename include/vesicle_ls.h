@@ -370,9 +370,12 @@ int vls_sdf3_normal_velocity(vls_sdf3_t h, const double *F, const double *vx, co
  * counterpart there.  What stays FP64:
  *   - the subcell crossing distances (the sixth-order locator and its root
  *     solve read F converted to double and store double distances);
- *   - every interface-list point (a point with a subcell distance, and in the
- *     reinitialization a point next to a sign change): its nonuniform WENO,
- *     Godunov term and RK combination run in FP64 on the float values;
+ *   - every interface-list point (a point with a subcell distance): its
+ *     nonuniform WENO, Godunov term and RK combination run in FP64 on the
+ *     float values (a bulk point that meets a sign change during the
+ *     reinitialization stays in the FP32 bulk: the reference's stencil there
+ *     is the nonuniform one at full spacings, the uniform WENO5 up to
+ *     rounding);
  *   - the interface values and velocities of the extension (list order);
  *   - the area/volume and max-speed reductions (compensated FP64 sums).
  * The bulk (every other point) runs its WENO5, upwinding, Godunov term and RK
