@@ -252,17 +252,6 @@ __device__ __forceinline__ void weno5_pair(float& d_fore, float& d_back, const f
     d_fore = (m.c + qf) * j.k;
 }
 
-// a*b < 0 without forming the product (an FP32 product of two tiny values
-// can underflow to -0 and hide the sign change the FP64 test sees), as one
-// value: min(a,b) < 0 < max(a,b) exactly when max(min(a,b), -max(a,b)) < 0
-// (a zero of either sign gives >= 0).  Computed without branches, so the
-// test costs a few VALU operations and no scalar mask arithmetic.
-__device__ __forceinline__ float opposite_key(float a, float b)
-{
-    return __builtin_fmaxf(__builtin_fminf(a, b), -__builtin_fmaxf(a, b));
-}
-__device__ __forceinline__ bool opposite(float a, float b) { return opposite_key(a, b) < 0.f; }
-
 // ---------------------------------------------------------------------------
 // z-axis WENO5 pairs along a z-march (one point per page, pages in order).
 // The back side of page p+1 and the fore side of page p take the same four
@@ -284,7 +273,6 @@ struct ZFace<float> {
     float ic;      // 39/64 D2^2 + e' of the next page (this page's fore-side ib)
     float x0, x2;  // back-side 1/s0^2, 1/s2^2 of the next page (this page's fore 1/s2^2, 1/s0^2)
     float A;       // back-side A of the next page (this page's B)
-    float key;     // opposite_key of the next page's z-values 2 and 3
 };
 
 template <>
@@ -327,13 +315,10 @@ __device__ __forceinline__ void zface_start(ZFace<float>& f, const float p[7], J
     for (int k = 0; k < 4; ++k) d[k] = fmaf(-2.f, p[k + 1], p[k]) + p[k + 2];
     const float ic = fmaf(0.609375f * d[1], d[1], j.e);
     (void)zface_fore(f, d[0], d[1], d[2], d[3], ic, fmaf(-2.f, d[1], d[0]) + d[2], j.e);
-    f.key = opposite_key(p[2], p[3]);
 }
 
-// weno5_pair on the page's z-line p[0..6] with the faces of the page before;
-// cross: the sign test of sign_change3 on p[2..4]
-__device__ __forceinline__ void weno5_pair_z(float& d_fore, float& d_back, bool& cross, const float p[7],
-                                             ZFace<float>& f, JPf j)
+// weno5_pair on the page's z-line p[0..6] with the faces of the page before
+__device__ __forceinline__ void weno5_pair_z(float& d_fore, float& d_back, const float p[7], ZFace<float>& f, JPf j)
 {
     const float d1 = f.d[1], d2 = f.d[2], d3 = f.d[3];
     const float d4 = fmaf(-2.f, p[5], p[4]) + p[6];
@@ -349,9 +334,6 @@ __device__ __forceinline__ void weno5_pair_z(float& d_fore, float& d_back, bool&
         const float P = fmaf(4.f * f.x0, f.A, (6.f * f.x2) * B);
         qb = fmaf(P, __builtin_amdgcn_rcpf(__builtin_fmaxf(W, 1e-30f)), -B);
     }
-    const float key = opposite_key(p[3], p[4]);
-    cross = __builtin_fminf(f.key, key) < 0.f;
-    f.key = key;
     const float qf = zface_fore(f, d1, d2, d3, d4, ic, B, j.e);
     d_back = (c - qb) * j.k;
     d_fore = (c + qf) * j.k;

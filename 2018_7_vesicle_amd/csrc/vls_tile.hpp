@@ -57,13 +57,8 @@ __device__ __forceinline__ float rk_combine(int stage, float fs, float step, flo
 }
 
 // the interface test of re_step.cu:260-267 at the middle node of v[2..4]:
-// a sign change between it and either neighbour (FP64: the reference's
-// product test; FP32: the sign test of opposite_key, one compare for both)
+// a sign change between it and either neighbour, the reference's product test
 __device__ __forceinline__ bool sign_change3(const double v[7]) { return (v[2] * v[3] < 0) | (v[3] * v[4] < 0); }
-__device__ __forceinline__ bool sign_change3(const float v[7])
-{
-    return __builtin_fminf(opposite_key(v[2], v[3]), opposite_key(v[3], v[4])) < 0.f;
-}
 
 // append `take` lanes' values to list[*count ...] with one atomic per wave
 __device__ __forceinline__ void wave_append(bool take, int value, int* __restrict__ list, int* __restrict__ count)
@@ -719,9 +714,9 @@ constexpr bool defers_crossings()
 
 // WENO5-RK3 reinitialization stage at a bulk point: x- and y-taps from the
 // staged tile (lds, the point at index base), z-taps z[0..6] (z[3] the point),
-// the start value fn and flags fl.  Returns the new value; cross is set for a
-// point next to a sign change of the stage field (it is left to the interface
-// pass).
+// the start value fn and flags fl.  Returns the new value; where the bulk
+// defers crossings (FP64), cross is set for a point next to a sign change of
+// the stage field (it is left to the interface pass), else it stays false.
 // zf: the z-faces the march's previous page left (first: this is the
 // block's first page, zf is set up here)
 template <int R, typename T>
@@ -735,13 +730,17 @@ __device__ __forceinline__ T reinit_value(const T* lds, int base, const T z[7], 
         T v[7];
         if (a == 0) tile_line_x<R>(lds, base, v);
         else tile_line_y(lds, base, v);
-        cross |= sign_change3(v);
+        if constexpr (defers_crossings<T>()) cross |= sign_change3(v);
         weno5_pair(fo[a], bk[a], v, axis_arg(invs, a, T()));
     }
     if (first) zface_start(zf, z, axis_arg(invs, 2, T()));
-    bool cz;
-    weno5_pair_z(fo[2], bk[2], cz, z, zf, axis_arg(invs, 2, T()));
-    cross |= cz;
+    if constexpr (defers_crossings<T>()) {
+        bool cz;
+        weno5_pair_z(fo[2], bk[2], cz, z, zf, axis_arg(invs, 2, T()));
+        cross |= cz;
+    } else {
+        weno5_pair_z(fo[2], bk[2], z, zf, axis_arg(invs, 2, T()));
+    }
     const T step = godunov_step((fl & FLAG_MASK) != 0, fo[0], bk[0], fo[1], bk[1], fo[2], bk[2], (T)invs.dt_bulk);
     return rk_combine(stage, z[3], step, fn);
 }
