@@ -299,6 +299,15 @@ __device__ __forceinline__ T ldp(const T* __restrict__ base, unsigned off)
 {
     return *(const T*)((const char*)base + off * (unsigned)sizeof(T));
 }
+// as ldp, marked non-temporal: for a stream read once per launch (the
+// velocity of the extension and advection stages), so that its lines do not
+// push the staged tiles' halos out of L2 (measured: extension stage at 256^3
+// FP32 103-107 -> 98-100 us)
+template <typename T>
+__device__ __forceinline__ T ldp_nt(const T* __restrict__ base, unsigned off)
+{
+    return __builtin_nontemporal_load((const T*)((const char*)base + off * (unsigned)sizeof(T)));
+}
 template <typename T>
 __device__ __forceinline__ void stp(T* __restrict__ base, unsigned off, T v)
 {
@@ -622,9 +631,9 @@ __device__ __forceinline__ void zstep_load(ZStep<R, T, VEL>& s, const ZCol<R>& z
     s.reg.s = ldp(fp, zc.s);
     s.w = ldp(fs + (idx_t)min(p + 3, b.pges - 1) * plane, zc.in);
     if (VEL) {
-        s.vel[0] = ldp(vx + po, zc.in);
-        s.vel[1] = ldp(vy + po, zc.in);
-        s.vel[2] = ldp(vz + po, zc.in);
+        s.vel[0] = ldp_nt(vx + po, zc.in);
+        s.vel[1] = ldp_nt(vy + po, zc.in);
+        s.vel[2] = ldp_nt(vz + po, zc.in);
     }
     // at stage 0 (no start value) the point's own value is read and not used
     // (rk_combine does not read fn at stage 0)
