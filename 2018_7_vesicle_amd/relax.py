@@ -17,12 +17,20 @@ as a scalar), as the driver's gather does.
 
 The toolbox writes into buffers allocated once; every operator is a HIP
 kernel of the library (the FFT through rocFFT).
+
+BatchRelax steps several vesicles together: the three extensions, setDistance
+and the reinitialization, which take almost all of a step's time, run as one
+batched launch sequence over every grid, and the rest (toolbox, bending
+speed, tension/pressure, smoothing, update) per grid on views of the packed
+fields, with one read-back of all time steps per step.  Each vesicle's
+results are bit for bit those of its own VesicleRelax.
 """
 from dataclasses import dataclass
 
 import torch
 
 from ._lib import VLSError
+from .batch import Batch
 from .sdf3 import GD3, SDF3, TOOLBOX_FIELDS, feval_set_calculus_toolbox
 
 
@@ -111,5 +119,100 @@ class VesicleRelax:
         if cfg.reinit_every > 0 and self.iteration % cfg.reinit_every == 0:
             m.F = m.WENO5RK3Reinitialization(m.F, cfg.reinit_iters)
         self.time += dt
+        self.info = info
+        return info
+
+
+class BatchRelax:
+    """VesicleRelax for several vesicles on grids of any sizes, stepped together.
+
+    F holds every grid's level set packed (Batch layout); `grid_view(g)` is
+    vesicle g's (pges, cols, rows) view of it.  Per step, in the order of
+    VesicleRelax.step: the toolbox of every grid, one batched WENORK3Extend
+    of the mean curvatures, the bending speeds, one batched ENORK2Extend, the
+    tension/pressure solves, one read-back of the time steps, the smoothings,
+    one batched ENORK2Extend of the speeds, the updates, one batched
+    setDistance and one batched WENO5RK3Reinitialization."""
+
+    def __init__(self, grids, Fs, cfg: RelaxConfig = RelaxConfig()):
+        if len(grids) == 0 or len(grids) != len(Fs):
+            raise VLSError("one level set per grid, at least one grid")
+        if any(F.dtype != torch.float64 for F in Fs):
+            raise VLSError("the relaxation step is FP64")
+        self.cfg = cfg
+        self.grids = list(grids)
+        self.B = Batch([(g.mrows, g.ncols, g.lshts) for g in grids], [(g.Dx, g.Dy, g.Dz) for g in grids],
+                       dtype=torch.float64, device=grids[0].device)
+        self.F = self.B.pack([F.contiguous() for F in Fs])
+        if cfg.autotune:
+            self.B.autotune(self.F, iterations=5)  # times copies; F is left as it is
+        # one SDF3 per grid over its view of F: the tension/pressure reductions
+        # and the initial area and volume run on it
+        self.maps = [SDF3(g, self.grid_view(i)) for i, g in enumerate(grids)]
+        self._tool = [self.B.zeros() for _ in TOOLBOX_FIELDS]
+        self._hp = self.B.zeros()
+        self._H = self.B.zeros()
+        self._Vb = self.B.zeros()
+        self._speed = self.B.zeros()
+        self._toolbox()
+        av = [m.calAreaVolume() for m in self.maps]
+        self.InitialArea = [a for a, _ in av]
+        self.InitialVolume = [v for _, v in av]
+        self.time = [0.0] * len(grids)
+        self.iteration = 1
+        self.info = None
+        self.energy = []
+
+    def grid_view(self, g, packed=None):
+        return self.B.grid_view(self.F if packed is None else packed, g)
+
+    def _toolbox(self):
+        torch.clamp(self.F, min=0.0, out=self._hp)
+        for i, (m, g) in enumerate(zip(self.maps, self.grids)):
+            tool = [self.grid_view(i, t) for t in self._tool]
+            hp = self.grid_view(i, self._hp)
+            feval_set_calculus_toolbox(tool, m.F, hp, g.mrows, g.ncols, g.lshts, g.Dx, g.Dy, g.Dz, g.Ds)
+            for name, t in zip(TOOLBOX_FIELDS, tool):
+                setattr(m, name, t)
+            m.HPrimal = hp
+
+    def step(self):
+        """one relaxation step of every vesicle; returns the (ngrids, 10) device
+        tensor of their tension/pressure scalars (TENSION_PRESSURE_FIELDS)"""
+        cfg, B = self.cfg, self.B
+        self.iteration += 1
+        self._toolbox()
+        self._H.copy_(self._tool[TOOLBOX_FIELDS.index("MeanCurvature")])
+        B.wenork3_extend(self.F, self._H, cfg.ext_iters)
+        for i, (m, g) in enumerate(zip(self.maps, self.grids)):
+            g.bendingSpeed(self.grid_view(i, self._H), m.GaussianCurvature, cfg.kappa, cfg.sponc,
+                           out=self.grid_view(i, self._Vb))
+        B.enork2_extend(self.F, self._Vb, cfg.ext_iters)
+        self._speed.zero_()
+        infos = []
+        for i, m in enumerate(self.maps):
+            _, info = m.tensionPressure(self.grid_view(i, self._H), self.grid_view(i, self._Vb),
+                                        self.InitialArea[i], self.InitialVolume[i], CFLNumber=cfg.cfl,
+                                        out=self.grid_view(i, self._speed))
+            infos.append(info)
+        info = torch.stack(infos)
+        dts = info[:, 2].tolist()  # the one read-back of the step
+        for i, g in enumerate(self.grids):
+            sp = self.grid_view(i, self._speed)
+            g.smoothGMRES(sp, dts[i], cfg.alpha, out=sp)
+        B.enork2_extend(self.F, self._speed, cfg.ext_iters)
+        for i in range(len(self.grids)):
+            Fi = self.grid_view(i)
+            Fi.copy_(Fi - dts[i] * self.grid_view(i, self._speed))
+        B.set_distance(self.F)
+        energy = []
+        for i, (m, g) in enumerate(zip(self.maps, self.grids)):
+            band = m.F.abs() < 2 * g.Ds
+            hs = self.grid_view(i, self._H) - cfg.sponc
+            energy.append((hs * hs * m.DiracDelta * m.FGradMag)[band].sum() * g.Ds ** 3)
+        self.energy.append(energy)
+        if cfg.reinit_every > 0 and self.iteration % cfg.reinit_every == 0:
+            B.reinitialize(self.F, cfg.reinit_iters)
+        self.time = [t + dt for t, dt in zip(self.time, dts)]
         self.info = info
         return info

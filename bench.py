@@ -654,8 +654,9 @@ def relax(with_cpu, steps=5):
     iterations, reinitialization every step): toolbox, WENORK3Extend,
     bending speed, ENORK2Extend, tension/pressure, smoothGMRES (rocFFT),
     ENORK2Extend, update, setDistance, WENO5RK3Reinitialization.  ms per step
-    on the GPU (one warm-up step, then `steps`), and one step of the CPU
-    oracle's composition of the same operators."""
+    on the GPU (one warm-up step, then `steps`); the same for eight vesicles
+    of reduced volumes 0.60 .. 0.88 stepped together (BatchRelax); and one
+    step of the CPU oracle's composition of the same operators."""
     import numpy as np
     import torch
 
@@ -682,13 +683,36 @@ def relax(with_cpu, steps=5):
     out = {"grid": n, "steps": steps, "gpu_ms_per_step": ms, "gpu_updates_per_s": upd / (ms * 1e-3),
            "area_drift": info["area"] / r.InitialArea - 1, "volume_drift": info["volume"] / r.InitialVolume - 1,
            "Tension": info["Tension"], "Pressure": info["Pressure"], "Dt": info["Dt"]}
+    a0, v0 = r.InitialArea, r.InitialVolume
+    del r
+    # eight vesicles of reduced volumes 0.60 .. 0.88 on 96^3 grids stepped
+    # together (BatchRelax: batched extensions, setDistance and reinitialization)
+    nb = 8
+    Fs = []
+    for k in range(nb):
+        mk = vls.SDF3(G, torch.from_numpy(vls.relax.oblate_ellipsoid(n, reduced_volume=0.6 + 0.04 * k)[3]).cuda())
+        mk.setDistance()
+        Fs.append(mk.WENO5RK3Reinitialization(mk.F, 500))
+        del mk
+    br = vls.relax.BatchRelax([G] * nb, Fs, cfg)
+    del Fs
+    br.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        br.step()
+    torch.cuda.synchronize()
+    bms = (time.perf_counter() - t0) * 1e3 / steps
+    out["batch"] = {"vesicles": nb, "grid": n, "gpu_ms_per_step": bms, "gpu_updates_per_s": nb * upd / (bms * 1e-3),
+                    "speedup_vs_one_at_a_time": nb * ms / bms}
+    del br
     if with_cpu:
         from oracle import pyoracle as po
         from oracle import relax_oracle as ro
 
         Fc = F.cpu().numpy()
         t0 = time.perf_counter()
-        ro.relax_step(Fc, r.InitialArea, r.InitialVolume, G.Dx, G.Dy, G.Dz, cfl=cfg.cfl)
+        ro.relax_step(Fc, a0, v0, G.Dx, G.Dy, G.Dz, cfl=cfg.cfl)
         cpu_s = time.perf_counter() - t0
         out["cpu_ms_per_step"] = cpu_s * 1e3
         out["cpu_threads"] = po.num_threads()

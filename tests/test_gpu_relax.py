@@ -211,3 +211,41 @@ def test_relax_reproduces_the_reference_grid_study(n, steps, tol, ttol):
         for k in ("t", "P", "ten", "ene"):
             bar = ttol if k == "t" else tol
             assert abs(got[k] - ref[k][i]) <= bar * abs(ref[k][i]), (n, i, k, got[k], ref[k][i])
+
+
+def test_batch_relax_equals_single_vesicles():
+    """BatchRelax on three vesicles of different grids (32^3, 40x36x44 and
+    48^3, distinct spacings) against a VesicleRelax per vesicle, bit for bit
+    over three loop bodies (reinitialization every second step, so steps
+    with and without it); the single-vesicle step is the oracle's
+    (test_relax_step_matches_the_oracle), and the first vesicle is checked
+    against the oracle here too"""
+    shapes = [((32, 32, 32), (0.7, 0.55, 0.45)), ((40, 36, 44), (0.6, 0.6, 0.4)), ((48, 48, 48), (0.5, 0.65, 0.5))]
+    grids, Fs = [], []
+    for (rows, cols, pges), axes in shapes:
+        xv, yv, zv = np.linspace(-1.0, 1.0, cols), np.linspace(-0.95, 0.95, rows), np.linspace(-0.9, 0.9, pges)
+        g = cases.Grid(xv, yv, zv)
+        F0 = np.sqrt(g.X ** 2 / axes[0] ** 2 + g.Y ** 2 / axes[1] ** 2 + g.Z ** 2 / axes[2] ** 2) - 1.0
+        d6, md = po.set_distance(np.ascontiguousarray(F0), *g.spacing)
+        Fs.append(po.weno5_rk3_reinit(np.ascontiguousarray(F0), 10, d6, md, *g.spacing))
+        grids.append((g, vls.GD3(xv, yv, zv)))
+    cfg = vls.relax.RelaxConfig(ext_iters=6, reinit_iters=6, reinit_every=2)
+    br = vls.relax.BatchRelax([G for _, G in grids], [T(F) for F in Fs], cfg)
+    singles = [vls.relax.VesicleRelax(G, T(F), cfg) for (_, G), F in zip(grids, Fs)]
+    assert br.InitialArea == [r.InitialArea for r in singles]
+    assert br.InitialVolume == [r.InitialVolume for r in singles]
+    Fo = Fs[0]
+    g0 = grids[0][0]
+    for step in range(3):
+        info = br.step()
+        for i, r in enumerate(singles):
+            want = r.step()
+            assert torch.equal(info[i], want), f"step {step} vesicle {i} tension/pressure scalars"
+            assert torch.equal(br.grid_view(i), r.map.F), f"step {step} vesicle {i} level set"
+            assert float(br.energy[-1][i]) == float(r.energy[-1]), f"step {step} vesicle {i} energy"
+        assert br.time == [r.time for r in singles]
+        Fo, want = ro.relax_step(Fo, singles[0].InitialArea, singles[0].InitialVolume, *g0.spacing, cfl=cfg.cfl,
+                                 ext_iters=6, reinit=step % 2 == 0, reinit_iters=6)
+        assert np.abs(N(br.grid_view(0)) - Fo).max() < 1e-12, f"step {step} vesicle 0 against the oracle"
+    with pytest.raises(vls.VLSError, match="FP64"):
+        vls.relax.BatchRelax([grids[0][1]], [T(Fs[0]).float()], cfg)
