@@ -487,34 +487,43 @@ __global__ void __launch_bounds__(256) kb_list_interp(double* __restrict__ cp, I
 // ENO-RK2 extension stage (ENORK2Extend.m:68-79, Heun; heun2 == 0:
 // out = cs - L(cs), heun2 == 1: out = (cn + cs - L(cs)) / 2), as the
 // single-grid k_enork2_extend_stage: the points without a subcell distance in
-// a tile pass (spacings for distances, pseudo-time step 0.3 min of them), the
-// listed points in a list pass (distances, and interface values in list
-// order)
+// a z-marching pass over the batch's work items (spacings for distances,
+// pseudo-time step 0.3 min of them; x- and y-taps from the staged tile,
+// z-taps from the register window, the same values line7 reads), the listed
+// points in a list pass (distances, and interface values in list order)
 template <int R>
-__global__ void __launch_bounds__(256) kb_eno2_extend_bulk(double* __restrict__ out, const double* __restrict__ cs,
-                                                           const double* __restrict__ cn,
-                                                           const double* __restrict__ vx,
-                                                           const double* __restrict__ vy,
-                                                           const double* __restrict__ vz,
-                                                           const uint8_t* __restrict__ flags, BatchView B, int heun2)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(stage_waves<double>())))
+kb_eno2_extend_bulk(double* __restrict__ out, const double* __restrict__ cs, const double* __restrict__ cn,
+                    const double* __restrict__ vx, const double* __restrict__ vy, const double* __restrict__ vz,
+                    const uint8_t* __restrict__ flags, BatchView B, int heun2)
 {
 #pragma clang fp contract(off)
-    const TileRange tr = tile_range(B.ntiles);
-    for (int tile = tr.first; tile < tr.end; tile += tr.step) {
-        const BGrid G = B.grids[B.tile_grid[tile]];
-        const Box b = box_of(G);
-        const Tile t = tile_at<R>(b, tile - G.tile0);
-        if (!t.valid) continue;
-        const idx_t i = G.off + lin(b, t.r, t.c, t.p);
-        if (flags[i] & FLAG_SUBCELL) continue;
-        Dists q;
-        q.r[0] = q.l[0] = b.dx;
-        q.r[1] = q.l[1] = b.dy;
-        q.r[2] = q.l[2] = b.dz;
-        const double h0[3] = {0.0, 0.0, 0.0};
-        const double step = enork2_residual(cs + G.off, t.r, t.c, t.p, b, q, h0, h0, vx[i], vy[i], vz[i], 0.3 * min6(q));
-        out[i] = heun2 ? (cn[i] + cs[i] - step) / 2 : cs[i] - step;
-    }
+    __shared__ double lds[2][Shape<R>::N];
+    const ZItem it = B.items[xcd_remap((int)blockIdx.x, (int)gridDim.x)];
+    const BGrid G = B.grids[it.g];
+    const Box b = box_of(G);
+    const idx_t off = G.off, plane = (idx_t)b.rows * b.cols;
+    double* o = out + off;
+    Dists q;
+    q.r[0] = q.l[0] = b.dx;
+    q.r[1] = q.l[1] = b.dy;
+    q.r[2] = q.l[2] = b.dz;
+    const double dt = 0.3 * min6(q);
+    // stage != 0 loads the start value cn at each point (ZStep::fn)
+    zmarch<R, double, true, true>(lds[0], lds[1], b, it.r0, it.c0, it.p0, it.p1, cs + off, shift(cn, off), vx + off,
+                                  vy + off, vz + off, flags + off, heun2,
+                                  [&](const double* l, const ZCol<R>& zc, const double w[7],
+                                      const ZStep<R, double, true>& st, int p) {
+                                      double v[7], fo[3], bk[3];
+                                      tile_line_x<R>(l, zc.base, v);
+                                      eno2_field_pair(fo[0], bk[0], v, b.dx, b.dx, b.dx, 0.0, 0.0);
+                                      tile_line_y(l, zc.base, v);
+                                      eno2_field_pair(fo[1], bk[1], v, b.dy, b.dy, b.dy, 0.0, 0.0);
+                                      eno2_field_pair(fo[2], bk[2], w, b.dz, b.dz, b.dz, 0.0, 0.0);
+                                      const double step = upwind_sum(fo, bk, st.vel[0], st.vel[1], st.vel[2], dt);
+                                      const double res = heun2 ? (st.fn + w[3] - step) / 2 : w[3] - step;
+                                      if (!(st.flags(zc) & FLAG_SUBCELL)) stp(o + (idx_t)p * plane, zc.in, res);
+                                  });
 }
 
 __global__ void __launch_bounds__(256) kb_eno2_extend_list(double* __restrict__ out, const double* __restrict__ cs,
@@ -858,8 +867,8 @@ hipError_t launch_batch_eno2_extend_stage(const BatchView& B, double* out, const
                                           const double* vx, const double* vy, const double* vz, const uint8_t* flags,
                                           const ListView& lv, const double* lcp, int heun2, hipStream_t s)
 {
-    if (B.tr == 32) kb_eno2_extend_bulk<32><<<B.blocks, 256, 0, s>>>(out, cs, cn, vx, vy, vz, flags, B, heun2);
-    else kb_eno2_extend_bulk<64><<<B.blocks, 256, 0, s>>>(out, cs, cn, vx, vy, vz, flags, B, heun2);
+    if (B.tr == 32) kb_eno2_extend_bulk<32><<<B.nitems, 256, 0, s>>>(out, cs, cn, vx, vy, vz, flags, B, heun2);
+    else kb_eno2_extend_bulk<64><<<B.nitems, 256, 0, s>>>(out, cs, cn, vx, vy, vz, flags, B, heun2);
     kb_eno2_extend_list<<<list_blocks(B), 256, 0, s>>>(out, cs, cn, vx, vy, vz, ilist(lv), lcp, B, heun2);
     return hipGetLastError();
 }
