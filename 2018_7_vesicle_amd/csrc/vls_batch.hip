@@ -486,20 +486,46 @@ __global__ void __launch_bounds__(256) kb_list_interp(double* __restrict__ cp, I
 
 // ENO-RK2 extension stage (ENORK2Extend.m:68-79, Heun; heun2 == 0:
 // out = cs - L(cs), heun2 == 1: out = (cn + cs - L(cs)) / 2), as the
-// single-grid k_enork2_extend_stage: the points without a subcell distance in
-// a z-marching pass over the batch's work items (spacings for distances,
-// pseudo-time step 0.3 min of them; x- and y-taps from the staged tile,
-// z-taps from the register window, the same values line7 reads), the listed
-// points in a list pass (distances, and interface values in list order)
+// single-grid k_enork2_extend_stage, in one launch: the first niface blocks
+// take the listed points (distances, and interface values in list order),
+// the others the points without a subcell distance in a z-marching pass over
+// the batch's work items (spacings for distances, pseudo-time step 0.3 min of
+// them; x- and y-taps from the staged tile, z-taps from the register window,
+// the same values line7 reads).  Each point reads cn only at itself, so out
+// may be cn.
 template <int R>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(stage_waves<double>())))
-kb_eno2_extend_bulk(double* __restrict__ out, const double* __restrict__ cs, const double* __restrict__ cn,
-                    const double* __restrict__ vx, const double* __restrict__ vy, const double* __restrict__ vz,
-                    const uint8_t* __restrict__ flags, BatchView B, int heun2)
+kb_eno2_extend_stage(double* __restrict__ out, const double* __restrict__ cs, const double* __restrict__ cn,
+                     const double* __restrict__ vx, const double* __restrict__ vy, const double* __restrict__ vz,
+                     const uint8_t* __restrict__ flags, IList il, const double* __restrict__ lcp, BatchView B,
+                     int heun2, int niface)
 {
 #pragma clang fp contract(off)
     __shared__ double lds[2][Shape<R>::N];
-    const ZItem it = B.items[xcd_remap((int)blockIdx.x, (int)gridDim.x)];
+    if ((int)blockIdx.x < niface) {
+        const int cnt = *il.count;
+        for (int j = (int)(blockIdx.x * 256 + threadIdx.x); j < cnt; j += niface * 256) {
+            const int64_t i = il.idx[j];
+            const BGrid G = B.grids[B.list_grid[j]];
+            const Box b = box_of(G);
+            int r, c, p;
+            coords_of(b, i - G.off, r, c, p);
+            Dists q;
+            double hf[3], hb[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                q.r[a] = il.dist[(size_t)(2 * a) * il.cap + j];
+                q.l[a] = il.dist[(size_t)(2 * a + 1) * il.cap + j];
+                hf[a] = lcp[(size_t)(2 * a) * il.cap + j];
+                hb[a] = lcp[(size_t)(2 * a + 1) * il.cap + j];
+            }
+            const double step =
+                enork2_residual(cs + G.off, r, c, p, b, q, hf, hb, vx[i], vy[i], vz[i], 0.3 * min6(q));
+            out[i] = heun2 ? (cn[i] + cs[i] - step) / 2 : cs[i] - step;
+        }
+        return;
+    }
+    const ZItem it = B.items[xcd_remap((int)blockIdx.x - niface, (int)gridDim.x - niface)];
     const BGrid G = B.grids[it.g];
     const Box b = box_of(G);
     const idx_t off = G.off, plane = (idx_t)b.rows * b.cols;
@@ -524,35 +550,6 @@ kb_eno2_extend_bulk(double* __restrict__ out, const double* __restrict__ cs, con
                                       const double res = heun2 ? (st.fn + w[3] - step) / 2 : w[3] - step;
                                       if (!(st.flags(zc) & FLAG_SUBCELL)) stp(o + (idx_t)p * plane, zc.in, res);
                                   });
-}
-
-__global__ void __launch_bounds__(256) kb_eno2_extend_list(double* __restrict__ out, const double* __restrict__ cs,
-                                                           const double* __restrict__ cn,
-                                                           const double* __restrict__ vx,
-                                                           const double* __restrict__ vy,
-                                                           const double* __restrict__ vz, IList il,
-                                                           const double* __restrict__ lcp, BatchView B, int heun2)
-{
-#pragma clang fp contract(off)
-    const int cnt = *il.count;
-    for (int j = (int)(blockIdx.x * 256 + threadIdx.x); j < cnt; j += (int)(gridDim.x * 256)) {
-        const int64_t i = il.idx[j];
-        const BGrid G = B.grids[B.list_grid[j]];
-        const Box b = box_of(G);
-        int r, c, p;
-        coords_of(b, i - G.off, r, c, p);
-        Dists q;
-        double hf[3], hb[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            q.r[a] = il.dist[(size_t)(2 * a) * il.cap + j];
-            q.l[a] = il.dist[(size_t)(2 * a + 1) * il.cap + j];
-            hf[a] = lcp[(size_t)(2 * a) * il.cap + j];
-            hb[a] = lcp[(size_t)(2 * a + 1) * il.cap + j];
-        }
-        const double step = enork2_residual(cs + G.off, r, c, p, b, q, hf, hb, vx[i], vy[i], vz[i], 0.3 * min6(q));
-        out[i] = heun2 ? (cn[i] + cs[i] - step) / 2 : cs[i] - step;
-    }
 }
 
 template <typename T>
@@ -867,9 +864,13 @@ hipError_t launch_batch_eno2_extend_stage(const BatchView& B, double* out, const
                                           const double* vx, const double* vy, const double* vz, const uint8_t* flags,
                                           const ListView& lv, const double* lcp, int heun2, hipStream_t s)
 {
-    if (B.tr == 32) kb_eno2_extend_bulk<32><<<B.nitems, 256, 0, s>>>(out, cs, cn, vx, vy, vz, flags, B, heun2);
-    else kb_eno2_extend_bulk<64><<<B.nitems, 256, 0, s>>>(out, cs, cn, vx, vy, vz, flags, B, heun2);
-    kb_eno2_extend_list<<<list_blocks(B), 256, 0, s>>>(out, cs, cn, vx, vy, vz, ilist(lv), lcp, B, heun2);
+    const int ni = fused_list_blocks(B);
+    if (B.tr == 32)
+        kb_eno2_extend_stage<32><<<B.nitems + ni, 256, 0, s>>>(out, cs, cn, vx, vy, vz, flags, ilist(lv), lcp, B,
+                                                               heun2, ni);
+    else
+        kb_eno2_extend_stage<64><<<B.nitems + ni, 256, 0, s>>>(out, cs, cn, vx, vy, vz, flags, ilist(lv), lcp, B,
+                                                               heun2, ni);
     return hipGetLastError();
 }
 
