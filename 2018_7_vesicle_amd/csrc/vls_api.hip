@@ -1908,8 +1908,10 @@ VLS_BATCH_ENTRIES(float, _f32)
 // reinitialization iterations of another copy, for every candidate launch
 // shape and keeps the shape with the smallest sum.  The results of every method are the same for
 // every shape; only the time differs.  Candidates: tile rows 64 and 32
-// (LDS tile 70 x 10 or 38 x 14 values) and, for a batch, 2, 4, 8 or 16
-// persistent blocks of 256 threads per CU (8 to 64 waves per CU).
+// (LDS tile 70 x 10 or 38 x 14 values) times the z-march depths (pages per
+// block or work item): for one grid the default depth, half and twice it;
+// for a batch 4, 8, 16 and 32 (the 512-vesicle batch of edges 32..96
+// measured 0.4-0.7% faster at 32 than at 16 in FP64 and FP32).
 namespace {
 
 struct EventTimer {
@@ -1942,10 +1944,10 @@ int batch_autotune(vls_batch* b, const T* F, int iterations, hipStream_t s, int*
     float best = 1e30f;
     int best_tr = b->tr, best_zd = b->zd;
     const int trs[2] = {64, 32};
-    const int depths[3] = {4, 8, 16};
+    const int depths[4] = {4, 8, 16, 32};
     if (hipEventCreate(&ev.a) != hipSuccess || hipEventCreate(&ev.b) != hipSuccess) rc = fail(VLS_ERR_HIP, "hipEventCreate");
     for (int i = 0; i < 2 && rc == VLS_OK; ++i) {
-        for (int j = 0; j < 3 && rc == VLS_OK; ++j) {
+        for (int j = 0; j < 4 && rc == VLS_OK; ++j) {
             b->zd = depths[j];
             if ((rc = build_tiles(b, trs[i]))) break;
             b->blocks = default_blocks(b->ntiles);

@@ -543,7 +543,7 @@ int vls_batch_enork2_extend(vls_batch_t b, const double *F, double *C, int itera
 /* Times `iterations` extension iterations of a scratch copy of F, off F's
  * interface, plus as many reinitialization iterations of another copy, for
  * each candidate launch shape and keeps the one with the smallest sum: tile rows
- * 64 or 32 and the z-march depth (a batch: 4, 8 or 16 pages; a single grid:
+ * 64 or 32 and the z-march depth (a batch: 4, 8, 16 or 32 pages; a single grid:
  * the depth chosen by the box size, half or twice it; read back with
  * vls_batch_get_zmarch_depth / vls_sdf3_get_launch).  A batch's persistent
  * blocks are left at 8 per CU.  Results of
