@@ -80,6 +80,18 @@ def test_relaxation_entries_reject_bad_arguments_without_device_work():
     assert b"null output 29" in lib.vls_last_error()
 
 
+def test_batch_relax_rejects_bad_arguments_before_device_work():
+    import torch
+
+    with pytest.raises(vls.VLSError, match="one level set per grid"):
+        vls.relax.BatchRelax([], [])
+    g = vls.GD3([0.0, 0.1, 0.2, 0.3], [0.0, 0.1, 0.2, 0.3], [0.0, 0.1, 0.2, 0.3], device="cpu")
+    with pytest.raises(vls.VLSError, match="one level set per grid"):
+        vls.relax.BatchRelax([g, g], [torch.zeros(4, 4, 4, dtype=torch.float64)])
+    with pytest.raises(vls.VLSError, match="FP64"):
+        vls.relax.BatchRelax([g], [torch.zeros(4, 4, 4, dtype=torch.float32)])
+
+
 def test_missing_library_fails_loudly(monkeypatch):
     lib_mod = importlib.import_module("2018_7_vesicle_amd._lib")
     monkeypatch.setattr(lib_mod, "_lib", None)
