@@ -146,16 +146,22 @@ class BatchRelax:
         self.F = self.B.pack([F.contiguous() for F in Fs])
         if cfg.autotune:
             self.B.autotune(self.F, iterations=5)  # times copies; F is left as it is
-        # one SDF3 per grid over its view of F: the tension/pressure reductions
-        # and the initial area and volume run on it
-        self.maps = [SDF3(g, self.grid_view(i)) for i, g in enumerate(grids)]
+        # the tension/pressure reductions and the initial area and volume run
+        # on an SDF3 handle; its work memory (about 450 bytes per point) is
+        # shared by every vesicle of one grid geometry, pointed at each
+        # vesicle's views in turn (its calls are ordered on the stream)
+        self._maps = {}
+        for g in grids:
+            key = (g.mrows, g.ncols, g.lshts, g.Dx, g.Dy, g.Dz)
+            if key not in self._maps:
+                self._maps[key] = SDF3(g, g.zeros())
         self._tool = [self.B.zeros() for _ in TOOLBOX_FIELDS]
         self._hp = self.B.zeros()
         self._H = self.B.zeros()
         self._Vb = self.B.zeros()
         self._speed = self.B.zeros()
         self._toolbox()
-        av = [m.calAreaVolume() for m in self.maps]
+        av = [self._map(i).calAreaVolume() for i in range(len(grids))]
         self.InitialArea = [a for a, _ in av]
         self.InitialVolume = [v for _, v in av]
         self.time = [0.0] * len(grids)
@@ -166,34 +172,43 @@ class BatchRelax:
     def grid_view(self, g, packed=None):
         return self.B.grid_view(self.F if packed is None else packed, g)
 
+    def _map(self, i):
+        """the shared SDF3 of vesicle i's geometry, pointed at its level set and toolbox"""
+        g = self.grids[i]
+        m = self._maps[(g.mrows, g.ncols, g.lshts, g.Dx, g.Dy, g.Dz)]
+        m.F = self.grid_view(i)
+        for name, t in zip(TOOLBOX_FIELDS, self._tool):
+            setattr(m, name, self.grid_view(i, t))
+        m.HPrimal = self.grid_view(i, self._hp)
+        return m
+
     def _toolbox(self):
         torch.clamp(self.F, min=0.0, out=self._hp)
-        for i, (m, g) in enumerate(zip(self.maps, self.grids)):
+        for i, g in enumerate(self.grids):
             tool = [self.grid_view(i, t) for t in self._tool]
-            hp = self.grid_view(i, self._hp)
-            feval_set_calculus_toolbox(tool, m.F, hp, g.mrows, g.ncols, g.lshts, g.Dx, g.Dy, g.Dz, g.Ds)
-            for name, t in zip(TOOLBOX_FIELDS, tool):
-                setattr(m, name, t)
-            m.HPrimal = hp
+            feval_set_calculus_toolbox(tool, self.grid_view(i), self.grid_view(i, self._hp), g.mrows, g.ncols,
+                                       g.lshts, g.Dx, g.Dy, g.Dz, g.Ds)
 
     def step(self):
         """one relaxation step of every vesicle; returns the (ngrids, 10) device
         tensor of their tension/pressure scalars (TENSION_PRESSURE_FIELDS)"""
         cfg, B = self.cfg, self.B
+        gauss = TOOLBOX_FIELDS.index("GaussianCurvature")
+        delta, gmag = TOOLBOX_FIELDS.index("DiracDelta"), TOOLBOX_FIELDS.index("FGradMag")
         self.iteration += 1
         self._toolbox()
         self._H.copy_(self._tool[TOOLBOX_FIELDS.index("MeanCurvature")])
         B.wenork3_extend(self.F, self._H, cfg.ext_iters)
-        for i, (m, g) in enumerate(zip(self.maps, self.grids)):
-            g.bendingSpeed(self.grid_view(i, self._H), m.GaussianCurvature, cfg.kappa, cfg.sponc,
+        for i, g in enumerate(self.grids):
+            g.bendingSpeed(self.grid_view(i, self._H), self.grid_view(i, self._tool[gauss]), cfg.kappa, cfg.sponc,
                            out=self.grid_view(i, self._Vb))
         B.enork2_extend(self.F, self._Vb, cfg.ext_iters)
         self._speed.zero_()
         infos = []
-        for i, m in enumerate(self.maps):
-            _, info = m.tensionPressure(self.grid_view(i, self._H), self.grid_view(i, self._Vb),
-                                        self.InitialArea[i], self.InitialVolume[i], CFLNumber=cfg.cfl,
-                                        out=self.grid_view(i, self._speed))
+        for i in range(len(self.grids)):
+            _, info = self._map(i).tensionPressure(self.grid_view(i, self._H), self.grid_view(i, self._Vb),
+                                                   self.InitialArea[i], self.InitialVolume[i], CFLNumber=cfg.cfl,
+                                                   out=self.grid_view(i, self._speed))
             infos.append(info)
         info = torch.stack(infos)
         dts = info[:, 2].tolist()  # the one read-back of the step
@@ -206,10 +221,11 @@ class BatchRelax:
             Fi.copy_(Fi - dts[i] * self.grid_view(i, self._speed))
         B.set_distance(self.F)
         energy = []
-        for i, (m, g) in enumerate(zip(self.maps, self.grids)):
-            band = m.F.abs() < 2 * g.Ds
+        for i, g in enumerate(self.grids):
+            band = self.grid_view(i).abs() < 2 * g.Ds
             hs = self.grid_view(i, self._H) - cfg.sponc
-            energy.append((hs * hs * m.DiracDelta * m.FGradMag)[band].sum() * g.Ds ** 3)
+            e = hs * hs * self.grid_view(i, self._tool[delta]) * self.grid_view(i, self._tool[gmag])
+            energy.append(e[band].sum() * g.Ds ** 3)
         self.energy.append(energy)
         if cfg.reinit_every > 0 and self.iteration % cfg.reinit_every == 0:
             B.reinitialize(self.F, cfg.reinit_iters)

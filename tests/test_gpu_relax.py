@@ -214,13 +214,14 @@ def test_relax_reproduces_the_reference_grid_study(n, steps, tol, ttol):
 
 
 def test_batch_relax_equals_single_vesicles():
-    """BatchRelax on three vesicles of different grids (32^3, 40x36x44 and
-    48^3, distinct spacings) against a VesicleRelax per vesicle, bit for bit
-    over three loop bodies (reinitialization every second step, so steps
-    with and without it); the single-vesicle step is the oracle's
-    (test_relax_step_matches_the_oracle), and the first vesicle is checked
-    against the oracle here too"""
-    shapes = [((32, 32, 32), (0.7, 0.55, 0.45)), ((40, 36, 44), (0.6, 0.6, 0.4)), ((48, 48, 48), (0.5, 0.65, 0.5))]
+    """BatchRelax on four vesicles, three grids (32^3 twice, 40x36x44 and
+    48^3, distinct spacings; the two 32^3 vesicles share one work handle)
+    against a VesicleRelax per vesicle, bit for bit over three loop bodies
+    (reinitialization every second step, so steps with and without it); the
+    single-vesicle step is the oracle's (test_relax_step_matches_the_oracle),
+    and the first vesicle is checked against the oracle here too"""
+    shapes = [((32, 32, 32), (0.7, 0.55, 0.45)), ((40, 36, 44), (0.6, 0.6, 0.4)), ((48, 48, 48), (0.5, 0.65, 0.5)),
+              ((32, 32, 32), (0.5, 0.7, 0.6))]
     grids, Fs = [], []
     for (rows, cols, pges), axes in shapes:
         xv, yv, zv = np.linspace(-1.0, 1.0, cols), np.linspace(-0.95, 0.95, rows), np.linspace(-0.9, 0.9, pges)
