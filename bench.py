@@ -778,8 +778,8 @@ def main():
     if args.pmc_summary:
         return pmc_summary(args.pmc_summary, args.trace_csv)
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported by this build "
-              "(the multi-GPU z-slab decomposition is a design note, not code)", file=sys.stderr)
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(a multi-GPU z-slab decomposition of the grid is not implemented)", file=sys.stderr)
         return 2
     import importlib
 
