@@ -12,7 +12,7 @@ from .sdf3 import (AUX_METHOD_FIELDS, AUX_TOOLBOX_FIELDS, GD3, TENSION_PRESSURE_
                    feval_enork2_upwind_normal, feval_extend_step, feval_re_step, feval_set_calculus_toolbox,
                    feval_upwind_normal, feval_weno_derivative, feval_auxi_set_calculus_toolbox, feval_aux_calculus_toolbox, feval_wenork3_boundary_interpolate,
                    feval_wenork3_extend_step, feval_wenork3_re_step, feval_wenork3_upwind_normal)
-from . import matio, pipeline, relax  # noqa: F401
+from . import adjoint, matio, pipeline, relax  # noqa: F401
 from .batch import Batch, BatchConfig, BatchPipeline, set_graphs  # noqa: F401
 
 __version__ = "0.1.0"
